@@ -1,0 +1,280 @@
+"""MI355X (gfx950) HIF-DE factorization: Python binding of libhifde_amd.so.
+
+The class mirrors the reference's HIFFactor<Scalar> interface
+(include/Factor.hpp:8-57): construct with (N, P, epsilon), fill the matrix
+(sp_matrix() / SetupStencil), Factor(), then Apply(u, inverse).  Every call
+goes through the C ABI declared in include/hifde_amd.h; there is no CPU
+fallback -- importing on a machine without the built library raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libhifde_amd.so")
+
+FP64 = 0
+MIXED = 1
+FLAG_FRONT_QUIRK = 1
+
+_lib = None
+
+_i = ctypes.c_int
+_ll = ctypes.c_longlong
+_d = ctypes.c_double
+_vp = ctypes.c_void_p
+_ip = ctypes.POINTER(ctypes.c_int)
+_dp = ctypes.POINTER(ctypes.c_double)
+
+EXPORTS = [
+    "hifde_create", "hifde_destroy", "hifde_last_error", "hifde_set_matrix_coo",
+    "hifde_set_stencil", "hifde_factor_run", "hifde_apply", "hifde_apply_device", "hifde_spmv",
+    "hifde_pcg", "hifde_pcg_device", "hifde_num_levels", "hifde_level_counts",
+    "hifde_box_count", "hifde_box_sizes", "hifde_box_data", "hifde_stage_times",
+    "hifde_stage_name", "hifde_factor_bytes", "hifde_interp_decomp_batch", "hifde_schur_batch",
+]
+
+
+class HifdeError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"hifde error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    """Load libhifde_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} not built; run `python -m hifde3d_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    L.hifde_create.argtypes = [_i, _i, _d, _i, _i, ctypes.POINTER(_vp)]
+    L.hifde_destroy.argtypes = [_vp]
+    L.hifde_destroy.restype = None
+    L.hifde_last_error.restype = ctypes.c_char_p
+    L.hifde_set_matrix_coo.argtypes = [_vp, _ll, _ip, _ip, _dp]
+    L.hifde_set_stencil.argtypes = [_vp, _dp, _dp, _i]
+    L.hifde_factor_run.argtypes = [_vp]
+    L.hifde_apply.argtypes = [_vp, _dp, _i]
+    L.hifde_apply_device.argtypes = [_vp, _vp, _i, _ll, _i]
+    L.hifde_spmv.argtypes = [_vp, _dp, _dp, _i]
+    L.hifde_pcg.argtypes = [_vp, _dp, _dp, _d, _i, _ip, _dp]
+    L.hifde_pcg_device.argtypes = [_vp, _vp, _vp, _d, _i, _ip, _dp]
+    L.hifde_num_levels.argtypes = [_vp]
+    L.hifde_level_counts.argtypes = [_vp, _i, _ip]
+    L.hifde_box_count.argtypes = [_vp, _i, _i]
+    L.hifde_box_sizes.argtypes = [_vp, _i, _i, _i, _ip]
+    L.hifde_box_data.argtypes = [_vp, _i, _i, _i, _ip, _ip, _dp, _dp, _dp, _dp]
+    L.hifde_stage_times.argtypes = [_vp, _dp, _i]
+    L.hifde_stage_name.argtypes = [_i]
+    L.hifde_stage_name.restype = ctypes.c_char_p
+    L.hifde_factor_bytes.argtypes = [_vp]
+    L.hifde_factor_bytes.restype = _ll
+    L.hifde_interp_decomp_batch.argtypes = [_i, _ip, _ip, _dp, _d, _ip, _ip, _dp]
+    L.hifde_schur_batch.argtypes = [_i, _i, _ip, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _ip]
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc < 0:
+        raise HifdeError(rc, lib().hifde_last_error().decode())
+    return rc
+
+
+def _ptr(a, t=ctypes.c_double):
+    return a.ctypes.data_as(ctypes.POINTER(t))
+
+
+class HIFFactor:
+    """HIFFactor(N, P, epsilon): N interior points per side, leaf width P,
+    ID relative tolerance epsilon (include/Factor.hpp:12-13).
+
+    precision: "fp64" (HIFFactor<double>) or "mixed" (FP32 Schur/LU blocks and
+    stored factors; FP64 pivots, rank decisions and level matrix).
+    front_quirk: reproduce the FRONT-face guard of src/Factor.cpp:486.
+    """
+
+    def __init__(self, N, P, epsilon, precision="fp64", front_quirk=True):
+        self.N, self.P, self.epsilon = int(N), int(P), float(epsilon)
+        self.NC = self.N + 1
+        self.precision = precision
+        prec = {"fp64": FP64, "mixed": MIXED}[precision]
+        h = _vp()
+        _check(lib().hifde_create(self.N, self.P, self.epsilon, prec,
+                                  FLAG_FRONT_QUIRK if front_quirk else 0, ctypes.byref(h)))
+        self._h = h
+        self.lu_info = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().hifde_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def size(self):
+        return self.NC ** 3
+
+    @property
+    def num_levels(self):
+        return _check(lib().hifde_num_levels(self._h))
+
+    # --- matrix -------------------------------------------------------------
+    def set_matrix_coo(self, rows, cols, vals):
+        rows = np.ascontiguousarray(rows, np.int32)
+        cols = np.ascontiguousarray(cols, np.int32)
+        vals = np.ascontiguousarray(vals, np.float64)
+        _check(lib().hifde_set_matrix_coo(self._h, rows.size, _ptr(rows, ctypes.c_int),
+                                          _ptr(cols, ctypes.c_int), _ptr(vals)))
+
+    def set_stencil(self, a, V, symmetric=False):
+        """SetupStencil on nodal tensors of shape (N+2)^3, i fastest."""
+        m = self.NC + 1
+        a = np.asfortranarray(a, np.float64).ravel(order="F")
+        V = np.asfortranarray(V, np.float64).ravel(order="F")
+        if a.size != m ** 3 or V.size != m ** 3:
+            raise ValueError("nodal tensors must have (N+2)^3 entries")
+        _check(lib().hifde_set_stencil(self._h, _ptr(a), _ptr(V), int(symmetric)))
+
+    # --- factor / apply -----------------------------------------------------
+    def Factor(self):
+        self.lu_info = _check(lib().hifde_factor_run(self._h))
+        return self.lu_info
+
+    factor = Factor
+
+    def Apply(self, u, apply_inverse=False):
+        """Return F u (or F^{-1} u) for a host vector of length (N+1)^3."""
+        u = np.array(u, dtype=np.float64, copy=True)
+        if u.size != self.size:
+            raise ValueError("vector length must be (N+1)^3")
+        _check(lib().hifde_apply(self._h, _ptr(u), int(apply_inverse)))
+        return u
+
+    apply = Apply
+
+    def apply_device(self, ptr, nrhs=1, ldu=None, inverse=True):
+        _check(lib().hifde_apply_device(self._h, ctypes.c_void_p(ptr), nrhs,
+                                        ldu if ldu is not None else self.size, int(inverse)))
+
+    def spmv(self, x, transpose=False):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros(self.size)
+        _check(lib().hifde_spmv(self._h, _ptr(x), _ptr(y), int(transpose)))
+        return y
+
+    def pcg(self, b, x0=None, tol=1e-10, maxit=200):
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.zeros(self.size) if x0 is None else np.array(x0, np.float64, copy=True)
+        it = ctypes.c_int(0)
+        rr = ctypes.c_double(0)
+        _check(lib().hifde_pcg(self._h, _ptr(b), _ptr(x), tol, maxit, ctypes.byref(it),
+                               ctypes.byref(rr)))
+        return x, it.value, rr.value
+
+    def pcg_device(self, b_ptr, x_ptr, tol=1e-10, maxit=200):
+        it = ctypes.c_int(0)
+        rr = ctypes.c_double(0)
+        _check(lib().hifde_pcg_device(self._h, ctypes.c_void_p(b_ptr), ctypes.c_void_p(x_ptr),
+                                      tol, maxit, ctypes.byref(it), ctypes.byref(rr)))
+        return it.value, rr.value
+
+    # --- introspection ------------------------------------------------------
+    def level_counts(self):
+        out = []
+        buf = np.zeros(5, np.int32)
+        for lv in range(self.num_levels):
+            _check(lib().hifde_level_counts(self._h, lv, _ptr(buf, ctypes.c_int)))
+            out.append(tuple(int(x) for x in buf))
+        return out
+
+    def num_boxes(self, level, kind):
+        return _check(lib().hifde_box_count(self._h, level, kind))
+
+    def box(self, level, kind, b):
+        sz = np.zeros(2, np.int32)
+        _check(lib().hifde_box_sizes(self._h, level, kind, b, _ptr(sz, ctypes.c_int)))
+        r, s = int(sz[0]), int(sz[1])
+        red = np.zeros(r, np.int32)
+        skel = np.zeros(s, np.int32)
+        A22 = np.zeros(r * r)
+        A22inv = np.zeros(r * r)
+        X = np.zeros(r * s)
+        W = np.zeros(r * s)
+        _check(lib().hifde_box_data(self._h, level, kind, b, _ptr(red, ctypes.c_int),
+                                    _ptr(skel, ctypes.c_int), _ptr(A22), _ptr(A22inv), _ptr(X),
+                                    _ptr(W)))
+        out = dict(red_gid=red, skel_gid=skel, A22=A22.reshape(r, r, order="F"),
+                   A22inv=A22inv.reshape(r, r, order="F"), X=X.reshape(r, s, order="F"))
+        if kind == 1:
+            out["W"] = W.reshape(s, r, order="F")
+        return out
+
+    def stage_times(self):
+        buf = np.zeros(16)
+        n = _check(lib().hifde_stage_times(self._h, _ptr(buf), 16))
+        return {lib().hifde_stage_name(i).decode(): float(buf[i]) for i in range(n)}
+
+    def factor_bytes(self):
+        return int(lib().hifde_factor_bytes(self._h))
+
+
+def interp_decomp_batch(mats, eps):
+    """InterpDecomp on a list of 2-D arrays.  Returns [(skel, red, W)]."""
+    m = np.array([a.shape[0] for a in mats], np.int32)
+    n = np.array([a.shape[1] for a in mats], np.int32)
+    packed = np.concatenate([np.asfortranarray(a, np.float64).ravel(order="F") for a in mats]
+                            + [np.zeros(1)])
+    k = np.zeros(len(mats), np.int32)
+    perm = np.zeros(max(int(n.sum()), 1), np.int32)
+    W = np.zeros(max(int((n.astype(np.int64) ** 2).sum()), 1))
+    _check(lib().hifde_interp_decomp_batch(len(mats), _ptr(m, ctypes.c_int), _ptr(n, ctypes.c_int),
+                                           _ptr(packed), eps, _ptr(k, ctypes.c_int),
+                                           _ptr(perm, ctypes.c_int), _ptr(W)))
+    out, po, wo = [], 0, 0
+    for b in range(len(mats)):
+        nb, kb = int(n[b]), int(k[b])
+        p = perm[po:po + nb]
+        Wb = W[wo:wo + kb * (nb - kb)].reshape(kb, nb - kb, order="F")
+        out.append((p[:kb].copy(), p[kb:].copy(), Wb.copy()))
+        po += nb
+        wo += nb * nb
+    return out
+
+
+def schur_batch(blocks, precision="fp64"):
+    """Schur on a list of (A22, A21, A12).  Returns [(A22inv, X, S, info)]."""
+    prec = {"fp64": FP64, "mixed": MIXED}[precision]
+    r = np.array([b[0].shape[0] for b in blocks], np.int32)
+    s = np.array([b[1].shape[1] for b in blocks], np.int32)
+
+    def pack(i):
+        return np.concatenate([np.asfortranarray(b[i], np.float64).ravel(order="F")
+                               for b in blocks] + [np.zeros(1)])
+    A22, A21, A12 = pack(0), pack(1), pack(2)
+    rr = int((r.astype(np.int64) ** 2).sum())
+    rs = int((r.astype(np.int64) * s).sum())
+    ss = int((s.astype(np.int64) ** 2).sum())
+    Ainv = np.zeros(max(rr, 1))
+    X = np.zeros(max(rs, 1))
+    S = np.zeros(max(ss, 1))
+    info = np.zeros(len(blocks), np.int32)
+    _check(lib().hifde_schur_batch(prec, len(blocks), _ptr(r, ctypes.c_int), _ptr(s, ctypes.c_int),
+                                   _ptr(A22), _ptr(A21), _ptr(A12), _ptr(Ainv), _ptr(X), _ptr(S),
+                                   _ptr(info, ctypes.c_int)))
+    out = []
+    o1 = o2 = o3 = 0
+    for b in range(len(blocks)):
+        rb, sb = int(r[b]), int(s[b])
+        out.append((Ainv[o1:o1 + rb * rb].reshape(rb, rb, order="F").copy(),
+                    X[o2:o2 + rb * sb].reshape(rb, sb, order="F").copy(),
+                    S[o3:o3 + sb * sb].reshape(sb, sb, order="F").copy(), int(info[b])))
+        o1 += rb * rb
+        o2 += rb * sb
+        o3 += sb * sb
+    return out

@@ -1,0 +1,261 @@
+// Apply sweeps of the factorization and the vector kernels of the PCG driver.
+//
+// Apply restates HIFFactor::Apply (src/Factor.cpp:992-1081): every box of a
+// level is a gather -> small GEMV -> scatter (GetSkeletonVector /
+// UpdateSkeleton / UpdateRedundant / ApplyA22, :798-975).  Boxes of one
+// level touch disjoint DOFs except the skeleton DOFs of a cell that lie on a
+// face shared with the neighbouring cell; those writes are split into two
+// passes (max-face DOFs first, then min-face DOFs) so every DOF receives its
+// two contributions in the reference's box order.
+//
+// The factor blocks may be stored in float or double; the vector and all
+// accumulations are double.
+
+#include "hifde_internal.hpp"
+
+namespace hifde {
+
+namespace {
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+// op 0: u_s[j] += alpha * sum_i M(i,j) u_r[i]   M: r x s   (X^H u_red)
+// op 1: u_r[i] += alpha * sum_j M(i,j) u_s[j]   M: r x s   (X u_skel)
+// op 2: u_r[j] += alpha * sum_i M(i,j) u_s[i]   M: s x r   (W^H u_skel)
+// op 3: u_s[i] += alpha * sum_j M(i,j) u_r[j]   M: s x r   (W u_red)
+// pass >= 0 restricts writes to skeleton DOFs whose pass flag equals pass.
+template <class T>
+__global__ void __launch_bounds__(256) k_apply_update(
+    const int* __restrict__ dr, const int* __restrict__ ds, const int* __restrict__ red_off,
+    const int* __restrict__ skel_off, const int* __restrict__ red_gid,
+    const int* __restrict__ skel_gid, const unsigned char* __restrict__ skel_pass,
+    const long long* __restrict__ off, const T* __restrict__ Mb, int op, int pass, double alpha,
+    double* __restrict__ u) {
+    const int b = blockIdx.x;
+    const int r = dr[b], s = ds[b];
+    if (r == 0 || s == 0) return;  // src/Factor.cpp:898-900, 937-939
+    const T* M = Mb + off[b];
+    const int* rg = red_gid + red_off[b];
+    const int* sg = skel_gid + skel_off[b];
+    const unsigned char* sp = skel_pass ? skel_pass + skel_off[b] : nullptr;
+    extern __shared__ double xs[];
+    const int tid = threadIdx.x, bs = blockDim.x;
+    const int lane = tid & (kWave - 1), wid = tid / kWave, nw = bs / kWave;
+    const bool in_red = (op == 0 || op == 3);  // input vector lives on red DOFs
+    const int nin = in_red ? r : s;
+    const int* ig = in_red ? rg : sg;
+    for (int t = tid; t < nin; t += bs) xs[t] = u[ig[t]];
+    __syncthreads();
+    if (op == 0 || op == 2) {
+        // transposed product: one wave per output, lanes over the column
+        const int nout = (op == 0) ? s : r;
+        const int rows = nin;
+        const int* og = (op == 0) ? sg : rg;
+        for (int j = wid; j < nout; j += nw) {
+            if (op == 0 && sp && (int)sp[j] != pass) continue;
+            const T* cj = M + (size_t)j * rows;
+            double acc = 0.0;
+            for (int i = lane; i < rows; i += kWave) acc += (double)cj[i] * xs[i];
+            acc = wave_sum(acc);
+            if (lane == 0) u[og[j]] += alpha * acc;
+        }
+    } else {
+        // plain product: one thread per output row
+        const int nout = (op == 1) ? r : s;
+        const int* og = (op == 1) ? rg : sg;
+        for (int i = tid; i < nout; i += bs) {
+            if (op == 3 && sp && (int)sp[i] != pass) continue;
+            double acc = 0.0;
+            for (int j = 0; j < nin; ++j) acc += (double)M[i + (size_t)j * nout] * xs[j];
+            u[og[i]] += alpha * acc;
+        }
+    }
+}
+
+// u_r := M u_r with M r x r (A22 or A22^{-1}); src/Factor.cpp:957-975.
+template <class T>
+__global__ void __launch_bounds__(256) k_apply_diag(const int* __restrict__ dr,
+                                                    const int* __restrict__ red_off,
+                                                    const int* __restrict__ red_gid,
+                                                    const long long* __restrict__ off,
+                                                    const T* __restrict__ Mb,
+                                                    double* __restrict__ u) {
+    const int b = blockIdx.x;
+    const int r = dr[b];
+    if (r == 0) return;
+    const T* M = Mb + off[b];
+    const int* rg = red_gid + red_off[b];
+    extern __shared__ double xs[];
+    for (int t = threadIdx.x; t < r; t += blockDim.x) xs[t] = u[rg[t]];
+    __syncthreads();
+    for (int i = threadIdx.x; i < r; i += blockDim.x) {
+        double acc = 0.0;
+        for (int j = 0; j < r; ++j) acc += (double)M[i + (size_t)j * r] * xs[j];
+        u[rg[i]] = acc;
+    }
+}
+
+__global__ void k_zero_boundary(double* u, int NC) {
+    long long n = (long long)NC * NC * NC;
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long stride = (long long)gridDim.x * blockDim.x;
+    for (; t < n; t += stride) {
+        int i = (int)(t % NC), j = (int)((t / NC) % NC), k = (int)(t / ((long long)NC * NC));
+        if (i == 0 || j == 0 || k == 0) u[t] = 0.0;
+    }
+}
+
+__global__ void k_csr_spmv(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                           const double* __restrict__ v, const double* __restrict__ x,
+                           double* __restrict__ y) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double acc = 0.0;
+    for (int e = rp[i]; e < rp[i + 1]; ++e) acc += v[e] * x[ci[e]];
+    y[i] = acc;
+}
+
+constexpr int kDotThreads = 256;
+
+__global__ void __launch_bounds__(256) k_dot_partial(const double* __restrict__ x,
+                                                     const double* __restrict__ y, long long n,
+                                                     double* part) {
+    __shared__ double red[kDotThreads / kWave];
+    double acc = 0.0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        acc += x[i] * y[i];
+    acc = wave_sum(acc);
+    if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < kDotThreads / kWave; ++w) t += red[w];
+        part[blockIdx.x] = t;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_reduce(const double* __restrict__ part, int nblk,
+                                                double* out) {
+    __shared__ double red[kDotThreads / kWave];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < nblk; i += blockDim.x) acc += part[i];
+    acc = wave_sum(acc);
+    if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < kDotThreads / kWave; ++w) t += red[w];
+        *out = t;
+    }
+}
+
+__global__ void k_axpy(double a, const double* __restrict__ x, double* __restrict__ y,
+                       long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        y[i] += a * x[i];
+}
+
+__global__ void k_xpby(const double* __restrict__ x, double b, double* __restrict__ y,
+                       long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        y[i] = x[i] + b * y[i];
+}
+
+__global__ void k_copy(const double* __restrict__ x, double* __restrict__ y, long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        y[i] = x[i];
+}
+
+int grid_for(long long n, int block = 256, int cap = 4096) {
+    long long g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (int)g;
+}
+
+}  // namespace
+
+template <class T>
+void apply_update(int nbox, const int* d_r, const int* d_s, const int* d_red_off,
+                  const int* d_skel_off, const int* d_red_gid, const int* d_skel_gid,
+                  const unsigned char* d_skel_pass, const long long* d_off, const T* d_M, int op,
+                  int pass, double alpha, double* d_u, int max_dim, hipStream_t s) {
+    if (nbox <= 0 || max_dim <= 0) return;
+    size_t shm = (size_t)max_dim * sizeof(double);
+    int threads = max_dim <= 64 ? 64 : 256;
+    k_apply_update<T><<<nbox, threads, shm, s>>>(d_r, d_s, d_red_off, d_skel_off, d_red_gid,
+                                                 d_skel_gid, d_skel_pass, d_off, d_M, op, pass,
+                                                 alpha, d_u);
+    HIFDE_LAUNCH_CHECK();
+}
+
+template <class T>
+void apply_diag(int nbox, const int* d_r, const int* d_red_off, const int* d_red_gid,
+                const long long* d_off_rr, const T* d_M, double* d_u, int max_r, hipStream_t s) {
+    if (nbox <= 0 || max_r <= 0) return;
+    size_t shm = (size_t)max_r * sizeof(double);
+    int threads = max_r <= 64 ? 64 : 256;
+    k_apply_diag<T><<<nbox, threads, shm, s>>>(d_r, d_red_off, d_red_gid, d_off_rr, d_M, d_u);
+    HIFDE_LAUNCH_CHECK();
+}
+
+void zero_boundary(double* d_u, int NC, hipStream_t s) {
+    k_zero_boundary<<<grid_for((long long)NC * NC * NC), 256, 0, s>>>(d_u, NC);
+    HIFDE_LAUNCH_CHECK();
+}
+
+void csr_spmv(int n, const int* d_rowptr, const int* d_colind, const double* d_val,
+              const double* d_x, double* d_y, hipStream_t s) {
+    k_csr_spmv<<<(n + 255) / 256, 256, 0, s>>>(n, d_rowptr, d_colind, d_val, d_x, d_y);
+    HIFDE_LAUNCH_CHECK();
+}
+
+int dot_blocks(long long n) { return grid_for(n, kDotThreads, 1024); }
+
+void dot_partial(const double* x, const double* y, long long n, double* d_part, int nblk,
+                 hipStream_t s) {
+    k_dot_partial<<<nblk, kDotThreads, 0, s>>>(x, y, n, d_part);
+    HIFDE_LAUNCH_CHECK();
+}
+
+void reduce_partials(const double* d_part, int nblk, double* d_out, hipStream_t s) {
+    k_reduce<<<1, kDotThreads, 0, s>>>(d_part, nblk, d_out);
+    HIFDE_LAUNCH_CHECK();
+}
+
+void axpy(double a, const double* x, double* y, long long n, hipStream_t s) {
+    k_axpy<<<grid_for(n), 256, 0, s>>>(a, x, y, n);
+    HIFDE_LAUNCH_CHECK();
+}
+
+void xpby(const double* x, double b, double* y, long long n, hipStream_t s) {
+    k_xpby<<<grid_for(n), 256, 0, s>>>(x, b, y, n);
+    HIFDE_LAUNCH_CHECK();
+}
+
+void vcopy(const double* x, double* y, long long n, hipStream_t s) {
+    k_copy<<<grid_for(n), 256, 0, s>>>(x, y, n);
+    HIFDE_LAUNCH_CHECK();
+}
+
+template void apply_update<double>(int, const int*, const int*, const int*, const int*,
+                                   const int*, const int*, const unsigned char*, const long long*,
+                                   const double*, int, int, double, double*, int, hipStream_t);
+template void apply_update<float>(int, const int*, const int*, const int*, const int*,
+                                  const int*, const int*, const unsigned char*, const long long*,
+                                  const float*, int, int, double, double*, int, hipStream_t);
+template void apply_diag<double>(int, const int*, const int*, const int*, const long long*,
+                                 const double*, double*, int, hipStream_t);
+template void apply_diag<float>(int, const int*, const int*, const int*, const long long*,
+                                const float*, double*, int, hipStream_t);
+
+}  // namespace hifde

@@ -1,0 +1,1098 @@
+// Host side of the library: index planning, the level loop of the
+// factorization, the apply sweeps, PCG, and the C ABI of include/hifde_amd.h.
+//
+// The level loop follows HIFFactor::Factor (src/Factor.cpp:41-63): at each
+// level every cell is eliminated (LevelFactorSchur, :184-213), the level
+// matrix is updated (UpdateMatrixAndDOFs, :262-311), then every face is
+// skeletonized (LevelFactorSkel, :232-260) and the matrix updated again.  All
+// boxes of a half level read the same pre-level matrix, so each half level is
+// one batch of boxes on the GPU.
+
+#include <omp.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/hifde_amd.h"
+#include "hifde_internal.hpp"
+
+using namespace hifde;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+enum Stage {
+    ST_PLAN = 0,
+    ST_UPLOAD,
+    ST_GATHER,
+    ST_INVERSE,
+    ST_GEMM,
+    ST_ID,
+    ST_TRANSFORM,
+    ST_SCATTER,
+    ST_COMPACT,
+    ST_DOWNLOAD,
+    ST_COUNT
+};
+const char* kStageNames[ST_COUNT] = {"plan",   "upload",    "gather",  "inverse", "gemm",
+                                     "interp_decomp", "face_transform", "scatter", "compact",
+                                     "download"};
+
+enum FaceId { TOP = 0, BOTTOM, RIGHT, LEFT, FRONT, BACK };  // include/data.hpp:7
+
+using clk = std::chrono::steady_clock;
+
+template <class V>
+std::vector<long long> prefix_ll(const V& a, const V& b) {
+    std::vector<long long> off(a.size() + 1, 0);
+    for (size_t i = 0; i < a.size(); ++i) off[i + 1] = off[i] + (long long)a[i] * b[i];
+    return off;
+}
+std::vector<int> prefix_i(const std::vector<int>& a) {
+    std::vector<int> off(a.size() + 1, 0);
+    for (size_t i = 0; i < a.size(); ++i) off[i + 1] = off[i] + a[i];
+    return off;
+}
+
+}  // namespace
+
+struct hifde_factor {
+    int N, P, NC, L;
+    double eps;
+    int precision;
+    bool quirk;
+    hipStream_t stream = nullptr;
+    std::vector<unsigned char> remaining;
+    DevBuf<unsigned char> d_alive;
+    LevelTable table;
+    bool have_matrix = false, factored = false;
+    // operator kept for SpMV / PCG (CSR)
+    std::vector<int> h_rp, h_ci;
+    std::vector<double> h_v;
+    DevBuf<int> d_rp, d_ci;
+    DevBuf<double> d_v;
+    std::vector<BoxBatch> cells, faces;
+    std::vector<int> face_red_count;
+    // workspaces
+    DevBuf<unsigned char> ws_a, ws_b, ws_c;
+    DevBuf<double> ws_M, ws_C, ws_W, ws_T;
+    DevBuf<int> ws_rows, ws_rows_off, ws_cols, ws_cols_off, ws_perm, ws_perm_off, ws_k, ws_info,
+        ws_m, ws_n, ws_tiles;
+    DevBuf<long long> ws_offM, ws_offC, ws_offW;
+    DevBuf<double> pcg_buf, pcg_part, pcg_scalar;
+    double stage_t[ST_COUNT] = {0};
+    int lu_info = 0;
+
+    int width(int level) const { return (1 << level) * P; }
+    int lin(int i, int j, int k) const { return i + NC * j + NC * NC * k; }
+    size_t elem() const { return precision == HIFDE_MIXED ? sizeof(float) : sizeof(double); }
+
+    void timed(Stage st, clk::time_point& t0) {
+        HIFDE_CHECK(hipStreamSynchronize(stream));
+        auto t1 = clk::now();
+        stage_t[st] += std::chrono::duration<double>(t1 - t0).count();
+        t0 = t1;
+    }
+
+    // ------------------------------------------------------------------
+    // Index planning (integer, host).  Orders follow the reference exactly:
+    // cells scanned i -> j -> k (InteriorCellIndexData, src/Factor.cpp:359-392);
+    // face columns/rows as in SkelInteractionIndexData (:645-755) built from
+    // InteriorFaceDOFs (:402-521) and InteriorEdgeDOFs (:523-643).
+    // ------------------------------------------------------------------
+    struct AxisClass {
+        std::vector<unsigned char> interior;  // per coordinate 0..NC
+    };
+    AxisClass axis_class(int level) const {
+        AxisClass a;
+        a.interior.assign(NC + 1, 0);
+        int w = width(level);
+        for (int x = 1; x <= N; ++x) a.interior[x] = (x % w) != 0;
+        return a;
+    }
+
+    void plan_cells(int level, BoxBatch& B) {
+        const int w = width(level), c = NC / w;
+        const AxisClass ac = axis_class(level);
+        const unsigned char* I = ac.interior.data();
+        B = BoxBatch();
+        B.nbox = c * c * c;
+        B.r.assign(B.nbox, 0);
+        B.s.assign(B.nbox, 0);
+        auto box_range = [&](int cc, int& lo, int& hi) {
+            lo = std::max(w * cc, 1);
+            hi = std::min(w * (cc + 1), N);
+        };
+#pragma omp parallel for schedule(dynamic, 16)
+        for (int b = 0; b < B.nbox; ++b) {
+            int ci = b / (c * c), cj = (b / c) % c, ck = b % c;
+            int lo[3], hi[3];
+            box_range(ci, lo[0], hi[0]);
+            box_range(cj, lo[1], hi[1]);
+            box_range(ck, lo[2], hi[2]);
+            int r = 0, s = 0;
+            for (int i = lo[0]; i <= hi[0]; ++i)
+                for (int j = lo[1]; j <= hi[1]; ++j)
+                    for (int k = lo[2]; k <= hi[2]; ++k) {
+                        if (!remaining[lin(i, j, k)]) continue;
+                        int ni = I[i] + I[j] + I[k];
+                        r += ni == 3;
+                        s += ni == 2;
+                    }
+            B.r[b] = r;
+            B.s[b] = s;
+        }
+        B.red_off = prefix_i(B.r);
+        B.skel_off = prefix_i(B.s);
+        B.red_gid.resize(B.red_off[B.nbox]);
+        B.skel_gid.resize(B.skel_off[B.nbox]);
+        B.skel_pass.resize(B.skel_off[B.nbox]);
+#pragma omp parallel for schedule(dynamic, 16)
+        for (int b = 0; b < B.nbox; ++b) {
+            int cc[3] = {b / (c * c), (b / c) % c, b % c};
+            int lo[3], hi[3];
+            for (int a = 0; a < 3; ++a) box_range(cc[a], lo[a], hi[a]);
+            int* rg = B.red_gid.data() + B.red_off[b];
+            int* sg = B.skel_gid.data() + B.skel_off[b];
+            unsigned char* sp = B.skel_pass.data() + B.skel_off[b];
+            for (int i = lo[0]; i <= hi[0]; ++i)
+                for (int j = lo[1]; j <= hi[1]; ++j)
+                    for (int k = lo[2]; k <= hi[2]; ++k) {
+                        int g = lin(i, j, k);
+                        if (!remaining[g]) continue;
+                        int ni = I[i] + I[j] + I[k];
+                        if (ni == 3) {
+                            *rg++ = g;
+                        } else if (ni == 2) {
+                            *sg++ = g;
+                            int x = !I[i] ? i : (!I[j] ? j : k);
+                            int a = !I[i] ? 0 : (!I[j] ? 1 : 2);
+                            *sp++ = (x == w * cc[a]) ? 1 : 0;  // on a min face
+                        }
+                    }
+        }
+        B.off_rr = prefix_ll(B.r, B.r);
+        B.off_rs = prefix_ll(B.r, B.s);
+        B.off_ss = prefix_ll(B.s, B.s);
+        B.tot_rr = B.off_rr.back();
+        B.tot_rs = B.off_rs.back();
+        B.tot_ss = B.off_ss.back();
+    }
+
+    // DOFs of face `face` of cell (ci, cj, ck) with `want` interior coordinates
+    // (2: face-interior, 1: edge-interior).  count-only when out == nullptr.
+    int face_dofs(int ci, int cj, int ck, int face, int want, const unsigned char* I, int w,
+                  int* out) const {
+        int lo[3] = {std::max(w * ci, 1), std::max(w * cj, 1), std::max(w * ck, 1)};
+        int hi[3] = {std::min(w * (ci + 1), N), std::min(w * (cj + 1), N),
+                     std::min(w * (ck + 1), N)};
+        int cnt = 0;
+        auto take = [&](int i, int j, int k) {
+            int g = lin(i, j, k);
+            if (remaining[g] && I[i] + I[j] + I[k] == want) {
+                if (out) out[cnt] = g;
+                ++cnt;
+            }
+        };
+        int fixed;
+        switch (face) {
+        case TOP:
+        case BOTTOM:
+            fixed = face == TOP ? lo[0] : hi[0];
+            if (fixed == (face == TOP ? 1 : N)) return 0;
+            for (int j = lo[1]; j <= hi[1]; ++j)
+                for (int k = lo[2]; k <= hi[2]; ++k) take(fixed, j, k);
+            return cnt;
+        case LEFT:
+        case RIGHT:
+            fixed = face == LEFT ? lo[1] : hi[1];
+            if (fixed == (face == LEFT ? 1 : N)) return 0;
+            for (int i = lo[0]; i <= hi[0]; ++i)
+                for (int k = lo[2]; k <= hi[2]; ++k) take(i, fixed, k);
+            return cnt;
+        default:  // FRONT / BACK
+            fixed = face == FRONT ? lo[2] : hi[2];
+            if (fixed == (face == FRONT ? (quirk ? 0 : 1) : N)) return 0;
+            for (int i = lo[0]; i <= hi[0]; ++i)
+                for (int j = lo[1]; j <= hi[1]; ++j) take(i, j, fixed);
+            return cnt;
+        }
+    }
+
+    // Rows of the ID matrix: this face's edges, the cell's other five faces,
+    // then the neighbour cell's five faces (src/Factor.cpp:665-685, 694-714,
+    // 723-743).
+    int face_rows(int ci, int cj, int ck, int face, const unsigned char* I, int w,
+                  int* out) const {
+        static const int own_top[5] = {BOTTOM, LEFT, RIGHT, FRONT, BACK};
+        static const int nbr_top[5] = {TOP, LEFT, RIGHT, FRONT, BACK};
+        static const int own_left[5] = {TOP, BOTTOM, RIGHT, FRONT, BACK};
+        static const int nbr_left[5] = {TOP, BOTTOM, LEFT, FRONT, BACK};
+        static const int own_front[5] = {TOP, BOTTOM, LEFT, RIGHT, BACK};
+        static const int nbr_front[5] = {TOP, BOTTOM, LEFT, RIGHT, FRONT};
+        const int* own = face == TOP ? own_top : (face == LEFT ? own_left : own_front);
+        const int* nbr = face == TOP ? nbr_top : (face == LEFT ? nbr_left : nbr_front);
+        int ni = ci - (face == TOP), nj = cj - (face == LEFT), nk = ck - (face == FRONT);
+        int cnt = face_dofs(ci, cj, ck, face, 1, I, w, out);
+        for (int t = 0; t < 5; ++t)
+            cnt += face_dofs(ci, cj, ck, own[t], 2, I, w, out ? out + cnt : nullptr);
+        for (int t = 0; t < 5; ++t)
+            cnt += face_dofs(ni, nj, nk, nbr[t], 2, I, w, out ? out + cnt : nullptr);
+        return cnt;
+    }
+
+    struct FacePlan {
+        int nface = 0;
+        std::vector<int> m, n, rows, cols, rows_off, cols_off;
+    };
+
+    void plan_faces(int level, FacePlan& F) {
+        const int w = width(level), c = NC / w;
+        const AxisClass ac = axis_class(level);
+        const unsigned char* I = ac.interior.data();
+        const int ncand = 3 * c * c * c;
+        static const int order[3] = {TOP, FRONT, LEFT};  // src/Factor.cpp:249-254
+        std::vector<int> cm(ncand, 0), cn(ncand, 0);
+#pragma omp parallel for schedule(dynamic, 16)
+        for (int t = 0; t < ncand; ++t) {
+            int b = t / 3, f = order[t % 3];
+            int ci = b / (c * c), cj = (b / c) % c, ck = b % c;
+            int cc = f == TOP ? ci : (f == LEFT ? cj : ck);
+            if (cc == 0) continue;  // lo == 1: no face (src/Factor.cpp:661,690,719)
+            cn[t] = face_dofs(ci, cj, ck, f, 2, I, w, nullptr);
+            if (cn[t] > 0) cm[t] = face_rows(ci, cj, ck, f, I, w, nullptr);
+        }
+        std::vector<int> slot(ncand, -1);
+        F = FacePlan();
+        for (int t = 0; t < ncand; ++t)
+            if (cn[t] > 0) {
+                slot[t] = F.nface++;
+                F.m.push_back(cm[t]);
+                F.n.push_back(cn[t]);
+            }
+        F.rows_off = prefix_i(F.m);
+        F.cols_off = prefix_i(F.n);
+        F.rows.resize(F.rows_off.back());
+        F.cols.resize(F.cols_off.back());
+#pragma omp parallel for schedule(dynamic, 16)
+        for (int t = 0; t < ncand; ++t) {
+            if (slot[t] < 0) continue;
+            int b = t / 3, f = order[t % 3];
+            int ci = b / (c * c), cj = (b / c) % c, ck = b % c;
+            int sidx = slot[t];
+            face_dofs(ci, cj, ck, f, 2, I, w, F.cols.data() + F.cols_off[sidx]);
+            face_rows(ci, cj, ck, f, I, w, F.rows.data() + F.rows_off[sidx]);
+        }
+    }
+
+    // ------------------------------------------------------------------
+    void upload_batch_index(BoxBatch& B) {
+        B.d_r.upload(B.r, stream);
+        B.d_s.upload(B.s, stream);
+        B.d_red_off.upload(B.red_off, stream);
+        B.d_skel_off.upload(B.skel_off, stream);
+        B.d_red_gid.upload(B.red_gid, stream);
+        B.d_skel_gid.upload(B.skel_gid, stream);
+        if (!B.skel_pass.empty()) B.d_skel_pass.upload(B.skel_pass, stream);
+        B.d_off_rr.upload(B.off_rr, stream);
+        B.d_off_rs.upload(B.off_rs, stream);
+        B.d_off_ss.upload(B.off_ss, stream);
+    }
+
+    // A22 (r x r) is in B.A22.  Computes A22inv, X = A22^{-1} A21, S = -A12 X.
+    template <class T>
+    void schur_batch(BoxBatch& B, const T* A21, const T* A12, T* S, clk::time_point& t0) {
+        const int nb = B.nbox;
+        int max_r = 0;
+        for (int x : B.r) max_r = std::max(max_r, x);
+        B.A22inv.reserve(B.tot_rr * sizeof(T));
+        B.X.reserve(B.tot_rs * sizeof(T));
+        T* A22 = reinterpret_cast<T*>(B.A22.p);
+        T* Ainv = reinterpret_cast<T*>(B.A22inv.p);
+        T* X = reinterpret_cast<T*>(B.X.p);
+        if (B.tot_rr)
+            HIFDE_CHECK(hipMemcpyAsync(Ainv, A22, B.tot_rr * sizeof(T), hipMemcpyDeviceToDevice,
+                                       stream));
+        ws_info.reserve(nb);
+        HIFDE_CHECK(hipMemsetAsync(ws_info.p, 0, nb * sizeof(int), stream));
+        batched_gj_inverse<T>(nb, B.d_r.p, B.d_off_rr.p, Ainv, ws_info.p, max_r, stream);
+        timed(ST_INVERSE, t0);
+        std::vector<int> info(nb);
+        HIFDE_CHECK(hipMemcpyAsync(info.data(), ws_info.p, nb * sizeof(int),
+                                   hipMemcpyDeviceToHost, stream));
+        batched_gemm_nn<T>(nb, B.r.data(), B.s.data(), B.r.data(), B.d_r.p, B.d_s.p, B.d_r.p,
+                           B.d_off_rr.p, B.d_off_rs.p, B.d_off_rs.p, Ainv, A21, X, (T)1, ws_tiles,
+                           stream);
+        batched_gemm_nn<T>(nb, B.s.data(), B.s.data(), B.r.data(), B.d_s.p, B.d_s.p, B.d_r.p,
+                           B.d_off_rs.p, B.d_off_rs.p, B.d_off_ss.p, A12, X, S, (T)-1, ws_tiles,
+                           stream);
+        timed(ST_GEMM, t0);
+        for (int b = 0; b < nb; ++b)
+            if (info[b] && !lu_info) lu_info = info[b];
+    }
+
+    void mark_eliminated(const BoxBatch& B) {
+        for (int g : B.red_gid) remaining[g] = 0;  // UpdateRemainingDOFs (:91-111)
+        d_alive.upload(remaining, stream);
+    }
+
+    template <class T>
+    void scatter(BoxBatch& B, const T* S, bool two_pass, clk::time_point& t0) {
+        // DeleteRow/DeleteCol of the eliminated DOFs, then Sparse::Add of S
+        // (src/Factor.cpp:301-309).  Compaction drops the deleted pairs.
+        table_compact(table, d_alive.p, (size_t)B.tot_ss, stream);
+        timed(ST_COMPACT, t0);
+        if (two_pass) {
+            for (int pass = 0; pass < 2; ++pass)
+                table_scatter<T>(table, B.nbox, B.d_s.p, B.d_skel_off.p, B.d_skel_gid.p,
+                                 B.d_skel_pass.p, B.d_off_ss.p, B.tot_ss, S, pass, stream);
+        } else {
+            table_scatter<T>(table, B.nbox, B.d_s.p, B.d_skel_off.p, B.d_skel_gid.p, nullptr,
+                             B.d_off_ss.p, B.tot_ss, S, -1, stream);
+        }
+        timed(ST_SCATTER, t0);
+    }
+
+    template <class T>
+    void level_cells(int level) {
+        auto t0 = clk::now();
+        BoxBatch& B = cells[level];
+        plan_cells(level, B);
+        timed(ST_PLAN, t0);
+        upload_batch_index(B);
+        timed(ST_UPLOAD, t0);
+        B.A22.reserve(B.tot_rr * sizeof(T));
+        ws_a.reserve(B.tot_rs * sizeof(T));
+        ws_b.reserve(B.tot_rs * sizeof(T));
+        ws_c.reserve(B.tot_ss * sizeof(T));
+        T* A21 = reinterpret_cast<T*>(ws_a.p);
+        T* A12 = reinterpret_cast<T*>(ws_b.p);
+        T* S = reinterpret_cast<T*>(ws_c.p);
+        // DenseSubmatrix of the cell's DOFs (include/Schur.hpp:44-61), only
+        // the three blocks Schur reads.
+        table_gather<T>(table, B.nbox, B.d_red_gid.p, B.d_red_off.p, B.d_red_gid.p,
+                        B.d_red_off.p, B.d_off_rr.p, B.tot_rr, reinterpret_cast<T*>(B.A22.p),
+                        stream);
+        table_gather<T>(table, B.nbox, B.d_red_gid.p, B.d_red_off.p, B.d_skel_gid.p,
+                        B.d_skel_off.p, B.d_off_rs.p, B.tot_rs, A21, stream);
+        table_gather<T>(table, B.nbox, B.d_skel_gid.p, B.d_skel_off.p, B.d_red_gid.p,
+                        B.d_red_off.p, B.d_off_rs.p, B.tot_rs, A12, stream);
+        timed(ST_GATHER, t0);
+        schur_batch<T>(B, A21, A12, S, t0);
+        mark_eliminated(B);
+        scatter<T>(B, S, true, t0);
+    }
+
+    template <class T>
+    void level_faces(int level) {
+        auto t0 = clk::now();
+        FacePlan F;
+        plan_faces(level, F);
+        timed(ST_PLAN, t0);
+        const int nf = F.nface;
+        BoxBatch& B = faces[level];
+        B = BoxBatch();
+        B.nbox = nf;
+        if (nf == 0) return;
+        std::vector<long long> offM = prefix_ll(F.m, F.n), offC = prefix_ll(F.n, F.n);
+        int max_n = 0, max_m = 0;
+        for (int f = 0; f < nf; ++f) {
+            max_n = std::max(max_n, F.n[f]);
+            max_m = std::max(max_m, F.m[f]);
+        }
+        ws_rows.upload(F.rows, stream);
+        ws_rows_off.upload(F.rows_off, stream);
+        ws_cols.upload(F.cols, stream);
+        ws_cols_off.upload(F.cols_off, stream);
+        ws_m.upload(F.m, stream);
+        ws_n.upload(F.n, stream);
+        ws_offM.upload(offM, stream);
+        ws_offC.upload(offC, stream);
+        timed(ST_UPLOAD, t0);
+        ws_M.reserve(offM.back());
+        ws_C.reserve(offC.back());
+        ws_W.reserve(offC.back());
+        ws_T.reserve(offC.back());
+        table_gather<double>(table, nf, ws_rows.p, ws_rows_off.p, ws_cols.p, ws_cols_off.p,
+                             ws_offM.p, offM.back(), ws_M.p, stream);
+        table_gather<double>(table, nf, ws_cols.p, ws_cols_off.p, ws_cols.p, ws_cols_off.p,
+                             ws_offC.p, offC.back(), ws_C.p, stream);
+        timed(ST_GATHER, t0);
+        ws_k.reserve(nf);
+        ws_perm.reserve(F.cols_off.back());
+        batched_interp_decomp(nf, ws_m.p, ws_n.p, ws_offM.p, ws_M.p, eps, ws_perm.p,
+                              ws_cols_off.p, ws_k.p, ws_W.p, ws_offC.p, max_n, max_m, stream);
+        timed(ST_ID, t0);
+        std::vector<int> kk(nf), perm(F.cols_off.back());
+        HIFDE_CHECK(hipMemcpyAsync(kk.data(), ws_k.p, nf * sizeof(int), hipMemcpyDeviceToHost,
+                                   stream));
+        HIFDE_CHECK(hipMemcpyAsync(perm.data(), ws_perm.p, perm.size() * sizeof(int),
+                                   hipMemcpyDeviceToHost, stream));
+        timed(ST_DOWNLOAD, t0);
+        // Face box: red = ID redundant columns, skel = ID skeleton columns
+        // (Skeletonize, src/Factor.cpp:171-178).
+        B.r.resize(nf);
+        B.s.resize(nf);
+        for (int f = 0; f < nf; ++f) {
+            B.s[f] = kk[f];
+            B.r[f] = F.n[f] - kk[f];
+        }
+        B.red_off = prefix_i(B.r);
+        B.skel_off = prefix_i(B.s);
+        B.red_gid.resize(B.red_off.back());
+        B.skel_gid.resize(B.skel_off.back());
+        for (int f = 0; f < nf; ++f) {
+            const int* p = perm.data() + F.cols_off[f];
+            const int* cg = F.cols.data() + F.cols_off[f];
+            for (int t = 0; t < B.s[f]; ++t) B.skel_gid[B.skel_off[f] + t] = cg[p[t]];
+            for (int t = 0; t < B.r[f]; ++t) B.red_gid[B.red_off[f] + t] = cg[p[B.s[f] + t]];
+        }
+        B.off_rr = prefix_ll(B.r, B.r);
+        B.off_rs = prefix_ll(B.r, B.s);
+        B.off_ss = prefix_ll(B.s, B.s);
+        B.tot_rr = B.off_rr.back();
+        B.tot_rs = B.off_rs.back();
+        B.tot_ss = B.off_ss.back();
+        timed(ST_PLAN, t0);
+        upload_batch_index(B);
+        timed(ST_UPLOAD, t0);
+        B.A22.reserve(B.tot_rr * sizeof(T));
+        ws_a.reserve(B.tot_rs * sizeof(T));
+        ws_b.reserve(B.tot_rs * sizeof(T));
+        ws_c.reserve(B.tot_ss * sizeof(T));
+        T* A21 = reinterpret_cast<T*>(ws_a.p);
+        T* A12 = reinterpret_cast<T*>(ws_b.p);
+        T* S = reinterpret_cast<T*>(ws_c.p);
+        batched_face_transform<T>(nf, ws_n.p, ws_k.p, ws_perm.p, ws_cols_off.p, ws_C.p,
+                                  ws_offC.p, ws_W.p, ws_offC.p, ws_T.p,
+                                  reinterpret_cast<T*>(B.A22.p), A21, A12, B.d_off_rr.p,
+                                  B.d_off_rs.p, stream);
+        // W (k x q) kept for Apply, packed at the r*s offsets.
+        B.W.reserve(B.tot_rs * sizeof(T));
+        pack_w<T>(B);
+        timed(ST_TRANSFORM, t0);
+        schur_batch<T>(B, A21, A12, S, t0);
+        mark_eliminated(B);
+        scatter<T>(B, S, false, t0);
+    }
+
+    template <class T>
+    void pack_w(BoxBatch& B) {
+        // Each face's W (k x q) from the n*n-strided ID workspace into the
+        // packed r*s layout, converted to the factor precision.
+        pack_boxes<T>(B.nbox, B.d_off_rs.p, ws_offC.p, ws_W.p, reinterpret_cast<T*>(B.W.p),
+                      B.tot_rs, stream);
+    }
+
+    template <class T>
+    void run_factor() {
+        if (!have_matrix) throw Error(HIFDE_ERR_STATE, "factor: no matrix set");
+        for (double& t : stage_t) t = 0.0;
+        lu_info = 0;
+        // Initialize (src/Factor.cpp:22-34)
+        remaining.assign((size_t)NC * NC * NC, 0);
+        for (int k = 1; k < NC; ++k)
+            for (int j = 1; j < NC; ++j)
+                for (int i = 1; i < NC; ++i) remaining[lin(i, j, k)] = 1;
+        cells.clear();
+        faces.clear();
+        cells.resize(L);
+        faces.resize(L);
+        for (int level = 0; level < L; ++level) {
+            level_cells<T>(level);
+            if (level < L - 1) level_faces<T>(level);
+        }
+        // the level matrix has been consumed (every DOF eliminated)
+        table.slots.release();
+        table.cap = 0;
+        ws_a.release(); ws_b.release(); ws_c.release();
+        ws_M.release(); ws_C.release(); ws_W.release(); ws_T.release();
+        factored = true;
+    }
+
+    // ------------------------------------------------------------------
+    template <class T>
+    void upd(BoxBatch& B, const DevBuf<unsigned char>& Mbuf, int op, int pass, double alpha,
+             double* u) {
+        int maxd = 0;
+        for (int b = 0; b < B.nbox; ++b) maxd = std::max(maxd, std::max(B.r[b], B.s[b]));
+        apply_update<T>(B.nbox, B.d_r.p, B.d_s.p, B.d_red_off.p, B.d_skel_off.p, B.d_red_gid.p,
+                        B.d_skel_gid.p, pass >= 0 ? B.d_skel_pass.p : nullptr, B.d_off_rs.p,
+                        reinterpret_cast<const T*>(Mbuf.p), op, pass, alpha, u, maxd, stream);
+    }
+    template <class T>
+    void diag(BoxBatch& B, const DevBuf<unsigned char>& Mbuf, double* u) {
+        int maxr = 0;
+        for (int x : B.r) maxr = std::max(maxr, x);
+        apply_diag<T>(B.nbox, B.d_r.p, B.d_red_off.p, B.d_red_gid.p, B.d_off_rr.p,
+                      reinterpret_cast<const T*>(Mbuf.p), u, maxr, stream);
+    }
+
+    // Apply (src/Factor.cpp:992-1081), one vector.
+    template <class T>
+    void apply_one(double* u, bool inv) {
+        zero_boundary(u, NC, stream);
+        const double sg = inv ? -1.0 : 1.0;
+        for (int lv = 0; lv < L - 1; ++lv) {
+            BoxBatch& C = cells[lv];
+            if (inv) {
+                for (int pass = 0; pass < 2; ++pass) upd<T>(C, C.X, 0, pass, -1.0, u);
+            } else {
+                upd<T>(C, C.X, 1, -1, 1.0, u);
+            }
+            BoxBatch& F = faces[lv];
+            if (F.nbox == 0) continue;
+            if (inv) {
+                upd<T>(F, F.W, 2, -1, -1.0, u);
+                upd<T>(F, F.X, 0, -1, -1.0, u);
+            } else {
+                upd<T>(F, F.W, 3, -1, 1.0, u);
+                upd<T>(F, F.X, 1, -1, 1.0, u);
+            }
+        }
+        for (int lv = 0; lv < L; ++lv) {
+            diag<T>(cells[lv], inv ? cells[lv].A22inv : cells[lv].A22, u);
+            if (faces[lv].nbox) diag<T>(faces[lv], inv ? faces[lv].A22inv : faces[lv].A22, u);
+        }
+        for (int lv = L - 2; lv >= 0; --lv) {
+            BoxBatch& F = faces[lv];
+            if (F.nbox) {
+                if (inv) {
+                    upd<T>(F, F.X, 1, -1, -1.0, u);
+                    upd<T>(F, F.W, 3, -1, -1.0, u);
+                } else {
+                    upd<T>(F, F.X, 0, -1, 1.0, u);
+                    upd<T>(F, F.W, 2, -1, 1.0, u);
+                }
+            }
+            BoxBatch& C = cells[lv];
+            if (inv) {
+                upd<T>(C, C.X, 1, -1, -1.0, u);
+            } else {
+                for (int pass = 0; pass < 2; ++pass) upd<T>(C, C.X, 0, pass, 1.0, u);
+            }
+        }
+        (void)sg;
+    }
+
+    void apply_dev(double* d_u, int nrhs, long long ldu, bool inv) {
+        if (!factored) throw Error(HIFDE_ERR_STATE, "apply: factor first");
+        for (int c = 0; c < nrhs; ++c) {
+            if (precision == HIFDE_MIXED) apply_one<float>(d_u + c * ldu, inv);
+            else apply_one<double>(d_u + c * ldu, inv);
+        }
+    }
+
+    // ------------------------------------------------------------------
+    double dot(const double* x, const double* y, long long n) {
+        int nblk = dot_blocks(n);
+        pcg_part.reserve(nblk);
+        pcg_scalar.reserve(1);
+        dot_partial(x, y, n, pcg_part.p, nblk, stream);
+        reduce_partials(pcg_part.p, nblk, pcg_scalar.p, stream);
+        double h = 0;
+        HIFDE_CHECK(hipMemcpyAsync(&h, pcg_scalar.p, sizeof(double), hipMemcpyDeviceToHost,
+                                   stream));
+        HIFDE_CHECK(hipStreamSynchronize(stream));
+        return h;
+    }
+
+    void spmv_dev(const double* x, double* y) {
+        csr_spmv(NC * NC * NC, d_rp.p, d_ci.p, d_v.p, x, y, stream);
+    }
+
+    void pcg_dev(const double* b, double* x, double tol, int maxit, int* iters, double* relres) {
+        if (!factored) throw Error(HIFDE_ERR_STATE, "pcg: factor first");
+        const long long n = (long long)NC * NC * NC;
+        pcg_buf.reserve(4 * n);
+        double* r = pcg_buf.p;
+        double* z = r + n;
+        double* p = z + n;
+        double* q = p + n;
+        spmv_dev(x, q);
+        vcopy(b, r, n, stream);
+        axpy(-1.0, q, r, n, stream);
+        zero_boundary(r, NC, stream);
+        double bnorm = std::sqrt(dot(b, b, n));
+        if (bnorm == 0.0) bnorm = 1.0;
+        double rnorm = std::sqrt(dot(r, r, n));
+        int it = 0;
+        if (rnorm / bnorm > tol) {
+            vcopy(r, z, n, stream);
+            apply_dev(z, 1, n, true);
+            vcopy(z, p, n, stream);
+            double rz = dot(r, z, n);
+            for (it = 1; it <= maxit; ++it) {
+                spmv_dev(p, q);
+                double pq = dot(p, q, n);
+                double alpha = rz / pq;
+                axpy(alpha, p, x, n, stream);
+                axpy(-alpha, q, r, n, stream);
+                rnorm = std::sqrt(dot(r, r, n));
+                if (rnorm / bnorm <= tol) break;
+                vcopy(r, z, n, stream);
+                apply_dev(z, 1, n, true);
+                double rz_new = dot(r, z, n);
+                double beta = rz_new / rz;
+                rz = rz_new;
+                xpby(z, beta, p, n, stream);
+            }
+            if (it > maxit) it = maxit;
+        }
+        HIFDE_CHECK(hipStreamSynchronize(stream));
+        if (iters) *iters = it;
+        if (relres) *relres = rnorm / bnorm;
+    }
+
+    // ------------------------------------------------------------------
+    // Operator assembly: rows/cols/vals with Sparse::Add semantics.
+    void set_coo(long long nnz, const int* rows, const int* cols, const double* vals) {
+        const long long nd = (long long)NC * NC * NC;
+        for (long long t = 0; t < nnz; ++t)
+            if (rows[t] < 0 || rows[t] >= nd || cols[t] < 0 || cols[t] >= nd)
+                throw Error(HIFDE_ERR_ARG, "set_matrix: index out of range");
+        // CSR with entries in the order given, duplicates merged in order.
+        std::vector<int> cnt(nd + 1, 0);
+        for (long long t = 0; t < nnz; ++t) cnt[rows[t] + 1]++;
+        std::vector<int> rp(nd + 1, 0);
+        for (long long i = 0; i < nd; ++i) rp[i + 1] = rp[i] + cnt[i + 1];
+        std::vector<int> ci(nnz);
+        std::vector<double> v(nnz);
+        std::vector<int> fill(rp.begin(), rp.end() - 1);
+        for (long long t = 0; t < nnz; ++t) {
+            int p = fill[rows[t]]++;
+            ci[p] = cols[t];
+            v[p] = vals[t];
+        }
+        h_rp.assign(nd + 1, 0);
+        h_ci.clear();
+        h_v.clear();
+        h_ci.reserve(nnz);
+        h_v.reserve(nnz);
+        for (long long i = 0; i < nd; ++i) {
+            size_t start = h_ci.size();
+            for (int p = rp[i]; p < rp[i + 1]; ++p) {
+                size_t q = start;
+                while (q < h_ci.size() && h_ci[q] != ci[p]) ++q;
+                if (q < h_ci.size()) h_v[q] += v[p];
+                else { h_ci.push_back(ci[p]); h_v.push_back(v[p]); }
+            }
+            h_rp[i + 1] = (int)h_ci.size();
+        }
+        d_rp.upload(h_rp, stream);
+        d_ci.upload(h_ci, stream);
+        d_v.upload(h_v, stream);
+        std::vector<int> rr(h_ci.size());
+        for (long long i = 0; i < nd; ++i)
+            for (int p = h_rp[i]; p < h_rp[i + 1]; ++p) rr[p] = (int)i;
+        DevBuf<int> d_rr;
+        d_rr.upload(rr, stream);
+        table_init(table, h_ci.size(), stream);
+        table_insert_coo(table, d_rr.p, d_ci.p, d_v.p, (long long)h_ci.size(), stream);
+        HIFDE_CHECK(hipStreamSynchronize(stream));
+        have_matrix = true;
+        factored = false;
+    }
+
+    void set_stencil(const double* a, const double* V, int symmetric) {
+        // SetupStencil (include/setup_stencil.hpp:18-74)
+        const int m = NC + 1;
+        auto T = [&](const double* t, int i, int j, int k) {
+            return t[i + (size_t)m * (j + (size_t)m * k)];
+        };
+        const double h = 1.0 / NC, hh = h * h;
+        const long long cap = 7LL * N * N * N;
+        std::vector<int> rows(cap), cols(cap);
+        std::vector<double> vals(cap);
+        // 7 slots per interior point, in the reference's push order
+        // (diagonal, i-1, i+1, j-1, j+1, k-1, k+1); absent neighbours are
+        // compacted out afterwards.
+#pragma omp parallel for schedule(static)
+        for (int i = 1; i <= N; ++i)
+            for (int j = 1; j <= N; ++j)
+                for (int k = 1; k <= N; ++k) {
+                    long long base = 7LL * (((long long)(i - 1) * N + (j - 1)) * N + (k - 1));
+                    const int nb[6][3] = {{i - 1, j, k}, {i + 1, j, k}, {i, j - 1, k},
+                                          {i, j + 1, k}, {i, j, k - 1}, {i, j, k + 1}};
+                    double ap = T(a, i, j, k);
+                    double d = 0.0;
+                    if (!symmetric) {
+                        d += (T(a, i - 1, j, k) + T(a, i + 1, j, k)) / hh;
+                        d += (T(a, i, j - 1, k) + T(a, i, j + 1, k)) / hh;
+                        d += (T(a, i, j, k - 1) + T(a, i, j, k + 1)) / hh;
+                    } else {
+                        for (int t = 0; t < 6; ++t)
+                            d += 0.5 * (ap + T(a, nb[t][0], nb[t][1], nb[t][2])) / hh;
+                    }
+                    d += T(V, i, j, k);
+                    int p = lin(i, j, k);
+                    rows[base] = p;
+                    cols[base] = p;
+                    vals[base] = d;
+                    for (int t = 0; t < 6; ++t) {
+                        int x = nb[t][0], y = nb[t][1], z = nb[t][2];
+                        bool in = x >= 1 && x <= N && y >= 1 && y <= N && z >= 1 && z <= N;
+                        rows[base + 1 + t] = in ? p : -1;
+                        cols[base + 1 + t] = in ? lin(x, y, z) : -1;
+                        double aq = T(a, x, y, z);
+                        vals[base + 1 + t] = symmetric ? -0.5 * (ap + aq) / hh : -aq / hh;
+                    }
+                }
+        long long w = 0;
+        for (long long t = 0; t < cap; ++t)
+            if (rows[t] >= 0) {
+                rows[w] = rows[t];
+                cols[w] = cols[t];
+                vals[w] = vals[t];
+                ++w;
+            }
+        set_coo(w, rows.data(), cols.data(), vals.data());
+    }
+};
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+namespace {
+
+template <class F>
+int guarded(F&& fn) {
+    try {
+        return fn();
+    } catch (const Error& e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return HIFDE_ERR_HIP;
+    }
+}
+
+template <class T>
+void read_dev(const DevBuf<unsigned char>& buf, long long off, long long cnt, double* out,
+              hipStream_t s) {
+    if (!out || cnt <= 0) return;
+    std::vector<T> tmp(cnt);
+    HIFDE_CHECK(hipMemcpyAsync(tmp.data(), reinterpret_cast<const T*>(buf.p) + off,
+                               cnt * sizeof(T), hipMemcpyDeviceToHost, s));
+    HIFDE_CHECK(hipStreamSynchronize(s));
+    for (long long i = 0; i < cnt; ++i) out[i] = (double)tmp[i];
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* hifde_last_error(void) { return g_last_error.c_str(); }
+
+int hifde_create(int N, int P, double epsilon, int precision, int flags, hifde_factor** out) {
+    return guarded([&]() -> int {
+        if (!out || N < 1 || P < 2 || !(epsilon > 0.0))
+            throw Error(HIFDE_ERR_ARG, "create: invalid N, P or epsilon");
+        if (precision != HIFDE_FP64 && precision != HIFDE_MIXED)
+            throw Error(HIFDE_ERR_ARG, "create: unknown precision");
+        int NC = N + 1;
+        int q = NC / P;
+        if (q < 1 || (q & (q - 1)) != 0 || NC % P != 0)
+            throw Error(HIFDE_ERR_ARG, "create: (N+1)/P must be a power of two");
+        std::unique_ptr<hifde_factor> f(new hifde_factor());
+        f->N = N;
+        f->P = P;
+        f->NC = NC;
+        f->L = (int)std::lround(std::log2((double)(NC / P))) + 1;  // src/Factor.cpp:36
+        f->eps = epsilon;
+        f->precision = precision;
+        f->quirk = (flags & HIFDE_FLAG_FRONT_QUIRK) != 0;
+        HIFDE_CHECK(hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking));
+        *out = f.release();
+        return HIFDE_OK;
+    });
+}
+
+void hifde_destroy(hifde_factor* f) {
+    if (!f) return;
+    hipStream_t s = f->stream;
+    delete f;
+    if (s) (void)hipStreamDestroy(s);
+}
+
+int hifde_set_matrix_coo(hifde_factor* f, long long nnz, const int* rows, const int* cols,
+                         const double* vals) {
+    return guarded([&]() -> int {
+        if (!f || nnz < 0 || (nnz && (!rows || !cols || !vals)))
+            throw Error(HIFDE_ERR_ARG, "set_matrix_coo: bad arguments");
+        f->set_coo(nnz, rows, cols, vals);
+        return HIFDE_OK;
+    });
+}
+
+int hifde_set_stencil(hifde_factor* f, const double* a, const double* v, int symmetric) {
+    return guarded([&]() -> int {
+        if (!f || !a || !v) throw Error(HIFDE_ERR_ARG, "set_stencil: null argument");
+        f->set_stencil(a, v, symmetric);
+        return HIFDE_OK;
+    });
+}
+
+int hifde_factor_run(hifde_factor* f) {
+    return guarded([&]() -> int {
+        if (!f) throw Error(HIFDE_ERR_ARG, "factor: null");
+        if (f->precision == HIFDE_MIXED) f->run_factor<float>();
+        else f->run_factor<double>();
+        HIFDE_CHECK(hipStreamSynchronize(f->stream));
+        return f->lu_info;
+    });
+}
+
+int hifde_apply_device(hifde_factor* f, double* d_u, int nrhs, long long ldu, int inverse) {
+    return guarded([&]() -> int {
+        if (!f || !d_u || nrhs < 0) throw Error(HIFDE_ERR_ARG, "apply: bad arguments");
+        long long n = (long long)f->NC * f->NC * f->NC;
+        if (nrhs > 1 && ldu < n) throw Error(HIFDE_ERR_ARG, "apply: ldu too small");
+        f->apply_dev(d_u, nrhs, ldu, inverse != 0);
+        HIFDE_CHECK(hipStreamSynchronize(f->stream));
+        return HIFDE_OK;
+    });
+}
+
+int hifde_apply(hifde_factor* f, double* u, int inverse) {
+    return guarded([&]() -> int {
+        if (!f || !u) throw Error(HIFDE_ERR_ARG, "apply: bad arguments");
+        long long n = (long long)f->NC * f->NC * f->NC;
+        DevBuf<double> d;
+        d.upload(u, n, f->stream);
+        f->apply_dev(d.p, 1, n, inverse != 0);
+        HIFDE_CHECK(hipMemcpyAsync(u, d.p, n * sizeof(double), hipMemcpyDeviceToHost, f->stream));
+        HIFDE_CHECK(hipStreamSynchronize(f->stream));
+        return HIFDE_OK;
+    });
+}
+
+int hifde_spmv(hifde_factor* f, const double* x, double* y, int transpose) {
+    return guarded([&]() -> int {
+        if (!f || !x || !y || !f->have_matrix) throw Error(HIFDE_ERR_ARG, "spmv: bad arguments");
+        long long n = (long long)f->NC * f->NC * f->NC;
+        if (transpose) {
+            // y = A^T x (the reference's column iteration, src/Factor.cpp:1099-1113)
+            std::fill(y, y + n, 0.0);
+            for (long long i = 0; i < n; ++i)
+                for (int p = f->h_rp[i]; p < f->h_rp[i + 1]; ++p) y[f->h_ci[p]] += f->h_v[p] * x[i];
+        } else {
+            DevBuf<double> dx, dy;
+            dx.upload(x, n, f->stream);
+            dy.reserve(n);
+            f->spmv_dev(dx.p, dy.p);
+            HIFDE_CHECK(hipMemcpyAsync(y, dy.p, n * sizeof(double), hipMemcpyDeviceToHost,
+                                       f->stream));
+            HIFDE_CHECK(hipStreamSynchronize(f->stream));
+        }
+        return HIFDE_OK;
+    });
+}
+
+int hifde_pcg_device(hifde_factor* f, const double* d_b, double* d_x, double tol, int maxit,
+                     int* iters, double* relres) {
+    return guarded([&]() -> int {
+        if (!f || !d_b || !d_x || maxit < 0) throw Error(HIFDE_ERR_ARG, "pcg: bad arguments");
+        f->pcg_dev(d_b, d_x, tol, maxit, iters, relres);
+        return HIFDE_OK;
+    });
+}
+
+int hifde_pcg(hifde_factor* f, const double* b, double* x, double tol, int maxit, int* iters,
+              double* relres) {
+    return guarded([&]() -> int {
+        if (!f || !b || !x) throw Error(HIFDE_ERR_ARG, "pcg: bad arguments");
+        long long n = (long long)f->NC * f->NC * f->NC;
+        DevBuf<double> db, dx;
+        db.upload(b, n, f->stream);
+        dx.upload(x, n, f->stream);
+        f->pcg_dev(db.p, dx.p, tol, maxit, iters, relres);
+        HIFDE_CHECK(hipMemcpyAsync(x, dx.p, n * sizeof(double), hipMemcpyDeviceToHost,
+                                   f->stream));
+        HIFDE_CHECK(hipStreamSynchronize(f->stream));
+        return HIFDE_OK;
+    });
+}
+
+int hifde_num_levels(const hifde_factor* f) { return f ? f->L : HIFDE_ERR_ARG; }
+
+int hifde_level_counts(const hifde_factor* f, int level, int* out) {
+    if (!f || !out || level < 0 || level >= f->L || !f->factored) return HIFDE_ERR_ARG;
+    const BoxBatch& C = f->cells[level];
+    const BoxBatch& F = f->faces[level];
+    out[0] = C.nbox;
+    out[1] = C.red_off.empty() ? 0 : C.red_off.back();
+    out[2] = F.nbox;
+    out[3] = F.red_off.empty() ? 0 : F.red_off.back();
+    out[4] = F.skel_off.empty() ? 0 : F.skel_off.back();
+    return HIFDE_OK;
+}
+
+int hifde_box_count(const hifde_factor* f, int level, int kind) {
+    if (!f || level < 0 || level >= f->L || !f->factored) return HIFDE_ERR_ARG;
+    return kind == 0 ? f->cells[level].nbox : f->faces[level].nbox;
+}
+
+int hifde_box_sizes(const hifde_factor* f, int level, int kind, int box, int* out) {
+    if (!f || !out || level < 0 || level >= f->L || !f->factored) return HIFDE_ERR_ARG;
+    const BoxBatch& B = kind == 0 ? f->cells[level] : f->faces[level];
+    if (box < 0 || box >= B.nbox) return HIFDE_ERR_ARG;
+    out[0] = B.r[box];
+    out[1] = B.s[box];
+    return HIFDE_OK;
+}
+
+int hifde_box_data(const hifde_factor* f, int level, int kind, int box, int* red_gid,
+                   int* skel_gid, double* A22, double* A22inv, double* X, double* W) {
+    return guarded([&]() -> int {
+        if (!f || level < 0 || level >= f->L || !f->factored)
+            throw Error(HIFDE_ERR_ARG, "box_data: bad arguments");
+        const BoxBatch& B = kind == 0 ? f->cells[level] : f->faces[level];
+        if (box < 0 || box >= B.nbox) throw Error(HIFDE_ERR_ARG, "box_data: bad box");
+        int r = B.r[box], s = B.s[box];
+        if (red_gid) std::copy_n(B.red_gid.data() + B.red_off[box], r, red_gid);
+        if (skel_gid) std::copy_n(B.skel_gid.data() + B.skel_off[box], s, skel_gid);
+        auto rd = [&](const DevBuf<unsigned char>& buf, long long off, long long cnt, double* o) {
+            if (f->precision == HIFDE_MIXED) read_dev<float>(buf, off, cnt, o, f->stream);
+            else read_dev<double>(buf, off, cnt, o, f->stream);
+        };
+        rd(B.A22, B.off_rr[box], (long long)r * r, A22);
+        rd(B.A22inv, B.off_rr[box], (long long)r * r, A22inv);
+        rd(B.X, B.off_rs[box], (long long)r * s, X);
+        if (kind == 1) rd(B.W, B.off_rs[box], (long long)r * s, W);
+        return HIFDE_OK;
+    });
+}
+
+int hifde_stage_times(const hifde_factor* f, double* out, int nout) {
+    if (!f || !out) return HIFDE_ERR_ARG;
+    int n = std::min(nout, (int)ST_COUNT);
+    for (int i = 0; i < n; ++i) out[i] = f->stage_t[i];
+    return n;
+}
+
+const char* hifde_stage_name(int stage) {
+    return (stage >= 0 && stage < ST_COUNT) ? kStageNames[stage] : "";
+}
+
+long long hifde_factor_bytes(const hifde_factor* f) {
+    if (!f) return HIFDE_ERR_ARG;
+    long long tot = 0;
+    for (int lv = 0; lv < (int)f->cells.size(); ++lv) {
+        for (const BoxBatch* B : {&f->cells[lv], &f->faces[lv]}) {
+            tot += (long long)(2 * B->tot_rr + B->tot_rs) * (long long)f->elem();
+            if (B == &f->faces[lv]) tot += (long long)B->tot_rs * (long long)f->elem();
+        }
+    }
+    return tot;
+}
+
+int hifde_interp_decomp_batch(int nbatch, const int* m, const int* n, const double* M_packed,
+                              double eps, int* k_out, int* perm_out, double* W_out) {
+    return guarded([&]() -> int {
+        if (nbatch < 0 || (nbatch && (!m || !n || !M_packed || !k_out || !perm_out || !W_out)))
+            throw Error(HIFDE_ERR_ARG, "interp_decomp_batch: bad arguments");
+        if (nbatch == 0) return HIFDE_OK;
+        hipStream_t s = nullptr;
+        std::vector<int> hm(m, m + nbatch), hn(n, n + nbatch);
+        std::vector<long long> offM = prefix_ll(hm, hn), offW = prefix_ll(hn, hn);
+        std::vector<int> poff = prefix_i(hn);
+        int max_n = 0, max_m = 0;
+        for (int b = 0; b < nbatch; ++b) {
+            if (hm[b] < 0 || hn[b] < 0) throw Error(HIFDE_ERR_ARG, "negative size");
+            max_n = std::max(max_n, hn[b]);
+            max_m = std::max(max_m, hm[b]);
+        }
+        DevBuf<int> dm, dn, dperm, dpoff, dk;
+        DevBuf<long long> doffM, doffW;
+        DevBuf<double> dM, dW;
+        dm.upload(hm, s);
+        dn.upload(hn, s);
+        doffM.upload(offM, s);
+        doffW.upload(offW, s);
+        dpoff.upload(poff, s);
+        dM.upload(M_packed, offM.back(), s);
+        dW.reserve(offW.back());
+        dperm.reserve(poff.back());
+        dk.reserve(nbatch);
+        batched_interp_decomp(nbatch, dm.p, dn.p, doffM.p, dM.p, eps, dperm.p, dpoff.p, dk.p,
+                              dW.p, doffW.p, max_n, max_m, s);
+        HIFDE_CHECK(hipMemcpy(k_out, dk.p, nbatch * sizeof(int), hipMemcpyDeviceToHost));
+        HIFDE_CHECK(hipMemcpy(perm_out, dperm.p, poff.back() * sizeof(int), hipMemcpyDeviceToHost));
+        HIFDE_CHECK(hipMemcpy(W_out, dW.p, offW.back() * sizeof(double), hipMemcpyDeviceToHost));
+        return HIFDE_OK;
+    });
+}
+
+}  // extern "C"
+
+namespace {
+template <class T>
+int schur_batch_impl(int nb, const int* r, const int* s, const double* A22, const double* A21,
+                     const double* A12, double* A22inv, double* X, double* S, int* info) {
+    hipStream_t st = nullptr;
+    std::vector<int> hr(r, r + nb), hs(s, s + nb);
+    std::vector<long long> orr = prefix_ll(hr, hr), ors = prefix_ll(hr, hs), oss = prefix_ll(hs, hs);
+    int max_r = 0;
+    for (int x : hr) max_r = std::max(max_r, x);
+    auto up = [&](const double* src, long long cnt, DevBuf<T>& d) {
+        std::vector<T> tmp(src, src + cnt);
+        d.upload(tmp, st);
+    };
+    DevBuf<T> dA, dA21, dA12, dX, dS;
+    DevBuf<int> dr, ds, dinfo, tiles;
+    DevBuf<long long> drr, drs, dss;
+    up(A22, orr.back(), dA);
+    up(A21, ors.back(), dA21);
+    up(A12, ors.back(), dA12);
+    dr.upload(hr, st);
+    ds.upload(hs, st);
+    drr.upload(orr, st);
+    drs.upload(ors, st);
+    dss.upload(oss, st);
+    dX.reserve(ors.back());
+    dS.reserve(oss.back());
+    dinfo.reserve(nb);
+    HIFDE_CHECK(hipMemset(dinfo.p, 0, nb * sizeof(int)));
+    batched_gj_inverse<T>(nb, dr.p, drr.p, dA.p, dinfo.p, max_r, st);
+    batched_gemm_nn<T>(nb, hr.data(), hs.data(), hr.data(), dr.p, ds.p, dr.p, drr.p, drs.p,
+                       drs.p, dA.p, dA21.p, dX.p, (T)1, tiles, st);
+    batched_gemm_nn<T>(nb, hs.data(), hs.data(), hr.data(), ds.p, ds.p, dr.p, drs.p, drs.p,
+                       dss.p, dA12.p, dX.p, dS.p, (T)-1, tiles, st);
+    auto down = [&](DevBuf<T>& d, long long cnt, double* dst) {
+        std::vector<T> tmp(cnt);
+        if (cnt) HIFDE_CHECK(hipMemcpy(tmp.data(), d.p, cnt * sizeof(T), hipMemcpyDeviceToHost));
+        for (long long i = 0; i < cnt; ++i) dst[i] = (double)tmp[i];
+    };
+    down(dA, orr.back(), A22inv);
+    down(dX, ors.back(), X);
+    down(dS, oss.back(), S);
+    HIFDE_CHECK(hipMemcpy(info, dinfo.p, nb * sizeof(int), hipMemcpyDeviceToHost));
+    return HIFDE_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int hifde_schur_batch(int precision, int nbatch, const int* r, const int* s, const double* A22,
+                      const double* A21, const double* A12, double* A22inv, double* X, double* S,
+                      int* info) {
+    return guarded([&]() -> int {
+        if (nbatch < 0 || (nbatch && (!r || !s || !A22 || !A21 || !A12 || !A22inv || !X || !S ||
+                                      !info)))
+            throw Error(HIFDE_ERR_ARG, "schur_batch: bad arguments");
+        if (nbatch == 0) return HIFDE_OK;
+        if (precision == HIFDE_MIXED)
+            return schur_batch_impl<float>(nbatch, r, s, A22, A21, A12, A22inv, X, S, info);
+        return schur_batch_impl<double>(nbatch, r, s, A22, A21, A12, A22inv, X, S, info);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,166 @@
+// Internal declarations shared by the HIP translation units of libhifde_amd.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace hifde {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIFDE_CHECK(x)                                                                   \
+    do {                                                                                 \
+        hipError_t e_ = (x);                                                             \
+        if (e_ != hipSuccess)                                                            \
+            throw ::hifde::Error(-2, std::string(#x) + ": " + hipGetErrorString(e_));    \
+    } while (0)
+
+#define HIFDE_LAUNCH_CHECK() HIFDE_CHECK(hipGetLastError())
+
+// Owning device buffer.  Grows, never shrinks, unless release() is called.
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void reserve(size_t cnt) {
+        if (cnt <= n && p) return;
+        release();
+        HIFDE_CHECK(hipMalloc(reinterpret_cast<void**>(&p), (cnt ? cnt : 1) * sizeof(T)));
+        n = cnt;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void upload(const T* h, size_t cnt, hipStream_t s) {
+        reserve(cnt);
+        if (cnt) HIFDE_CHECK(hipMemcpyAsync(p, h, cnt * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void upload(const std::vector<T>& h, hipStream_t s) { upload(h.data(), h.size(), s); }
+};
+
+// Level-matrix slot: key = row << 32 | col, EMPTY when unused.
+struct __align__(16) Slot {
+    unsigned long long key;
+    double val;
+};
+constexpr unsigned long long kEmptyKey = ~0ull;
+
+// Device hash table holding the level matrix A(i, j) for live DOFs.
+struct LevelTable {
+    DevBuf<Slot> slots;
+    size_t cap = 0;  // power of two
+};
+
+// One batch of boxes of a half level (cells of level l, or faces of level l).
+// Dense blocks are column-major and packed box after box; offsets are in
+// elements.
+struct BoxBatch {
+    int nbox = 0;
+    std::vector<int> r, s;                      // |red|, |skel| per box
+    std::vector<long long> off_rr, off_rs, off_ss;  // prefix sums of r*r, r*s, s*s (nbox+1)
+    std::vector<int> red_off, skel_off;         // prefix sums of r, s (nbox+1)
+    std::vector<int> red_gid, skel_gid;         // global DOF ids, reference order
+    std::vector<unsigned char> skel_pass;       // cells: 1 if the DOF lies on a min face
+    DevBuf<int> d_r, d_s, d_red_off, d_skel_off, d_red_gid, d_skel_gid;
+    DevBuf<unsigned char> d_skel_pass;
+    DevBuf<long long> d_off_rr, d_off_rs, d_off_ss;
+    // Factor data in the factor precision (float or double), as raw bytes.
+    DevBuf<unsigned char> A22, A22inv, X, W;
+    long long tot_rr = 0, tot_rs = 0, tot_ss = 0;
+    // Faces only: per-face box geometry.
+    std::vector<int> face_kind;
+};
+
+}  // namespace hifde
+
+// ---------------------------------------------------------------------------
+// Kernel launchers (defined in the .hip translation units)
+// ---------------------------------------------------------------------------
+namespace hifde {
+
+// level table
+void table_init(LevelTable& t, size_t min_entries, hipStream_t s);
+void table_insert_coo(LevelTable& t, const int* d_rows, const int* d_cols, const double* d_vals,
+                      long long nnz, hipStream_t s);
+// out(box) = A(rows(box), cols(box)), column-major, converted to T.
+template <class T>
+void table_gather(const LevelTable& t, int nbox, const int* d_rows, const int* d_row_off,
+                  const int* d_cols, const int* d_col_off, const long long* d_out_off,
+                  long long total, T* d_out, hipStream_t s);
+// A(skel_i, skel_j) += S(i, j) for pass-selected pairs (pass < 0: all pairs).
+template <class T>
+void table_scatter(LevelTable& t, int nbox, const int* d_s, const int* d_skel_off,
+                   const int* d_skel_gid, const unsigned char* d_skel_pass,
+                   const long long* d_off_ss, long long total, const T* d_S, int pass,
+                   hipStream_t s);
+// Keep only entries whose row and column are alive; resize to hold extra inserts.
+void table_compact(LevelTable& t, const unsigned char* d_alive, size_t extra, hipStream_t s);
+size_t table_count(const LevelTable& t, hipStream_t s);
+
+// dense batched kernels
+template <class T>
+void batched_gj_inverse(int nbox, const int* d_n, const long long* d_off, T* d_A, int* d_info,
+                        int max_n, hipStream_t s);
+template <class T>
+void batched_gemm_nn(int nbox, const int* h_m, const int* h_n, const int* h_k, const int* d_m,
+                     const int* d_n, const int* d_k, const long long* d_offA,
+                     const long long* d_offB, const long long* d_offC, const T* d_A,
+                     const T* d_B, T* d_C, T alpha, DevBuf<int>& tile_scratch, hipStream_t s);
+void batched_interp_decomp(int nface, const int* d_m, const int* d_n, const long long* d_offM,
+                           double* d_M, double eps, int* d_perm, const int* d_perm_off,
+                           int* d_k, double* d_W, const long long* d_offW, int max_n,
+                           int max_m, hipStream_t s);
+template <class T>
+void batched_face_transform(int nface, const int* d_n, const int* d_k, const int* d_perm,
+                            const int* d_perm_off, const double* d_C, const long long* d_offC,
+                            const double* d_W, const long long* d_offW, double* d_Tsr, T* d_A22,
+                            T* d_A21, T* d_A12, const long long* d_off_rr,
+                            const long long* d_off_rs, hipStream_t s);
+void convert_d2f(const double* src, float* dst, long long n, hipStream_t s);
+template <class D>
+void pack_boxes(int nbox, const long long* d_dst_off, const long long* d_src_off,
+                const double* d_src, D* d_dst, long long total, hipStream_t s);
+
+// apply sweeps: u (double, length NC^3) updated in place
+template <class T>
+void apply_update(int nbox, const int* d_r, const int* d_s, const int* d_red_off,
+                  const int* d_skel_off, const int* d_red_gid, const int* d_skel_gid,
+                  const unsigned char* d_skel_pass, const long long* d_off, const T* d_M,
+                  int op, int pass, double alpha, double* d_u, int max_dim, hipStream_t s);
+template <class T>
+void apply_diag(int nbox, const int* d_r, const int* d_red_off, const int* d_red_gid,
+                const long long* d_off_rr, const T* d_M, double* d_u, int max_r,
+                hipStream_t s);
+void zero_boundary(double* d_u, int NC, hipStream_t s);
+
+// vectors / CSR SpMV for PCG
+void csr_spmv(int n, const int* d_rowptr, const int* d_colind, const double* d_val,
+              const double* d_x, double* d_y, hipStream_t s);
+void dot_partial(const double* x, const double* y, long long n, double* d_part, int nblk,
+                 hipStream_t s);
+void reduce_partials(const double* d_part, int nblk, double* d_out, hipStream_t s);
+void axpy(double a, const double* x, double* y, long long n, hipStream_t s);       // y += a x
+void xpby(const double* x, double b, double* y, long long n, hipStream_t s);       // y = x + b y
+void vcopy(const double* x, double* y, long long n, hipStream_t s);
+
+int dot_blocks(long long n);
+
+}  // namespace hifde

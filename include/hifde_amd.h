@@ -1,0 +1,132 @@
+/*
+ * hifde_amd.h -- C ABI of the MI355X (gfx950) HIF-DE factorization library.
+ *
+ * The reference (arbenson/HIFDE3D) exposes its hot path only as a C++
+ * template API (include/Factor.hpp, include/InterpDecomp.hpp,
+ * include/Schur.hpp).  This ABI is what that API binds to when it runs on the
+ * GPU; include/hifde3d_amd.hpp is the C++ drop-in built on top of it.  Each
+ * entry point names the reference interface it replaces.
+ *
+ * Conventions
+ *   - Grid: N interior points per side, NC = N + 1 stored points per side,
+ *     DOF (i, j, k) has linear index i + NC*j + NC*NC*k; planes i/j/k = 0 are
+ *     the zero Dirichlet boundary (src/Factor.cpp:78-89).
+ *   - Dense blocks are column-major.  Indices are 0-based int.
+ *   - Status: 0 ok; > 0 a singular pivot was met (LAPACK-style info, the
+ *     1-based pivot index); < 0 an error (see HIFDE_ERR_*).  No entry point
+ *     throws; hifde_last_error() describes the last failure on this thread.
+ *   - Host pointers are read or written only during the call.  Device
+ *     pointers (the *_device entry points) must be device memory of the
+ *     current HIP device; work is ordered on the factor's own stream and the
+ *     call returns after it completes.
+ *   - One host thread per factor object.
+ */
+#ifndef HIFDE_AMD_H_
+#define HIFDE_AMD_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hifde_factor hifde_factor;
+
+/* precision of the factorization */
+#define HIFDE_FP64 0  /* every block in FP64 (the reference's HIFFactor<double>) */
+#define HIFDE_MIXED 1 /* Schur/LU blocks and stored factors in FP32; ID pivots,
+                         rank decisions and the level matrix in FP64 */
+
+/* flags */
+#define HIFDE_FLAG_FRONT_QUIRK 1 /* reproduce the FRONT-face guard of
+                                    src/Factor.cpp:486 (tests k == 0) */
+
+#define HIFDE_OK 0
+#define HIFDE_ERR_ARG (-1)
+#define HIFDE_ERR_HIP (-2)
+#define HIFDE_ERR_SIZE (-3)
+#define HIFDE_ERR_STATE (-4)
+
+/* HIFFactor<Scalar>(N, P, epsilon) -- include/Factor.hpp:12-13.
+ * Requires NC/P to be a power of two (src/Factor.cpp:36,51-53). */
+int hifde_create(int N, int P, double epsilon, int precision, int flags, hifde_factor** out);
+/* ~HIFFactor -- include/Factor.hpp:15 */
+void hifde_destroy(hifde_factor* f);
+const char* hifde_last_error(void);
+
+/* sp_matrix() filled through Sparse::Add(i, jidx, vals)
+ * (include/hifde3d/core/sparse.hpp:103-115): A(rows[t], cols[t]) += vals[t],
+ * duplicates summed in the order given. */
+int hifde_set_matrix_coo(hifde_factor* f, long long nnz, const int* rows, const int* cols,
+                         const double* vals);
+/* SetupStencil(factor, N, h, A, V) -- include/setup_stencil.hpp:6-76.
+ * a_nodal, v_nodal: (N+2)^3 doubles, index i + (N+2)*(j + (N+2)*k)
+ * (include/numtns.hpp:99), h = 1/(N+1).  symmetric == 0 is the reference's
+ * stencil (off-diagonal -a(q)/h^2); symmetric == 1 uses -(a(p)+a(q))/(2h^2)
+ * with the matching diagonal, identical to the reference for constant a. */
+int hifde_set_stencil(hifde_factor* f, const double* a_nodal, const double* v_nodal,
+                      int symmetric);
+
+/* Initialize() + Factor() -- src/Factor.cpp:6-39, 41-63.  The level matrix is
+ * consumed, as in the reference; the assembled operator is kept for SpMV
+ * and PCG. */
+int hifde_factor_run(hifde_factor* f);
+
+/* Apply(u, apply_inverse) -- src/Factor.cpp:992-1081.  u: host vector of
+ * (N+1)^3 doubles, overwritten with F u or F^{-1} u (boundary entries are
+ * zeroed first, src/Factor.cpp:1001). */
+int hifde_apply(hifde_factor* f, double* u, int inverse);
+/* Same on device memory: nrhs vectors at d_u + c*ldu. */
+int hifde_apply_device(hifde_factor* f, double* d_u, int nrhs, long long ldu, int inverse);
+
+/* SpMV(A, x, y) -- src/Factor.cpp:1099-1113.  transpose == 1 computes the
+ * reference's y = A^T x; transpose == 0 computes y = A x.  Host vectors. */
+int hifde_spmv(hifde_factor* f, const double* x, double* y, int transpose);
+
+/* Preconditioned CG on A x = b with F^{-1} as preconditioner (no reference
+ * counterpart; the reference stops at Apply).  FP64 residuals and dots.
+ * Stops when ||b - A x|| <= tol ||b||.  x holds the initial guess on entry. */
+int hifde_pcg(hifde_factor* f, const double* b, double* x, double tol, int maxit, int* iters,
+              double* relres);
+int hifde_pcg_device(hifde_factor* f, const double* d_b, double* d_x, double tol, int maxit,
+                     int* iters, double* relres);
+
+/* Introspection (the reference prints these, src/Factor.cpp:211-212,258-259). */
+int hifde_num_levels(const hifde_factor* f);
+/* out[0..4] = cells, DOFs eliminated by cells, faces, DOFs eliminated by
+ * faces, skeleton DOFs kept by faces */
+int hifde_level_counts(const hifde_factor* f, int level, int* out);
+/* kind 0 = cells (schur_level_data), 1 = faces (skel_level_data) */
+int hifde_box_count(const hifde_factor* f, int level, int kind);
+/* out[0..1] = |red|, |skel| */
+int hifde_box_sizes(const hifde_factor* f, int level, int kind, int box, int* out);
+/* FactorData of one box (include/data.hpp:83-110): global ids of red and
+ * skel DOFs in the box's order, A_22, A_22^{-1}, X (r x s) and, for faces,
+ * W (s x r).  Any pointer may be null. */
+int hifde_box_data(const hifde_factor* f, int level, int kind, int box, int* red_gid,
+                   int* skel_gid, double* A22, double* A22inv, double* X, double* W);
+/* Wall-clock seconds of the last factor by stage; returns the number of
+ * stages written (names from hifde_stage_name). */
+int hifde_stage_times(const hifde_factor* f, double* out, int nout);
+const char* hifde_stage_name(int stage);
+/* Bytes of factor data held on the device. */
+long long hifde_factor_bytes(const hifde_factor* f);
+
+/* InterpDecomp(M, W, skel, red, eps) -- src/InterpDecomp.cpp:86-138, over a
+ * batch.  M_packed: the m[b] x n[b] matrices packed one after another.
+ * Outputs per face: k_out[b]; perm_out (sum n[b] ints): the k skeleton
+ * columns then the n-k redundant columns, in the reference's order;
+ * W_out (sum n[b]^2 doubles, face b at offset sum_{c<b} n[c]^2): W, k x (n-k). */
+int hifde_interp_decomp_batch(int nbatch, const int* m, const int* n, const double* M_packed,
+                              double eps, int* k_out, int* perm_out, double* W_out);
+/* Schur(matrix, data) -- include/Schur.hpp:72-100, over a batch, given the
+ * three blocks A22 = M(red,red) (r x r), A21 = M(red,skel) (r x s),
+ * A12 = M(skel,red) (s x r), packed per box.  Outputs packed A22^{-1}, X and
+ * S = -A12 X.  precision selects FP64 or FP32 arithmetic.  info[b] as LU. */
+int hifde_schur_batch(int precision, int nbatch, const int* r, const int* s, const double* A22,
+                      const double* A21, const double* A12, double* A22inv, double* X, double* S,
+                      int* info);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HIFDE_AMD_H_ */

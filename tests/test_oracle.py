@@ -1,0 +1,303 @@
+"""Pins for the CPU restatement (oracle/hif_oracle.cpp).
+
+The reference ships no tests or golden values for this path (its test files
+are empty `int main(){}`), and it cannot be built here (it needs a
+CMake-generated config.h, MPI headers and an external BLAS/LAPACK).  The
+oracle is therefore pinned by:
+  * library routines: SciPy's LAPACK dgeqp3 (pivot sequence and R) and
+    numpy's LU-based inverse / solve;
+  * closed forms: Schur complement, ID reconstruction bound, exact-rank IDs;
+  * invariants: F^{-1} F = I for any tolerance; eps -> 0 gives F = A;
+  * brute force: dense solves on tiny grids;
+  * the exact integer level sizes of the reference's partition.
+A test that plants a mistake (wrong pivot rule) checks the QRCP pin bites.
+"""
+import numpy as np
+import pytest
+import scipy.linalg as sla
+import scipy.sparse as sp
+
+from hifde3d_amd import problems as pb
+
+
+def _rng(seed):
+    return np.random.default_rng(seed)
+
+
+# --------------------------------------------------------------------------
+# dense kernels
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("m,n", [(30, 9), (102, 9), (9, 9), (60, 41), (7, 12), (1, 5), (5, 1)])
+def test_qrcp_matches_lapack_dgeqp3(oracle_mod, m, n):
+    A = _rng(m * 100 + n).standard_normal((m, n))
+    R, jpvt, tau = oracle_mod.qrcp(A)
+    Rs, P = sla.qr(A, pivoting=True, mode="r")
+    assert np.array_equal(jpvt, P), (jpvt, P)
+    k = min(m, n)
+    Ru = np.triu(R[:k, :])
+    scale = np.abs(Rs).max()
+    assert np.allclose(np.abs(Ru), np.abs(Rs[:k, :]), atol=1e-12 * scale)
+    # Q R = A P: rebuild Q from the Householder vectors
+    Q = np.eye(m)
+    for i in range(k):
+        v = np.zeros(m)
+        v[i] = 1.0
+        v[i + 1:] = R[i + 1:, i]
+        Q = Q @ (np.eye(m) - tau[i] * np.outer(v, v))
+    assert np.allclose(Q[:, :k] @ Ru, A[:, jpvt], atol=1e-12 * np.abs(A).max() * m)
+
+
+def test_qrcp_pin_detects_wrong_pivot_rule(oracle_mod):
+    """Taking the last instead of the first maximal column must fail the pin."""
+    A = _rng(5).standard_normal((20, 6))
+    A[:, 4] = A[:, 1]  # exact tie in the initial norms
+    _, jpvt, _ = oracle_mod.qrcp(A)
+    _, P = sla.qr(A, pivoting=True, mode="r")
+    assert np.array_equal(jpvt, P)
+    planted = jpvt.copy()
+    i1, i4 = list(planted).index(1), list(planted).index(4)
+    planted[i1], planted[i4] = planted[i4], planted[i1]
+    assert not np.array_equal(planted, P)
+
+
+def test_qrcp_norm_downdating_with_recompute(oracle_mod):
+    """Columns that are nearly parallel force LAPACK's norm recomputation path."""
+    rng = _rng(11)
+    base = rng.standard_normal((50, 1))
+    A = np.hstack([base + 1e-9 * rng.standard_normal((50, 1)) for _ in range(6)]
+                  + [rng.standard_normal((50, 3))])
+    _, jpvt, _ = oracle_mod.qrcp(A)
+    _, P = sla.qr(A, pivoting=True, mode="r")
+    assert np.array_equal(jpvt, P)
+
+
+@pytest.mark.parametrize("n", [1, 8, 27, 64])
+def test_lu_inverse_matches_numpy(oracle_mod, n):
+    A = _rng(n).standard_normal((n, n)) + n * np.eye(n) * 0.1
+    inv, info = oracle_mod.lu_inverse(A)
+    assert info == 0
+    ref = np.linalg.inv(A)
+    assert np.allclose(inv, ref, rtol=1e-10, atol=1e-12 * np.abs(ref).max())
+
+
+def test_lu_inverse_needs_pivoting(oracle_mod):
+    A = np.array([[0.0, 1.0, 2.0], [3.0, 4.0, 5.0], [6.0, 7.0, 9.0]])
+    inv, info = oracle_mod.lu_inverse(A)
+    assert info == 0
+    assert np.allclose(inv @ A, np.eye(3), atol=1e-13)
+
+
+def test_lu_inverse_singular(oracle_mod):
+    A = np.ones((4, 4))
+    _, info = oracle_mod.lu_inverse(A)
+    assert info > 0
+
+
+def test_schur_closed_form(oracle_mod):
+    rng = _rng(3)
+    n = 40
+    B = rng.standard_normal((n, n))
+    M = B @ B.T + n * np.eye(n)
+    perm = rng.permutation(n)
+    red, skel = np.sort(perm[:15]), np.sort(perm[15:])
+    Ainv, X, S, info = oracle_mod.schur(M, red, skel)
+    assert info == 0
+    Arr = M[np.ix_(red, red)]
+    Xr = np.linalg.solve(Arr, M[np.ix_(red, skel)])
+    Sr = -M[np.ix_(skel, red)] @ Xr
+    assert np.allclose(X, Xr, atol=1e-12)
+    assert np.allclose(S, Sr, atol=1e-10)
+    assert np.allclose(Ainv, np.linalg.inv(Arr), atol=1e-12)
+
+
+def test_schur_empty_red(oracle_mod):
+    M = np.eye(3)
+    Ainv, X, S, info = oracle_mod.schur(M, np.array([], np.int32), np.arange(3))
+    assert X.shape == (0, 3) and np.all(S == 0)
+
+
+@pytest.mark.parametrize("k", [1, 3, 6])
+def test_interp_decomp_exact_rank(oracle_mod, k):
+    rng = _rng(k)
+    M = rng.standard_normal((60, k)) @ rng.standard_normal((k, 12))
+    skel, red, W, _ = oracle_mod.interp_decomp(M, 1e-8)
+    assert len(skel) == k and len(red) == 12 - k
+    assert sorted(np.concatenate([skel, red]).tolist()) == list(range(12))
+    assert np.allclose(M[:, skel] @ W, M[:, red], atol=1e-9 * np.abs(M).max())
+
+
+def test_interp_decomp_tolerance_bound(oracle_mod):
+    rng = _rng(7)
+    U, _ = np.linalg.qr(rng.standard_normal((80, 20)))
+    V, _ = np.linalg.qr(rng.standard_normal((20, 20)))
+    M = U @ np.diag(10.0 ** -np.arange(20) * 0.5) @ V.T
+    for eps in (1e-2, 1e-4, 1e-6):
+        skel, red, W, _ = oracle_mod.interp_decomp(M, eps)
+        err = np.linalg.norm(M[:, red] - M[:, skel] @ W, 2)
+        assert err <= 20 * eps * np.linalg.norm(M, 2)
+        # skeleton choice equals the rank rule applied to LAPACK's own QRCP
+        Rs, P = sla.qr(M, pivoting=True, mode="r")
+        d = np.abs(np.diag(Rs))
+        assert sorted(skel.tolist()) == sorted(P[d > eps * d[0]].tolist())
+
+
+def test_interp_decomp_zero_matrix(oracle_mod):
+    skel, red, W, _ = oracle_mod.interp_decomp(np.zeros((10, 5)), 1e-6)
+    assert len(skel) == 0 and len(red) == 5 and W.shape == (0, 5)
+
+
+# --------------------------------------------------------------------------
+# stencil assembly
+# --------------------------------------------------------------------------
+def _laplacian(N):
+    NC = N + 1
+    h = 1.0 / NC
+    T = sp.diags([-np.ones(N - 1), 2 * np.ones(N), -np.ones(N - 1)], [-1, 0, 1]) / h ** 2
+    I = sp.identity(N)
+    return (sp.kron(sp.kron(I, I), T) + sp.kron(sp.kron(I, T), I) + sp.kron(sp.kron(T, I), I)).toarray()
+
+
+def _interior_ids(NC):
+    idx = np.arange(NC ** 3)
+    i, j, k = idx % NC, (idx // NC) % NC, idx // (NC * NC)
+    return idx[(i > 0) & (j > 0) & (k > 0)]
+
+
+@pytest.mark.parametrize("sym", [False, True])
+def test_stencil_constant_coefficient_is_laplacian(oracle_mod, sym):
+    N = 5
+    NC = N + 1
+    r, c, v = oracle_mod.stencil_coo(N, pb.ones_coeff(NC), pb.zeros_coeff(NC), sym)
+    A = sp.csr_matrix((v, (r, c)), shape=(NC ** 3, NC ** 3)).toarray()
+    ids = _interior_ids(NC)
+    assert np.allclose(A[np.ix_(ids, ids)], _laplacian(N), rtol=1e-14)
+
+
+def test_stencil_reference_form_and_symmetric_variant(oracle_mod):
+    N = 5
+    NC = N + 1
+    a = pb.pow10_coeff(NC, 1)
+    V = pb.uniform_coeff(NC, 4)
+    h2 = (1.0 / NC) ** 2
+    r, c, v = oracle_mod.stencil_coo(N, a, V, False)
+    A = sp.csr_matrix((v, (r, c)), shape=(NC ** 3, NC ** 3)).toarray()
+    p = 2 + NC * 3 + NC * NC * 4
+    q = 3 + NC * 3 + NC * NC * 4
+    assert A[p, q] == -a[3, 3, 4] / h2  # -a(q)/h^2 (include/setup_stencil.hpp:148-182)
+    assert not np.allclose(A, A.T)
+    r, c, v = oracle_mod.stencil_coo(N, a, V, True)
+    S = sp.csr_matrix((v, (r, c)), shape=(NC ** 3, NC ** 3)).toarray()
+    assert np.allclose(S, S.T, rtol=0, atol=0)
+    ids = _interior_ids(NC)
+    assert np.all(np.linalg.eigvalsh(S[np.ix_(ids, ids)]) > 0)
+
+
+# --------------------------------------------------------------------------
+# whole factorization
+# --------------------------------------------------------------------------
+def _problem(oracle_mod, N, coeff="ones", sym=False, seed=1):
+    NC = N + 1
+    a = pb.ones_coeff(NC) if coeff == "ones" else pb.pow10_coeff(NC, seed)
+    r, c, v = oracle_mod.stencil_coo(N, a, pb.zeros_coeff(NC), sym)
+    A = sp.csr_matrix((v, (r, c)), shape=(NC ** 3, NC ** 3))
+    return (r, c, v), A
+
+
+def _factor(oracle_mod, N, P, eps, coo, quirk=True):
+    o = oracle_mod.Oracle(N, P, eps, front_quirk=quirk)
+    o.set_matrix_coo(*coo)
+    assert o.factor(2) == 0
+    return o
+
+
+def test_level_sizes_match_partition(oracle_mod):
+    # level-0 cells: (NC/P)^3 cells, each with (P-1)^3 interior DOFs;
+    # faces per level: 3 (c-1) c^2
+    for N, P in [(15, 4), (31, 4), (23, 3), (39, 5)]:
+        coo, _ = _problem(oracle_mod, N)
+        o = _factor(oracle_mod, N, P, 1e-3, coo)
+        NC = N + 1
+        cnt = o.level_counts()
+        assert o.num_levels == round(np.log2(NC // P)) + 1
+        for lv, (ncell, nred, nface, _) in enumerate(cnt):
+            c = NC // (P * 2 ** lv)
+            assert ncell == c ** 3
+            if lv < o.num_levels - 1:
+                assert nface == 3 * (c - 1) * c * c
+        assert cnt[0][1] == (NC // P) ** 3 * (P - 1) ** 3
+        assert sum(x[1] + x[3] for x in cnt) == N ** 3
+
+
+def test_eps_to_zero_reproduces_A(oracle_mod):
+    for coeff, sym in [("ones", False), ("pow10", True)]:
+        coo, A = _problem(oracle_mod, 15, coeff, sym)
+        o = _factor(oracle_mod, 15, 4, 1e-14, coo)
+        v = pb.rhs_normal(16, 0)
+        Av = A @ v
+        assert np.linalg.norm(o.apply(v, False) - Av) / np.linalg.norm(Av) < 1e-12
+        assert np.linalg.norm(o.apply(Av, True) - v) / np.linalg.norm(v) < 1e-11
+
+
+def test_inverse_undoes_forward_for_any_tolerance(oracle_mod):
+    coo, _ = _problem(oracle_mod, 15, "pow10", True)
+    o = _factor(oracle_mod, 15, 4, 1e-2, coo)
+    v = pb.rhs_normal(16, 3)
+    w = o.apply(o.apply(v, False), True)
+    assert np.linalg.norm(w - v) / np.linalg.norm(v) < 1e-12
+
+
+def test_dense_solve_tiny_grid(oracle_mod):
+    N, P = 7, 4
+    NC = N + 1
+    coo, A = _problem(oracle_mod, N, "pow10", True)
+    o = _factor(oracle_mod, N, P, 1e-13, coo)
+    b = pb.rhs_normal(NC, 2)
+    ids = _interior_ids(NC)
+    x = np.zeros(NC ** 3)
+    x[ids] = np.linalg.solve(A.toarray()[np.ix_(ids, ids)], b[ids])
+    y = o.apply(b, True)
+    assert np.linalg.norm(y - x) / np.linalg.norm(x) < 1e-11
+
+
+def test_reference_protocol_accuracy_tracks_tolerance(oracle_mod):
+    """Driver protocol of tests/HIFDE3D/main.cpp: w = A^T v, e_a, e_s."""
+    coo, A = _problem(oracle_mod, 15)
+    v = pb.reference_test_vector(16)
+    w = A.T @ v
+    errs = []
+    for eps in (1e-3, 1e-6, 1e-9):
+        o = _factor(oracle_mod, 15, 4, eps, coo)
+        ea = np.linalg.norm(w - o.apply(v, False)) / np.linalg.norm(w)
+        es = np.linalg.norm(v - o.apply(w, True)) / np.linalg.norm(v)
+        errs.append((ea, es))
+        assert ea < 100 * eps and es < 100 * eps
+    assert errs[0][0] > errs[1][0] > errs[2][0]
+
+
+def test_front_quirk_changes_rows_not_validity(oracle_mod):
+    coo, A = _problem(oracle_mod, 31, "pow10", True)
+    v = pb.rhs_normal(32, 5)
+    Av = A @ v
+    errs = {}
+    for quirk in (True, False):
+        o = _factor(oracle_mod, 31, 4, 1e-4, coo, quirk=quirk)
+        errs[quirk] = np.linalg.norm(o.apply(v, False) - Av) / np.linalg.norm(Av)
+        assert errs[quirk] < 1e-2
+    assert errs[True] != errs[False]
+
+
+# --------------------------------------------------------------------------
+# generators
+# --------------------------------------------------------------------------
+def test_splitmix64_published_first_output():
+    assert int(pb.splitmix64(0, 1)[0]) == 0xE220A8397B1DCDAF
+
+
+def test_normal_and_field_statistics():
+    z = pb.normal(1, 200000)
+    assert abs(z.mean()) < 0.01 and abs(z.std() - 1) < 0.01
+    g = pb.gaussian_field(32, 2, 0.1)
+    assert abs(g.mean()) < 1e-12 and abs(g.std() - 1) < 1e-12
+    # neighbouring nodes (distance h = 1/32 << 0.1) are strongly correlated
+    c = np.corrcoef(g[:-1].ravel(), g[1:].ravel())[0, 1]
+    assert 0.8 < c < 1.0
