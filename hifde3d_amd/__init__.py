@@ -33,6 +33,7 @@ EXPORTS = [
     "hifde_pcg", "hifde_pcg_device", "hifde_num_levels", "hifde_level_counts",
     "hifde_box_count", "hifde_box_sizes", "hifde_box_data", "hifde_stage_times",
     "hifde_stage_name", "hifde_factor_bytes", "hifde_interp_decomp_batch", "hifde_schur_batch",
+    "hifde_set_profiling", "hifde_reset_kernel_stats", "hifde_kernel_stats",
 ]
 
 
@@ -72,6 +73,9 @@ def lib():
     L.hifde_stage_name.restype = ctypes.c_char_p
     L.hifde_factor_bytes.argtypes = [_vp]
     L.hifde_factor_bytes.restype = _ll
+    L.hifde_set_profiling.argtypes = [_vp, _i]
+    L.hifde_reset_kernel_stats.argtypes = [_vp]
+    L.hifde_kernel_stats.argtypes = [_vp, _i, ctypes.c_char_p, _i, _dp]
     L.hifde_interp_decomp_batch.argtypes = [_i, _ip, _ip, _dp, _d, _ip, _ip, _dp]
     L.hifde_schur_batch.argtypes = [_i, _i, _ip, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _ip]
     _lib = L
@@ -219,6 +223,25 @@ class HIFFactor:
         buf = np.zeros(16)
         n = _check(lib().hifde_stage_times(self._h, _ptr(buf), 16))
         return {lib().hifde_stage_name(i).decode(): float(buf[i]) for i in range(n)}
+
+    def set_profiling(self, on=True):
+        _check(lib().hifde_set_profiling(self._h, int(on)))
+
+    def reset_kernel_stats(self):
+        _check(lib().hifde_reset_kernel_stats(self._h))
+
+    def kernel_stats(self):
+        """{family: dict(launches, sec, flops, bytes)} of device time per kernel family."""
+        L = lib()
+        n = _check(L.hifde_kernel_stats(self._h, -1, None, 0, None))
+        out = {}
+        name = ctypes.create_string_buffer(64)
+        vals = np.zeros(4)
+        for i in range(n):
+            L.hifde_kernel_stats(self._h, i, name, 64, _ptr(vals))
+            out[name.value.decode()] = dict(launches=int(vals[0]), sec=float(vals[1]),
+                                            flops=float(vals[2]), bytes=float(vals[3]))
+        return out
 
     def factor_bytes(self):
         return int(lib().hifde_factor_bytes(self._h))

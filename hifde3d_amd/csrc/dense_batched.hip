@@ -77,60 +77,79 @@ __device__ void block_argmax(T v, int idx, T* redv, int* redi, T* outv, int* out
     __syncthreads();
 }
 
-// In-place Gauss-Jordan inverse with row partial pivoting.  Dynamic shared
-// memory: max_n ints (pivots) + 2*max_n T (pivot column, scaled pivot row).
+// ---------------------------------------------------------------------------
+// Explicit inverse by Gauss-Jordan elimination with row partial pivoting (the
+// pivot of column c is the first row >= c of largest magnitude, as in getrf).
+// In-place GJ keeps [A | I] in one array: once column c has been eliminated
+// it holds column c of the accumulated elimination matrix, and the final
+// column interchanges (in reverse pivot order) turn that into A^{-1}.
+//
+// Boxes with n <= kSmallGJ are inverted in LDS, one workgroup per box.
+// Larger boxes use a blocked form: for each panel K of kGJB columns
+//   1. k_bgj_panel   eliminates the panel columns for all rows (one WG/box),
+//                    recording pivots; the panel then holds G(:, K), the
+//                    panel's elimination matrix.
+//   2. k_bgj_swap    applies the panel's row interchanges to every other
+//                    column and copies the pivot rows B = A(K, J) aside.
+//   3. k_bgj_update  A(i, J) = [i not in K] A(i, J) + G(i, K) B  for J != K,
+//                    a rank-kGJB update tiled over the whole GPU.
+// followed by k_bgj_colswap for the column interchanges.
+// ---------------------------------------------------------------------------
+constexpr int kSmallGJ = 64;
+constexpr int kGJB = 32;
+
 template <class T>
-__global__ void __launch_bounds__(256) k_gj_inverse(const int* __restrict__ dims,
-                                                    const long long* __restrict__ off, T* mats,
-                                                    int* info) {
-    int b = blockIdx.x;
-    int n = dims[b];
-    if (n == 0) return;
+__global__ void __launch_bounds__(128) k_gj_small(const int* __restrict__ dims,
+                                                  const long long* __restrict__ off, T* mats,
+                                                  int* info) {
+    const int b = blockIdx.x;
+    const int n = dims[b];
+    if (n == 0 || n > kSmallGJ) return;
     T* A = mats + off[b];
-    extern __shared__ unsigned char smem[];
-    int* piv = reinterpret_cast<int*>(smem);
-    T* colk = reinterpret_cast<T*>(smem + ((n * sizeof(int) + 15) & ~size_t(15)));
-    T* rowk = colk + n;
-    __shared__ T redv[16];
-    __shared__ int redi[16];
+    __shared__ T S[kSmallGJ][kSmallGJ + 1];  // S[col][row]
+    __shared__ T colk[kSmallGJ], rowk[kSmallGJ];
+    __shared__ int piv[kSmallGJ];
+    __shared__ T redv[4];
+    __shared__ int redi[4];
     __shared__ T s_pv;
     __shared__ int s_pi;
     const int tid = threadIdx.x, bs = blockDim.x;
+    for (int e = tid; e < n * n; e += bs) S[e / n][e % n] = A[e];
+    __syncthreads();
     for (int k = 0; k < n; ++k) {
         T best = (T)-1;
         int bi = n;
         for (int i = k + tid; i < n; i += bs) {
-            T v = fabs(A[i + (size_t)k * n]);
+            T v = fabs(S[k][i]);
             if (v > best) { best = v; bi = i; }
         }
         block_argmax(best, bi, redv, redi, &s_pv, &s_pi);
-        int p = s_pi;
-        if (!(s_pv > (T)0)) {  // exactly singular pivot column (or NaN)
+        const int p = s_pi;
+        if (!(s_pv > (T)0)) {
             if (tid == 0) info[b] = k + 1;
             return;
         }
         if (p != k)
             for (int c = tid; c < n; c += bs) {
-                T t = A[k + (size_t)c * n];
-                A[k + (size_t)c * n] = A[p + (size_t)c * n];
-                A[p + (size_t)c * n] = t;
+                T t = S[c][k];
+                S[c][k] = S[c][p];
+                S[c][p] = t;
             }
         if (tid == 0) piv[k] = p;
         __syncthreads();
-        T d = (T)1 / A[k + (size_t)k * n];
+        const T d = (T)1 / S[k][k];
         for (int i = tid; i < n; i += bs) {
-            colk[i] = A[i + (size_t)k * n];
-            rowk[i] = (i == k) ? d : A[k + (size_t)i * n] * d;
+            colk[i] = S[k][i];
+            rowk[i] = (i == k) ? d : S[i][k] * d;
         }
         __syncthreads();
-        size_t nn = (size_t)n * n;
-        for (size_t e = tid; e < nn; e += bs) {
-            int i = (int)(e % n), c = (int)(e / n);
+        for (int e = tid; e < n * n; e += bs) {
+            int c = e / n, i = e - c * n;
             if (i == k) {
-                A[e] = rowk[c];
+                S[c][i] = rowk[c];
             } else {
-                T a = (c == k) ? (T)0 : A[e];
-                A[e] = a - colk[i] * rowk[c];
+                T a = (c == k) ? (T)0 : S[c][i];
+                S[c][i] = a - colk[i] * rowk[c];
             }
         }
         __syncthreads();
@@ -139,13 +158,200 @@ __global__ void __launch_bounds__(256) k_gj_inverse(const int* __restrict__ dims
         int p = piv[k];
         if (p != k)
             for (int i = tid; i < n; i += bs) {
-                T t = A[i + (size_t)k * n];
-                A[i + (size_t)k * n] = A[i + (size_t)p * n];
-                A[i + (size_t)p * n] = t;
+                T t = S[k][i];
+                S[k][i] = S[p][i];
+                S[p][i] = t;
             }
         __syncthreads();
     }
+    for (int e = tid; e < n * n; e += bs) A[e] = S[e / n][e % n];
     if (tid == 0) info[b] = 0;
+}
+
+// Panel elimination, one workgroup per large box.  Dynamic shared memory:
+// max_n T (pivot column) .
+template <class T>
+__global__ void __launch_bounds__(256) k_bgj_panel(const int* __restrict__ dims,
+                                                   const long long* __restrict__ off, T* mats,
+                                                   int* piv, const int* __restrict__ piv_off,
+                                                   int step, int* info) {
+    const int b = blockIdx.x;
+    const int n = dims[b];
+    const int k0 = step * kGJB;
+    if (n <= kSmallGJ || k0 >= n || info[b]) return;
+    const int kb = min(kGJB, n - k0);
+    T* A = mats + off[b];
+    int* pv = piv + piv_off[b];
+    extern __shared__ unsigned char smem[];
+    T* colk = reinterpret_cast<T*>(smem);
+    __shared__ T rowk[kGJB];
+    __shared__ T redv[4];
+    __shared__ int redi[4];
+    __shared__ T s_pv;
+    __shared__ int s_pi;
+    const int tid = threadIdx.x, bs = blockDim.x;
+    for (int c = k0; c < k0 + kb; ++c) {
+        T best = (T)-1;
+        int bi = n;
+        for (int i = c + tid; i < n; i += bs) {
+            T v = fabs(A[i + (size_t)c * n]);
+            if (v > best) { best = v; bi = i; }
+        }
+        block_argmax(best, bi, redv, redi, &s_pv, &s_pi);
+        const int p = s_pi;
+        if (!(s_pv > (T)0)) {
+            if (tid == 0) info[b] = c + 1;
+            return;
+        }
+        if (p != c && tid < kb) {
+            T* cj = A + (size_t)(k0 + tid) * n;
+            T t = cj[c];
+            cj[c] = cj[p];
+            cj[p] = t;
+        }
+        if (tid == 0) pv[c] = p;
+        __syncthreads();
+        const T d = (T)1 / A[c + (size_t)c * n];
+        for (int i = tid; i < n; i += bs) colk[i] = A[i + (size_t)c * n];
+        if (tid < kb) {
+            int j = k0 + tid;
+            rowk[tid] = (j == c) ? d : A[c + (size_t)j * n] * d;
+        }
+        __syncthreads();
+        for (int jj = 0; jj < kb; ++jj) {
+            const int j = k0 + jj;
+            T* cj = A + (size_t)j * n;
+            const T rk = rowk[jj];
+            for (int i = tid; i < n; i += bs) {
+                if (i == c) {
+                    cj[i] = rk;
+                } else {
+                    T a = (j == c) ? (T)0 : cj[i];
+                    cj[i] = a - colk[i] * rk;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Row interchanges of the panel applied to column j (j outside the panel),
+// then B(:, j) = A(K, j).  Grid: x over columns, y over boxes.
+template <class T>
+__global__ void __launch_bounds__(256) k_bgj_swap(const int* __restrict__ dims,
+                                                  const long long* __restrict__ off, T* mats,
+                                                  const int* __restrict__ piv,
+                                                  const int* __restrict__ piv_off, int step,
+                                                  T* Bws, const long long* __restrict__ b_off,
+                                                  const int* __restrict__ info) {
+    const int b = blockIdx.y;
+    const int n = dims[b];
+    const int k0 = step * kGJB;
+    if (n <= kSmallGJ || k0 >= n || info[b]) return;
+    const int kb = min(kGJB, n - k0);
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n || (j >= k0 && j < k0 + kb)) return;
+    T* cj = mats + off[b] + (size_t)j * n;
+    const int* pv = piv + piv_off[b];
+    for (int c = k0; c < k0 + kb; ++c) {
+        int p = pv[c];
+        if (p != c) {
+            T t = cj[c];
+            cj[c] = cj[p];
+            cj[p] = t;
+        }
+    }
+    T* B = Bws + b_off[b] + (size_t)j * kGJB;
+    for (int c = 0; c < kb; ++c) B[c] = cj[k0 + c];
+}
+
+// A(i, j) = [i not in K] A(i, j) + sum_c G(i, c) B(c, j), j not in K.
+template <class T>
+__global__ void __launch_bounds__(256) k_bgj_update(int nbox, const int* __restrict__ tile_off,
+                                                    const int* __restrict__ dims,
+                                                    const long long* __restrict__ off, T* mats,
+                                                    const T* __restrict__ Bws,
+                                                    const long long* __restrict__ b_off,
+                                                    int step, const int* __restrict__ info) {
+    const int t = blockIdx.x;
+    const int b = find_box(tile_off, nbox, t);
+    const int n = dims[b];
+    const int k0 = step * kGJB;
+    if (n <= kSmallGJ || k0 >= n || info[b]) return;
+    const int kb = min(kGJB, n - k0);
+    const int local = t - tile_off[b];
+    const int tm = (n + 63) / 64;
+    const int i0 = (local % tm) * 64, j0 = (local / tm) * 64;
+    if (j0 + 64 > k0 && j0 < k0 + kb && j0 >= k0 && j0 + 64 <= k0 + kb) return;
+    T* A = mats + off[b];
+    const T* B = Bws + b_off[b];
+    __shared__ T Gs[kGJB][64 + 1];
+    __shared__ T Bs[kGJB][64 + 1];
+    const int tid = threadIdx.x;
+    for (int e = tid; e < 64 * kGJB; e += 256) {
+        int r = e & 63, c = e >> 6;
+        int gi = i0 + r;
+        Gs[c][r] = (gi < n && c < kb) ? A[gi + (size_t)(k0 + c) * n] : (T)0;
+        int cb = e & (kGJB - 1), jb = e / kGJB;
+        int gj = j0 + jb;
+        bool skip = gj >= n || (gj >= k0 && gj < k0 + kb) || cb >= kb;
+        Bs[cb][jb] = skip ? (T)0 : B[cb + (size_t)gj * kGJB];
+    }
+    __syncthreads();
+    const int tx = tid & 15, ty = tid >> 4;
+    T acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = (T)0;
+    for (int c = 0; c < kb; ++c) {
+        T a[4], bv[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a[i] = Gs[c][tx + 16 * i];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) bv[j] = Bs[c][ty + 16 * j];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] = fma(a[i], bv[j], acc[i][j]);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int gj = j0 + ty + 16 * j;
+        if (gj >= n || (gj >= k0 && gj < k0 + kb)) continue;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int gi = i0 + tx + 16 * i;
+            if (gi >= n) continue;
+            T* e = A + gi + (size_t)gj * n;
+            const bool inK = gi >= k0 && gi < k0 + kb;
+            *e = (inK ? (T)0 : *e) + acc[i][j];
+        }
+    }
+}
+
+// Column interchanges in reverse pivot order; one thread per row.
+template <class T>
+__global__ void __launch_bounds__(256) k_bgj_colswap(const int* __restrict__ dims,
+                                                     const long long* __restrict__ off, T* mats,
+                                                     const int* __restrict__ piv,
+                                                     const int* __restrict__ piv_off,
+                                                     int* info) {
+    const int b = blockIdx.y;
+    const int n = dims[b];
+    if (n <= kSmallGJ || info[b]) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    T* A = mats + off[b];
+    const int* pv = piv + piv_off[b];
+    for (int k = n - 1; k >= 0; --k) {
+        int p = pv[k];
+        if (p != k) {
+            T t = A[i + (size_t)k * n];
+            A[i + (size_t)k * n] = A[i + (size_t)p * n];
+            A[i + (size_t)p * n] = t;
+        }
+    }
 }
 
 // C(m x n) = alpha * A(m x k) * B(k x n), all column-major with ld = rows.
@@ -464,13 +670,46 @@ int grid_for(long long n, int block = 256, int cap = 1 << 16) {
 }  // namespace
 
 template <class T>
-void batched_gj_inverse(int nbox, const int* d_n, const long long* d_off, T* d_A, int* d_info,
-                        int max_n, hipStream_t s) {
-    if (nbox <= 0 || max_n <= 0) return;
-    size_t shm = ((size_t)max_n * sizeof(int) + 15) / 16 * 16 + 2 * (size_t)max_n * sizeof(T);
-    if (shm > 160 * 1024) throw Error(-3, "gj_inverse: block too large for LDS");
-    int threads = max_n <= 32 ? 64 : 256;
-    k_gj_inverse<T><<<nbox, threads, shm, s>>>(d_n, d_off, d_A, d_info);
+void batched_gj_inverse(int nbox, const int* h_n, const int* d_n, const long long* d_off, T* d_A,
+                        int* d_info, GJWork& ws, hipStream_t s) {
+    if (nbox <= 0) return;
+    int max_n = 0;
+    bool any_small = false;
+    std::vector<int> piv_off(nbox + 1, 0), tile_off(nbox + 1, 0);
+    std::vector<long long> b_off(nbox + 1, 0);
+    for (int b = 0; b < nbox; ++b) {
+        int n = h_n[b];
+        max_n = std::max(max_n, n);
+        bool big = n > kSmallGJ;
+        any_small |= (n > 0 && !big);
+        piv_off[b + 1] = piv_off[b] + (big ? n : 0);
+        b_off[b + 1] = b_off[b] + (big ? (long long)kGJB * n : 0);
+        int tm = big ? (n + 63) / 64 : 0;
+        tile_off[b + 1] = tile_off[b] + tm * tm;
+    }
+    if (any_small) {
+        k_gj_small<T><<<nbox, 128, 0, s>>>(d_n, d_off, d_A, d_info);
+        HIFDE_LAUNCH_CHECK();
+    }
+    if (max_n <= kSmallGJ) return;
+    ws.piv.reserve(piv_off[nbox]);
+    ws.piv_off.upload(piv_off, s);
+    ws.tile_off.upload(tile_off, s);
+    ws.b_off.upload(b_off, s);
+    ws.B.reserve((size_t)b_off[nbox] * sizeof(T));
+    T* Bws = reinterpret_cast<T*>(ws.B.p);
+    const int steps = (max_n + kGJB - 1) / kGJB;
+    const size_t shm = (size_t)max_n * sizeof(T);
+    if (shm > 150 * 1024) throw Error(-3, "gj_inverse: box too large");
+    dim3 gsw((max_n + 255) / 256, nbox);
+    for (int st = 0; st < steps; ++st) {
+        k_bgj_panel<T><<<nbox, 256, shm, s>>>(d_n, d_off, d_A, ws.piv.p, ws.piv_off.p, st, d_info);
+        k_bgj_swap<T><<<gsw, 256, 0, s>>>(d_n, d_off, d_A, ws.piv.p, ws.piv_off.p, st, Bws,
+                                          ws.b_off.p, d_info);
+        k_bgj_update<T><<<tile_off[nbox], 256, 0, s>>>(nbox, ws.tile_off.p, d_n, d_off, d_A, Bws,
+                                                       ws.b_off.p, st, d_info);
+    }
+    k_bgj_colswap<T><<<gsw, 256, 0, s>>>(d_n, d_off, d_A, ws.piv.p, ws.piv_off.p, d_info);
     HIFDE_LAUNCH_CHECK();
 }
 
@@ -540,10 +779,10 @@ void convert_d2f(const double* src, float* dst, long long n, hipStream_t s) {
     HIFDE_LAUNCH_CHECK();
 }
 
-template void batched_gj_inverse<double>(int, const int*, const long long*, double*, int*, int,
-                                         hipStream_t);
-template void batched_gj_inverse<float>(int, const int*, const long long*, float*, int*, int,
-                                        hipStream_t);
+template void batched_gj_inverse<double>(int, const int*, const int*, const long long*, double*,
+                                         int*, GJWork&, hipStream_t);
+template void batched_gj_inverse<float>(int, const int*, const int*, const long long*, float*, int*,
+                                        GJWork&, hipStream_t);
 template void batched_gemm_nn<double>(int, const int*, const int*, const int*, const int*,
                                       const int*, const int*, const long long*, const long long*,
                                       const long long*, const double*, const double*, double*,

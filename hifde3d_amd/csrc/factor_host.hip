@@ -75,8 +75,23 @@ struct hifde_factor {
     // operator kept for SpMV / PCG (CSR)
     std::vector<int> h_rp, h_ci;
     std::vector<double> h_v;
-    DevBuf<int> d_rp, d_ci;
+    DevBuf<int> d_rp, d_ci, d_rr;
     DevBuf<double> d_v;
+    // per-kernel-family device time (HIP events), enabled by hifde_set_profiling
+    struct KStat {
+        long long launches = 0;
+        double sec = 0, flops = 0, bytes = 0;
+    };
+    struct PendingRec {
+        int fam;
+        hipEvent_t e0, e1;
+        double flops, bytes;
+    };
+    bool profiling = false;
+    std::vector<std::string> fam_names;
+    std::vector<KStat> fam_stats;
+    std::vector<PendingRec> pending;
+    std::vector<hipEvent_t> ev_pool;
     std::vector<BoxBatch> cells, faces;
     std::vector<int> face_red_count;
     // workspaces
@@ -85,13 +100,76 @@ struct hifde_factor {
     DevBuf<int> ws_rows, ws_rows_off, ws_cols, ws_cols_off, ws_perm, ws_perm_off, ws_k, ws_info,
         ws_m, ws_n, ws_tiles;
     DevBuf<long long> ws_offM, ws_offC, ws_offW;
+    GJWork gj_ws;
     DevBuf<double> pcg_buf, pcg_part, pcg_scalar;
     double stage_t[ST_COUNT] = {0};
     int lu_info = 0;
 
+    ~hifde_factor() {
+        for (PendingRec& r : pending) {
+            (void)hipEventDestroy(r.e0);
+            (void)hipEventDestroy(r.e1);
+        }
+        for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
+    }
+
     int width(int level) const { return (1 << level) * P; }
     int lin(int i, int j, int k) const { return i + NC * j + NC * NC * k; }
     size_t elem() const { return precision == HIFDE_MIXED ? sizeof(float) : sizeof(double); }
+
+    int family(const char* name) {
+        for (size_t i = 0; i < fam_names.size(); ++i)
+            if (fam_names[i] == name) return (int)i;
+        fam_names.push_back(name);
+        fam_stats.emplace_back();
+        return (int)fam_names.size() - 1;
+    }
+    hipEvent_t take_event() {
+        if (!ev_pool.empty()) {
+            hipEvent_t e = ev_pool.back();
+            ev_pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        HIFDE_CHECK(hipEventCreate(&e));
+        return e;
+    }
+    // Records one launch (or a short launch sequence) of a kernel family on
+    // the factor's stream, with its algorithmic flops and bytes.
+    struct Prof {
+        hifde_factor* f;
+        int fam = -1;
+        hipEvent_t e0 = nullptr;
+        double flops, bytes;
+        Prof(hifde_factor* f_, const char* name, double fl, double by) : f(f_), flops(fl), bytes(by) {
+            if (!f->profiling) return;
+            fam = f->family(name);
+            e0 = f->take_event();
+            HIFDE_CHECK(hipEventRecord(e0, f->stream));
+        }
+        ~Prof() noexcept(false) {
+            if (fam < 0) return;
+            hipEvent_t e1 = f->take_event();
+            HIFDE_CHECK(hipEventRecord(e1, f->stream));
+            f->pending.push_back({fam, e0, e1, flops, bytes});
+        }
+    };
+    void resolve_profile() {
+        if (pending.empty()) return;
+        HIFDE_CHECK(hipStreamSynchronize(stream));
+        for (PendingRec& r : pending) {
+            float ms = 0.f;
+            HIFDE_CHECK(hipEventElapsedTime(&ms, r.e0, r.e1));
+            KStat& k = fam_stats[r.fam];
+            k.launches += 1;
+            k.sec += ms * 1e-3;
+            k.flops += r.flops;
+            k.bytes += r.bytes;
+            ev_pool.push_back(r.e0);
+            ev_pool.push_back(r.e1);
+        }
+        pending.clear();
+    }
 
     void timed(Stage st, clk::time_point& t0) {
         HIFDE_CHECK(hipStreamSynchronize(stream));
@@ -321,17 +399,38 @@ struct hifde_factor {
                                        stream));
         ws_info.reserve(nb);
         HIFDE_CHECK(hipMemsetAsync(ws_info.p, 0, nb * sizeof(int), stream));
-        batched_gj_inverse<T>(nb, B.d_r.p, B.d_off_rr.p, Ainv, ws_info.p, max_r, stream);
+        const double w = sizeof(T);
+        double fl_inv = 0, by_inv = 0, fl_x = 0, by_x = 0, fl_s = 0, by_s = 0;
+        for (int b = 0; b < nb; ++b) {
+            double r = B.r[b], sd = B.s[b];
+            fl_inv += 2 * r * r * r;
+            by_inv += 2 * r * r * w;
+            fl_x += 2 * r * r * sd;
+            by_x += (r * r + 2 * r * sd) * w;
+            fl_s += 2 * r * sd * sd;
+            by_s += (2 * r * sd + sd * sd) * w;
+        }
+        {
+            Prof pr(this, "gj_inverse", fl_inv, by_inv);
+            batched_gj_inverse<T>(nb, B.r.data(), B.d_r.p, B.d_off_rr.p, Ainv, ws_info.p, gj_ws,
+                                  stream);
+        }
         timed(ST_INVERSE, t0);
         std::vector<int> info(nb);
         HIFDE_CHECK(hipMemcpyAsync(info.data(), ws_info.p, nb * sizeof(int),
                                    hipMemcpyDeviceToHost, stream));
-        batched_gemm_nn<T>(nb, B.r.data(), B.s.data(), B.r.data(), B.d_r.p, B.d_s.p, B.d_r.p,
-                           B.d_off_rr.p, B.d_off_rs.p, B.d_off_rs.p, Ainv, A21, X, (T)1, ws_tiles,
-                           stream);
-        batched_gemm_nn<T>(nb, B.s.data(), B.s.data(), B.r.data(), B.d_s.p, B.d_s.p, B.d_r.p,
-                           B.d_off_rs.p, B.d_off_rs.p, B.d_off_ss.p, A12, X, S, (T)-1, ws_tiles,
-                           stream);
+        {
+            Prof pr(this, "gemm_x", fl_x, by_x);
+            batched_gemm_nn<T>(nb, B.r.data(), B.s.data(), B.r.data(), B.d_r.p, B.d_s.p, B.d_r.p,
+                               B.d_off_rr.p, B.d_off_rs.p, B.d_off_rs.p, Ainv, A21, X, (T)1,
+                               ws_tiles, stream);
+        }
+        {
+            Prof pr(this, "gemm_s", fl_s, by_s);
+            batched_gemm_nn<T>(nb, B.s.data(), B.s.data(), B.r.data(), B.d_s.p, B.d_s.p, B.d_r.p,
+                               B.d_off_rs.p, B.d_off_rs.p, B.d_off_ss.p, A12, X, S, (T)-1,
+                               ws_tiles, stream);
+        }
         timed(ST_GEMM, t0);
         for (int b = 0; b < nb; ++b)
             if (info[b] && !lu_info) lu_info = info[b];
@@ -346,15 +445,21 @@ struct hifde_factor {
     void scatter(BoxBatch& B, const T* S, bool two_pass, clk::time_point& t0) {
         // DeleteRow/DeleteCol of the eliminated DOFs, then Sparse::Add of S
         // (src/Factor.cpp:301-309).  Compaction drops the deleted pairs.
-        table_compact(table, d_alive.p, (size_t)B.tot_ss, stream);
+        {
+            Prof pr(this, "compact", 0.0, (double)table.cap * 16 * 2);
+            table_compact(table, d_alive.p, (size_t)B.tot_ss, stream);
+        }
         timed(ST_COMPACT, t0);
-        if (two_pass) {
-            for (int pass = 0; pass < 2; ++pass)
-                table_scatter<T>(table, B.nbox, B.d_s.p, B.d_skel_off.p, B.d_skel_gid.p,
-                                 B.d_skel_pass.p, B.d_off_ss.p, B.tot_ss, S, pass, stream);
-        } else {
-            table_scatter<T>(table, B.nbox, B.d_s.p, B.d_skel_off.p, B.d_skel_gid.p, nullptr,
-                             B.d_off_ss.p, B.tot_ss, S, -1, stream);
+        {
+            Prof pr(this, "scatter", 0.0, (double)B.tot_ss * (sizeof(T) + 32));
+            if (two_pass) {
+                for (int pass = 0; pass < 2; ++pass)
+                    table_scatter<T>(table, B.nbox, B.d_s.p, B.d_skel_off.p, B.d_skel_gid.p,
+                                     B.d_skel_pass.p, B.d_off_ss.p, B.tot_ss, S, pass, stream);
+            } else {
+                table_scatter<T>(table, B.nbox, B.d_s.p, B.d_skel_off.p, B.d_skel_gid.p, nullptr,
+                                 B.d_off_ss.p, B.tot_ss, S, -1, stream);
+            }
         }
         timed(ST_SCATTER, t0);
     }
@@ -376,13 +481,16 @@ struct hifde_factor {
         T* S = reinterpret_cast<T*>(ws_c.p);
         // DenseSubmatrix of the cell's DOFs (include/Schur.hpp:44-61), only
         // the three blocks Schur reads.
-        table_gather<T>(table, B.nbox, B.d_red_gid.p, B.d_red_off.p, B.d_red_gid.p,
-                        B.d_red_off.p, B.d_off_rr.p, B.tot_rr, reinterpret_cast<T*>(B.A22.p),
-                        stream);
-        table_gather<T>(table, B.nbox, B.d_red_gid.p, B.d_red_off.p, B.d_skel_gid.p,
-                        B.d_skel_off.p, B.d_off_rs.p, B.tot_rs, A21, stream);
-        table_gather<T>(table, B.nbox, B.d_skel_gid.p, B.d_skel_off.p, B.d_red_gid.p,
-                        B.d_red_off.p, B.d_off_rs.p, B.tot_rs, A12, stream);
+        {
+            Prof pr_g(this, "gather", 0.0, (double)(B.tot_rr + 2 * B.tot_rs) * (sizeof(T) + 16));
+            table_gather<T>(table, B.nbox, B.d_red_gid.p, B.d_red_off.p, B.d_red_gid.p,
+                            B.d_red_off.p, B.d_off_rr.p, B.tot_rr, reinterpret_cast<T*>(B.A22.p),
+                            stream);
+            table_gather<T>(table, B.nbox, B.d_red_gid.p, B.d_red_off.p, B.d_skel_gid.p,
+                            B.d_skel_off.p, B.d_off_rs.p, B.tot_rs, A21, stream);
+            table_gather<T>(table, B.nbox, B.d_skel_gid.p, B.d_skel_off.p, B.d_red_gid.p,
+                            B.d_red_off.p, B.d_off_rs.p, B.tot_rs, A12, stream);
+        }
         timed(ST_GATHER, t0);
         schur_batch<T>(B, A21, A12, S, t0);
         mark_eliminated(B);
@@ -419,15 +527,27 @@ struct hifde_factor {
         ws_C.reserve(offC.back());
         ws_W.reserve(offC.back());
         ws_T.reserve(offC.back());
-        table_gather<double>(table, nf, ws_rows.p, ws_rows_off.p, ws_cols.p, ws_cols_off.p,
-                             ws_offM.p, offM.back(), ws_M.p, stream);
-        table_gather<double>(table, nf, ws_cols.p, ws_cols_off.p, ws_cols.p, ws_cols_off.p,
-                             ws_offC.p, offC.back(), ws_C.p, stream);
+        {
+            Prof pr(this, "gather", 0.0, (double)(offM.back() + offC.back()) * (8 + 16));
+            table_gather<double>(table, nf, ws_rows.p, ws_rows_off.p, ws_cols.p, ws_cols_off.p,
+                                 ws_offM.p, offM.back(), ws_M.p, stream);
+            table_gather<double>(table, nf, ws_cols.p, ws_cols_off.p, ws_cols.p, ws_cols_off.p,
+                                 ws_offC.p, offC.back(), ws_C.p, stream);
+        }
         timed(ST_GATHER, t0);
         ws_k.reserve(nf);
         ws_perm.reserve(F.cols_off.back());
-        batched_interp_decomp(nf, ws_m.p, ws_n.p, ws_offM.p, ws_M.p, eps, ws_perm.p,
-                              ws_cols_off.p, ws_k.p, ws_W.p, ws_offC.p, max_n, max_m, stream);
+        {
+            double fl = 0, by = 0;
+            for (int f = 0; f < nf; ++f) {
+                double m = F.m[f], n = F.n[f], k = std::min(m, n);
+                fl += 4 * m * n * k - 2 * k * k * (m + n) + 4.0 / 3.0 * k * k * k;
+                by += 8 * (2 * m * n + n * n);
+            }
+            Prof pr(this, "interp_decomp", fl, by);
+            batched_interp_decomp(nf, ws_m.p, ws_n.p, ws_offM.p, ws_M.p, eps, ws_perm.p,
+                                  ws_cols_off.p, ws_k.p, ws_W.p, ws_offC.p, max_n, max_m, stream);
+        }
         timed(ST_ID, t0);
         std::vector<int> kk(nf), perm(F.cols_off.back());
         HIFDE_CHECK(hipMemcpyAsync(kk.data(), ws_k.p, nf * sizeof(int), hipMemcpyDeviceToHost,
@@ -469,10 +589,19 @@ struct hifde_factor {
         T* A21 = reinterpret_cast<T*>(ws_a.p);
         T* A12 = reinterpret_cast<T*>(ws_b.p);
         T* S = reinterpret_cast<T*>(ws_c.p);
-        batched_face_transform<T>(nf, ws_n.p, ws_k.p, ws_perm.p, ws_cols_off.p, ws_C.p,
-                                  ws_offC.p, ws_W.p, ws_offC.p, ws_T.p,
-                                  reinterpret_cast<T*>(B.A22.p), A21, A12, B.d_off_rr.p,
-                                  B.d_off_rs.p, stream);
+        {
+            double fl = 0, by = 0;
+            for (int f = 0; f < nf; ++f) {
+                double k = B.s[f], q = B.r[f], n = k + q;
+                fl += 4 * k * k * q + 4 * k * q * q;
+                by += 8 * (n * n + k * q) + sizeof(T) * (q * q + 2 * k * q);
+            }
+            Prof pr(this, "face_transform", fl, by);
+            batched_face_transform<T>(nf, ws_n.p, ws_k.p, ws_perm.p, ws_cols_off.p, ws_C.p,
+                                      ws_offC.p, ws_W.p, ws_offC.p, ws_T.p,
+                                      reinterpret_cast<T*>(B.A22.p), A21, A12, B.d_off_rr.p,
+                                      B.d_off_rs.p, stream);
+        }
         // W (k x q) kept for Apply, packed at the r*s offsets.
         B.W.reserve(B.tot_rs * sizeof(T));
         pack_w<T>(B);
@@ -504,6 +633,16 @@ struct hifde_factor {
         faces.clear();
         cells.resize(L);
         faces.resize(L);
+        factored = false;
+        {
+            // Initialize(): the level matrix starts as the assembled operator.
+            auto t0 = clk::now();
+            long long nnz = (long long)h_ci.size();
+            table_init(table, (size_t)nnz, stream);
+            Prof pr(this, "table_insert", 0.0, (double)nnz * (4 + 4 + 8 + 16));
+            table_insert_coo(table, d_rr.p, d_ci.p, d_v.p, nnz, stream);
+            timed(ST_UPLOAD, t0);
+        }
         for (int level = 0; level < L; ++level) {
             level_cells<T>(level);
             if (level < L - 1) level_faces<T>(level);
@@ -513,6 +652,7 @@ struct hifde_factor {
         table.cap = 0;
         ws_a.release(); ws_b.release(); ws_c.release();
         ws_M.release(); ws_C.release(); ws_W.release(); ws_T.release();
+        resolve_profile();
         factored = true;
     }
 
@@ -521,7 +661,12 @@ struct hifde_factor {
     void upd(BoxBatch& B, const DevBuf<unsigned char>& Mbuf, int op, int pass, double alpha,
              double* u) {
         int maxd = 0;
-        for (int b = 0; b < B.nbox; ++b) maxd = std::max(maxd, std::max(B.r[b], B.s[b]));
+        double by = 0;
+        for (int b = 0; b < B.nbox; ++b) {
+            maxd = std::max(maxd, std::max(B.r[b], B.s[b]));
+            by += (double)B.r[b] * B.s[b] * sizeof(T) + 16.0 * (B.r[b] + B.s[b]);
+        }
+        Prof pr(this, "apply_update", 2.0 * B.tot_rs, by);
         apply_update<T>(B.nbox, B.d_r.p, B.d_s.p, B.d_red_off.p, B.d_skel_off.p, B.d_red_gid.p,
                         B.d_skel_gid.p, pass >= 0 ? B.d_skel_pass.p : nullptr, B.d_off_rs.p,
                         reinterpret_cast<const T*>(Mbuf.p), op, pass, alpha, u, maxd, stream);
@@ -530,6 +675,8 @@ struct hifde_factor {
     void diag(BoxBatch& B, const DevBuf<unsigned char>& Mbuf, double* u) {
         int maxr = 0;
         for (int x : B.r) maxr = std::max(maxr, x);
+        Prof pr(this, "apply_diag", 2.0 * B.tot_rr,
+                (double)B.tot_rr * sizeof(T) + 16.0 * (B.red_off.empty() ? 0 : B.red_off.back()));
         apply_diag<T>(B.nbox, B.d_r.p, B.d_red_off.p, B.d_red_gid.p, B.d_off_rr.p,
                       reinterpret_cast<const T*>(Mbuf.p), u, maxr, stream);
     }
@@ -587,6 +734,7 @@ struct hifde_factor {
             if (precision == HIFDE_MIXED) apply_one<float>(d_u + c * ldu, inv);
             else apply_one<double>(d_u + c * ldu, inv);
         }
+        resolve_profile();
     }
 
     // ------------------------------------------------------------------
@@ -604,6 +752,7 @@ struct hifde_factor {
     }
 
     void spmv_dev(const double* x, double* y) {
+        Prof pr(this, "spmv", 2.0 * h_ci.size(), 12.0 * h_ci.size() + 20.0 * NC * NC * NC);
         csr_spmv(NC * NC * NC, d_rp.p, d_ci.p, d_v.p, x, y, stream);
     }
 
@@ -646,6 +795,7 @@ struct hifde_factor {
             if (it > maxit) it = maxit;
         }
         HIFDE_CHECK(hipStreamSynchronize(stream));
+        resolve_profile();
         if (iters) *iters = it;
         if (relres) *relres = rnorm / bnorm;
     }
@@ -691,10 +841,7 @@ struct hifde_factor {
         std::vector<int> rr(h_ci.size());
         for (long long i = 0; i < nd; ++i)
             for (int p = h_rp[i]; p < h_rp[i + 1]; ++p) rr[p] = (int)i;
-        DevBuf<int> d_rr;
         d_rr.upload(rr, stream);
-        table_init(table, h_ci.size(), stream);
-        table_insert_coo(table, d_rr.p, d_ci.p, d_v.p, (long long)h_ci.size(), stream);
         HIFDE_CHECK(hipStreamSynchronize(stream));
         have_matrix = true;
         factored = false;
@@ -978,6 +1125,39 @@ int hifde_stage_times(const hifde_factor* f, double* out, int nout) {
     return n;
 }
 
+int hifde_set_profiling(hifde_factor* f, int on) {
+    if (!f) return HIFDE_ERR_ARG;
+    f->profiling = on != 0;
+    return HIFDE_OK;
+}
+
+int hifde_reset_kernel_stats(hifde_factor* f) {
+    return guarded([&]() -> int {
+        if (!f) throw Error(HIFDE_ERR_ARG, "null");
+        f->resolve_profile();
+        for (auto& k : f->fam_stats) k = hifde_factor::KStat();
+        return HIFDE_OK;
+    });
+}
+
+int hifde_kernel_stats(const hifde_factor* f, int idx, char* name, int name_len, double* out) {
+    if (!f) return HIFDE_ERR_ARG;
+    int n = (int)f->fam_names.size();
+    if (idx < 0 || idx >= n) return n;
+    if (name && name_len > 0) {
+        std::strncpy(name, f->fam_names[idx].c_str(), name_len - 1);
+        name[name_len - 1] = 0;
+    }
+    if (out) {
+        const auto& k = f->fam_stats[idx];
+        out[0] = (double)k.launches;
+        out[1] = k.sec;
+        out[2] = k.flops;
+        out[3] = k.bytes;
+    }
+    return n;
+}
+
 const char* hifde_stage_name(int stage) {
     return (stage >= 0 && stage < ST_COUNT) ? kStageNames[stage] : "";
 }
@@ -1061,7 +1241,8 @@ int schur_batch_impl(int nb, const int* r, const int* s, const double* A22, cons
     dS.reserve(oss.back());
     dinfo.reserve(nb);
     HIFDE_CHECK(hipMemset(dinfo.p, 0, nb * sizeof(int)));
-    batched_gj_inverse<T>(nb, dr.p, drr.p, dA.p, dinfo.p, max_r, st);
+    GJWork gw;
+    batched_gj_inverse<T>(nb, hr.data(), dr.p, drr.p, dA.p, dinfo.p, gw, st);
     batched_gemm_nn<T>(nb, hr.data(), hs.data(), hr.data(), dr.p, ds.p, dr.p, drr.p, drs.p,
                        drs.p, dA.p, dA21.p, dX.p, (T)1, tiles, st);
     batched_gemm_nn<T>(nb, hs.data(), hs.data(), hr.data(), ds.p, ds.p, dr.p, drs.p, drs.p,

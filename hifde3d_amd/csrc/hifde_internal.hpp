@@ -116,9 +116,15 @@ void table_compact(LevelTable& t, const unsigned char* d_alive, size_t extra, hi
 size_t table_count(const LevelTable& t, hipStream_t s);
 
 // dense batched kernels
+// Workspace of the blocked inverse (pivots, pivot rows, tile list).
+struct GJWork {
+    DevBuf<int> piv, piv_off, tile_off;
+    DevBuf<long long> b_off;
+    DevBuf<unsigned char> B;
+};
 template <class T>
-void batched_gj_inverse(int nbox, const int* d_n, const long long* d_off, T* d_A, int* d_info,
-                        int max_n, hipStream_t s);
+void batched_gj_inverse(int nbox, const int* h_n, const int* d_n, const long long* d_off, T* d_A,
+                        int* d_info, GJWork& ws, hipStream_t s);
 template <class T>
 void batched_gemm_nn(int nbox, const int* h_m, const int* h_n, const int* h_k, const int* d_m,
                      const int* d_n, const int* d_k, const long long* d_offA,

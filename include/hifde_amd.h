@@ -107,6 +107,13 @@ int hifde_box_data(const hifde_factor* f, int level, int kind, int box, int* red
  * stages written (names from hifde_stage_name). */
 int hifde_stage_times(const hifde_factor* f, double* out, int nout);
 const char* hifde_stage_name(int stage);
+/* Per-kernel-family device time, recorded with HIP events on the factor's
+ * stream while profiling is on.  hifde_kernel_stats returns the number of
+ * families; for 0 <= idx < that number it fills name and
+ * out[0..3] = launches, seconds, algorithmic flops, algorithmic bytes. */
+int hifde_set_profiling(hifde_factor* f, int on);
+int hifde_reset_kernel_stats(hifde_factor* f);
+int hifde_kernel_stats(const hifde_factor* f, int idx, char* name, int name_len, double* out);
 /* Bytes of factor data held on the device. */
 long long hifde_factor_bytes(const hifde_factor* f);
 
