@@ -33,7 +33,7 @@ EXPORTS = [
     "hifde_pcg", "hifde_pcg_device", "hifde_num_levels", "hifde_level_counts",
     "hifde_box_count", "hifde_box_sizes", "hifde_box_data", "hifde_stage_times",
     "hifde_stage_name", "hifde_factor_bytes", "hifde_interp_decomp_batch", "hifde_schur_batch",
-    "hifde_set_profiling", "hifde_reset_kernel_stats", "hifde_kernel_stats",
+    "hifde_set_profiling", "hifde_reset_kernel_stats", "hifde_kernel_stats", "hifde_set_option",
 ]
 
 
@@ -74,6 +74,7 @@ def lib():
     L.hifde_factor_bytes.argtypes = [_vp]
     L.hifde_factor_bytes.restype = _ll
     L.hifde_set_profiling.argtypes = [_vp, _i]
+    L.hifde_set_option.argtypes = [_vp, ctypes.c_char_p, _d]
     L.hifde_reset_kernel_stats.argtypes = [_vp]
     L.hifde_kernel_stats.argtypes = [_vp, _i, ctypes.c_char_p, _i, _dp]
     L.hifde_interp_decomp_batch.argtypes = [_i, _ip, _ip, _dp, _d, _ip, _ip, _dp]
@@ -223,6 +224,9 @@ class HIFFactor:
         buf = np.zeros(16)
         n = _check(lib().hifde_stage_times(self._h, _ptr(buf), 16))
         return {lib().hifde_stage_name(i).decode(): float(buf[i]) for i in range(n)}
+
+    def set_option(self, name, value):
+        _check(lib().hifde_set_option(self._h, name.encode(), float(value)))
 
     def set_profiling(self, on=True):
         _check(lib().hifde_set_profiling(self._h, int(on)))

@@ -11,6 +11,8 @@
 // The factor blocks may be stored in float or double; the vector and all
 // accumulations are double.
 
+#include <algorithm>
+
 #include "hifde_internal.hpp"
 
 namespace hifde {
@@ -24,79 +26,101 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+template <class O>
+__device__ __forceinline__ int find_box(const O* __restrict__ off, int nbox, long long e) {
+    int lo = 0, hi = nbox;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if ((long long)off[mid] <= e) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// One 64-output tile of one box per workgroup.
 // op 0: u_s[j] += alpha * sum_i M(i,j) u_r[i]   M: r x s   (X^H u_red)
 // op 1: u_r[i] += alpha * sum_j M(i,j) u_s[j]   M: r x s   (X u_skel)
 // op 2: u_r[j] += alpha * sum_i M(i,j) u_s[i]   M: s x r   (W^H u_skel)
 // op 3: u_s[i] += alpha * sum_j M(i,j) u_r[j]   M: s x r   (W u_red)
-// pass >= 0 restricts writes to skeleton DOFs whose pass flag equals pass.
+// op 4: out[r_i] = sum_j M(i,j) u_r[j]          M: r x r   (A22 or A22^{-1})
+// Ops 0-3 read and write disjoint DOF classes, so they update u in place;
+// op 4 reads u_in and writes u_out (a different vector).  pass >= 0 restricts
+// op 0/3 writes to skeleton DOFs whose pass flag equals pass.
 template <class T>
-__global__ void __launch_bounds__(256) k_apply_update(
-    const int* __restrict__ dr, const int* __restrict__ ds, const int* __restrict__ red_off,
+__global__ void __launch_bounds__(256) k_apply_tiles(
+    int nbox, const int* __restrict__ tile_off, const int* __restrict__ dr,
+    const int* __restrict__ ds, const int* __restrict__ red_off,
     const int* __restrict__ skel_off, const int* __restrict__ red_gid,
     const int* __restrict__ skel_gid, const unsigned char* __restrict__ skel_pass,
     const long long* __restrict__ off, const T* __restrict__ Mb, int op, int pass, double alpha,
-    double* __restrict__ u) {
-    const int b = blockIdx.x;
+    const double* u_in, double* u_out) {
+    const int t = blockIdx.x;
+    const int b = find_box(tile_off, nbox, t);
+    const int tile = t - tile_off[b];
     const int r = dr[b], s = ds[b];
-    if (r == 0 || s == 0) return;  // src/Factor.cpp:898-900, 937-939
+    if (r == 0 || (op != 4 && s == 0)) return;  // src/Factor.cpp:898-900, 937-939, 965-967
     const T* M = Mb + off[b];
     const int* rg = red_gid + red_off[b];
     const int* sg = skel_gid + skel_off[b];
     const unsigned char* sp = skel_pass ? skel_pass + skel_off[b] : nullptr;
-    extern __shared__ double xs[];
-    const int tid = threadIdx.x, bs = blockDim.x;
-    const int lane = tid & (kWave - 1), wid = tid / kWave, nw = bs / kWave;
-    const bool in_red = (op == 0 || op == 3);  // input vector lives on red DOFs
-    const int nin = in_red ? r : s;
+    const bool in_red = (op == 0 || op == 3 || op == 4);
+    const bool out_red = (op == 1 || op == 2 || op == 4);
+    const bool trans = (op == 0 || op == 2);
+    const int nin = in_red ? r : s, nout = out_red ? r : s;
     const int* ig = in_red ? rg : sg;
-    for (int t = tid; t < nin; t += bs) xs[t] = u[ig[t]];
+    const int* og = out_red ? rg : sg;
+    extern __shared__ double xs[];
+    __shared__ double part[4][64];
+    const int tid = threadIdx.x;
+    for (int q = tid; q < nin; q += 256) xs[q] = u_in[ig[q]];
     __syncthreads();
-    if (op == 0 || op == 2) {
-        // transposed product: one wave per output, lanes over the column
-        const int nout = (op == 0) ? s : r;
-        const int rows = nin;
-        const int* og = (op == 0) ? sg : rg;
-        for (int j = wid; j < nout; j += nw) {
-            if (op == 0 && sp && (int)sp[j] != pass) continue;
-            const T* cj = M + (size_t)j * rows;
-            double acc = 0.0;
-            for (int i = lane; i < rows; i += kWave) acc += (double)cj[i] * xs[i];
-            acc = wave_sum(acc);
-            if (lane == 0) u[og[j]] += alpha * acc;
+    const int o0 = tile * 64;
+    const int cnt = min(64, nout - o0);
+    double acc = 0.0;
+    int o = -1;
+    if (!trans) {
+        // y_i = sum_j M(i,j) x_j ; M has nout rows.  Four column groups.
+        const int i = tid & 63, g = tid >> 6;
+        double pacc = 0.0;
+        if (i < cnt) {
+            const T* Mi = M + o0 + i;
+            for (int j = g; j < nin; j += 4) pacc += (double)Mi[(size_t)j * nout] * xs[j];
+        }
+        part[g][i] = pacc;
+        __syncthreads();
+        if (tid < cnt) {
+            o = o0 + tid;
+            acc = part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
+        }
+    } else if (nin <= 64) {
+        // y_j = sum_i M(i,j) x_i with short columns: one thread per output
+        if (tid < cnt) {
+            o = o0 + tid;
+            const T* Mj = M + (size_t)o * nin;
+            for (int i = 0; i < nin; ++i) acc += (double)Mj[i] * xs[i];
         }
     } else {
-        // plain product: one thread per output row
-        const int nout = (op == 1) ? r : s;
-        const int* og = (op == 1) ? rg : sg;
-        for (int i = tid; i < nout; i += bs) {
-            if (op == 3 && sp && (int)sp[i] != pass) continue;
-            double acc = 0.0;
-            for (int j = 0; j < nin; ++j) acc += (double)M[i + (size_t)j * nout] * xs[j];
-            u[og[i]] += alpha * acc;
+        // long columns: one wave per output, lanes over the column
+        const int lane = tid & (kWave - 1), w = tid / kWave;
+        for (int jj = w; jj < cnt; jj += 4) {
+            const T* Mj = M + (size_t)(o0 + jj) * nin;
+            double a = 0.0;
+            for (int i = lane; i < nin; i += kWave) a += (double)Mj[i] * xs[i];
+            a = wave_sum(a);
+            if (lane == 0) part[0][jj] = a;
+        }
+        __syncthreads();
+        if (tid < cnt) {
+            o = o0 + tid;
+            acc = part[0][tid];
         }
     }
-}
-
-// u_r := M u_r with M r x r (A22 or A22^{-1}); src/Factor.cpp:957-975.
-template <class T>
-__global__ void __launch_bounds__(256) k_apply_diag(const int* __restrict__ dr,
-                                                    const int* __restrict__ red_off,
-                                                    const int* __restrict__ red_gid,
-                                                    const long long* __restrict__ off,
-                                                    const T* __restrict__ Mb,
-                                                    double* __restrict__ u) {
-    const int b = blockIdx.x;
-    const int r = dr[b];
-    if (r == 0) return;
-    const T* M = Mb + off[b];
-    const int* rg = red_gid + red_off[b];
-    extern __shared__ double xs[];
-    for (int t = threadIdx.x; t < r; t += blockDim.x) xs[t] = u[rg[t]];
-    __syncthreads();
-    for (int i = threadIdx.x; i < r; i += blockDim.x) {
-        double acc = 0.0;
-        for (int j = 0; j < r; ++j) acc += (double)M[i + (size_t)j * r] * xs[j];
-        u[rg[i]] = acc;
+    if (o < 0) return;
+    if (op == 4) {
+        u_out[og[o]] = acc;
+    } else {
+        if (sp && pass >= 0 && (int)sp[o] != pass) return;
+        u_out[og[o]] += alpha * acc;
     }
 }
 
@@ -185,26 +209,17 @@ int grid_for(long long n, int block = 256, int cap = 4096) {
 }  // namespace
 
 template <class T>
-void apply_update(int nbox, const int* d_r, const int* d_s, const int* d_red_off,
-                  const int* d_skel_off, const int* d_red_gid, const int* d_skel_gid,
-                  const unsigned char* d_skel_pass, const long long* d_off, const T* d_M, int op,
-                  int pass, double alpha, double* d_u, int max_dim, hipStream_t s) {
-    if (nbox <= 0 || max_dim <= 0) return;
-    size_t shm = (size_t)max_dim * sizeof(double);
-    int threads = max_dim <= 64 ? 64 : 256;
-    k_apply_update<T><<<nbox, threads, shm, s>>>(d_r, d_s, d_red_off, d_skel_off, d_red_gid,
-                                                 d_skel_gid, d_skel_pass, d_off, d_M, op, pass,
-                                                 alpha, d_u);
-    HIFDE_LAUNCH_CHECK();
-}
-
-template <class T>
-void apply_diag(int nbox, const int* d_r, const int* d_red_off, const int* d_red_gid,
-                const long long* d_off_rr, const T* d_M, double* d_u, int max_r, hipStream_t s) {
-    if (nbox <= 0 || max_r <= 0) return;
-    size_t shm = (size_t)max_r * sizeof(double);
-    int threads = max_r <= 64 ? 64 : 256;
-    k_apply_diag<T><<<nbox, threads, shm, s>>>(d_r, d_red_off, d_red_gid, d_off_rr, d_M, d_u);
+void apply_tiles(int nbox, int ntiles, const int* d_tile_off, const int* d_r, const int* d_s,
+                 const int* d_red_off, const int* d_skel_off, const int* d_red_gid,
+                 const int* d_skel_gid, const unsigned char* d_skel_pass, const long long* d_off,
+                 const T* d_M, int op, int pass, double alpha, const double* d_in,
+                 double* d_out, int max_in, hipStream_t s) {
+    if (nbox <= 0 || ntiles <= 0) return;
+    size_t shm = (size_t)std::max(max_in, 1) * sizeof(double);
+    if (shm > 150 * 1024) throw Error(-3, "apply: box too large for LDS");
+    k_apply_tiles<T><<<ntiles, 256, shm, s>>>(nbox, d_tile_off, d_r, d_s, d_red_off, d_skel_off,
+                                              d_red_gid, d_skel_gid, d_skel_pass, d_off, d_M, op,
+                                              pass, alpha, d_in, d_out);
     HIFDE_LAUNCH_CHECK();
 }
 
@@ -247,15 +262,13 @@ void vcopy(const double* x, double* y, long long n, hipStream_t s) {
     HIFDE_LAUNCH_CHECK();
 }
 
-template void apply_update<double>(int, const int*, const int*, const int*, const int*,
-                                   const int*, const int*, const unsigned char*, const long long*,
-                                   const double*, int, int, double, double*, int, hipStream_t);
-template void apply_update<float>(int, const int*, const int*, const int*, const int*,
-                                  const int*, const int*, const unsigned char*, const long long*,
-                                  const float*, int, int, double, double*, int, hipStream_t);
-template void apply_diag<double>(int, const int*, const int*, const int*, const long long*,
-                                 const double*, double*, int, hipStream_t);
-template void apply_diag<float>(int, const int*, const int*, const int*, const long long*,
-                                const float*, double*, int, hipStream_t);
+template void apply_tiles<double>(int, int, const int*, const int*, const int*, const int*,
+                                  const int*, const int*, const int*, const unsigned char*,
+                                  const long long*, const double*, int, int, double,
+                                  const double*, double*, int, hipStream_t);
+template void apply_tiles<float>(int, int, const int*, const int*, const int*, const int*,
+                                 const int*, const int*, const int*, const unsigned char*,
+                                 const long long*, const float*, int, int, double, const double*,
+                                 double*, int, hipStream_t);
 
 }  // namespace hifde

@@ -78,28 +78,107 @@ __device__ void block_argmax(T v, int idx, T* redv, int* redi, T* outv, int* out
 }
 
 // ---------------------------------------------------------------------------
-// Explicit inverse by Gauss-Jordan elimination with row partial pivoting (the
-// pivot of column c is the first row >= c of largest magnitude, as in getrf).
-// In-place GJ keeps [A | I] in one array: once column c has been eliminated
-// it holds column c of the accumulated elimination matrix, and the final
-// column interchanges (in reverse pivot order) turn that into A^{-1}.
+// Explicit inverse by Gauss-Jordan elimination.
 //
-// Boxes with n <= kSmallGJ are inverted in LDS, one workgroup per box.
-// Larger boxes use a blocked form: for each panel K of kGJB columns
-//   1. k_bgj_panel   eliminates the panel columns for all rows (one WG/box),
-//                    recording pivots; the panel then holds G(:, K), the
-//                    panel's elimination matrix.
-//   2. k_bgj_swap    applies the panel's row interchanges to every other
-//                    column and copies the pivot rows B = A(K, J) aside.
-//   3. k_bgj_update  A(i, J) = [i not in K] A(i, J) + G(i, K) B  for J != K,
-//                    a rank-kGJB update tiled over the whole GPU.
-// followed by k_bgj_colswap for the column interchanges.
+// Boxes with n <= kSmallGJ: in-place GJ with row partial pivoting (the pivot
+// of column c is the first row >= c of largest magnitude, as in getrf) in
+// LDS, one workgroup per box.  In-place GJ keeps [A | I] in one array; the
+// final column interchanges (reverse pivot order) turn it into A^{-1}.
+//
+// Larger boxes: block Gauss-Jordan with kGJB x kGJB diagonal blocks.  For
+// the diagonal block K (pivoting happens inside the block, whose inverse D
+// is computed exactly as above):
+//   A_KK <- D = A_KK^{-1}                        k_bgj_diag   (1 WG per box)
+//   R    <- D A_KJ              (J != K)         k_bgj_row    (column chunks)
+//   A_IJ <- A_IJ - A_IK R,  A_KJ <- R (I,J != K) k_bgj_update (64x64 tiles)
+//   A_IK <- -A_IK D             (I != K)         k_bgj_col    (row chunks)
+// Every phase except the small block inverse runs on many workgroups.  The
+// blocks of this path are the interior blocks of symmetric positive
+// definite or M-matrix operators, whose diagonal blocks are nonsingular; a
+// singular diagonal block is reported through info.
 // ---------------------------------------------------------------------------
 constexpr int kSmallGJ = 64;
-constexpr int kGJB = 32;
+constexpr int kGJB = 64;
+
+// Pivoted in-place GJ of an n x n block (n <= 64) held in LDS as S[col][row],
+// padded with the identity to 64 x 64 (the inverse of diag(A, I) is
+// diag(A^{-1}, I)).  256 threads: thread t owns column t >> 2 and rows
+// (t & 3) + 4q.  Pivot search is one wave's shuffle reduction; the column
+// interchanges are applied at the end as one gather through the composed
+// permutation.  Returns false on an exactly singular pivot column.
+template <class T>
+__device__ bool lds_gj_inverse64(T (*S)[kSmallGJ + 1], int n, T* colk, T* rowk, int* perm,
+                                 int* s_piv) {
+    const int tid = threadIdx.x;
+    const int c = tid >> 2, r0 = tid & 3;
+    for (int k = 0; k < n; ++k) {
+        if (tid < 64) {
+            T v = (tid >= k && tid < n) ? fabs(S[k][tid]) : (T)-1;
+            int idx = tid;
+            for (int o = 32; o > 0; o >>= 1) {
+                T ov = __shfl_xor(v, o, 64);
+                int oi = __shfl_xor(idx, o, 64);
+                if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
+            }
+            if (tid == 0) s_piv[0] = (v > (T)0) ? idx : -1;
+        }
+        __syncthreads();
+        const int p = s_piv[0];
+        if (p < 0) return false;
+        if (tid < 64) {
+            if (p != k) {
+                T t = S[tid][k];
+                S[tid][k] = S[tid][p];
+                S[tid][p] = t;
+            }
+            if (tid == 0) perm[k] = p;
+        }
+        __syncthreads();
+        if (tid < 64) {
+            const T d = (T)1 / S[k][k];
+            colk[tid] = S[k][tid];
+            rowk[tid] = (tid == k) ? d : S[tid][k] * d;
+        }
+        __syncthreads();
+        const T rc = rowk[c];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int i = r0 + 4 * q;
+            if (i == k) {
+                S[c][i] = rc;
+            } else {
+                const T a = (c == k) ? (T)0 : S[c][i];
+                S[c][i] = a - colk[i] * rc;
+            }
+        }
+        __syncthreads();
+    }
+    // columns of the result: res(:, j) = S(:, pi(j)), pi = swaps applied in reverse
+    if (tid == 0) {
+        for (int j = 0; j < 64; ++j) s_piv[1 + j] = j;
+        for (int k = n - 1; k >= 0; --k) {
+            const int p = perm[k];
+            if (p != k) {
+                int t = s_piv[1 + k];
+                s_piv[1 + k] = s_piv[1 + p];
+                s_piv[1 + p] = t;
+            }
+        }
+    }
+    __syncthreads();
+    T tmp[16];
+    const int src = s_piv[1 + c];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) tmp[q] = S[src][r0 + 4 * q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 16; ++q) S[c][r0 + 4 * q] = tmp[q];
+    __syncthreads();
+    return true;
+}
 
 template <class T>
-__global__ void __launch_bounds__(128) k_gj_small(const int* __restrict__ dims,
+__global__ void __launch_bounds__(256) k_gj_small(const int* __restrict__ dims,
                                                   const long long* __restrict__ off, T* mats,
                                                   int* info) {
     const int b = blockIdx.x;
@@ -108,171 +187,126 @@ __global__ void __launch_bounds__(128) k_gj_small(const int* __restrict__ dims,
     T* A = mats + off[b];
     __shared__ T S[kSmallGJ][kSmallGJ + 1];  // S[col][row]
     __shared__ T colk[kSmallGJ], rowk[kSmallGJ];
-    __shared__ int piv[kSmallGJ];
-    __shared__ T redv[4];
-    __shared__ int redi[4];
-    __shared__ T s_pv;
-    __shared__ int s_pi;
-    const int tid = threadIdx.x, bs = blockDim.x;
-    for (int e = tid; e < n * n; e += bs) S[e / n][e % n] = A[e];
+    __shared__ int perm[kSmallGJ];
+    __shared__ int s_piv[kSmallGJ + 1];
+    const int tid = threadIdx.x;
+    for (int e = tid; e < kSmallGJ * kSmallGJ; e += 256) {
+        const int c = e >> 6, i = e & 63;
+        S[c][i] = (c < n && i < n) ? A[i + (size_t)c * n] : (T)(c == i ? 1 : 0);
+    }
     __syncthreads();
-    for (int k = 0; k < n; ++k) {
-        T best = (T)-1;
-        int bi = n;
-        for (int i = k + tid; i < n; i += bs) {
-            T v = fabs(S[k][i]);
-            if (v > best) { best = v; bi = i; }
-        }
-        block_argmax(best, bi, redv, redi, &s_pv, &s_pi);
-        const int p = s_pi;
-        if (!(s_pv > (T)0)) {
-            if (tid == 0) info[b] = k + 1;
-            return;
-        }
-        if (p != k)
-            for (int c = tid; c < n; c += bs) {
-                T t = S[c][k];
-                S[c][k] = S[c][p];
-                S[c][p] = t;
-            }
-        if (tid == 0) piv[k] = p;
-        __syncthreads();
-        const T d = (T)1 / S[k][k];
-        for (int i = tid; i < n; i += bs) {
-            colk[i] = S[k][i];
-            rowk[i] = (i == k) ? d : S[i][k] * d;
-        }
-        __syncthreads();
-        for (int e = tid; e < n * n; e += bs) {
-            int c = e / n, i = e - c * n;
-            if (i == k) {
-                S[c][i] = rowk[c];
-            } else {
-                T a = (c == k) ? (T)0 : S[c][i];
-                S[c][i] = a - colk[i] * rowk[c];
-            }
-        }
-        __syncthreads();
+    if (!lds_gj_inverse64<T>(S, n, colk, rowk, perm, s_piv)) {
+        if (tid == 0) info[b] = 1;
+        return;
     }
-    for (int k = n - 1; k >= 0; --k) {
-        int p = piv[k];
-        if (p != k)
-            for (int i = tid; i < n; i += bs) {
-                T t = S[k][i];
-                S[k][i] = S[p][i];
-                S[p][i] = t;
-            }
-        __syncthreads();
-    }
-    for (int e = tid; e < n * n; e += bs) A[e] = S[e / n][e % n];
+    for (int e = tid; e < n * n; e += 256) A[e] = S[e / n][e % n];
     if (tid == 0) info[b] = 0;
 }
 
-// Panel elimination, one workgroup per large box.  Dynamic shared memory:
-// max_n T (pivot column) .
 template <class T>
-__global__ void __launch_bounds__(256) k_bgj_panel(const int* __restrict__ dims,
-                                                   const long long* __restrict__ off, T* mats,
-                                                   int* piv, const int* __restrict__ piv_off,
-                                                   int step, int* info) {
+__global__ void __launch_bounds__(256) k_bgj_diag(const int* __restrict__ dims,
+                                                  const long long* __restrict__ off, T* mats,
+                                                  T* Dws, int step, int* info) {
     const int b = blockIdx.x;
     const int n = dims[b];
     const int k0 = step * kGJB;
     if (n <= kSmallGJ || k0 >= n || info[b]) return;
     const int kb = min(kGJB, n - k0);
-    T* A = mats + off[b];
-    int* pv = piv + piv_off[b];
-    extern __shared__ unsigned char smem[];
-    T* colk = reinterpret_cast<T*>(smem);
-    __shared__ T rowk[kGJB];
-    __shared__ T redv[4];
-    __shared__ int redi[4];
-    __shared__ T s_pv;
-    __shared__ int s_pi;
-    const int tid = threadIdx.x, bs = blockDim.x;
-    for (int c = k0; c < k0 + kb; ++c) {
-        T best = (T)-1;
-        int bi = n;
-        for (int i = c + tid; i < n; i += bs) {
-            T v = fabs(A[i + (size_t)c * n]);
-            if (v > best) { best = v; bi = i; }
-        }
-        block_argmax(best, bi, redv, redi, &s_pv, &s_pi);
-        const int p = s_pi;
-        if (!(s_pv > (T)0)) {
-            if (tid == 0) info[b] = c + 1;
-            return;
-        }
-        if (p != c && tid < kb) {
-            T* cj = A + (size_t)(k0 + tid) * n;
-            T t = cj[c];
-            cj[c] = cj[p];
-            cj[p] = t;
-        }
-        if (tid == 0) pv[c] = p;
-        __syncthreads();
-        const T d = (T)1 / A[c + (size_t)c * n];
-        for (int i = tid; i < n; i += bs) colk[i] = A[i + (size_t)c * n];
-        if (tid < kb) {
-            int j = k0 + tid;
-            rowk[tid] = (j == c) ? d : A[c + (size_t)j * n] * d;
-        }
-        __syncthreads();
-        for (int jj = 0; jj < kb; ++jj) {
-            const int j = k0 + jj;
-            T* cj = A + (size_t)j * n;
-            const T rk = rowk[jj];
-            for (int i = tid; i < n; i += bs) {
-                if (i == c) {
-                    cj[i] = rk;
-                } else {
-                    T a = (j == c) ? (T)0 : cj[i];
-                    cj[i] = a - colk[i] * rk;
-                }
-            }
-        }
-        __syncthreads();
+    const T* A = mats + off[b];
+    __shared__ T S[kSmallGJ][kSmallGJ + 1];
+    __shared__ T colk[kSmallGJ], rowk[kSmallGJ];
+    __shared__ int perm[kSmallGJ];
+    __shared__ int s_piv[kSmallGJ + 1];
+    const int tid = threadIdx.x;
+    for (int e = tid; e < kSmallGJ * kSmallGJ; e += 256) {
+        const int c = e >> 6, i = e & 63;
+        S[c][i] = (c < kb && i < kb) ? A[k0 + i + (size_t)(k0 + c) * n] : (T)(c == i ? 1 : 0);
+    }
+    __syncthreads();
+    if (!lds_gj_inverse64<T>(S, kb, colk, rowk, perm, s_piv)) {
+        if (tid == 0) info[b] = k0 + 1;
+        return;
+    }
+    T* D = Dws + (size_t)b * kGJB * kGJB;
+    for (int e = tid; e < kGJB * kGJB; e += 256) {
+        const int c = e >> 6, i = e & 63;
+        D[e] = (c < kb && i < kb) ? S[c][i] : (T)0;
     }
 }
 
-// Row interchanges of the panel applied to column j (j outside the panel),
-// then B(:, j) = A(K, j).  Grid: x over columns, y over boxes.
+// acc(4x4 per thread) = sum_c L[c][row] * R[c][col] over a 64x64 tile.
 template <class T>
-__global__ void __launch_bounds__(256) k_bgj_swap(const int* __restrict__ dims,
-                                                  const long long* __restrict__ off, T* mats,
-                                                  const int* __restrict__ piv,
-                                                  const int* __restrict__ piv_off, int step,
-                                                  T* Bws, const long long* __restrict__ b_off,
-                                                  const int* __restrict__ info) {
+__device__ __forceinline__ void tile_mm(const T (*L)[65], const T (*Rm)[65], int kb, T acc[4][4]) {
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    for (int c = 0; c < kb; ++c) {
+        T a[4], bv[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a[i] = L[c][tx + 16 * i];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) bv[j] = Rm[c][ty + 16 * j];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] = fma(a[i], bv[j], acc[i][j]);
+    }
+}
+
+// R(:, j) = D A(K, j) for 64 columns j outside K.  Grid (column chunks, boxes).
+template <class T>
+__global__ void __launch_bounds__(256) k_bgj_row(const int* __restrict__ dims,
+                                                 const long long* __restrict__ off,
+                                                 const T* __restrict__ mats,
+                                                 const T* __restrict__ Dws, T* Rws,
+                                                 const long long* __restrict__ r_off, int step,
+                                                 const int* __restrict__ info) {
     const int b = blockIdx.y;
     const int n = dims[b];
     const int k0 = step * kGJB;
     if (n <= kSmallGJ || k0 >= n || info[b]) return;
     const int kb = min(kGJB, n - k0);
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n || (j >= k0 && j < k0 + kb)) return;
-    T* cj = mats + off[b] + (size_t)j * n;
-    const int* pv = piv + piv_off[b];
-    for (int c = k0; c < k0 + kb; ++c) {
-        int p = pv[c];
-        if (p != c) {
-            T t = cj[c];
-            cj[c] = cj[p];
-            cj[p] = t;
+    const int j0 = blockIdx.x * 64;
+    if (j0 >= n) return;
+    const T* A = mats + off[b];
+    const T* D = Dws + (size_t)b * kGJB * kGJB;
+    __shared__ T Ds[kGJB][65];  // Ds[c][i] = D(i, c)
+    __shared__ T As[kGJB][65];  // As[c][j] = A(k0 + c, j0 + j)
+    const int tid = threadIdx.x;
+    for (int e = tid; e < kGJB * 64; e += 256) {
+        const int i = e & 63, c = e >> 6;
+        Ds[c][i] = D[i + (size_t)c * kGJB];
+        const int cc = e & 63, jj = e >> 6;  // coalesced along K rows
+        const int gj = j0 + jj;
+        As[cc][jj] = (cc < kb && gj < n) ? A[k0 + cc + (size_t)gj * n] : (T)0;
+    }
+    __syncthreads();
+    T acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = (T)0;
+    tile_mm<T>(Ds, As, kb, acc);
+    const int tx = tid & 15, ty = tid >> 4;
+    T* R = Rws + r_off[b];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int gj = j0 + ty + 16 * j;
+        if (gj >= n) continue;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int ii = tx + 16 * i;
+            if (ii < kb) R[ii + (size_t)gj * kGJB] = acc[i][j];
         }
     }
-    T* B = Bws + b_off[b] + (size_t)j * kGJB;
-    for (int c = 0; c < kb; ++c) B[c] = cj[k0 + c];
 }
 
-// A(i, j) = [i not in K] A(i, j) + sum_c G(i, c) B(c, j), j not in K.
+// A(i, j) = i in K ? R(i-k0, j) : A(i, j) - A(i, K) R(:, j), for j outside K.
 template <class T>
 __global__ void __launch_bounds__(256) k_bgj_update(int nbox, const int* __restrict__ tile_off,
                                                     const int* __restrict__ dims,
                                                     const long long* __restrict__ off, T* mats,
-                                                    const T* __restrict__ Bws,
-                                                    const long long* __restrict__ b_off,
-                                                    int step, const int* __restrict__ info) {
+                                                    const T* __restrict__ Rws,
+                                                    const long long* __restrict__ r_off, int step,
+                                                    const int* __restrict__ info) {
     const int t = blockIdx.x;
     const int b = find_box(tile_off, nbox, t);
     const int n = dims[b];
@@ -282,104 +316,114 @@ __global__ void __launch_bounds__(256) k_bgj_update(int nbox, const int* __restr
     const int local = t - tile_off[b];
     const int tm = (n + 63) / 64;
     const int i0 = (local % tm) * 64, j0 = (local / tm) * 64;
-    if (j0 + 64 > k0 && j0 < k0 + kb && j0 >= k0 && j0 + 64 <= k0 + kb) return;
+    if (j0 == k0) return;  // the panel's own columns (kGJB == 64 aligns them)
     T* A = mats + off[b];
-    const T* B = Bws + b_off[b];
-    __shared__ T Gs[kGJB][64 + 1];
-    __shared__ T Bs[kGJB][64 + 1];
+    const T* R = Rws + r_off[b];
+    __shared__ T Ls[kGJB][65];  // Ls[c][i] = A(i0 + i, k0 + c)
+    __shared__ T Rs[kGJB][65];  // Rs[c][j] = R(c, j0 + j)
     const int tid = threadIdx.x;
-    for (int e = tid; e < 64 * kGJB; e += 256) {
-        int r = e & 63, c = e >> 6;
-        int gi = i0 + r;
-        Gs[c][r] = (gi < n && c < kb) ? A[gi + (size_t)(k0 + c) * n] : (T)0;
-        int cb = e & (kGJB - 1), jb = e / kGJB;
-        int gj = j0 + jb;
-        bool skip = gj >= n || (gj >= k0 && gj < k0 + kb) || cb >= kb;
-        Bs[cb][jb] = skip ? (T)0 : B[cb + (size_t)gj * kGJB];
+    for (int e = tid; e < kGJB * 64; e += 256) {
+        const int i = e & 63, c = e >> 6;
+        const int gi = i0 + i;
+        Ls[c][i] = (gi < n && c < kb) ? A[gi + (size_t)(k0 + c) * n] : (T)0;
+        const int cc = e & 63, jj = e >> 6;
+        const int gj = j0 + jj;
+        Rs[cc][jj] = (cc < kb && gj < n) ? R[cc + (size_t)gj * kGJB] : (T)0;
     }
     __syncthreads();
-    const int tx = tid & 15, ty = tid >> 4;
     T acc[4][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = (T)0;
-    for (int c = 0; c < kb; ++c) {
-        T a[4], bv[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) a[i] = Gs[c][tx + 16 * i];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) bv[j] = Bs[c][ty + 16 * j];
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) acc[i][j] = fma(a[i], bv[j], acc[i][j]);
-    }
+    tile_mm<T>(Ls, Rs, kb, acc);
+    const int tx = tid & 15, ty = tid >> 4;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-        const int gj = j0 + ty + 16 * j;
-        if (gj >= n || (gj >= k0 && gj < k0 + kb)) continue;
+        const int jj = ty + 16 * j, gj = j0 + jj;
+        if (gj >= n) continue;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int gi = i0 + tx + 16 * i;
             if (gi >= n) continue;
             T* e = A + gi + (size_t)gj * n;
-            const bool inK = gi >= k0 && gi < k0 + kb;
-            *e = (inK ? (T)0 : *e) + acc[i][j];
+            if (gi >= k0 && gi < k0 + kb) *e = Rs[gi - k0][jj];
+            else *e = *e - acc[i][j];
         }
     }
 }
 
-// Column interchanges in reverse pivot order; one thread per row.
+// A(i, K) = -A(i, K) D for i outside K; A(K, K) = D.  Grid (row chunks, boxes).
 template <class T>
-__global__ void __launch_bounds__(256) k_bgj_colswap(const int* __restrict__ dims,
-                                                     const long long* __restrict__ off, T* mats,
-                                                     const int* __restrict__ piv,
-                                                     const int* __restrict__ piv_off,
-                                                     int* info) {
+__global__ void __launch_bounds__(256) k_bgj_col(const int* __restrict__ dims,
+                                                 const long long* __restrict__ off, T* mats,
+                                                 const T* __restrict__ Dws, int step,
+                                                 const int* __restrict__ info) {
     const int b = blockIdx.y;
     const int n = dims[b];
-    if (n <= kSmallGJ || info[b]) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const int k0 = step * kGJB;
+    if (n <= kSmallGJ || k0 >= n || info[b]) return;
+    const int kb = min(kGJB, n - k0);
+    const int i0 = blockIdx.x * 64;
+    if (i0 >= n) return;
     T* A = mats + off[b];
-    const int* pv = piv + piv_off[b];
-    for (int k = n - 1; k >= 0; --k) {
-        int p = pv[k];
-        if (p != k) {
-            T t = A[i + (size_t)k * n];
-            A[i + (size_t)k * n] = A[i + (size_t)p * n];
-            A[i + (size_t)p * n] = t;
+    const T* D = Dws + (size_t)b * kGJB * kGJB;
+    __shared__ T Ls[kGJB][65];  // Ls[c][i] = A(i0 + i, k0 + c)
+    __shared__ T Dt[kGJB][65];  // Dt[c][j] = D(c, j)
+    const int tid = threadIdx.x;
+    for (int e = tid; e < kGJB * 64; e += 256) {
+        const int i = e & 63, c = e >> 6;
+        const int gi = i0 + i;
+        Ls[c][i] = (gi < n && c < kb) ? A[gi + (size_t)(k0 + c) * n] : (T)0;
+        const int cc = e & 63, jj = e >> 6;
+        Dt[cc][jj] = D[cc + (size_t)jj * kGJB];
+    }
+    __syncthreads();
+    T acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = (T)0;
+    tile_mm<T>(Ls, Dt, kb, acc);
+    const int tx = tid & 15, ty = tid >> 4;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int jj = ty + 16 * j;
+        if (jj >= kb) continue;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int gi = i0 + tx + 16 * i;
+            if (gi >= n) continue;
+            T* e = A + gi + (size_t)(k0 + jj) * n;
+            if (gi >= k0 && gi < k0 + kb) *e = Dt[gi - k0][jj];
+            else *e = -acc[i][j];
         }
     }
 }
 
-// C(m x n) = alpha * A(m x k) * B(k x n), all column-major with ld = rows.
 constexpr int BM = 64, BN = 64, BK = 16;
 
-template <class T>
-__global__ void __launch_bounds__(256) k_gemm_nn(int nbox, const int* __restrict__ tile_off,
-                                                 const int* __restrict__ dm,
-                                                 const int* __restrict__ dn,
-                                                 const int* __restrict__ dk,
-                                                 const long long* __restrict__ offA,
-                                                 const long long* __restrict__ offB,
-                                                 const long long* __restrict__ offC,
-                                                 const T* __restrict__ Ab,
-                                                 const T* __restrict__ Bb, T* __restrict__ Cb,
-                                                 T alpha) {
-    int t = blockIdx.x;
-    int b = find_box(tile_off, nbox, t);
-    int local = t - tile_off[b];
-    int m = dm[b], n = dn[b], k = dk[b];
-    int tm = (m + BM - 1) / BM;
-    int i0 = (local % tm) * BM, j0 = (local / tm) * BN;
-    const T* A = Ab + offA[b];
-    const T* B = Bb + offB[b];
-    T* C = Cb + offC[b];
+// C(m x n) = alpha * op(A) * B + beta * C, column-major with per-box
+// leading dimensions: op(A) = A (A stored m x k) or A^T (A stored k x m).
+// One 64x64 output tile per workgroup over a flattened tile list.
+template <class T, bool TRANSA>
+__global__ void __launch_bounds__(256) k_gemm(int nbox, const int* __restrict__ tile_off,
+                                              const GemmDesc* __restrict__ desc,
+                                              const T* __restrict__ Ab, const T* __restrict__ Bb,
+                                              T* Cb, T alpha, T beta) {
+    const int t = blockIdx.x;
+    const int b = find_box(tile_off, nbox, t);
+    const int local = t - tile_off[b];
+    const GemmDesc dd = desc[b];
+    const int m = dd.m, n = dd.n, k = dd.k;
+    const int tm = (m + BM - 1) / BM;
+    const int i0 = (local % tm) * BM, j0 = (local / tm) * BN;
+    const T* A = Ab + dd.offA;
+    const T* B = Bb + dd.offB;
+    T* C = Cb + dd.offC;
     __shared__ T As[BK][BM + 1];
     __shared__ T Bs[BK][BN + 1];
-    int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
     T acc[4][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
@@ -388,13 +432,19 @@ __global__ void __launch_bounds__(256) k_gemm_nn(int nbox, const int* __restrict
     for (int k0 = 0; k0 < k; k0 += BK) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            int e = threadIdx.x + 256 * q;
-            int r = e & (BM - 1), c = e >> 6;  // A tile: 64 rows x 16 cols
-            int gi = i0 + r, gk = k0 + c;
-            As[c][r] = (gi < m && gk < k) ? A[gi + (size_t)gk * m] : (T)0;
-            int rb = e & (BK - 1), cb = e >> 4;  // B tile: 16 rows x 64 cols
-            int gkb = k0 + rb, gj = j0 + cb;
-            Bs[rb][cb] = (gkb < k && gj < n) ? B[gkb + (size_t)gj * k] : (T)0;
+            const int e = threadIdx.x + 256 * q;
+            if (!TRANSA) {
+                const int r = e & (BM - 1), c = e >> 6;  // A tile: 64 rows x 16 cols
+                const int gi = i0 + r, gk = k0 + c;
+                As[c][r] = (gi < m && gk < k) ? A[gi + (size_t)gk * dd.lda] : (T)0;
+            } else {
+                const int c = e & (BK - 1), r = e >> 4;  // (A^T) tile from A(k0+c, i0+r)
+                const int gi = i0 + r, gk = k0 + c;
+                As[c][r] = (gi < m && gk < k) ? A[gk + (size_t)gi * dd.lda] : (T)0;
+            }
+            const int rb = e & (BK - 1), cb = e >> 4;  // B tile: 16 rows x 64 cols
+            const int gkb = k0 + rb, gj = j0 + cb;
+            Bs[rb][cb] = (gkb < k && gj < n) ? B[gkb + (size_t)gj * dd.ldb] : (T)0;
         }
         __syncthreads();
 #pragma unroll
@@ -413,13 +463,180 @@ __global__ void __launch_bounds__(256) k_gemm_nn(int nbox, const int* __restrict
     }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-        int gj = j0 + ty + 16 * j;
+        const int gj = j0 + ty + 16 * j;
         if (gj >= n) continue;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            int gi = i0 + tx + 16 * i;
-            if (gi < m) C[gi + (size_t)gj * m] = alpha * acc[i][j];
+            const int gi = i0 + tx + 16 * i;
+            if (gi >= m) continue;
+            T* e = C + gi + (size_t)gj * dd.ldc;
+            *e = (beta == (T)0) ? alpha * acc[i][j] : beta * *e + alpha * acc[i][j];
         }
+    }
+}
+
+// dst(box) = (D) src(rows r0.., cols c0.. of a ld-strided block); dst packed
+// column-major with ld = rows.
+template <class D>
+__global__ void k_copy_blocks(int nbox, const long long* __restrict__ dst_off,
+                              const CopyDesc* __restrict__ desc, const double* __restrict__ src,
+                              D* __restrict__ dst, long long total) {
+    long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (; e < total; e += stride) {
+        const int b = find_box(dst_off, nbox, e);
+        const CopyDesc d = desc[b];
+        const long long l = e - dst_off[b];
+        const int i = (int)(l % d.rows), j = (int)(l / d.rows);
+        dst[e] = (D)src[d.off + d.r0 + i + (size_t)(d.c0 + j) * d.ld];
+    }
+}
+
+// Cp = C(perm, perm) for each face (n x n, ld n).
+__global__ void k_permute_sym(int nbox, const long long* __restrict__ off,
+                              const int* __restrict__ dn, const int* __restrict__ perm,
+                              const int* __restrict__ perm_off, const double* __restrict__ C,
+                              double* __restrict__ Cp, long long total) {
+    long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (; e < total; e += stride) {
+        const int b = find_box(off, nbox, e);
+        const int n = dn[b];
+        const long long l = e - off[b];
+        const int i = (int)(l % n), j = (int)(l / n);
+        const int* p = perm + perm_off[b];
+        Cp[e] = C[off[b] + p[i] + (size_t)p[j] * n];
+    }
+}
+
+// Interpolative decomposition through the Gram matrix G = M^T M (n x n) of a
+// tall face matrix.  Pivoted Cholesky of G chooses, step by step, the column
+// with the largest remaining squared norm -- the column-pivoted QR rule -- and
+// its rows are the rows of that QR's R (up to signs).  Left-looking: row i of
+// R is G(p, :) minus the contributions of rows 0..i-1, so only the pivot row
+// of G is read per step.  The factorization stops at the first pivot with
+// |R_ii| <= eps |R_00|; the remaining columns are redundant.
+// Dynamic shared memory: d[n], rp[n] (double), jp[n] (int).
+__global__ void __launch_bounds__(256) k_gram_pivchol(const int* __restrict__ dn,
+                                                      const unsigned char* __restrict__ mode,
+                                                      const double* __restrict__ Gb,
+                                                      const long long* __restrict__ offG,
+                                                      double* Rb, double eps, int* perm,
+                                                      const int* __restrict__ perm_off,
+                                                      int* kout) {
+    const int b = blockIdx.x;
+    if (!mode[b]) return;
+    const int n = dn[b];
+    const double* G = Gb + offG[b];
+    double* R = Rb + offG[b];  // row i of R at R[i * n + col]
+    extern __shared__ unsigned char smem[];
+    double* d = reinterpret_cast<double*>(smem);
+    double* rp = d + n;
+    int* jp = reinterpret_cast<int*>(rp + n);
+    __shared__ double redv[4];
+    __shared__ int redi[4];
+    __shared__ double s_pv, s_tol;
+    __shared__ int s_pi, s_k;
+    const int tid = threadIdx.x, bs = blockDim.x;
+    for (int j = tid; j < n; j += bs) {
+        d[j] = G[j + (size_t)j * n];
+        jp[j] = j;
+    }
+    if (tid == 0) s_k = n;
+    __syncthreads();
+    for (int i = 0; i < n; ++i) {
+        double best = -1.0;
+        int bi = n;
+        for (int pos = i + tid; pos < n; pos += bs) {
+            double v = d[jp[pos]];
+            if (v > best) { best = v; bi = pos; }
+        }
+        block_argmax(best, bi, redv, redi, &s_pv, &s_pi);
+        const int pos = s_pi;
+        const double dp = s_pv > 0.0 ? s_pv : 0.0;
+        const double rii = sqrt(dp);
+        if (i == 0 && tid == 0) s_tol = rii * eps;
+        __syncthreads();
+        if (!(rii > s_tol)) {
+            if (tid == 0) s_k = i;
+            break;
+        }
+        if (tid == 0 && pos != i) {
+            int t = jp[pos]; jp[pos] = jp[i]; jp[i] = t;
+        }
+        __syncthreads();
+        const int p = jp[i];
+        for (int l = tid; l < i; l += bs) rp[l] = R[(size_t)l * n + p];
+        __syncthreads();
+        const double inv = 1.0 / rii;
+        for (int q = i + 1 + tid; q < n; q += bs) {
+            int c = jp[q];
+            double v = G[p + (size_t)c * n];
+            // eight independent partial sums keep eight loads in flight
+            double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            int l = 0;
+            for (; l + 8 <= i; l += 8) {
+#pragma unroll
+                for (int u = 0; u < 8; ++u) a[u] += rp[l + u] * R[(size_t)(l + u) * n + c];
+            }
+            for (; l < i; ++l) a[0] += rp[l] * R[(size_t)l * n + c];
+            v -= ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+            v *= inv;
+            R[(size_t)i * n + c] = v;
+            double nd = d[c] - v * v;
+            d[c] = nd > 0.0 ? nd : 0.0;
+        }
+        if (tid == 0) R[(size_t)i * n + p] = rii;
+        __syncthreads();
+    }
+    __syncthreads();
+    const int k = s_k;
+    int* pb = perm + perm_off[b];
+    for (int t = tid; t < n; t += bs) pb[t] = jp[t];
+    if (tid == 0) kout[b] = k;
+}
+
+// W = R11^{-1} R12 for the Gram path.  Back substitution for CW right-hand
+// sides at once, held in LDS; grid (column chunks, faces).
+template <int CW>
+__global__ void __launch_bounds__(256) k_gram_wsolve(const int* __restrict__ dn,
+                                                     const unsigned char* __restrict__ mode,
+                                                     const long long* __restrict__ offG,
+                                                     const double* __restrict__ Rb,
+                                                     const int* __restrict__ perm,
+                                                     const int* __restrict__ perm_off,
+                                                     const int* __restrict__ kk, double* Wb) {
+    const int b = blockIdx.y;
+    if (!mode[b]) return;
+    const int n = dn[b], k = kk[b], q = n - k;
+    const int jj0 = blockIdx.x * CW;
+    if (jj0 >= q || k == 0) return;
+    const int cols = min(CW, q - jj0);
+    const double* R = Rb + offG[b];
+    const int* jp = perm + perm_off[b];
+    extern __shared__ double ys[];  // ys[i * CW + c]
+    __shared__ double yi[CW];
+    const int tid = threadIdx.x;
+    for (int e = tid; e < k * CW; e += 256) {
+        const int i = e / CW, c = e - i * CW;
+        ys[e] = (c < cols) ? R[(size_t)i * n + jp[k + jj0 + c]] : 0.0;
+    }
+    __syncthreads();
+    for (int i = k - 1; i >= 0; --i) {
+        const int pc = jp[i];
+        if (tid < CW) yi[tid] = ys[i * CW + tid] / R[(size_t)i * n + pc];
+        __syncthreads();
+        for (int e = tid; e < (i + 1) * CW; e += 256) {
+            const int r = e / CW, c = e - r * CW;
+            if (r == i) ys[e] = yi[c];
+            else ys[e] -= R[(size_t)r * n + pc] * yi[c];
+        }
+        __syncthreads();
+    }
+    double* W = Wb + offG[b];
+    for (int e = tid; e < k * cols; e += 256) {
+        const int c = e / k, i = e - c * k;
+        W[i + (size_t)(jj0 + c) * k] = ys[i * CW + c];
     }
 }
 
@@ -431,8 +648,10 @@ __global__ void __launch_bounds__(256) k_interp_decomp(const int* __restrict__ d
                                                        double* Mb, double eps, int* perm,
                                                        const int* __restrict__ perm_off,
                                                        int* kout, double* Wb,
-                                                       const long long* __restrict__ offW) {
+                                                       const long long* __restrict__ offW,
+                                                       const unsigned char* __restrict__ mode) {
     const int b = blockIdx.x;
+    if (mode && mode[b]) return;
     const int m = dm[b], n = dn[b];
     if (n == 0) {
         if (threadIdx.x == 0) kout[b] = 0;
@@ -590,55 +809,6 @@ __global__ void __launch_bounds__(256) k_interp_decomp(const int* __restrict__ d
     }
 }
 
-// Blocks of Rot^T C Rot with Rot = I - E_{skel,red} W, C = A(F,F) (n x n):
-//   A12 = C_sr - C_ss W                  (k x q)
-//   A21 = C_rs - W^T C_ss                (q x k)
-//   A22 = C_rr - C_rs W - W^T A12        (q x q)
-// Computed in FP64; Tsr (FP64 scratch, k x q) holds A12 for the A22 step.
-template <class T>
-__global__ void __launch_bounds__(256) k_face_transform(
-    const int* __restrict__ dn, const int* __restrict__ dk, const int* __restrict__ perm,
-    const int* __restrict__ perm_off, const double* __restrict__ Cb,
-    const long long* __restrict__ offC, const double* __restrict__ Wb,
-    const long long* __restrict__ offW, double* Tsrb, T* A22b, T* A21b, T* A12b,
-    const long long* __restrict__ off_rr, const long long* __restrict__ off_rs) {
-    const int b = blockIdx.x;
-    const int n = dn[b], k = dk[b], q = n - k;
-    if (n == 0) return;
-    const int* p = perm + perm_off[b];
-    const int* sk = p;
-    const int* rd = p + k;
-    const double* C = Cb + offC[b];
-    const double* W = Wb + offW[b];
-    double* Tsr = Tsrb + offW[b];
-    T* A22 = A22b + off_rr[b];
-    T* A21 = A21b + off_rs[b];
-    T* A12 = A12b + off_rs[b];
-    const int tid = threadIdx.x, bs = blockDim.x;
-    auto Cv = [&](int i, int j) { return C[i + (size_t)j * n]; };
-    for (int e = tid; e < k * q; e += bs) {
-        int i = e % k, jj = e / k;
-        double v = Cv(sk[i], rd[jj]);
-        for (int l = 0; l < k; ++l) v -= Cv(sk[i], sk[l]) * W[l + (size_t)jj * k];
-        Tsr[e] = v;
-        A12[e] = (T)v;
-    }
-    for (int e = tid; e < q * k; e += bs) {
-        int ii = e % q, j = e / q;
-        double v = Cv(rd[ii], sk[j]);
-        for (int l = 0; l < k; ++l) v -= W[l + (size_t)ii * k] * Cv(sk[l], sk[j]);
-        A21[e] = (T)v;
-    }
-    __syncthreads();
-    for (int e = tid; e < q * q; e += bs) {
-        int ii = e % q, jj = e / q;
-        double v = Cv(rd[ii], rd[jj]);
-        for (int l = 0; l < k; ++l) v -= Cv(rd[ii], sk[l]) * W[l + (size_t)jj * k];
-        for (int l = 0; l < k; ++l) v -= W[l + (size_t)ii * k] * Tsr[l + (size_t)jj * k];
-        A22[e] = (T)v;
-    }
-}
-
 template <class S, class D>
 __global__ void k_convert(const S* __restrict__ src, D* __restrict__ dst, long long n) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -675,88 +845,142 @@ void batched_gj_inverse(int nbox, const int* h_n, const int* d_n, const long lon
     if (nbox <= 0) return;
     int max_n = 0;
     bool any_small = false;
-    std::vector<int> piv_off(nbox + 1, 0), tile_off(nbox + 1, 0);
-    std::vector<long long> b_off(nbox + 1, 0);
+    std::vector<int> tile_off(nbox + 1, 0);
+    std::vector<long long> r_off(nbox + 1, 0);
     for (int b = 0; b < nbox; ++b) {
-        int n = h_n[b];
+        const int n = h_n[b];
         max_n = std::max(max_n, n);
-        bool big = n > kSmallGJ;
+        const bool big = n > kSmallGJ;
         any_small |= (n > 0 && !big);
-        piv_off[b + 1] = piv_off[b] + (big ? n : 0);
-        b_off[b + 1] = b_off[b] + (big ? (long long)kGJB * n : 0);
-        int tm = big ? (n + 63) / 64 : 0;
+        r_off[b + 1] = r_off[b] + (big ? (long long)kGJB * n : 0);
+        const int tm = big ? (n + 63) / 64 : 0;
         tile_off[b + 1] = tile_off[b] + tm * tm;
     }
     if (any_small) {
-        k_gj_small<T><<<nbox, 128, 0, s>>>(d_n, d_off, d_A, d_info);
+        k_gj_small<T><<<nbox, 256, 0, s>>>(d_n, d_off, d_A, d_info);
         HIFDE_LAUNCH_CHECK();
     }
     if (max_n <= kSmallGJ) return;
-    ws.piv.reserve(piv_off[nbox]);
-    ws.piv_off.upload(piv_off, s);
     ws.tile_off.upload(tile_off, s);
-    ws.b_off.upload(b_off, s);
-    ws.B.reserve((size_t)b_off[nbox] * sizeof(T));
-    T* Bws = reinterpret_cast<T*>(ws.B.p);
+    ws.b_off.upload(r_off, s);
+    ws.B.reserve((size_t)r_off[nbox] * sizeof(T));
+    ws.D.reserve((size_t)nbox * kGJB * kGJB * sizeof(T));
+    T* Rws = reinterpret_cast<T*>(ws.B.p);
+    T* Dws = reinterpret_cast<T*>(ws.D.p);
     const int steps = (max_n + kGJB - 1) / kGJB;
-    const size_t shm = (size_t)max_n * sizeof(T);
-    if (shm > 150 * 1024) throw Error(-3, "gj_inverse: box too large");
-    dim3 gsw((max_n + 255) / 256, nbox);
+    const dim3 gch((max_n + 63) / 64, nbox);
     for (int st = 0; st < steps; ++st) {
-        k_bgj_panel<T><<<nbox, 256, shm, s>>>(d_n, d_off, d_A, ws.piv.p, ws.piv_off.p, st, d_info);
-        k_bgj_swap<T><<<gsw, 256, 0, s>>>(d_n, d_off, d_A, ws.piv.p, ws.piv_off.p, st, Bws,
-                                          ws.b_off.p, d_info);
-        k_bgj_update<T><<<tile_off[nbox], 256, 0, s>>>(nbox, ws.tile_off.p, d_n, d_off, d_A, Bws,
+        k_bgj_diag<T><<<nbox, 256, 0, s>>>(d_n, d_off, d_A, Dws, st, d_info);
+        k_bgj_row<T><<<gch, 256, 0, s>>>(d_n, d_off, d_A, Dws, Rws, ws.b_off.p, st, d_info);
+        k_bgj_update<T><<<tile_off[nbox], 256, 0, s>>>(nbox, ws.tile_off.p, d_n, d_off, d_A, Rws,
                                                        ws.b_off.p, st, d_info);
+        k_bgj_col<T><<<gch, 256, 0, s>>>(d_n, d_off, d_A, Dws, st, d_info);
     }
-    k_bgj_colswap<T><<<gsw, 256, 0, s>>>(d_n, d_off, d_A, ws.piv.p, ws.piv_off.p, d_info);
     HIFDE_LAUNCH_CHECK();
 }
 
 template <class T>
-void batched_gemm_nn(int nbox, const int* h_m, const int* h_n, const int* h_k, const int* d_m,
-                     const int* d_n, const int* d_k, const long long* d_offA,
-                     const long long* d_offB, const long long* d_offC, const T* d_A,
-                     const T* d_B, T* d_C, T alpha, DevBuf<int>& tile_scratch, hipStream_t s) {
+void batched_gemm_desc(bool transA, const std::vector<GemmDesc>& desc, const T* d_A, const T* d_B,
+                       T* d_C, T alpha, T beta, GemmWork& ws, hipStream_t s) {
+    const int nbox = (int)desc.size();
     if (nbox <= 0) return;
     std::vector<int> toff(nbox + 1, 0);
     for (int b = 0; b < nbox; ++b) {
-        long long tiles = (h_m[b] > 0 && h_n[b] > 0)
-                              ? (long long)((h_m[b] + BM - 1) / BM) * ((h_n[b] + BN - 1) / BN)
-                              : 0;
+        const GemmDesc& d = desc[b];
+        long long tiles = (d.m > 0 && d.n > 0) ? (long long)((d.m + BM - 1) / BM) *
+                                                     ((d.n + BN - 1) / BN)
+                                               : 0;
         toff[b + 1] = toff[b] + (int)tiles;
     }
-    int total = toff[nbox];
+    const int total = toff[nbox];
     if (total == 0) return;
-    tile_scratch.upload(toff, s);
-    k_gemm_nn<T><<<total, 256, 0, s>>>(nbox, tile_scratch.p, d_m, d_n, d_k, d_offA, d_offB,
-                                       d_offC, d_A, d_B, d_C, alpha);
+    ws.tiles.upload(toff, s);
+    ws.desc.upload(desc, s);
+    if (transA)
+        k_gemm<T, true><<<total, 256, 0, s>>>(nbox, ws.tiles.p, ws.desc.p, d_A, d_B, d_C, alpha,
+                                              beta);
+    else
+        k_gemm<T, false><<<total, 256, 0, s>>>(nbox, ws.tiles.p, ws.desc.p, d_A, d_B, d_C, alpha,
+                                               beta);
+    HIFDE_LAUNCH_CHECK();
+}
+
+template <class T>
+void batched_gemm(bool transA, int nbox, const int* h_m, const int* h_n, const int* h_k,
+                  const long long* h_offA, const long long* h_offB, const long long* h_offC,
+                  const T* d_A, const T* d_B, T* d_C, T alpha, GemmWork& ws, hipStream_t s) {
+    std::vector<GemmDesc> desc(nbox);
+    for (int b = 0; b < nbox; ++b) {
+        GemmDesc& d = desc[b];
+        d.offA = h_offA[b];
+        d.offB = h_offB[b];
+        d.offC = h_offC[b];
+        d.m = h_m[b];
+        d.n = h_n[b];
+        d.k = h_k[b];
+        d.lda = std::max(transA ? d.k : d.m, 1);
+        d.ldb = std::max(d.k, 1);
+        d.ldc = std::max(d.m, 1);
+    }
+    batched_gemm_desc<T>(transA, desc, d_A, d_B, d_C, alpha, (T)0, ws, s);
+}
+
+template <class D>
+void copy_blocks(const std::vector<CopyDesc>& desc, const std::vector<long long>& dst_off,
+                 const double* d_src, D* d_dst, GemmWork& ws, hipStream_t s) {
+    const int nbox = (int)desc.size();
+    if (nbox <= 0 || dst_off.back() <= 0) return;
+    ws.cdesc.upload(desc, s);
+    ws.off.upload(dst_off, s);
+    k_copy_blocks<D><<<grid_for(dst_off.back()), 256, 0, s>>>(nbox, ws.off.p, ws.cdesc.p, d_src,
+                                                              d_dst, dst_off.back());
+    HIFDE_LAUNCH_CHECK();
+}
+
+void permute_sym(int nbox, const long long* d_off, const int* d_n, const int* d_perm,
+                 const int* d_perm_off, const double* d_C, double* d_Cp, long long total,
+                 hipStream_t s) {
+    if (nbox <= 0 || total <= 0) return;
+    k_permute_sym<<<grid_for(total), 256, 0, s>>>(nbox, d_off, d_n, d_perm, d_perm_off, d_C, d_Cp,
+                                                  total);
+    HIFDE_LAUNCH_CHECK();
+}
+
+void gram_interp_decomp(int nface, const int* d_n, const unsigned char* d_mode,
+                        const double* d_G, const long long* d_offG, double* d_R, double eps,
+                        int* d_perm, const int* d_perm_off, int* d_k, double* d_W, int max_n,
+                        hipStream_t s) {
+    if (nface <= 0 || max_n <= 0) return;
+    size_t shm = (size_t)max_n * (2 * sizeof(double) + sizeof(int));
+    if (shm > 150 * 1024) throw Error(-3, "gram_interp_decomp: face too large for LDS");
+    k_gram_pivchol<<<nface, 256, shm, s>>>(d_n, d_mode, d_G, d_offG, d_R, eps, d_perm, d_perm_off,
+                                           d_k);
+    HIFDE_LAUNCH_CHECK();
+    // the right-hand-side chunk width keeps k x CW doubles within LDS
+    if ((size_t)max_n * 32 * sizeof(double) <= 140 * 1024) {
+        dim3 g((max_n + 31) / 32, nface);
+        k_gram_wsolve<32><<<g, 256, (size_t)max_n * 32 * sizeof(double), s>>>(
+            d_n, d_mode, d_offG, d_R, d_perm, d_perm_off, d_k, d_W);
+    } else if ((size_t)max_n * 8 * sizeof(double) <= 140 * 1024) {
+        dim3 g((max_n + 7) / 8, nface);
+        k_gram_wsolve<8><<<g, 256, (size_t)max_n * 8 * sizeof(double), s>>>(
+            d_n, d_mode, d_offG, d_R, d_perm, d_perm_off, d_k, d_W);
+    } else {
+        throw Error(-3, "gram_interp_decomp: face too large for the LDS solve");
+    }
     HIFDE_LAUNCH_CHECK();
 }
 
 void batched_interp_decomp(int nface, const int* d_m, const int* d_n, const long long* d_offM,
                            double* d_M, double eps, int* d_perm, const int* d_perm_off, int* d_k,
                            double* d_W, const long long* d_offW, int max_n, int max_m,
-                           hipStream_t s) {
+                           const unsigned char* d_mode, hipStream_t s) {
     if (nface <= 0) return;
     size_t shm = (size_t)max_n * (2 * sizeof(double) + 2 * sizeof(int));
     if (shm > 150 * 1024) throw Error(-3, "interp_decomp: face too large for LDS");
     int threads = (max_m <= 128 && max_n <= 16) ? 64 : 256;
     k_interp_decomp<<<nface, threads, shm, s>>>(d_m, d_n, d_offM, d_M, eps, d_perm, d_perm_off,
-                                                d_k, d_W, d_offW);
-    HIFDE_LAUNCH_CHECK();
-}
-
-template <class T>
-void batched_face_transform(int nface, const int* d_n, const int* d_k, const int* d_perm,
-                            const int* d_perm_off, const double* d_C, const long long* d_offC,
-                            const double* d_W, const long long* d_offW, double* d_Tsr, T* d_A22,
-                            T* d_A21, T* d_A12, const long long* d_off_rr,
-                            const long long* d_off_rs, hipStream_t s) {
-    if (nface <= 0) return;
-    k_face_transform<T><<<nface, 256, 0, s>>>(d_n, d_k, d_perm, d_perm_off, d_C, d_offC, d_W,
-                                              d_offW, d_Tsr, d_A22, d_A21, d_A12, d_off_rr,
-                                              d_off_rs);
+                                                d_k, d_W, d_offW, d_mode);
     HIFDE_LAUNCH_CHECK();
 }
 
@@ -783,22 +1007,24 @@ template void batched_gj_inverse<double>(int, const int*, const int*, const long
                                          int*, GJWork&, hipStream_t);
 template void batched_gj_inverse<float>(int, const int*, const int*, const long long*, float*, int*,
                                         GJWork&, hipStream_t);
-template void batched_gemm_nn<double>(int, const int*, const int*, const int*, const int*,
-                                      const int*, const int*, const long long*, const long long*,
-                                      const long long*, const double*, const double*, double*,
-                                      double, DevBuf<int>&, hipStream_t);
-template void batched_gemm_nn<float>(int, const int*, const int*, const int*, const int*,
-                                     const int*, const int*, const long long*, const long long*,
-                                     const long long*, const float*, const float*, float*, float,
-                                     DevBuf<int>&, hipStream_t);
-template void batched_face_transform<double>(int, const int*, const int*, const int*, const int*,
-                                             const double*, const long long*, const double*,
-                                             const long long*, double*, double*, double*,
-                                             double*, const long long*, const long long*,
-                                             hipStream_t);
-template void batched_face_transform<float>(int, const int*, const int*, const int*, const int*,
-                                            const double*, const long long*, const double*,
-                                            const long long*, double*, float*, float*, float*,
-                                            const long long*, const long long*, hipStream_t);
+
+template void batched_gemm_desc<double>(bool, const std::vector<GemmDesc>&, const double*,
+                                        const double*, double*, double, double, GemmWork&,
+                                        hipStream_t);
+template void batched_gemm_desc<float>(bool, const std::vector<GemmDesc>&, const float*,
+                                       const float*, float*, float, float, GemmWork&,
+                                       hipStream_t);
+template void batched_gemm<double>(bool, int, const int*, const int*, const int*,
+                                   const long long*, const long long*, const long long*,
+                                   const double*, const double*, double*, double, GemmWork&,
+                                   hipStream_t);
+template void batched_gemm<float>(bool, int, const int*, const int*, const int*,
+                                  const long long*, const long long*, const long long*,
+                                  const float*, const float*, float*, float, GemmWork&,
+                                  hipStream_t);
+template void copy_blocks<double>(const std::vector<CopyDesc>&, const std::vector<long long>&,
+                                  const double*, double*, GemmWork&, hipStream_t);
+template void copy_blocks<float>(const std::vector<CopyDesc>&, const std::vector<long long>&,
+                                 const double*, float*, GemmWork&, hipStream_t);
 
 }  // namespace hifde

@@ -70,7 +70,11 @@ struct hifde_factor {
     hipStream_t stream = nullptr;
     std::vector<unsigned char> remaining;
     DevBuf<unsigned char> d_alive;
-    LevelTable table;
+    LevelTable tabs[2];  // ping-pong level matrices (compaction writes the other)
+    int cur = 0;
+    DevBuf<unsigned long long> tab_counter;
+    size_t tab_inserted = 0;  // upper bound of the entries in tabs[cur]
+    LevelTable& table_ref() { return tabs[cur]; }
     bool have_matrix = false, factored = false;
     // operator kept for SpMV / PCG (CSR)
     std::vector<int> h_rp, h_ci;
@@ -98,10 +102,17 @@ struct hifde_factor {
     DevBuf<unsigned char> ws_a, ws_b, ws_c;
     DevBuf<double> ws_M, ws_C, ws_W, ws_T;
     DevBuf<int> ws_rows, ws_rows_off, ws_cols, ws_cols_off, ws_perm, ws_perm_off, ws_k, ws_info,
-        ws_m, ws_n, ws_tiles;
+        ws_m, ws_n;
     DevBuf<long long> ws_offM, ws_offC, ws_offW;
     GJWork gj_ws;
-    DevBuf<double> pcg_buf, pcg_part, pcg_scalar;
+    GemmWork gemm_ws;
+    DevBuf<unsigned char> ws_mode;
+    DevBuf<int> ws_gm, ws_gk;
+    DevBuf<long long> ws_goffM, ws_goffC;
+    DevBuf<double> ws_G;
+    int gram_min_n = 64;          // faces with more columns use the Gram path
+    double gram_min_eps = 1e-7;   // below this tolerance every face uses Householder
+    DevBuf<double> pcg_buf, pcg_part, pcg_scalar, apply_tmp;
     double stage_t[ST_COUNT] = {0};
     int lu_info = 0;
 
@@ -115,7 +126,7 @@ struct hifde_factor {
 
     int width(int level) const { return (1 << level) * P; }
     int lin(int i, int j, int k) const { return i + NC * j + NC * NC * k; }
-    size_t elem() const { return precision == HIFDE_MIXED ? sizeof(float) : sizeof(double); }
+    int fp32_levels = -1;  // mixed precision: levels factored in FP32 (-1: all but the top two)
 
     int family(const char* name) {
         for (size_t i = 0; i < fam_names.size(); ++i)
@@ -371,6 +382,7 @@ struct hifde_factor {
 
     // ------------------------------------------------------------------
     void upload_batch_index(BoxBatch& B) {
+        finalize_tiles(B);
         B.d_r.upload(B.r, stream);
         B.d_s.upload(B.s, stream);
         B.d_red_off.upload(B.red_off, stream);
@@ -421,15 +433,14 @@ struct hifde_factor {
                                    hipMemcpyDeviceToHost, stream));
         {
             Prof pr(this, "gemm_x", fl_x, by_x);
-            batched_gemm_nn<T>(nb, B.r.data(), B.s.data(), B.r.data(), B.d_r.p, B.d_s.p, B.d_r.p,
-                               B.d_off_rr.p, B.d_off_rs.p, B.d_off_rs.p, Ainv, A21, X, (T)1,
-                               ws_tiles, stream);
+            batched_gemm<T>(false, nb, B.r.data(), B.s.data(), B.r.data(), B.off_rr.data(),
+                            B.off_rs.data(), B.off_rs.data(), Ainv, A21, X, (T)1, gemm_ws,
+                            stream);
         }
         {
             Prof pr(this, "gemm_s", fl_s, by_s);
-            batched_gemm_nn<T>(nb, B.s.data(), B.s.data(), B.r.data(), B.d_s.p, B.d_s.p, B.d_r.p,
-                               B.d_off_rs.p, B.d_off_rs.p, B.d_off_ss.p, A12, X, S, (T)-1,
-                               ws_tiles, stream);
+            batched_gemm<T>(false, nb, B.s.data(), B.s.data(), B.r.data(), B.off_rs.data(),
+                            B.off_rs.data(), B.off_ss.data(), A12, X, S, (T)-1, gemm_ws, stream);
         }
         timed(ST_GEMM, t0);
         for (int b = 0; b < nb; ++b)
@@ -445,19 +456,25 @@ struct hifde_factor {
     void scatter(BoxBatch& B, const T* S, bool two_pass, clk::time_point& t0) {
         // DeleteRow/DeleteCol of the eliminated DOFs, then Sparse::Add of S
         // (src/Factor.cpp:301-309).  Compaction drops the deleted pairs.
-        {
-            Prof pr(this, "compact", 0.0, (double)table.cap * 16 * 2);
-            table_compact(table, d_alive.p, (size_t)B.tot_ss, stream);
+        // Entries of eliminated DOFs are never looked up again, so they may
+        // stay in the table; it is compacted only when the coming inserts
+        // would push the load factor above one half.
+        if ((tab_inserted + (size_t)B.tot_ss) * 2 > tabs[cur].cap) {
+            Prof pr(this, "compact", 0.0, (double)tabs[cur].cap * 16 * 2);
+            tab_inserted = table_compact(tabs[cur], tabs[1 - cur], d_alive.p, (size_t)B.tot_ss,
+                                         tab_counter, stream);
+            cur = 1 - cur;
         }
+        tab_inserted += (size_t)B.tot_ss;
         timed(ST_COMPACT, t0);
         {
             Prof pr(this, "scatter", 0.0, (double)B.tot_ss * (sizeof(T) + 32));
             if (two_pass) {
                 for (int pass = 0; pass < 2; ++pass)
-                    table_scatter<T>(table, B.nbox, B.d_s.p, B.d_skel_off.p, B.d_skel_gid.p,
+                    table_scatter<T>(tabs[cur], B.nbox, B.d_s.p, B.d_skel_off.p, B.d_skel_gid.p,
                                      B.d_skel_pass.p, B.d_off_ss.p, B.tot_ss, S, pass, stream);
             } else {
-                table_scatter<T>(table, B.nbox, B.d_s.p, B.d_skel_off.p, B.d_skel_gid.p, nullptr,
+                table_scatter<T>(tabs[cur], B.nbox, B.d_s.p, B.d_skel_off.p, B.d_skel_gid.p, nullptr,
                                  B.d_off_ss.p, B.tot_ss, S, -1, stream);
             }
         }
@@ -469,6 +486,7 @@ struct hifde_factor {
         auto t0 = clk::now();
         BoxBatch& B = cells[level];
         plan_cells(level, B);
+        B.esize = sizeof(T);
         timed(ST_PLAN, t0);
         upload_batch_index(B);
         timed(ST_UPLOAD, t0);
@@ -483,12 +501,12 @@ struct hifde_factor {
         // the three blocks Schur reads.
         {
             Prof pr_g(this, "gather", 0.0, (double)(B.tot_rr + 2 * B.tot_rs) * (sizeof(T) + 16));
-            table_gather<T>(table, B.nbox, B.d_red_gid.p, B.d_red_off.p, B.d_red_gid.p,
+            table_gather<T>(tabs[cur], B.nbox, B.d_red_gid.p, B.d_red_off.p, B.d_red_gid.p,
                             B.d_red_off.p, B.d_off_rr.p, B.tot_rr, reinterpret_cast<T*>(B.A22.p),
                             stream);
-            table_gather<T>(table, B.nbox, B.d_red_gid.p, B.d_red_off.p, B.d_skel_gid.p,
+            table_gather<T>(tabs[cur], B.nbox, B.d_red_gid.p, B.d_red_off.p, B.d_skel_gid.p,
                             B.d_skel_off.p, B.d_off_rs.p, B.tot_rs, A21, stream);
-            table_gather<T>(table, B.nbox, B.d_skel_gid.p, B.d_skel_off.p, B.d_red_gid.p,
+            table_gather<T>(tabs[cur], B.nbox, B.d_skel_gid.p, B.d_skel_off.p, B.d_red_gid.p,
                             B.d_red_off.p, B.d_off_rs.p, B.tot_rs, A12, stream);
         }
         timed(ST_GATHER, t0);
@@ -507,6 +525,7 @@ struct hifde_factor {
         BoxBatch& B = faces[level];
         B = BoxBatch();
         B.nbox = nf;
+        B.esize = sizeof(T);
         if (nf == 0) return;
         std::vector<long long> offM = prefix_ll(F.m, F.n), offC = prefix_ll(F.n, F.n);
         int max_n = 0, max_m = 0;
@@ -529,24 +548,70 @@ struct hifde_factor {
         ws_T.reserve(offC.back());
         {
             Prof pr(this, "gather", 0.0, (double)(offM.back() + offC.back()) * (8 + 16));
-            table_gather<double>(table, nf, ws_rows.p, ws_rows_off.p, ws_cols.p, ws_cols_off.p,
+            table_gather<double>(tabs[cur], nf, ws_rows.p, ws_rows_off.p, ws_cols.p, ws_cols_off.p,
                                  ws_offM.p, offM.back(), ws_M.p, stream);
-            table_gather<double>(table, nf, ws_cols.p, ws_cols_off.p, ws_cols.p, ws_cols_off.p,
+            table_gather<double>(tabs[cur], nf, ws_cols.p, ws_cols_off.p, ws_cols.p, ws_cols_off.p,
                                  ws_offC.p, offC.back(), ws_C.p, stream);
         }
         timed(ST_GATHER, t0);
         ws_k.reserve(nf);
         ws_perm.reserve(F.cols_off.back());
-        {
-            double fl = 0, by = 0;
-            for (int f = 0; f < nf; ++f) {
-                double m = F.m[f], n = F.n[f], k = std::min(m, n);
-                fl += 4 * m * n * k - 2 * k * k * (m + n) + 4.0 / 3.0 * k * k * k;
-                by += 8 * (2 * m * n + n * n);
+        // Large tall faces take the Gram path (G = M^T M, pivoted Cholesky);
+        // the rest, and every face when eps is below the Gram path's noise
+        // floor, take the Householder column-pivoted QR.
+        std::vector<unsigned char> mode(nf, 0);
+        std::vector<int> gm, gn, gk;
+        std::vector<long long> goffM, goffC;
+        int max_n_house = 0, max_m_house = 0, max_n_gram = 0;
+        double fl_h = 0, by_h = 0, fl_g = 0, by_g = 0, fl_c = 0;
+        for (int f = 0; f < nf; ++f) {
+            double m = F.m[f], n = F.n[f];
+            bool gram = F.n[f] > gram_min_n && F.m[f] >= F.n[f] && eps >= gram_min_eps;
+            mode[f] = gram ? 1 : 0;
+            if (gram) {
+                gm.push_back(F.n[f]);
+                gn.push_back(F.n[f]);
+                gk.push_back(F.m[f]);
+                goffM.push_back(offM[f]);
+                goffC.push_back(offC[f]);
+                max_n_gram = std::max(max_n_gram, F.n[f]);
+                fl_g += 2 * m * n * n;
+                by_g += 8 * (m * n + n * n);
+                fl_c += n * n * n / 3;
+            } else {
+                max_n_house = std::max(max_n_house, F.n[f]);
+                max_m_house = std::max(max_m_house, F.m[f]);
+                double k = std::min(m, n);
+                fl_h += 4 * m * n * k - 2 * k * k * (m + n) + 4.0 / 3.0 * k * k * k;
+                by_h += 8 * (2 * m * n + n * n);
             }
-            Prof pr(this, "interp_decomp", fl, by);
+        }
+        ws_mode.upload(mode, stream);
+        if (!gm.empty()) {
+            int ng = (int)gm.size();
+            ws_gm.upload(gm, stream);
+            ws_gk.upload(gk, stream);
+            ws_goffM.upload(goffM, stream);
+            ws_goffC.upload(goffC, stream);
+            ws_G.reserve(offC.back());
+            ws_T.reserve(offC.back());
+            {
+                Prof pr(this, "gram_gemm", fl_g, by_g);
+                batched_gemm<double>(true, ng, gm.data(), gn.data(), gk.data(), goffM.data(),
+                                     goffM.data(), goffC.data(), ws_M.p, ws_M.p, ws_G.p, 1.0,
+                                     gemm_ws, stream);
+            }
+            {
+                Prof pr(this, "gram_pivchol", fl_c, 8.0 * offC.back());
+                gram_interp_decomp(nf, ws_n.p, ws_mode.p, ws_G.p, ws_offC.p, ws_T.p, eps,
+                                   ws_perm.p, ws_cols_off.p, ws_k.p, ws_W.p, max_n_gram, stream);
+            }
+        }
+        if (max_n_house > 0 || (int)gm.size() < nf) {
+            Prof pr(this, "interp_decomp", fl_h, by_h);
             batched_interp_decomp(nf, ws_m.p, ws_n.p, ws_offM.p, ws_M.p, eps, ws_perm.p,
-                                  ws_cols_off.p, ws_k.p, ws_W.p, ws_offC.p, max_n, max_m, stream);
+                                  ws_cols_off.p, ws_k.p, ws_W.p, ws_offC.p, max_n_house,
+                                  max_m_house, ws_mode.p, stream);
         }
         timed(ST_ID, t0);
         std::vector<int> kk(nf), perm(F.cols_off.back());
@@ -597,10 +662,7 @@ struct hifde_factor {
                 by += 8 * (n * n + k * q) + sizeof(T) * (q * q + 2 * k * q);
             }
             Prof pr(this, "face_transform", fl, by);
-            batched_face_transform<T>(nf, ws_n.p, ws_k.p, ws_perm.p, ws_cols_off.p, ws_C.p,
-                                      ws_offC.p, ws_W.p, ws_offC.p, ws_T.p,
-                                      reinterpret_cast<T*>(B.A22.p), A21, A12, B.d_off_rr.p,
-                                      B.d_off_rs.p, stream);
+            face_transform<T>(F, offC, B, A21, A12);
         }
         // W (k x q) kept for Apply, packed at the r*s offsets.
         B.W.reserve(B.tot_rs * sizeof(T));
@@ -611,6 +673,43 @@ struct hifde_factor {
         scatter<T>(B, S, false, t0);
     }
 
+    // Blocks of Rot^T C Rot with Rot = I - E_{skel,red} W and C = A(F,F)
+    // (SchurAfterID, src/Factor.cpp:113-153), without forming Rot:
+    //   A12 = Tsr = C_sr - C_ss W                 (k x q)
+    //   A22 = C_rr - C_rs W - W^T Tsr             (q x q)
+    //   A21 = C_rs - W^T C_ss                     (q x k)
+    // C is first permuted to skeleton-then-redundant order; all products are
+    // FP64 batched GEMMs on sub-blocks of it; the blocks are then packed in T.
+    template <class T>
+    void face_transform(const FacePlan& F, const std::vector<long long>& offC, BoxBatch& B,
+                        T* A21, T* A12) {
+        const int nf = B.nbox;
+        ws_T.reserve(offC.back());
+        permute_sym(nf, ws_offC.p, ws_n.p, ws_perm.p, ws_cols_off.p, ws_C.p, ws_T.p, offC.back(),
+                    stream);
+        std::vector<GemmDesc> d1, d2, d3, d4;
+        std::vector<CopyDesc> c22(nf), c21(nf), c12(nf);
+        for (int f = 0; f < nf; ++f) {
+            const int n = F.n[f], k = B.s[f], q = B.r[f];
+            const long long base = offC[f], wb = offC[f], kn = (long long)k * n;
+            c22[f] = {base, k, k, q, n};
+            c21[f] = {base, k, 0, q, n};
+            c12[f] = {base, 0, k, k, n};
+            if (k == 0 || q == 0) continue;
+            d1.push_back({base, wb, base + kn, k, q, k, n, k, n});          // Csr -= Css W
+            d2.push_back({base + k, wb, base + k + kn, q, q, k, n, k, n});  // Crr -= Crs W
+            d3.push_back({wb, base + kn, base + k + kn, q, q, k, k, n, n}); // Crr -= W^T Tsr
+            d4.push_back({wb, base, base + k, q, k, k, k, n, n});           // Crs -= W^T Css
+        }
+        batched_gemm_desc<double>(false, d1, ws_T.p, ws_W.p, ws_T.p, -1.0, 1.0, gemm_ws, stream);
+        batched_gemm_desc<double>(false, d2, ws_T.p, ws_W.p, ws_T.p, -1.0, 1.0, gemm_ws, stream);
+        batched_gemm_desc<double>(true, d3, ws_W.p, ws_T.p, ws_T.p, -1.0, 1.0, gemm_ws, stream);
+        batched_gemm_desc<double>(true, d4, ws_W.p, ws_T.p, ws_T.p, -1.0, 1.0, gemm_ws, stream);
+        copy_blocks<T>(c22, B.off_rr, ws_T.p, reinterpret_cast<T*>(B.A22.p), gemm_ws, stream);
+        copy_blocks<T>(c21, B.off_rs, ws_T.p, A21, gemm_ws, stream);
+        copy_blocks<T>(c12, B.off_rs, ws_T.p, A12, gemm_ws, stream);
+    }
+
     template <class T>
     void pack_w(BoxBatch& B) {
         // Each face's W (k x q) from the n*n-strided ID workspace into the
@@ -619,7 +718,14 @@ struct hifde_factor {
                       B.tot_rs, stream);
     }
 
-    template <class T>
+    // Mixed precision keeps the two top levels in FP64: their blocks are the
+    // largest and worst conditioned, and FP32 inverses there cost the PCG
+    // several times its FP64 iteration count.
+    int fp32_levels_eff() const {
+        if (precision != HIFDE_MIXED) return 0;
+        return fp32_levels < 0 ? std::max(L - 2, 0) : std::min(fp32_levels, L);
+    }
+
     void run_factor() {
         if (!have_matrix) throw Error(HIFDE_ERR_STATE, "factor: no matrix set");
         for (double& t : stage_t) t = 0.0;
@@ -638,91 +744,126 @@ struct hifde_factor {
             // Initialize(): the level matrix starts as the assembled operator.
             auto t0 = clk::now();
             long long nnz = (long long)h_ci.size();
-            table_init(table, (size_t)nnz, stream);
+            cur = 0;
+            table_init(tabs[cur], (size_t)nnz, stream);
             Prof pr(this, "table_insert", 0.0, (double)nnz * (4 + 4 + 8 + 16));
-            table_insert_coo(table, d_rr.p, d_ci.p, d_v.p, nnz, stream);
+            table_insert_coo(tabs[cur], d_rr.p, d_ci.p, d_v.p, nnz, stream);
+            tab_inserted = (size_t)nnz;
             timed(ST_UPLOAD, t0);
         }
         for (int level = 0; level < L; ++level) {
-            level_cells<T>(level);
-            if (level < L - 1) level_faces<T>(level);
+            if (level < fp32_levels_eff()) {
+                level_cells<float>(level);
+                if (level < L - 1) level_faces<float>(level);
+            } else {
+                level_cells<double>(level);
+                if (level < L - 1) level_faces<double>(level);
+            }
         }
-        // the level matrix has been consumed (every DOF eliminated)
-        table.slots.release();
-        table.cap = 0;
-        ws_a.release(); ws_b.release(); ws_c.release();
-        ws_M.release(); ws_C.release(); ws_W.release(); ws_T.release();
+        // Every DOF has been eliminated: the level matrix is consumed.  Its
+        // buffers and the workspaces are kept for the next factorization.
         resolve_profile();
         factored = true;
     }
 
     // ------------------------------------------------------------------
-    template <class T>
     void upd(BoxBatch& B, const DevBuf<unsigned char>& Mbuf, int op, int pass, double alpha,
              double* u) {
-        int maxd = 0;
-        double by = 0;
-        for (int b = 0; b < B.nbox; ++b) {
-            maxd = std::max(maxd, std::max(B.r[b], B.s[b]));
-            by += (double)B.r[b] * B.s[b] * sizeof(T) + 16.0 * (B.r[b] + B.s[b]);
-        }
-        Prof pr(this, "apply_update", 2.0 * B.tot_rs, by);
-        apply_update<T>(B.nbox, B.d_r.p, B.d_s.p, B.d_red_off.p, B.d_skel_off.p, B.d_red_gid.p,
-                        B.d_skel_gid.p, pass >= 0 ? B.d_skel_pass.p : nullptr, B.d_off_rs.p,
-                        reinterpret_cast<const T*>(Mbuf.p), op, pass, alpha, u, maxd, stream);
+        if (B.esize == sizeof(float)) upd_t<float>(B, Mbuf, op, pass, alpha, u);
+        else upd_t<double>(B, Mbuf, op, pass, alpha, u);
+    }
+    void diag(BoxBatch& B, const DevBuf<unsigned char>& Mbuf, const double* u, double* out) {
+        if (B.esize == sizeof(float)) diag_t<float>(B, Mbuf, u, out);
+        else diag_t<double>(B, Mbuf, u, out);
     }
     template <class T>
-    void diag(BoxBatch& B, const DevBuf<unsigned char>& Mbuf, double* u) {
-        int maxr = 0;
-        for (int x : B.r) maxr = std::max(maxr, x);
+    void upd_t(BoxBatch& B, const DevBuf<unsigned char>& Mbuf, int op, int pass, double alpha,
+               double* u) {
+        if (B.nbox == 0) return;
+        const bool out_red = (op == 1 || op == 2);
+        const bool in_red = (op == 0 || op == 3);
+        const std::vector<int>& tl = out_red ? B.tiles_r : B.tiles_s;
+        double by = (double)B.tot_rs * sizeof(T) +
+                    16.0 * ((B.red_off.empty() ? 0 : B.red_off.back()) +
+                            (B.skel_off.empty() ? 0 : B.skel_off.back()));
+        Prof pr(this, "apply_update", 2.0 * B.tot_rs, by);
+        apply_tiles<T>(B.nbox, tl.back(), out_red ? B.d_tiles_r.p : B.d_tiles_s.p, B.d_r.p,
+                       B.d_s.p, B.d_red_off.p, B.d_skel_off.p, B.d_red_gid.p, B.d_skel_gid.p,
+                       pass >= 0 ? B.d_skel_pass.p : nullptr, B.d_off_rs.p,
+                       reinterpret_cast<const T*>(Mbuf.p), op, pass, alpha, u, u,
+                       in_red ? B.max_r : B.max_s, stream);
+    }
+    template <class T>
+    void diag_t(BoxBatch& B, const DevBuf<unsigned char>& Mbuf, const double* u, double* out) {
+        if (B.nbox == 0) return;
         Prof pr(this, "apply_diag", 2.0 * B.tot_rr,
                 (double)B.tot_rr * sizeof(T) + 16.0 * (B.red_off.empty() ? 0 : B.red_off.back()));
-        apply_diag<T>(B.nbox, B.d_r.p, B.d_red_off.p, B.d_red_gid.p, B.d_off_rr.p,
-                      reinterpret_cast<const T*>(Mbuf.p), u, maxr, stream);
+        apply_tiles<T>(B.nbox, B.tiles_r.back(), B.d_tiles_r.p, B.d_r.p, B.d_s.p, B.d_red_off.p,
+                       B.d_skel_off.p, B.d_red_gid.p, B.d_skel_gid.p, nullptr, B.d_off_rr.p,
+                       reinterpret_cast<const T*>(Mbuf.p), 4, -1, 1.0, u, out, B.max_r, stream);
+    }
+    // Apply tiling of a planned batch.
+    void finalize_tiles(BoxBatch& B) {
+        B.tiles_r.assign(B.nbox + 1, 0);
+        B.tiles_s.assign(B.nbox + 1, 0);
+        B.max_r = B.max_s = 0;
+        for (int b = 0; b < B.nbox; ++b) {
+            B.tiles_r[b + 1] = B.tiles_r[b] + (B.r[b] + 63) / 64;
+            B.tiles_s[b + 1] = B.tiles_s[b] + (B.s[b] + 63) / 64;
+            B.max_r = std::max(B.max_r, B.r[b]);
+            B.max_s = std::max(B.max_s, B.s[b]);
+        }
+        B.d_tiles_r.upload(B.tiles_r, stream);
+        B.d_tiles_s.upload(B.tiles_s, stream);
     }
 
-    // Apply (src/Factor.cpp:992-1081), one vector.
-    template <class T>
+    // Apply (src/Factor.cpp:992-1081), one vector.  Each batch's blocks are
+    // read in the precision they were factored in.
     void apply_one(double* u, bool inv) {
         zero_boundary(u, NC, stream);
         const double sg = inv ? -1.0 : 1.0;
         for (int lv = 0; lv < L - 1; ++lv) {
             BoxBatch& C = cells[lv];
             if (inv) {
-                for (int pass = 0; pass < 2; ++pass) upd<T>(C, C.X, 0, pass, -1.0, u);
+                for (int pass = 0; pass < 2; ++pass) upd(C, C.X, 0, pass, -1.0, u);
             } else {
-                upd<T>(C, C.X, 1, -1, 1.0, u);
+                upd(C, C.X, 1, -1, 1.0, u);
             }
             BoxBatch& F = faces[lv];
             if (F.nbox == 0) continue;
             if (inv) {
-                upd<T>(F, F.W, 2, -1, -1.0, u);
-                upd<T>(F, F.X, 0, -1, -1.0, u);
+                upd(F, F.W, 2, -1, -1.0, u);
+                upd(F, F.X, 0, -1, -1.0, u);
             } else {
-                upd<T>(F, F.W, 3, -1, 1.0, u);
-                upd<T>(F, F.X, 1, -1, 1.0, u);
+                upd(F, F.W, 3, -1, 1.0, u);
+                upd(F, F.X, 1, -1, 1.0, u);
             }
         }
+        // Every interior DOF is redundant in exactly one box, so the diagonal
+        // stage maps u to a fresh vector box by box and the result is copied back.
+        double* tmp = apply_tmp.p;
+        vcopy(u, tmp, (long long)NC * NC * NC, stream);
         for (int lv = 0; lv < L; ++lv) {
-            diag<T>(cells[lv], inv ? cells[lv].A22inv : cells[lv].A22, u);
-            if (faces[lv].nbox) diag<T>(faces[lv], inv ? faces[lv].A22inv : faces[lv].A22, u);
+            diag(cells[lv], inv ? cells[lv].A22inv : cells[lv].A22, u, tmp);
+            diag(faces[lv], inv ? faces[lv].A22inv : faces[lv].A22, u, tmp);
         }
+        vcopy(tmp, u, (long long)NC * NC * NC, stream);
         for (int lv = L - 2; lv >= 0; --lv) {
             BoxBatch& F = faces[lv];
             if (F.nbox) {
                 if (inv) {
-                    upd<T>(F, F.X, 1, -1, -1.0, u);
-                    upd<T>(F, F.W, 3, -1, -1.0, u);
+                    upd(F, F.X, 1, -1, -1.0, u);
+                    upd(F, F.W, 3, -1, -1.0, u);
                 } else {
-                    upd<T>(F, F.X, 0, -1, 1.0, u);
-                    upd<T>(F, F.W, 2, -1, 1.0, u);
+                    upd(F, F.X, 0, -1, 1.0, u);
+                    upd(F, F.W, 2, -1, 1.0, u);
                 }
             }
             BoxBatch& C = cells[lv];
             if (inv) {
-                upd<T>(C, C.X, 1, -1, -1.0, u);
+                upd(C, C.X, 1, -1, -1.0, u);
             } else {
-                for (int pass = 0; pass < 2; ++pass) upd<T>(C, C.X, 0, pass, 1.0, u);
+                for (int pass = 0; pass < 2; ++pass) upd(C, C.X, 0, pass, 1.0, u);
             }
         }
         (void)sg;
@@ -730,9 +871,9 @@ struct hifde_factor {
 
     void apply_dev(double* d_u, int nrhs, long long ldu, bool inv) {
         if (!factored) throw Error(HIFDE_ERR_STATE, "apply: factor first");
+        apply_tmp.reserve((size_t)NC * NC * NC);
         for (int c = 0; c < nrhs; ++c) {
-            if (precision == HIFDE_MIXED) apply_one<float>(d_u + c * ldu, inv);
-            else apply_one<double>(d_u + c * ldu, inv);
+            apply_one(d_u + c * ldu, inv);
         }
         resolve_profile();
     }
@@ -990,8 +1131,7 @@ int hifde_set_stencil(hifde_factor* f, const double* a, const double* v, int sym
 int hifde_factor_run(hifde_factor* f) {
     return guarded([&]() -> int {
         if (!f) throw Error(HIFDE_ERR_ARG, "factor: null");
-        if (f->precision == HIFDE_MIXED) f->run_factor<float>();
-        else f->run_factor<double>();
+        f->run_factor();
         HIFDE_CHECK(hipStreamSynchronize(f->stream));
         return f->lu_info;
     });
@@ -1107,7 +1247,7 @@ int hifde_box_data(const hifde_factor* f, int level, int kind, int box, int* red
         if (red_gid) std::copy_n(B.red_gid.data() + B.red_off[box], r, red_gid);
         if (skel_gid) std::copy_n(B.skel_gid.data() + B.skel_off[box], s, skel_gid);
         auto rd = [&](const DevBuf<unsigned char>& buf, long long off, long long cnt, double* o) {
-            if (f->precision == HIFDE_MIXED) read_dev<float>(buf, off, cnt, o, f->stream);
+            if (B.esize == sizeof(float)) read_dev<float>(buf, off, cnt, o, f->stream);
             else read_dev<double>(buf, off, cnt, o, f->stream);
         };
         rd(B.A22, B.off_rr[box], (long long)r * r, A22);
@@ -1123,6 +1263,18 @@ int hifde_stage_times(const hifde_factor* f, double* out, int nout) {
     int n = std::min(nout, (int)ST_COUNT);
     for (int i = 0; i < n; ++i) out[i] = f->stage_t[i];
     return n;
+}
+
+int hifde_set_option(hifde_factor* f, const char* name, double value) {
+    return guarded([&]() -> int {
+        if (!f || !name) throw Error(HIFDE_ERR_ARG, "set_option: null");
+        std::string n(name);
+        if (n == "gram_min_n") f->gram_min_n = (int)value;
+        else if (n == "gram_min_eps") f->gram_min_eps = value;
+        else if (n == "fp32_levels") f->fp32_levels = (int)value;
+        else throw Error(HIFDE_ERR_ARG, "set_option: unknown option " + n);
+        return HIFDE_OK;
+    });
 }
 
 int hifde_set_profiling(hifde_factor* f, int on) {
@@ -1167,8 +1319,8 @@ long long hifde_factor_bytes(const hifde_factor* f) {
     long long tot = 0;
     for (int lv = 0; lv < (int)f->cells.size(); ++lv) {
         for (const BoxBatch* B : {&f->cells[lv], &f->faces[lv]}) {
-            tot += (long long)(2 * B->tot_rr + B->tot_rs) * (long long)f->elem();
-            if (B == &f->faces[lv]) tot += (long long)B->tot_rs * (long long)f->elem();
+            tot += (long long)(2 * B->tot_rr + B->tot_rs) * (long long)B->esize;
+            if (B == &f->faces[lv]) tot += (long long)B->tot_rs * (long long)B->esize;
         }
     }
     return tot;
@@ -1203,7 +1355,7 @@ int hifde_interp_decomp_batch(int nbatch, const int* m, const int* n, const doub
         dperm.reserve(poff.back());
         dk.reserve(nbatch);
         batched_interp_decomp(nbatch, dm.p, dn.p, doffM.p, dM.p, eps, dperm.p, dpoff.p, dk.p,
-                              dW.p, doffW.p, max_n, max_m, s);
+                              dW.p, doffW.p, max_n, max_m, nullptr, s);
         HIFDE_CHECK(hipMemcpy(k_out, dk.p, nbatch * sizeof(int), hipMemcpyDeviceToHost));
         HIFDE_CHECK(hipMemcpy(perm_out, dperm.p, poff.back() * sizeof(int), hipMemcpyDeviceToHost));
         HIFDE_CHECK(hipMemcpy(W_out, dW.p, offW.back() * sizeof(double), hipMemcpyDeviceToHost));
@@ -1243,10 +1395,11 @@ int schur_batch_impl(int nb, const int* r, const int* s, const double* A22, cons
     HIFDE_CHECK(hipMemset(dinfo.p, 0, nb * sizeof(int)));
     GJWork gw;
     batched_gj_inverse<T>(nb, hr.data(), dr.p, drr.p, dA.p, dinfo.p, gw, st);
-    batched_gemm_nn<T>(nb, hr.data(), hs.data(), hr.data(), dr.p, ds.p, dr.p, drr.p, drs.p,
-                       drs.p, dA.p, dA21.p, dX.p, (T)1, tiles, st);
-    batched_gemm_nn<T>(nb, hs.data(), hs.data(), hr.data(), ds.p, ds.p, dr.p, drs.p, drs.p,
-                       dss.p, dA12.p, dX.p, dS.p, (T)-1, tiles, st);
+    GemmWork gws;
+    batched_gemm<T>(false, nb, hr.data(), hs.data(), hr.data(), orr.data(), ors.data(), ors.data(),
+                    dA.p, dA21.p, dX.p, (T)1, gws, st);
+    batched_gemm<T>(false, nb, hs.data(), hs.data(), hr.data(), ors.data(), ors.data(), oss.data(),
+                    dA12.p, dX.p, dS.p, (T)-1, gws, st);
     auto down = [&](DevBuf<T>& d, long long cnt, double* dst) {
         std::vector<T> tmp(cnt);
         if (cnt) HIFDE_CHECK(hipMemcpy(tmp.data(), d.p, cnt * sizeof(T), hipMemcpyDeviceToHost));

@@ -74,6 +74,7 @@ struct LevelTable {
 // elements.
 struct BoxBatch {
     int nbox = 0;
+    int esize = 8;  // bytes per element of the stored factor blocks
     std::vector<int> r, s;                      // |red|, |skel| per box
     std::vector<long long> off_rr, off_rs, off_ss;  // prefix sums of r*r, r*s, s*s (nbox+1)
     std::vector<int> red_off, skel_off;         // prefix sums of r, s (nbox+1)
@@ -85,6 +86,10 @@ struct BoxBatch {
     // Factor data in the factor precision (float or double), as raw bytes.
     DevBuf<unsigned char> A22, A22inv, X, W;
     long long tot_rr = 0, tot_rs = 0, tot_ss = 0;
+    // apply tiling: prefix sums of ceil(r/64) and ceil(s/64)
+    std::vector<int> tiles_r, tiles_s;
+    DevBuf<int> d_tiles_r, d_tiles_s;
+    int max_r = 0, max_s = 0;
     // Faces only: per-face box geometry.
     std::vector<int> face_kind;
 };
@@ -111,50 +116,73 @@ void table_scatter(LevelTable& t, int nbox, const int* d_s, const int* d_skel_of
                    const int* d_skel_gid, const unsigned char* d_skel_pass,
                    const long long* d_off_ss, long long total, const T* d_S, int pass,
                    hipStream_t s);
-// Keep only entries whose row and column are alive; resize to hold extra inserts.
-void table_compact(LevelTable& t, const unsigned char* d_alive, size_t extra, hipStream_t s);
+// dst = entries of src whose row and column are alive, in a table sized to
+// also hold `extra` further inserts.  dst's buffer only ever grows.
+// Returns the number of live entries copied.
+size_t table_compact(const LevelTable& src, LevelTable& dst, const unsigned char* d_alive,
+                     size_t extra, DevBuf<unsigned long long>& counter, hipStream_t s);
 size_t table_count(const LevelTable& t, hipStream_t s);
 
 // dense batched kernels
 // Workspace of the blocked inverse (pivots, pivot rows, tile list).
 struct GJWork {
-    DevBuf<int> piv, piv_off, tile_off;
+    DevBuf<int> tile_off;
     DevBuf<long long> b_off;
-    DevBuf<unsigned char> B;
+    DevBuf<unsigned char> B, D;
 };
 template <class T>
 void batched_gj_inverse(int nbox, const int* h_n, const int* d_n, const long long* d_off, T* d_A,
                         int* d_info, GJWork& ws, hipStream_t s);
+struct GemmDesc {
+    long long offA, offB, offC;
+    int m, n, k, lda, ldb, ldc;
+};
+struct CopyDesc {
+    long long off;  // source block base
+    int r0, c0, rows, ld;
+};
+struct GemmWork {
+    DevBuf<int> tiles;
+    DevBuf<GemmDesc> desc;
+    DevBuf<CopyDesc> cdesc;
+    DevBuf<long long> off;
+};
+// C = alpha op(A) B + beta C per descriptor (op(A) = A^T when transA).
 template <class T>
-void batched_gemm_nn(int nbox, const int* h_m, const int* h_n, const int* h_k, const int* d_m,
-                     const int* d_n, const int* d_k, const long long* d_offA,
-                     const long long* d_offB, const long long* d_offC, const T* d_A,
-                     const T* d_B, T* d_C, T alpha, DevBuf<int>& tile_scratch, hipStream_t s);
+void batched_gemm_desc(bool transA, const std::vector<GemmDesc>& desc, const T* d_A, const T* d_B,
+                       T* d_C, T alpha, T beta, GemmWork& ws, hipStream_t s);
+// Packed boxes (ld = rows): C = alpha op(A) B with host offsets.
+template <class T>
+void batched_gemm(bool transA, int nbox, const int* h_m, const int* h_n, const int* h_k,
+                  const long long* h_offA, const long long* h_offB, const long long* h_offC,
+                  const T* d_A, const T* d_B, T* d_C, T alpha, GemmWork& ws, hipStream_t s);
+template <class D>
+void copy_blocks(const std::vector<CopyDesc>& desc, const std::vector<long long>& dst_off,
+                 const double* d_src, D* d_dst, GemmWork& ws, hipStream_t s);
+void permute_sym(int nbox, const long long* d_off, const int* d_n, const int* d_perm,
+                 const int* d_perm_off, const double* d_C, double* d_Cp, long long total,
+                 hipStream_t s);
+// mode[f] != 0 selects the Gram path for face f; the Householder kernel skips it.
 void batched_interp_decomp(int nface, const int* d_m, const int* d_n, const long long* d_offM,
                            double* d_M, double eps, int* d_perm, const int* d_perm_off,
                            int* d_k, double* d_W, const long long* d_offW, int max_n,
-                           int max_m, hipStream_t s);
-template <class T>
-void batched_face_transform(int nface, const int* d_n, const int* d_k, const int* d_perm,
-                            const int* d_perm_off, const double* d_C, const long long* d_offC,
-                            const double* d_W, const long long* d_offW, double* d_Tsr, T* d_A22,
-                            T* d_A21, T* d_A12, const long long* d_off_rr,
-                            const long long* d_off_rs, hipStream_t s);
+                           int max_m, const unsigned char* d_mode, hipStream_t s);
+void gram_interp_decomp(int nface, const int* d_n, const unsigned char* d_mode,
+                        const double* d_G, const long long* d_offG, double* d_R, double eps,
+                        int* d_perm, const int* d_perm_off, int* d_k, double* d_W, int max_n,
+                        hipStream_t s);
 void convert_d2f(const double* src, float* dst, long long n, hipStream_t s);
 template <class D>
 void pack_boxes(int nbox, const long long* d_dst_off, const long long* d_src_off,
                 const double* d_src, D* d_dst, long long total, hipStream_t s);
 
-// apply sweeps: u (double, length NC^3) updated in place
+// apply sweeps: one 64-output tile of one box per workgroup (see apply_pcg.hip)
 template <class T>
-void apply_update(int nbox, const int* d_r, const int* d_s, const int* d_red_off,
-                  const int* d_skel_off, const int* d_red_gid, const int* d_skel_gid,
-                  const unsigned char* d_skel_pass, const long long* d_off, const T* d_M,
-                  int op, int pass, double alpha, double* d_u, int max_dim, hipStream_t s);
-template <class T>
-void apply_diag(int nbox, const int* d_r, const int* d_red_off, const int* d_red_gid,
-                const long long* d_off_rr, const T* d_M, double* d_u, int max_r,
-                hipStream_t s);
+void apply_tiles(int nbox, int ntiles, const int* d_tile_off, const int* d_r, const int* d_s,
+                 const int* d_red_off, const int* d_skel_off, const int* d_red_gid,
+                 const int* d_skel_gid, const unsigned char* d_skel_pass, const long long* d_off,
+                 const T* d_M, int op, int pass, double alpha, const double* d_in,
+                 double* d_out, int max_in, hipStream_t s);
 void zero_boundary(double* d_u, int NC, hipStream_t s);
 
 // vectors / CSR SpMV for PCG

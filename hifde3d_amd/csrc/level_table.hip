@@ -182,8 +182,7 @@ size_t pow2_at_least(size_t x) {
 
 void table_init(LevelTable& t, size_t min_entries, hipStream_t s) {
     size_t cap = pow2_at_least(min_entries * 2 + 16);
-    t.slots.release();
-    t.slots.reserve(cap);
+    t.slots.reserve(cap);  // grow-only: the buffer is kept across factorizations
     t.cap = cap;
     k_table_clear<<<grid_for((long long)cap), 256, 0, s>>>(t.slots.p, cap);
     HIFDE_LAUNCH_CHECK();
@@ -232,28 +231,21 @@ size_t table_count(const LevelTable& t, hipStream_t s) {
     return (size_t)h;
 }
 
-void table_compact(LevelTable& t, const unsigned char* d_alive, size_t extra, hipStream_t s) {
-    DevBuf<unsigned long long> cnt;
-    cnt.reserve(1);
-    HIFDE_CHECK(hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long), s));
-    k_table_count_live<<<grid_for((long long)t.cap), 256, 0, s>>>(t.slots.p, t.cap, d_alive,
-                                                                   cnt.p);
+size_t table_compact(const LevelTable& src, LevelTable& dst, const unsigned char* d_alive,
+                     size_t extra, DevBuf<unsigned long long>& counter, hipStream_t s) {
+    counter.reserve(1);
+    HIFDE_CHECK(hipMemsetAsync(counter.p, 0, sizeof(unsigned long long), s));
+    k_table_count_live<<<grid_for((long long)src.cap), 256, 0, s>>>(src.slots.p, src.cap, d_alive,
+                                                                     counter.p);
     HIFDE_LAUNCH_CHECK();
     unsigned long long live = 0;
-    HIFDE_CHECK(hipMemcpyAsync(&live, cnt.p, sizeof(live), hipMemcpyDeviceToHost, s));
+    HIFDE_CHECK(hipMemcpyAsync(&live, counter.p, sizeof(live), hipMemcpyDeviceToHost, s));
     HIFDE_CHECK(hipStreamSynchronize(s));
-    size_t ncap = pow2_at_least((size_t)(live + extra) * 2 + 16);
-    LevelTable nt;
-    nt.slots.reserve(ncap);
-    nt.cap = ncap;
-    k_table_clear<<<grid_for((long long)ncap), 256, 0, s>>>(nt.slots.p, ncap);
+    table_init(dst, (size_t)(live + extra), s);
+    k_table_rehash<<<grid_for((long long)src.cap), 256, 0, s>>>(src.slots.p, src.cap, d_alive,
+                                                                 dst.slots.p, dst.cap - 1);
     HIFDE_LAUNCH_CHECK();
-    k_table_rehash<<<grid_for((long long)t.cap), 256, 0, s>>>(t.slots.p, t.cap, d_alive,
-                                                               nt.slots.p, ncap - 1);
-    HIFDE_LAUNCH_CHECK();
-    HIFDE_CHECK(hipStreamSynchronize(s));
-    t.slots = std::move(nt.slots);
-    t.cap = ncap;
+    return (size_t)live;
 }
 
 template void table_gather<double>(const LevelTable&, int, const int*, const int*, const int*,

@@ -32,8 +32,10 @@ typedef struct hifde_factor hifde_factor;
 
 /* precision of the factorization */
 #define HIFDE_FP64 0  /* every block in FP64 (the reference's HIFFactor<double>) */
-#define HIFDE_MIXED 1 /* Schur/LU blocks and stored factors in FP32; ID pivots,
-                         rank decisions and the level matrix in FP64 */
+#define HIFDE_MIXED 1 /* Schur/LU blocks and stored factors of the lower levels
+                         in FP32 (all but the top two by default); ID pivots,
+                         rank decisions, the level matrix and the top levels
+                         in FP64 */
 
 /* flags */
 #define HIFDE_FLAG_FRONT_QUIRK 1 /* reproduce the FRONT-face guard of
@@ -107,6 +109,15 @@ int hifde_box_data(const hifde_factor* f, int level, int kind, int box, int* red
  * stages written (names from hifde_stage_name). */
 int hifde_stage_times(const hifde_factor* f, double* out, int nout);
 const char* hifde_stage_name(int stage);
+/* Tuning options (no reference counterpart):
+ *   "gram_min_n"   faces with more ID columns than this use the Gram-matrix
+ *                  path (G = M^T M, pivoted Cholesky); default 64
+ *   "gram_min_eps" below this ID tolerance every face uses the Householder
+ *                  column-pivoted QR; default 1e-7
+ *   "fp32_levels"  HIFDE_MIXED only: levels 0 .. fp32_levels-1 are factored
+ *                  and stored in FP32, the levels above in FP64; default
+ *                  num_levels - 2 (the two top levels in FP64) */
+int hifde_set_option(hifde_factor* f, const char* name, double value);
 /* Per-kernel-family device time, recorded with HIP events on the factor's
  * stream while profiling is on.  hifde_kernel_stats returns the number of
  * families; for 0 <= idx < that number it fills name and

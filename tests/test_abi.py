@@ -43,3 +43,12 @@ def test_bad_arguments_return_error_codes_without_gpu():
     assert L.hifde_create(15, 4, 1e-6, 7, 0, ctypes.byref(h)) == -1
     assert b"power of two" in L.hifde_last_error() or b"precision" in L.hifde_last_error()
     assert L.hifde_num_levels(None) == -1
+
+
+def test_library_has_no_unresolved_internal_symbols():
+    import subprocess
+    out = subprocess.run(["nm", "-D", "--undefined-only", hifde3d_amd.LIB_PATH],
+                         capture_output=True, text=True, check=True).stdout
+    assert "hifde" not in out, out
+    # load with immediate binding, as the runtime loader on the GPU host does
+    ctypes.CDLL(hifde3d_amd.LIB_PATH, mode=os.RTLD_NOW)

@@ -217,3 +217,23 @@ def test_nonpower_of_two_leaf_p3_and_p5(H, oracle_mod):
         assert f.level_counts()[0][1] == o.level_counts()[0][1]
         b = pb.rhs_normal(N + 1, 1)
         assert _rel(A @ f.Apply(b, True), b) <= 2 * _rel(A @ o.apply(b, True), b) + 1e-13
+
+
+@pytest.mark.parametrize("eps", [1e-6, 1e-4])
+def test_gram_id_path_matches_oracle(H, oracle_mod, eps):
+    """Faces forced onto the Gram-matrix ID path (pivoted Cholesky of M^T M)."""
+    N, P = 31, 4
+    a, V, coo, A = _coo(oracle_mod, N, "pow10", True)
+    o = oracle_mod.Oracle(N, P, eps)
+    o.set_matrix_coo(*coo)
+    assert o.factor(4) == 0
+    f = H.HIFFactor(N, P, eps)
+    f.set_option("gram_min_n", 4)
+    f.set_stencil(a, V, True)
+    assert f.Factor() == 0
+    cg, co = f.level_counts(), o.level_counts()
+    for lv in range(len(co)):
+        for idx in (1, 3):
+            assert abs(cg[lv][idx] - co[lv][idx]) <= 0.05 * max(co[lv][idx], 20)
+    b = pb.rhs_normal(N + 1, 0)
+    assert _rel(A @ f.Apply(b, True), b) <= 2 * _rel(A @ o.apply(b, True), b)
