@@ -23,7 +23,20 @@ sys.path.insert(0, ROOT)
 
 METRIC = "factor s + PCG solves/s at 128^3 to 1e-10 resid: mixed vs FP64 GPU vs CPU"
 PEAK = {"hbm_GBs": 8000.0, "fp64_TF": 78.6, "fp32_TF": 157.3}
-COMPUTE_FAMILIES = {"gemm_x", "gemm_s", "gj_inverse", "interp_decomp", "face_transform"}
+COMPUTE_FAMILIES = ("gemm_x", "gemm_s", "gj_inverse", "interp_decomp", "face_transform",
+                    "gram_gemm", "gram_pivchol")
+# kernels (as rocprofv3 names them, arguments stripped) that make up a family
+FAMILY_KERNELS = {
+    "gather_f32": ["k_table_gather<float>"], "gather_f64": ["k_table_gather<double>"],
+    "scatter_f32": ["k_table_scatter<float>"], "scatter_f64": ["k_table_scatter<double>"],
+    "gj_inverse_f32": ["k_gj_small<float>", "k_bgj_diag<float>", "k_bgj_row<float>",
+                       "k_bgj_update<float>", "k_bgj_col<float>"],
+    "gj_inverse_f64": ["k_gj_small<double>", "k_bgj_diag<double>", "k_bgj_row<double>",
+                       "k_bgj_update<double>", "k_bgj_col<double>"],
+    "interp_decomp": ["k_interp_decomp"], "gram_pivchol": ["k_gram_pivchol"],
+    "spmv": ["k_csr_spmv"], "compact": ["k_table_count_live", "k_table_rehash"],
+}
+PMC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_n128.json")
 
 
 def log(*a):
@@ -38,18 +51,47 @@ def make_problem(n, seed, corr):
     return a, V, b
 
 
-def roofline_from_stats(stats, precision):
+def family_traffic(fam, launches_per_step):
+    """HBM bytes per launch of a kernel family from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (one factor + one solve of this workload),
+    scaled by the copy-kernel calibration; None if not available."""
+    if fam not in FAMILY_KERNELS or not os.path.exists(PMC_FILE) or launches_per_step <= 0:
+        return None
+    pm = json.load(open(PMC_FILE))
+    tot = 0.0
+    for k in FAMILY_KERNELS[fam]:
+        if k in pm["kernels"]:
+            e = pm["kernels"][k]
+            tot += 1024.0 * (pm["fetch_scale"] * e["fetch_kib"] + pm["write_scale"] * e["write_kib"])
+    return tot / launches_per_step if tot > 0 else None
+
+
+def roofline_from_stats(stats, steps):
     fam = max(stats, key=lambda k: stats[k]["sec"])
     st = stats[fam]
     avg = st["sec"] / max(st["launches"], 1)
-    if fam in COMPUTE_FAMILIES:
-        peak = PEAK["fp32_TF"] if precision == "mixed" and fam != "interp_decomp" else PEAK["fp64_TF"]
+    traffic = family_traffic(fam, st["launches"] / max(steps, 1))
+    if fam.startswith(COMPUTE_FAMILIES):
+        peak = PEAK["fp32_TF"] if fam.endswith("_f32") else PEAK["fp64_TF"]
         achieved = st["flops"] / st["sec"] / 1e12
         return fam, avg, {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                          "frac": achieved / peak, "traffic": None}
+                          "frac": achieved / peak, "traffic": traffic}
     achieved = st["bytes"] / st["sec"] / 1e9
     return fam, avg, {"bound": "hbm", "achieved": achieved, "peak": PEAK["hbm_GBs"],
-                      "unit": "GB/s", "frac": achieved / PEAK["hbm_GBs"], "traffic": None}
+                      "unit": "GB/s", "frac": achieved / PEAK["hbm_GBs"], "traffic": traffic}
+
+
+def copy_calibration():
+    """1 GiB device-to-device copy, three times: the reference kernel for
+    scaling rocprofv3's FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import torch
+    x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": x.numel() * 8, "copies": 3}))
 
 
 def cpu_baseline(n_sample, n_target, seed, corr, eps, P, tol):
@@ -109,7 +151,11 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=32)
+    ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
@@ -158,7 +204,7 @@ def main():
     stages = f.stage_times()
     counts = f.level_counts()
     x_mixed = d_x.cpu().numpy()
-    fam, avg_launch, roof = roofline_from_stats(stats, "mixed")
+    fam, avg_launch, roof = roofline_from_stats(stats, args.steps)
     log(f"[bench] dominant kernel family {fam}: avg launch {avg_launch * 1e3:.3f} ms")
     for k_, v_ in sorted(stats.items(), key=lambda kv: -kv[1]["sec"]):
         log(f"[bench]   {k_:15s} launches {v_['launches']:6d} dev {v_['sec']:.4f}s")

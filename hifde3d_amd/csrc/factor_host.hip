@@ -110,7 +110,7 @@ struct hifde_factor {
     DevBuf<int> ws_gm, ws_gk;
     DevBuf<long long> ws_goffM, ws_goffC;
     DevBuf<double> ws_G;
-    int gram_min_n = 64;          // faces with more columns use the Gram path
+    int gram_min_n = 16;          // faces with more columns use the Gram path
     double gram_min_eps = 1e-7;   // below this tolerance every face uses Householder
     DevBuf<double> pcg_buf, pcg_part, pcg_scalar, apply_tmp;
     double stage_t[ST_COUNT] = {0};
@@ -128,12 +128,16 @@ struct hifde_factor {
     int lin(int i, int j, int k) const { return i + NC * j + NC * NC * k; }
     int fp32_levels = -1;  // mixed precision: levels factored in FP32 (-1: all but the top two)
 
-    int family(const char* name) {
+    int family(const std::string& name) {
         for (size_t i = 0; i < fam_names.size(); ++i)
             if (fam_names[i] == name) return (int)i;
         fam_names.push_back(name);
         fam_stats.emplace_back();
         return (int)fam_names.size() - 1;
+    }
+    template <class T>
+    static std::string fam(const char* base) {
+        return std::string(base) + (sizeof(T) == sizeof(float) ? "_f32" : "_f64");
     }
     hipEvent_t take_event() {
         if (!ev_pool.empty()) {
@@ -152,7 +156,8 @@ struct hifde_factor {
         int fam = -1;
         hipEvent_t e0 = nullptr;
         double flops, bytes;
-        Prof(hifde_factor* f_, const char* name, double fl, double by) : f(f_), flops(fl), bytes(by) {
+        Prof(hifde_factor* f_, const std::string& name, double fl, double by)
+            : f(f_), flops(fl), bytes(by) {
             if (!f->profiling) return;
             fam = f->family(name);
             e0 = f->take_event();
@@ -423,7 +428,7 @@ struct hifde_factor {
             by_s += (2 * r * sd + sd * sd) * w;
         }
         {
-            Prof pr(this, "gj_inverse", fl_inv, by_inv);
+            Prof pr(this, fam<T>("gj_inverse"), fl_inv, by_inv);
             batched_gj_inverse<T>(nb, B.r.data(), B.d_r.p, B.d_off_rr.p, Ainv, ws_info.p, gj_ws,
                                   stream);
         }
@@ -432,13 +437,13 @@ struct hifde_factor {
         HIFDE_CHECK(hipMemcpyAsync(info.data(), ws_info.p, nb * sizeof(int),
                                    hipMemcpyDeviceToHost, stream));
         {
-            Prof pr(this, "gemm_x", fl_x, by_x);
+            Prof pr(this, fam<T>("gemm_x"), fl_x, by_x);
             batched_gemm<T>(false, nb, B.r.data(), B.s.data(), B.r.data(), B.off_rr.data(),
                             B.off_rs.data(), B.off_rs.data(), Ainv, A21, X, (T)1, gemm_ws,
                             stream);
         }
         {
-            Prof pr(this, "gemm_s", fl_s, by_s);
+            Prof pr(this, fam<T>("gemm_s"), fl_s, by_s);
             batched_gemm<T>(false, nb, B.s.data(), B.s.data(), B.r.data(), B.off_rs.data(),
                             B.off_rs.data(), B.off_ss.data(), A12, X, S, (T)-1, gemm_ws, stream);
         }
@@ -450,6 +455,18 @@ struct hifde_factor {
     void mark_eliminated(const BoxBatch& B) {
         for (int g : B.red_gid) remaining[g] = 0;  // UpdateRemainingDOFs (:91-111)
         d_alive.upload(remaining, stream);
+    }
+
+    // Number of S entries written in the second scatter pass (both DOFs on
+    // min faces of their cell).
+    double pass2_entries(const BoxBatch& B) const {
+        double tot = 0;
+        for (int b = 0; b < B.nbox; ++b) {
+            double m = 0;
+            for (int t = B.skel_off[b]; t < B.skel_off[b + 1]; ++t) m += B.skel_pass[t];
+            tot += m * m;
+        }
+        return tot;
     }
 
     template <class T>
@@ -467,16 +484,19 @@ struct hifde_factor {
         }
         tab_inserted += (size_t)B.tot_ss;
         timed(ST_COMPACT, t0);
-        {
-            Prof pr(this, "scatter", 0.0, (double)B.tot_ss * (sizeof(T) + 32));
-            if (two_pass) {
-                for (int pass = 0; pass < 2; ++pass)
-                    table_scatter<T>(tabs[cur], B.nbox, B.d_s.p, B.d_skel_off.p, B.d_skel_gid.p,
-                                     B.d_skel_pass.p, B.d_off_ss.p, B.tot_ss, S, pass, stream);
-            } else {
-                table_scatter<T>(tabs[cur], B.nbox, B.d_s.p, B.d_skel_off.p, B.d_skel_gid.p, nullptr,
-                                 B.d_off_ss.p, B.tot_ss, S, -1, stream);
+        // One profiling record per launch: algorithmic bytes are the S values
+        // read plus one 16-byte slot read and written per entry added.
+        if (two_pass) {
+            for (int pass = 0; pass < 2; ++pass) {
+                double cnt = pass == 0 ? (double)B.tot_ss - pass2_entries(B) : pass2_entries(B);
+                Prof pr(this, fam<T>("scatter"), 0.0, (double)B.tot_ss * (4 + 4) + cnt * (sizeof(T) + 32));
+                table_scatter<T>(tabs[cur], B.nbox, B.d_s.p, B.d_skel_off.p, B.d_skel_gid.p,
+                                 B.d_skel_pass.p, B.d_off_ss.p, B.tot_ss, S, pass, stream);
             }
+        } else {
+            Prof pr(this, fam<T>("scatter"), 0.0, (double)B.tot_ss * (sizeof(T) + 32));
+            table_scatter<T>(tabs[cur], B.nbox, B.d_s.p, B.d_skel_off.p, B.d_skel_gid.p, nullptr,
+                             B.d_off_ss.p, B.tot_ss, S, -1, stream);
         }
         timed(ST_SCATTER, t0);
     }
@@ -499,13 +519,21 @@ struct hifde_factor {
         T* S = reinterpret_cast<T*>(ws_c.p);
         // DenseSubmatrix of the cell's DOFs (include/Schur.hpp:44-61), only
         // the three blocks Schur reads.
+        // One profiling record per launch: algorithmic bytes are one 16-byte
+        // slot read and one value written per element.
         {
-            Prof pr_g(this, "gather", 0.0, (double)(B.tot_rr + 2 * B.tot_rs) * (sizeof(T) + 16));
+            Prof pr(this, fam<T>("gather"), 0.0, (double)B.tot_rr * (sizeof(T) + 16));
             table_gather<T>(tabs[cur], B.nbox, B.d_red_gid.p, B.d_red_off.p, B.d_red_gid.p,
                             B.d_red_off.p, B.d_off_rr.p, B.tot_rr, reinterpret_cast<T*>(B.A22.p),
                             stream);
+        }
+        {
+            Prof pr(this, fam<T>("gather"), 0.0, (double)B.tot_rs * (sizeof(T) + 16));
             table_gather<T>(tabs[cur], B.nbox, B.d_red_gid.p, B.d_red_off.p, B.d_skel_gid.p,
                             B.d_skel_off.p, B.d_off_rs.p, B.tot_rs, A21, stream);
+        }
+        {
+            Prof pr(this, fam<T>("gather"), 0.0, (double)B.tot_rs * (sizeof(T) + 16));
             table_gather<T>(tabs[cur], B.nbox, B.d_skel_gid.p, B.d_skel_off.p, B.d_red_gid.p,
                             B.d_red_off.p, B.d_off_rs.p, B.tot_rs, A12, stream);
         }
@@ -547,9 +575,12 @@ struct hifde_factor {
         ws_W.reserve(offC.back());
         ws_T.reserve(offC.back());
         {
-            Prof pr(this, "gather", 0.0, (double)(offM.back() + offC.back()) * (8 + 16));
+            Prof pr(this, "gather_f64", 0.0, (double)offM.back() * (8 + 16));
             table_gather<double>(tabs[cur], nf, ws_rows.p, ws_rows_off.p, ws_cols.p, ws_cols_off.p,
                                  ws_offM.p, offM.back(), ws_M.p, stream);
+        }
+        {
+            Prof pr(this, "gather_f64", 0.0, (double)offC.back() * (8 + 16));
             table_gather<double>(tabs[cur], nf, ws_cols.p, ws_cols_off.p, ws_cols.p, ws_cols_off.p,
                                  ws_offC.p, offC.back(), ws_C.p, stream);
         }
@@ -786,7 +817,7 @@ struct hifde_factor {
         double by = (double)B.tot_rs * sizeof(T) +
                     16.0 * ((B.red_off.empty() ? 0 : B.red_off.back()) +
                             (B.skel_off.empty() ? 0 : B.skel_off.back()));
-        Prof pr(this, "apply_update", 2.0 * B.tot_rs, by);
+        Prof pr(this, fam<T>("apply_update"), 2.0 * B.tot_rs, by);
         apply_tiles<T>(B.nbox, tl.back(), out_red ? B.d_tiles_r.p : B.d_tiles_s.p, B.d_r.p,
                        B.d_s.p, B.d_red_off.p, B.d_skel_off.p, B.d_red_gid.p, B.d_skel_gid.p,
                        pass >= 0 ? B.d_skel_pass.p : nullptr, B.d_off_rs.p,
@@ -796,7 +827,7 @@ struct hifde_factor {
     template <class T>
     void diag_t(BoxBatch& B, const DevBuf<unsigned char>& Mbuf, const double* u, double* out) {
         if (B.nbox == 0) return;
-        Prof pr(this, "apply_diag", 2.0 * B.tot_rr,
+        Prof pr(this, fam<T>("apply_diag"), 2.0 * B.tot_rr,
                 (double)B.tot_rr * sizeof(T) + 16.0 * (B.red_off.empty() ? 0 : B.red_off.back()));
         apply_tiles<T>(B.nbox, B.tiles_r.back(), B.d_tiles_r.p, B.d_r.p, B.d_s.p, B.d_red_off.p,
                        B.d_skel_off.p, B.d_red_gid.p, B.d_skel_gid.p, nullptr, B.d_off_rr.p,

@@ -1,0 +1,444 @@
+// hifde3d_amd.hpp -- C++ drop-in for the hot path of HIFDE3D on MI355X.
+//
+// Mirrors the reference's public C++ API (include/hifde3d.hpp and the headers
+// it pulls in) for the factor / apply path, implemented over the C ABI of
+// hifde_amd.h (link with libhifde_amd.so).  A driver written against the
+// reference -- e.g. tests/HIFDE3D/main.cpp -- compiles unchanged after
+// swapping the include:
+//
+//     #include "hifde3d_amd.hpp"   // instead of "hifde3d.hpp"
+//
+// Kept: HIFFactor<Scalar> (include/Factor.hpp:8-203), Dense/Vector/Sparse
+// (include/hifde3d/core/{dense,vector,sparse}.hpp, the members the path
+// uses), NumTns (include/numtns.hpp), Index3 (include/vec3t.hpp),
+// SetupStencil (include/setup_stencil.hpp), SpMV / RelativeErrorNorm2
+// (include/Factor.hpp:205-209), InterpDecomp (include/InterpDecomp.hpp:18-21)
+// and Schur (include/Schur.hpp:72-100).
+// Scalar = double runs the FP64 path; Scalar = float runs the mixed path
+// (FP32 blocks on the lower levels, FP64 pivots and top levels).  Complex
+// scalars are not supported.
+// Errors: the reference checks nothing in RELEASE builds; here a failing
+// library call throws std::runtime_error with the library's message.
+#ifndef HIFDE3D_AMD_HPP_
+#define HIFDE3D_AMD_HPP_
+
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "hifde_amd.h"
+
+// Faces of a cube (include/data.hpp:7)
+enum Face { TOP = 0, BOTTOM, RIGHT, LEFT, FRONT, BACK };
+
+namespace hifde3d {
+
+namespace detail {
+inline void check(int rc) {
+    if (rc < 0) throw std::runtime_error(std::string("hifde: ") + hifde_last_error());
+}
+}  // namespace detail
+
+// ---------------------------------------------------------------------------
+// Index3 (include/vec3t.hpp): three ints with element access.
+class Index3 {
+public:
+    Index3() : v_{0, 0, 0} {}
+    Index3(int i, int j, int k) : v_{i, j, k} {}
+    int& operator()(int d) { return v_[d]; }
+    int operator()(int d) const { return v_[d]; }
+    int& operator[](int d) { return v_[d]; }
+    int operator[](int d) const { return v_[d]; }
+
+private:
+    int v_[3];
+};
+
+// ---------------------------------------------------------------------------
+// Vector<T> (include/hifde3d/core/vector.hpp)
+template <class T>
+class Vector {
+public:
+    Vector() {}
+    explicit Vector(int n) : d_(n, T(0)) {}
+    int Size() const { return (int)d_.size(); }
+    void Resize(int n) { d_.resize(n, T(0)); }
+    void PushBack(T v) { d_.push_back(v); }
+    T Get(int i) const { return d_[i]; }
+    void Set(int i, T v) { d_[i] = v; }
+    T& operator[](int i) { return d_[i]; }
+    const T& operator[](int i) const { return d_[i]; }
+    T* Buffer() { return d_.data(); }
+    const T* LockedBuffer() const { return d_.data(); }
+    void Clear() { d_.clear(); }
+
+private:
+    std::vector<T> d_;
+};
+
+// ---------------------------------------------------------------------------
+// Dense<T> (include/hifde3d/core/dense.hpp): column-major with leading dim.
+template <class T>
+class Dense {
+public:
+    Dense() : h_(0), w_(0), ld_(1) {}
+    Dense(int h, int w) : h_(0), w_(0), ld_(1) { Resize(h, w); }
+    template <class Ignored>
+    Dense(int h, int w, Ignored) : h_(0), w_(0), ld_(1) { Resize(h, w); }
+    int Height() const { return h_; }
+    int Width() const { return w_; }
+    int LDim() const { return ld_; }
+    void Resize(int h, int w) {
+        if (h > ld_) ld_ = h > 1 ? h : 1;
+        h_ = h;
+        w_ = w;
+        d_.resize((size_t)ld_ * w, T(0));
+    }
+    T Get(int i, int j) const { return d_[i + (size_t)j * ld_]; }
+    void Set(int i, int j, T v) { d_[i + (size_t)j * ld_] = v; }
+    T* Buffer(int i = 0, int j = 0) { return d_.data() + i + (size_t)j * ld_; }
+    const T* LockedBuffer(int i = 0, int j = 0) const { return d_.data() + i + (size_t)j * ld_; }
+    void Clear() {
+        h_ = w_ = 0;
+        ld_ = 1;
+        d_.clear();
+    }
+
+private:
+    int h_, w_, ld_;
+    std::vector<T> d_;
+};
+
+// ---------------------------------------------------------------------------
+// Sparse<T> (include/hifde3d/core/sparse.hpp): column-keyed map of maps;
+// Add sums into existing entries, Find reads absent entries as 0.
+template <class T>
+class Sparse {
+public:
+    Sparse() : h_(0), w_(0) {}
+    Sparse(int h, int w) : h_(h), w_(w) {}
+    int Height() const { return h_; }
+    int Width() const { return w_; }
+    void Add(int i, int j, T val) { m_[j][i] += val; }
+    void Add(int i, Vector<int>& jidx, Vector<T>& vals) {
+        for (int t = 0; t < jidx.Size(); ++t) m_[jidx.Get(t)][i] += vals.Get(t);
+    }
+    void Add(Vector<int>& iidx, int j, Vector<T>& vals) {
+        std::map<int, T>& col = m_[j];
+        for (int t = 0; t < iidx.Size(); ++t) col[iidx.Get(t)] += vals.Get(t);
+    }
+    T Find(int i, int j) const {
+        auto c = m_.find(j);
+        if (c == m_.end()) return T(0);
+        auto r = c->second.find(i);
+        return r == c->second.end() ? T(0) : r->second;
+    }
+    void Find(Vector<int>& iidx, Vector<int>& jidx, Dense<T>& D) const {
+        for (int b = 0; b < jidx.Size(); ++b)
+            for (int a = 0; a < iidx.Size(); ++a) D.Set(a, b, Find(iidx.Get(a), jidx.Get(b)));
+    }
+    void FindCol(int j, Vector<T>& vals, Vector<int>& inds) const {
+        auto c = m_.find(j);
+        if (c == m_.end()) return;
+        for (auto& kv : c->second) {
+            inds.PushBack(kv.first);
+            vals.PushBack(kv.second);
+        }
+    }
+    void DeleteRow(int i) {
+        for (auto& c : m_) c.second.erase(i);
+    }
+    void DeleteCol(int j) { m_[j].clear(); }
+    void Clear() { m_.clear(); }
+    int NonZeros() const {
+        int n = 0;
+        for (auto& c : m_) n += (int)c.second.size();
+        return n;
+    }
+    // COO view (rows, cols, vals), column by column.
+    void ToCOO(std::vector<int>& rows, std::vector<int>& cols, std::vector<double>& vals) const {
+        for (auto& c : m_)
+            for (auto& kv : c.second) {
+                rows.push_back(kv.first);
+                cols.push_back(c.first);
+                vals.push_back((double)kv.second);
+            }
+    }
+
+private:
+    int h_, w_;
+    std::map<int, std::map<int, T>> m_;
+};
+
+// ---------------------------------------------------------------------------
+// NumTns<T> (include/numtns.hpp): 3-tensor, index i + j*m + k*m*n.
+template <class T>
+class NumTns {
+public:
+    NumTns(int m = 0, int n = 0, int p = 0) : m_(m), n_(n), p_(p), d_((size_t)m * n * p, T(0)) {}
+    int m() const { return m_; }
+    int n() const { return n_; }
+    int p() const { return p_; }
+    T& operator()(int i, int j, int k) { return d_[i + (size_t)m_ * (j + (size_t)n_ * k)]; }
+    const T& operator()(int i, int j, int k) const {
+        return d_[i + (size_t)m_ * (j + (size_t)n_ * k)];
+    }
+    T& operator()(Index3 a) { return (*this)(a(0), a(1), a(2)); }
+    T* data() { return d_.data(); }
+
+private:
+    int m_, n_, p_;
+    std::vector<T> d_;
+};
+typedef NumTns<int> IntNumTns;
+
+// ---------------------------------------------------------------------------
+// HIFFactor<Scalar> (include/Factor.hpp:8-203)
+template <class Scalar>
+class HIFFactor {
+    static_assert(std::is_same<Scalar, double>::value || std::is_same<Scalar, float>::value,
+                  "HIFFactor on MI355X supports float (mixed) and double");
+
+public:
+    HIFFactor(int N, int P, double epsilon)
+        : N_(N), P_(P), epsilon_(epsilon), sp_matrix_(N * N * N, N * N * N), h_(nullptr) {}
+    ~HIFFactor() { hifde_destroy(h_); }
+    HIFFactor(const HIFFactor&) = delete;
+    HIFFactor& operator=(const HIFFactor&) = delete;
+
+    // Creates the device factor object (src/Factor.cpp:6-39).
+    void Initialize() {
+        hifde_destroy(h_);
+        h_ = nullptr;
+        detail::check(hifde_create(N_, P_, epsilon_, precision(), HIFDE_FLAG_FRONT_QUIRK, &h_));
+    }
+    // Uploads sp_matrix() and factors it (src/Factor.cpp:41-63).  As in the
+    // reference, the level matrix is consumed: sp_matrix() is left empty.
+    void Factor() {
+        if (!h_) Initialize();
+        std::vector<int> rows, cols;
+        std::vector<double> vals;
+        sp_matrix_.ToCOO(rows, cols, vals);
+        detail::check(hifde_set_matrix_coo(h_, (long long)rows.size(), rows.data(), cols.data(),
+                                           vals.data()));
+        sp_matrix_.Clear();
+        int rc = hifde_factor_run(h_);
+        detail::check(rc);
+        if (rc > 0) throw std::runtime_error("hifde: singular pivot in a Schur block");
+    }
+    // Apply(u, apply_inverse) (src/Factor.cpp:992-1081)
+    void Apply(Vector<Scalar>& u, bool apply_inverse = false) {
+        std::vector<double> x(u.Size());
+        for (int i = 0; i < u.Size(); ++i) x[i] = (double)u.Get(i);
+        detail::check(hifde_apply(h_, x.data(), apply_inverse ? 1 : 0));
+        for (int i = 0; i < u.Size(); ++i) u.Set(i, (Scalar)x[i]);
+    }
+    // Preconditioned CG on A x = b with this factorization (no counterpart
+    // in the reference).  Returns the iteration count.
+    int Solve(const Vector<Scalar>& b, Vector<Scalar>& x, double tol = 1e-10, int maxit = 500,
+              double* relres = nullptr) {
+        std::vector<double> bb(b.Size()), xx(b.Size(), 0.0);
+        for (int i = 0; i < b.Size(); ++i) bb[i] = (double)b.Get(i);
+        int it = 0;
+        double rr = 0;
+        detail::check(hifde_pcg(h_, bb.data(), xx.data(), tol, maxit, &it, &rr));
+        x.Resize(b.Size());
+        for (int i = 0; i < b.Size(); ++i) x.Set(i, (Scalar)xx[i]);
+        if (relres) *relres = rr;
+        return it;
+    }
+
+    Index3 Linear2TensorInd(int ind) {
+        int NC = N_ + 1;
+        int i = ind % NC;
+        int j = ((ind - i) % (NC * NC)) / NC;
+        int k = (ind - i - NC * j) / (NC * NC);
+        return Index3(i, j, k);
+    }
+    int Tensor2LinearInd(Index3 ind) {
+        int NC = N_ + 1;
+        return ind(0) + NC * ind(1) + (NC * NC) * ind(2);
+    }
+
+    void set_N(int N) { N_ = N; }
+    int N() { return N_; }
+    void set_P(int P) { P_ = P; }
+    int P() { return P_; }
+    void set_epsilon(double e) { epsilon_ = e; }
+    double epsilon() { return epsilon_; }
+    Sparse<Scalar>& sp_matrix() { return sp_matrix_; }
+    // Per-level counts the reference prints (src/Factor.cpp:211-212,258-259).
+    std::vector<std::vector<int>> LevelCounts() const {
+        std::vector<std::vector<int>> out;
+        int L = hifde_num_levels(h_);
+        for (int l = 0; l < L; ++l) {
+            std::vector<int> c(5);
+            detail::check(hifde_level_counts(h_, l, c.data()));
+            out.push_back(c);
+        }
+        return out;
+    }
+    hifde_factor* handle() { return h_; }
+
+private:
+    static int precision() {
+        return std::is_same<Scalar, float>::value ? HIFDE_MIXED : HIFDE_FP64;
+    }
+    int N_, P_;
+    double epsilon_;
+    Sparse<Scalar> sp_matrix_;
+    hifde_factor* h_;
+};
+
+// SetupStencil (include/setup_stencil.hpp:6-76): assembles the reference's
+// 7-point stencil into factor.sp_matrix() through Sparse::Add.
+template <class Scalar>
+void SetupStencil(HIFFactor<Scalar>& factor, int N, double h, NumTns<Scalar>& A,
+                  NumTns<Scalar>& V) {
+    Sparse<Scalar>& matrix = factor.sp_matrix();
+    double hh = h * h;
+    for (int i = 1; i <= N; ++i)
+        for (int j = 1; j <= N; ++j)
+            for (int k = 1; k <= N; ++k) {
+                Vector<int> jidx;
+                Vector<Scalar> vals;
+                Index3 ind(i, j, k);
+                Scalar a = Scalar(0);
+                a += (A(i - 1, j, k) + A(i + 1, j, k)) / hh;
+                a += (A(i, j - 1, k) + A(i, j + 1, k)) / hh;
+                a += (A(i, j, k - 1) + A(i, j, k + 1)) / hh;
+                a += V(i, j, k);
+                jidx.PushBack(factor.Tensor2LinearInd(ind));
+                vals.PushBack(a);
+                const int nb[6][3] = {{i - 1, j, k}, {i + 1, j, k}, {i, j - 1, k},
+                                      {i, j + 1, k}, {i, j, k - 1}, {i, j, k + 1}};
+                for (int t = 0; t < 6; ++t) {
+                    int x = nb[t][0], y = nb[t][1], z = nb[t][2];
+                    if (x < 1 || x > N || y < 1 || y > N || z < 1 || z > N) continue;
+                    Index3 q(x, y, z);
+                    jidx.PushBack(factor.Tensor2LinearInd(q));
+                    vals.PushBack(-A(q) / hh);
+                }
+                matrix.Add(factor.Tensor2LinearInd(ind), jidx, vals);
+            }
+}
+
+// ||x - y||_2 / ||x||_2 (src/Factor.cpp:1083-1097)
+template <class Scalar>
+double RelativeErrorNorm2(Vector<Scalar>& x, Vector<Scalar>& y) {
+    double err = 0, norm = 0;
+    for (int i = 0; i < x.Size(); ++i) {
+        double d = std::abs(x.Get(i) - y.Get(i));
+        err += d * d;
+        norm += std::abs(x.Get(i)) * std::abs(x.Get(i));
+    }
+    return std::sqrt(err / norm);
+}
+
+// SpMV(A, x, y) (src/Factor.cpp:1099-1113): y = A^T x, by columns.
+template <class Scalar>
+void SpMV(Sparse<Scalar>& A, Vector<Scalar>& x, Vector<Scalar>& y) {
+    for (int i = 0; i < x.Size(); ++i) {
+        Vector<Scalar> col;
+        Vector<int> inds;
+        A.FindCol(i, col, inds);
+        Scalar val = Scalar(0);
+        for (int j = 0; j < col.Size(); ++j) val += col.Get(j) * x.Get(inds.Get(j));
+        y.Set(i, val);
+    }
+}
+
+// InterpDecomp(M, W, skel, red, eps) (src/InterpDecomp.cpp:86-138), on the
+// GPU as a batch of one.  skel/red are appended to, W is resized; M is left
+// unchanged (the reference overwrites it with the QR factors).
+template <class Scalar>
+void InterpDecomp(Dense<Scalar>& M, Dense<Scalar>& W, std::vector<int>& skeleton_cols,
+                  std::vector<int>& redundant_cols, double epsilon) {
+    int m = M.Height(), n = M.Width();
+    std::vector<double> Mp((size_t)m * n + 1);
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < m; ++i) Mp[i + (size_t)j * m] = (double)M.Get(i, j);
+    int k = 0;
+    std::vector<int> perm(n + 1);
+    std::vector<double> Wp((size_t)n * n + 1);
+    detail::check(hifde_interp_decomp_batch(1, &m, &n, Mp.data(), epsilon, &k, perm.data(),
+                                            Wp.data()));
+    for (int t = 0; t < k; ++t) skeleton_cols.push_back(perm[t]);
+    for (int t = k; t < n; ++t) redundant_cols.push_back(perm[t]);
+    W.Resize(k, n - k);
+    for (int j = 0; j < n - k; ++j)
+        for (int i = 0; i < k; ++i) W.Set(i, j, (Scalar)Wp[i + (size_t)j * k]);
+}
+
+// Per-box factor record (include/data.hpp:11-110), the members Schur fills.
+class IndexData {
+public:
+    std::vector<int>& global_inds() { return global_; }
+    std::vector<int>& redundant_inds() { return red_; }
+    std::vector<int>& skeleton_inds() { return skel_; }
+
+private:
+    std::vector<int> global_, red_, skel_;
+};
+
+template <class Scalar>
+class FactorData {
+public:
+    Dense<Scalar>& A_22() { return A22_; }
+    Dense<Scalar>& A_22_inv() { return A22inv_; }
+    Dense<Scalar>& X_mat() { return X_; }
+    Dense<Scalar>& Schur_comp() { return S_; }
+    Dense<Scalar>& W_mat() { return W_; }
+    IndexData& ind_data() { return ind_; }
+    int NumDOFsEliminated() { return (int)ind_.redundant_inds().size(); }
+    void set_face(Face f) { face_ = f; }
+    Face face() { return face_; }
+
+private:
+    IndexData ind_;
+    Dense<Scalar> A22_, A22inv_, X_, S_, W_;
+    Face face_ = TOP;
+};
+
+// Schur(matrix, data) (include/Schur.hpp:72-100) on the GPU as a batch of one:
+// A22 = M(red, red), X = A22^{-1} M(red, skel), S = -M(skel, red) X.
+template <class Scalar>
+void Schur(Dense<Scalar>& matrix, FactorData<Scalar>& data) {
+    std::vector<int>& red = data.ind_data().redundant_inds();
+    std::vector<int>& skel = data.ind_data().skeleton_inds();
+    int r = (int)red.size(), s = (int)skel.size();
+    std::vector<double> A22((size_t)r * r + 1), A21((size_t)r * s + 1), A12((size_t)s * r + 1);
+    data.A_22().Resize(r, r);
+    for (int j = 0; j < r; ++j)
+        for (int i = 0; i < r; ++i) {
+            A22[i + (size_t)j * r] = (double)matrix.Get(red[i], red[j]);
+            data.A_22().Set(i, j, matrix.Get(red[i], red[j]));
+        }
+    for (int j = 0; j < s; ++j)
+        for (int i = 0; i < r; ++i) A21[i + (size_t)j * r] = (double)matrix.Get(red[i], skel[j]);
+    for (int j = 0; j < r; ++j)
+        for (int i = 0; i < s; ++i) A12[i + (size_t)j * s] = (double)matrix.Get(skel[i], red[j]);
+    std::vector<double> Ainv((size_t)r * r + 1), X((size_t)r * s + 1), S((size_t)s * s + 1);
+    int info = 0;
+    int prec = std::is_same<Scalar, float>::value ? HIFDE_MIXED : HIFDE_FP64;
+    detail::check(hifde_schur_batch(prec, 1, &r, &s, A22.data(), A21.data(), A12.data(),
+                                    Ainv.data(), X.data(), S.data(), &info));
+    data.A_22_inv().Resize(r, r);
+    data.X_mat().Resize(r, s);
+    data.Schur_comp().Resize(s, s);
+    for (int j = 0; j < r; ++j)
+        for (int i = 0; i < r; ++i) data.A_22_inv().Set(i, j, (Scalar)Ainv[i + (size_t)j * r]);
+    for (int j = 0; j < s; ++j)
+        for (int i = 0; i < r; ++i) data.X_mat().Set(i, j, (Scalar)X[i + (size_t)j * r]);
+    for (int j = 0; j < s; ++j)
+        for (int i = 0; i < s; ++i) data.Schur_comp().Set(i, j, (Scalar)S[i + (size_t)j * s]);
+}
+
+}  // namespace hifde3d
+
+#endif  // HIFDE3D_AMD_HPP_

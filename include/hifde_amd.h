@@ -111,7 +111,7 @@ int hifde_stage_times(const hifde_factor* f, double* out, int nout);
 const char* hifde_stage_name(int stage);
 /* Tuning options (no reference counterpart):
  *   "gram_min_n"   faces with more ID columns than this use the Gram-matrix
- *                  path (G = M^T M, pivoted Cholesky); default 64
+ *                  path (G = M^T M, pivoted Cholesky); default 16
  *   "gram_min_eps" below this ID tolerance every face uses the Householder
  *                  column-pivoted QR; default 1e-7
  *   "fp32_levels"  HIFDE_MIXED only: levels 0 .. fp32_levels-1 are factored

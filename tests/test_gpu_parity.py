@@ -237,3 +237,25 @@ def test_gram_id_path_matches_oracle(H, oracle_mod, eps):
             assert abs(cg[lv][idx] - co[lv][idx]) <= 0.05 * max(co[lv][idx], 20)
     b = pb.rhs_normal(N + 1, 0)
     assert _rel(A @ f.Apply(b, True), b) <= 2 * _rel(A @ o.apply(b, True), b)
+
+
+def test_cpp_dropin_driver_reference_protocol(H, oracle_mod):
+    """tests/cpp/hif_driver (the reference driver protocol written against
+    include/hifde3d_amd.hpp) against the oracle's e_a / e_s."""
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    out = subprocess.run([os.path.join(root, "tests", "cpp", "hif_driver"), "-tol", "1e-6",
+                          "-N", "15", "-P", "4"], capture_output=True, text=True, timeout=120,
+                         check=True).stdout
+    vals = dict(line.split() for line in out.splitlines() if line.startswith("e_"))
+    a, V, coo, A = _coo(oracle_mod, 15, "ones", False)
+    o = oracle_mod.Oracle(15, 4, 1e-6)
+    o.set_matrix_coo(*coo)
+    o.factor(2)
+    v = pb.reference_test_vector(16)
+    w = A.T @ v
+    ea_o = _rel(o.apply(v, False), w)
+    es_o = _rel(o.apply(w, True), v)
+    assert float(vals["e_a"]) <= 2 * ea_o + 1e-14
+    assert float(vals["e_s"]) <= 2 * es_o + 1e-14
