@@ -34,6 +34,7 @@ EXPORTS = [
     "hifde_box_count", "hifde_box_sizes", "hifde_box_data", "hifde_stage_times",
     "hifde_stage_name", "hifde_factor_bytes", "hifde_interp_decomp_batch", "hifde_schur_batch",
     "hifde_set_profiling", "hifde_reset_kernel_stats", "hifde_kernel_stats", "hifde_set_option",
+    "hifde_gemm_batch",
 ]
 
 
@@ -78,6 +79,7 @@ def lib():
     L.hifde_reset_kernel_stats.argtypes = [_vp]
     L.hifde_kernel_stats.argtypes = [_vp, _i, ctypes.c_char_p, _i, _dp]
     L.hifde_interp_decomp_batch.argtypes = [_i, _ip, _ip, _dp, _d, _ip, _ip, _dp]
+    L.hifde_gemm_batch.argtypes = [_i, _i, _i, _ip, _ip, _ip, _dp, _dp, _dp]
     L.hifde_schur_batch.argtypes = [_i, _i, _ip, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _ip]
     _lib = L
     return L
@@ -271,6 +273,28 @@ def interp_decomp_batch(mats, eps):
         out.append((p[:kb].copy(), p[kb:].copy(), Wb.copy()))
         po += nb
         wo += nb * nb
+    return out
+
+
+def gemm_batch(pairs, precision="fp64", transA=False):
+    """[op(A) @ B for (A, B) in pairs] through the library's batched GEMM."""
+    prec = {"fp64": FP64, "mixed": MIXED}[precision]
+    m = np.array([(a.shape[1] if transA else a.shape[0]) for a, _ in pairs], np.int32)
+    k = np.array([(a.shape[0] if transA else a.shape[1]) for a, _ in pairs], np.int32)
+    n = np.array([b.shape[1] for _, b in pairs], np.int32)
+    A = np.concatenate([np.asfortranarray(a, np.float64).ravel(order="F") for a, _ in pairs]
+                       + [np.zeros(1)])
+    B = np.concatenate([np.asfortranarray(b, np.float64).ravel(order="F") for _, b in pairs]
+                       + [np.zeros(1)])
+    C = np.zeros(max(int((m.astype(np.int64) * n).sum()), 1))
+    _check(lib().hifde_gemm_batch(prec, int(transA), len(pairs), _ptr(m, ctypes.c_int),
+                                  _ptr(n, ctypes.c_int), _ptr(k, ctypes.c_int), _ptr(A), _ptr(B),
+                                  _ptr(C)))
+    out, o = [], 0
+    for b in range(len(pairs)):
+        cnt = int(m[b]) * int(n[b])
+        out.append(C[o:o + cnt].reshape(int(m[b]), int(n[b]), order="F").copy())
+        o += cnt
     return out
 
 

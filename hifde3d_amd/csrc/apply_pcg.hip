@@ -83,8 +83,17 @@ __global__ void __launch_bounds__(256) k_apply_tiles(
         const int i = tid & 63, g = tid >> 6;
         double pacc = 0.0;
         if (i < cnt) {
+            // eight independent sums keep eight loads per thread in flight
             const T* Mi = M + o0 + i;
-            for (int j = g; j < nin; j += 4) pacc += (double)Mi[(size_t)j * nout] * xs[j];
+            double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            int j = g;
+            for (; j + 28 < nin; j += 32) {
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    a[u] += (double)Mi[(size_t)(j + 4 * u) * nout] * xs[j + 4 * u];
+            }
+            for (; j < nin; j += 4) a[0] += (double)Mi[(size_t)j * nout] * xs[j];
+            pacc = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
         }
         part[g][i] = pacc;
         __syncthreads();
@@ -104,9 +113,14 @@ __global__ void __launch_bounds__(256) k_apply_tiles(
         const int lane = tid & (kWave - 1), w = tid / kWave;
         for (int jj = w; jj < cnt; jj += 4) {
             const T* Mj = M + (size_t)(o0 + jj) * nin;
-            double a = 0.0;
-            for (int i = lane; i < nin; i += kWave) a += (double)Mj[i] * xs[i];
-            a = wave_sum(a);
+            double p4[4] = {0, 0, 0, 0};
+            int i = lane;
+            for (; i + 3 * kWave < nin; i += 4 * kWave) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) p4[u] += (double)Mj[i + u * kWave] * xs[i + u * kWave];
+            }
+            for (; i < nin; i += kWave) p4[0] += (double)Mj[i] * xs[i];
+            double a = wave_sum((p4[0] + p4[1]) + (p4[2] + p4[3]));
             if (lane == 0) part[0][jj] = a;
         }
         __syncthreads();

@@ -475,6 +475,141 @@ __global__ void __launch_bounds__(256) k_gemm(int nbox, const int* __restrict__ 
     }
 }
 
+// MFMA variant of k_gemm for the larger boxes.  Four waves in a 2x2 grid;
+// each wave owns a (2 MT) x (2 MT) block of 2x2 MFMA tiles.
+//   float : v_mfma_f32_32x32x2f32, MT = 32, workgroup tile 128 x 128
+//   double: v_mfma_f64_16x16x4f64, MT = 16, workgroup tile  64 x  64
+// Operand layout (lane l of a wave): A element (row l % MT, k l / MT),
+// B element (k l / MT, col l % MT).  Accumulator element i of lane l is
+// (row, col) = (8 (i / 4) + 4 (l / 32) + i % 4, l % 32) for 32x32x2 and
+// (4 i + l / 16, l % 16) for the f64 16x16x4.
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+template <class T>
+struct MfmaCfg;
+template <>
+struct MfmaCfg<float> {
+    static constexpr int MT = 32, KT = 2, NACC = 16;
+    typedef floatx16 acc_t;
+    __device__ static acc_t mma(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
+    }
+    __device__ static void pos(int i, int lane, int& r, int& c) {
+        r = 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+        c = lane & 31;
+    }
+};
+template <>
+struct MfmaCfg<double> {
+    static constexpr int MT = 16, KT = 4, NACC = 4;
+    typedef doublex4 acc_t;
+    __device__ static acc_t mma(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    __device__ static void pos(int i, int lane, int& r, int& c) {
+        r = 4 * i + (lane >> 4);
+        c = lane & 15;
+    }
+};
+
+template <class T, bool TRANSA>
+__global__ void __launch_bounds__(256) k_gemm_mfma(int nbox, const int* __restrict__ tile_off,
+                                                   const GemmDesc* __restrict__ desc,
+                                                   const T* __restrict__ Ab,
+                                                   const T* __restrict__ Bb, T* Cb, T alpha,
+                                                   T beta) {
+    typedef MfmaCfg<T> C;
+    constexpr int MT = C::MT, KT = C::KT, WT = 2 * MT, TILE = 2 * WT, KB = 16;
+    const int t = blockIdx.x;
+    const int b = find_box(tile_off, nbox, t);
+    const int local = t - tile_off[b];
+    const GemmDesc dd = desc[b];
+    const int m = dd.m, n = dd.n, k = dd.k;
+    const int tm = (m + TILE - 1) / TILE;
+    const int i0 = (local % tm) * TILE, j0 = (local / tm) * TILE;
+    const T* A = Ab + dd.offA;
+    const T* B = Bb + dd.offB;
+    T* Cm = Cb + dd.offC;
+    __shared__ T As[KB][TILE + 4];
+    __shared__ T Bs[KB][TILE + 4];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = (wave & 1) * WT, wc = (wave >> 1) * WT;
+    typename C::acc_t acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int i = 0; i < C::NACC; ++i) acc[a][c][i] = (T)0;
+    for (int k0 = 0; k0 < k; k0 += KB) {
+        for (int e = tid; e < TILE * KB; e += 256) {
+            if (!TRANSA) {
+                const int r = e % TILE, c = e / TILE;
+                const int gi = i0 + r, gk = k0 + c;
+                As[c][r] = (gi < m && gk < k) ? A[gi + (size_t)gk * dd.lda] : (T)0;
+            } else {
+                const int c = e % KB, r = e / KB;
+                const int gi = i0 + r, gk = k0 + c;
+                As[c][r] = (gi < m && gk < k) ? A[gk + (size_t)gi * dd.lda] : (T)0;
+            }
+            const int cb = e % KB, jb = e / KB;
+            const int gkb = k0 + cb, gj = j0 + jb;
+            Bs[cb][jb] = (gkb < k && gj < n) ? B[gkb + (size_t)gj * dd.ldb] : (T)0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < KB; kk += KT) {
+            const int kr = kk + lane / MT, rr = lane % MT;
+            T av[2], bv[2];
+#pragma unroll
+            for (int a = 0; a < 2; ++a) av[a] = As[kr][wr + a * MT + rr];
+#pragma unroll
+            for (int c = 0; c < 2; ++c) bv[c] = Bs[kr][wc + c * MT + rr];
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int c = 0; c < 2; ++c) acc[a][c] = C::mma(av[a], bv[c], acc[a][c]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int i = 0; i < C::NACC; ++i) {
+                int r, cc;
+                C::pos(i, lane, r, cc);
+                const int gi = i0 + wr + a * MT + r, gj = j0 + wc + c * MT + cc;
+                if (gi < m && gj < n) {
+                    T* e = Cm + gi + (size_t)gj * dd.ldc;
+                    const T v = alpha * acc[a][c][i];
+                    *e = (beta == (T)0) ? v : beta * *e + v;
+                }
+            }
+}
+
+// A(K, j) = R(:, j) for the columns j outside the panel K (block GJ).
+template <class T>
+__global__ void __launch_bounds__(256) k_bgj_rowcopy(const int* __restrict__ dims,
+                                                     const long long* __restrict__ off, T* mats,
+                                                     const T* __restrict__ Rws,
+                                                     const long long* __restrict__ r_off,
+                                                     int step, const int* __restrict__ info) {
+    const int b = blockIdx.y;
+    const int n = dims[b];
+    const int k0 = step * kGJB;
+    if (n <= kSmallGJ || k0 >= n || info[b]) return;
+    const int kb = min(kGJB, n - k0);
+    for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < (long long)kb * n;
+         e += (long long)gridDim.x * 256) {
+        const int j = (int)(e / kb), c = (int)(e - (long long)j * kb);
+        if (j >= k0 && j < k0 + kb) continue;
+        mats[off[b] + k0 + c + (size_t)j * n] = Rws[r_off[b] + c + (size_t)j * kGJB];
+    }
+}
+
 // dst(box) = (D) src(rows r0.., cols c0.. of a ld-strided block); dst packed
 // column-major with ld = rows.
 template <class D>
@@ -840,8 +975,9 @@ int grid_for(long long n, int block = 256, int cap = 1 << 16) {
 }  // namespace
 
 template <class T>
-void batched_gj_inverse(int nbox, const int* h_n, const int* d_n, const long long* d_off, T* d_A,
-                        int* d_info, GJWork& ws, hipStream_t s) {
+void batched_gj_inverse(int nbox, const int* h_n, const long long* h_off, const int* d_n,
+                        const long long* d_off, T* d_A, int* d_info, GJWork& ws, GemmWork& gws,
+                        hipStream_t s) {
     if (nbox <= 0) return;
     int max_n = 0;
     bool any_small = false;
@@ -872,8 +1008,23 @@ void batched_gj_inverse(int nbox, const int* h_n, const int* d_n, const long lon
     for (int st = 0; st < steps; ++st) {
         k_bgj_diag<T><<<nbox, 256, 0, s>>>(d_n, d_off, d_A, Dws, st, d_info);
         k_bgj_row<T><<<gch, 256, 0, s>>>(d_n, d_off, d_A, Dws, Rws, ws.b_off.p, st, d_info);
-        k_bgj_update<T><<<tile_off[nbox], 256, 0, s>>>(nbox, ws.tile_off.p, d_n, d_off, d_A, Rws,
-                                                       ws.b_off.p, st, d_info);
+        // A(:, J) -= A(:, K) R(:, J) for J left and right of the panel, then
+        // the panel rows A(K, J) = R(:, J).
+        std::vector<GemmDesc> upd;
+        for (int b = 0; b < nbox; ++b) {
+            const int n = h_n[b], k0 = st * kGJB;
+            if (n <= kSmallGJ || k0 >= n) continue;
+            const int kb = std::min(kGJB, n - k0);
+            const long long o = h_off[b], ro = r_off[b];
+            if (k0 > 0) upd.push_back({o + (long long)k0 * n, ro, o, n, k0, kb, n, kGJB, n});
+            const int j1 = k0 + kb;
+            if (j1 < n)
+                upd.push_back({o + (long long)k0 * n, ro + (long long)j1 * kGJB,
+                               o + (long long)j1 * n, n, n - j1, kb, n, kGJB, n});
+        }
+        batched_gemm_desc<T>(false, upd, d_A, Rws, d_A, (T)-1, (T)1, gws, s);
+        k_bgj_rowcopy<T><<<dim3((kGJB * max_n + 255) / 256, nbox), 256, 0, s>>>(
+            d_n, d_off, d_A, Rws, ws.b_off.p, st, d_info);
         k_bgj_col<T><<<gch, 256, 0, s>>>(d_n, d_off, d_A, Dws, st, d_info);
     }
     HIFDE_LAUNCH_CHECK();
@@ -882,27 +1033,44 @@ void batched_gj_inverse(int nbox, const int* h_n, const int* d_n, const long lon
 template <class T>
 void batched_gemm_desc(bool transA, const std::vector<GemmDesc>& desc, const T* d_A, const T* d_B,
                        T* d_C, T alpha, T beta, GemmWork& ws, hipStream_t s) {
-    const int nbox = (int)desc.size();
-    if (nbox <= 0) return;
-    std::vector<int> toff(nbox + 1, 0);
-    for (int b = 0; b < nbox; ++b) {
-        const GemmDesc& d = desc[b];
-        long long tiles = (d.m > 0 && d.n > 0) ? (long long)((d.m + BM - 1) / BM) *
-                                                     ((d.n + BN - 1) / BN)
-                                               : 0;
-        toff[b + 1] = toff[b] + (int)tiles;
+    if (desc.empty()) return;
+    // Boxes whose output spans at least half an MFMA workgroup tile in both
+    // directions go to the MFMA kernel, the rest to the 64x64 FMA kernel.
+    constexpr int TILE = 4 * MfmaCfg<T>::MT;
+    std::vector<GemmDesc> small, large;
+    for (const GemmDesc& d : desc) {
+        if (d.m <= 0 || d.n <= 0) continue;
+        if (d.m >= TILE / 2 && d.n >= TILE / 2) large.push_back(d);
+        else small.push_back(d);
     }
-    const int total = toff[nbox];
-    if (total == 0) return;
-    ws.tiles.upload(toff, s);
-    ws.desc.upload(desc, s);
-    if (transA)
-        k_gemm<T, true><<<total, 256, 0, s>>>(nbox, ws.tiles.p, ws.desc.p, d_A, d_B, d_C, alpha,
-                                              beta);
-    else
-        k_gemm<T, false><<<total, 256, 0, s>>>(nbox, ws.tiles.p, ws.desc.p, d_A, d_B, d_C, alpha,
-                                               beta);
-    HIFDE_LAUNCH_CHECK();
+    for (int pass = 0; pass < 2; ++pass) {
+        const std::vector<GemmDesc>& dv = pass == 0 ? small : large;
+        const int nb = (int)dv.size();
+        if (nb == 0) continue;
+        const int tile = pass == 0 ? BM : TILE;
+        std::vector<int> toff(nb + 1, 0);
+        for (int b = 0; b < nb; ++b)
+            toff[b + 1] = toff[b] + ((dv[b].m + tile - 1) / tile) * ((dv[b].n + tile - 1) / tile);
+        DevBuf<int>& tb = pass == 0 ? ws.tiles : ws.tiles2;
+        DevBuf<GemmDesc>& db = pass == 0 ? ws.desc : ws.desc2;
+        tb.upload(toff, s);
+        db.upload(dv, s);
+        const int total = toff[nb];
+        if (pass == 0) {
+            if (transA)
+                k_gemm<T, true><<<total, 256, 0, s>>>(nb, tb.p, db.p, d_A, d_B, d_C, alpha, beta);
+            else
+                k_gemm<T, false><<<total, 256, 0, s>>>(nb, tb.p, db.p, d_A, d_B, d_C, alpha, beta);
+        } else {
+            if (transA)
+                k_gemm_mfma<T, true><<<total, 256, 0, s>>>(nb, tb.p, db.p, d_A, d_B, d_C, alpha,
+                                                           beta);
+            else
+                k_gemm_mfma<T, false><<<total, 256, 0, s>>>(nb, tb.p, db.p, d_A, d_B, d_C, alpha,
+                                                            beta);
+        }
+        HIFDE_LAUNCH_CHECK();
+    }
 }
 
 template <class T>
@@ -1003,10 +1171,12 @@ void convert_d2f(const double* src, float* dst, long long n, hipStream_t s) {
     HIFDE_LAUNCH_CHECK();
 }
 
-template void batched_gj_inverse<double>(int, const int*, const int*, const long long*, double*,
-                                         int*, GJWork&, hipStream_t);
-template void batched_gj_inverse<float>(int, const int*, const int*, const long long*, float*, int*,
-                                        GJWork&, hipStream_t);
+template void batched_gj_inverse<double>(int, const int*, const long long*, const int*,
+                                         const long long*, double*, int*, GJWork&, GemmWork&,
+                                         hipStream_t);
+template void batched_gj_inverse<float>(int, const int*, const long long*, const int*,
+                                        const long long*, float*, int*, GJWork&, GemmWork&,
+                                        hipStream_t);
 
 template void batched_gemm_desc<double>(bool, const std::vector<GemmDesc>&, const double*,
                                         const double*, double*, double, double, GemmWork&,

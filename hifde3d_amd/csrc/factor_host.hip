@@ -429,8 +429,8 @@ struct hifde_factor {
         }
         {
             Prof pr(this, fam<T>("gj_inverse"), fl_inv, by_inv);
-            batched_gj_inverse<T>(nb, B.r.data(), B.d_r.p, B.d_off_rr.p, Ainv, ws_info.p, gj_ws,
-                                  stream);
+            batched_gj_inverse<T>(nb, B.r.data(), B.off_rr.data(), B.d_r.p, B.d_off_rr.p, Ainv,
+                                  ws_info.p, gj_ws, gemm_ws, stream);
         }
         timed(ST_INVERSE, t0);
         std::vector<int> info(nb);
@@ -1425,8 +1425,8 @@ int schur_batch_impl(int nb, const int* r, const int* s, const double* A22, cons
     dinfo.reserve(nb);
     HIFDE_CHECK(hipMemset(dinfo.p, 0, nb * sizeof(int)));
     GJWork gw;
-    batched_gj_inverse<T>(nb, hr.data(), dr.p, drr.p, dA.p, dinfo.p, gw, st);
     GemmWork gws;
+    batched_gj_inverse<T>(nb, hr.data(), orr.data(), dr.p, drr.p, dA.p, dinfo.p, gw, gws, st);
     batched_gemm<T>(false, nb, hr.data(), hs.data(), hr.data(), orr.data(), ors.data(), ors.data(),
                     dA.p, dA21.p, dX.p, (T)1, gws, st);
     batched_gemm<T>(false, nb, hs.data(), hs.data(), hr.data(), ors.data(), ors.data(), oss.data(),
@@ -1444,7 +1444,49 @@ int schur_batch_impl(int nb, const int* r, const int* s, const double* A22, cons
 }
 }  // namespace
 
+namespace {
+template <class T>
+int gemm_batch_impl(int transA, int nb, const int* m, const int* n, const int* k, const double* A,
+                    const double* B, double* C) {
+    hipStream_t st = nullptr;
+    std::vector<long long> oA(nb + 1, 0), oB(nb + 1, 0), oC(nb + 1, 0);
+    for (int b = 0; b < nb; ++b) {
+        if (m[b] < 0 || n[b] < 0 || k[b] < 0) throw Error(HIFDE_ERR_ARG, "negative size");
+        oA[b + 1] = oA[b] + (long long)m[b] * k[b];
+        oB[b + 1] = oB[b] + (long long)k[b] * n[b];
+        oC[b + 1] = oC[b] + (long long)m[b] * n[b];
+    }
+    auto up = [&](const double* src, long long cnt, DevBuf<T>& d) {
+        std::vector<T> tmp(src, src + cnt);
+        d.upload(tmp, st);
+    };
+    DevBuf<T> dA, dB, dC;
+    up(A, oA[nb], dA);
+    up(B, oB[nb], dB);
+    dC.reserve(oC[nb]);
+    GemmWork gws;
+    batched_gemm<T>(transA != 0, nb, m, n, k, oA.data(), oB.data(), oC.data(), dA.p, dB.p, dC.p,
+                    (T)1, gws, st);
+    std::vector<T> tmp(oC[nb]);
+    if (oC[nb])
+        HIFDE_CHECK(hipMemcpy(tmp.data(), dC.p, oC[nb] * sizeof(T), hipMemcpyDeviceToHost));
+    for (long long i = 0; i < oC[nb]; ++i) C[i] = (double)tmp[i];
+    return HIFDE_OK;
+}
+}  // namespace
+
 extern "C" {
+
+int hifde_gemm_batch(int precision, int transA, int nbatch, const int* m, const int* n,
+                     const int* k, const double* A, const double* B, double* C) {
+    return guarded([&]() -> int {
+        if (nbatch < 0 || (nbatch && (!m || !n || !k || !A || !B || !C)))
+            throw Error(HIFDE_ERR_ARG, "gemm_batch: bad arguments");
+        if (nbatch == 0) return HIFDE_OK;
+        if (precision == HIFDE_MIXED) return gemm_batch_impl<float>(transA, nbatch, m, n, k, A, B, C);
+        return gemm_batch_impl<double>(transA, nbatch, m, n, k, A, B, C);
+    });
+}
 
 int hifde_schur_batch(int precision, int nbatch, const int* r, const int* s, const double* A22,
                       const double* A21, const double* A12, double* A22inv, double* X, double* S,

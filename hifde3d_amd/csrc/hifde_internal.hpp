@@ -130,9 +130,7 @@ struct GJWork {
     DevBuf<long long> b_off;
     DevBuf<unsigned char> B, D;
 };
-template <class T>
-void batched_gj_inverse(int nbox, const int* h_n, const int* d_n, const long long* d_off, T* d_A,
-                        int* d_info, GJWork& ws, hipStream_t s);
+
 struct GemmDesc {
     long long offA, offB, offC;
     int m, n, k, lda, ldb, ldc;
@@ -142,8 +140,8 @@ struct CopyDesc {
     int r0, c0, rows, ld;
 };
 struct GemmWork {
-    DevBuf<int> tiles;
-    DevBuf<GemmDesc> desc;
+    DevBuf<int> tiles, tiles2;
+    DevBuf<GemmDesc> desc, desc2;
     DevBuf<CopyDesc> cdesc;
     DevBuf<long long> off;
 };
@@ -159,6 +157,10 @@ void batched_gemm(bool transA, int nbox, const int* h_m, const int* h_n, const i
 template <class D>
 void copy_blocks(const std::vector<CopyDesc>& desc, const std::vector<long long>& dst_off,
                  const double* d_src, D* d_dst, GemmWork& ws, hipStream_t s);
+template <class T>
+void batched_gj_inverse(int nbox, const int* h_n, const long long* h_off, const int* d_n,
+                        const long long* d_off, T* d_A, int* d_info, GJWork& ws, GemmWork& gws,
+                        hipStream_t s);
 void permute_sym(int nbox, const long long* d_off, const int* d_n, const int* d_perm,
                  const int* d_perm_off, const double* d_C, double* d_Cp, long long total,
                  hipStream_t s);

@@ -143,6 +143,15 @@ int hifde_schur_batch(int precision, int nbatch, const int* r, const int* s, con
                       const double* A21, const double* A12, double* A22inv, double* X, double* S,
                       int* info);
 
+/* hmat_tools::Multiply(alpha=1, A, B, C) -- src/hmat_tools/MultiplyMatrix.cpp:15-69
+ * (C := A B) and AdjointMultiply (C := A^T B, AdjointMultiplyMatrix.cpp:15-96)
+ * over a batch: C_b (m x n) = op(A_b) B_b, op(A) = A (m x k) or A^T (A stored
+ * k x m) when transA != 0.  Operands packed one after another, column-major.
+ * precision selects FP64 or FP32 arithmetic (the kernels the factorization's
+ * Schur and Gram products use). */
+int hifde_gemm_batch(int precision, int transA, int nbatch, const int* m, const int* n,
+                     const int* k, const double* A, const double* B, double* C);
+
 #ifdef __cplusplus
 }
 #endif

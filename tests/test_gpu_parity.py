@@ -105,12 +105,14 @@ def _coo(oracle_mod, N, coeff, sym, seed=1):
     return a, V, (r, c, v), sp.csr_matrix((v, (r, c)), shape=(NC ** 3, NC ** 3))
 
 
-def _both(H, oracle_mod, N, P, eps, coeff="ones", sym=False, precision="fp64"):
+def _both(H, oracle_mod, N, P, eps, coeff="ones", sym=False, precision="fp64", options=None):
     a, V, coo, A = _coo(oracle_mod, N, coeff, sym)
     o = oracle_mod.Oracle(N, P, eps)
     o.set_matrix_coo(*coo)
     assert o.factor(4) == 0
     f = H.HIFFactor(N, P, eps, precision=precision)
+    for k, v in (options or {}).items():
+        f.set_option(k, v)
     f.set_stencil(a, V, sym)
     assert f.Factor() == 0
     return f, o, A
@@ -143,7 +145,9 @@ def test_c1_reference_protocol_matches_oracle(H, oracle_mod):
 
 
 def test_c1_cell_boxes_match_oracle(H, oracle_mod):
-    f, o, _ = _both(H, oracle_mod, 15, 4, 1e-6)
+    # Entrywise comparison needs the Householder ID (the oracle's method) on
+    # every face: the Gram path's W agrees only to eps_mach cond(R11)^2.
+    f, o, _ = _both(H, oracle_mod, 15, 4, 1e-6, options={"gram_min_n": 1 << 30})
     for lv in range(f.num_levels):
         assert f.num_boxes(lv, 0) == o.num_boxes(lv, 0)
         for b in range(f.num_boxes(lv, 0)):
@@ -152,9 +156,15 @@ def test_c1_cell_boxes_match_oracle(H, oracle_mod):
             assert np.array_equal(g["red_gid"], r["g"][r["red"]])
             assert np.array_equal(g["skel_gid"], r["g"][r["skel"]])
             if r["A22inv"].size:
-                assert _rel(g["A22inv"], r["A22inv"]) < 1e-10
-            if r["X"].size:
-                assert _rel(g["X"], r["X"]) < 1e-10
+                # Level 0 gathers the stencil itself; upper-level blocks are sums
+                # of Schur updates with cancellation, so the two codes' summation
+                # orders differ by rounding that A22^{-1} amplifies by cond(A22).
+                d22 = _rel(g["A22"], r["A22"])
+                assert d22 < (1e-15 if lv == 0 else 1e-8)
+                bound = 10 * np.linalg.cond(r["A22"]) * max(d22, 1e-16) + 1e-12
+                assert _rel(g["A22inv"], r["A22inv"]) < bound
+                if r["X"].size:
+                    assert _rel(g["X"], r["X"]) < bound
 
 
 def test_c1_face_boxes_match_oracle(H, oracle_mod):
@@ -259,3 +269,56 @@ def test_cpp_dropin_driver_reference_protocol(H, oracle_mod):
     es_o = _rel(o.apply(w, True), v)
     assert float(vals["e_a"]) <= 2 * ea_o + 1e-14
     assert float(vals["e_s"]) <= 2 * es_o + 1e-14
+
+
+def _golden(name):
+    import json
+    import os
+    here = os.path.dirname(os.path.abspath(__file__))
+    return json.load(open(os.path.join(here, "golden", name)))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_c2_n64_against_oracle_golden(H, precision):
+    """n=64, P=4, eps=1e-6, a = 10^U[-1,1] (seed 1): counts within 5% and
+    e_a, e_s, ||A F^{-1} b - b|| / ||b|| within 2x of the oracle's fixture."""
+    g = _golden("c2_n64.json")
+    n, P, eps = g["n"], g["P"], g["eps"]
+    N = n - 1
+    a = pb.pow10_coeff(n, g["coeff_seed"])
+    V = pb.zeros_coeff(n)
+    f = H.HIFFactor(N, P, eps, precision=precision)
+    f.set_stencil(a, V, True)
+    assert f.Factor() == 0
+    cg = f.level_counts()
+    for lv, co in enumerate(g["level_counts"]):
+        assert cg[lv][0] == co[0] and cg[lv][2] == co[2]
+        for idx in (1, 3):
+            assert abs(cg[lv][idx] - co[idx]) <= 0.05 * max(co[idx], 20)
+    v = pb.reference_test_vector(n)
+    w = f.spmv(v, transpose=True)
+    ea = _rel(f.Apply(v, False), w)
+    es = _rel(f.Apply(w, True), v)
+    b = pb.rhs_normal(n, g["rhs_seed"])
+    res = _rel(f.spmv(f.Apply(b, True)), b)
+    slack = 2.0 if precision == "fp64" else 4.0
+    assert ea <= slack * g["e_a"] and es <= slack * g["e_s"]
+    assert res <= slack * g["precond_residual"]
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 1e-5)])
+@pytest.mark.parametrize("transA", [False, True])
+def test_gemm_batch_matches_numpy(H, precision, tol, transA):
+    """Both GEMM kernels (64x64 FMA tiles for small boxes, MFMA tiles for large
+    ones: f32 32x32x2 and f64 16x16x4 accumulator layouts) against numpy."""
+    rng = _rng(4)
+    shapes = [(27, 54, 27), (54, 54, 27), (130, 70, 33), (200, 260, 64), (64, 64, 1), (1, 300, 5),
+              (129, 131, 17), (33, 40, 200)]
+    pairs = []
+    for m, n, k in shapes:
+        A = rng.standard_normal((k, m) if transA else (m, k))
+        pairs.append((A, rng.standard_normal((k, n))))
+    got = H.gemm_batch(pairs, precision, transA)
+    for (A, B), C in zip(pairs, got):
+        ref = (A.T if transA else A) @ B
+        assert _rel(C, ref) < tol

@@ -301,3 +301,16 @@ def test_normal_and_field_statistics():
     # neighbouring nodes (distance h = 1/32 << 0.1) are strongly correlated
     c = np.corrcoef(g[:-1].ravel(), g[1:].ravel())[0, 1]
     assert 0.8 < c < 1.0
+
+
+def test_golden_fixture_consistent():
+    import json
+    import os
+    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "c2_n64.json")))
+    assert g["info"] == 0
+    N = g["n"] - 1
+    assert sum(c[1] + c[3] for c in g["level_counts"]) == N ** 3
+    for lv, c in enumerate(g["level_counts"]):
+        cells = g["n"] // (g["P"] * 2 ** lv)
+        assert c[0] == cells ** 3
+    assert 0 < g["e_a"] < 1e-4 and 0 < g["e_s"] < 1e-4
