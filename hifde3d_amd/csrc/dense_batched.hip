@@ -418,6 +418,7 @@ __global__ void __launch_bounds__(256) k_gemm(int nbox, const int* __restrict__ 
     const int m = dd.m, n = dd.n, k = dd.k;
     const int tm = (m + BM - 1) / BM;
     const int i0 = (local % tm) * BM, j0 = (local / tm) * BN;
+    if (dd.lower && j0 > i0) return;
     const T* A = Ab + dd.offA;
     const T* B = Bb + dd.offB;
     T* C = Cb + dd.offC;
@@ -528,6 +529,7 @@ __global__ void __launch_bounds__(256) k_gemm_mfma(int nbox, const int* __restri
     const int m = dd.m, n = dd.n, k = dd.k;
     const int tm = (m + TILE - 1) / TILE;
     const int i0 = (local % tm) * TILE, j0 = (local / tm) * TILE;
+    if (dd.lower && j0 > i0) return;
     const T* A = Ab + dd.offA;
     const T* B = Bb + dd.offB;
     T* Cm = Cb + dd.offC;

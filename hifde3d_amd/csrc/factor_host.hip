@@ -74,6 +74,9 @@ struct hifde_factor {
     int cur = 0;
     DevBuf<unsigned long long> tab_counter;
     size_t tab_inserted = 0;  // upper bound of the entries in tabs[cur]
+    bool op_symmetric = false;  // the assembled operator equals its transpose bitwise
+    int sym_option = -1;        // -1 auto, 0 off
+    bool sym_on = false;        // symmetric fast path of the current factorization
     LevelTable& table_ref() { return tabs[cur]; }
     bool have_matrix = false, factored = false;
     // operator kept for SpMV / PCG (CSR)
@@ -442,7 +445,20 @@ struct hifde_factor {
                             B.off_rs.data(), B.off_rs.data(), Ainv, A21, X, (T)1, gemm_ws,
                             stream);
         }
-        {
+        if (sym_on) {
+            // S = -A21^T X, symmetric: only the tiles on or below the diagonal
+            std::vector<GemmDesc> d;
+            for (int b = 0; b < nb; ++b) {
+                // r == 0 still writes S = 0 (the reference's k == 0 GEMM)
+                if (B.s[b] == 0) continue;
+                GemmDesc g{B.off_rs[b], B.off_rs[b], B.off_ss[b], B.s[b], B.s[b], B.r[b],
+                           B.r[b], B.r[b], B.s[b]};
+                g.lower = 1;
+                d.push_back(g);
+            }
+            Prof pr(this, fam<T>("gemm_s"), fl_s / 2, by_s);
+            batched_gemm_desc<T>(true, d, A21, X, S, (T)-1, (T)0, gemm_ws, stream);
+        } else {
             Prof pr(this, fam<T>("gemm_s"), fl_s, by_s);
             batched_gemm<T>(false, nb, B.s.data(), B.s.data(), B.r.data(), B.off_rs.data(),
                             B.off_rs.data(), B.off_ss.data(), A12, X, S, (T)-1, gemm_ws, stream);
@@ -476,13 +492,15 @@ struct hifde_factor {
         // Entries of eliminated DOFs are never looked up again, so they may
         // stay in the table; it is compacted only when the coming inserts
         // would push the load factor above one half.
-        if ((tab_inserted + (size_t)B.tot_ss) * 2 > tabs[cur].cap) {
+        const long long skel_tot = B.skel_off.empty() ? 0 : B.skel_off.back();
+        const size_t adds = sym_on ? (size_t)((B.tot_ss + skel_tot) / 2) : (size_t)B.tot_ss;
+        if ((tab_inserted + adds) * 2 > tabs[cur].cap) {
             Prof pr(this, "compact", 0.0, (double)tabs[cur].cap * 16 * 2);
-            tab_inserted = table_compact(tabs[cur], tabs[1 - cur], d_alive.p, (size_t)B.tot_ss,
-                                         tab_counter, stream);
+            tab_inserted = table_compact(tabs[cur], tabs[1 - cur], d_alive.p, adds, tab_counter,
+                                         stream);
             cur = 1 - cur;
         }
-        tab_inserted += (size_t)B.tot_ss;
+        tab_inserted += adds;
         timed(ST_COMPACT, t0);
         // One profiling record per launch: algorithmic bytes are the S values
         // read plus one 16-byte slot read and written per entry added.
@@ -532,7 +550,7 @@ struct hifde_factor {
             table_gather<T>(tabs[cur], B.nbox, B.d_red_gid.p, B.d_red_off.p, B.d_skel_gid.p,
                             B.d_skel_off.p, B.d_off_rs.p, B.tot_rs, A21, stream);
         }
-        {
+        if (!sym_on) {  // A12 = A21^T exactly when the level matrix is stored symmetric
             Prof pr(this, fam<T>("gather"), 0.0, (double)B.tot_rs * (sizeof(T) + 16));
             table_gather<T>(tabs[cur], B.nbox, B.d_skel_gid.p, B.d_skel_off.p, B.d_red_gid.p,
                             B.d_red_off.p, B.d_off_rs.p, B.tot_rs, A12, stream);
@@ -738,7 +756,7 @@ struct hifde_factor {
         batched_gemm_desc<double>(true, d4, ws_W.p, ws_T.p, ws_T.p, -1.0, 1.0, gemm_ws, stream);
         copy_blocks<T>(c22, B.off_rr, ws_T.p, reinterpret_cast<T*>(B.A22.p), gemm_ws, stream);
         copy_blocks<T>(c21, B.off_rs, ws_T.p, A21, gemm_ws, stream);
-        copy_blocks<T>(c12, B.off_rs, ws_T.p, A12, gemm_ws, stream);
+        if (!sym_on) copy_blocks<T>(c12, B.off_rs, ws_T.p, A12, gemm_ws, stream);
     }
 
     template <class T>
@@ -776,7 +794,9 @@ struct hifde_factor {
             auto t0 = clk::now();
             long long nnz = (long long)h_ci.size();
             cur = 0;
-            table_init(tabs[cur], (size_t)nnz, stream);
+            sym_on = op_symmetric && sym_option != 0;
+            tabs[0].sym = tabs[1].sym = sym_on;
+            table_init(tabs[cur], (size_t)(sym_on ? (nnz + NC * NC * NC) / 2 : nnz), stream);
             Prof pr(this, "table_insert", 0.0, (double)nnz * (4 + 4 + 8 + 16));
             table_insert_coo(tabs[cur], d_rr.p, d_ci.p, d_v.p, nnz, stream);
             tab_inserted = (size_t)nnz;
@@ -1007,6 +1027,19 @@ struct hifde_factor {
             }
             h_rp[i + 1] = (int)h_ci.size();
         }
+        // bitwise symmetry of the assembled operator
+        op_symmetric = true;
+        for (long long i = 0; i < nd && op_symmetric; ++i)
+            for (int p = h_rp[i]; p < h_rp[i + 1] && op_symmetric; ++p) {
+                const int j = h_ci[p];
+                bool found = false;
+                for (int q = h_rp[j]; q < h_rp[j + 1]; ++q)
+                    if (h_ci[q] == (int)i) {
+                        found = h_v[q] == h_v[p];
+                        break;
+                    }
+                op_symmetric = found;
+            }
         d_rp.upload(h_rp, stream);
         d_ci.upload(h_ci, stream);
         d_v.upload(h_v, stream);
@@ -1303,6 +1336,7 @@ int hifde_set_option(hifde_factor* f, const char* name, double value) {
         if (n == "gram_min_n") f->gram_min_n = (int)value;
         else if (n == "gram_min_eps") f->gram_min_eps = value;
         else if (n == "fp32_levels") f->fp32_levels = (int)value;
+        else if (n == "symmetric") f->sym_option = (int)value;
         else throw Error(HIFDE_ERR_ARG, "set_option: unknown option " + n);
         return HIFDE_OK;
     });

@@ -66,7 +66,8 @@ constexpr unsigned long long kEmptyKey = ~0ull;
 // Device hash table holding the level matrix A(i, j) for live DOFs.
 struct LevelTable {
     DevBuf<Slot> slots;
-    size_t cap = 0;  // power of two
+    size_t cap = 0;    // power of two
+    bool sym = false;  // one entry per unordered pair, keyed (min, max)
 };
 
 // One batch of boxes of a half level (cells of level l, or faces of level l).
@@ -134,6 +135,7 @@ struct GJWork {
 struct GemmDesc {
     long long offA, offB, offC;
     int m, n, k, lda, ldb, ldc;
+    int lower = 0;  // 1: only output tiles on or below the block diagonal
 };
 struct CopyDesc {
     long long off;  // source block base

@@ -27,6 +27,11 @@ __device__ __forceinline__ unsigned long long make_key(int r, int c) {
     return ((unsigned long long)(unsigned)r << 32) | (unsigned)c;
 }
 
+// Symmetric storage keeps one entry per unordered pair under (min, max).
+__device__ __forceinline__ unsigned long long make_key_s(int r, int c, bool sym) {
+    return (sym && r > c) ? make_key(c, r) : make_key(r, c);
+}
+
 __device__ __forceinline__ double table_find(const Slot* __restrict__ t, unsigned long long mask,
                                              unsigned long long key) {
     unsigned long long h = hmix(key) & mask;
@@ -82,10 +87,13 @@ __global__ void k_table_clear(Slot* t, size_t cap) {
 
 __global__ void k_table_insert_coo(Slot* t, unsigned long long mask, const int* __restrict__ rows,
                                    const int* __restrict__ cols, const double* __restrict__ vals,
-                                   long long nnz) {
+                                   long long nnz, bool sym) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     long long stride = (long long)gridDim.x * blockDim.x;
-    for (; i < nnz; i += stride) table_add(t, mask, make_key(rows[i], cols[i]), vals[i]);
+    for (; i < nnz; i += stride) {
+        if (sym && rows[i] > cols[i]) continue;
+        table_add(t, mask, make_key(rows[i], cols[i]), vals[i]);
+    }
 }
 
 template <class T>
@@ -93,7 +101,7 @@ __global__ void k_table_gather(const Slot* __restrict__ t, unsigned long long ma
                                const int* __restrict__ rows, const int* __restrict__ row_off,
                                const int* __restrict__ cols, const int* __restrict__ col_off,
                                const long long* __restrict__ out_off, long long total,
-                               T* __restrict__ out) {
+                               T* __restrict__ out, bool sym) {
     long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     long long stride = (long long)gridDim.x * blockDim.x;
     for (; e < total; e += stride) {
@@ -102,7 +110,7 @@ __global__ void k_table_gather(const Slot* __restrict__ t, unsigned long long ma
         int m = row_off[b + 1] - row_off[b];
         int i = (int)(l % m), j = (int)(l / m);
         int r = rows[row_off[b] + i], c = cols[col_off[b] + j];
-        out[e] = (T)table_find(t, mask, make_key(r, c));
+        out[e] = (T)table_find(t, mask, make_key_s(r, c, sym));
     }
 }
 
@@ -112,7 +120,7 @@ __global__ void k_table_scatter(Slot* t, unsigned long long mask, int nbox,
                                 const int* __restrict__ skel_gid,
                                 const unsigned char* __restrict__ skel_pass,
                                 const long long* __restrict__ off_ss, long long total,
-                                const T* __restrict__ S, int pass) {
+                                const T* __restrict__ S, int pass, bool sym) {
     long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     long long stride = (long long)gridDim.x * blockDim.x;
     for (; e < total; e += stride) {
@@ -121,11 +129,12 @@ __global__ void k_table_scatter(Slot* t, unsigned long long mask, int nbox,
         int s = s_dim[b];
         int i = (int)(l % s), j = (int)(l / s);
         int so = skel_off[b];
+        if (sym && i < j) continue;  // one entry per unordered pair, from the lower triangle
         if (pass >= 0) {
             int p2 = (skel_pass[so + i] && skel_pass[so + j]) ? 1 : 0;
             if (p2 != pass) continue;
         }
-        table_add(t, mask, make_key(skel_gid[so + i], skel_gid[so + j]), (double)S[e]);
+        table_add(t, mask, make_key_s(skel_gid[so + i], skel_gid[so + j], sym), (double)S[e]);
     }
 }
 
@@ -190,9 +199,10 @@ void table_init(LevelTable& t, size_t min_entries, hipStream_t s) {
 
 void table_insert_coo(LevelTable& t, const int* d_rows, const int* d_cols, const double* d_vals,
                       long long nnz, hipStream_t s) {
+    const bool sym = t.sym;
     if (nnz <= 0) return;
     k_table_insert_coo<<<grid_for(nnz), 256, 0, s>>>(t.slots.p, t.cap - 1, d_rows, d_cols, d_vals,
-                                                     nnz);
+                                                     nnz, sym);
     HIFDE_LAUNCH_CHECK();
 }
 
@@ -202,7 +212,8 @@ void table_gather(const LevelTable& t, int nbox, const int* d_rows, const int* d
                   long long total, T* d_out, hipStream_t s) {
     if (total <= 0 || nbox <= 0) return;
     k_table_gather<T><<<grid_for(total, 256, 1 << 20), 256, 0, s>>>(
-        t.slots.p, t.cap - 1, nbox, d_rows, d_row_off, d_cols, d_col_off, d_out_off, total, d_out);
+        t.slots.p, t.cap - 1, nbox, d_rows, d_row_off, d_cols, d_col_off, d_out_off, total, d_out,
+        t.sym);
     HIFDE_LAUNCH_CHECK();
 }
 
@@ -214,7 +225,7 @@ void table_scatter(LevelTable& t, int nbox, const int* d_s, const int* d_skel_of
     if (total <= 0 || nbox <= 0) return;
     k_table_scatter<T><<<grid_for(total, 256, 1 << 20), 256, 0, s>>>(
         t.slots.p, t.cap - 1, nbox, d_s, d_skel_off, d_skel_gid, d_skel_pass, d_off_ss, total, d_S,
-        pass);
+        pass, t.sym);
     HIFDE_LAUNCH_CHECK();
 }
 
@@ -241,6 +252,7 @@ size_t table_compact(const LevelTable& src, LevelTable& dst, const unsigned char
     unsigned long long live = 0;
     HIFDE_CHECK(hipMemcpyAsync(&live, counter.p, sizeof(live), hipMemcpyDeviceToHost, s));
     HIFDE_CHECK(hipStreamSynchronize(s));
+    dst.sym = src.sym;
     table_init(dst, (size_t)(live + extra), s);
     k_table_rehash<<<grid_for((long long)src.cap), 256, 0, s>>>(src.slots.p, src.cap, d_alive,
                                                                  dst.slots.p, dst.cap - 1);

@@ -116,7 +116,11 @@ const char* hifde_stage_name(int stage);
  *                  column-pivoted QR; default 1e-7
  *   "fp32_levels"  HIFDE_MIXED only: levels 0 .. fp32_levels-1 are factored
  *                  and stored in FP32, the levels above in FP64; default
- *                  num_levels - 2 (the two top levels in FP64) */
+ *                  num_levels - 2 (the two top levels in FP64)
+ *   "symmetric"    -1 (default): when the assembled operator equals its
+ *                  transpose bitwise, store the level matrix by unordered
+ *                  pairs and form only the lower triangle of each Schur
+ *                  complement S = -A21^T A22^{-1} A21; 0: general path */
 int hifde_set_option(hifde_factor* f, const char* name, double value);
 /* Per-kernel-family device time, recorded with HIP events on the factor's
  * stream while profiling is on.  hifde_kernel_stats returns the number of

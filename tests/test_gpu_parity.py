@@ -322,3 +322,25 @@ def test_gemm_batch_matches_numpy(H, precision, tol, transA):
     for (A, B), C in zip(pairs, got):
         ref = (A.T if transA else A) @ B
         assert _rel(C, ref) < tol
+
+
+def test_symmetric_path_matches_general_path(H, oracle_mod):
+    """The symmetric fast path (pair-keyed level matrix, lower-triangle S)
+    against the general path on the same symmetric operator."""
+    N, P, eps = 31, 4, 1e-6
+    a, V, coo, A = _coo(oracle_mod, N, "pow10", True)
+    fs = H.HIFFactor(N, P, eps)
+    fs.set_stencil(a, V, True)
+    assert fs.Factor() == 0
+    fg = H.HIFFactor(N, P, eps)
+    fg.set_option("symmetric", 0)
+    fg.set_stencil(a, V, True)
+    assert fg.Factor() == 0
+    cs, cg = fs.level_counts(), fg.level_counts()
+    for lv in range(len(cs)):
+        for idx in (1, 3):
+            assert abs(cs[lv][idx] - cg[lv][idx]) <= 0.05 * max(cg[lv][idx], 20)
+    b = pb.rhs_normal(N + 1, 3)
+    rs = _rel(A @ fs.Apply(b, True), b)
+    rg = _rel(A @ fg.Apply(b, True), b)
+    assert rs <= 2 * rg and rg <= 2 * rs
