@@ -85,20 +85,29 @@ __device__ void block_argmax(T v, int idx, T* redv, int* redi, T* outv, int* out
 // LDS, one workgroup per box.  In-place GJ keeps [A | I] in one array; the
 // final column interchanges (reverse pivot order) turn it into A^{-1}.
 //
-// Larger boxes: block Gauss-Jordan with kGJB x kGJB diagonal blocks.  For
-// the diagonal block K (pivoting happens inside the block, whose inverse D
-// is computed exactly as above):
-//   A_KK <- D = A_KK^{-1}                        k_bgj_diag   (1 WG per box)
-//   R    <- D A_KJ              (J != K)         k_bgj_row    (column chunks)
-//   A_IJ <- A_IJ - A_IK R,  A_KJ <- R (I,J != K) k_bgj_update (64x64 tiles)
-//   A_IK <- -A_IK D             (I != K)         k_bgj_col    (row chunks)
-// Every phase except the small block inverse runs on many workgroups.  The
-// blocks of this path are the interior blocks of symmetric positive
-// definite or M-matrix operators, whose diagonal blocks are nonsingular; a
-// singular diagonal block is reported through info.
+// Larger boxes: two-level block Gauss-Jordan.  The outer level walks
+// kGJPanel-wide diagonal windows K of the box:
+//   A_KK <- D = A_KK^{-1}            inner level below, in place
+//   R    <- D A_KJ                   (J != K)      GEMM into a workspace
+//   A_IJ <- A_IJ - A_IK R            (I, J != K)   GEMM, rank kGJPanel
+//   C    <- -A_IK D                  (I != K)      GEMM into a workspace
+//   A_IK <- C,  A_KJ <- R                          strided copies
+// so the bulk of the 2n^3 flops is a rank-256 update, which is compute-bound
+// on the matrix cores.  The inner level inverts a window with kGJB-wide
+// diagonal blocks and the same five phases:
+//   A_kk <- D = A_kk^{-1}                        k_bgj_diag   (1 wave per box)
+//   R    <- D A_kj              (j != k)         k_bgj_row    (column chunks)
+//   A_ij <- A_ij - A_ik R       (i, j != k)      GEMM
+//   A_kj <- R                                    k_bgj_rowcopy
+//   A_ik <- -A_ik D, A_kk <- D  (i != k)         k_bgj_col    (row chunks)
+// Pivoting is partial pivoting inside each kGJB diagonal block.  The blocks
+// of this path are the interior blocks of symmetric positive definite or
+// M-matrix operators, whose diagonal blocks are nonsingular; a singular
+// diagonal block is reported through info.
 // ---------------------------------------------------------------------------
 constexpr int kSmallGJ = 64;
 constexpr int kGJB = 64;
+constexpr int kGJPanel = 256;
 
 // Pivoted in-place GJ of an n x n block (n <= 64) held in LDS as S[col][row],
 // padded with the identity to 64 x 64 (the inverse of diag(A, I) is
@@ -203,34 +212,84 @@ __global__ void __launch_bounds__(256) k_gj_small(const int* __restrict__ dims,
     if (tid == 0) info[b] = 0;
 }
 
+// Inverse D of the kb x kb diagonal block K of a window (kb <= 64), one wave
+// per box.  Lane i owns row i of diag(A_KK, I_{64-kb}) in registers.  Rotated
+// in-place Gauss-Jordan: the pivot column is always register 0; every step
+// shifts each row left by one and appends the new in-place column as register
+// 63, so after 64 steps register t holds in-place column t and no register is
+// indexed at run time.  The pivot of step k is the not yet used row of
+// largest |a_ik| (ties to the lower row), i.e. partial pivoting.  With p_k the
+// pivot row of step k, the in-place array Z satisfies A^{-1}(k, p_t) =
+// Z(p_k, t), which is how each lane writes its row out.
 template <class T>
-__global__ void __launch_bounds__(256) k_bgj_diag(const int* __restrict__ dims,
-                                                  const long long* __restrict__ off, T* mats,
-                                                  T* Dws, int step, int* info) {
+__global__ void __launch_bounds__(64) k_bgj_diag(const int* __restrict__ dims,
+                                                 const long long* __restrict__ off,
+                                                 const int* __restrict__ lds,
+                                                 const T* __restrict__ mats, T* Dws, int step,
+                                                 int ibase, int* info) {
     const int b = blockIdx.x;
     const int n = dims[b];
     const int k0 = step * kGJB;
-    if (n <= kSmallGJ || k0 >= n || info[b]) return;
+    if (k0 >= n || info[b]) return;
     const int kb = min(kGJB, n - k0);
-    const T* A = mats + off[b];
-    __shared__ T S[kSmallGJ][kSmallGJ + 1];
-    __shared__ T colk[kSmallGJ], rowk[kSmallGJ];
-    __shared__ int perm[kSmallGJ];
-    __shared__ int s_piv[kSmallGJ + 1];
-    const int tid = threadIdx.x;
-    for (int e = tid; e < kSmallGJ * kSmallGJ; e += 256) {
-        const int c = e >> 6, i = e & 63;
-        S[c][i] = (c < kb && i < kb) ? A[k0 + i + (size_t)(k0 + c) * n] : (T)(c == i ? 1 : 0);
+    const int ld = lds[b];
+    const T* A = mats + off[b] + k0 + (size_t)k0 * ld;
+    const int lane = threadIdx.x;
+    __shared__ T prow[kGJB];
+    __shared__ int piv[kGJB];
+    T reg[kGJB];
+#pragma unroll
+    for (int c = 0; c < kGJB; ++c)
+        reg[c] = (lane < kb && c < kb) ? A[lane + (size_t)c * ld] : (T)(c == lane ? 1 : 0);
+    bool used = false;
+    int my_step = 0;
+    int bad = 0;
+    for (int k = 0; k < kGJB; ++k) {
+        T v = used ? (T)-1 : fabs(reg[0]);
+        int idx = lane;
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+            const T ov = __shfl_xor(v, o, kWave);
+            const int oi = __shfl_xor(idx, o, kWave);
+            if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
+        }
+        const int p = __builtin_amdgcn_readfirstlane(idx);
+        if (__builtin_amdgcn_readfirstlane(v > (T)0 ? 0 : 1)) { bad = k + 1; break; }
+        const bool me = lane == p;
+        if (me) {
+            const T d = (T)1 / reg[0];
+#pragma unroll
+            for (int j = 1; j < kGJB; ++j) {
+                reg[j] *= d;
+                prow[j] = reg[j];
+            }
+            prow[0] = d;
+            piv[k] = lane;
+            used = true;
+            my_step = k;
+        }
+        __syncthreads();
+        const T f = me ? (T)0 : reg[0];
+#pragma unroll
+        for (int j = 1; j < kGJB; ++j) reg[j - 1] = fma(-f, prow[j], reg[j]);
+        reg[kGJB - 1] = me ? prow[0] : -f * prow[0];
+        __syncthreads();
     }
-    __syncthreads();
-    if (!lds_gj_inverse64<T>(S, kb, colk, rowk, perm, s_piv)) {
-        if (tid == 0) info[b] = k0 + 1;
+    if (bad) {
+        if (lane == 0) info[b] = ibase + k0 + bad;
         return;
     }
     T* D = Dws + (size_t)b * kGJB * kGJB;
-    for (int e = tid; e < kGJB * kGJB; e += 256) {
+    if (my_step < kb) {
+#pragma unroll
+        for (int t = 0; t < kGJB; ++t) {
+            const int c = piv[t];
+            if (c < kb) D[my_step + (size_t)c * kGJB] = reg[t];
+        }
+    }
+    // rows and columns >= kb of the 64 x 64 workspace are zero
+    for (int e = lane; e < kGJB * kGJB; e += kWave) {
         const int c = e >> 6, i = e & 63;
-        D[e] = (c < kb && i < kb) ? S[c][i] : (T)0;
+        if (c >= kb || i >= kb) D[e] = (T)0;
     }
 }
 
@@ -255,6 +314,7 @@ __device__ __forceinline__ void tile_mm(const T (*L)[65], const T (*Rm)[65], int
 template <class T>
 __global__ void __launch_bounds__(256) k_bgj_row(const int* __restrict__ dims,
                                                  const long long* __restrict__ off,
+                                                 const int* __restrict__ lds,
                                                  const T* __restrict__ mats,
                                                  const T* __restrict__ Dws, T* Rws,
                                                  const long long* __restrict__ r_off, int step,
@@ -262,10 +322,11 @@ __global__ void __launch_bounds__(256) k_bgj_row(const int* __restrict__ dims,
     const int b = blockIdx.y;
     const int n = dims[b];
     const int k0 = step * kGJB;
-    if (n <= kSmallGJ || k0 >= n || info[b]) return;
+    if (k0 >= n || info[b]) return;
     const int kb = min(kGJB, n - k0);
     const int j0 = blockIdx.x * 64;
     if (j0 >= n) return;
+    const int ld = lds[b];
     const T* A = mats + off[b];
     const T* D = Dws + (size_t)b * kGJB * kGJB;
     __shared__ T Ds[kGJB][65];  // Ds[c][i] = D(i, c)
@@ -276,7 +337,7 @@ __global__ void __launch_bounds__(256) k_bgj_row(const int* __restrict__ dims,
         Ds[c][i] = D[i + (size_t)c * kGJB];
         const int cc = e & 63, jj = e >> 6;  // coalesced along K rows
         const int gj = j0 + jj;
-        As[cc][jj] = (cc < kb && gj < n) ? A[k0 + cc + (size_t)gj * n] : (T)0;
+        As[cc][jj] = (cc < kb && gj < n) ? A[k0 + cc + (size_t)gj * ld] : (T)0;
     }
     __syncthreads();
     T acc[4][4];
@@ -299,73 +360,21 @@ __global__ void __launch_bounds__(256) k_bgj_row(const int* __restrict__ dims,
     }
 }
 
-// A(i, j) = i in K ? R(i-k0, j) : A(i, j) - A(i, K) R(:, j), for j outside K.
-template <class T>
-__global__ void __launch_bounds__(256) k_bgj_update(int nbox, const int* __restrict__ tile_off,
-                                                    const int* __restrict__ dims,
-                                                    const long long* __restrict__ off, T* mats,
-                                                    const T* __restrict__ Rws,
-                                                    const long long* __restrict__ r_off, int step,
-                                                    const int* __restrict__ info) {
-    const int t = blockIdx.x;
-    const int b = find_box(tile_off, nbox, t);
-    const int n = dims[b];
-    const int k0 = step * kGJB;
-    if (n <= kSmallGJ || k0 >= n || info[b]) return;
-    const int kb = min(kGJB, n - k0);
-    const int local = t - tile_off[b];
-    const int tm = (n + 63) / 64;
-    const int i0 = (local % tm) * 64, j0 = (local / tm) * 64;
-    if (j0 == k0) return;  // the panel's own columns (kGJB == 64 aligns them)
-    T* A = mats + off[b];
-    const T* R = Rws + r_off[b];
-    __shared__ T Ls[kGJB][65];  // Ls[c][i] = A(i0 + i, k0 + c)
-    __shared__ T Rs[kGJB][65];  // Rs[c][j] = R(c, j0 + j)
-    const int tid = threadIdx.x;
-    for (int e = tid; e < kGJB * 64; e += 256) {
-        const int i = e & 63, c = e >> 6;
-        const int gi = i0 + i;
-        Ls[c][i] = (gi < n && c < kb) ? A[gi + (size_t)(k0 + c) * n] : (T)0;
-        const int cc = e & 63, jj = e >> 6;
-        const int gj = j0 + jj;
-        Rs[cc][jj] = (cc < kb && gj < n) ? R[cc + (size_t)gj * kGJB] : (T)0;
-    }
-    __syncthreads();
-    T acc[4][4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = (T)0;
-    tile_mm<T>(Ls, Rs, kb, acc);
-    const int tx = tid & 15, ty = tid >> 4;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int jj = ty + 16 * j, gj = j0 + jj;
-        if (gj >= n) continue;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int gi = i0 + tx + 16 * i;
-            if (gi >= n) continue;
-            T* e = A + gi + (size_t)gj * n;
-            if (gi >= k0 && gi < k0 + kb) *e = Rs[gi - k0][jj];
-            else *e = *e - acc[i][j];
-        }
-    }
-}
-
 // A(i, K) = -A(i, K) D for i outside K; A(K, K) = D.  Grid (row chunks, boxes).
 template <class T>
 __global__ void __launch_bounds__(256) k_bgj_col(const int* __restrict__ dims,
-                                                 const long long* __restrict__ off, T* mats,
+                                                 const long long* __restrict__ off,
+                                                 const int* __restrict__ lds, T* mats,
                                                  const T* __restrict__ Dws, int step,
                                                  const int* __restrict__ info) {
     const int b = blockIdx.y;
     const int n = dims[b];
     const int k0 = step * kGJB;
-    if (n <= kSmallGJ || k0 >= n || info[b]) return;
+    if (k0 >= n || info[b]) return;
     const int kb = min(kGJB, n - k0);
     const int i0 = blockIdx.x * 64;
     if (i0 >= n) return;
+    const int ld = lds[b];
     T* A = mats + off[b];
     const T* D = Dws + (size_t)b * kGJB * kGJB;
     __shared__ T Ls[kGJB][65];  // Ls[c][i] = A(i0 + i, k0 + c)
@@ -374,7 +383,7 @@ __global__ void __launch_bounds__(256) k_bgj_col(const int* __restrict__ dims,
     for (int e = tid; e < kGJB * 64; e += 256) {
         const int i = e & 63, c = e >> 6;
         const int gi = i0 + i;
-        Ls[c][i] = (gi < n && c < kb) ? A[gi + (size_t)(k0 + c) * n] : (T)0;
+        Ls[c][i] = (gi < n && c < kb) ? A[gi + (size_t)(k0 + c) * ld] : (T)0;
         const int cc = e & 63, jj = e >> 6;
         Dt[cc][jj] = D[cc + (size_t)jj * kGJB];
     }
@@ -394,7 +403,7 @@ __global__ void __launch_bounds__(256) k_bgj_col(const int* __restrict__ dims,
         for (int i = 0; i < 4; ++i) {
             const int gi = i0 + tx + 16 * i;
             if (gi >= n) continue;
-            T* e = A + gi + (size_t)(k0 + jj) * n;
+            T* e = A + gi + (size_t)(k0 + jj) * ld;
             if (gi >= k0 && gi < k0 + kb) *e = Dt[gi - k0][jj];
             else *e = -acc[i][j];
         }
@@ -487,41 +496,59 @@ __global__ void __launch_bounds__(256) k_gemm(int nbox, const int* __restrict__ 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef double doublex4 __attribute__((ext_vector_type(4)));
 
+// Matrix-core configuration per element type.  Each wave owns a
+// (BLK*MT) x (BLK*MT) part of the 128 x 128 workgroup tile.  The products
+// are issued transposed, D = op(B)^T op(A)^T = C^T per MT x MT block, so that
+// the accumulator element i of lane l is C(row(l), col(i, l)) with the row
+// running along the lanes: the epilogue then reads and writes whole column
+// segments of C.
 template <class T>
 struct MfmaCfg;
 template <>
 struct MfmaCfg<float> {
-    static constexpr int MT = 32, KT = 2, NACC = 16;
+    // v_mfma_f32_32x32x2f32: D row 8(i/4) + 4(l/32) + i%4, column l%32
+    static constexpr int MT = 32, KT = 2, NACC = 16, BLK = 2;
     typedef floatx16 acc_t;
     __device__ static acc_t mma(float a, float b, acc_t c) {
         return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
     }
     __device__ static void pos(int i, int lane, int& r, int& c) {
-        r = 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
-        c = lane & 31;
+        r = lane & 31;
+        c = 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
     }
 };
 template <>
 struct MfmaCfg<double> {
-    static constexpr int MT = 16, KT = 4, NACC = 4;
+    // v_mfma_f64_16x16x4f64: D row 4i + l/16, column l%16
+    static constexpr int MT = 16, KT = 4, NACC = 4, BLK = 4;
     typedef doublex4 acc_t;
     __device__ static acc_t mma(double a, double b, acc_t c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
     __device__ static void pos(int i, int lane, int& r, int& c) {
-        r = 4 * i + (lane >> 4);
-        c = lane & 15;
+        r = lane & 15;
+        c = 4 * i + (lane >> 4);
     }
 };
 
+constexpr int kMfmaTile = 128;
+
+// C(m x n) = alpha op(A) B + beta C over a flattened list of 128 x 128 output
+// tiles of every descriptor, 4 waves in a 2 x 2 arrangement.  The k dimension
+// is walked in slabs of 16 held in two LDS buffers: the next slab is loaded
+// into registers while the current one feeds the matrix cores, so global
+// latency overlaps the products and each slab costs one barrier.
 template <class T, bool TRANSA>
-__global__ void __launch_bounds__(256) k_gemm_mfma(int nbox, const int* __restrict__ tile_off,
+__global__ void __launch_bounds__(256, 2) k_gemm_mfma(int nbox, const int* __restrict__ tile_off,
                                                    const GemmDesc* __restrict__ desc,
                                                    const T* __restrict__ Ab,
                                                    const T* __restrict__ Bb, T* Cb, T alpha,
                                                    T beta) {
     typedef MfmaCfg<T> C;
-    constexpr int MT = C::MT, KT = C::KT, WT = 2 * MT, TILE = 2 * WT, KB = 16;
+    constexpr int MT = C::MT, KT = C::KT, BLK = C::BLK, WT = BLK * MT, TILE = 2 * WT, KB = 16;
+    static_assert(TILE == kMfmaTile, "tile");
+    constexpr int LDT = TILE + 4;
+    constexpr int NLD = TILE * KB / 256;
     const int t = blockIdx.x;
     const int b = find_box(tile_off, nbox, t);
     const int local = t - tile_off[b];
@@ -533,52 +560,72 @@ __global__ void __launch_bounds__(256) k_gemm_mfma(int nbox, const int* __restri
     const T* A = Ab + dd.offA;
     const T* B = Bb + dd.offB;
     T* Cm = Cb + dd.offC;
-    __shared__ T As[KB][TILE + 4];
-    __shared__ T Bs[KB][TILE + 4];
+    __shared__ T As[2][KB][LDT];
+    __shared__ T Bs[2][KB][LDT];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wr = (wave & 1) * WT, wc = (wave >> 1) * WT;
-    typename C::acc_t acc[2][2];
+    T ra[NLD], rb[NLD];
+    auto gload = [&](int k0) {
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int c = 0; c < 2; ++c)
-#pragma unroll
-            for (int i = 0; i < C::NACC; ++i) acc[a][c][i] = (T)0;
-    for (int k0 = 0; k0 < k; k0 += KB) {
-        for (int e = tid; e < TILE * KB; e += 256) {
+        for (int q = 0; q < NLD; ++q) {
+            const int e = tid + 256 * q;
             if (!TRANSA) {
                 const int r = e % TILE, c = e / TILE;
                 const int gi = i0 + r, gk = k0 + c;
-                As[c][r] = (gi < m && gk < k) ? A[gi + (size_t)gk * dd.lda] : (T)0;
+                ra[q] = (gi < m && gk < k) ? A[gi + (size_t)gk * dd.lda] : (T)0;
             } else {
                 const int c = e % KB, r = e / KB;
                 const int gi = i0 + r, gk = k0 + c;
-                As[c][r] = (gi < m && gk < k) ? A[gk + (size_t)gi * dd.lda] : (T)0;
+                ra[q] = (gi < m && gk < k) ? A[gk + (size_t)gi * dd.lda] : (T)0;
             }
             const int cb = e % KB, jb = e / KB;
             const int gkb = k0 + cb, gj = j0 + jb;
-            Bs[cb][jb] = (gkb < k && gj < n) ? B[gkb + (size_t)gj * dd.ldb] : (T)0;
+            rb[q] = (gkb < k && gj < n) ? B[gkb + (size_t)gj * dd.ldb] : (T)0;
         }
-        __syncthreads();
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < NLD; ++q) {
+            const int e = tid + 256 * q;
+            if (!TRANSA) As[buf][e / TILE][e % TILE] = ra[q];
+            else As[buf][e % KB][e / KB] = ra[q];
+            Bs[buf][e % KB][e / KB] = rb[q];
+        }
+    };
+    typename C::acc_t acc[BLK][BLK];
+#pragma unroll
+    for (int a = 0; a < BLK; ++a)
+#pragma unroll
+        for (int c = 0; c < BLK; ++c)
+#pragma unroll
+            for (int i = 0; i < C::NACC; ++i) acc[a][c][i] = (T)0;
+    const int nsl = (k + KB - 1) / KB;
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    for (int sl = 0; sl < nsl; ++sl) {
+        const int cur = sl & 1;
+        if (sl + 1 < nsl) gload((sl + 1) * KB);
 #pragma unroll
         for (int kk = 0; kk < KB; kk += KT) {
             const int kr = kk + lane / MT, rr = lane % MT;
-            T av[2], bv[2];
+            T av[BLK], bv[BLK];
 #pragma unroll
-            for (int a = 0; a < 2; ++a) av[a] = As[kr][wr + a * MT + rr];
+            for (int a = 0; a < BLK; ++a) av[a] = As[cur][kr][wr + a * MT + rr];
 #pragma unroll
-            for (int c = 0; c < 2; ++c) bv[c] = Bs[kr][wc + c * MT + rr];
+            for (int c = 0; c < BLK; ++c) bv[c] = Bs[cur][kr][wc + c * MT + rr];
 #pragma unroll
-            for (int a = 0; a < 2; ++a)
+            for (int a = 0; a < BLK; ++a)
 #pragma unroll
-                for (int c = 0; c < 2; ++c) acc[a][c] = C::mma(av[a], bv[c], acc[a][c]);
+                for (int c = 0; c < BLK; ++c) acc[a][c] = C::mma(bv[c], av[a], acc[a][c]);
         }
+        if (sl + 1 < nsl) sstore(cur ^ 1);
         __syncthreads();
     }
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < BLK; ++a)
 #pragma unroll
-        for (int c = 0; c < 2; ++c)
+        for (int c = 0; c < BLK; ++c)
 #pragma unroll
             for (int i = 0; i < C::NACC; ++i) {
                 int r, cc;
@@ -595,21 +642,52 @@ __global__ void __launch_bounds__(256) k_gemm_mfma(int nbox, const int* __restri
 // A(K, j) = R(:, j) for the columns j outside the panel K (block GJ).
 template <class T>
 __global__ void __launch_bounds__(256) k_bgj_rowcopy(const int* __restrict__ dims,
-                                                     const long long* __restrict__ off, T* mats,
+                                                     const long long* __restrict__ off,
+                                                     const int* __restrict__ lds, T* mats,
                                                      const T* __restrict__ Rws,
                                                      const long long* __restrict__ r_off,
                                                      int step, const int* __restrict__ info) {
     const int b = blockIdx.y;
     const int n = dims[b];
     const int k0 = step * kGJB;
-    if (n <= kSmallGJ || k0 >= n || info[b]) return;
+    if (k0 >= n || info[b]) return;
     const int kb = min(kGJB, n - k0);
+    const int ld = lds[b];
     for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < (long long)kb * n;
          e += (long long)gridDim.x * 256) {
         const int j = (int)(e / kb), c = (int)(e - (long long)j * kb);
         if (j >= k0 && j < k0 + kb) continue;
-        mats[off[b] + k0 + c + (size_t)j * n] = Rws[r_off[b] + c + (size_t)j * kGJB];
+        mats[off[b] + k0 + c + (size_t)j * ld] = Rws[r_off[b] + c + (size_t)j * kGJB];
     }
+}
+
+__global__ void k_fill_int(int* p, int n, int v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// info[bix[j]] = cinfo[j]: blocked-path status back to the caller's boxes.
+__global__ void k_scatter_info(int nbig, const int* __restrict__ bix,
+                               const int* __restrict__ cinfo, int* info) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < nbig) info[bix[j]] = cinfo[j];
+}
+
+// Strided block copies dst(rows x cols, ldd) = src(rows x cols, lds) over a
+// descriptor list; src is src1 if sel else src0.  Grid (descriptors, chunks of
+// 4 columns).
+template <class T>
+__global__ void __launch_bounds__(256) k_copy2d(const Copy2D* __restrict__ desc,
+                                                const T* __restrict__ src0,
+                                                const T* __restrict__ src1, T* dst) {
+    const Copy2D d = desc[blockIdx.x];
+    const T* src = d.sel ? src1 : src0;
+    const int c0 = blockIdx.y * 4;
+    if (c0 >= d.cols) return;
+    const int c1 = min(c0 + 4, d.cols);
+    for (int c = c0; c < c1; ++c)
+        for (int i = threadIdx.x; i < d.rows; i += 256)
+            dst[d.dst + i + (size_t)c * d.ldd] = src[d.src + i + (size_t)c * d.lds];
 }
 
 // dst(box) = (D) src(rows r0.., cols c0.. of a ld-strided block); dst packed
@@ -981,54 +1059,144 @@ void batched_gj_inverse(int nbox, const int* h_n, const long long* h_off, const 
                         const long long* d_off, T* d_A, int* d_info, GJWork& ws, GemmWork& gws,
                         hipStream_t s) {
     if (nbox <= 0) return;
-    int max_n = 0;
+    // Boxes of at most kSmallGJ rows: one workgroup each.  Larger boxes are
+    // listed in slots 0..nbig-1 for the blocked path.
+    std::vector<int> bix;
     bool any_small = false;
-    std::vector<int> tile_off(nbox + 1, 0);
-    std::vector<long long> r_off(nbox + 1, 0);
+    int max_n = 0;
     for (int b = 0; b < nbox; ++b) {
         const int n = h_n[b];
-        max_n = std::max(max_n, n);
-        const bool big = n > kSmallGJ;
-        any_small |= (n > 0 && !big);
-        r_off[b + 1] = r_off[b] + (big ? (long long)kGJB * n : 0);
-        const int tm = big ? (n + 63) / 64 : 0;
-        tile_off[b + 1] = tile_off[b] + tm * tm;
+        if (n > kSmallGJ) {
+            bix.push_back(b);
+            max_n = std::max(max_n, n);
+        } else if (n > 0) {
+            any_small = true;
+        }
     }
     if (any_small) {
         k_gj_small<T><<<nbox, 256, 0, s>>>(d_n, d_off, d_A, d_info);
         HIFDE_LAUNCH_CHECK();
     }
-    if (max_n <= kSmallGJ) return;
-    ws.tile_off.upload(tile_off, s);
+    const int nbig = (int)bix.size();
+    if (nbig == 0) return;
+    const int osteps = (max_n + kGJPanel - 1) / kGJPanel;
+    // Per outer step and slot: window size, offset of A_KK, leading dimension.
+    std::vector<int> wn((size_t)osteps * nbig, 0), wld((size_t)osteps * nbig, 1);
+    std::vector<long long> woff((size_t)osteps * nbig, 0), r_off(nbig);
+    std::vector<long long> ro_off(nbig + 1, 0), cw_off(nbig + 1, 0);
+    for (int j = 0; j < nbig; ++j) {
+        const int b = bix[j], n = h_n[b];
+        r_off[j] = (long long)j * kGJB * kGJPanel;
+        const bool outer = n > kGJPanel;
+        ro_off[j + 1] = ro_off[j] + (outer ? (long long)kGJPanel * n : 0);
+        cw_off[j + 1] = cw_off[j] + (outer ? (long long)n * kGJPanel : 0);
+        for (int so = 0; so * kGJPanel < n; ++so) {
+            const int K0 = so * kGJPanel;
+            const size_t w = (size_t)so * nbig + j;
+            wn[w] = std::min(kGJPanel, n - K0);
+            woff[w] = h_off[b] + K0 + (long long)K0 * n;
+            wld[w] = n;
+        }
+    }
+    ws.wn.upload(wn, s);
+    ws.wld.upload(wld, s);
+    ws.woff.upload(woff, s);
     ws.b_off.upload(r_off, s);
-    ws.B.reserve((size_t)r_off[nbox] * sizeof(T));
-    ws.D.reserve((size_t)nbox * kGJB * kGJB * sizeof(T));
+    ws.bix.upload(bix, s);
+    ws.info.reserve(nbig);
+    k_fill_int<<<(nbig + 255) / 256, 256, 0, s>>>(ws.info.p, nbig, 0);
+    ws.B.reserve((size_t)nbig * kGJB * kGJPanel * sizeof(T));
+    ws.D.reserve((size_t)nbig * kGJB * kGJB * sizeof(T));
+    ws.RO.reserve((size_t)std::max<long long>(ro_off[nbig], 1) * sizeof(T));
+    ws.CW.reserve((size_t)std::max<long long>(cw_off[nbig], 1) * sizeof(T));
     T* Rws = reinterpret_cast<T*>(ws.B.p);
     T* Dws = reinterpret_cast<T*>(ws.D.p);
-    const int steps = (max_n + kGJB - 1) / kGJB;
-    const dim3 gch((max_n + 63) / 64, nbox);
-    for (int st = 0; st < steps; ++st) {
-        k_bgj_diag<T><<<nbox, 256, 0, s>>>(d_n, d_off, d_A, Dws, st, d_info);
-        k_bgj_row<T><<<gch, 256, 0, s>>>(d_n, d_off, d_A, Dws, Rws, ws.b_off.p, st, d_info);
-        // A(:, J) -= A(:, K) R(:, J) for J left and right of the panel, then
-        // the panel rows A(K, J) = R(:, J).
-        std::vector<GemmDesc> upd;
-        for (int b = 0; b < nbox; ++b) {
-            const int n = h_n[b], k0 = st * kGJB;
-            if (n <= kSmallGJ || k0 >= n) continue;
-            const int kb = std::min(kGJB, n - k0);
-            const long long o = h_off[b], ro = r_off[b];
-            if (k0 > 0) upd.push_back({o + (long long)k0 * n, ro, o, n, k0, kb, n, kGJB, n});
-            const int j1 = k0 + kb;
-            if (j1 < n)
-                upd.push_back({o + (long long)k0 * n, ro + (long long)j1 * kGJB,
-                               o + (long long)j1 * n, n, n - j1, kb, n, kGJB, n});
+    T* RO = reinterpret_cast<T*>(ws.RO.p);
+    T* CW = reinterpret_cast<T*>(ws.CW.p);
+    int* cinfo = ws.info.p;
+    std::vector<GemmDesc> upd, g1, g2, g3;
+    std::vector<Copy2D> cp;
+    for (int so = 0; so < osteps; ++so) {
+        const int K0 = so * kGJPanel;
+        const size_t w0 = (size_t)so * nbig;
+        const int* dwn = ws.wn.p + w0;
+        const int* dwld = ws.wld.p + w0;
+        const long long* dwoff = ws.woff.p + w0;
+        int max_w = 0;
+        for (int j = 0; j < nbig; ++j) max_w = std::max(max_w, wn[w0 + j]);
+        // inner level: every window inverted in place
+        const int isteps = (max_w + kGJB - 1) / kGJB;
+        const dim3 gch((max_w + 63) / 64, nbig);
+        for (int st = 0; st < isteps; ++st) {
+            k_bgj_diag<T><<<nbig, kWave, 0, s>>>(dwn, dwoff, dwld, d_A, Dws, st, K0, cinfo);
+            k_bgj_row<T><<<gch, 256, 0, s>>>(dwn, dwoff, dwld, d_A, Dws, Rws, ws.b_off.p, st,
+                                              cinfo);
+            // A(:, J) -= A(:, K) R(:, J) over the window's columns left and
+            // right of the block; the block rows are then set to R.
+            upd.clear();
+            for (int j = 0; j < nbig; ++j) {
+                const size_t w = w0 + j;
+                const int nw = wn[w], k0 = st * kGJB;
+                if (k0 >= nw) continue;
+                const int kb = std::min(kGJB, nw - k0), ld = wld[w];
+                const long long o = woff[w], ro = r_off[j];
+                if (k0 > 0) upd.push_back({o + (long long)k0 * ld, ro, o, nw, k0, kb, ld, kGJB, ld});
+                const int j1 = k0 + kb;
+                if (j1 < nw)
+                    upd.push_back({o + (long long)k0 * ld, ro + (long long)j1 * kGJB,
+                                   o + (long long)j1 * ld, nw, nw - j1, kb, ld, kGJB, ld});
+            }
+            batched_gemm_desc<T>(false, upd, d_A, Rws, d_A, (T)-1, (T)1, gws, s);
+            k_bgj_rowcopy<T><<<dim3((kGJB * max_w + 255) / 256, nbig), 256, 0, s>>>(
+                dwn, dwoff, dwld, d_A, Rws, ws.b_off.p, st, cinfo);
+            k_bgj_col<T><<<gch, 256, 0, s>>>(dwn, dwoff, dwld, d_A, Dws, st, cinfo);
         }
-        batched_gemm_desc<T>(false, upd, d_A, Rws, d_A, (T)-1, (T)1, gws, s);
-        k_bgj_rowcopy<T><<<dim3((kGJB * max_n + 255) / 256, nbox), 256, 0, s>>>(
-            d_n, d_off, d_A, Rws, ws.b_off.p, st, d_info);
-        k_bgj_col<T><<<gch, 256, 0, s>>>(d_n, d_off, d_A, Dws, st, d_info);
+        // outer level, boxes wider than one window
+        g1.clear();
+        g2.clear();
+        g3.clear();
+        cp.clear();
+        for (int j = 0; j < nbig; ++j) {
+            const int n = h_n[bix[j]];
+            const int kb = wn[w0 + j];
+            if (kb == 0 || n <= kGJPanel) continue;
+            const long long o = h_off[bix[j]], dk = woff[w0 + j];
+            const long long ro = ro_off[j], co = cw_off[j];
+            const int j1 = K0 + kb;
+            const int lo[2] = {0, j1}, len[2] = {K0, n - j1};
+            for (int q = 0; q < 2; ++q) {
+                if (len[q] == 0) continue;
+                // R(:, J) = D A(K, J)
+                g1.push_back({dk, o + K0 + (long long)lo[q] * n, ro + (long long)lo[q] * kGJPanel,
+                              kb, len[q], kb, n, n, kGJPanel});
+                // C(I, :) = -A(I, K) D
+                g3.push_back({o + lo[q] + (long long)K0 * n, dk, co + lo[q], len[q], kb, kb, n, n,
+                              n});
+                // A(I, K) = C(I, :);  A(K, J) = R(:, J)
+                cp.push_back({co + lo[q], o + lo[q] + (long long)K0 * n, len[q], kb, n, n, 0});
+                cp.push_back({ro + (long long)lo[q] * kGJPanel, o + K0 + (long long)lo[q] * n, kb,
+                              len[q], kGJPanel, n, 1});
+                // A(I, J) -= A(I, K) R(:, J)
+                for (int r = 0; r < 2; ++r) {
+                    if (len[r] == 0) continue;
+                    g2.push_back({o + lo[r] + (long long)K0 * n, ro + (long long)lo[q] * kGJPanel,
+                                  o + lo[r] + (long long)lo[q] * n, len[r], len[q], kb, n,
+                                  kGJPanel, n});
+                }
+            }
+        }
+        if (g1.empty()) continue;
+        batched_gemm_desc<T>(false, g1, d_A, d_A, RO, (T)1, (T)0, gws, s);
+        batched_gemm_desc<T>(false, g2, d_A, RO, d_A, (T)-1, (T)1, gws, s);
+        batched_gemm_desc<T>(false, g3, d_A, d_A, CW, (T)-1, (T)0, gws, s);
+        int maxc = 0;
+        for (const Copy2D& c : cp) maxc = std::max(maxc, c.cols);
+        ws.cdesc.upload(cp, s);
+        k_copy2d<T><<<dim3((unsigned)cp.size(), (maxc + 3) / 4), 256, 0, s>>>(ws.cdesc.p, CW, RO,
+                                                                             d_A);
+        HIFDE_LAUNCH_CHECK();
     }
+    k_scatter_info<<<(nbig + 255) / 256, 256, 0, s>>>(nbig, ws.bix.p, cinfo, d_info);
     HIFDE_LAUNCH_CHECK();
 }
 
@@ -1038,7 +1206,7 @@ void batched_gemm_desc(bool transA, const std::vector<GemmDesc>& desc, const T* 
     if (desc.empty()) return;
     // Boxes whose output spans at least half an MFMA workgroup tile in both
     // directions go to the MFMA kernel, the rest to the 64x64 FMA kernel.
-    constexpr int TILE = 4 * MfmaCfg<T>::MT;
+    constexpr int TILE = kMfmaTile;
     std::vector<GemmDesc> small, large;
     for (const GemmDesc& d : desc) {
         if (d.m <= 0 || d.n <= 0) continue;

@@ -126,10 +126,20 @@ size_t table_count(const LevelTable& t, hipStream_t s);
 
 // dense batched kernels
 // Workspace of the blocked inverse (pivots, pivot rows, tile list).
+// One strided block copy: rows x cols from src (leading dim lds) to dst
+// (leading dim ldd); sel picks one of two source buffers.
+struct Copy2D {
+    long long src, dst;
+    int rows, cols, lds, ldd, sel;
+};
+
 struct GJWork {
-    DevBuf<int> tile_off;
-    DevBuf<long long> b_off;
-    DevBuf<unsigned char> B, D;
+    DevBuf<int> wn, wld;              // per outer step and box: window size, leading dim
+    DevBuf<long long> woff, b_off;    // window offsets; inner R offsets
+    DevBuf<unsigned char> B, D;       // inner R, inner diagonal inverses
+    DevBuf<unsigned char> RO, CW;     // outer R (kGJPanel x n), outer column panel (n x kGJPanel)
+    DevBuf<Copy2D> cdesc;
+    DevBuf<int> bix, info;            // blocked-path boxes and their status
 };
 
 struct GemmDesc {

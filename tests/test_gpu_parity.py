@@ -67,6 +67,43 @@ def test_schur_batch_reports_singular_block(H):
     assert out[0][3] > 0 and out[1][3] == 0
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 1e-5)])
+def test_blocked_inverse_large_nonsymmetric(H, precision, tol):
+    """Inverses across the small, one-window and multi-window blocked paths.
+
+    Pivoting is partial pivoting inside each 64-wide diagonal block, which
+    suits the interior blocks of the factorization (their leading blocks are
+    nonsingular).  The test matrices keep that property: A = R/sqrt(n) + 3I
+    (singular values of every principal block in about [1, 5]), with the rows
+    of every aligned 64-row group shuffled so that each diagonal block needs
+    row interchanges.  Checked against numpy."""
+    rng = _rng(7)
+    sizes = [40, 64, 65, 255, 256, 257, 513, 700, 1100]
+    blocks, refs = [], []
+    for n in sizes:
+        A = rng.standard_normal((n, n)) / np.sqrt(n) + 3.0 * np.eye(n)
+        perm = np.concatenate([g + rng.permutation(min(64, n - g)) for g in range(0, n, 64)])
+        A = A[perm]
+        blocks.append((A, rng.standard_normal((n, 2)), rng.standard_normal((2, n))))
+        refs.append(np.linalg.inv(A))
+    out = H.schur_batch(blocks, precision)
+    for n, (Ainv, X, S, info), ref in zip(sizes, out, refs):
+        assert info == 0, n
+        assert _rel(Ainv, ref) < tol, (n, _rel(Ainv, ref))
+
+
+def test_blocked_inverse_reports_singular_column(H):
+    """A zero column in the second diagonal window is reported 1-based, as
+    getrf's info."""
+    rng = _rng(8)
+    n = 300
+    A = rng.standard_normal((n, n)) + n * np.eye(n)
+    A[:, 270] = 0.0
+    out = H.schur_batch([(A, np.ones((n, 1)), np.ones((1, n))),
+                         (np.eye(3) * 2.0, np.ones((3, 1)), np.ones((1, 3)))])
+    assert out[0][3] == 271 and out[1][3] == 0
+
+
 def test_interp_decomp_batch_matches_oracle(H, oracle_mod):
     rng = _rng(2)
     mats = [rng.standard_normal((102, 9)), rng.standard_normal((30, 9)),
