@@ -13,7 +13,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libhifde_amd.so")
-SOURCES = ["level_table.hip", "dense_batched.hip", "apply_pcg.hip", "factor_host.hip"]
+SOURCES = ["level_table.hip", "level_elem.hip", "dense_batched.hip", "apply_pcg.hip",
+           "factor_host.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

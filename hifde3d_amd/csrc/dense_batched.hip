@@ -415,11 +415,11 @@ constexpr int BM = 64, BN = 64, BK = 16;
 // C(m x n) = alpha * op(A) * B + beta * C, column-major with per-box
 // leading dimensions: op(A) = A (A stored m x k) or A^T (A stored k x m).
 // One 64x64 output tile per workgroup over a flattened tile list.
-template <class T, bool TRANSA>
+template <class T, bool TRANSA, class TC = T>
 __global__ void __launch_bounds__(256) k_gemm(int nbox, const int* __restrict__ tile_off,
                                               const GemmDesc* __restrict__ desc,
                                               const T* __restrict__ Ab, const T* __restrict__ Bb,
-                                              T* Cb, T alpha, T beta) {
+                                              TC* Cb, T alpha, T beta) {
     const int t = blockIdx.x;
     const int b = find_box(tile_off, nbox, t);
     const int local = t - tile_off[b];
@@ -430,7 +430,7 @@ __global__ void __launch_bounds__(256) k_gemm(int nbox, const int* __restrict__ 
     if (dd.lower && j0 > i0) return;
     const T* A = Ab + dd.offA;
     const T* B = Bb + dd.offB;
-    T* C = Cb + dd.offC;
+    TC* C = Cb + dd.offC;
     __shared__ T As[BK][BM + 1];
     __shared__ T Bs[BK][BN + 1];
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
@@ -479,8 +479,9 @@ __global__ void __launch_bounds__(256) k_gemm(int nbox, const int* __restrict__ 
         for (int i = 0; i < 4; ++i) {
             const int gi = i0 + tx + 16 * i;
             if (gi >= m) continue;
-            T* e = C + gi + (size_t)gj * dd.ldc;
-            *e = (beta == (T)0) ? alpha * acc[i][j] : beta * *e + alpha * acc[i][j];
+            TC* e = C + gi + (size_t)gj * dd.ldc;
+            const TC v = (TC)(alpha * acc[i][j]);
+            *e = (beta == (T)0) ? v : (TC)beta * *e + v;
         }
     }
 }
@@ -538,12 +539,12 @@ constexpr int kMfmaTile = 128;
 // is walked in slabs of 16 held in two LDS buffers: the next slab is loaded
 // into registers while the current one feeds the matrix cores, so global
 // latency overlaps the products and each slab costs one barrier.
-template <class T, bool TRANSA>
+template <class T, bool TRANSA, class TC = T>
 __global__ void __launch_bounds__(256, 2) k_gemm_mfma(int nbox, const int* __restrict__ tile_off,
-                                                   const GemmDesc* __restrict__ desc,
-                                                   const T* __restrict__ Ab,
-                                                   const T* __restrict__ Bb, T* Cb, T alpha,
-                                                   T beta) {
+                                                      const GemmDesc* __restrict__ desc,
+                                                      const T* __restrict__ Ab,
+                                                      const T* __restrict__ Bb, TC* Cb, T alpha,
+                                                      T beta) {
     typedef MfmaCfg<T> C;
     constexpr int MT = C::MT, KT = C::KT, BLK = C::BLK, WT = BLK * MT, TILE = 2 * WT, KB = 16;
     static_assert(TILE == kMfmaTile, "tile");
@@ -559,7 +560,7 @@ __global__ void __launch_bounds__(256, 2) k_gemm_mfma(int nbox, const int* __res
     if (dd.lower && j0 > i0) return;
     const T* A = Ab + dd.offA;
     const T* B = Bb + dd.offB;
-    T* Cm = Cb + dd.offC;
+    TC* Cm = Cb + dd.offC;
     __shared__ T As[2][KB][LDT];
     __shared__ T Bs[2][KB][LDT];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -632,9 +633,9 @@ __global__ void __launch_bounds__(256, 2) k_gemm_mfma(int nbox, const int* __res
                 C::pos(i, lane, r, cc);
                 const int gi = i0 + wr + a * MT + r, gj = j0 + wc + c * MT + cc;
                 if (gi < m && gj < n) {
-                    T* e = Cm + gi + (size_t)gj * dd.ldc;
-                    const T v = alpha * acc[a][c][i];
-                    *e = (beta == (T)0) ? v : beta * *e + v;
+                    TC* e = Cm + gi + (size_t)gj * dd.ldc;
+                    const TC v = (TC)(alpha * acc[a][c][i]);
+                    *e = (beta == (T)0) ? v : (TC)beta * *e + v;
                 }
             }
 }
@@ -1200,9 +1201,9 @@ void batched_gj_inverse(int nbox, const int* h_n, const long long* h_off, const 
     HIFDE_LAUNCH_CHECK();
 }
 
-template <class T>
+template <class T, class TC>
 void batched_gemm_desc(bool transA, const std::vector<GemmDesc>& desc, const T* d_A, const T* d_B,
-                       T* d_C, T alpha, T beta, GemmWork& ws, hipStream_t s) {
+                       TC* d_C, T alpha, T beta, GemmWork& ws, hipStream_t s) {
     if (desc.empty()) return;
     // Boxes whose output spans at least half an MFMA workgroup tile in both
     // directions go to the MFMA kernel, the rest to the 64x64 FMA kernel.
@@ -1228,15 +1229,15 @@ void batched_gemm_desc(bool transA, const std::vector<GemmDesc>& desc, const T* 
         const int total = toff[nb];
         if (pass == 0) {
             if (transA)
-                k_gemm<T, true><<<total, 256, 0, s>>>(nb, tb.p, db.p, d_A, d_B, d_C, alpha, beta);
+                k_gemm<T, true, TC><<<total, 256, 0, s>>>(nb, tb.p, db.p, d_A, d_B, d_C, alpha, beta);
             else
-                k_gemm<T, false><<<total, 256, 0, s>>>(nb, tb.p, db.p, d_A, d_B, d_C, alpha, beta);
+                k_gemm<T, false, TC><<<total, 256, 0, s>>>(nb, tb.p, db.p, d_A, d_B, d_C, alpha, beta);
         } else {
             if (transA)
-                k_gemm_mfma<T, true><<<total, 256, 0, s>>>(nb, tb.p, db.p, d_A, d_B, d_C, alpha,
+                k_gemm_mfma<T, true, TC><<<total, 256, 0, s>>>(nb, tb.p, db.p, d_A, d_B, d_C, alpha,
                                                            beta);
             else
-                k_gemm_mfma<T, false><<<total, 256, 0, s>>>(nb, tb.p, db.p, d_A, d_B, d_C, alpha,
+                k_gemm_mfma<T, false, TC><<<total, 256, 0, s>>>(nb, tb.p, db.p, d_A, d_B, d_C, alpha,
                                                             beta);
         }
         HIFDE_LAUNCH_CHECK();
@@ -1348,12 +1349,15 @@ template void batched_gj_inverse<float>(int, const int*, const long long*, const
                                         const long long*, float*, int*, GJWork&, GemmWork&,
                                         hipStream_t);
 
-template void batched_gemm_desc<double>(bool, const std::vector<GemmDesc>&, const double*,
-                                        const double*, double*, double, double, GemmWork&,
-                                        hipStream_t);
-template void batched_gemm_desc<float>(bool, const std::vector<GemmDesc>&, const float*,
-                                       const float*, float*, float, float, GemmWork&,
-                                       hipStream_t);
+template void batched_gemm_desc<double, double>(bool, const std::vector<GemmDesc>&,
+                                                const double*, const double*, double*, double,
+                                                double, GemmWork&, hipStream_t);
+template void batched_gemm_desc<float, float>(bool, const std::vector<GemmDesc>&, const float*,
+                                              const float*, float*, float, float, GemmWork&,
+                                              hipStream_t);
+template void batched_gemm_desc<float, double>(bool, const std::vector<GemmDesc>&, const float*,
+                                               const float*, double*, float, float, GemmWork&,
+                                               hipStream_t);
 template void batched_gemm<double>(bool, int, const int*, const int*, const int*,
                                    const long long*, const long long*, const long long*,
                                    const double*, const double*, double*, double, GemmWork&,

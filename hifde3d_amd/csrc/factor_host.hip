@@ -118,6 +118,24 @@ struct hifde_factor {
     DevBuf<double> pcg_buf, pcg_part, pcg_scalar, apply_tmp;
     double stage_t[ST_COUNT] = {0};
     int lu_info = 0;
+    // Element storage of the level matrix (level_elem.hip), used when every
+    // coupling of the operator lies inside a level-0 cell closure.
+    struct CellTab {
+        std::vector<int> lo, ext, tab;  // per cell: closure box and frontal position by coordinate
+        std::vector<long long> off;
+    } ctab;
+    bool local_ok = false;   // operator couplings stay inside level-0 cell closures
+    int elem_option = -1;    // -1 auto, 0 off (hash-table level matrix)
+    bool elem_on = false;
+    DevBuf<double> elem[2];  // FP64 element blocks of the previous / current level
+    int elem_cur = 0;
+    DevBuf<InitDesc> d_init;
+    DevBuf<ExtendDesc> d_ext;
+    DevBuf<FaceGatherDesc> d_fg;
+    DevBuf<FaceAddDesc> d_fa;
+    DevBuf<MirrorDesc> d_mir;
+    DevBuf<long long> d_eoff;
+    DevBuf<int> d_emap, d_rpos, d_cpos, d_spos, d_etab, d_egid;
 
     ~hifde_factor() {
         for (PendingRec& r : pending) {
@@ -222,6 +240,7 @@ struct hifde_factor {
         B.nbox = c * c * c;
         B.r.assign(B.nbox, 0);
         B.s.assign(B.nbox, 0);
+        B.o.assign(B.nbox, 0);
         auto box_range = [&](int cc, int& lo, int& hi) {
             lo = std::max(w * cc, 1);
             hi = std::min(w * (cc + 1), N);
@@ -233,7 +252,7 @@ struct hifde_factor {
             box_range(ci, lo[0], hi[0]);
             box_range(cj, lo[1], hi[1]);
             box_range(ck, lo[2], hi[2]);
-            int r = 0, s = 0;
+            int r = 0, s = 0, o = 0;
             for (int i = lo[0]; i <= hi[0]; ++i)
                 for (int j = lo[1]; j <= hi[1]; ++j)
                     for (int k = lo[2]; k <= hi[2]; ++k) {
@@ -241,15 +260,38 @@ struct hifde_factor {
                         int ni = I[i] + I[j] + I[k];
                         r += ni == 3;
                         s += ni == 2;
+                        o += ni < 2;
                     }
             B.r[b] = r;
             B.s[b] = s;
+            B.o[b] = o;
         }
         B.red_off = prefix_i(B.r);
         B.skel_off = prefix_i(B.s);
+        B.other_off = prefix_i(B.o);
         B.red_gid.resize(B.red_off[B.nbox]);
         B.skel_gid.resize(B.skel_off[B.nbox]);
+        B.other_gid.resize(B.other_off[B.nbox]);
         B.skel_pass.resize(B.skel_off[B.nbox]);
+        if (elem_on) {
+            // per cell: frontal position [red | skel | other] by local coordinate
+            ctab.lo.assign(3 * B.nbox, 0);
+            ctab.ext.assign(3 * B.nbox, 0);
+            ctab.off.assign(B.nbox + 1, 0);
+            for (int b = 0; b < B.nbox; ++b) {
+                int cc[3] = {b / (c * c), (b / c) % c, b % c};
+                long long vol = 1;
+                for (int a = 0; a < 3; ++a) {
+                    int lo, hi;
+                    box_range(cc[a], lo, hi);
+                    ctab.lo[3 * b + a] = lo;
+                    ctab.ext[3 * b + a] = hi - lo + 1;
+                    vol *= hi - lo + 1;
+                }
+                ctab.off[b + 1] = ctab.off[b] + vol;
+            }
+            ctab.tab.assign(ctab.off.back(), -1);
+        }
 #pragma omp parallel for schedule(dynamic, 16)
         for (int b = 0; b < B.nbox; ++b) {
             int cc[3] = {b / (c * c), (b / c) % c, b % c};
@@ -257,21 +299,32 @@ struct hifde_factor {
             for (int a = 0; a < 3; ++a) box_range(cc[a], lo[a], hi[a]);
             int* rg = B.red_gid.data() + B.red_off[b];
             int* sg = B.skel_gid.data() + B.skel_off[b];
+            int* og = B.other_gid.data() + B.other_off[b];
             unsigned char* sp = B.skel_pass.data() + B.skel_off[b];
+            int* tb = elem_on ? ctab.tab.data() + ctab.off[b] : nullptr;
+            const int ex = hi[0] - lo[0] + 1, ey = hi[1] - lo[1] + 1;
+            int pr = 0, ps = B.r[b], po = B.r[b] + B.s[b];
             for (int i = lo[0]; i <= hi[0]; ++i)
                 for (int j = lo[1]; j <= hi[1]; ++j)
                     for (int k = lo[2]; k <= hi[2]; ++k) {
                         int g = lin(i, j, k);
                         if (!remaining[g]) continue;
                         int ni = I[i] + I[j] + I[k];
+                        int pos;
                         if (ni == 3) {
                             *rg++ = g;
+                            pos = pr++;
                         } else if (ni == 2) {
                             *sg++ = g;
                             int x = !I[i] ? i : (!I[j] ? j : k);
                             int a = !I[i] ? 0 : (!I[j] ? 1 : 2);
                             *sp++ = (x == w * cc[a]) ? 1 : 0;  // on a min face
+                            pos = ps++;
+                        } else {
+                            *og++ = g;
+                            pos = po++;
                         }
+                        if (tb) tb[(i - lo[0]) + ex * ((j - lo[1]) + ey * (k - lo[2]))] = pos;
                     }
         }
         B.off_rr = prefix_ll(B.r, B.r);
@@ -280,6 +333,19 @@ struct hifde_factor {
         B.tot_rr = B.off_rr.back();
         B.tot_rs = B.off_rs.back();
         B.tot_ss = B.off_ss.back();
+        std::vector<int> e(B.nbox);
+        for (int b = 0; b < B.nbox; ++b) e[b] = B.s[b] + B.o[b];
+        B.off_ee = prefix_ll(e, e);
+        B.tot_ee = B.off_ee.back();
+    }
+
+    // Frontal position of DOF g in cell b of the current level's table, or -1.
+    int cell_pos(int b, int g) const {
+        const int x = g % NC, y = (g / NC) % NC, z = g / (NC * NC);
+        const int lx = x - ctab.lo[3 * b], ly = y - ctab.lo[3 * b + 1], lz = z - ctab.lo[3 * b + 2];
+        const int ex = ctab.ext[3 * b], ey = ctab.ext[3 * b + 1], ez = ctab.ext[3 * b + 2];
+        if (lx < 0 || ly < 0 || lz < 0 || lx >= ex || ly >= ey || lz >= ez) return -1;
+        return ctab.tab[ctab.off[b] + lx + (long long)ex * (ly + (long long)ey * lz)];
     }
 
     // DOFs of face `face` of cell (ci, cj, ck) with `want` interior coordinates
@@ -347,6 +413,7 @@ struct hifde_factor {
     struct FacePlan {
         int nface = 0;
         std::vector<int> m, n, rows, cols, rows_off, cols_off;
+        std::vector<int> c1, c2;  // the face's cell (max side) and its neighbour
     };
 
     void plan_faces(int level, FacePlan& F) {
@@ -372,6 +439,9 @@ struct hifde_factor {
                 slot[t] = F.nface++;
                 F.m.push_back(cm[t]);
                 F.n.push_back(cn[t]);
+                const int b = t / 3, f = order[t % 3];
+                F.c1.push_back(b);
+                F.c2.push_back(b - (f == TOP ? c * c : (f == LEFT ? c : 1)));
             }
         F.rows_off = prefix_i(F.m);
         F.cols_off = prefix_i(F.n);
@@ -403,9 +473,12 @@ struct hifde_factor {
         B.d_off_ss.upload(B.off_ss, stream);
     }
 
-    // A22 (r x r) is in B.A22.  Computes A22inv, X = A22^{-1} A21, S = -A12 X.
-    template <class T>
-    void schur_batch(BoxBatch& B, const T* A21, const T* A12, T* S, clk::time_point& t0) {
+    // A22 (r x r) is in B.A22.  Computes A22inv, X = A22^{-1} A21 and
+    // S = -A12 X, written to S + s_off[b] with leading dimension s_ld[b]
+    // (accumulate: added to what is there).
+    template <class T, class TS = T>
+    void schur_batch(BoxBatch& B, const T* A21, const T* A12, TS* S, const long long* s_off,
+                     const int* s_ld, bool accumulate, clk::time_point& t0) {
         const int nb = B.nbox;
         int max_r = 0;
         for (int x : B.r) max_r = std::max(max_r, x);
@@ -445,23 +518,22 @@ struct hifde_factor {
                             B.off_rs.data(), B.off_rs.data(), Ainv, A21, X, (T)1, gemm_ws,
                             stream);
         }
-        if (sym_on) {
-            // S = -A21^T X, symmetric: only the tiles on or below the diagonal
+        {
+            // S = -A12 X; symmetric: S = -A21^T X, only the tiles on or below
+            // the diagonal.  A box with r == 0 still writes S = 0 (the
+            // reference's k == 0 GEMM) unless S accumulates.
             std::vector<GemmDesc> d;
             for (int b = 0; b < nb; ++b) {
-                // r == 0 still writes S = 0 (the reference's k == 0 GEMM)
-                if (B.s[b] == 0) continue;
-                GemmDesc g{B.off_rs[b], B.off_rs[b], B.off_ss[b], B.s[b], B.s[b], B.r[b],
-                           B.r[b], B.r[b], B.s[b]};
-                g.lower = 1;
+                const int r = B.r[b], sb = B.s[b];
+                if (sb == 0 || (accumulate && r == 0)) continue;
+                GemmDesc g{B.off_rs[b], B.off_rs[b], s_off[b], sb, sb, r,
+                           std::max(sym_on ? r : sb, 1), std::max(r, 1), s_ld[b]};
+                g.lower = sym_on ? 1 : 0;
                 d.push_back(g);
             }
-            Prof pr(this, fam<T>("gemm_s"), fl_s / 2, by_s);
-            batched_gemm_desc<T>(true, d, A21, X, S, (T)-1, (T)0, gemm_ws, stream);
-        } else {
-            Prof pr(this, fam<T>("gemm_s"), fl_s, by_s);
-            batched_gemm<T>(false, nb, B.s.data(), B.s.data(), B.r.data(), B.off_rs.data(),
-                            B.off_rs.data(), B.off_ss.data(), A12, X, S, (T)-1, gemm_ws, stream);
+            Prof pr(this, fam<T>("gemm_s"), sym_on ? fl_s / 2 : fl_s, by_s);
+            batched_gemm_desc<T, TS>(sym_on, d, sym_on ? A21 : A12, X, S, (T)-1,
+                                     accumulate ? (T)1 : (T)0, gemm_ws, stream);
         }
         timed(ST_GEMM, t0);
         for (int b = 0; b < nb; ++b)
@@ -528,6 +600,10 @@ struct hifde_factor {
         timed(ST_PLAN, t0);
         upload_batch_index(B);
         timed(ST_UPLOAD, t0);
+        if (elem_on) {
+            level_cells_elem<T>(level, t0);
+            return;
+        }
         B.A22.reserve(B.tot_rr * sizeof(T));
         ws_a.reserve(B.tot_rs * sizeof(T));
         ws_b.reserve(B.tot_rs * sizeof(T));
@@ -556,9 +632,131 @@ struct hifde_factor {
                             B.d_red_off.p, B.d_off_rs.p, B.tot_rs, A12, stream);
         }
         timed(ST_GATHER, t0);
-        schur_batch<T>(B, A21, A12, S, t0);
+        schur_batch<T>(B, A21, A12, S, B.off_ss.data(), B.s.data(), false, t0);
         mark_eliminated(B);
         scatter<T>(B, S, true, t0);
+    }
+
+    // Cells of a level with the element storage: the frontal blocks A22,
+    // A21 (A12) and the element over [skel | other] are assembled from the
+    // operator (level 0) or by extend-add of the eight children's elements;
+    // S = -A12 A22^{-1} A21 is then added to the element by the Schur GEMM.
+    template <class T>
+    void level_cells_elem(int level, clk::time_point& t0) {
+        BoxBatch& B = cells[level];
+        const int nb = B.nbox;
+        B.A22.reserve(B.tot_rr * sizeof(T));
+        ws_a.reserve(B.tot_rs * sizeof(T));
+        if (!sym_on) ws_b.reserve(B.tot_rs * sizeof(T));
+        const int ne = 1 - elem_cur;
+        elem[ne].reserve(std::max<long long>(B.tot_ee, 1));
+        T* A22 = reinterpret_cast<T*>(B.A22.p);
+        T* A21 = reinterpret_cast<T*>(ws_a.p);
+        T* A12 = sym_on ? nullptr : reinterpret_cast<T*>(ws_b.p);
+        double* E = elem[ne].p;
+        const double w = sizeof(T);
+        const double fr_bytes =
+            ((double)B.tot_rr + (sym_on ? 1.0 : 2.0) * B.tot_rs) * w + (double)B.tot_ee * 8;
+        if (level == 0) {
+            const int c = NC / P;
+            std::vector<InitDesc> desc(nb);
+            std::vector<int> gid_off(nb + 1, 0);
+            for (int b = 0; b < nb; ++b) gid_off[b + 1] = gid_off[b] + B.r[b] + B.s[b] + B.o[b];
+            std::vector<int> gids(gid_off[nb]);
+#pragma omp parallel for schedule(static)
+            for (int b = 0; b < nb; ++b) {
+                InitDesc& d = desc[b];
+                d.rr = B.off_rr[b];
+                d.rs = B.off_rs[b];
+                d.ee = B.off_ee[b];
+                d.tab = ctab.off[b];
+                d.gid = gid_off[b];
+                d.r = B.r[b];
+                d.s = B.s[b];
+                d.o = B.o[b];
+                const int cc[3] = {b / (c * c), (b / c) % c, b % c};
+                for (int a = 0; a < 3; ++a) {
+                    d.lo[a] = ctab.lo[3 * b + a];
+                    d.ext[a] = ctab.ext[3 * b + a];
+                    d.cc[a] = cc[a];
+                }
+                int* g = gids.data() + gid_off[b];
+                g = std::copy(B.red_gid.begin() + B.red_off[b], B.red_gid.begin() + B.red_off[b + 1], g);
+                g = std::copy(B.skel_gid.begin() + B.skel_off[b],
+                              B.skel_gid.begin() + B.skel_off[b + 1], g);
+                std::copy(B.other_gid.begin() + B.other_off[b],
+                          B.other_gid.begin() + B.other_off[b + 1], g);
+            }
+            d_etab.upload(ctab.tab, stream);
+            d_egid.upload(gids, stream);
+            timed(ST_UPLOAD, t0);
+            Prof pr(this, fam<T>("assemble"), 0.0, fr_bytes + (double)h_ci.size() * 12.0);
+            frontal_init<T>(desc, d_init, d_etab.p, d_egid.p, d_rp.p, d_ci.p, d_v.p, NC, P, c,
+                            A22, A21, A12, E, stream);
+        } else {
+            const BoxBatch& Cb = cells[level - 1];
+            const int c = NC / width(level), cc = 2 * c;
+            // child k of a parent: (2 pi + k/4, 2 pj + k/2 % 2, 2 pk + k % 2)
+            auto child = [&](int b, int k) {
+                const int pi = b / (c * c), pj = (b / c) % c, pk = b % c;
+                return (2 * pi + (k >> 2)) * cc * cc + (2 * pj + ((k >> 1) & 1)) * cc +
+                       (2 * pk + (k & 1));
+            };
+            std::vector<long long> moff((size_t)nb * 8 + 1, 0);
+            for (int b = 0; b < nb; ++b)
+                for (int k = 0; k < 8; ++k) {
+                    const int ch = child(b, k);
+                    moff[(size_t)b * 8 + k + 1] = Cb.s[ch] + Cb.o[ch];
+                }
+            for (size_t t = 0; t < (size_t)nb * 8; ++t) moff[t + 1] += moff[t];
+            if (moff.back() > (long long)INT32_MAX) throw Error(HIFDE_ERR_SIZE, "extend-add maps");
+            std::vector<int> maps(moff.back());
+            std::vector<ExtendDesc> pass[8];
+            for (int k = 0; k < 8; ++k) pass[k].resize(nb);
+#pragma omp parallel for schedule(dynamic, 4)
+            for (int b = 0; b < nb; ++b)
+                for (int k = 0; k < 8; ++k) {
+                    const int ch = child(b, k);
+                    ExtendDesc& d = pass[k][b];
+                    d.src = Cb.off_ee[ch];
+                    d.rr = B.off_rr[b];
+                    d.rs = B.off_rs[b];
+                    d.ee = B.off_ee[b];
+                    d.e = Cb.s[ch] + Cb.o[ch];
+                    d.map = (int)moff[(size_t)b * 8 + k];
+                    d.r = B.r[b];
+                    d.s = B.s[b];
+                    d.ep = B.s[b] + B.o[b];
+                    int* m = maps.data() + d.map;
+                    for (int t = 0; t < Cb.s[ch]; ++t)
+                        *m++ = cell_pos(b, Cb.skel_gid[Cb.skel_off[ch] + t]);
+                    for (int t = 0; t < Cb.o[ch]; ++t)
+                        *m++ = cell_pos(b, Cb.other_gid[Cb.other_off[ch] + t]);
+                }
+            d_emap.upload(maps, stream);
+            timed(ST_PLAN, t0);
+            Prof pr(this, fam<T>("assemble"), 0.0, fr_bytes + (double)Cb.tot_ee * (8 + 4.0));
+            fill_zero<T>(A22, B.tot_rr, stream);
+            fill_zero<T>(A21, B.tot_rs, stream);
+            if (A12) fill_zero<T>(A12, B.tot_rs, stream);
+            fill_zero<double>(E, B.tot_ee, stream);
+            for (int k = 0; k < 8; ++k)
+                extend_add<T>(pass[k], d_ext, d_eoff, d_emap.p, elem[elem_cur].p, A22, A21, A12,
+                              E, stream);
+        }
+        elem_cur = ne;
+        timed(ST_GATHER, t0);
+        std::vector<int> eld(nb);
+        for (int b = 0; b < nb; ++b) eld[b] = std::max(B.s[b] + B.o[b], 1);
+        schur_batch<T, double>(B, A21, A12, E, B.off_ee.data(), eld.data(), true, t0);
+        if (sym_on) {
+            std::vector<MirrorDesc> md;
+            for (int b = 0; b < nb; ++b)
+                if (B.s[b] > 1 && B.r[b] > 0) md.push_back({B.off_ee[b], B.s[b], eld[b]});
+            mirror_lower(md, d_mir, d_eoff, E, stream);
+        }
+        timed(ST_SCATTER, t0);
+        mark_eliminated(B);
     }
 
     template <class T>
@@ -592,15 +790,65 @@ struct hifde_factor {
         ws_C.reserve(offC.back());
         ws_W.reserve(offC.back());
         ws_T.reserve(offC.back());
-        {
-            Prof pr(this, "gather_f64", 0.0, (double)offM.back() * (8 + 16));
-            table_gather<double>(tabs[cur], nf, ws_rows.p, ws_rows_off.p, ws_cols.p, ws_cols_off.p,
-                                 ws_offM.p, offM.back(), ws_M.p, stream);
-        }
-        {
-            Prof pr(this, "gather_f64", 0.0, (double)offC.back() * (8 + 16));
-            table_gather<double>(tabs[cur], nf, ws_cols.p, ws_cols_off.p, ws_cols.p, ws_cols_off.p,
-                                 ws_offC.p, offC.back(), ws_C.p, stream);
+        std::vector<int> cpos1;  // element positions of the face DOFs in the owner cell
+        if (elem_on) {
+            // A(rows, F) and A(F, F) summed from the two adjacent cells' elements
+            const BoxBatch& Cb = cells[level];
+            std::vector<int> rpos(2 * (size_t)F.rows_off.back()), cpos(2 * (size_t)F.cols_off.back());
+            cpos1.resize(F.cols_off.back());
+            std::vector<FaceGatherDesc> gd(nf);
+#pragma omp parallel for schedule(dynamic, 16)
+            for (int f = 0; f < nf; ++f) {
+                const int c1 = F.c1[f], c2 = F.c2[f], m = F.m[f], n = F.n[f];
+                auto epos = [&](int cell, int g) {
+                    const int p = cell_pos(cell, g);
+                    return p >= Cb.r[cell] ? p - Cb.r[cell] : -1;
+                };
+                int* r1 = rpos.data() + 2 * (size_t)F.rows_off[f];
+                int* q1 = cpos.data() + 2 * (size_t)F.cols_off[f];
+                for (int i = 0; i < m; ++i) {
+                    const int g = F.rows[F.rows_off[f] + i];
+                    r1[i] = epos(c1, g);
+                    r1[m + i] = epos(c2, g);
+                }
+                for (int j = 0; j < n; ++j) {
+                    const int g = F.cols[F.cols_off[f] + j];
+                    q1[j] = epos(c1, g);
+                    q1[n + j] = epos(c2, g);
+                    cpos1[F.cols_off[f] + j] = q1[j];
+                }
+                FaceGatherDesc& d = gd[f];
+                d.e1 = Cb.off_ee[c1];
+                d.e2 = Cb.off_ee[c2];
+                d.offM = offM[f];
+                d.offC = offC[f];
+                d.ld1 = Cb.s[c1] + Cb.o[c1];
+                d.ld2 = Cb.s[c2] + Cb.o[c2];
+                d.m = m;
+                d.n = n;
+                d.rows = 2 * F.rows_off[f];
+                d.cols = 2 * F.cols_off[f];
+            }
+            for (int x : cpos)
+                if (x < 0) throw Error(HIFDE_ERR_STATE, "face DOF outside its cells");
+            d_rpos.upload(rpos, stream);
+            d_cpos.upload(cpos, stream);
+            timed(ST_UPLOAD, t0);
+            Prof pr(this, "gather_f64", 0.0,
+                    (double)(offM.back() + offC.back()) * (8 + 2.0 * 8));
+            face_gather(gd, d_fg, d_eoff, d_rpos.p, d_cpos.p, elem[elem_cur].p, ws_M.p, ws_C.p,
+                        stream);
+        } else {
+            {
+                Prof pr(this, "gather_f64", 0.0, (double)offM.back() * (8 + 16));
+                table_gather<double>(tabs[cur], nf, ws_rows.p, ws_rows_off.p, ws_cols.p,
+                                     ws_cols_off.p, ws_offM.p, offM.back(), ws_M.p, stream);
+            }
+            {
+                Prof pr(this, "gather_f64", 0.0, (double)offC.back() * (8 + 16));
+                table_gather<double>(tabs[cur], nf, ws_cols.p, ws_cols_off.p, ws_cols.p,
+                                     ws_cols_off.p, ws_offC.p, offC.back(), ws_C.p, stream);
+            }
         }
         timed(ST_GATHER, t0);
         ws_k.reserve(nf);
@@ -717,9 +965,29 @@ struct hifde_factor {
         B.W.reserve(B.tot_rs * sizeof(T));
         pack_w<T>(B);
         timed(ST_TRANSFORM, t0);
-        schur_batch<T>(B, A21, A12, S, t0);
+        schur_batch<T>(B, A21, A12, S, B.off_ss.data(), B.s.data(), false, t0);
         mark_eliminated(B);
-        scatter<T>(B, S, false, t0);
+        if (elem_on) {
+            // A(skel_f, skel_f) += S_f, into the element of the face's own cell
+            const BoxBatch& Cb = cells[level];
+            std::vector<int> spos(B.skel_off.back());
+            std::vector<FaceAddDesc> ad;
+            for (int f = 0; f < nf; ++f) {
+                const int* p = perm.data() + F.cols_off[f];
+                for (int t = 0; t < B.s[f]; ++t)
+                    spos[B.skel_off[f] + t] = cpos1[F.cols_off[f] + p[t]];
+                if (B.s[f] == 0) continue;
+                const int c1 = F.c1[f];
+                ad.push_back({Cb.off_ee[c1], B.off_ss[f], Cb.s[c1] + Cb.o[c1], B.s[f],
+                              B.skel_off[f]});
+            }
+            d_spos.upload(spos, stream);
+            Prof pr(this, fam<T>("face_add"), 0.0, (double)B.tot_ss * (sizeof(T) + 16));
+            face_add<T>(ad, d_fa, d_eoff, d_spos.p, S, elem[elem_cur].p, sym_on ? 1 : 0, stream);
+            timed(ST_SCATTER, t0);
+        } else {
+            scatter<T>(B, S, false, t0);
+        }
     }
 
     // Blocks of Rot^T C Rot with Rot = I - E_{skel,red} W and C = A(F,F)
@@ -789,12 +1057,14 @@ struct hifde_factor {
         cells.resize(L);
         faces.resize(L);
         factored = false;
-        {
+        sym_on = op_symmetric && sym_option != 0;
+        elem_on = local_ok && elem_option != 0;
+        elem_cur = 0;
+        if (!elem_on) {
             // Initialize(): the level matrix starts as the assembled operator.
             auto t0 = clk::now();
             long long nnz = (long long)h_ci.size();
             cur = 0;
-            sym_on = op_symmetric && sym_option != 0;
             tabs[0].sym = tabs[1].sym = sym_on;
             table_init(tabs[cur], (size_t)(sym_on ? (nnz + NC * NC * NC) / 2 : nnz), stream);
             Prof pr(this, "table_insert", 0.0, (double)nnz * (4 + 4 + 8 + 16));
@@ -1047,6 +1317,42 @@ struct hifde_factor {
         for (long long i = 0; i < nd; ++i)
             for (int p = h_rp[i]; p < h_rp[i + 1]; ++p) rr[p] = (int)i;
         d_rr.upload(rr, stream);
+        // Element storage needs every coupling between two unknowns to lie
+        // inside one level-0 cell closure: per axis the coordinates must
+        // share a cell (a coordinate on a cell plane belongs to both sides).
+        {
+            const int w = P, c = NC / P;
+            auto cells_of = [&](int x, int& lo, int& hi) {
+                if (x % w == 0) {
+                    lo = std::max(x / w - 1, 0);
+                    hi = std::min(x / w, c - 1);
+                } else {
+                    lo = hi = x / w;
+                }
+            };
+            auto interior = [&](int g) {
+                const int x = g % NC, y = (g / NC) % NC, z = g / (NC * NC);
+                return x >= 1 && x <= N && y >= 1 && y <= N && z >= 1 && z <= N;
+            };
+            int ok = 1;
+#pragma omp parallel for schedule(static) reduction(min : ok)
+            for (long long i = 0; i < nd; ++i) {
+                if (!interior((int)i)) continue;
+                const int ic[3] = {(int)(i % NC), (int)((i / NC) % NC), (int)(i / ((long long)NC * NC))};
+                for (int p = h_rp[i]; p < h_rp[i + 1]; ++p) {
+                    const int j = h_ci[p];
+                    if (!interior(j)) continue;
+                    const int jc[3] = {j % NC, (j / NC) % NC, j / (NC * NC)};
+                    for (int a = 0; a < 3; ++a) {
+                        int l1, h1, l2, h2;
+                        cells_of(ic[a], l1, h1);
+                        cells_of(jc[a], l2, h2);
+                        if (std::max(l1, l2) > std::min(h1, h2)) ok = 0;
+                    }
+                }
+            }
+            local_ok = ok != 0;
+        }
         HIFDE_CHECK(hipStreamSynchronize(stream));
         have_matrix = true;
         factored = false;
@@ -1337,6 +1643,7 @@ int hifde_set_option(hifde_factor* f, const char* name, double value) {
         else if (n == "gram_min_eps") f->gram_min_eps = value;
         else if (n == "fp32_levels") f->fp32_levels = (int)value;
         else if (n == "symmetric") f->sym_option = (int)value;
+        else if (n == "elements") f->elem_option = (int)value;
         else throw Error(HIFDE_ERR_ARG, "set_option: unknown option " + n);
         return HIFDE_OK;
     });

@@ -93,6 +93,42 @@ struct BoxBatch {
     int max_r = 0, max_s = 0;
     // Faces only: per-face box geometry.
     std::vector<int> face_kind;
+    // Cells, element storage of the level matrix: the live edge and corner
+    // DOFs of the closure ("other"), and the element block over
+    // [skel | other] kept after the cell's Schur complement.
+    std::vector<int> o, other_off, other_gid;
+    std::vector<long long> off_ee;
+    long long tot_ee = 0;
+};
+
+// Element storage descriptors (level_elem.hip).
+struct InitDesc {            // one level-0 cell
+    long long rr, rs, ee;    // offsets of its A22, A21/A12 and element blocks
+    long long tab;           // offset of its position table
+    int gid;                 // offset of its DOF list [red | skel | other]
+    int r, s, o;
+    int lo[3], ext[3], cc[3];  // closure box origin, extent, cell coordinates
+};
+struct ExtendDesc {          // one (parent, child) pair of an extend-add pass
+    long long src;           // child element offset
+    long long rr, rs, ee;    // parent block offsets
+    int e;                   // child element size (its leading dimension)
+    int map;                 // offset of the child's parent-position map
+    int r, s, ep;            // parent red / skel counts and element size
+};
+struct FaceGatherDesc {      // one face
+    long long e1, e2;        // elements of the two adjacent cells
+    long long offM, offC;
+    int ld1, ld2, m, n;
+    int rows, cols;          // offsets of the row / column position arrays
+};
+struct FaceAddDesc {         // one face's Schur complement into its owner's element
+    long long e1, offS;
+    int ld1, k, sp;
+};
+struct MirrorDesc {
+    long long ee;
+    int s, ld;
 };
 
 }  // namespace hifde
@@ -157,10 +193,11 @@ struct GemmWork {
     DevBuf<CopyDesc> cdesc;
     DevBuf<long long> off;
 };
-// C = alpha op(A) B + beta C per descriptor (op(A) = A^T when transA).
-template <class T>
+// C = alpha op(A) B + beta C per descriptor (op(A) = A^T when transA).  C
+// may be FP64 for FP32 products (accumulated in FP32, added in FP64).
+template <class T, class TC = T>
 void batched_gemm_desc(bool transA, const std::vector<GemmDesc>& desc, const T* d_A, const T* d_B,
-                       T* d_C, T alpha, T beta, GemmWork& ws, hipStream_t s);
+                       TC* d_C, T alpha, T beta, GemmWork& ws, hipStream_t s);
 // Packed boxes (ld = rows): C = alpha op(A) B with host offsets.
 template <class T>
 void batched_gemm(bool transA, int nbox, const int* h_m, const int* h_n, const int* h_k,
@@ -189,6 +226,28 @@ void convert_d2f(const double* src, float* dst, long long n, hipStream_t s);
 template <class D>
 void pack_boxes(int nbox, const long long* d_dst_off, const long long* d_src_off,
                 const double* d_src, D* d_dst, long long total, hipStream_t s);
+
+// element storage of the level matrix (level_elem.hip); elements are FP64,
+// the frontal blocks A22 / A21 / A12 are in the level's factor precision T
+template <class T>
+void fill_zero(T* p, long long n, hipStream_t s);
+template <class T>
+void frontal_init(const std::vector<InitDesc>& desc, DevBuf<InitDesc>& dbuf, const int* d_tab,
+                  const int* d_gids, const int* d_rp, const int* d_ci, const double* d_v, int NC,
+                  int w, int c, T* A22, T* A21, T* A12, double* E, hipStream_t s);
+template <class Td>
+void extend_add(const std::vector<ExtendDesc>& desc, DevBuf<ExtendDesc>& dbuf,
+                DevBuf<long long>& obuf, const int* d_maps, const double* src, Td* A22, Td* A21,
+                Td* A12, double* E, hipStream_t s);
+void face_gather(const std::vector<FaceGatherDesc>& desc, DevBuf<FaceGatherDesc>& dbuf,
+                 DevBuf<long long>& obuf, const int* d_rpos, const int* d_cpos, const double* E,
+                 double* M, double* C, hipStream_t s);
+template <class T>
+void face_add(const std::vector<FaceAddDesc>& desc, DevBuf<FaceAddDesc>& dbuf,
+              DevBuf<long long>& obuf, const int* d_spos, const T* S, double* E, int sym,
+              hipStream_t s);
+void mirror_lower(const std::vector<MirrorDesc>& desc, DevBuf<MirrorDesc>& dbuf,
+                  DevBuf<long long>& obuf, double* E, hipStream_t s);
 
 // apply sweeps: one 64-output tile of one box per workgroup (see apply_pcg.hip)
 template <class T>

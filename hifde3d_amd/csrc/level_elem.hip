@@ -1,0 +1,330 @@
+// Element storage of the level matrix.
+//
+// The level matrix of HIFFactor (sp_matrix_, updated by UpdateMatrixAndDOFs,
+// src/Factor.cpp:262-311) is held unassembled: A = sum over the cells c of
+// the current level of an element block E_c on the live DOFs of the cell's
+// closure that are not interior to it ([face-interior | edges and corners]).
+// This is exact for operators whose couplings stay inside level-0 cell
+// closures (every stencil of include/setup_stencil.hpp and its variable
+// coefficient variant): cell-interior DOFs only couple inside their own cell,
+// Schur complements only add couplings among one cell's face DOFs, and the
+// skeletonization of a face only adds couplings among its own skeleton.  So
+//   * a cell's Schur blocks A22, A21, A12 come from one frontal block;
+//   * S = -A12 A22^{-1} A21 is added to the cell's own element, by the GEMM
+//     that computes it (beta = 1), instead of a scatter into a global table;
+//   * the ID matrix A(N, F) of a face is the sum of the two adjacent cells'
+//     elements, and its Schur complement goes into one of them;
+//   * the next level's frontal block of a cell is the extend-add of its
+//     eight children's elements.
+// Every entry is written by exactly one thread per pass, so the summation
+// order is fixed and the result deterministic.
+
+#include "hifde_internal.hpp"
+
+namespace hifde {
+
+namespace {
+
+template <class O>
+__device__ __forceinline__ int find_slot(const O* __restrict__ off, int n, long long e) {
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if ((long long)off[mid] <= e) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+inline unsigned elem_grid(long long n, int per = 256) {
+    long long g = (n + per - 1) / per;
+    if (g < 1) g = 1;
+    if (g > (1LL << 20)) g = 1LL << 20;
+    return (unsigned)g;
+}
+
+template <class T>
+__global__ void k_fill(T* p, long long n) {
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (long long)gridDim.x * blockDim.x)
+        p[e] = (T)0;
+}
+
+// Level-0 frontal blocks from the assembled operator (CSR).  One workgroup
+// per cell.  An entry A(a, b) is given to the lowest-numbered level-0 cell
+// whose closure holds both DOFs (per axis the candidates of a coordinate x
+// are x/w - 1 and x/w when x lies on a cell plane, else x/w).
+__device__ __forceinline__ void axis_cells(int x, int w, int c, int& lo, int& hi) {
+    if (x % w == 0) {
+        lo = max(x / w - 1, 0);
+        hi = min(x / w, c - 1);
+    } else {
+        lo = hi = x / w;
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_frontal_init(const InitDesc* __restrict__ desc,
+                                                      const int* __restrict__ tab,
+                                                      const int* __restrict__ gids,
+                                                      const int* __restrict__ rp,
+                                                      const int* __restrict__ ci,
+                                                      const double* __restrict__ v, int NC,
+                                                      int w, int c, T* A22, T* A21, T* A12,
+                                                      double* E) {
+    const InitDesc d = desc[blockIdx.x];
+    const int r = d.r, s = d.s, e = d.s + d.o, t = d.r + d.s + d.o;
+    for (long long q = threadIdx.x; q < (long long)r * r; q += blockDim.x) A22[d.rr + q] = (T)0;
+    for (long long q = threadIdx.x; q < (long long)r * s; q += blockDim.x) {
+        A21[d.rs + q] = (T)0;
+        if (A12) A12[d.rs + q] = (T)0;
+    }
+    for (long long q = threadIdx.x; q < (long long)e * e; q += blockDim.x) E[d.ee + q] = 0.0;
+    __syncthreads();
+    const int NC2 = NC * NC;
+    for (int pa = threadIdx.x; pa < t; pa += blockDim.x) {
+        const int a = gids[d.gid + pa];
+        const int ax = a % NC, ay = (a / NC) % NC, az = a / NC2;
+        int alo[3], ahi[3];
+        axis_cells(ax, w, c, alo[0], ahi[0]);
+        axis_cells(ay, w, c, alo[1], ahi[1]);
+        axis_cells(az, w, c, alo[2], ahi[2]);
+        for (int q = rp[a]; q < rp[a + 1]; ++q) {
+            const int b = ci[q];
+            const int bx = b % NC, by = (b / NC) % NC, bz = b / NC2;
+            const int lx = bx - d.lo[0], ly = by - d.lo[1], lz = bz - d.lo[2];
+            if (lx < 0 || ly < 0 || lz < 0 || lx >= d.ext[0] || ly >= d.ext[1] || lz >= d.ext[2])
+                continue;
+            const int pb = tab[d.tab + lx + d.ext[0] * (ly + d.ext[1] * lz)];
+            if (pb < 0) continue;
+            // owner: per axis the lowest cell holding both coordinates
+            int blo, bhi, own = 1;
+            const int bc[3] = {bx, by, bz};
+#pragma unroll
+            for (int x = 0; x < 3; ++x) {
+                axis_cells(bc[x], w, c, blo, bhi);
+                const int lo = max(alo[x], blo);
+                own &= (lo == d.cc[x]);
+            }
+            if (!own) continue;
+            const T val = (T)v[q];
+            if (pa < r) {
+                if (pb < r) A22[d.rr + pa + (long long)pb * r] = val;
+                else if (pb < r + s) A21[d.rs + pa + (long long)(pb - r) * r] = val;
+            } else if (pb >= r) {
+                E[d.ee + (pa - r) + (long long)(pb - r) * e] = v[q];
+            } else if (pa < r + s && A12) {
+                A12[d.rs + (pa - r) + (long long)pb * s] = val;
+            }
+        }
+    }
+}
+
+// One pass of the extend-add: for every parent, the element of one child is
+// added into the parent's frontal blocks.  Workgroup = one child column.
+template <class Td>
+__global__ void __launch_bounds__(128) k_extend_add(int npar, const long long* __restrict__ coff,
+                                                    const ExtendDesc* __restrict__ desc,
+                                                    const int* __restrict__ maps,
+                                                    const double* __restrict__ src, Td* A22,
+                                                    Td* A21, Td* A12, double* E) {
+    const long long blk = blockIdx.x;
+    const int p = find_slot(coff, npar, blk);
+    const ExtendDesc d = desc[p];
+    const int j = (int)(blk - coff[p]);
+    const int* mp = maps + d.map;
+    const int pj = mp[j];
+    if (pj < 0) return;
+    const int r = d.r, s = d.s;
+    const double* col = src + d.src + (long long)j * d.e;
+    if (pj < r) {
+        for (int i = threadIdx.x; i < d.e; i += blockDim.x) {
+            const int pi = mp[i];
+            if (pi < 0) continue;
+            const Td x = (Td)col[i];
+            if (pi < r) A22[d.rr + pi + (long long)pj * r] += x;
+            else if (pi < r + s && A12) A12[d.rs + (pi - r) + (long long)pj * s] += x;
+        }
+    } else if (pj < r + s) {
+        const int sj = pj - r;
+        for (int i = threadIdx.x; i < d.e; i += blockDim.x) {
+            const int pi = mp[i];
+            if (pi < 0) continue;
+            if (pi < r) A21[d.rs + pi + (long long)sj * r] += (Td)col[i];
+            else E[d.ee + (pi - r) + (long long)sj * d.ep] += col[i];
+        }
+    } else {
+        const int ej = pj - r;
+        for (int i = threadIdx.x; i < d.e; i += blockDim.x) {
+            const int pi = mp[i];
+            if (pi < r) continue;  // dead, or red x other (identically zero)
+            E[d.ee + (pi - r) + (long long)ej * d.ep] += col[i];
+        }
+    }
+}
+
+// M(i, j) = A(rows_i, cols_j) and C(i, j) = A(cols_i, cols_j) of every face,
+// in FP64, summed from the two adjacent cells' elements.  Workgroup = one
+// face column.
+__global__ void __launch_bounds__(128) k_face_gather(int nface, const long long* __restrict__ coff,
+                                                     const FaceGatherDesc* __restrict__ desc,
+                                                     const int* __restrict__ rpos,
+                                                     const int* __restrict__ cpos,
+                                                     const double* __restrict__ E, double* M,
+                                                     double* C) {
+    const long long blk = blockIdx.x;
+    const int f = find_slot(coff, nface, blk);
+    const FaceGatherDesc d = desc[f];
+    const int j = (int)(blk - coff[f]);
+    // position arrays: [pos in cell 1 | pos in cell 2] for the rows and the columns
+    const int* r1 = rpos + d.rows;
+    const int* r2 = r1 + d.m;
+    const int* c1 = cpos + d.cols;
+    const int* c2 = c1 + d.n;
+    const double* e1 = E + d.e1 + (long long)c1[j] * d.ld1;
+    const double* e2 = E + d.e2 + (long long)c2[j] * d.ld2;
+    for (int i = threadIdx.x; i < d.m; i += blockDim.x) {
+        const int p1 = r1[i], p2 = r2[i];
+        double x = 0.0;
+        if (p1 >= 0) x += e1[p1];
+        if (p2 >= 0) x += e2[p2];
+        M[d.offM + i + (long long)j * d.m] = x;
+    }
+    for (int i = threadIdx.x; i < d.n; i += blockDim.x)
+        C[d.offC + i + (long long)j * d.n] = e1[c1[i]] + e2[c2[i]];
+}
+
+// E_owner(skel_f, skel_f) += S_f.  Workgroup = one face column.  With sym,
+// only the lower triangle of S_f is valid and is read for both halves.
+template <class T>
+__global__ void __launch_bounds__(128) k_face_add(int nface, const long long* __restrict__ coff,
+                                                  const FaceAddDesc* __restrict__ desc,
+                                                  const int* __restrict__ spos,
+                                                  const T* __restrict__ S, double* E, int sym) {
+    const long long blk = blockIdx.x;
+    const int f = find_slot(coff, nface, blk);
+    const FaceAddDesc d = desc[f];
+    const int j = (int)(blk - coff[f]);
+    const int* sp = spos + d.sp;
+    double* col = E + d.e1 + (long long)sp[j] * d.ld1;
+    const T* Sf = S + d.offS;
+    for (int i = threadIdx.x; i < d.k; i += blockDim.x) {
+        const int a = (sym && i < j) ? j : i, b = (sym && i < j) ? i : j;
+        col[sp[i]] += (double)Sf[a + (long long)b * d.k];
+    }
+}
+
+// E(j, i) = E(i, j) for i > j inside the leading s x s block of each element
+// (the symmetric Schur product fills the lower triangle).  Workgroup = one
+// column.
+__global__ void __launch_bounds__(128) k_mirror_lower(int nbox, const long long* __restrict__ coff,
+                                                      const MirrorDesc* __restrict__ desc,
+                                                      double* E) {
+    const long long blk = blockIdx.x;
+    const int b = find_slot(coff, nbox, blk);
+    const MirrorDesc d = desc[b];
+    const int j = (int)(blk - coff[b]);
+    double* base = E + d.ee;
+    for (int i = j + 1 + threadIdx.x; i < d.s; i += blockDim.x)
+        base[j + (long long)i * d.ld] = base[i + (long long)j * d.ld];
+}
+
+template <class D>
+std::vector<long long> col_prefix(const std::vector<D>& desc, int (*cols)(const D&)) {
+    std::vector<long long> off(desc.size() + 1, 0);
+    for (size_t i = 0; i < desc.size(); ++i) off[i + 1] = off[i] + cols(desc[i]);
+    return off;
+}
+
+}  // namespace
+
+template <class T>
+void fill_zero(T* p, long long n, hipStream_t s) {
+    if (n <= 0) return;
+    k_fill<T><<<elem_grid(n), 256, 0, s>>>(p, n);
+    HIFDE_LAUNCH_CHECK();
+}
+
+template <class T>
+void frontal_init(const std::vector<InitDesc>& desc, DevBuf<InitDesc>& dbuf, const int* d_tab,
+                  const int* d_gids, const int* d_rp, const int* d_ci, const double* d_v, int NC,
+                  int w, int c, T* A22, T* A21, T* A12, double* E, hipStream_t s) {
+    if (desc.empty()) return;
+    dbuf.upload(desc, s);
+    k_frontal_init<T><<<(unsigned)desc.size(), 256, 0, s>>>(dbuf.p, d_tab, d_gids, d_rp, d_ci,
+                                                            d_v, NC, w, c, A22, A21, A12, E);
+    HIFDE_LAUNCH_CHECK();
+}
+
+template <class Td>
+void extend_add(const std::vector<ExtendDesc>& desc, DevBuf<ExtendDesc>& dbuf,
+                DevBuf<long long>& obuf, const int* d_maps, const double* src, Td* A22, Td* A21,
+                Td* A12, double* E, hipStream_t s) {
+    if (desc.empty()) return;
+    const std::vector<long long> off =
+        col_prefix<ExtendDesc>(desc, [](const ExtendDesc& d) { return d.e; });
+    if (off.back() == 0) return;
+    dbuf.upload(desc, s);
+    obuf.upload(off, s);
+    k_extend_add<Td><<<(unsigned)off.back(), 128, 0, s>>>((int)desc.size(), obuf.p, dbuf.p,
+                                                          d_maps, src, A22, A21, A12, E);
+    HIFDE_LAUNCH_CHECK();
+}
+
+void face_gather(const std::vector<FaceGatherDesc>& desc, DevBuf<FaceGatherDesc>& dbuf,
+                 DevBuf<long long>& obuf, const int* d_rpos, const int* d_cpos, const double* E,
+                 double* M, double* C, hipStream_t s) {
+    if (desc.empty()) return;
+    const std::vector<long long> off =
+        col_prefix<FaceGatherDesc>(desc, [](const FaceGatherDesc& d) { return d.n; });
+    if (off.back() == 0) return;
+    dbuf.upload(desc, s);
+    obuf.upload(off, s);
+    k_face_gather<<<(unsigned)off.back(), 128, 0, s>>>((int)desc.size(), obuf.p, dbuf.p, d_rpos,
+                                                       d_cpos, E, M, C);
+    HIFDE_LAUNCH_CHECK();
+}
+
+template <class T>
+void face_add(const std::vector<FaceAddDesc>& desc, DevBuf<FaceAddDesc>& dbuf,
+              DevBuf<long long>& obuf, const int* d_spos, const T* S, double* E, int sym,
+              hipStream_t s) {
+    if (desc.empty()) return;
+    const std::vector<long long> off =
+        col_prefix<FaceAddDesc>(desc, [](const FaceAddDesc& d) { return d.k; });
+    if (off.back() == 0) return;
+    dbuf.upload(desc, s);
+    obuf.upload(off, s);
+    k_face_add<T><<<(unsigned)off.back(), 128, 0, s>>>((int)desc.size(), obuf.p, dbuf.p, d_spos,
+                                                       S, E, sym);
+    HIFDE_LAUNCH_CHECK();
+}
+
+void mirror_lower(const std::vector<MirrorDesc>& desc, DevBuf<MirrorDesc>& dbuf,
+                  DevBuf<long long>& obuf, double* E, hipStream_t s) {
+    if (desc.empty()) return;
+    const std::vector<long long> off =
+        col_prefix<MirrorDesc>(desc, [](const MirrorDesc& d) { return d.s; });
+    if (off.back() == 0) return;
+    dbuf.upload(desc, s);
+    obuf.upload(off, s);
+    k_mirror_lower<<<(unsigned)off.back(), 128, 0, s>>>((int)desc.size(), obuf.p, dbuf.p, E);
+    HIFDE_LAUNCH_CHECK();
+}
+
+#define HIFDE_INST1(T)                                                                          \
+    template void fill_zero<T>(T*, long long, hipStream_t);                                     \
+    template void frontal_init<T>(const std::vector<InitDesc>&, DevBuf<InitDesc>&, const int*,  \
+                                  const int*, const int*, const int*, const double*, int, int,  \
+                                  int, T*, T*, T*, double*, hipStream_t);                       \
+    template void extend_add<T>(const std::vector<ExtendDesc>&, DevBuf<ExtendDesc>&,            \
+                                DevBuf<long long>&, const int*, const double*, T*, T*, T*,      \
+                                double*, hipStream_t);                                          \
+    template void face_add<T>(const std::vector<FaceAddDesc>&, DevBuf<FaceAddDesc>&,            \
+                              DevBuf<long long>&, const int*, const T*, double*, int,           \
+                              hipStream_t);
+HIFDE_INST1(float)
+HIFDE_INST1(double)
+#undef HIFDE_INST1
+
+}  // namespace hifde

@@ -118,9 +118,14 @@ const char* hifde_stage_name(int stage);
  *                  and stored in FP32, the levels above in FP64; default
  *                  num_levels - 2 (the two top levels in FP64)
  *   "symmetric"    -1 (default): when the assembled operator equals its
- *                  transpose bitwise, store the level matrix by unordered
- *                  pairs and form only the lower triangle of each Schur
- *                  complement S = -A21^T A22^{-1} A21; 0: general path */
+ *                  transpose bitwise, form only the lower triangle of each
+ *                  Schur complement S = -A21^T A22^{-1} A21 (and store the
+ *                  hash-table level matrix by unordered pairs); 0: general
+ *                  path
+ *   "elements"     -1 (default): when every coupling of the operator lies
+ *                  inside one level-0 cell closure (all stencils), hold the
+ *                  level matrix as per-cell dense elements assembled by
+ *                  extend-add; 0: device hash table of (row, col) entries */
 int hifde_set_option(hifde_factor* f, const char* name, double value);
 /* Per-kernel-family device time, recorded with HIP events on the factor's
  * stream while profiling is on.  hifde_kernel_stats returns the number of

@@ -381,3 +381,55 @@ def test_symmetric_path_matches_general_path(H, oracle_mod):
     rs = _rel(A @ fs.Apply(b, True), b)
     rg = _rel(A @ fg.Apply(b, True), b)
     assert rs <= 2 * rg and rg <= 2 * rs
+
+
+@pytest.mark.parametrize("sym", [True, False])
+def test_element_storage_matches_hash_table(H, oracle_mod, sym):
+    """The per-cell element level matrix (extend-add) against the device hash
+    table on the same operator: the same matrix, summed in another order."""
+    N, P, eps = 31, 4, 1e-6
+    a, V, coo, A = _coo(oracle_mod, N, "pow10", sym)
+    fe = H.HIFFactor(N, P, eps)
+    fe.set_stencil(a, V, sym)
+    assert fe.Factor() == 0
+    ft = H.HIFFactor(N, P, eps)
+    ft.set_option("elements", 0)
+    ft.set_stencil(a, V, sym)
+    assert ft.Factor() == 0
+    ce, ct = fe.level_counts(), ft.level_counts()
+    for lv in range(len(ce)):
+        for idx in (1, 3):
+            assert abs(ce[lv][idx] - ct[lv][idx]) <= 0.05 * max(ct[lv][idx], 20)
+    # level 0 needs no ID decision before it: identical boxes
+    for box in (0, 5, 63):
+        ge, gt = fe.box(0, 0, box), ft.box(0, 0, box)
+        assert np.array_equal(ge["red_gid"], gt["red_gid"])
+        assert _rel(ge["A22inv"], gt["A22inv"]) < 1e-12
+        assert _rel(ge["X"], gt["X"]) < 1e-12
+    b = pb.rhs_normal(N + 1, 3)
+    re = _rel(A @ fe.Apply(b, True), b)
+    rt = _rel(A @ ft.Apply(b, True), b)
+    assert re <= 2 * rt and rt <= 2 * re
+
+
+def test_nonlocal_operator_takes_table_path(H, oracle_mod):
+    """An operator with a coupling between DOFs of non-adjacent level-0 cells
+    cannot be held as cell elements; the factorization must still match the
+    oracle (hash-table level matrix)."""
+    N, P, eps = 15, 4, 1e-6
+    a, V, (r, c, v), _ = _coo(oracle_mod, N, "ones", False)
+    NC = N + 1
+    i, j = 1 + NC * 1 + NC * NC * 1, 14 + NC * 14 + NC * NC * 14
+    r = np.concatenate([r, [i, j]]).astype(np.int32)
+    c = np.concatenate([c, [j, i]]).astype(np.int32)
+    v = np.concatenate([v, [-1.0, -1.0]])
+    o = oracle_mod.Oracle(N, P, eps)
+    o.set_matrix_coo(r, c, v)
+    assert o.factor(4) == 0
+    f = H.HIFFactor(N, P, eps)
+    f.set_matrix_coo(r, c, v)
+    assert f.Factor() == 0
+    assert [x[:4] for x in f.level_counts()] == [tuple(x) for x in o.level_counts()]
+    b = pb.rhs_normal(NC, 4)
+    y, y_o = f.Apply(b, True), o.apply(b, True)
+    assert _rel(y, y_o) < 1e-8
