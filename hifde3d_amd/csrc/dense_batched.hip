@@ -667,6 +667,24 @@ __global__ void k_fill_int(int* p, int n, int v) {
     if (i < n) p[i] = v;
 }
 
+__global__ void __launch_bounds__(256) k_fold_info(int nbox, const int* __restrict__ info,
+                                                   int* status) {
+    __shared__ int best[256];
+    int b0 = INT32_MAX;
+    for (int b = threadIdx.x; b < nbox; b += 256)
+        if (info[b] != 0) {
+            b0 = b;
+            break;
+        }
+    best[threadIdx.x] = b0;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) best[threadIdx.x] = min(best[threadIdx.x], best[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && best[0] != INT32_MAX && *status == 0) *status = info[best[0]];
+}
+
 // info[bix[j]] = cinfo[j]: blocked-path status back to the caller's boxes.
 __global__ void k_scatter_info(int nbig, const int* __restrict__ bix,
                                const int* __restrict__ cinfo, int* info) {
@@ -1198,6 +1216,18 @@ void batched_gj_inverse(int nbox, const int* h_n, const long long* h_off, const 
         HIFDE_LAUNCH_CHECK();
     }
     k_scatter_info<<<(nbig + 255) / 256, 256, 0, s>>>(nbig, ws.bix.p, cinfo, d_info);
+    HIFDE_LAUNCH_CHECK();
+}
+
+void fold_info(int nbox, const int* d_info, int* d_status, hipStream_t s) {
+    if (nbox <= 0) return;
+    k_fold_info<<<1, 256, 0, s>>>(nbox, d_info, d_status);
+    HIFDE_LAUNCH_CHECK();
+}
+
+void fill_int(int* p, int n, int v, hipStream_t s) {
+    if (n <= 0) return;
+    k_fill_int<<<(n + 255) / 256, 256, 0, s>>>(p, n, v);
     HIFDE_LAUNCH_CHECK();
 }
 

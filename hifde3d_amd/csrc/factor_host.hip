@@ -21,6 +21,32 @@
 #include "../../include/hifde_amd.h"
 #include "hifde_internal.hpp"
 
+namespace hifde {
+namespace {
+thread_local Stager* t_stager = nullptr;
+}
+Stager* active_stager() { return t_stager; }
+void set_active_stager(Stager* st) { t_stager = st; }
+
+const void* Stager::stage(const void* src, size_t bytes) {
+    const size_t need = (bytes + 255) & ~(size_t)255;
+    if (used + need > cap) {
+        HIFDE_CHECK(hipStreamSynchronize(stream));
+        used = 0;
+        if (need > cap) {
+            if (buf) HIFDE_CHECK(hipHostFree(buf));
+            buf = nullptr;
+            cap = std::max(need, std::max(2 * cap, (size_t)64 << 20));
+            HIFDE_CHECK(hipHostMalloc(reinterpret_cast<void**>(&buf), cap));
+        }
+    }
+    std::memcpy(buf + used, src, bytes);
+    const void* p = buf + used;
+    used += need;
+    return p;
+}
+}  // namespace hifde
+
 using namespace hifde;
 
 namespace {
@@ -117,7 +143,10 @@ struct hifde_factor {
     double gram_min_eps = 1e-7;   // below this tolerance every face uses Householder
     DevBuf<double> pcg_buf, pcg_part, pcg_scalar, apply_tmp;
     double stage_t[ST_COUNT] = {0};
+    bool stage_sync = false;  // synchronize the stream at every stage boundary
     int lu_info = 0;
+    DevBuf<int> d_status;     // first nonzero block-inverse status of the factorization
+    Stager stager;
     // Element storage of the level matrix (level_elem.hip), used when every
     // coupling of the operator lies inside a level-0 cell closure.
     struct CellTab {
@@ -208,8 +237,10 @@ struct hifde_factor {
         pending.clear();
     }
 
+    // Host time per stage; with stage_sync the stream is drained first, so
+    // the time includes the stage's device work.
     void timed(Stage st, clk::time_point& t0) {
-        HIFDE_CHECK(hipStreamSynchronize(stream));
+        if (stage_sync) HIFDE_CHECK(hipStreamSynchronize(stream));
         auto t1 = clk::now();
         stage_t[st] += std::chrono::duration<double>(t1 - t0).count();
         t0 = t1;
@@ -508,10 +539,8 @@ struct hifde_factor {
             batched_gj_inverse<T>(nb, B.r.data(), B.off_rr.data(), B.d_r.p, B.d_off_rr.p, Ainv,
                                   ws_info.p, gj_ws, gemm_ws, stream);
         }
+        fold_info(nb, ws_info.p, d_status.p, stream);
         timed(ST_INVERSE, t0);
-        std::vector<int> info(nb);
-        HIFDE_CHECK(hipMemcpyAsync(info.data(), ws_info.p, nb * sizeof(int),
-                                   hipMemcpyDeviceToHost, stream));
         {
             Prof pr(this, fam<T>("gemm_x"), fl_x, by_x);
             batched_gemm<T>(false, nb, B.r.data(), B.s.data(), B.r.data(), B.off_rr.data(),
@@ -536,13 +565,11 @@ struct hifde_factor {
                                      accumulate ? (T)1 : (T)0, gemm_ws, stream);
         }
         timed(ST_GEMM, t0);
-        for (int b = 0; b < nb; ++b)
-            if (info[b] && !lu_info) lu_info = info[b];
     }
 
     void mark_eliminated(const BoxBatch& B) {
         for (int g : B.red_gid) remaining[g] = 0;  // UpdateRemainingDOFs (:91-111)
-        d_alive.upload(remaining, stream);
+        if (!elem_on) d_alive.upload(remaining, stream);  // read by the table compaction
     }
 
     // Number of S entries written in the second scatter pass (both DOFs on
@@ -916,6 +943,9 @@ struct hifde_factor {
                                    stream));
         HIFDE_CHECK(hipMemcpyAsync(perm.data(), ws_perm.p, perm.size() * sizeof(int),
                                    hipMemcpyDeviceToHost, stream));
+        // the level's one synchronization: the skeleton sets plan what follows
+        HIFDE_CHECK(hipStreamSynchronize(stream));
+        stager.synced();
         timed(ST_DOWNLOAD, t0);
         // Face box: red = ID redundant columns, skel = ID skeleton columns
         // (Skeletonize, src/Factor.cpp:171-178).
@@ -1047,6 +1077,14 @@ struct hifde_factor {
         if (!have_matrix) throw Error(HIFDE_ERR_STATE, "factor: no matrix set");
         for (double& t : stage_t) t = 0.0;
         lu_info = 0;
+        stager.stream = stream;
+        struct StagerGuard {
+            Stager* prev;
+            explicit StagerGuard(Stager* s) : prev(active_stager()) { set_active_stager(s); }
+            ~StagerGuard() { set_active_stager(prev); }
+        } guard(&stager);
+        d_status.reserve(1);
+        fill_int(d_status.p, 1, 0, stream);
         // Initialize (src/Factor.cpp:22-34)
         remaining.assign((size_t)NC * NC * NC, 0);
         for (int k = 1; k < NC; ++k)
@@ -1083,6 +1121,10 @@ struct hifde_factor {
         }
         // Every DOF has been eliminated: the level matrix is consumed.  Its
         // buffers and the workspaces are kept for the next factorization.
+        HIFDE_CHECK(hipMemcpyAsync(&lu_info, d_status.p, sizeof(int), hipMemcpyDeviceToHost,
+                                   stream));
+        HIFDE_CHECK(hipStreamSynchronize(stream));
+        stager.synced();
         resolve_profile();
         factored = true;
     }
@@ -1644,6 +1686,7 @@ int hifde_set_option(hifde_factor* f, const char* name, double value) {
         else if (n == "fp32_levels") f->fp32_levels = (int)value;
         else if (n == "symmetric") f->sym_option = (int)value;
         else if (n == "elements") f->elem_option = (int)value;
+        else if (n == "stage_sync") f->stage_sync = value != 0;
         else throw Error(HIFDE_ERR_ARG, "set_option: unknown option " + n);
         return HIFDE_OK;
     });

@@ -24,6 +24,33 @@ struct Error : std::runtime_error {
 
 #define HIFDE_LAUNCH_CHECK() HIFDE_CHECK(hipGetLastError())
 
+// Pinned staging arena for host-to-device uploads.  hipMemcpyAsync from
+// pageable memory returns only once the data is copied; staged through
+// pinned memory an upload is queued on the stream like a kernel and the host
+// runs ahead.  The arena is reused once the stream has been synchronized
+// (synced()), and synchronizes the stream itself when it runs out of room.
+struct Stager {
+    unsigned char* buf = nullptr;
+    size_t cap = 0, used = 0;
+    hipStream_t stream = nullptr;
+    Stager() = default;
+    Stager(const Stager&) = delete;
+    Stager& operator=(const Stager&) = delete;
+    ~Stager() {
+        if (buf) (void)hipHostFree(buf);
+    }
+    const void* stage(const void* src, size_t bytes);
+    void synced() { used = 0; }
+};
+// Stager of the calling thread's current factorization (null: plain copies).
+Stager* active_stager();
+void set_active_stager(Stager* st);
+inline void upload_bytes(void* dst, const void* src, size_t bytes, hipStream_t s) {
+    Stager* st = active_stager();
+    if (st && st->stream == s) src = st->stage(src, bytes);
+    HIFDE_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+}
+
 // Owning device buffer.  Grows, never shrinks, unless release() is called.
 template <class T>
 struct DevBuf {
@@ -40,6 +67,8 @@ struct DevBuf {
     ~DevBuf() { release(); }
     void reserve(size_t cnt) {
         if (cnt <= n && p) return;
+        // work queued earlier may still read the old buffer
+        if (p) HIFDE_CHECK(hipDeviceSynchronize());
         release();
         HIFDE_CHECK(hipMalloc(reinterpret_cast<void**>(&p), (cnt ? cnt : 1) * sizeof(T)));
         n = cnt;
@@ -51,7 +80,7 @@ struct DevBuf {
     }
     void upload(const T* h, size_t cnt, hipStream_t s) {
         reserve(cnt);
-        if (cnt) HIFDE_CHECK(hipMemcpyAsync(p, h, cnt * sizeof(T), hipMemcpyHostToDevice, s));
+        if (cnt) upload_bytes(p, h, cnt * sizeof(T), s);
     }
     void upload(const std::vector<T>& h, hipStream_t s) { upload(h.data(), h.size(), s); }
 };
@@ -223,6 +252,10 @@ void gram_interp_decomp(int nface, const int* d_n, const unsigned char* d_mode,
                         int* d_perm, const int* d_perm_off, int* d_k, double* d_W, int max_n,
                         hipStream_t s);
 void convert_d2f(const double* src, float* dst, long long n, hipStream_t s);
+// *d_status = info[b] of the first box b with a nonzero status, unless
+// *d_status is already nonzero.
+void fold_info(int nbox, const int* d_info, int* d_status, hipStream_t s);
+void fill_int(int* p, int n, int v, hipStream_t s);
 template <class D>
 void pack_boxes(int nbox, const long long* d_dst_off, const long long* d_src_off,
                 const double* d_src, D* d_dst, long long total, hipStream_t s);

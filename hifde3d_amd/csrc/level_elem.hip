@@ -216,17 +216,38 @@ __global__ void __launch_bounds__(128) k_face_add(int nface, const long long* __
 
 // E(j, i) = E(i, j) for i > j inside the leading s x s block of each element
 // (the symmetric Schur product fills the lower triangle).  Workgroup = one
-// column.
-__global__ void __launch_bounds__(128) k_mirror_lower(int nbox, const long long* __restrict__ coff,
+// 64 x 64 tile (I, J), I >= J, of the lower triangle: it is read along
+// columns, transposed through LDS and written along the columns of tile
+// (J, I), so both sides are coalesced.
+__global__ void __launch_bounds__(256) k_mirror_lower(int nbox, const long long* __restrict__ toff,
                                                       const MirrorDesc* __restrict__ desc,
                                                       double* E) {
     const long long blk = blockIdx.x;
-    const int b = find_slot(coff, nbox, blk);
+    const int b = find_slot(toff, nbox, blk);
     const MirrorDesc d = desc[b];
-    const int j = (int)(blk - coff[b]);
+    // tile t -> (I, J) with I >= J, column-major over the lower triangle
+    int t = (int)(blk - toff[b]);
+    const int nt = (d.s + 63) / 64;
+    int J = 0;
+    while (t >= nt - J) {
+        t -= nt - J;
+        ++J;
+    }
+    const int I = J + t;
+    __shared__ double tile[64][65];
     double* base = E + d.ee;
-    for (int i = j + 1 + threadIdx.x; i < d.s; i += blockDim.x)
-        base[j + (long long)i * d.ld] = base[i + (long long)j * d.ld];
+    const int i0 = I * 64, j0 = J * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int c = ty; c < 64; c += 4) {
+        const int gi = i0 + tx, gj = j0 + c;
+        tile[c][tx] = (gi < d.s && gj < d.s) ? base[gi + (long long)gj * d.ld] : 0.0;
+    }
+    __syncthreads();
+    // write E(j0 + tx, i0 + c) = E(i0 + c, j0 + tx) where i0 + c > j0 + tx
+    for (int c = ty; c < 64; c += 4) {
+        const int gi = i0 + c, gj = j0 + tx;
+        if (gi < d.s && gj < d.s && gi > gj) base[gj + (long long)gi * d.ld] = tile[tx][c];
+    }
 }
 
 template <class D>
@@ -303,12 +324,14 @@ void face_add(const std::vector<FaceAddDesc>& desc, DevBuf<FaceAddDesc>& dbuf,
 void mirror_lower(const std::vector<MirrorDesc>& desc, DevBuf<MirrorDesc>& dbuf,
                   DevBuf<long long>& obuf, double* E, hipStream_t s) {
     if (desc.empty()) return;
-    const std::vector<long long> off =
-        col_prefix<MirrorDesc>(desc, [](const MirrorDesc& d) { return d.s; });
+    const std::vector<long long> off = col_prefix<MirrorDesc>(desc, [](const MirrorDesc& d) {
+        const int nt = (d.s + 63) / 64;
+        return nt * (nt + 1) / 2;
+    });
     if (off.back() == 0) return;
     dbuf.upload(desc, s);
     obuf.upload(off, s);
-    k_mirror_lower<<<(unsigned)off.back(), 128, 0, s>>>((int)desc.size(), obuf.p, dbuf.p, E);
+    k_mirror_lower<<<(unsigned)off.back(), 256, 0, s>>>((int)desc.size(), obuf.p, dbuf.p, E);
     HIFDE_LAUNCH_CHECK();
 }
 
