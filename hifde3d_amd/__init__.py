@@ -32,6 +32,7 @@ EXPORTS = [
     "hifde_set_stencil", "hifde_factor_run", "hifde_apply", "hifde_apply_device", "hifde_spmv",
     "hifde_pcg", "hifde_pcg_device", "hifde_num_levels", "hifde_level_counts",
     "hifde_box_count", "hifde_box_sizes", "hifde_box_data", "hifde_stage_times",
+    "hifde_level_stage_times",
     "hifde_stage_name", "hifde_factor_bytes", "hifde_interp_decomp_batch", "hifde_schur_batch",
     "hifde_set_profiling", "hifde_reset_kernel_stats", "hifde_kernel_stats", "hifde_set_option",
     "hifde_gemm_batch",
@@ -70,6 +71,7 @@ def lib():
     L.hifde_box_sizes.argtypes = [_vp, _i, _i, _i, _ip]
     L.hifde_box_data.argtypes = [_vp, _i, _i, _i, _ip, _ip, _dp, _dp, _dp, _dp]
     L.hifde_stage_times.argtypes = [_vp, _dp, _i]
+    L.hifde_level_stage_times.argtypes = [_vp, _i, _dp, _i]
     L.hifde_stage_name.argtypes = [_i]
     L.hifde_stage_name.restype = ctypes.c_char_p
     L.hifde_factor_bytes.argtypes = [_vp]
@@ -226,6 +228,15 @@ class HIFFactor:
         buf = np.zeros(16)
         n = _check(lib().hifde_stage_times(self._h, _ptr(buf), 16))
         return {lib().hifde_stage_name(i).decode(): float(buf[i]) for i in range(n)}
+
+    def level_stage_times(self):
+        """Per level of the last factor: {stage name: seconds}."""
+        out = []
+        buf = np.zeros(16)
+        for lv in range(self.num_levels):
+            n = _check(lib().hifde_level_stage_times(self._h, lv, _ptr(buf), 16))
+            out.append({lib().hifde_stage_name(i).decode(): float(buf[i]) for i in range(n)})
+        return out
 
     def set_option(self, name, value):
         _check(lib().hifde_set_option(self._h, name.encode(), float(value)))

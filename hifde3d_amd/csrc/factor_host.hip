@@ -130,9 +130,7 @@ struct hifde_factor {
     // workspaces
     DevBuf<unsigned char> ws_a, ws_b, ws_c;
     DevBuf<double> ws_M, ws_C, ws_W, ws_T;
-    DevBuf<int> ws_rows, ws_rows_off, ws_cols, ws_cols_off, ws_perm, ws_perm_off, ws_k, ws_info,
-        ws_m, ws_n;
-    DevBuf<long long> ws_offM, ws_offC, ws_offW;
+    DevBuf<int> ws_perm, ws_k, ws_info;
     GJWork gj_ws;
     GemmWork gemm_ws;
     DevBuf<unsigned char> ws_mode;
@@ -143,6 +141,8 @@ struct hifde_factor {
     double gram_min_eps = 1e-7;   // below this tolerance every face uses Householder
     DevBuf<double> pcg_buf, pcg_part, pcg_scalar, apply_tmp;
     double stage_t[ST_COUNT] = {0};
+    std::vector<double> stage_lt;  // per level: [level * ST_COUNT + stage]
+    int cur_level = 0;
     bool stage_sync = false;  // synchronize the stream at every stage boundary
     int lu_info = 0;
     DevBuf<int> d_status;     // first nonzero block-inverse status of the factorization
@@ -156,6 +156,7 @@ struct hifde_factor {
     bool local_ok = false;   // operator couplings stay inside level-0 cell closures
     int elem_option = -1;    // -1 auto, 0 off (hash-table level matrix)
     bool elem_on = false;
+    bool plan0_valid = false;  // level-0 cell and face index built and on the device
     DevBuf<double> elem[2];  // FP64 element blocks of the previous / current level
     int elem_cur = 0;
     DevBuf<InitDesc> d_init;
@@ -164,7 +165,7 @@ struct hifde_factor {
     DevBuf<FaceAddDesc> d_fa;
     DevBuf<MirrorDesc> d_mir;
     DevBuf<long long> d_eoff;
-    DevBuf<int> d_emap, d_rpos, d_cpos, d_spos, d_etab, d_egid;
+    DevBuf<int> d_emap, d_spos, d_etab, d_egid;
 
     ~hifde_factor() {
         for (PendingRec& r : pending) {
@@ -242,7 +243,10 @@ struct hifde_factor {
     void timed(Stage st, clk::time_point& t0) {
         if (stage_sync) HIFDE_CHECK(hipStreamSynchronize(stream));
         auto t1 = clk::now();
-        stage_t[st] += std::chrono::duration<double>(t1 - t0).count();
+        const double dt = std::chrono::duration<double>(t1 - t0).count();
+        stage_t[st] += dt;
+        if ((size_t)(cur_level * ST_COUNT + st) < stage_lt.size())
+            stage_lt[cur_level * ST_COUNT + st] += dt;
         t0 = t1;
     }
 
@@ -267,7 +271,7 @@ struct hifde_factor {
         const int w = width(level), c = NC / w;
         const AxisClass ac = axis_class(level);
         const unsigned char* I = ac.interior.data();
-        B = BoxBatch();
+        B.reset_host();
         B.nbox = c * c * c;
         B.r.assign(B.nbox, 0);
         B.s.assign(B.nbox, 0);
@@ -276,23 +280,26 @@ struct hifde_factor {
             lo = std::max(w * cc, 1);
             hi = std::min(w * (cc + 1), N);
         };
+        // The boxes are walked in memory order (i fastest); the lists are
+        // then put in the reference's scan order (i, j, k lexicographic, k
+        // fastest) by sorting the live points.
 #pragma omp parallel for schedule(dynamic, 16)
         for (int b = 0; b < B.nbox; ++b) {
-            int ci = b / (c * c), cj = (b / c) % c, ck = b % c;
+            int cc[3] = {b / (c * c), (b / c) % c, b % c};
             int lo[3], hi[3];
-            box_range(ci, lo[0], hi[0]);
-            box_range(cj, lo[1], hi[1]);
-            box_range(ck, lo[2], hi[2]);
+            for (int a = 0; a < 3; ++a) box_range(cc[a], lo[a], hi[a]);
             int r = 0, s = 0, o = 0;
-            for (int i = lo[0]; i <= hi[0]; ++i)
-                for (int j = lo[1]; j <= hi[1]; ++j)
-                    for (int k = lo[2]; k <= hi[2]; ++k) {
-                        if (!remaining[lin(i, j, k)]) continue;
+            for (int k = lo[2]; k <= hi[2]; ++k)
+                for (int j = lo[1]; j <= hi[1]; ++j) {
+                    const unsigned char* row = remaining.data() + lin(0, j, k);
+                    for (int i = lo[0]; i <= hi[0]; ++i) {
+                        if (!row[i]) continue;
                         int ni = I[i] + I[j] + I[k];
                         r += ni == 3;
                         s += ni == 2;
                         o += ni < 2;
                     }
+                }
             B.r[b] = r;
             B.s[b] = s;
             B.o[b] = o;
@@ -323,40 +330,54 @@ struct hifde_factor {
             }
             ctab.tab.assign(ctab.off.back(), -1);
         }
-#pragma omp parallel for schedule(dynamic, 16)
-        for (int b = 0; b < B.nbox; ++b) {
-            int cc[3] = {b / (c * c), (b / c) % c, b % c};
-            int lo[3], hi[3];
-            for (int a = 0; a < 3; ++a) box_range(cc[a], lo[a], hi[a]);
-            int* rg = B.red_gid.data() + B.red_off[b];
-            int* sg = B.skel_gid.data() + B.skel_off[b];
-            int* og = B.other_gid.data() + B.other_off[b];
-            unsigned char* sp = B.skel_pass.data() + B.skel_off[b];
-            int* tb = elem_on ? ctab.tab.data() + ctab.off[b] : nullptr;
-            const int ex = hi[0] - lo[0] + 1, ey = hi[1] - lo[1] + 1;
-            int pr = 0, ps = B.r[b], po = B.r[b] + B.s[b];
-            for (int i = lo[0]; i <= hi[0]; ++i)
-                for (int j = lo[1]; j <= hi[1]; ++j)
-                    for (int k = lo[2]; k <= hi[2]; ++k) {
-                        int g = lin(i, j, k);
-                        if (!remaining[g]) continue;
-                        int ni = I[i] + I[j] + I[k];
-                        int pos;
-                        if (ni == 3) {
-                            *rg++ = g;
-                            pos = pr++;
-                        } else if (ni == 2) {
-                            *sg++ = g;
-                            int x = !I[i] ? i : (!I[j] ? j : k);
-                            int a = !I[i] ? 0 : (!I[j] ? 1 : 2);
-                            *sp++ = (x == w * cc[a]) ? 1 : 0;  // on a min face
-                            pos = ps++;
-                        } else {
-                            *og++ = g;
-                            pos = po++;
-                        }
-                        if (tb) tb[(i - lo[0]) + ex * ((j - lo[1]) + ey * (k - lo[2]))] = pos;
+#pragma omp parallel
+        {
+            std::vector<std::pair<long long, int>> pts;  // (scan key, gid)
+#pragma omp for schedule(dynamic, 16)
+            for (int b = 0; b < B.nbox; ++b) {
+                int cc[3] = {b / (c * c), (b / c) % c, b % c};
+                int lo[3], hi[3];
+                for (int a = 0; a < 3; ++a) box_range(cc[a], lo[a], hi[a]);
+                const long long ey = hi[1] - lo[1] + 1, ez = hi[2] - lo[2] + 1;
+                pts.clear();
+                for (int k = lo[2]; k <= hi[2]; ++k)
+                    for (int j = lo[1]; j <= hi[1]; ++j) {
+                        const int g0 = lin(0, j, k);
+                        const unsigned char* row = remaining.data() + g0;
+                        for (int i = lo[0]; i <= hi[0]; ++i)
+                            if (row[i])
+                                pts.push_back({((i - lo[0]) * ey + (j - lo[1])) * ez + (k - lo[2]),
+                                               g0 + i});
                     }
+                std::sort(pts.begin(), pts.end());
+                int* rg = B.red_gid.data() + B.red_off[b];
+                int* sg = B.skel_gid.data() + B.skel_off[b];
+                int* og = B.other_gid.data() + B.other_off[b];
+                unsigned char* sp = B.skel_pass.data() + B.skel_off[b];
+                int* tb = elem_on ? ctab.tab.data() + ctab.off[b] : nullptr;
+                const int ex = hi[0] - lo[0] + 1;
+                int pr = 0, ps = B.r[b], po = B.r[b] + B.s[b];
+                for (const auto& pt : pts) {
+                    const int g = pt.second;
+                    const int i = g % NC, j = (g / NC) % NC, k = g / (NC * NC);
+                    int ni = I[i] + I[j] + I[k];
+                    int pos;
+                    if (ni == 3) {
+                        *rg++ = g;
+                        pos = pr++;
+                    } else if (ni == 2) {
+                        *sg++ = g;
+                        int x = !I[i] ? i : (!I[j] ? j : k);
+                        int a = !I[i] ? 0 : (!I[j] ? 1 : 2);
+                        *sp++ = (x == w * cc[a]) ? 1 : 0;  // on a min face
+                        pos = ps++;
+                    } else {
+                        *og++ = g;
+                        pos = po++;
+                    }
+                    if (tb) tb[(i - lo[0]) + ex * ((j - lo[1]) + ey * (k - lo[2]))] = pos;
+                }
+            }
         }
         B.off_rr = prefix_ll(B.r, B.r);
         B.off_rs = prefix_ll(B.r, B.s);
@@ -622,10 +643,12 @@ struct hifde_factor {
     void level_cells(int level) {
         auto t0 = clk::now();
         BoxBatch& B = cells[level];
-        plan_cells(level, B);
+        // the level-0 cells depend on the grid only (kept with the element storage)
+        const bool cached = level == 0 && elem_on && plan0_valid;
+        if (!cached) plan_cells(level, B);
         B.esize = sizeof(T);
         timed(ST_PLAN, t0);
-        upload_batch_index(B);
+        if (!cached) upload_batch_index(B);
         timed(ST_UPLOAD, t0);
         if (elem_on) {
             level_cells_elem<T>(level, t0);
@@ -664,6 +687,43 @@ struct hifde_factor {
         scatter<T>(B, S, true, t0);
     }
 
+    // Level-0 frontal assembly index: per cell its DOF list [red | skel |
+    // other], position table and block offsets, on the device.
+    void init_level0_index(const BoxBatch& B) {
+        const int nb = B.nbox, c = NC / P;
+        std::vector<InitDesc> desc(nb);
+        std::vector<int> gid_off(nb + 1, 0);
+        for (int b = 0; b < nb; ++b) gid_off[b + 1] = gid_off[b] + B.r[b] + B.s[b] + B.o[b];
+        std::vector<int> gids(gid_off[nb]);
+#pragma omp parallel for schedule(static)
+        for (int b = 0; b < nb; ++b) {
+            InitDesc& d = desc[b];
+            d.rr = B.off_rr[b];
+            d.rs = B.off_rs[b];
+            d.ee = B.off_ee[b];
+            d.tab = ctab.off[b];
+            d.gid = gid_off[b];
+            d.r = B.r[b];
+            d.s = B.s[b];
+            d.o = B.o[b];
+            const int cc[3] = {b / (c * c), (b / c) % c, b % c};
+            for (int a = 0; a < 3; ++a) {
+                d.lo[a] = ctab.lo[3 * b + a];
+                d.ext[a] = ctab.ext[3 * b + a];
+                d.cc[a] = cc[a];
+            }
+            int* g = gids.data() + gid_off[b];
+            g = std::copy(B.red_gid.begin() + B.red_off[b], B.red_gid.begin() + B.red_off[b + 1], g);
+            g = std::copy(B.skel_gid.begin() + B.skel_off[b], B.skel_gid.begin() + B.skel_off[b + 1],
+                          g);
+            std::copy(B.other_gid.begin() + B.other_off[b], B.other_gid.begin() + B.other_off[b + 1],
+                      g);
+        }
+        d_init.upload(desc, stream);
+        d_etab.upload(ctab.tab, stream);
+        d_egid.upload(gids, stream);
+    }
+
     // Cells of a level with the element storage: the frontal blocks A22,
     // A21 (A12) and the element over [skel | other] are assembled from the
     // operator (level 0) or by extend-add of the eight children's elements;
@@ -686,39 +746,9 @@ struct hifde_factor {
             ((double)B.tot_rr + (sym_on ? 1.0 : 2.0) * B.tot_rs) * w + (double)B.tot_ee * 8;
         if (level == 0) {
             const int c = NC / P;
-            std::vector<InitDesc> desc(nb);
-            std::vector<int> gid_off(nb + 1, 0);
-            for (int b = 0; b < nb; ++b) gid_off[b + 1] = gid_off[b] + B.r[b] + B.s[b] + B.o[b];
-            std::vector<int> gids(gid_off[nb]);
-#pragma omp parallel for schedule(static)
-            for (int b = 0; b < nb; ++b) {
-                InitDesc& d = desc[b];
-                d.rr = B.off_rr[b];
-                d.rs = B.off_rs[b];
-                d.ee = B.off_ee[b];
-                d.tab = ctab.off[b];
-                d.gid = gid_off[b];
-                d.r = B.r[b];
-                d.s = B.s[b];
-                d.o = B.o[b];
-                const int cc[3] = {b / (c * c), (b / c) % c, b % c};
-                for (int a = 0; a < 3; ++a) {
-                    d.lo[a] = ctab.lo[3 * b + a];
-                    d.ext[a] = ctab.ext[3 * b + a];
-                    d.cc[a] = cc[a];
-                }
-                int* g = gids.data() + gid_off[b];
-                g = std::copy(B.red_gid.begin() + B.red_off[b], B.red_gid.begin() + B.red_off[b + 1], g);
-                g = std::copy(B.skel_gid.begin() + B.skel_off[b],
-                              B.skel_gid.begin() + B.skel_off[b + 1], g);
-                std::copy(B.other_gid.begin() + B.other_off[b],
-                          B.other_gid.begin() + B.other_off[b + 1], g);
-            }
-            d_etab.upload(ctab.tab, stream);
-            d_egid.upload(gids, stream);
-            timed(ST_UPLOAD, t0);
+            if (!plan0_valid) init_level0_index(B);
             Prof pr(this, fam<T>("assemble"), 0.0, fr_bytes + (double)h_ci.size() * 12.0);
-            frontal_init<T>(desc, d_init, d_etab.p, d_egid.p, d_rp.p, d_ci.p, d_v.p, NC, P, c,
+            frontal_init<T>(nb, d_init.p, d_etab.p, d_egid.p, d_rp.p, d_ci.p, d_v.p, NC, P, c,
                             A22, A21, A12, E, stream);
         } else {
             const BoxBatch& Cb = cells[level - 1];
@@ -786,95 +816,127 @@ struct hifde_factor {
         mark_eliminated(B);
     }
 
+    // Index of one level's faces: the plan and its device copies.
+    struct FaceDev {
+        FacePlan F;
+        std::vector<long long> offM, offC;
+        std::vector<int> cpos1;  // element position of each face DOF in the face's own cell
+        DevBuf<int> rows, rows_off, cols, cols_off, m, n;
+        DevBuf<long long> offMd, offCd;
+        DevBuf<unsigned> rpos, cpos;  // packed positions in the two adjacent elements
+        DevBuf<FaceGatherDesc> gd;
+        DevBuf<long long> gd_off;
+        long long gd_cols = 0;
+    };
+    FaceDev fdev[2];  // [0]: level 0, kept while plan0_valid; [1]: the other levels
+
+    // Element positions of the face rows and columns in the two adjacent
+    // cells, packed as low / high halves (kNoPos when absent).
+    void face_positions(int level, FaceDev& fd) {
+        const FacePlan& F = fd.F;
+        const BoxBatch& Cb = cells[level];
+        const int nf = F.nface;
+        std::vector<unsigned> rpos(F.rows_off.back()), cpos(F.cols_off.back());
+        fd.cpos1.assign(F.cols_off.back(), 0);
+        std::vector<FaceGatherDesc> gd(nf);
+        int bad = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(max : bad)
+        for (int f = 0; f < nf; ++f) {
+            const int c1 = F.c1[f], c2 = F.c2[f], m = F.m[f], n = F.n[f];
+            auto epos = [&](int cell, int g) -> unsigned {
+                const int p = cell_pos(cell, g);
+                return p >= Cb.r[cell] ? (unsigned)(p - Cb.r[cell]) : kNoPos;
+            };
+            auto pack = [&](unsigned p1, unsigned p2) {
+                if ((p1 != kNoPos && p1 >= kNoPos) || (p2 != kNoPos && p2 >= kNoPos)) bad = 1;
+                return (p1 & 0xFFFFu) | (p2 << 16);
+            };
+            for (int i = 0; i < m; ++i) {
+                const int g = F.rows[F.rows_off[f] + i];
+                rpos[F.rows_off[f] + i] = pack(epos(c1, g), epos(c2, g));
+            }
+            for (int j = 0; j < n; ++j) {
+                const int g = F.cols[F.cols_off[f] + j];
+                const unsigned p1 = epos(c1, g), p2 = epos(c2, g);
+                if (p1 == kNoPos || p2 == kNoPos) bad = 2;
+                cpos[F.cols_off[f] + j] = pack(p1, p2);
+                fd.cpos1[F.cols_off[f] + j] = (int)p1;
+            }
+            FaceGatherDesc& d = gd[f];
+            d.e1 = Cb.off_ee[c1];
+            d.e2 = Cb.off_ee[c2];
+            d.offM = fd.offM[f];
+            d.offC = fd.offC[f];
+            d.ld1 = Cb.s[c1] + Cb.o[c1];
+            d.ld2 = Cb.s[c2] + Cb.o[c2];
+            d.m = m;
+            d.n = n;
+            d.rows = F.rows_off[f];
+            d.cols = F.cols_off[f];
+        }
+        if (bad == 1) throw Error(HIFDE_ERR_SIZE, "element larger than 65535 DOFs");
+        if (bad == 2) throw Error(HIFDE_ERR_STATE, "face DOF outside its cells");
+        fd.rpos.upload(rpos, stream);
+        fd.cpos.upload(cpos, stream);
+        fd.gd_cols = face_gather_prepare(gd, fd.gd, fd.gd_off, stream);
+    }
+
     template <class T>
     void level_faces(int level) {
         auto t0 = clk::now();
-        FacePlan F;
-        plan_faces(level, F);
+        // The level-0 faces depend on the grid only: with the element
+        // storage their index is built once and kept.
+        const bool cached = level == 0 && elem_on && plan0_valid;
+        FaceDev& fd = fdev[level == 0 ? 0 : 1];
+        const FacePlan& F = fd.F;
+        if (!cached) {
+            plan_faces(level, fd.F);
+            fd.offM = prefix_ll(F.m, F.n);
+            fd.offC = prefix_ll(F.n, F.n);
+        }
         timed(ST_PLAN, t0);
         const int nf = F.nface;
         BoxBatch& B = faces[level];
-        B = BoxBatch();
+        B.reset_host();
         B.nbox = nf;
         B.esize = sizeof(T);
         if (nf == 0) return;
-        std::vector<long long> offM = prefix_ll(F.m, F.n), offC = prefix_ll(F.n, F.n);
-        int max_n = 0, max_m = 0;
-        for (int f = 0; f < nf; ++f) {
-            max_n = std::max(max_n, F.n[f]);
-            max_m = std::max(max_m, F.m[f]);
+        const std::vector<long long>& offM = fd.offM;
+        const std::vector<long long>& offC = fd.offC;
+        if (!cached) {
+            if (!elem_on) {
+                fd.rows.upload(F.rows, stream);
+                fd.rows_off.upload(F.rows_off, stream);
+                fd.cols.upload(F.cols, stream);
+            }
+            fd.cols_off.upload(F.cols_off, stream);
+            fd.m.upload(F.m, stream);
+            fd.n.upload(F.n, stream);
+            fd.offMd.upload(offM, stream);
+            fd.offCd.upload(offC, stream);
+            if (elem_on) face_positions(level, fd);
+            if (level == 0 && elem_on) plan0_valid = true;
         }
-        ws_rows.upload(F.rows, stream);
-        ws_rows_off.upload(F.rows_off, stream);
-        ws_cols.upload(F.cols, stream);
-        ws_cols_off.upload(F.cols_off, stream);
-        ws_m.upload(F.m, stream);
-        ws_n.upload(F.n, stream);
-        ws_offM.upload(offM, stream);
-        ws_offC.upload(offC, stream);
         timed(ST_UPLOAD, t0);
         ws_M.reserve(offM.back());
         ws_C.reserve(offC.back());
         ws_W.reserve(offC.back());
         ws_T.reserve(offC.back());
-        std::vector<int> cpos1;  // element positions of the face DOFs in the owner cell
         if (elem_on) {
             // A(rows, F) and A(F, F) summed from the two adjacent cells' elements
-            const BoxBatch& Cb = cells[level];
-            std::vector<int> rpos(2 * (size_t)F.rows_off.back()), cpos(2 * (size_t)F.cols_off.back());
-            cpos1.resize(F.cols_off.back());
-            std::vector<FaceGatherDesc> gd(nf);
-#pragma omp parallel for schedule(dynamic, 16)
-            for (int f = 0; f < nf; ++f) {
-                const int c1 = F.c1[f], c2 = F.c2[f], m = F.m[f], n = F.n[f];
-                auto epos = [&](int cell, int g) {
-                    const int p = cell_pos(cell, g);
-                    return p >= Cb.r[cell] ? p - Cb.r[cell] : -1;
-                };
-                int* r1 = rpos.data() + 2 * (size_t)F.rows_off[f];
-                int* q1 = cpos.data() + 2 * (size_t)F.cols_off[f];
-                for (int i = 0; i < m; ++i) {
-                    const int g = F.rows[F.rows_off[f] + i];
-                    r1[i] = epos(c1, g);
-                    r1[m + i] = epos(c2, g);
-                }
-                for (int j = 0; j < n; ++j) {
-                    const int g = F.cols[F.cols_off[f] + j];
-                    q1[j] = epos(c1, g);
-                    q1[n + j] = epos(c2, g);
-                    cpos1[F.cols_off[f] + j] = q1[j];
-                }
-                FaceGatherDesc& d = gd[f];
-                d.e1 = Cb.off_ee[c1];
-                d.e2 = Cb.off_ee[c2];
-                d.offM = offM[f];
-                d.offC = offC[f];
-                d.ld1 = Cb.s[c1] + Cb.o[c1];
-                d.ld2 = Cb.s[c2] + Cb.o[c2];
-                d.m = m;
-                d.n = n;
-                d.rows = 2 * F.rows_off[f];
-                d.cols = 2 * F.cols_off[f];
-            }
-            for (int x : cpos)
-                if (x < 0) throw Error(HIFDE_ERR_STATE, "face DOF outside its cells");
-            d_rpos.upload(rpos, stream);
-            d_cpos.upload(cpos, stream);
-            timed(ST_UPLOAD, t0);
-            Prof pr(this, "gather_f64", 0.0,
-                    (double)(offM.back() + offC.back()) * (8 + 2.0 * 8));
-            face_gather(gd, d_fg, d_eoff, d_rpos.p, d_cpos.p, elem[elem_cur].p, ws_M.p, ws_C.p,
-                        stream);
+            Prof pr(this, "gather_f64", 0.0, (double)(offM.back() + offC.back()) * (8 + 2.0 * 8));
+            face_gather(nf, fd.gd_cols, fd.gd.p, fd.gd_off.p, fd.rpos.p, fd.cpos.p,
+                        elem[elem_cur].p, ws_M.p, ws_C.p, stream);
         } else {
             {
                 Prof pr(this, "gather_f64", 0.0, (double)offM.back() * (8 + 16));
-                table_gather<double>(tabs[cur], nf, ws_rows.p, ws_rows_off.p, ws_cols.p,
-                                     ws_cols_off.p, ws_offM.p, offM.back(), ws_M.p, stream);
+                table_gather<double>(tabs[cur], nf, fd.rows.p, fd.rows_off.p, fd.cols.p,
+                                     fd.cols_off.p, fd.offMd.p, offM.back(), ws_M.p, stream);
             }
             {
                 Prof pr(this, "gather_f64", 0.0, (double)offC.back() * (8 + 16));
-                table_gather<double>(tabs[cur], nf, ws_cols.p, ws_cols_off.p, ws_cols.p,
-                                     ws_cols_off.p, ws_offC.p, offC.back(), ws_C.p, stream);
+                table_gather<double>(tabs[cur], nf, fd.cols.p, fd.cols_off.p, fd.cols.p,
+                                     fd.cols_off.p, fd.offCd.p, offC.back(), ws_C.p, stream);
             }
         }
         timed(ST_GATHER, t0);
@@ -927,14 +989,14 @@ struct hifde_factor {
             }
             {
                 Prof pr(this, "gram_pivchol", fl_c, 8.0 * offC.back());
-                gram_interp_decomp(nf, ws_n.p, ws_mode.p, ws_G.p, ws_offC.p, ws_T.p, eps,
-                                   ws_perm.p, ws_cols_off.p, ws_k.p, ws_W.p, max_n_gram, stream);
+                gram_interp_decomp(nf, fd.n.p, ws_mode.p, ws_G.p, fd.offCd.p, ws_T.p, eps,
+                                   ws_perm.p, fd.cols_off.p, ws_k.p, ws_W.p, max_n_gram, stream);
             }
         }
         if (max_n_house > 0 || (int)gm.size() < nf) {
             Prof pr(this, "interp_decomp", fl_h, by_h);
-            batched_interp_decomp(nf, ws_m.p, ws_n.p, ws_offM.p, ws_M.p, eps, ws_perm.p,
-                                  ws_cols_off.p, ws_k.p, ws_W.p, ws_offC.p, max_n_house,
+            batched_interp_decomp(nf, fd.m.p, fd.n.p, fd.offMd.p, ws_M.p, eps, ws_perm.p,
+                                  fd.cols_off.p, ws_k.p, ws_W.p, fd.offCd.p, max_n_house,
                                   max_m_house, ws_mode.p, stream);
         }
         timed(ST_ID, t0);
@@ -989,11 +1051,11 @@ struct hifde_factor {
                 by += 8 * (n * n + k * q) + sizeof(T) * (q * q + 2 * k * q);
             }
             Prof pr(this, "face_transform", fl, by);
-            face_transform<T>(F, offC, B, A21, A12);
+            face_transform<T>(fd, B, A21, A12);
         }
         // W (k x q) kept for Apply, packed at the r*s offsets.
         B.W.reserve(B.tot_rs * sizeof(T));
-        pack_w<T>(B);
+        pack_w<T>(B, fd);
         timed(ST_TRANSFORM, t0);
         schur_batch<T>(B, A21, A12, S, B.off_ss.data(), B.s.data(), false, t0);
         mark_eliminated(B);
@@ -1005,7 +1067,7 @@ struct hifde_factor {
             for (int f = 0; f < nf; ++f) {
                 const int* p = perm.data() + F.cols_off[f];
                 for (int t = 0; t < B.s[f]; ++t)
-                    spos[B.skel_off[f] + t] = cpos1[F.cols_off[f] + p[t]];
+                    spos[B.skel_off[f] + t] = fd.cpos1[F.cols_off[f] + p[t]];
                 if (B.s[f] == 0) continue;
                 const int c1 = F.c1[f];
                 ad.push_back({Cb.off_ee[c1], B.off_ss[f], Cb.s[c1] + Cb.o[c1], B.s[f],
@@ -1028,11 +1090,12 @@ struct hifde_factor {
     // C is first permuted to skeleton-then-redundant order; all products are
     // FP64 batched GEMMs on sub-blocks of it; the blocks are then packed in T.
     template <class T>
-    void face_transform(const FacePlan& F, const std::vector<long long>& offC, BoxBatch& B,
-                        T* A21, T* A12) {
+    void face_transform(const FaceDev& fd, BoxBatch& B, T* A21, T* A12) {
+        const FacePlan& F = fd.F;
+        const std::vector<long long>& offC = fd.offC;
         const int nf = B.nbox;
         ws_T.reserve(offC.back());
-        permute_sym(nf, ws_offC.p, ws_n.p, ws_perm.p, ws_cols_off.p, ws_C.p, ws_T.p, offC.back(),
+        permute_sym(nf, fd.offCd.p, fd.n.p, ws_perm.p, fd.cols_off.p, ws_C.p, ws_T.p, offC.back(),
                     stream);
         std::vector<GemmDesc> d1, d2, d3, d4;
         std::vector<CopyDesc> c22(nf), c21(nf), c12(nf);
@@ -1058,10 +1121,10 @@ struct hifde_factor {
     }
 
     template <class T>
-    void pack_w(BoxBatch& B) {
+    void pack_w(BoxBatch& B, const FaceDev& fd) {
         // Each face's W (k x q) from the n*n-strided ID workspace into the
         // packed r*s layout, converted to the factor precision.
-        pack_boxes<T>(B.nbox, B.d_off_rs.p, ws_offC.p, ws_W.p, reinterpret_cast<T*>(B.W.p),
+        pack_boxes<T>(B.nbox, B.d_off_rs.p, fd.offCd.p, ws_W.p, reinterpret_cast<T*>(B.W.p),
                       B.tot_rs, stream);
     }
 
@@ -1076,6 +1139,7 @@ struct hifde_factor {
     void run_factor() {
         if (!have_matrix) throw Error(HIFDE_ERR_STATE, "factor: no matrix set");
         for (double& t : stage_t) t = 0.0;
+        stage_lt.assign((size_t)L * ST_COUNT, 0.0);
         lu_info = 0;
         stager.stream = stream;
         struct StagerGuard {
@@ -1090,13 +1154,12 @@ struct hifde_factor {
         for (int k = 1; k < NC; ++k)
             for (int j = 1; j < NC; ++j)
                 for (int i = 1; i < NC; ++i) remaining[lin(i, j, k)] = 1;
-        cells.clear();
-        faces.clear();
         cells.resize(L);
         faces.resize(L);
         factored = false;
         sym_on = op_symmetric && sym_option != 0;
         elem_on = local_ok && elem_option != 0;
+        if (!elem_on) plan0_valid = false;
         elem_cur = 0;
         if (!elem_on) {
             // Initialize(): the level matrix starts as the assembled operator.
@@ -1111,6 +1174,7 @@ struct hifde_factor {
             timed(ST_UPLOAD, t0);
         }
         for (int level = 0; level < L; ++level) {
+            cur_level = level;
             if (level < fp32_levels_eff()) {
                 level_cells<float>(level);
                 if (level < L - 1) level_faces<float>(level);
@@ -1674,6 +1738,17 @@ int hifde_stage_times(const hifde_factor* f, double* out, int nout) {
     if (!f || !out) return HIFDE_ERR_ARG;
     int n = std::min(nout, (int)ST_COUNT);
     for (int i = 0; i < n; ++i) out[i] = f->stage_t[i];
+    return n;
+}
+
+int hifde_level_stage_times(const hifde_factor* f, int level, double* out, int nout) {
+    if (!f || !out || nout < 0) return HIFDE_ERR_ARG;
+    if (level < 0 || level >= f->L) return HIFDE_ERR_ARG;
+    const int n = std::min(nout, (int)ST_COUNT);
+    for (int i = 0; i < n; ++i)
+        out[i] = (size_t)(level * ST_COUNT + i) < f->stage_lt.size()
+                     ? f->stage_lt[level * ST_COUNT + i]
+                     : 0.0;
     return n;
 }
 

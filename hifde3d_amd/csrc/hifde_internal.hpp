@@ -128,6 +128,18 @@ struct BoxBatch {
     std::vector<int> o, other_off, other_gid;
     std::vector<long long> off_ee;
     long long tot_ee = 0;
+    // Clears the index description; device buffers are kept for reuse.
+    void reset_host() {
+        nbox = 0;
+        r.clear(); s.clear(); o.clear();
+        off_rr.clear(); off_rs.clear(); off_ss.clear(); off_ee.clear();
+        red_off.clear(); skel_off.clear(); other_off.clear();
+        red_gid.clear(); skel_gid.clear(); other_gid.clear(); skel_pass.clear();
+        tot_rr = tot_rs = tot_ss = tot_ee = 0;
+        tiles_r.clear(); tiles_s.clear();
+        max_r = max_s = 0;
+        face_kind.clear();
+    }
 };
 
 // Element storage descriptors (level_elem.hip).
@@ -145,6 +157,7 @@ struct ExtendDesc {          // one (parent, child) pair of an extend-add pass
     int map;                 // offset of the child's parent-position map
     int r, s, ep;            // parent red / skel counts and element size
 };
+constexpr unsigned kNoPos = 0xFFFFu;  // absent position in a packed pair
 struct FaceGatherDesc {      // one face
     long long e1, e2;        // elements of the two adjacent cells
     long long offM, offC;
@@ -265,16 +278,21 @@ void pack_boxes(int nbox, const long long* d_dst_off, const long long* d_src_off
 template <class T>
 void fill_zero(T* p, long long n, hipStream_t s);
 template <class T>
-void frontal_init(const std::vector<InitDesc>& desc, DevBuf<InitDesc>& dbuf, const int* d_tab,
-                  const int* d_gids, const int* d_rp, const int* d_ci, const double* d_v, int NC,
-                  int w, int c, T* A22, T* A21, T* A12, double* E, hipStream_t s);
+void frontal_init(int ncell, const InitDesc* d_desc, const int* d_tab, const int* d_gids,
+                  const int* d_rp, const int* d_ci, const double* d_v, int NC, int w, int c,
+                  T* A22, T* A21, T* A12, double* E, hipStream_t s);
 template <class Td>
 void extend_add(const std::vector<ExtendDesc>& desc, DevBuf<ExtendDesc>& dbuf,
                 DevBuf<long long>& obuf, const int* d_maps, const double* src, Td* A22, Td* A21,
                 Td* A12, double* E, hipStream_t s);
-void face_gather(const std::vector<FaceGatherDesc>& desc, DevBuf<FaceGatherDesc>& dbuf,
-                 DevBuf<long long>& obuf, const int* d_rpos, const int* d_cpos, const double* E,
-                 double* M, double* C, hipStream_t s);
+// Uploads the face descriptors and their column prefix; returns the number of
+// face columns (the launch size of face_gather).
+long long face_gather_prepare(const std::vector<FaceGatherDesc>& desc,
+                              DevBuf<FaceGatherDesc>& dbuf, DevBuf<long long>& obuf,
+                              hipStream_t s);
+void face_gather(int nface, long long total_cols, const FaceGatherDesc* d_desc,
+                 const long long* d_off, const unsigned* d_rpos, const unsigned* d_cpos,
+                 const double* E, double* M, double* C, hipStream_t s);
 template <class T>
 void face_add(const std::vector<FaceAddDesc>& desc, DevBuf<FaceAddDesc>& dbuf,
               DevBuf<long long>& obuf, const int* d_spos, const T* S, double* E, int sym,

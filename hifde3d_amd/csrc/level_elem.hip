@@ -166,32 +166,35 @@ __global__ void __launch_bounds__(128) k_extend_add(int npar, const long long* _
 // M(i, j) = A(rows_i, cols_j) and C(i, j) = A(cols_i, cols_j) of every face,
 // in FP64, summed from the two adjacent cells' elements.  Workgroup = one
 // face column.
+// Positions are packed two to a word: the low half is the position in the
+// first cell's element, the high half in the second's, kNoPos when absent.
 __global__ void __launch_bounds__(128) k_face_gather(int nface, const long long* __restrict__ coff,
                                                      const FaceGatherDesc* __restrict__ desc,
-                                                     const int* __restrict__ rpos,
-                                                     const int* __restrict__ cpos,
+                                                     const unsigned* __restrict__ rpos,
+                                                     const unsigned* __restrict__ cpos,
                                                      const double* __restrict__ E, double* M,
                                                      double* C) {
     const long long blk = blockIdx.x;
     const int f = find_slot(coff, nface, blk);
     const FaceGatherDesc d = desc[f];
     const int j = (int)(blk - coff[f]);
-    // position arrays: [pos in cell 1 | pos in cell 2] for the rows and the columns
-    const int* r1 = rpos + d.rows;
-    const int* r2 = r1 + d.m;
-    const int* c1 = cpos + d.cols;
-    const int* c2 = c1 + d.n;
-    const double* e1 = E + d.e1 + (long long)c1[j] * d.ld1;
-    const double* e2 = E + d.e2 + (long long)c2[j] * d.ld2;
+    const unsigned* rp = rpos + d.rows;
+    const unsigned* cp = cpos + d.cols;
+    const unsigned cj = cp[j];
+    const double* e1 = E + d.e1 + (long long)(cj & 0xFFFFu) * d.ld1;
+    const double* e2 = E + d.e2 + (long long)(cj >> 16) * d.ld2;
     for (int i = threadIdx.x; i < d.m; i += blockDim.x) {
-        const int p1 = r1[i], p2 = r2[i];
-        double x = 0.0;
-        if (p1 >= 0) x += e1[p1];
-        if (p2 >= 0) x += e2[p2];
-        M[d.offM + i + (long long)j * d.m] = x;
+        const unsigned x = rp[i];
+        const unsigned p1 = x & 0xFFFFu, p2 = x >> 16;
+        double v = 0.0;
+        if (p1 != kNoPos) v += e1[p1];
+        if (p2 != kNoPos) v += e2[p2];
+        M[d.offM + i + (long long)j * d.m] = v;
     }
-    for (int i = threadIdx.x; i < d.n; i += blockDim.x)
-        C[d.offC + i + (long long)j * d.n] = e1[c1[i]] + e2[c2[i]];
+    for (int i = threadIdx.x; i < d.n; i += blockDim.x) {
+        const unsigned x = cp[i];
+        C[d.offC + i + (long long)j * d.n] = e1[x & 0xFFFFu] + e2[x >> 16];
+    }
 }
 
 // E_owner(skel_f, skel_f) += S_f.  Workgroup = one face column.  With sym,
@@ -267,13 +270,12 @@ void fill_zero(T* p, long long n, hipStream_t s) {
 }
 
 template <class T>
-void frontal_init(const std::vector<InitDesc>& desc, DevBuf<InitDesc>& dbuf, const int* d_tab,
-                  const int* d_gids, const int* d_rp, const int* d_ci, const double* d_v, int NC,
-                  int w, int c, T* A22, T* A21, T* A12, double* E, hipStream_t s) {
-    if (desc.empty()) return;
-    dbuf.upload(desc, s);
-    k_frontal_init<T><<<(unsigned)desc.size(), 256, 0, s>>>(dbuf.p, d_tab, d_gids, d_rp, d_ci,
-                                                            d_v, NC, w, c, A22, A21, A12, E);
+void frontal_init(int ncell, const InitDesc* d_desc, const int* d_tab, const int* d_gids,
+                  const int* d_rp, const int* d_ci, const double* d_v, int NC, int w, int c,
+                  T* A22, T* A21, T* A12, double* E, hipStream_t s) {
+    if (ncell <= 0) return;
+    k_frontal_init<T><<<(unsigned)ncell, 256, 0, s>>>(d_desc, d_tab, d_gids, d_rp, d_ci, d_v, NC,
+                                                      w, c, A22, A21, A12, E);
     HIFDE_LAUNCH_CHECK();
 }
 
@@ -292,17 +294,23 @@ void extend_add(const std::vector<ExtendDesc>& desc, DevBuf<ExtendDesc>& dbuf,
     HIFDE_LAUNCH_CHECK();
 }
 
-void face_gather(const std::vector<FaceGatherDesc>& desc, DevBuf<FaceGatherDesc>& dbuf,
-                 DevBuf<long long>& obuf, const int* d_rpos, const int* d_cpos, const double* E,
-                 double* M, double* C, hipStream_t s) {
-    if (desc.empty()) return;
+long long face_gather_prepare(const std::vector<FaceGatherDesc>& desc,
+                              DevBuf<FaceGatherDesc>& dbuf, DevBuf<long long>& obuf,
+                              hipStream_t s) {
+    if (desc.empty()) return 0;
     const std::vector<long long> off =
         col_prefix<FaceGatherDesc>(desc, [](const FaceGatherDesc& d) { return d.n; });
-    if (off.back() == 0) return;
     dbuf.upload(desc, s);
     obuf.upload(off, s);
-    k_face_gather<<<(unsigned)off.back(), 128, 0, s>>>((int)desc.size(), obuf.p, dbuf.p, d_rpos,
-                                                       d_cpos, E, M, C);
+    return off.back();
+}
+
+void face_gather(int nface, long long total_cols, const FaceGatherDesc* d_desc,
+                 const long long* d_off, const unsigned* d_rpos, const unsigned* d_cpos,
+                 const double* E, double* M, double* C, hipStream_t s) {
+    if (nface <= 0 || total_cols <= 0) return;
+    k_face_gather<<<(unsigned)total_cols, 128, 0, s>>>(nface, d_off, d_desc, d_rpos, d_cpos, E,
+                                                       M, C);
     HIFDE_LAUNCH_CHECK();
 }
 
@@ -337,9 +345,9 @@ void mirror_lower(const std::vector<MirrorDesc>& desc, DevBuf<MirrorDesc>& dbuf,
 
 #define HIFDE_INST1(T)                                                                          \
     template void fill_zero<T>(T*, long long, hipStream_t);                                     \
-    template void frontal_init<T>(const std::vector<InitDesc>&, DevBuf<InitDesc>&, const int*,  \
-                                  const int*, const int*, const int*, const double*, int, int,  \
-                                  int, T*, T*, T*, double*, hipStream_t);                       \
+    template void frontal_init<T>(int, const InitDesc*, const int*, const int*, const int*,      \
+                                  const int*, const double*, int, int, int, T*, T*, T*,         \
+                                  double*, hipStream_t);                                        \
     template void extend_add<T>(const std::vector<ExtendDesc>&, DevBuf<ExtendDesc>&,            \
                                 DevBuf<long long>&, const int*, const double*, T*, T*, T*,      \
                                 double*, hipStream_t);                                          \

@@ -108,6 +108,8 @@ int hifde_box_data(const hifde_factor* f, int level, int kind, int box, int* red
 /* Wall-clock seconds of the last factor by stage; returns the number of
  * stages written (names from hifde_stage_name). */
 int hifde_stage_times(const hifde_factor* f, double* out, int nout);
+/* The same, for one level of the last factor. */
+int hifde_level_stage_times(const hifde_factor* f, int level, double* out, int nout);
 const char* hifde_stage_name(int stage);
 /* Tuning options (no reference counterpart):
  *   "gram_min_n"   faces with more ID columns than this use the Gram-matrix
@@ -122,6 +124,9 @@ const char* hifde_stage_name(int stage);
  *                  Schur complement S = -A21^T A22^{-1} A21 (and store the
  *                  hash-table level matrix by unordered pairs); 0: general
  *                  path
+ *   "stage_sync"   1: drain the stream at every stage boundary so that the
+ *                  stage times include device time; default 0 (stage times
+ *                  are host time and the host runs ahead of the device)
  *   "elements"     -1 (default): when every coupling of the operator lies
  *                  inside one level-0 cell closure (all stencils), hold the
  *                  level matrix as per-cell dense elements assembled by
