@@ -419,7 +419,8 @@ template <class T, bool TRANSA, class TC = T>
 __global__ void __launch_bounds__(256) k_gemm(int nbox, const int* __restrict__ tile_off,
                                               const GemmDesc* __restrict__ desc,
                                               const T* __restrict__ Ab, const T* __restrict__ Bb,
-                                              TC* Cb, T alpha, T beta) {
+                                              TC* Cb, T alpha, T beta,
+                                              const int* __restrict__ flags) {
     const int t = blockIdx.x;
     const int b = find_box(tile_off, nbox, t);
     const int local = t - tile_off[b];
@@ -428,6 +429,7 @@ __global__ void __launch_bounds__(256) k_gemm(int nbox, const int* __restrict__ 
     const int tm = (m + BM - 1) / BM;
     const int i0 = (local % tm) * BM, j0 = (local / tm) * BN;
     if (dd.lower && j0 > i0) return;
+    if (dd.flag >= 0 && flags && flags[dd.flag] == 0) return;
     const T* A = Ab + dd.offA;
     const T* B = Bb + dd.offB;
     TC* C = Cb + dd.offC;
@@ -544,7 +546,7 @@ __global__ void __launch_bounds__(256, 2) k_gemm_mfma(int nbox, const int* __res
                                                       const GemmDesc* __restrict__ desc,
                                                       const T* __restrict__ Ab,
                                                       const T* __restrict__ Bb, TC* Cb, T alpha,
-                                                      T beta) {
+                                                      T beta, const int* __restrict__ flags) {
     typedef MfmaCfg<T> C;
     constexpr int MT = C::MT, KT = C::KT, BLK = C::BLK, WT = BLK * MT, TILE = 2 * WT, KB = 16;
     static_assert(TILE == kMfmaTile, "tile");
@@ -558,6 +560,7 @@ __global__ void __launch_bounds__(256, 2) k_gemm_mfma(int nbox, const int* __res
     const int tm = (m + TILE - 1) / TILE;
     const int i0 = (local % tm) * TILE, j0 = (local / tm) * TILE;
     if (dd.lower && j0 > i0) return;
+    if (dd.flag >= 0 && flags && flags[dd.flag] == 0) return;
     const T* A = Ab + dd.offA;
     const T* B = Bb + dd.offB;
     TC* Cm = Cb + dd.offC;
@@ -685,6 +688,20 @@ __global__ void __launch_bounds__(256) k_fold_info(int nbox, const int* __restri
     if (threadIdx.x == 0 && best[0] != INT32_MAX && *status == 0) *status = info[best[0]];
 }
 
+__global__ void __launch_bounds__(256) k_count_nonzero(int n, const int* __restrict__ a,
+                                                       int* count) {
+    __shared__ int part[256];
+    int c = 0;
+    for (int i = threadIdx.x; i < n; i += 256) c += a[i] != 0;
+    part[threadIdx.x] = c;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *count = part[0];
+}
+
 // info[bix[j]] = cinfo[j]: blocked-path status back to the caller's boxes.
 __global__ void k_scatter_info(int nbig, const int* __restrict__ bix,
                                const int* __restrict__ cinfo, int* info) {
@@ -744,133 +761,121 @@ __global__ void k_permute_sym(int nbox, const long long* __restrict__ off,
 }
 
 // Interpolative decomposition through the Gram matrix G = M^T M (n x n) of a
-// tall face matrix.  Pivoted Cholesky of G chooses, step by step, the column
-// with the largest remaining squared norm -- the column-pivoted QR rule -- and
-// its rows are the rows of that QR's R (up to signs).  Left-looking: row i of
-// R is G(p, :) minus the contributions of rows 0..i-1, so only the pivot row
-// of G is read per step.  The factorization stops at the first pivot with
+// tall face matrix, of which only the lower triangle is read.  Pivoted
+// Cholesky of G chooses, step by step, the column with the largest remaining
+// squared norm -- the column-pivoted QR rule -- and its rows are the rows of
+// that QR's R (up to signs).  It stops at the first pivot with
 // |R_ii| <= eps |R_00|; the remaining columns are redundant.
-// Dynamic shared memory: d[n], rp[n] (double), jp[n] (int).
-__global__ void __launch_bounds__(256) k_gram_pivchol(const int* __restrict__ dn,
-                                                      const unsigned char* __restrict__ mode,
-                                                      const double* __restrict__ Gb,
-                                                      const long long* __restrict__ offG,
-                                                      double* Rb, double eps, int* perm,
-                                                      const int* __restrict__ perm_off,
-                                                      int* kout) {
+//
+// Blocked as LAPACK's dpstrf: a panel of kGramNB pivots is chosen by one
+// workgroup per face (left-looking inside the panel: row i of R is the pivot
+// row of the trailing matrix minus the panel's earlier rows), then the
+// trailing matrix takes the panel's rank-kGramNB update as a batched GEMM.
+// The pivot order, remaining diagonal and stop state persist in global
+// memory between panels.  Row i of R is stored at R[i n + col]; the panel's
+// rows are also kept column by column in P (P[l + col kGramNB]) for the GEMM.
+// Dynamic shared memory: d[n] (double), jp[n] (int).
+__global__ void __launch_bounds__(256) k_gram_panel(
+    const int* __restrict__ dn, const unsigned char* __restrict__ mode, const double* Gb,
+    const long long* __restrict__ offG, double* Rb, double* Pb, const long long* __restrict__ offP,
+    double* dgb, int* perm, const int* __restrict__ perm_off, int* kout, int* active, double* tolv,
+    double eps, int j0) {
+    constexpr int NB = kGramNB;
     const int b = blockIdx.x;
-    if (!mode[b]) return;
+    if (!mode[b] || !active[b]) return;
     const int n = dn[b];
     const double* G = Gb + offG[b];
-    double* R = Rb + offG[b];  // row i of R at R[i * n + col]
+    double* R = Rb + offG[b];
+    double* P = Pb + offP[b];
+    double* dg = dgb + perm_off[b];
+    int* jpg = perm + perm_off[b];
     extern __shared__ unsigned char smem[];
     double* d = reinterpret_cast<double*>(smem);
-    double* rp = d + n;
-    int* jp = reinterpret_cast<int*>(rp + n);
+    int* jp = reinterpret_cast<int*>(d + n);
     __shared__ double redv[4];
     __shared__ int redi[4];
     __shared__ double s_pv, s_tol;
-    __shared__ int s_pi, s_k;
+    __shared__ int s_pi, s_stop;
     const int tid = threadIdx.x, bs = blockDim.x;
     for (int j = tid; j < n; j += bs) {
-        d[j] = G[j + (size_t)j * n];
-        jp[j] = j;
+        d[j] = j0 == 0 ? G[j + (size_t)j * n] : dg[j];
+        jp[j] = j0 == 0 ? j : jpg[j];
     }
-    if (tid == 0) s_k = n;
+    if (tid == 0) {
+        s_stop = -1;
+        s_tol = j0 == 0 ? 0.0 : tolv[b];
+    }
     __syncthreads();
-    for (int i = 0; i < n; ++i) {
+    const int i1 = min(j0 + NB, n);
+    for (int i = j0; i < i1; ++i) {
         double best = -1.0;
         int bi = n;
         for (int pos = i + tid; pos < n; pos += bs) {
-            double v = d[jp[pos]];
+            const double v = d[jp[pos]];
             if (v > best) { best = v; bi = pos; }
         }
         block_argmax(best, bi, redv, redi, &s_pv, &s_pi);
         const int pos = s_pi;
-        const double dp = s_pv > 0.0 ? s_pv : 0.0;
-        const double rii = sqrt(dp);
+        const double rii = sqrt(s_pv > 0.0 ? s_pv : 0.0);
         if (i == 0 && tid == 0) s_tol = rii * eps;
         __syncthreads();
         if (!(rii > s_tol)) {
-            if (tid == 0) s_k = i;
+            if (tid == 0) s_stop = i;
             break;
         }
         if (tid == 0 && pos != i) {
-            int t = jp[pos]; jp[pos] = jp[i]; jp[i] = t;
+            const int t = jp[pos];
+            jp[pos] = jp[i];
+            jp[i] = t;
         }
         __syncthreads();
-        const int p = jp[i];
-        for (int l = tid; l < i; l += bs) rp[l] = R[(size_t)l * n + p];
-        __syncthreads();
+        const int p = jp[i], li = i - j0;
+        const double* pp = P + (size_t)p * NB;
         const double inv = 1.0 / rii;
         for (int q = i + 1 + tid; q < n; q += bs) {
-            int c = jp[q];
-            double v = G[p + (size_t)c * n];
-            // eight independent partial sums keep eight loads in flight
-            double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-            int l = 0;
-            for (; l + 8 <= i; l += 8) {
-#pragma unroll
-                for (int u = 0; u < 8; ++u) a[u] += rp[l + u] * R[(size_t)(l + u) * n + c];
-            }
-            for (; l < i; ++l) a[0] += rp[l] * R[(size_t)l * n + c];
-            v -= ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+            const int c = jp[q];
+            // G holds its lower triangle: G(p, c) = G(max, min)
+            double v = p >= c ? G[p + (size_t)c * n] : G[c + (size_t)p * n];
+            double* pc = P + (size_t)c * NB;
+            for (int l = 0; l < li; ++l) v -= pp[l] * pc[l];
             v *= inv;
             R[(size_t)i * n + c] = v;
-            double nd = d[c] - v * v;
+            pc[li] = v;
+            const double nd = d[c] - v * v;
             d[c] = nd > 0.0 ? nd : 0.0;
         }
+        // the panel row is zero on the columns pivoted so far, rii on its own
+        for (int q = tid; q <= i; q += bs) P[li + (size_t)jp[q] * NB] = (q == i) ? rii : 0.0;
         if (tid == 0) R[(size_t)i * n + p] = rii;
         __syncthreads();
     }
     __syncthreads();
-    const int k = s_k;
-    int* pb = perm + perm_off[b];
-    for (int t = tid; t < n; t += bs) pb[t] = jp[t];
-    if (tid == 0) kout[b] = k;
+    for (int j = tid; j < n; j += bs) {
+        dg[j] = d[j];
+        jpg[j] = jp[j];
+    }
+    if (tid == 0) {
+        tolv[b] = s_tol;
+        if (s_stop >= 0 || i1 == n) {
+            kout[b] = s_stop >= 0 ? s_stop : n;
+            active[b] = 0;
+        }
+    }
 }
 
-// W = R11^{-1} R12 for the Gram path.  Back substitution for CW right-hand
-// sides at once, held in LDS; grid (column chunks, faces).
-template <int CW>
-__global__ void __launch_bounds__(256) k_gram_wsolve(const int* __restrict__ dn,
-                                                     const unsigned char* __restrict__ mode,
-                                                     const long long* __restrict__ offG,
-                                                     const double* __restrict__ Rb,
-                                                     const int* __restrict__ perm,
-                                                     const int* __restrict__ perm_off,
-                                                     const int* __restrict__ kk, double* Wb) {
-    const int b = blockIdx.y;
-    if (!mode[b]) return;
-    const int n = dn[b], k = kk[b], q = n - k;
-    const int jj0 = blockIdx.x * CW;
-    if (jj0 >= q || k == 0) return;
-    const int cols = min(CW, q - jj0);
-    const double* R = Rb + offG[b];
-    const int* jp = perm + perm_off[b];
-    extern __shared__ double ys[];  // ys[i * CW + c]
-    __shared__ double yi[CW];
-    const int tid = threadIdx.x;
-    for (int e = tid; e < k * CW; e += 256) {
-        const int i = e / CW, c = e - i * CW;
-        ys[e] = (c < cols) ? R[(size_t)i * n + jp[k + jj0 + c]] : 0.0;
-    }
-    __syncthreads();
-    for (int i = k - 1; i >= 0; --i) {
-        const int pc = jp[i];
-        if (tid < CW) yi[tid] = ys[i * CW + tid] / R[(size_t)i * n + pc];
-        __syncthreads();
-        for (int e = tid; e < (i + 1) * CW; e += 256) {
-            const int r = e / CW, c = e - r * CW;
-            if (r == i) ys[e] = yi[c];
-            else ys[e] -= R[(size_t)r * n + pc] * yi[c];
-        }
-        __syncthreads();
-    }
-    double* W = Wb + offG[b];
-    for (int e = tid; e < k * cols; e += 256) {
-        const int c = e / k, i = e - c * k;
-        W[i + (size_t)(jj0 + c) * k] = ys[i * CW + c];
+// [R11 | R12] (k x n, ld k) of each Gram face from the rows of R in pivot
+// order: R11(i, j) = R(i, jp[j]) for j >= i, zero below the diagonal.
+__global__ void k_gram_gather_r(int nface, const long long* __restrict__ toff,
+                                const GramRDesc* __restrict__ desc, const double* __restrict__ Rb,
+                                const int* __restrict__ perm, double* out, long long total) {
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int f = find_box(toff, nface, e);
+        const GramRDesc d = desc[f];
+        const long long l = e - toff[f];
+        const int i = (int)(l % d.k), j = (int)(l / d.k);
+        const int c = perm[d.perm + j];
+        out[d.out + l] = (j < d.k && j < i) ? 0.0 : Rb[d.r + (size_t)i * d.n + c];
     }
 }
 
@@ -1221,7 +1226,14 @@ void batched_gj_inverse(int nbox, const int* h_n, const long long* h_off, const 
 
 void fold_info(int nbox, const int* d_info, int* d_status, hipStream_t s) {
     if (nbox <= 0) return;
+    if (!d_info || !d_status) throw Error(-4, "fold_info: buffer not allocated");
     k_fold_info<<<1, 256, 0, s>>>(nbox, d_info, d_status);
+    HIFDE_LAUNCH_CHECK();
+}
+
+void fold_active(int n, const int* d_active, int* d_count, hipStream_t s) {
+    if (!d_active || !d_count) throw Error(-4, "fold_active: buffer not allocated");
+    k_count_nonzero<<<1, 256, 0, s>>>(n, d_active, d_count);
     HIFDE_LAUNCH_CHECK();
 }
 
@@ -1233,7 +1245,8 @@ void fill_int(int* p, int n, int v, hipStream_t s) {
 
 template <class T, class TC>
 void batched_gemm_desc(bool transA, const std::vector<GemmDesc>& desc, const T* d_A, const T* d_B,
-                       TC* d_C, T alpha, T beta, GemmWork& ws, hipStream_t s) {
+                       TC* d_C, T alpha, T beta, GemmWork& ws, hipStream_t s,
+                       const int* d_flags) {
     if (desc.empty()) return;
     // Boxes whose output spans at least half an MFMA workgroup tile in both
     // directions go to the MFMA kernel, the rest to the 64x64 FMA kernel.
@@ -1259,16 +1272,18 @@ void batched_gemm_desc(bool transA, const std::vector<GemmDesc>& desc, const T* 
         const int total = toff[nb];
         if (pass == 0) {
             if (transA)
-                k_gemm<T, true, TC><<<total, 256, 0, s>>>(nb, tb.p, db.p, d_A, d_B, d_C, alpha, beta);
+                k_gemm<T, true, TC><<<total, 256, 0, s>>>(nb, tb.p, db.p, d_A, d_B, d_C, alpha,
+                                                          beta, d_flags);
             else
-                k_gemm<T, false, TC><<<total, 256, 0, s>>>(nb, tb.p, db.p, d_A, d_B, d_C, alpha, beta);
+                k_gemm<T, false, TC><<<total, 256, 0, s>>>(nb, tb.p, db.p, d_A, d_B, d_C, alpha,
+                                                           beta, d_flags);
         } else {
             if (transA)
                 k_gemm_mfma<T, true, TC><<<total, 256, 0, s>>>(nb, tb.p, db.p, d_A, d_B, d_C, alpha,
-                                                           beta);
+                                                           beta, d_flags);
             else
                 k_gemm_mfma<T, false, TC><<<total, 256, 0, s>>>(nb, tb.p, db.p, d_A, d_B, d_C, alpha,
-                                                            beta);
+                                                            beta, d_flags);
         }
         HIFDE_LAUNCH_CHECK();
     }
@@ -1315,29 +1330,89 @@ void permute_sym(int nbox, const long long* d_off, const int* d_n, const int* d_
     HIFDE_LAUNCH_CHECK();
 }
 
-void gram_interp_decomp(int nface, const int* d_n, const unsigned char* d_mode,
-                        const double* d_G, const long long* d_offG, double* d_R, double eps,
-                        int* d_perm, const int* d_perm_off, int* d_k, double* d_W, int max_n,
-                        hipStream_t s) {
-    if (nface <= 0 || max_n <= 0) return;
-    size_t shm = (size_t)max_n * (2 * sizeof(double) + sizeof(int));
-    if (shm > 150 * 1024) throw Error(-3, "gram_interp_decomp: face too large for LDS");
-    k_gram_pivchol<<<nface, 256, shm, s>>>(d_n, d_mode, d_G, d_offG, d_R, eps, d_perm, d_perm_off,
-                                           d_k);
-    HIFDE_LAUNCH_CHECK();
-    // the right-hand-side chunk width keeps k x CW doubles within LDS
-    if ((size_t)max_n * 32 * sizeof(double) <= 140 * 1024) {
-        dim3 g((max_n + 31) / 32, nface);
-        k_gram_wsolve<32><<<g, 256, (size_t)max_n * 32 * sizeof(double), s>>>(
-            d_n, d_mode, d_offG, d_R, d_perm, d_perm_off, d_k, d_W);
-    } else if ((size_t)max_n * 8 * sizeof(double) <= 140 * 1024) {
-        dim3 g((max_n + 7) / 8, nface);
-        k_gram_wsolve<8><<<g, 256, (size_t)max_n * 8 * sizeof(double), s>>>(
-            d_n, d_mode, d_offG, d_R, d_perm, d_perm_off, d_k, d_W);
-    } else {
-        throw Error(-3, "gram_interp_decomp: face too large for the LDS solve");
+void gram_pivchol(int nface, const int* h_n, const unsigned char* h_mode, const int* d_n,
+                  const unsigned char* d_mode, double* d_G, const long long* h_offG,
+                  const long long* d_offG, double* d_R, double eps, int* d_perm,
+                  const int* h_perm_off, const int* d_perm_off, int* d_k, GramWork& ws,
+                  GemmWork& gws, hipStream_t s) {
+    int max_n = 0;
+    std::vector<long long> offP(nface + 1, 0);
+    std::vector<int> act(nface, 0);
+    std::vector<GemmDesc> upd;
+    for (int f = 0; f < nface; ++f) {
+        const bool g = h_mode[f] && h_n[f] > 0;
+        offP[f + 1] = offP[f] + (g ? (long long)h_n[f] * kGramNB : 0);
+        if (!g) continue;
+        act[f] = 1;
+        max_n = std::max(max_n, h_n[f]);
+        // trailing update G -= P^T P over the lower triangle
+        GemmDesc u{offP[f], offP[f], h_offG[f], h_n[f], h_n[f], kGramNB, kGramNB, kGramNB,
+                   h_n[f]};
+        u.lower = 1;
+        u.flag = f;
+        upd.push_back(u);
     }
+    if (max_n == 0) return;
+    const size_t shm = (size_t)max_n * (sizeof(double) + sizeof(int));
+    if (shm > 150 * 1024) throw Error(-3, "gram_pivchol: face too large for LDS");
+    ws.P.reserve(std::max<long long>(offP.back(), 1));
+    ws.offP.upload(offP, s);
+    ws.dg.reserve(std::max(h_perm_off[nface], 1));
+    ws.tol.reserve(nface);
+    ws.active.upload(act, s);
+    ws.count.reserve(1);
+    ws.host_active.resize(1);
+    for (int j0 = 0; j0 < max_n; j0 += kGramNB) {
+        k_gram_panel<<<nface, 256, shm, s>>>(d_n, d_mode, d_G, d_offG, d_R, ws.P.p, ws.offP.p,
+                                             ws.dg.p, d_perm, d_perm_off, d_k, ws.active.p,
+                                             ws.tol.p, eps, j0);
+        HIFDE_LAUNCH_CHECK();
+        if (j0 + kGramNB >= max_n) break;
+        batched_gemm_desc<double>(true, upd, ws.P.p, ws.P.p, d_G, -1.0, 1.0, gws, s,
+                                  ws.active.p);
+        // every fourth panel, stop once every face has found its rank
+        if ((j0 / kGramNB) % 4 == 3) {
+            fold_active(nface, ws.active.p, ws.count.p, s);
+            HIFDE_CHECK(hipMemcpyAsync(ws.host_active.data(), ws.count.p, sizeof(int),
+                                       hipMemcpyDeviceToHost, s));
+            HIFDE_CHECK(hipStreamSynchronize(s));
+            if (ws.host_active[0] == 0) break;
+        }
+    }
+}
+
+void gram_w(const std::vector<GramRDesc>& desc, const std::vector<long long>& toff,
+            const double* d_R, const int* d_perm, const std::vector<long long>& offW,
+            double* d_W, GramWork& ws, GJWork& gj, GemmWork& gws, hipStream_t s) {
+    const int nb = (int)desc.size();
+    if (nb == 0 || toff.back() == 0) return;
+    ws.rdesc.upload(desc, s);
+    ws.roff.upload(toff, s);
+    ws.RK.reserve(toff.back());
+    k_gram_gather_r<<<grid_for(toff.back()), 256, 0, s>>>(nb, ws.roff.p, ws.rdesc.p, d_R, d_perm,
+                                                         ws.RK.p, toff.back());
     HIFDE_LAUNCH_CHECK();
+    // R11^{-1} in place (the leading k x k block of each [R11 | R12])
+    std::vector<int> hk(nb);
+    std::vector<long long> ho(nb);
+    for (int t = 0; t < nb; ++t) {
+        hk[t] = desc[t].k;
+        ho[t] = toff[t];
+    }
+    ws.kd.upload(hk, s);
+    ws.ko.upload(ho, s);
+    ws.info.reserve(nb);
+    fill_int(ws.info.p, nb, 0, s);
+    batched_gj_inverse<double>(nb, hk.data(), ho.data(), ws.kd.p, ws.ko.p, ws.RK.p, ws.info.p, gj,
+                               gws, s);
+    // W = R11^{-1} R12 (k x q, ld k)
+    std::vector<GemmDesc> g;
+    for (int t = 0; t < nb; ++t) {
+        const int k = desc[t].k, q = desc[t].n - desc[t].k;
+        if (k == 0 || q == 0) continue;
+        g.push_back({toff[t], toff[t] + (long long)k * k, offW[t], k, q, k, k, k, k});
+    }
+    batched_gemm_desc<double>(false, g, ws.RK.p, ws.RK.p, d_W, 1.0, 0.0, gws, s);
 }
 
 void batched_interp_decomp(int nface, const int* d_m, const int* d_n, const long long* d_offM,
@@ -1381,13 +1456,13 @@ template void batched_gj_inverse<float>(int, const int*, const long long*, const
 
 template void batched_gemm_desc<double, double>(bool, const std::vector<GemmDesc>&,
                                                 const double*, const double*, double*, double,
-                                                double, GemmWork&, hipStream_t);
+                                                double, GemmWork&, hipStream_t, const int*);
 template void batched_gemm_desc<float, float>(bool, const std::vector<GemmDesc>&, const float*,
                                               const float*, float*, float, float, GemmWork&,
-                                              hipStream_t);
+                                              hipStream_t, const int*);
 template void batched_gemm_desc<float, double>(bool, const std::vector<GemmDesc>&, const float*,
                                                const float*, double*, float, float, GemmWork&,
-                                               hipStream_t);
+                                               hipStream_t, const int*);
 template void batched_gemm<double>(bool, int, const int*, const int*, const int*,
                                    const long long*, const long long*, const long long*,
                                    const double*, const double*, double*, double, GemmWork&,

@@ -133,6 +133,7 @@ struct hifde_factor {
     DevBuf<int> ws_perm, ws_k, ws_info;
     GJWork gj_ws;
     GemmWork gemm_ws;
+    GramWork gram_ws;
     DevBuf<unsigned char> ws_mode;
     DevBuf<int> ws_gm, ws_gk;
     DevBuf<long long> ws_goffM, ws_goffC;
@@ -948,7 +949,7 @@ struct hifde_factor {
         std::vector<unsigned char> mode(nf, 0);
         std::vector<int> gm, gn, gk;
         std::vector<long long> goffM, goffC;
-        int max_n_house = 0, max_m_house = 0, max_n_gram = 0;
+        int max_n_house = 0, max_m_house = 0;
         double fl_h = 0, by_h = 0, fl_g = 0, by_g = 0, fl_c = 0;
         for (int f = 0; f < nf; ++f) {
             double m = F.m[f], n = F.n[f];
@@ -960,7 +961,6 @@ struct hifde_factor {
                 gk.push_back(F.m[f]);
                 goffM.push_back(offM[f]);
                 goffC.push_back(offC[f]);
-                max_n_gram = std::max(max_n_gram, F.n[f]);
                 fl_g += 2 * m * n * n;
                 by_g += 8 * (m * n + n * n);
                 fl_c += n * n * n / 3;
@@ -982,15 +982,23 @@ struct hifde_factor {
             ws_G.reserve(offC.back());
             ws_T.reserve(offC.back());
             {
-                Prof pr(this, "gram_gemm", fl_g, by_g);
-                batched_gemm<double>(true, ng, gm.data(), gn.data(), gk.data(), goffM.data(),
-                                     goffM.data(), goffC.data(), ws_M.p, ws_M.p, ws_G.p, 1.0,
-                                     gemm_ws, stream);
+                // G = M^T M is symmetric: only the tiles on or below the diagonal
+                std::vector<GemmDesc> gd;
+                for (int t = 0; t < ng; ++t) {
+                    GemmDesc g{goffM[t], goffM[t], goffC[t], gm[t], gn[t], gk[t],
+                               std::max(gk[t], 1), std::max(gk[t], 1), std::max(gm[t], 1)};
+                    g.lower = 1;
+                    gd.push_back(g);
+                }
+                Prof pr(this, "gram_gemm", fl_g / 2, by_g);
+                batched_gemm_desc<double>(true, gd, ws_M.p, ws_M.p, ws_G.p, 1.0, 0.0, gemm_ws,
+                                          stream);
             }
             {
                 Prof pr(this, "gram_pivchol", fl_c, 8.0 * offC.back());
-                gram_interp_decomp(nf, fd.n.p, ws_mode.p, ws_G.p, fd.offCd.p, ws_T.p, eps,
-                                   ws_perm.p, fd.cols_off.p, ws_k.p, ws_W.p, max_n_gram, stream);
+                gram_pivchol(nf, F.n.data(), mode.data(), fd.n.p, ws_mode.p, ws_G.p, offC.data(),
+                             fd.offCd.p, ws_T.p, eps, ws_perm.p, F.cols_off.data(),
+                             fd.cols_off.p, ws_k.p, gram_ws, gemm_ws, stream);
             }
         }
         if (max_n_house > 0 || (int)gm.size() < nf) {
@@ -1009,6 +1017,22 @@ struct hifde_factor {
         HIFDE_CHECK(hipStreamSynchronize(stream));
         stager.synced();
         timed(ST_DOWNLOAD, t0);
+        if (!gm.empty()) {
+            // Gram faces: W = R11^{-1} R12 now that the ranks are known
+            std::vector<GramRDesc> rd;
+            std::vector<long long> toff(1, 0), offW;
+            double fl_w = 0;
+            for (int f = 0; f < nf; ++f) {
+                const int k = kk[f], n = F.n[f];
+                if (!mode[f] || k == 0 || k == n) continue;
+                rd.push_back({offC[f], toff.back(), k, n, F.cols_off[f]});
+                toff.push_back(toff.back() + (long long)k * n);
+                offW.push_back(offC[f]);
+                fl_w += 2.0 * k * k * k + 2.0 * k * k * (n - k);
+            }
+            Prof pr(this, "gram_w", fl_w, 8.0 * toff.back() * 2);
+            gram_w(rd, toff, ws_T.p, ws_perm.p, offW, ws_W.p, gram_ws, gj_ws, gemm_ws, stream);
+        }
         // Face box: red = ID redundant columns, skel = ID skeleton columns
         // (Skeletonize, src/Factor.cpp:171-178).
         B.r.resize(nf);

@@ -224,6 +224,7 @@ struct GemmDesc {
     long long offA, offB, offC;
     int m, n, k, lda, ldb, ldc;
     int lower = 0;  // 1: only output tiles on or below the block diagonal
+    int flag = -1;  // >= 0: skipped when flags[flag] == 0 (device flags)
 };
 struct CopyDesc {
     long long off;  // source block base
@@ -239,7 +240,8 @@ struct GemmWork {
 // may be FP64 for FP32 products (accumulated in FP32, added in FP64).
 template <class T, class TC = T>
 void batched_gemm_desc(bool transA, const std::vector<GemmDesc>& desc, const T* d_A, const T* d_B,
-                       TC* d_C, T alpha, T beta, GemmWork& ws, hipStream_t s);
+                       TC* d_C, T alpha, T beta, GemmWork& ws, hipStream_t s,
+                       const int* d_flags = nullptr);
 // Packed boxes (ld = rows): C = alpha op(A) B with host offsets.
 template <class T>
 void batched_gemm(bool transA, int nbox, const int* h_m, const int* h_n, const int* h_k,
@@ -260,10 +262,30 @@ void batched_interp_decomp(int nface, const int* d_m, const int* d_n, const long
                            double* d_M, double eps, int* d_perm, const int* d_perm_off,
                            int* d_k, double* d_W, const long long* d_offW, int max_n,
                            int max_m, const unsigned char* d_mode, hipStream_t s);
-void gram_interp_decomp(int nface, const int* d_n, const unsigned char* d_mode,
-                        const double* d_G, const long long* d_offG, double* d_R, double eps,
-                        int* d_perm, const int* d_perm_off, int* d_k, double* d_W, int max_n,
-                        hipStream_t s);
+// Gram-path ID (faces with mode != 0): blocked pivoted Cholesky of the
+// lower triangle of G (overwritten), rows of R, pivot order and rank; then,
+// once the ranks are known on the host, W = R11^{-1} R12.
+constexpr int kGramNB = 32;
+struct GramRDesc {
+    long long r, out;  // R of the face (rows at r + i n), [R11 | R12] output
+    int k, n, perm;    // rank, columns, offset of the face's pivot order
+};
+struct GramWork {
+    DevBuf<double> P, dg, tol, RK;
+    DevBuf<long long> offP, roff, ko;
+    DevBuf<int> active, count, kd, info;
+    DevBuf<GramRDesc> rdesc;
+    std::vector<int> host_active;
+};
+void gram_pivchol(int nface, const int* h_n, const unsigned char* h_mode, const int* d_n,
+                  const unsigned char* d_mode, double* d_G, const long long* h_offG,
+                  const long long* d_offG, double* d_R, double eps, int* d_perm,
+                  const int* h_perm_off, const int* d_perm_off, int* d_k, GramWork& ws,
+                  GemmWork& gws, hipStream_t s);
+void gram_w(const std::vector<GramRDesc>& desc, const std::vector<long long>& toff,
+            const double* d_R, const int* d_perm, const std::vector<long long>& offW,
+            double* d_W, GramWork& ws, GJWork& gj, GemmWork& gws, hipStream_t s);
+void fold_active(int n, const int* d_active, int* d_count, hipStream_t s);
 void convert_d2f(const double* src, float* dst, long long n, hipStream_t s);
 // *d_status = info[b] of the first box b with a nonzero status, unless
 // *d_status is already nonzero.
