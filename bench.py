@@ -23,19 +23,6 @@ sys.path.insert(0, ROOT)
 
 METRIC = "factor s + PCG solves/s at 128^3 to 1e-10 resid: mixed vs FP64 GPU vs CPU"
 PEAK = {"hbm_GBs": 8000.0, "fp64_TF": 78.6, "fp32_TF": 157.3}
-COMPUTE_FAMILIES = ("gemm_x", "gemm_s", "gj_inverse", "interp_decomp", "face_transform",
-                    "gram_gemm", "gram_pivchol")
-# kernels (as rocprofv3 names them, arguments stripped) that make up a family
-FAMILY_KERNELS = {
-    "gather_f32": ["k_table_gather<float>"], "gather_f64": ["k_table_gather<double>"],
-    "scatter_f32": ["k_table_scatter<float>"], "scatter_f64": ["k_table_scatter<double>"],
-    "gj_inverse_f32": ["k_gj_small<float>", "k_bgj_diag<float>", "k_bgj_row<float>",
-                       "k_bgj_update<float>", "k_bgj_col<float>"],
-    "gj_inverse_f64": ["k_gj_small<double>", "k_bgj_diag<double>", "k_bgj_row<double>",
-                       "k_bgj_update<double>", "k_bgj_col<double>"],
-    "interp_decomp": ["k_interp_decomp"], "gram_pivchol": ["k_gram_pivchol"],
-    "spmv": ["k_csr_spmv"], "compact": ["k_table_count_live", "k_table_rehash"],
-}
 PMC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_n128.json")
 
 
@@ -51,34 +38,72 @@ def make_problem(n, seed, corr):
     return a, V, b
 
 
-def family_traffic(fam, launches_per_step):
-    """HBM bytes per launch of a kernel family from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary (one factor + one solve of this workload),
-    scaled by the copy-kernel calibration; None if not available."""
-    if fam not in FAMILY_KERNELS or not os.path.exists(PMC_FILE) or launches_per_step <= 0:
+def short_kernel_name(name):
+    """rocprofv3's kernel name without return type, namespaces and arguments."""
+    for pre in ("void ", "hifde::(anonymous namespace)::", "hifde::"):
+        if name.startswith(pre):
+            name = name[len(pre):]
+    return name.split("(")[0]
+
+
+def kernel_traffic(kernel):
+    """HBM bytes per launch of a kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of this workload, scaled by the
+    copy-kernel calibration; None if not available."""
+    if not os.path.exists(PMC_FILE):
         return None
     pm = json.load(open(PMC_FILE))
-    tot = 0.0
-    for k in FAMILY_KERNELS[fam]:
-        if k in pm["kernels"]:
-            e = pm["kernels"][k]
-            tot += 1024.0 * (pm["fetch_scale"] * e["fetch_kib"] + pm["write_scale"] * e["write_kib"])
-    return tot / launches_per_step if tot > 0 else None
+    e = pm["kernels"].get(kernel)
+    if not e or e["dispatches"] <= 0:
+        return None
+    return 1024.0 * (pm["fetch_scale"] * e["fetch_kib"] + pm["write_scale"] * e["write_kib"]) / \
+        e["dispatches"]
 
 
-def roofline_from_stats(stats, steps):
-    fam = max(stats, key=lambda k: stats[k]["sec"])
-    st = stats[fam]
+def roofline_from_stats(stats):
+    """Roofline of the kernel with the most device time, from the per-launch
+    HIP-event records (families named kernel:<name>)."""
+    kern = {k[len("kernel:"):]: v for k, v in stats.items() if k.startswith("kernel:")}
+    name = max(kern, key=lambda k: kern[k]["sec"])
+    st = kern[name]
     avg = st["sec"] / max(st["launches"], 1)
-    traffic = family_traffic(fam, st["launches"] / max(steps, 1))
-    if fam.startswith(COMPUTE_FAMILIES):
-        peak = PEAK["fp32_TF"] if fam.endswith("_f32") else PEAK["fp64_TF"]
+    traffic = kernel_traffic(name)
+    if "gemm" in name:
+        peak = PEAK["fp32_TF"] if name.split("<")[1].startswith("float") else PEAK["fp64_TF"]
         achieved = st["flops"] / st["sec"] / 1e12
-        return fam, avg, {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                          "frac": achieved / peak, "traffic": traffic}
+        return name, avg, {"bound": "mfma", "achieved": achieved, "peak": peak,
+                           "unit": "TFLOP/s", "frac": achieved / peak, "traffic": traffic}
     achieved = st["bytes"] / st["sec"] / 1e9
-    return fam, avg, {"bound": "hbm", "achieved": achieved, "peak": PEAK["hbm_GBs"],
-                      "unit": "GB/s", "frac": achieved / PEAK["hbm_GBs"], "traffic": traffic}
+    return name, avg, {"bound": "hbm", "achieved": achieved, "peak": PEAK["hbm_GBs"],
+                       "unit": "GB/s", "frac": achieved / PEAK["hbm_GBs"], "traffic": traffic}
+
+
+def pmc_json(fetch_csv, write_csv, calfetch_csv, calwrite_csv, workload):
+    """Per-kernel FETCH_SIZE / WRITE_SIZE totals of two rocprofv3 --pmc runs,
+    with the scale factors that make a 1 GiB device copy read and write 1 GiB."""
+    import csv
+
+    def load(path):
+        out = {}
+        for r in csv.DictReader(open(path)):
+            k = short_kernel_name(r["Kernel_Name"])
+            e = out.setdefault(k, [0, 0.0])
+            e[0] += 1
+            e[1] += float(r["Counter_Value"])
+        return out
+
+    f, w = load(fetch_csv), load(write_csv)
+    cf = load(calfetch_csv)["__amd_rocclr_copyBuffer"]
+    cw = load(calwrite_csv)["__amd_rocclr_copyBuffer"]
+    gib_kib = float(1 << 20)
+    fetch_scale, write_scale = gib_kib / (cf[1] / cf[0]), gib_kib / (cw[1] / cw[0])
+    kernels = {k: {"dispatches": f[k][0], "fetch_kib": f[k][1], "write_kib": w.get(k, [0, 0.0])[1]}
+               for k in f}
+    return {"workload": workload, "fetch_scale": round(fetch_scale, 4),
+            "write_scale": round(write_scale, 4),
+            "calibration": (f"1 GiB device copy (__amd_rocclr_copyBuffer): FETCH_SIZE "
+                            f"{cf[1] / cf[0]:.0f} KiB, WRITE_SIZE {cw[1] / cw[0]:.0f} KiB per dispatch"),
+            "kernels": kernels}
 
 
 def copy_calibration():
@@ -152,9 +177,18 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=32)
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--pmc-json", nargs=4, metavar=("FETCH", "WRITE", "CALFETCH", "CALWRITE"),
+                    help="write the per-kernel HBM traffic summary of four rocprofv3 --pmc "
+                         "counter_collection.csv files to profiles/pmc_traffic_n128.json")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.pmc_json:
+        out = pmc_json(*args.pmc_json, workload="bench.py --steps 1 --warmup 0 --no-cpu --no-fp64 "
+                       "(one factor + one PCG solve, n=128 mixed)")
+        json.dump(out, open(PMC_FILE, "w"), indent=1, sort_keys=True)
+        print(f"wrote {PMC_FILE}: {len(out['kernels'])} kernels")
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -205,10 +239,10 @@ def main():
     level_stages = f.level_stage_times()
     counts = f.level_counts()
     x_mixed = d_x.cpu().numpy()
-    fam, avg_launch, roof = roofline_from_stats(stats, args.steps)
-    log(f"[bench] dominant kernel family {fam}: avg launch {avg_launch * 1e3:.3f} ms")
+    kname, avg_launch, roof = roofline_from_stats(stats)
+    log(f"[bench] dominant kernel {kname}: avg launch {avg_launch * 1e3:.3f} ms")
     for k_, v_ in sorted(stats.items(), key=lambda kv: -kv[1]["sec"]):
-        log(f"[bench]   {k_:15s} launches {v_['launches']:6d} dev {v_['sec']:.4f}s")
+        log(f"[bench]   {k_:40s} launches {v_['launches']:6d} dev {v_['sec']:.4f}s")
     fbytes_mixed = f.factor_bytes()
     f.close()
 
@@ -257,7 +291,8 @@ def main():
         "kernel_families": {k_: {"launches": v_["launches"], "sec": v_["sec"],
                                  "avg_ms": 1e3 * v_["sec"] / max(v_["launches"], 1)}
                             for k_, v_ in stats.items()},
-        "roofline": dict(roof, kernel=fam, avg_launch_ms=avg_launch * 1e3),
+        "roofline": dict(roof, kernel=kname, avg_launch_ms=avg_launch * 1e3,
+                         launches=stats["kernel:" + kname]["launches"]),
         "fp64_gpu": fp64,
         "cpu_baseline": cpu,
     }

@@ -227,14 +227,19 @@ void apply_tiles(int nbox, int ntiles, const int* d_tile_off, const int* d_r, co
                  const int* d_red_off, const int* d_skel_off, const int* d_red_gid,
                  const int* d_skel_gid, const unsigned char* d_skel_pass, const long long* d_off,
                  const T* d_M, int op, int pass, double alpha, const double* d_in,
-                 double* d_out, int max_in, hipStream_t s) {
+                 double* d_out, int max_in, double flops, double bytes, hipStream_t s) {
     if (nbox <= 0 || ntiles <= 0) return;
     size_t shm = (size_t)std::max(max_in, 1) * sizeof(double);
     if (shm > 150 * 1024) throw Error(-3, "apply: box too large for LDS");
+    KernelTimer* kt = active_timer();
+    if (kt) kt->begin(s);
     k_apply_tiles<T><<<ntiles, 256, shm, s>>>(nbox, d_tile_off, d_r, d_s, d_red_off, d_skel_off,
                                               d_red_gid, d_skel_gid, d_skel_pass, d_off, d_M, op,
                                               pass, alpha, d_in, d_out);
     HIFDE_LAUNCH_CHECK();
+    if (kt)
+        kt->end(s, sizeof(T) == sizeof(float) ? "k_apply_tiles<float>" : "k_apply_tiles<double>",
+                flops, bytes);
 }
 
 void zero_boundary(double* d_u, int NC, hipStream_t s) {
@@ -279,10 +284,10 @@ void vcopy(const double* x, double* y, long long n, hipStream_t s) {
 template void apply_tiles<double>(int, int, const int*, const int*, const int*, const int*,
                                   const int*, const int*, const int*, const unsigned char*,
                                   const long long*, const double*, int, int, double,
-                                  const double*, double*, int, hipStream_t);
+                                  const double*, double*, int, double, double, hipStream_t);
 template void apply_tiles<float>(int, int, const int*, const int*, const int*, const int*,
                                  const int*, const int*, const int*, const unsigned char*,
                                  const long long*, const float*, int, int, double, const double*,
-                                 double*, int, hipStream_t);
+                                 double*, int, double, double, hipStream_t);
 
 }  // namespace hifde

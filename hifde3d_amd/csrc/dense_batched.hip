@@ -15,6 +15,8 @@
 //                   (src/Factor.cpp:113-153) restricted to the three blocks
 //                   Schur needs.
 
+#include <cstdio>
+
 #include "hifde_internal.hpp"
 
 namespace hifde {
@@ -1243,6 +1245,11 @@ void fill_int(int* p, int n, int v, hipStream_t s) {
     HIFDE_LAUNCH_CHECK();
 }
 
+template <class T>
+constexpr const char* type_name() {
+    return sizeof(T) == sizeof(float) ? "float" : "double";
+}
+
 template <class T, class TC>
 void batched_gemm_desc(bool transA, const std::vector<GemmDesc>& desc, const T* d_A, const T* d_B,
                        TC* d_C, T alpha, T beta, GemmWork& ws, hipStream_t s,
@@ -1257,20 +1264,34 @@ void batched_gemm_desc(bool transA, const std::vector<GemmDesc>& desc, const T* 
         if (d.m >= TILE / 2 && d.n >= TILE / 2) large.push_back(d);
         else small.push_back(d);
     }
+    KernelTimer* kt = active_timer();
     for (int pass = 0; pass < 2; ++pass) {
         const std::vector<GemmDesc>& dv = pass == 0 ? small : large;
         const int nb = (int)dv.size();
         if (nb == 0) continue;
         const int tile = pass == 0 ? BM : TILE;
         std::vector<int> toff(nb + 1, 0);
-        for (int b = 0; b < nb; ++b)
-            toff[b + 1] = toff[b] + ((dv[b].m + tile - 1) / tile) * ((dv[b].n + tile - 1) / tile);
+        double flops = 0, bytes = 0;
+        for (int b = 0; b < nb; ++b) {
+            const GemmDesc& d = dv[b];
+            const int tm = (d.m + tile - 1) / tile, tn = (d.n + tile - 1) / tile;
+            toff[b + 1] = toff[b] + tm * tn;
+            // algorithmic work: the output entries the launch forms
+            const double cells = d.lower ? 0.5 * d.m * (d.n + 1.0) : (double)d.m * d.n;
+            flops += 2.0 * cells * d.k;
+            bytes += ((double)d.m * d.k + (double)d.k * d.n) * sizeof(T) +
+                     cells * sizeof(TC) * (beta == (T)0 ? 1 : 2);
+        }
         DevBuf<int>& tb = pass == 0 ? ws.tiles : ws.tiles2;
         DevBuf<GemmDesc>& db = pass == 0 ? ws.desc : ws.desc2;
         tb.upload(toff, s);
         db.upload(dv, s);
         const int total = toff[nb];
+        if (kt) kt->begin(s);
+        char name[96];
         if (pass == 0) {
+            std::snprintf(name, sizeof name, "k_gemm<%s, %s, %s>", type_name<T>(),
+                          transA ? "true" : "false", type_name<TC>());
             if (transA)
                 k_gemm<T, true, TC><<<total, 256, 0, s>>>(nb, tb.p, db.p, d_A, d_B, d_C, alpha,
                                                           beta, d_flags);
@@ -1278,16 +1299,20 @@ void batched_gemm_desc(bool transA, const std::vector<GemmDesc>& desc, const T* 
                 k_gemm<T, false, TC><<<total, 256, 0, s>>>(nb, tb.p, db.p, d_A, d_B, d_C, alpha,
                                                            beta, d_flags);
         } else {
+            std::snprintf(name, sizeof name, "k_gemm_mfma<%s, %s, %s>", type_name<T>(),
+                          transA ? "true" : "false", type_name<TC>());
             if (transA)
-                k_gemm_mfma<T, true, TC><<<total, 256, 0, s>>>(nb, tb.p, db.p, d_A, d_B, d_C, alpha,
-                                                           beta, d_flags);
+                k_gemm_mfma<T, true, TC><<<total, 256, 0, s>>>(nb, tb.p, db.p, d_A, d_B, d_C,
+                                                               alpha, beta, d_flags);
             else
-                k_gemm_mfma<T, false, TC><<<total, 256, 0, s>>>(nb, tb.p, db.p, d_A, d_B, d_C, alpha,
-                                                            beta, d_flags);
+                k_gemm_mfma<T, false, TC><<<total, 256, 0, s>>>(nb, tb.p, db.p, d_A, d_B, d_C,
+                                                                alpha, beta, d_flags);
         }
         HIFDE_LAUNCH_CHECK();
+        if (kt) kt->end(s, name, flops, bytes);
     }
 }
+
 
 template <class T>
 void batched_gemm(bool transA, int nbox, const int* h_m, const int* h_n, const int* h_k,

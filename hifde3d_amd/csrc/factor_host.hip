@@ -27,6 +27,11 @@ thread_local Stager* t_stager = nullptr;
 }
 Stager* active_stager() { return t_stager; }
 void set_active_stager(Stager* st) { t_stager = st; }
+namespace {
+thread_local KernelTimer* t_timer = nullptr;
+}
+KernelTimer* active_timer() { return t_timer; }
+void set_active_timer(KernelTimer* t) { t_timer = t; }
 
 const void* Stager::stage(const void* src, size_t bytes) {
     const size_t need = (bytes + 255) & ~(size_t)255;
@@ -222,6 +227,31 @@ struct hifde_factor {
             f->pending.push_back({fam, e0, e1, flops, bytes});
         }
     };
+    // Per-launch kernel timing (families named "kernel:<name>").
+    struct LaunchTimer : KernelTimer {
+        hifde_factor* f;
+        hipEvent_t e0 = nullptr;
+        explicit LaunchTimer(hifde_factor* f_) : f(f_) {}
+        void begin(hipStream_t s) override {
+            e0 = f->take_event();
+            HIFDE_CHECK(hipEventRecord(e0, s));
+        }
+        void end(hipStream_t s, const char* kernel, double flops, double bytes) override {
+            hipEvent_t e1 = f->take_event();
+            HIFDE_CHECK(hipEventRecord(e1, s));
+            f->pending.push_back({f->family(std::string("kernel:") + kernel), e0, e1, flops, bytes});
+        }
+    };
+    LaunchTimer launch_timer{this};
+    // Installs the launch timer for the duration of a profiled call.
+    struct TimerGuard {
+        KernelTimer* prev;
+        TimerGuard(hifde_factor* f) : prev(active_timer()) {
+            if (f->profiling) set_active_timer(&f->launch_timer);
+        }
+        ~TimerGuard() { set_active_timer(prev); }
+    };
+
     void resolve_profile() {
         if (pending.empty()) return;
         HIFDE_CHECK(hipStreamSynchronize(stream));
@@ -1171,6 +1201,7 @@ struct hifde_factor {
             explicit StagerGuard(Stager* s) : prev(active_stager()) { set_active_stager(s); }
             ~StagerGuard() { set_active_stager(prev); }
         } guard(&stager);
+        TimerGuard tguard(this);
         d_status.reserve(1);
         fill_int(d_status.p, 1, 0, stream);
         // Initialize (src/Factor.cpp:22-34)
@@ -1242,16 +1273,18 @@ struct hifde_factor {
                        B.d_s.p, B.d_red_off.p, B.d_skel_off.p, B.d_red_gid.p, B.d_skel_gid.p,
                        pass >= 0 ? B.d_skel_pass.p : nullptr, B.d_off_rs.p,
                        reinterpret_cast<const T*>(Mbuf.p), op, pass, alpha, u, u,
-                       in_red ? B.max_r : B.max_s, stream);
+                       in_red ? B.max_r : B.max_s, 2.0 * B.tot_rs, by, stream);
     }
     template <class T>
     void diag_t(BoxBatch& B, const DevBuf<unsigned char>& Mbuf, const double* u, double* out) {
         if (B.nbox == 0) return;
-        Prof pr(this, fam<T>("apply_diag"), 2.0 * B.tot_rr,
-                (double)B.tot_rr * sizeof(T) + 16.0 * (B.red_off.empty() ? 0 : B.red_off.back()));
+        const double by =
+            (double)B.tot_rr * sizeof(T) + 16.0 * (B.red_off.empty() ? 0 : B.red_off.back());
+        Prof pr(this, fam<T>("apply_diag"), 2.0 * B.tot_rr, by);
         apply_tiles<T>(B.nbox, B.tiles_r.back(), B.d_tiles_r.p, B.d_r.p, B.d_s.p, B.d_red_off.p,
                        B.d_skel_off.p, B.d_red_gid.p, B.d_skel_gid.p, nullptr, B.d_off_rr.p,
-                       reinterpret_cast<const T*>(Mbuf.p), 4, -1, 1.0, u, out, B.max_r, stream);
+                       reinterpret_cast<const T*>(Mbuf.p), 4, -1, 1.0, u, out, B.max_r,
+                       2.0 * B.tot_rr, by, stream);
     }
     // Apply tiling of a planned batch.
     void finalize_tiles(BoxBatch& B) {
@@ -1321,6 +1354,7 @@ struct hifde_factor {
     }
 
     void apply_dev(double* d_u, int nrhs, long long ldu, bool inv) {
+        TimerGuard tguard(this);
         if (!factored) throw Error(HIFDE_ERR_STATE, "apply: factor first");
         apply_tmp.reserve((size_t)NC * NC * NC);
         for (int c = 0; c < nrhs; ++c) {
@@ -1349,6 +1383,7 @@ struct hifde_factor {
     }
 
     void pcg_dev(const double* b, double* x, double tol, int maxit, int* iters, double* relres) {
+        TimerGuard tguard(this);
         if (!factored) throw Error(HIFDE_ERR_STATE, "pcg: factor first");
         const long long n = (long long)NC * NC * NC;
         pcg_buf.reserve(4 * n);

@@ -51,6 +51,17 @@ inline void upload_bytes(void* dst, const void* src, size_t bytes, hipStream_t s
     HIFDE_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
 }
 
+// Per-launch device timing of individual kernels: HIP events recorded on the
+// launch stream around each launch, with the launch's algorithmic flops and
+// bytes.  Active while a profiled factorization or solve runs.
+struct KernelTimer {
+    virtual ~KernelTimer() = default;
+    virtual void begin(hipStream_t s) = 0;
+    virtual void end(hipStream_t s, const char* kernel, double flops, double bytes) = 0;
+};
+KernelTimer* active_timer();
+void set_active_timer(KernelTimer* t);
+
 // Owning device buffer.  Grows, never shrinks, unless release() is called.
 template <class T>
 struct DevBuf {
@@ -328,7 +339,7 @@ void apply_tiles(int nbox, int ntiles, const int* d_tile_off, const int* d_r, co
                  const int* d_red_off, const int* d_skel_off, const int* d_red_gid,
                  const int* d_skel_gid, const unsigned char* d_skel_pass, const long long* d_off,
                  const T* d_M, int op, int pass, double alpha, const double* d_in,
-                 double* d_out, int max_in, hipStream_t s);
+                 double* d_out, int max_in, double flops, double bytes, hipStream_t s);
 void zero_boundary(double* d_u, int NC, hipStream_t s);
 
 // vectors / CSR SpMV for PCG
