@@ -103,7 +103,8 @@ __global__ void __launch_bounds__(256) k_apply_tiles(
         }
     } else if (nin <= 64) {
         // y_j = sum_i M(i,j) x_i with short columns: one thread per output
-        if (tid < cnt) {
+        // (outputs of the other pass are not formed)
+        if (tid < cnt && !(sp && pass >= 0 && (int)sp[o0 + tid] != pass)) {
             o = o0 + tid;
             const T* Mj = M + (size_t)o * nin;
             for (int i = 0; i < nin; ++i) acc += (double)Mj[i] * xs[i];
@@ -112,6 +113,7 @@ __global__ void __launch_bounds__(256) k_apply_tiles(
         // long columns: one wave per output, lanes over the column
         const int lane = tid & (kWave - 1), w = tid / kWave;
         for (int jj = w; jj < cnt; jj += 4) {
+            if (sp && pass >= 0 && (int)sp[o0 + jj] != pass) continue;
             const T* Mj = M + (size_t)(o0 + jj) * nin;
             double p4[4] = {0, 0, 0, 0};
             int i = lane;
@@ -222,6 +224,70 @@ int grid_for(long long n, int block = 256, int cap = 4096) {
 
 }  // namespace
 
+// Boxes whose inputs and outputs are at most 64 long (the level-0 boxes):
+// one thread per output over the flattened outputs of the batch, same ops and
+// pass rule as k_apply_tiles, inputs read from the vector directly.
+template <class T>
+__global__ void __launch_bounds__(256) k_apply_small(
+    int nbox, int nout_total, const int* __restrict__ dr, const int* __restrict__ ds,
+    const int* __restrict__ red_off, const int* __restrict__ skel_off,
+    const int* __restrict__ red_gid, const int* __restrict__ skel_gid,
+    const unsigned char* __restrict__ skel_pass, const long long* __restrict__ off,
+    const T* __restrict__ Mb, int op, int pass, double alpha, const double* u_in,
+    double* u_out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nout_total) return;
+    const bool out_red = (op == 1 || op == 2 || op == 4);
+    const int* oof = out_red ? red_off : skel_off;
+    const int b = find_box(oof, nbox, t);
+    const int o = t - oof[b];
+    const int r = dr[b], s = ds[b];
+    if (r == 0 || (op != 4 && s == 0)) return;
+    if (op == 0 && skel_pass && pass >= 0 && (int)skel_pass[skel_off[b] + o] != pass) return;
+    const T* M = Mb + off[b];
+    const int* rg = red_gid + red_off[b];
+    const int* sg = skel_gid + skel_off[b];
+    double acc = 0.0;
+    switch (op) {
+    case 0:  // M r x s, column o, inputs red
+        for (int i = 0; i < r; ++i) acc += (double)M[i + (size_t)o * r] * u_in[rg[i]];
+        break;
+    case 1:  // M r x s, row o, inputs skel
+        for (int j = 0; j < s; ++j) acc += (double)M[o + (size_t)j * r] * u_in[sg[j]];
+        break;
+    case 2:  // M s x r, column o, inputs skel
+        for (int i = 0; i < s; ++i) acc += (double)M[i + (size_t)o * s] * u_in[sg[i]];
+        break;
+    case 3:  // M s x r, row o, inputs red
+        for (int j = 0; j < r; ++j) acc += (double)M[o + (size_t)j * s] * u_in[rg[j]];
+        break;
+    default:  // 4: M r x r, row o
+        for (int j = 0; j < r; ++j) acc += (double)M[o + (size_t)j * r] * u_in[rg[j]];
+        break;
+    }
+    const int g = out_red ? rg[o] : sg[o];
+    if (op == 4) u_out[g] = acc;
+    else u_out[g] += alpha * acc;
+}
+
+template <class T>
+void apply_small(int nbox, int nout_total, const int* d_r, const int* d_s, const int* d_red_off,
+                 const int* d_skel_off, const int* d_red_gid, const int* d_skel_gid,
+                 const unsigned char* d_skel_pass, const long long* d_off, const T* d_M, int op,
+                 int pass, double alpha, const double* d_in, double* d_out, double flops,
+                 double bytes, hipStream_t s) {
+    if (nbox <= 0 || nout_total <= 0) return;
+    KernelTimer* kt = active_timer();
+    if (kt) kt->begin(s);
+    k_apply_small<T><<<(nout_total + 255) / 256, 256, 0, s>>>(
+        nbox, nout_total, d_r, d_s, d_red_off, d_skel_off, d_red_gid, d_skel_gid, d_skel_pass,
+        d_off, d_M, op, pass, alpha, d_in, d_out);
+    HIFDE_LAUNCH_CHECK();
+    if (kt)
+        kt->end(s, sizeof(T) == sizeof(float) ? "k_apply_small<float>" : "k_apply_small<double>",
+                flops, bytes);
+}
+
 template <class T>
 void apply_tiles(int nbox, int ntiles, const int* d_tile_off, const int* d_r, const int* d_s,
                  const int* d_red_off, const int* d_skel_off, const int* d_red_gid,
@@ -281,6 +347,14 @@ void vcopy(const double* x, double* y, long long n, hipStream_t s) {
     HIFDE_LAUNCH_CHECK();
 }
 
+template void apply_small<double>(int, int, const int*, const int*, const int*, const int*,
+                                  const int*, const int*, const unsigned char*, const long long*,
+                                  const double*, int, int, double, const double*, double*, double,
+                                  double, hipStream_t);
+template void apply_small<float>(int, int, const int*, const int*, const int*, const int*,
+                                 const int*, const int*, const unsigned char*, const long long*,
+                                 const float*, int, int, double, const double*, double*, double,
+                                 double, hipStream_t);
 template void apply_tiles<double>(int, int, const int*, const int*, const int*, const int*,
                                   const int*, const int*, const int*, const unsigned char*,
                                   const long long*, const double*, int, int, double,

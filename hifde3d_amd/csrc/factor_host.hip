@@ -1269,6 +1269,14 @@ struct hifde_factor {
                     16.0 * ((B.red_off.empty() ? 0 : B.red_off.back()) +
                             (B.skel_off.empty() ? 0 : B.skel_off.back()));
         Prof pr(this, fam<T>("apply_update"), 2.0 * B.tot_rs, by);
+        if (B.max_r <= 64 && B.max_s <= 64) {
+            const int nout = out_red ? B.red_off.back() : B.skel_off.back();
+            apply_small<T>(B.nbox, nout, B.d_r.p, B.d_s.p, B.d_red_off.p, B.d_skel_off.p,
+                           B.d_red_gid.p, B.d_skel_gid.p, pass >= 0 ? B.d_skel_pass.p : nullptr,
+                           B.d_off_rs.p, reinterpret_cast<const T*>(Mbuf.p), op, pass, alpha, u,
+                           u, 2.0 * B.tot_rs, by, stream);
+            return;
+        }
         apply_tiles<T>(B.nbox, tl.back(), out_red ? B.d_tiles_r.p : B.d_tiles_s.p, B.d_r.p,
                        B.d_s.p, B.d_red_off.p, B.d_skel_off.p, B.d_red_gid.p, B.d_skel_gid.p,
                        pass >= 0 ? B.d_skel_pass.p : nullptr, B.d_off_rs.p,
@@ -1281,6 +1289,13 @@ struct hifde_factor {
         const double by =
             (double)B.tot_rr * sizeof(T) + 16.0 * (B.red_off.empty() ? 0 : B.red_off.back());
         Prof pr(this, fam<T>("apply_diag"), 2.0 * B.tot_rr, by);
+        if (B.max_r <= 64) {
+            apply_small<T>(B.nbox, B.red_off.back(), B.d_r.p, B.d_s.p, B.d_red_off.p,
+                           B.d_skel_off.p, B.d_red_gid.p, B.d_skel_gid.p, nullptr, B.d_off_rr.p,
+                           reinterpret_cast<const T*>(Mbuf.p), 4, -1, 1.0, u, out, 2.0 * B.tot_rr,
+                           by, stream);
+            return;
+        }
         apply_tiles<T>(B.nbox, B.tiles_r.back(), B.d_tiles_r.p, B.d_r.p, B.d_s.p, B.d_red_off.p,
                        B.d_skel_off.p, B.d_red_gid.p, B.d_skel_gid.p, nullptr, B.d_off_rr.p,
                        reinterpret_cast<const T*>(Mbuf.p), 4, -1, 1.0, u, out, B.max_r,
