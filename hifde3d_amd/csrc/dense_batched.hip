@@ -1140,87 +1140,118 @@ void batched_gj_inverse(int nbox, const int* h_n, const long long* h_off, const 
     T* RO = reinterpret_cast<T*>(ws.RO.p);
     T* CW = reinterpret_cast<T*>(ws.CW.p);
     int* cinfo = ws.info.p;
+    // The same walk runs twice: first it records every GEMM and strided copy
+    // (uploaded in one copy each), then it launches them in order between
+    // the window kernels.
+    GemmPlan& plan = ws.plan;
+    plan.clear();
+    std::vector<std::pair<int, int>> gh;  // GEMM launch ranges in walk order
+    std::vector<Copy2D> cps;
+    std::vector<int> cfirst, cnum, cmaxc;  // per outer step: copy range and widest copy
     std::vector<GemmDesc> upd, g1, g2, g3;
-    std::vector<Copy2D> cp;
-    for (int so = 0; so < osteps; ++so) {
-        const int K0 = so * kGJPanel;
-        const size_t w0 = (size_t)so * nbig;
-        const int* dwn = ws.wn.p + w0;
-        const int* dwld = ws.wld.p + w0;
-        const long long* dwoff = ws.woff.p + w0;
-        int max_w = 0;
-        for (int j = 0; j < nbig; ++j) max_w = std::max(max_w, wn[w0 + j]);
-        // inner level: every window inverted in place
-        const int isteps = (max_w + kGJB - 1) / kGJB;
-        const dim3 gch((max_w + 63) / 64, nbig);
-        for (int st = 0; st < isteps; ++st) {
-            k_bgj_diag<T><<<nbig, kWave, 0, s>>>(dwn, dwoff, dwld, d_A, Dws, st, K0, cinfo);
-            k_bgj_row<T><<<gch, 256, 0, s>>>(dwn, dwoff, dwld, d_A, Dws, Rws, ws.b_off.p, st,
-                                              cinfo);
-            // A(:, J) -= A(:, K) R(:, J) over the window's columns left and
-            // right of the block; the block rows are then set to R.
-            upd.clear();
-            for (int j = 0; j < nbig; ++j) {
-                const size_t w = w0 + j;
-                const int nw = wn[w], k0 = st * kGJB;
-                if (k0 >= nw) continue;
-                const int kb = std::min(kGJB, nw - k0), ld = wld[w];
-                const long long o = woff[w], ro = r_off[j];
-                if (k0 > 0) upd.push_back({o + (long long)k0 * ld, ro, o, nw, k0, kb, ld, kGJB, ld});
-                const int j1 = k0 + kb;
-                if (j1 < nw)
-                    upd.push_back({o + (long long)k0 * ld, ro + (long long)j1 * kGJB,
-                                   o + (long long)j1 * ld, nw, nw - j1, kb, ld, kGJB, ld});
-            }
-            batched_gemm_desc<T>(false, upd, d_A, Rws, d_A, (T)-1, (T)1, gws, s);
-            k_bgj_rowcopy<T><<<dim3((kGJB * max_w + 255) / 256, nbig), 256, 0, s>>>(
-                dwn, dwoff, dwld, d_A, Rws, ws.b_off.p, st, cinfo);
-            k_bgj_col<T><<<gch, 256, 0, s>>>(dwn, dwoff, dwld, d_A, Dws, st, cinfo);
-        }
-        // outer level, boxes wider than one window
-        g1.clear();
-        g2.clear();
-        g3.clear();
-        cp.clear();
-        for (int j = 0; j < nbig; ++j) {
-            const int n = h_n[bix[j]];
-            const int kb = wn[w0 + j];
-            if (kb == 0 || n <= kGJPanel) continue;
-            const long long o = h_off[bix[j]], dk = woff[w0 + j];
-            const long long ro = ro_off[j], co = cw_off[j];
-            const int j1 = K0 + kb;
-            const int lo[2] = {0, j1}, len[2] = {K0, n - j1};
-            for (int q = 0; q < 2; ++q) {
-                if (len[q] == 0) continue;
-                // R(:, J) = D A(K, J)
-                g1.push_back({dk, o + K0 + (long long)lo[q] * n, ro + (long long)lo[q] * kGJPanel,
-                              kb, len[q], kb, n, n, kGJPanel});
-                // C(I, :) = -A(I, K) D
-                g3.push_back({o + lo[q] + (long long)K0 * n, dk, co + lo[q], len[q], kb, kb, n, n,
-                              n});
-                // A(I, K) = C(I, :);  A(K, J) = R(:, J)
-                cp.push_back({co + lo[q], o + lo[q] + (long long)K0 * n, len[q], kb, n, n, 0});
-                cp.push_back({ro + (long long)lo[q] * kGJPanel, o + K0 + (long long)lo[q] * n, kb,
-                              len[q], kGJPanel, n, 1});
-                // A(I, J) -= A(I, K) R(:, J)
-                for (int r = 0; r < 2; ++r) {
-                    if (len[r] == 0) continue;
-                    g2.push_back({o + lo[r] + (long long)K0 * n, ro + (long long)lo[q] * kGJPanel,
-                                  o + lo[r] + (long long)lo[q] * n, len[r], len[q], kb, n,
-                                  kGJPanel, n});
+    for (int exec = 0; exec < 2; ++exec) {
+        size_t gi = 0;
+        auto gemm = [&](const std::vector<GemmDesc>& d, const T* A, const T* B, T* C, T alpha,
+                        T beta) {
+            if (!exec) gh.push_back(plan.add(d, false));
+            else gemm_run<T, T>(plan, gh[gi++], A, B, C, alpha, beta, s);
+        };
+        for (int so = 0; so < osteps; ++so) {
+            const int K0 = so * kGJPanel;
+            const size_t w0 = (size_t)so * nbig;
+            const int* dwn = ws.wn.p + w0;
+            const int* dwld = ws.wld.p + w0;
+            const long long* dwoff = ws.woff.p + w0;
+            int max_w = 0;
+            for (int j = 0; j < nbig; ++j) max_w = std::max(max_w, wn[w0 + j]);
+            // inner level: every window inverted in place
+            const int isteps = (max_w + kGJB - 1) / kGJB;
+            const dim3 gch((max_w + 63) / 64, nbig);
+            for (int st = 0; st < isteps; ++st) {
+                if (exec) {
+                    k_bgj_diag<T><<<nbig, kWave, 0, s>>>(dwn, dwoff, dwld, d_A, Dws, st, K0,
+                                                          cinfo);
+                    k_bgj_row<T><<<gch, 256, 0, s>>>(dwn, dwoff, dwld, d_A, Dws, Rws, ws.b_off.p,
+                                                      st, cinfo);
+                }
+                // A(:, J) -= A(:, K) R(:, J) over the window's columns left and
+                // right of the block; the block rows are then set to R.
+                upd.clear();
+                for (int j = 0; j < nbig && !exec; ++j) {
+                    const size_t w = w0 + j;
+                    const int nw = wn[w], k0 = st * kGJB;
+                    if (k0 >= nw) continue;
+                    const int kb = std::min(kGJB, nw - k0), ld = wld[w];
+                    const long long o = woff[w], ro = r_off[j];
+                    if (k0 > 0) upd.push_back({o + (long long)k0 * ld, ro, o, nw, k0, kb, ld, kGJB, ld});
+                    const int j1 = k0 + kb;
+                    if (j1 < nw)
+                        upd.push_back({o + (long long)k0 * ld, ro + (long long)j1 * kGJB,
+                                       o + (long long)j1 * ld, nw, nw - j1, kb, ld, kGJB, ld});
+                }
+                gemm(upd, d_A, Rws, d_A, (T)-1, (T)1);
+                if (exec) {
+                    k_bgj_rowcopy<T><<<dim3((kGJB * max_w + 255) / 256, nbig), 256, 0, s>>>(
+                        dwn, dwoff, dwld, d_A, Rws, ws.b_off.p, st, cinfo);
+                    k_bgj_col<T><<<gch, 256, 0, s>>>(dwn, dwoff, dwld, d_A, Dws, st, cinfo);
                 }
             }
+            // outer level, boxes wider than one window
+            g1.clear();
+            g2.clear();
+            g3.clear();
+            if (!exec) {
+                cfirst.push_back((int)cps.size());
+                int maxc = 0;
+                for (int j = 0; j < nbig; ++j) {
+                    const int n = h_n[bix[j]];
+                    const int kb = wn[w0 + j];
+                    if (kb == 0 || n <= kGJPanel) continue;
+                    const long long o = h_off[bix[j]], dk = woff[w0 + j];
+                    const long long ro = ro_off[j], co = cw_off[j];
+                    const int j1 = K0 + kb;
+                    const int lo[2] = {0, j1}, len[2] = {K0, n - j1};
+                    for (int q = 0; q < 2; ++q) {
+                        if (len[q] == 0) continue;
+                        // R(:, J) = D A(K, J)
+                        g1.push_back({dk, o + K0 + (long long)lo[q] * n,
+                                      ro + (long long)lo[q] * kGJPanel, kb, len[q], kb, n, n,
+                                      kGJPanel});
+                        // C(I, :) = -A(I, K) D
+                        g3.push_back({o + lo[q] + (long long)K0 * n, dk, co + lo[q], len[q], kb,
+                                      kb, n, n, n});
+                        // A(I, K) = C(I, :);  A(K, J) = R(:, J)
+                        cps.push_back({co + lo[q], o + lo[q] + (long long)K0 * n, len[q], kb, n,
+                                       n, 0});
+                        cps.push_back({ro + (long long)lo[q] * kGJPanel,
+                                       o + K0 + (long long)lo[q] * n, kb, len[q], kGJPanel, n, 1});
+                        maxc = std::max(maxc, std::max(kb, len[q]));
+                        // A(I, J) -= A(I, K) R(:, J)
+                        for (int r = 0; r < 2; ++r) {
+                            if (len[r] == 0) continue;
+                            g2.push_back({o + lo[r] + (long long)K0 * n,
+                                          ro + (long long)lo[q] * kGJPanel,
+                                          o + lo[r] + (long long)lo[q] * n, len[r], len[q], kb, n,
+                                          kGJPanel, n});
+                        }
+                    }
+                }
+                cnum.push_back((int)cps.size() - cfirst.back());
+                cmaxc.push_back(maxc);
+            }
+            gemm(g1, d_A, d_A, RO, (T)1, (T)0);
+            gemm(g2, d_A, RO, d_A, (T)-1, (T)1);
+            gemm(g3, d_A, d_A, CW, (T)-1, (T)0);
+            if (exec && cnum[so] > 0) {
+                k_copy2d<T><<<dim3((unsigned)cnum[so], (cmaxc[so] + 3) / 4), 256, 0, s>>>(
+                    ws.cdesc.p + cfirst[so], CW, RO, d_A);
+                HIFDE_LAUNCH_CHECK();
+            }
         }
-        if (g1.empty()) continue;
-        batched_gemm_desc<T>(false, g1, d_A, d_A, RO, (T)1, (T)0, gws, s);
-        batched_gemm_desc<T>(false, g2, d_A, RO, d_A, (T)-1, (T)1, gws, s);
-        batched_gemm_desc<T>(false, g3, d_A, d_A, CW, (T)-1, (T)0, gws, s);
-        int maxc = 0;
-        for (const Copy2D& c : cp) maxc = std::max(maxc, c.cols);
-        ws.cdesc.upload(cp, s);
-        k_copy2d<T><<<dim3((unsigned)cp.size(), (maxc + 3) / 4), 256, 0, s>>>(ws.cdesc.p, CW, RO,
-                                                                             d_A);
-        HIFDE_LAUNCH_CHECK();
+        if (!exec) {
+            plan.upload(s);
+            ws.cdesc.upload(cps, s);
+        }
     }
     k_scatter_info<<<(nbig + 255) / 256, 256, 0, s>>>(nbig, ws.bix.p, cinfo, d_info);
     HIFDE_LAUNCH_CHECK();
@@ -1250,68 +1281,86 @@ constexpr const char* type_name() {
     return sizeof(T) == sizeof(float) ? "float" : "double";
 }
 
+std::pair<int, int> GemmPlan::add(const std::vector<GemmDesc>& d, bool transA) {
+    constexpr int TILE = kMfmaTile;
+    const int first = (int)launches.size();
+    for (int pass = 0; pass < 2; ++pass) {
+        const int d0 = (int)desc.size(), t0 = (int)tiles.size();
+        const int tile = pass == 0 ? BM : TILE;
+        Launch L{pass, transA ? 1 : 0, 0, d0, t0, 0, 0.0, 0.0, 0.0};
+        tiles.push_back(0);
+        for (const GemmDesc& g : d) {
+            if (g.m <= 0 || g.n <= 0) continue;
+            const bool large = g.m >= TILE / 2 && g.n >= TILE / 2;
+            if (large != (pass == 1)) continue;
+            const int tm = (g.m + tile - 1) / tile, tn = (g.n + tile - 1) / tile;
+            desc.push_back(g);
+            tiles.push_back(tiles.back() + tm * tn);
+            ++L.nb;
+            const double cells = g.lower ? 0.5 * g.m * (g.n + 1.0) : (double)g.m * g.n;
+            L.flops += 2.0 * cells * g.k;
+            L.cells += cells;
+            L.ab_elems += (double)g.m * g.k + (double)g.k * g.n;
+        }
+        if (L.nb == 0) {
+            tiles.pop_back();
+            continue;
+        }
+        L.total = tiles.back();
+        launches.push_back(L);
+    }
+    return {first, (int)launches.size()};
+}
+
+template <class T, class TC>
+void gemm_run(const GemmPlan& p, std::pair<int, int> h, const T* d_A, const T* d_B, TC* d_C,
+              T alpha, T beta, hipStream_t s, const int* d_flags) {
+    KernelTimer* kt = active_timer();
+    for (int li = h.first; li < h.second; ++li) {
+        const GemmPlan::Launch& L = p.launches[li];
+        const GemmDesc* dd = p.d_desc.p + L.d0;
+        const int* tt = p.d_tiles.p + L.t0;
+        if (kt) kt->begin(s);
+        char name[96];
+        const char* tn = type_name<T>();
+        const char* cn = type_name<TC>();
+        const char* ta = L.transA ? "true" : "false";
+        if (!L.mfma) {
+            std::snprintf(name, sizeof name, "k_gemm<%s, %s, %s>", tn, ta, cn);
+            if (L.transA)
+                k_gemm<T, true, TC><<<L.total, 256, 0, s>>>(L.nb, tt, dd, d_A, d_B, d_C, alpha,
+                                                            beta, d_flags);
+            else
+                k_gemm<T, false, TC><<<L.total, 256, 0, s>>>(L.nb, tt, dd, d_A, d_B, d_C, alpha,
+                                                             beta, d_flags);
+        } else {
+            std::snprintf(name, sizeof name, "k_gemm_mfma<%s, %s, %s>", tn, ta, cn);
+            if (L.transA)
+                k_gemm_mfma<T, true, TC><<<L.total, 256, 0, s>>>(L.nb, tt, dd, d_A, d_B, d_C,
+                                                                 alpha, beta, d_flags);
+            else
+                k_gemm_mfma<T, false, TC><<<L.total, 256, 0, s>>>(L.nb, tt, dd, d_A, d_B, d_C,
+                                                                  alpha, beta, d_flags);
+        }
+        HIFDE_LAUNCH_CHECK();
+        if (kt)
+            kt->end(s, name, L.flops,
+                    L.ab_elems * sizeof(T) + L.cells * sizeof(TC) * (beta == (T)0 ? 1 : 2));
+    }
+}
+
 template <class T, class TC>
 void batched_gemm_desc(bool transA, const std::vector<GemmDesc>& desc, const T* d_A, const T* d_B,
                        TC* d_C, T alpha, T beta, GemmWork& ws, hipStream_t s,
                        const int* d_flags) {
     if (desc.empty()) return;
-    // Boxes whose output spans at least half an MFMA workgroup tile in both
-    // directions go to the MFMA kernel, the rest to the 64x64 FMA kernel.
-    constexpr int TILE = kMfmaTile;
-    std::vector<GemmDesc> small, large;
-    for (const GemmDesc& d : desc) {
-        if (d.m <= 0 || d.n <= 0) continue;
-        if (d.m >= TILE / 2 && d.n >= TILE / 2) large.push_back(d);
-        else small.push_back(d);
-    }
-    KernelTimer* kt = active_timer();
-    for (int pass = 0; pass < 2; ++pass) {
-        const std::vector<GemmDesc>& dv = pass == 0 ? small : large;
-        const int nb = (int)dv.size();
-        if (nb == 0) continue;
-        const int tile = pass == 0 ? BM : TILE;
-        std::vector<int> toff(nb + 1, 0);
-        double flops = 0, bytes = 0;
-        for (int b = 0; b < nb; ++b) {
-            const GemmDesc& d = dv[b];
-            const int tm = (d.m + tile - 1) / tile, tn = (d.n + tile - 1) / tile;
-            toff[b + 1] = toff[b] + tm * tn;
-            // algorithmic work: the output entries the launch forms
-            const double cells = d.lower ? 0.5 * d.m * (d.n + 1.0) : (double)d.m * d.n;
-            flops += 2.0 * cells * d.k;
-            bytes += ((double)d.m * d.k + (double)d.k * d.n) * sizeof(T) +
-                     cells * sizeof(TC) * (beta == (T)0 ? 1 : 2);
-        }
-        DevBuf<int>& tb = pass == 0 ? ws.tiles : ws.tiles2;
-        DevBuf<GemmDesc>& db = pass == 0 ? ws.desc : ws.desc2;
-        tb.upload(toff, s);
-        db.upload(dv, s);
-        const int total = toff[nb];
-        if (kt) kt->begin(s);
-        char name[96];
-        if (pass == 0) {
-            std::snprintf(name, sizeof name, "k_gemm<%s, %s, %s>", type_name<T>(),
-                          transA ? "true" : "false", type_name<TC>());
-            if (transA)
-                k_gemm<T, true, TC><<<total, 256, 0, s>>>(nb, tb.p, db.p, d_A, d_B, d_C, alpha,
-                                                          beta, d_flags);
-            else
-                k_gemm<T, false, TC><<<total, 256, 0, s>>>(nb, tb.p, db.p, d_A, d_B, d_C, alpha,
-                                                           beta, d_flags);
-        } else {
-            std::snprintf(name, sizeof name, "k_gemm_mfma<%s, %s, %s>", type_name<T>(),
-                          transA ? "true" : "false", type_name<TC>());
-            if (transA)
-                k_gemm_mfma<T, true, TC><<<total, 256, 0, s>>>(nb, tb.p, db.p, d_A, d_B, d_C,
-                                                               alpha, beta, d_flags);
-            else
-                k_gemm_mfma<T, false, TC><<<total, 256, 0, s>>>(nb, tb.p, db.p, d_A, d_B, d_C,
-                                                                alpha, beta, d_flags);
-        }
-        HIFDE_LAUNCH_CHECK();
-        if (kt) kt->end(s, name, flops, bytes);
-    }
+    ws.plan.clear();
+    const std::pair<int, int> h = ws.plan.add(desc, transA);
+    if (h.first == h.second) return;
+    ws.plan.upload(s);
+    gemm_run<T, TC>(ws.plan, h, d_A, d_B, d_C, alpha, beta, s, d_flags);
 }
+
 
 
 template <class T>
@@ -1387,14 +1436,17 @@ void gram_pivchol(int nface, const int* h_n, const unsigned char* h_mode, const 
     ws.active.upload(act, s);
     ws.count.reserve(1);
     ws.host_active.resize(1);
+    // the trailing update has the same descriptors after every panel
+    ws.plan.clear();
+    const std::pair<int, int> uh = ws.plan.add(upd, true);
+    ws.plan.upload(s);
     for (int j0 = 0; j0 < max_n; j0 += kGramNB) {
         k_gram_panel<<<nface, 256, shm, s>>>(d_n, d_mode, d_G, d_offG, d_R, ws.P.p, ws.offP.p,
                                              ws.dg.p, d_perm, d_perm_off, d_k, ws.active.p,
                                              ws.tol.p, eps, j0);
         HIFDE_LAUNCH_CHECK();
         if (j0 + kGramNB >= max_n) break;
-        batched_gemm_desc<double>(true, upd, ws.P.p, ws.P.p, d_G, -1.0, 1.0, gws, s,
-                                  ws.active.p);
+        gemm_run<double, double>(ws.plan, uh, ws.P.p, ws.P.p, d_G, -1.0, 1.0, s, ws.active.p);
         // every fourth panel, stop once every face has found its rank
         if ((j0 / kGramNB) % 4 == 3) {
             fold_active(nface, ws.active.p, ws.count.p, s);
@@ -1479,6 +1531,14 @@ template void batched_gj_inverse<float>(int, const int*, const long long*, const
                                         const long long*, float*, int*, GJWork&, GemmWork&,
                                         hipStream_t);
 
+template void gemm_run<double, double>(const GemmPlan&, std::pair<int, int>, const double*,
+                                       const double*, double*, double, double, hipStream_t,
+                                       const int*);
+template void gemm_run<float, float>(const GemmPlan&, std::pair<int, int>, const float*,
+                                     const float*, float*, float, float, hipStream_t, const int*);
+template void gemm_run<float, double>(const GemmPlan&, std::pair<int, int>, const float*,
+                                      const float*, double*, float, float, hipStream_t,
+                                      const int*);
 template void batched_gemm_desc<double, double>(bool, const std::vector<GemmDesc>&,
                                                 const double*, const double*, double*, double,
                                                 double, GemmWork&, hipStream_t, const int*);

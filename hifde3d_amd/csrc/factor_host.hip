@@ -799,8 +799,7 @@ struct hifde_factor {
             for (size_t t = 0; t < (size_t)nb * 8; ++t) moff[t + 1] += moff[t];
             if (moff.back() > (long long)INT32_MAX) throw Error(HIFDE_ERR_SIZE, "extend-add maps");
             std::vector<int> maps(moff.back());
-            std::vector<ExtendDesc> pass[8];
-            for (int k = 0; k < 8; ++k) pass[k].resize(nb);
+            std::vector<std::vector<ExtendDesc>> pass(8, std::vector<ExtendDesc>(nb));
 #pragma omp parallel for schedule(dynamic, 4)
             for (int b = 0; b < nb; ++b)
                 for (int k = 0; k < 8; ++k) {
@@ -828,9 +827,8 @@ struct hifde_factor {
             fill_zero<T>(A21, B.tot_rs, stream);
             if (A12) fill_zero<T>(A12, B.tot_rs, stream);
             fill_zero<double>(E, B.tot_ee, stream);
-            for (int k = 0; k < 8; ++k)
-                extend_add<T>(pass[k], d_ext, d_eoff, d_emap.p, elem[elem_cur].p, A22, A21, A12,
-                              E, stream);
+            extend_add<T>(pass, d_ext, d_eoff, d_emap.p, elem[elem_cur].p, A22, A21, A12, E,
+                          stream);
         }
         elem_cur = ne;
         timed(ST_GATHER, t0);

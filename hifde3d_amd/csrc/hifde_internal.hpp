@@ -222,14 +222,6 @@ struct Copy2D {
     int rows, cols, lds, ldd, sel;
 };
 
-struct GJWork {
-    DevBuf<int> wn, wld;              // per outer step and box: window size, leading dim
-    DevBuf<long long> woff, b_off;    // window offsets; inner R offsets
-    DevBuf<unsigned char> B, D;       // inner R, inner diagonal inverses
-    DevBuf<unsigned char> RO, CW;     // outer R (kGJPanel x n), outer column panel (n x kGJPanel)
-    DevBuf<Copy2D> cdesc;
-    DevBuf<int> bix, info;            // blocked-path boxes and their status
-};
 
 struct GemmDesc {
     long long offA, offB, offC;
@@ -241,12 +233,51 @@ struct CopyDesc {
     long long off;  // source block base
     int r0, c0, rows, ld;
 };
+// Descriptors of many batched GEMM launches, built on the host, uploaded in
+// one copy each and then launched one by one (gemm_run).
+struct GemmPlan {
+    struct Launch {
+        int mfma, transA, nb, d0, t0, total;
+        double flops, ab_elems, cells;  // algorithmic work of the launch
+    };
+    std::vector<GemmDesc> desc;
+    std::vector<int> tiles;  // per launch: nb + 1 local tile offsets
+    std::vector<Launch> launches;
+    DevBuf<GemmDesc> d_desc;
+    DevBuf<int> d_tiles;
+    void clear() {
+        desc.clear();
+        tiles.clear();
+        launches.clear();
+    }
+    // Appends one batched GEMM (outputs narrower than half an MFMA tile on
+    // the FMA kernel, the rest on the MFMA kernel); returns its launches
+    // [first, second).
+    std::pair<int, int> add(const std::vector<GemmDesc>& d, bool transA);
+    void upload(hipStream_t s) {
+        d_desc.upload(desc, s);
+        d_tiles.upload(tiles, s);
+    }
+};
+template <class T, class TC = T>
+void gemm_run(const GemmPlan& p, std::pair<int, int> h, const T* d_A, const T* d_B, TC* d_C,
+              T alpha, T beta, hipStream_t s, const int* d_flags = nullptr);
+
 struct GemmWork {
-    DevBuf<int> tiles, tiles2;
-    DevBuf<GemmDesc> desc, desc2;
+    GemmPlan plan;
     DevBuf<CopyDesc> cdesc;
     DevBuf<long long> off;
 };
+struct GJWork {
+    DevBuf<int> wn, wld;              // per outer step and box: window size, leading dim
+    DevBuf<long long> woff, b_off;    // window offsets; inner R offsets
+    DevBuf<unsigned char> B, D;       // inner R, inner diagonal inverses
+    DevBuf<unsigned char> RO, CW;     // outer R (kGJPanel x n), outer column panel (n x kGJPanel)
+    DevBuf<Copy2D> cdesc;
+    DevBuf<int> bix, info;            // blocked-path boxes and their status
+    GemmPlan plan;                    // every GEMM of one inversion
+};
+
 // C = alpha op(A) B + beta C per descriptor (op(A) = A^T when transA).  C
 // may be FP64 for FP32 products (accumulated in FP32, added in FP64).
 template <class T, class TC = T>
@@ -282,6 +313,7 @@ struct GramRDesc {
     int k, n, perm;    // rank, columns, offset of the face's pivot order
 };
 struct GramWork {
+    GemmPlan plan;
     DevBuf<double> P, dg, tol, RK;
     DevBuf<long long> offP, roff, ko;
     DevBuf<int> active, count, kd, info;
@@ -314,8 +346,9 @@ template <class T>
 void frontal_init(int ncell, const InitDesc* d_desc, const int* d_tab, const int* d_gids,
                   const int* d_rp, const int* d_ci, const double* d_v, int NC, int w, int c,
                   T* A22, T* A21, T* A12, double* E, hipStream_t s);
+// One launch per pass (child position); each pass adds one child per parent.
 template <class Td>
-void extend_add(const std::vector<ExtendDesc>& desc, DevBuf<ExtendDesc>& dbuf,
+void extend_add(const std::vector<std::vector<ExtendDesc>>& passes, DevBuf<ExtendDesc>& dbuf,
                 DevBuf<long long>& obuf, const int* d_maps, const double* src, Td* A22, Td* A21,
                 Td* A12, double* E, hipStream_t s);
 // Uploads the face descriptors and their column prefix; returns the number of

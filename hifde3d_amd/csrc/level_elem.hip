@@ -280,18 +280,33 @@ void frontal_init(int ncell, const InitDesc* d_desc, const int* d_tab, const int
 }
 
 template <class Td>
-void extend_add(const std::vector<ExtendDesc>& desc, DevBuf<ExtendDesc>& dbuf,
+void extend_add(const std::vector<std::vector<ExtendDesc>>& passes, DevBuf<ExtendDesc>& dbuf,
                 DevBuf<long long>& obuf, const int* d_maps, const double* src, Td* A22, Td* A21,
                 Td* A12, double* E, hipStream_t s) {
-    if (desc.empty()) return;
-    const std::vector<long long> off =
-        col_prefix<ExtendDesc>(desc, [](const ExtendDesc& d) { return d.e; });
-    if (off.back() == 0) return;
-    dbuf.upload(desc, s);
+    // all passes' descriptors and column prefixes in one upload each
+    std::vector<ExtendDesc> all;
+    std::vector<long long> off;
+    std::vector<size_t> d0, o0;
+    std::vector<long long> tot;
+    for (const std::vector<ExtendDesc>& p : passes) {
+        d0.push_back(all.size());
+        o0.push_back(off.size());
+        all.insert(all.end(), p.begin(), p.end());
+        const std::vector<long long> po =
+            col_prefix<ExtendDesc>(p, [](const ExtendDesc& d) { return d.e; });
+        off.insert(off.end(), po.begin(), po.end());
+        tot.push_back(po.back());
+    }
+    if (all.empty()) return;
+    dbuf.upload(all, s);
     obuf.upload(off, s);
-    k_extend_add<Td><<<(unsigned)off.back(), 128, 0, s>>>((int)desc.size(), obuf.p, dbuf.p,
-                                                          d_maps, src, A22, A21, A12, E);
-    HIFDE_LAUNCH_CHECK();
+    for (size_t k = 0; k < passes.size(); ++k) {
+        if (passes[k].empty() || tot[k] == 0) continue;
+        k_extend_add<Td><<<(unsigned)tot[k], 128, 0, s>>>((int)passes[k].size(), obuf.p + o0[k],
+                                                          dbuf.p + d0[k], d_maps, src, A22, A21,
+                                                          A12, E);
+        HIFDE_LAUNCH_CHECK();
+    }
 }
 
 long long face_gather_prepare(const std::vector<FaceGatherDesc>& desc,
@@ -348,9 +363,9 @@ void mirror_lower(const std::vector<MirrorDesc>& desc, DevBuf<MirrorDesc>& dbuf,
     template void frontal_init<T>(int, const InitDesc*, const int*, const int*, const int*,      \
                                   const int*, const double*, int, int, int, T*, T*, T*,         \
                                   double*, hipStream_t);                                        \
-    template void extend_add<T>(const std::vector<ExtendDesc>&, DevBuf<ExtendDesc>&,            \
-                                DevBuf<long long>&, const int*, const double*, T*, T*, T*,      \
-                                double*, hipStream_t);                                          \
+    template void extend_add<T>(const std::vector<std::vector<ExtendDesc>>&,                    \
+                                DevBuf<ExtendDesc>&, DevBuf<long long>&, const int*,            \
+                                const double*, T*, T*, T*, double*, hipStream_t);               \
     template void face_add<T>(const std::vector<FaceAddDesc>&, DevBuf<FaceAddDesc>&,            \
                               DevBuf<long long>&, const int*, const T*, double*, int,           \
                               hipStream_t);
