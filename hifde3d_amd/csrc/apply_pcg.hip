@@ -53,7 +53,7 @@ __global__ void __launch_bounds__(256) k_apply_tiles(
     const int* __restrict__ skel_off, const int* __restrict__ red_gid,
     const int* __restrict__ skel_gid, const unsigned char* __restrict__ skel_pass,
     const long long* __restrict__ off, const T* __restrict__ Mb, int op, int pass, double alpha,
-    const double* u_in, double* u_out) {
+    const double* u_in, double* u_out, long long side_n) {
     const int t = blockIdx.x;
     const int b = find_box(tile_off, nbox, t);
     const int tile = t - tile_off[b];
@@ -110,30 +110,58 @@ __global__ void __launch_bounds__(256) k_apply_tiles(
             for (int i = 0; i < nin; ++i) acc += (double)Mj[i] * xs[i];
         }
     } else {
-        // long columns: one wave per output, lanes over the column
-        const int lane = tid & (kWave - 1), w = tid / kWave;
-        for (int jj = w; jj < cnt; jj += 4) {
-            if (sp && pass >= 0 && (int)sp[o0 + jj] != pass) continue;
-            const T* Mj = M + (size_t)(o0 + jj) * nin;
-            double p4[4] = {0, 0, 0, 0};
-            int i = lane;
-            for (; i + 3 * kWave < nin; i += 4 * kWave) {
+        // long columns: 64 x 64 blocks of M are read along the columns
+        // (coalesced), transposed through LDS, and each thread sums a
+        // quarter of every block for one output; outputs of the other pass
+        // are not read at all
+        __shared__ T blk[64][65];  // blk[output][row]
+        const int oo = tid & 63, g = tid >> 6;
+        bool keep = oo < cnt;
+        if (keep && sp && pass >= 0 && (int)sp[o0 + oo] != pass) keep = false;
+        // this thread loads rows ii = tid & 63 of the columns g + 4q, q < 16:
+        // which of them it reads is fixed for the whole tile
+        unsigned cmask = 0;
 #pragma unroll
-                for (int u = 0; u < 4; ++u) p4[u] += (double)Mj[i + u * kWave] * xs[i + u * kWave];
-            }
-            for (; i < nin; i += kWave) p4[0] += (double)Mj[i] * xs[i];
-            double a = wave_sum((p4[0] + p4[1]) + (p4[2] + p4[3]));
-            if (lane == 0) part[0][jj] = a;
+        for (int q = 0; q < 16; ++q) {
+            const int jj = g + 4 * q;
+            bool c = jj < cnt;
+            if (c && sp && pass >= 0 && (int)sp[o0 + jj] != pass) c = false;
+            cmask |= (c ? 1u : 0u) << q;
         }
+        double pacc = 0.0;
+        for (int i0 = 0; i0 < nin; i0 += 64) {
+            const int ib = min(64, nin - i0);
+            const bool row_ok = oo < ib;
+            T v[16];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int jj = g + 4 * q;
+                v[q] = (row_ok && ((cmask >> q) & 1u)) ? M[(size_t)(o0 + jj) * nin + i0 + oo]
+                                                       : (T)0;
+            }
+#pragma unroll
+            for (int q = 0; q < 16; ++q) blk[g + 4 * q][oo] = v[q];
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int ii = g * 16 + q;
+                if (ii < ib) pacc += (double)blk[oo][ii] * xs[i0 + ii];
+            }
+            __syncthreads();
+        }
+        part[g][oo] = pacc;
         __syncthreads();
-        if (tid < cnt) {
+        if (tid < cnt && keep) {
             o = o0 + tid;
-            acc = part[0][tid];
+            acc = part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
         }
     }
     if (o < 0) return;
     if (op == 4) {
         u_out[og[o]] = acc;
+    } else if (side_n > 0) {
+        // side buffers: the contribution goes to slot [pass flag][DOF]
+        u_out[(long long)sp[o] * side_n + og[o]] = alpha * acc;
     } else {
         if (sp && pass >= 0 && (int)sp[o] != pass) return;
         u_out[og[o]] += alpha * acc;
@@ -225,49 +253,58 @@ int grid_for(long long n, int block = 256, int cap = 4096) {
 }  // namespace
 
 // Boxes whose inputs and outputs are at most 64 long (the level-0 boxes):
-// one thread per output over the flattened outputs of the batch, same ops and
-// pass rule as k_apply_tiles, inputs read from the vector directly.
+// one wave per box, four boxes per workgroup.  The wave stages the box's
+// inputs in LDS; lane o forms output o.  Same ops and pass rule as
+// k_apply_tiles.
 template <class T>
 __global__ void __launch_bounds__(256) k_apply_small(
-    int nbox, int nout_total, const int* __restrict__ dr, const int* __restrict__ ds,
+    int nbox, const int* __restrict__ dr, const int* __restrict__ ds,
     const int* __restrict__ red_off, const int* __restrict__ skel_off,
     const int* __restrict__ red_gid, const int* __restrict__ skel_gid,
     const unsigned char* __restrict__ skel_pass, const long long* __restrict__ off,
     const T* __restrict__ Mb, int op, int pass, double alpha, const double* u_in,
-    double* u_out) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nout_total) return;
-    const bool out_red = (op == 1 || op == 2 || op == 4);
-    const int* oof = out_red ? red_off : skel_off;
-    const int b = find_box(oof, nbox, t);
-    const int o = t - oof[b];
+    double* u_out, long long side_n) {
+    __shared__ double xs[4][64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int b = blockIdx.x * 4 + wv;
+    if (b >= nbox) return;
     const int r = dr[b], s = ds[b];
     if (r == 0 || (op != 4 && s == 0)) return;
-    if (op == 0 && skel_pass && pass >= 0 && (int)skel_pass[skel_off[b] + o] != pass) return;
-    const T* M = Mb + off[b];
+    const bool in_red = (op == 0 || op == 3 || op == 4);
+    const bool out_red = (op == 1 || op == 2 || op == 4);
+    const int nin = in_red ? r : s, nout = out_red ? r : s;
     const int* rg = red_gid + red_off[b];
     const int* sg = skel_gid + skel_off[b];
+    if (lane < nin) xs[wv][lane] = u_in[(in_red ? rg : sg)[lane]];
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (lane >= nout) return;
+    if (op == 0 && skel_pass && pass >= 0 && (int)skel_pass[skel_off[b] + lane] != pass) return;
+    const T* M = Mb + off[b];
+    const double* x = xs[wv];
     double acc = 0.0;
-    switch (op) {
-    case 0:  // M r x s, column o, inputs red
-        for (int i = 0; i < r; ++i) acc += (double)M[i + (size_t)o * r] * u_in[rg[i]];
-        break;
-    case 1:  // M r x s, row o, inputs skel
-        for (int j = 0; j < s; ++j) acc += (double)M[o + (size_t)j * r] * u_in[sg[j]];
-        break;
-    case 2:  // M s x r, column o, inputs skel
-        for (int i = 0; i < s; ++i) acc += (double)M[i + (size_t)o * s] * u_in[sg[i]];
-        break;
-    case 3:  // M s x r, row o, inputs red
-        for (int j = 0; j < r; ++j) acc += (double)M[o + (size_t)j * s] * u_in[rg[j]];
-        break;
-    default:  // 4: M r x r, row o
-        for (int j = 0; j < r; ++j) acc += (double)M[o + (size_t)j * r] * u_in[rg[j]];
-        break;
+    if (op == 0 || op == 2) {  // column lane of M (nin x nout)
+        const T* Mj = M + (size_t)lane * nin;
+        for (int i = 0; i < nin; ++i) acc += (double)Mj[i] * x[i];
+    } else {  // row lane of M (nout x nin)
+        for (int j = 0; j < nin; ++j) acc += (double)M[lane + (size_t)j * nout] * x[j];
     }
-    const int g = out_red ? rg[o] : sg[o];
+    const int g = out_red ? rg[lane] : sg[lane];
     if (op == 4) u_out[g] = acc;
+    else if (side_n > 0) u_out[(long long)skel_pass[skel_off[b] + lane] * side_n + g] = alpha * acc;
     else u_out[g] += alpha * acc;
+}
+
+// u[g] += side[p][g] for the skeleton entries with pass flag p: the two
+// contributions a shared face DOF receives, added in pass order.
+__global__ void k_side_add(int n, const int* __restrict__ gid, const unsigned char* __restrict__ sp,
+                           const double* __restrict__ side, long long side_n, int p, double* u) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n || (int)sp[t] != p) return;
+    const int g = gid[t];
+    u[g] += side[(long long)p * side_n + g];
 }
 
 template <class T>
@@ -275,13 +312,13 @@ void apply_small(int nbox, int nout_total, const int* d_r, const int* d_s, const
                  const int* d_skel_off, const int* d_red_gid, const int* d_skel_gid,
                  const unsigned char* d_skel_pass, const long long* d_off, const T* d_M, int op,
                  int pass, double alpha, const double* d_in, double* d_out, double flops,
-                 double bytes, hipStream_t s) {
+                 double bytes, hipStream_t s, long long side_n) {
     if (nbox <= 0 || nout_total <= 0) return;
     KernelTimer* kt = active_timer();
     if (kt) kt->begin(s);
-    k_apply_small<T><<<(nout_total + 255) / 256, 256, 0, s>>>(
-        nbox, nout_total, d_r, d_s, d_red_off, d_skel_off, d_red_gid, d_skel_gid, d_skel_pass,
-        d_off, d_M, op, pass, alpha, d_in, d_out);
+    k_apply_small<T><<<(nbox + 3) / 4, 256, 0, s>>>(nbox, d_r, d_s, d_red_off, d_skel_off,
+                                                    d_red_gid, d_skel_gid, d_skel_pass, d_off,
+                                                    d_M, op, pass, alpha, d_in, d_out, side_n);
     HIFDE_LAUNCH_CHECK();
     if (kt)
         kt->end(s, sizeof(T) == sizeof(float) ? "k_apply_small<float>" : "k_apply_small<double>",
@@ -293,7 +330,8 @@ void apply_tiles(int nbox, int ntiles, const int* d_tile_off, const int* d_r, co
                  const int* d_red_off, const int* d_skel_off, const int* d_red_gid,
                  const int* d_skel_gid, const unsigned char* d_skel_pass, const long long* d_off,
                  const T* d_M, int op, int pass, double alpha, const double* d_in,
-                 double* d_out, int max_in, double flops, double bytes, hipStream_t s) {
+                 double* d_out, int max_in, double flops, double bytes, hipStream_t s,
+                 long long side_n) {
     if (nbox <= 0 || ntiles <= 0) return;
     size_t shm = (size_t)std::max(max_in, 1) * sizeof(double);
     if (shm > 150 * 1024) throw Error(-3, "apply: box too large for LDS");
@@ -301,11 +339,20 @@ void apply_tiles(int nbox, int ntiles, const int* d_tile_off, const int* d_r, co
     if (kt) kt->begin(s);
     k_apply_tiles<T><<<ntiles, 256, shm, s>>>(nbox, d_tile_off, d_r, d_s, d_red_off, d_skel_off,
                                               d_red_gid, d_skel_gid, d_skel_pass, d_off, d_M, op,
-                                              pass, alpha, d_in, d_out);
+                                              pass, alpha, d_in, d_out, side_n);
     HIFDE_LAUNCH_CHECK();
     if (kt)
         kt->end(s, sizeof(T) == sizeof(float) ? "k_apply_tiles<float>" : "k_apply_tiles<double>",
                 flops, bytes);
+}
+
+void side_add(int n, const int* d_gid, const unsigned char* d_sp, const double* d_side,
+              long long side_n, double* d_u, hipStream_t s) {
+    if (n <= 0) return;
+    for (int p = 0; p < 2; ++p) {
+        k_side_add<<<(n + 255) / 256, 256, 0, s>>>(n, d_gid, d_sp, d_side, side_n, p, d_u);
+        HIFDE_LAUNCH_CHECK();
+    }
 }
 
 void zero_boundary(double* d_u, int NC, hipStream_t s) {
@@ -350,18 +397,19 @@ void vcopy(const double* x, double* y, long long n, hipStream_t s) {
 template void apply_small<double>(int, int, const int*, const int*, const int*, const int*,
                                   const int*, const int*, const unsigned char*, const long long*,
                                   const double*, int, int, double, const double*, double*, double,
-                                  double, hipStream_t);
+                                  double, hipStream_t, long long);
 template void apply_small<float>(int, int, const int*, const int*, const int*, const int*,
                                  const int*, const int*, const unsigned char*, const long long*,
                                  const float*, int, int, double, const double*, double*, double,
-                                 double, hipStream_t);
+                                 double, hipStream_t, long long);
 template void apply_tiles<double>(int, int, const int*, const int*, const int*, const int*,
                                   const int*, const int*, const int*, const unsigned char*,
                                   const long long*, const double*, int, int, double,
-                                  const double*, double*, int, double, double, hipStream_t);
+                                  const double*, double*, int, double, double, hipStream_t,
+                                  long long);
 template void apply_tiles<float>(int, int, const int*, const int*, const int*, const int*,
                                  const int*, const int*, const int*, const unsigned char*,
                                  const long long*, const float*, int, int, double, const double*,
-                                 double*, int, double, double, hipStream_t);
+                                 double*, int, double, double, hipStream_t, long long);
 
 }  // namespace hifde

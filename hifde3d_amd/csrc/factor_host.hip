@@ -145,7 +145,7 @@ struct hifde_factor {
     DevBuf<double> ws_G;
     int gram_min_n = 16;          // faces with more columns use the Gram path
     double gram_min_eps = 1e-7;   // below this tolerance every face uses Householder
-    DevBuf<double> pcg_buf, pcg_part, pcg_scalar, apply_tmp;
+    DevBuf<double> pcg_buf, pcg_part, pcg_scalar, apply_tmp, apply_side;
     double stage_t[ST_COUNT] = {0};
     std::vector<double> stage_lt;  // per level: [level * ST_COUNT + stage]
     int cur_level = 0;
@@ -1248,9 +1248,20 @@ struct hifde_factor {
 
     // ------------------------------------------------------------------
     void upd(BoxBatch& B, const DevBuf<unsigned char>& Mbuf, int op, int pass, double alpha,
-             double* u) {
-        if (B.esize == sizeof(float)) upd_t<float>(B, Mbuf, op, pass, alpha, u);
-        else upd_t<double>(B, Mbuf, op, pass, alpha, u);
+             double* u, double* side = nullptr) {
+        if (B.esize == sizeof(float)) upd_t<float>(B, Mbuf, op, pass, alpha, u, side);
+        else upd_t<double>(B, Mbuf, op, pass, alpha, u, side);
+    }
+    // op 0 of a cell batch, whose skeleton DOFs on shared faces get two
+    // contributions: every output is formed once into the side buffers, then
+    // the contributions are added in pass order (max faces, then min faces).
+    void upd_sides(BoxBatch& B, const DevBuf<unsigned char>& Mbuf, double alpha, double* u) {
+        if (B.nbox == 0) return;
+        const long long n3 = (long long)NC * NC * NC;
+        apply_side.reserve(2 * n3);
+        upd(B, Mbuf, 0, -1, alpha, u, apply_side.p);
+        side_add(B.skel_off.back(), B.d_skel_gid.p, B.d_skel_pass.p, apply_side.p, n3, u,
+                 stream);
     }
     void diag(BoxBatch& B, const DevBuf<unsigned char>& Mbuf, const double* u, double* out) {
         if (B.esize == sizeof(float)) diag_t<float>(B, Mbuf, u, out);
@@ -1258,7 +1269,11 @@ struct hifde_factor {
     }
     template <class T>
     void upd_t(BoxBatch& B, const DevBuf<unsigned char>& Mbuf, int op, int pass, double alpha,
-               double* u) {
+               double* u, double* side) {
+        // with side, u_in is u and the outputs go to the side buffers
+        double* uo = side ? side : u;
+        const long long side_n = side ? (long long)NC * NC * NC : 0;
+        const unsigned char* spp = (pass >= 0 || side) ? B.d_skel_pass.p : nullptr;
         if (B.nbox == 0) return;
         const bool out_red = (op == 1 || op == 2);
         const bool in_red = (op == 0 || op == 3);
@@ -1270,16 +1285,15 @@ struct hifde_factor {
         if (B.max_r <= 64 && B.max_s <= 64) {
             const int nout = out_red ? B.red_off.back() : B.skel_off.back();
             apply_small<T>(B.nbox, nout, B.d_r.p, B.d_s.p, B.d_red_off.p, B.d_skel_off.p,
-                           B.d_red_gid.p, B.d_skel_gid.p, pass >= 0 ? B.d_skel_pass.p : nullptr,
-                           B.d_off_rs.p, reinterpret_cast<const T*>(Mbuf.p), op, pass, alpha, u,
-                           u, 2.0 * B.tot_rs, by, stream);
+                           B.d_red_gid.p, B.d_skel_gid.p, spp, B.d_off_rs.p,
+                           reinterpret_cast<const T*>(Mbuf.p), op, pass, alpha, u, uo,
+                           2.0 * B.tot_rs, by, stream, side_n);
             return;
         }
         apply_tiles<T>(B.nbox, tl.back(), out_red ? B.d_tiles_r.p : B.d_tiles_s.p, B.d_r.p,
-                       B.d_s.p, B.d_red_off.p, B.d_skel_off.p, B.d_red_gid.p, B.d_skel_gid.p,
-                       pass >= 0 ? B.d_skel_pass.p : nullptr, B.d_off_rs.p,
-                       reinterpret_cast<const T*>(Mbuf.p), op, pass, alpha, u, u,
-                       in_red ? B.max_r : B.max_s, 2.0 * B.tot_rs, by, stream);
+                       B.d_s.p, B.d_red_off.p, B.d_skel_off.p, B.d_red_gid.p, B.d_skel_gid.p, spp,
+                       B.d_off_rs.p, reinterpret_cast<const T*>(Mbuf.p), op, pass, alpha, u, uo,
+                       in_red ? B.max_r : B.max_s, 2.0 * B.tot_rs, by, stream, side_n);
     }
     template <class T>
     void diag_t(BoxBatch& B, const DevBuf<unsigned char>& Mbuf, const double* u, double* out) {
@@ -1322,7 +1336,7 @@ struct hifde_factor {
         for (int lv = 0; lv < L - 1; ++lv) {
             BoxBatch& C = cells[lv];
             if (inv) {
-                for (int pass = 0; pass < 2; ++pass) upd(C, C.X, 0, pass, -1.0, u);
+                upd_sides(C, C.X, -1.0, u);
             } else {
                 upd(C, C.X, 1, -1, 1.0, u);
             }
@@ -1360,7 +1374,7 @@ struct hifde_factor {
             if (inv) {
                 upd(C, C.X, 1, -1, -1.0, u);
             } else {
-                for (int pass = 0; pass < 2; ++pass) upd(C, C.X, 0, pass, 1.0, u);
+                upd_sides(C, C.X, 1.0, u);
             }
         }
         (void)sg;

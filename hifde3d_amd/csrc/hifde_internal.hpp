@@ -372,14 +372,20 @@ void apply_tiles(int nbox, int ntiles, const int* d_tile_off, const int* d_r, co
                  const int* d_red_off, const int* d_skel_off, const int* d_red_gid,
                  const int* d_skel_gid, const unsigned char* d_skel_pass, const long long* d_off,
                  const T* d_M, int op, int pass, double alpha, const double* d_in,
-                 double* d_out, int max_in, double flops, double bytes, hipStream_t s);
+                 double* d_out, int max_in, double flops, double bytes, hipStream_t s,
+                 long long side_n = 0);
 // Batches whose boxes have at most 64 inputs and outputs: one thread per output.
 template <class T>
 void apply_small(int nbox, int nout_total, const int* d_r, const int* d_s, const int* d_red_off,
                  const int* d_skel_off, const int* d_red_gid, const int* d_skel_gid,
                  const unsigned char* d_skel_pass, const long long* d_off, const T* d_M, int op,
                  int pass, double alpha, const double* d_in, double* d_out, double flops,
-                 double bytes, hipStream_t s);
+                 double bytes, hipStream_t s, long long side_n = 0);
+// side_n > 0 (op 0): each output goes to d_out[pass flag * side_n + DOF]
+// instead of being added to the DOF; side_add then adds the pass-0 and then
+// the pass-1 contributions of a batch's skeleton DOFs to u.
+void side_add(int n, const int* d_gid, const unsigned char* d_sp, const double* d_side,
+              long long side_n, double* d_u, hipStream_t s);
 void zero_boundary(double* d_u, int NC, hipStream_t s);
 
 // vectors / CSR SpMV for PCG
