@@ -111,6 +111,64 @@ constexpr int kSmallGJ = 64;
 constexpr int kGJB = 64;
 constexpr int kGJPanel = 256;
 
+// Wave-wide arg-max of non-negative values, ties to the lower lane (values
+// equal in all but their 6 lowest mantissa bits count as tied).  The key is
+// the value's bit pattern with those 6 bits replaced by 63 - lane; for
+// non-negative floating-point values the bit pattern orders like the value,
+// so the largest key is found with floating-point max.  Four DPP row shifts
+// reduce each row of 16 lanes (lanes shifted in from outside the row read 0),
+// and the four row maxima are read back with readlane.  Returns the winning
+// lane (the same in every lane); nonzero is false when every value is zero.
+template <int N>
+__device__ __forceinline__ double row_shr(double x) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), 0x110 + N, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), 0x110 + N, 0xF, 0xF, true);
+    return __hiloint2double(hi, lo);
+}
+template <int N>
+__device__ __forceinline__ float row_shr(float x) {
+    return __int_as_float(
+        __builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x110 + N, 0xF, 0xF, true));
+}
+__device__ __forceinline__ double read_lane(double x, int l) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(x), l),
+                            __builtin_amdgcn_readlane(__double2loint(x), l));
+}
+__device__ __forceinline__ float read_lane(float x, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+__device__ __forceinline__ double with_lane(double v, int lane) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    return __longlong_as_double((long long)((b & ~0x3Full) | (unsigned long long)(63 - lane)));
+}
+__device__ __forceinline__ float with_lane(float v, int lane) {
+    const unsigned b = (unsigned)__float_as_int(v);
+    return __int_as_float((int)((b & ~0x3Fu) | (unsigned)(63 - lane)));
+}
+__device__ __forceinline__ int key_lane(double k) {
+    return 63 - (int)((unsigned long long)__double_as_longlong(k) & 0x3Full);
+}
+__device__ __forceinline__ int key_lane(float k) {
+    return 63 - (int)((unsigned)__float_as_int(k) & 0x3Fu);
+}
+__device__ __forceinline__ bool key_nonzero(double k) {
+    return ((unsigned long long)__double_as_longlong(k) & ~0x3Full) != 0;
+}
+__device__ __forceinline__ bool key_nonzero(float k) {
+    return ((unsigned)__float_as_int(k) & ~0x3Fu) != 0;
+}
+template <class T>
+__device__ __forceinline__ int wave_argmax_nonneg(T v, int lane, bool& nonzero) {
+    T k = with_lane(v, lane);
+    k = fmax(k, row_shr<1>(k));
+    k = fmax(k, row_shr<2>(k));
+    k = fmax(k, row_shr<4>(k));
+    k = fmax(k, row_shr<8>(k));
+    T m = fmax(fmax(read_lane(k, 15), read_lane(k, 31)), fmax(read_lane(k, 47), read_lane(k, 63)));
+    nonzero = key_nonzero(m);
+    return key_lane(m);
+}
+
 // Pivoted in-place GJ of an n x n block (n <= 64) held in LDS as S[col][row],
 // padded with the identity to 64 x 64 (the inverse of diag(A, I) is
 // diag(A^{-1}, I)).  256 threads: thread t owns column t >> 2 and rows
@@ -123,15 +181,11 @@ __device__ bool lds_gj_inverse64(T (*S)[kSmallGJ + 1], int n, T* colk, T* rowk, 
     const int tid = threadIdx.x;
     const int c = tid >> 2, r0 = tid & 3;
     for (int k = 0; k < n; ++k) {
-        if (tid < 64) {
-            T v = (tid >= k && tid < n) ? fabs(S[k][tid]) : (T)-1;
-            int idx = tid;
-            for (int o = 32; o > 0; o >>= 1) {
-                T ov = __shfl_xor(v, o, 64);
-                int oi = __shfl_xor(idx, o, 64);
-                if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
-            }
-            if (tid == 0) s_piv[0] = (v > (T)0) ? idx : -1;
+        if (tid < 64) {  // wave 0, all lanes
+            bool nz;
+            const int idx =
+                wave_argmax_nonneg<T>((tid >= k && tid < n) ? fabs(S[k][tid]) : (T)0, tid, nz);
+            if (tid == 0) s_piv[0] = nz ? idx : -1;
         }
         __syncthreads();
         const int p = s_piv[0];
@@ -247,15 +301,10 @@ __global__ void __launch_bounds__(64) k_bgj_diag(const int* __restrict__ dims,
     int my_step = 0;
     int bad = 0;
     for (int k = 0; k < kGJB; ++k) {
-        T v = used ? (T)-1 : fabs(reg[0]);
-        int idx = lane;
-        for (int o = kWave / 2; o > 0; o >>= 1) {
-            const T ov = __shfl_xor(v, o, kWave);
-            const int oi = __shfl_xor(idx, o, kWave);
-            if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
-        }
-        const int p = __builtin_amdgcn_readfirstlane(idx);
-        if (__builtin_amdgcn_readfirstlane(v > (T)0 ? 0 : 1)) { bad = k + 1; break; }
+        bool nz;
+        const int p = __builtin_amdgcn_readfirstlane(
+            wave_argmax_nonneg<T>(used ? (T)0 : fabs(reg[0]), lane, nz));
+        if (__builtin_amdgcn_readfirstlane(nz ? 0 : 1)) { bad = k + 1; break; }
         const bool me = lane == p;
         if (me) {
             const T d = (T)1 / reg[0];

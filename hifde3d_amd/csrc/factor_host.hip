@@ -45,7 +45,20 @@ const void* Stager::stage(const void* src, size_t bytes) {
             HIFDE_CHECK(hipHostMalloc(reinterpret_cast<void**>(&buf), cap));
         }
     }
-    std::memcpy(buf + used, src, bytes);
+    if (bytes >= ((size_t)4 << 20)) {
+        // large uploads: the copy into pinned memory is split over the host cores
+        const size_t chunk = (size_t)1 << 20;
+        const long long nch = (long long)((bytes + chunk - 1) / chunk);
+        unsigned char* dst = buf + used;
+        const unsigned char* sp = static_cast<const unsigned char*>(src);
+#pragma omp parallel for schedule(static)
+        for (long long c = 0; c < nch; ++c) {
+            const size_t o = (size_t)c * chunk;
+            std::memcpy(dst + o, sp + o, std::min(chunk, bytes - o));
+        }
+    } else {
+        std::memcpy(buf + used, src, bytes);
+    }
     const void* p = buf + used;
     used += need;
     return p;
