@@ -273,7 +273,8 @@ __global__ void __launch_bounds__(256) k_gj_small(const int* __restrict__ dims,
 // in-place Gauss-Jordan: the pivot column is always register 0; every step
 // shifts each row left by one and appends the new in-place column as register
 // 63, so after 64 steps register t holds in-place column t and no register is
-// indexed at run time.  The pivot of step k is the not yet used row of
+// indexed at run time.  The pivot row is broadcast with readlane, so a step
+// needs no LDS traffic and no barrier.  The pivot of step k is the not yet used row of
 // largest |a_ik| (ties to the lower row), i.e. partial pivoting.  With p_k the
 // pivot row of step k, the in-place array Z satisfies A^{-1}(k, p_t) =
 // Z(p_k, t), which is how each lane writes its row out.
@@ -291,7 +292,6 @@ __global__ void __launch_bounds__(64) k_bgj_diag(const int* __restrict__ dims,
     const int ld = lds[b];
     const T* A = mats + off[b] + k0 + (size_t)k0 * ld;
     const int lane = threadIdx.x;
-    __shared__ T prow[kGJB];
     __shared__ int piv[kGJB];
     T reg[kGJB];
 #pragma unroll
@@ -306,25 +306,23 @@ __global__ void __launch_bounds__(64) k_bgj_diag(const int* __restrict__ dims,
             wave_argmax_nonneg<T>(used ? (T)0 : fabs(reg[0]), lane, nz));
         if (__builtin_amdgcn_readfirstlane(nz ? 0 : 1)) { bad = k + 1; break; }
         const bool me = lane == p;
+        T d = (T)0;
         if (me) {
-            const T d = (T)1 / reg[0];
+            d = (T)1 / reg[0];
 #pragma unroll
-            for (int j = 1; j < kGJB; ++j) {
-                reg[j] *= d;
-                prow[j] = reg[j];
-            }
-            prow[0] = d;
+            for (int j = 1; j < kGJB; ++j) reg[j] *= d;
             piv[k] = lane;
             used = true;
             my_step = k;
         }
-        __syncthreads();
+        // the scaled pivot row is read from lane p register by register
+        const T dd = read_lane(d, p);
         const T f = me ? (T)0 : reg[0];
 #pragma unroll
-        for (int j = 1; j < kGJB; ++j) reg[j - 1] = fma(-f, prow[j], reg[j]);
-        reg[kGJB - 1] = me ? prow[0] : -f * prow[0];
-        __syncthreads();
+        for (int j = 1; j < kGJB; ++j) reg[j - 1] = fma(-f, read_lane(reg[j], p), reg[j]);
+        reg[kGJB - 1] = me ? dd : -f * dd;
     }
+    __syncthreads();
     if (bad) {
         if (lane == 0) info[b] = ibase + k0 + bad;
         return;
