@@ -30,7 +30,7 @@ _dp = ctypes.POINTER(ctypes.c_double)
 EXPORTS = [
     "hifde_create", "hifde_destroy", "hifde_last_error", "hifde_set_matrix_coo",
     "hifde_set_stencil", "hifde_factor_run", "hifde_apply", "hifde_apply_device", "hifde_spmv",
-    "hifde_pcg", "hifde_pcg_device", "hifde_num_levels", "hifde_level_counts",
+    "hifde_pcg", "hifde_pcg_device", "hifde_pcg_multi_device", "hifde_num_levels", "hifde_level_counts",
     "hifde_box_count", "hifde_box_sizes", "hifde_box_data", "hifde_stage_times",
     "hifde_level_stage_times",
     "hifde_stage_name", "hifde_factor_bytes", "hifde_interp_decomp_batch", "hifde_schur_batch",
@@ -52,6 +52,14 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} not built; run `python -m hifde3d_amd.build`")
+    # One HIP runtime per process: PyTorch ships its own libamdhip64.so.7
+    # under the same soname, whichever is loaded first serves both, and
+    # PyTorch finds no GPU on the system copy.  Device buffers are exchanged
+    # with PyTorch, so its runtime is loaded first when it is installed.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     L.hifde_create.argtypes = [_i, _i, _d, _i, _i, ctypes.POINTER(_vp)]
     L.hifde_destroy.argtypes = [_vp]
@@ -65,6 +73,7 @@ def lib():
     L.hifde_spmv.argtypes = [_vp, _dp, _dp, _i]
     L.hifde_pcg.argtypes = [_vp, _dp, _dp, _d, _i, _ip, _dp]
     L.hifde_pcg_device.argtypes = [_vp, _vp, _vp, _d, _i, _ip, _dp]
+    L.hifde_pcg_multi_device.argtypes = [_vp, _vp, _ll, _vp, _ll, _i, _d, _i, _ip, _dp]
     L.hifde_num_levels.argtypes = [_vp]
     L.hifde_level_counts.argtypes = [_vp, _i, _ip]
     L.hifde_box_count.argtypes = [_vp, _i, _i]
@@ -192,6 +201,17 @@ class HIFFactor:
         _check(lib().hifde_pcg_device(self._h, ctypes.c_void_p(b_ptr), ctypes.c_void_p(x_ptr),
                                       tol, maxit, ctypes.byref(it), ctypes.byref(rr)))
         return it.value, rr.value
+
+    def pcg_multi_device(self, b_ptr, x_ptr, nrhs, ldb=None, ldx=None, tol=1e-10, maxit=200):
+        """nrhs independent PCG solves (columns of B and X in device memory,
+        column-major).  Returns per-column (iterations, relative residuals)."""
+        it = np.zeros(max(nrhs, 1), np.int32)
+        rr = np.zeros(max(nrhs, 1))
+        _check(lib().hifde_pcg_multi_device(
+            self._h, ctypes.c_void_p(b_ptr), ldb if ldb is not None else self.size,
+            ctypes.c_void_p(x_ptr), ldx if ldx is not None else self.size, nrhs, tol, maxit,
+            it.ctypes.data_as(_ip), rr.ctypes.data_as(_dp)))
+        return it[:nrhs].copy(), rr[:nrhs].copy()
 
     # --- introspection ------------------------------------------------------
     def level_counts(self):

@@ -1222,6 +1222,9 @@ struct hifde_factor {
                 for (int i = 1; i < NC; ++i) remaining[lin(i, j, k)] = 1;
         cells.resize(L);
         faces.resize(L);
+        for (std::vector<BoxBatch>* v : {&cells, &faces})
+            for (BoxBatch& B : *v)
+                for (int& x : B.rplan_R) x = 0;
         factored = false;
         sym_on = op_symmetric && sym_option != 0;
         elem_on = local_ok && elem_option != 0;
@@ -1396,11 +1399,168 @@ struct hifde_factor {
     void apply_dev(double* d_u, int nrhs, long long ldu, bool inv) {
         TimerGuard tguard(this);
         if (!factored) throw Error(HIFDE_ERR_STATE, "apply: factor first");
-        apply_tmp.reserve((size_t)NC * NC * NC);
-        for (int c = 0; c < nrhs; ++c) {
-            apply_one(d_u + c * ldu, inv);
+        if (nrhs > 1) {
+            apply_multi(d_u, nrhs, ldu, inv);
+        } else if (nrhs == 1) {
+            apply_tmp.reserve((size_t)NC * NC * NC);
+            apply_one(d_u, inv);
         }
         resolve_profile();
+    }
+
+    // ------------------------------------------------------------------
+    // Several right-hand sides (columns of U, leading dimension ldu).  Every
+    // box step of apply_one becomes gather -> batched GEMM -> scatter over
+    // packed per-box blocks (multi_rhs.hip), so each factor block is read
+    // once for all R vectors.  The GEMM descriptors of a batch are built once
+    // per factorization and R.
+    DevBuf<unsigned char> rhs_in, rhs_out;
+    DevBuf<double> rhs_side;
+    // batches whose boxes have at most this many inputs and outputs take the
+    // fused kernel; larger ones the packed GEMM path
+    int multi_fused_max = 512;
+
+    static const int* ogid_of(const BoxBatch& B, bool red) {
+        return red ? B.d_red_gid.p : B.d_skel_gid.p;
+    }
+    void upd_multi(BoxBatch& B, const DevBuf<unsigned char>& Mbuf, int op, double alpha,
+                   double* U, long long ldu, int R, bool side = false) {
+        if (B.esize == sizeof(float)) upd_multi_t<float>(B, Mbuf, op, alpha, U, ldu, R, side);
+        else upd_multi_t<double>(B, Mbuf, op, alpha, U, ldu, R, side);
+    }
+    // op as in apply_one: 0 X^T u_red -> skel, 1 X u_skel -> red, 2 W^T u_skel
+    // -> red, 3 W u_red -> skel, 4 M u_red -> red (written, not added).  With
+    // side, the outputs (op 0 of a cell batch) are parked by pass flag and
+    // added in pass order.
+    template <class T>
+    void upd_multi_t(BoxBatch& B, const DevBuf<unsigned char>& Mbuf, int op, double alpha,
+                     double* U, long long ldu, int R, bool side) {
+        if (B.nbox == 0) return;
+        const bool in_red = (op == 0 || op == 3 || op == 4);
+        const bool out_red = (op == 1 || op == 2 || op == 4);
+        const std::vector<int>& ioff = in_red ? B.red_off : B.skel_off;
+        const std::vector<int>& ooff = out_red ? B.red_off : B.skel_off;
+        const long long nin = ioff.back(), nout = ooff.back();
+        if (nout == 0) return;
+        const long long n3 = (long long)NC * NC * NC;
+        const int max_in = in_red ? B.max_r : B.max_s, max_out = out_red ? B.max_r : B.max_s;
+        const bool fused = op == 4 ? B.max_r <= 64
+                                   : (max_in <= multi_fused_max && max_out <= multi_fused_max);
+        if (fused) {
+            const double fl = 2.0 * (op == 4 ? B.tot_rr : B.tot_rs) * R;
+            const double by = (double)(op == 4 ? B.tot_rr : B.tot_rs) * sizeof(T) +
+                              (double)(nin + 2 * nout) * R * 8.0;
+            Prof pr(this, fam<T>("apply_multi"), fl, by);
+            if (side) rhs_side.reserve((size_t)2 * R * n3);
+            const std::vector<int>& tl = out_red ? B.tiles_r : B.tiles_s;
+            apply_multi_fused<T>(B.nbox, tl.back(), out_red ? B.d_tiles_r.p : B.d_tiles_s.p,
+                                 B.d_r.p, B.d_s.p, B.d_red_off.p, B.d_skel_off.p, B.d_red_gid.p,
+                                 B.d_skel_gid.p, B.d_skel_pass.p,
+                                 op == 4 ? B.d_off_rr.p : B.d_off_rs.p,
+                                 reinterpret_cast<const T*>(Mbuf.p), op, alpha, U, ldu, R,
+                                 side ? rhs_side.p : nullptr, side ? n3 : 0, fl, by, stream);
+            if (side)
+                rhs_side_add(nout, ogid_of(B, out_red), B.d_skel_pass.p, rhs_side.p, n3, R, U,
+                             ldu, stream);
+            return;
+        }
+        GemmPlan& pl = B.rplan[op];
+        if (B.rplan_R[op] != R) {
+            pl.clear();
+            std::vector<GemmDesc> d;
+            d.reserve(B.nbox);
+            B.rplan_zero[op] = 0;
+            for (int b = 0; b < B.nbox; ++b) {
+                const int r = B.r[b], s = B.s[b];
+                const int m = out_red ? r : s, k = in_red ? r : s;
+                if (m == 0) continue;
+                if (k == 0) {  // no inputs: the box adds nothing
+                    B.rplan_zero[op] = 1;
+                    continue;
+                }
+                GemmDesc g;
+                g.offA = op == 4 ? B.off_rr[b] : B.off_rs[b];
+                g.offB = (long long)ioff[b] * R;
+                g.offC = (long long)ooff[b] * R;
+                g.m = m;
+                g.n = R;
+                g.k = k;
+                g.lda = (op == 0 || op == 1 || op == 4) ? r : s;
+                g.ldb = k;
+                g.ldc = m;
+                d.push_back(g);
+            }
+            pl.add(d, op == 0 || op == 2);
+            pl.upload(stream);
+            B.rplan_R[op] = R;
+        }
+        const double fl = 2.0 * (op == 4 ? B.tot_rr : B.tot_rs) * R;
+        const double by = (double)(op == 4 ? B.tot_rr : B.tot_rs) * sizeof(T) +
+                          (double)(nin + nout) * R * (sizeof(T) + 8.0) * 2;
+        Prof pr(this, fam<T>("apply_multi"), fl, by);
+        rhs_in.reserve((size_t)std::max(nin, 1LL) * R * sizeof(T));
+        rhs_out.reserve((size_t)nout * R * sizeof(T));
+        T* pin = reinterpret_cast<T*>(rhs_in.p);
+        T* pout = reinterpret_cast<T*>(rhs_out.p);
+        const int* d_ioff = in_red ? B.d_red_off.p : B.d_skel_off.p;
+        const int* d_ooff = out_red ? B.d_red_off.p : B.d_skel_off.p;
+        const int* igid = in_red ? B.d_red_gid.p : B.d_skel_gid.p;
+        const int* ogid = out_red ? B.d_red_gid.p : B.d_skel_gid.p;
+        rhs_gather<T>(B.nbox, nin, d_ioff, igid, U, ldu, R, pin, stream);
+        if (B.rplan_zero[op]) fill_zero<T>(pout, nout * R, stream);
+        if (!pl.launches.empty())
+            gemm_run<T, T>(pl, {0, (int)pl.launches.size()}, reinterpret_cast<const T*>(Mbuf.p),
+                           pin, pout, (T)1, (T)0, stream);
+        if (side) {
+            rhs_side.reserve((size_t)2 * R * n3);
+            rhs_scatter<T>(B.nbox, nout, d_ooff, ogid, pout, R, rhs_side.p, 0, alpha, 2,
+                           B.d_skel_pass.p, n3, stream);
+            rhs_side_add(nout, ogid, B.d_skel_pass.p, rhs_side.p, n3, R, U, ldu, stream);
+        } else {
+            rhs_scatter<T>(B.nbox, nout, d_ooff, ogid, pout, R, U, ldu, alpha, op == 4 ? 1 : 0,
+                           nullptr, 0, stream);
+        }
+    }
+
+    // apply_one for R columns.  The diagonal stage works in place: each box
+    // reads and writes only its own redundant DOFs, every interior DOF is
+    // redundant in exactly one box, and a box's inputs are gathered before
+    // its outputs are written.
+    void apply_multi(double* U, int R, long long ldu, bool inv) {
+        for (int c = 0; c < R; ++c) zero_boundary(U + c * ldu, NC, stream);
+        for (int lv = 0; lv < L - 1; ++lv) {
+            BoxBatch& C = cells[lv];
+            if (inv) upd_multi(C, C.X, 0, -1.0, U, ldu, R, true);
+            else upd_multi(C, C.X, 1, 1.0, U, ldu, R);
+            BoxBatch& F = faces[lv];
+            if (F.nbox == 0) continue;
+            if (inv) {
+                upd_multi(F, F.W, 2, -1.0, U, ldu, R);
+                upd_multi(F, F.X, 0, -1.0, U, ldu, R);
+            } else {
+                upd_multi(F, F.W, 3, 1.0, U, ldu, R);
+                upd_multi(F, F.X, 1, 1.0, U, ldu, R);
+            }
+        }
+        for (int lv = 0; lv < L; ++lv) {
+            upd_multi(cells[lv], inv ? cells[lv].A22inv : cells[lv].A22, 4, 1.0, U, ldu, R);
+            upd_multi(faces[lv], inv ? faces[lv].A22inv : faces[lv].A22, 4, 1.0, U, ldu, R);
+        }
+        for (int lv = L - 2; lv >= 0; --lv) {
+            BoxBatch& F = faces[lv];
+            if (F.nbox) {
+                if (inv) {
+                    upd_multi(F, F.X, 1, -1.0, U, ldu, R);
+                    upd_multi(F, F.W, 3, -1.0, U, ldu, R);
+                } else {
+                    upd_multi(F, F.X, 0, 1.0, U, ldu, R);
+                    upd_multi(F, F.W, 2, 1.0, U, ldu, R);
+                }
+            }
+            BoxBatch& C = cells[lv];
+            if (inv) upd_multi(C, C.X, 1, -1.0, U, ldu, R);
+            else upd_multi(C, C.X, 0, 1.0, U, ldu, R, true);
+        }
     }
 
     // ------------------------------------------------------------------
@@ -1465,6 +1625,131 @@ struct hifde_factor {
         resolve_profile();
         if (iters) *iters = it;
         if (relres) *relres = rnorm / bnorm;
+    }
+
+    // R independent PCG solves advanced together (one SpMV over R columns,
+    // one preconditioner apply over R columns, R dots per reduction).  Each
+    // column follows pcg_dev's recurrence; a converged column is frozen (its
+    // step lengths are zero) while the others continue.
+    DevBuf<double> mpcg_buf, mpcg_part, mpcg_scal;
+    // F^{-1} on R columns (the single-vector sweeps for one column)
+    void apply_cols(double* Z, int R, long long ld) {
+        if (R == 1) {
+            apply_tmp.reserve((size_t)NC * NC * NC);
+            apply_one(Z, true);
+        } else {
+            apply_multi(Z, R, ld, true);
+        }
+    }
+    void pcg_multi_dev(const double* B, long long ldb, double* X, long long ldx, int R,
+                       double tol, int maxit, int* iters, double* relres) {
+        TimerGuard tguard(this);
+        if (!factored) throw Error(HIFDE_ERR_STATE, "pcg: factor first");
+        const long long n = (long long)NC * NC * NC;
+        mpcg_buf.reserve((size_t)4 * n * R);
+        double* Rm = mpcg_buf.p;
+        double* Z = Rm + n * R;
+        double* Pm = Z + n * R;
+        double* Q = Pm + n * R;
+        const int nblk = std::max(1, std::min(256, 4096 / R));
+        mpcg_part.reserve((size_t)nblk * R);
+        mpcg_scal.reserve((size_t)4 * R);
+        double* d_dot = mpcg_scal.p;       // R dots
+        double* d_a = d_dot + R;           // R step lengths
+        double* d_ma = d_a + R;            // their negatives
+        double* d_b = d_ma + R;            // R betas
+        std::vector<double> h(4 * R);
+        auto dots = [&](const double* x, const double* y) {
+            dot_cols(n, x, n, y, n, R, mpcg_part.p, nblk, d_dot, stream);
+            HIFDE_CHECK(hipMemcpyAsync(h.data(), d_dot, R * sizeof(double),
+                                       hipMemcpyDeviceToHost, stream));
+            HIFDE_CHECK(hipStreamSynchronize(stream));
+            return std::vector<double>(h.begin(), h.begin() + R);
+        };
+        auto upload_scalars = [&](const std::vector<double>& a, const std::vector<double>& b) {
+            for (int c = 0; c < R; ++c) {
+                h[R + c] = a[c];
+                h[2 * R + c] = -a[c];
+                h[3 * R + c] = b[c];
+            }
+            HIFDE_CHECK(hipMemcpyAsync(d_a, h.data() + R, 3 * R * sizeof(double),
+                                       hipMemcpyHostToDevice, stream));
+            HIFDE_CHECK(hipStreamSynchronize(stream));
+        };
+        // r = b - A x, boundary rows cleared
+        {
+            Prof pr(this, "spmv", 2.0 * h_ci.size() * R, (12.0 * h_ci.size() + 16.0 * n) * R);
+            csr_spmm((int)n, d_rp.p, d_ci.p, d_v.p, X, ldx, Q, n, R, stream);
+        }
+        for (int c = 0; c < R; ++c) vcopy(B + c * ldb, Rm + c * n, n, stream);
+        std::vector<double> ones(R, -1.0), zeros(R, 0.0);
+        upload_scalars(ones, zeros);
+        axpy_cols(n, d_a, Q, n, Rm, n, R, false, stream);
+        for (int c = 0; c < R; ++c) zero_boundary(Rm + c * n, NC, stream);
+        std::vector<double> bnorm(R), rnorm(R);
+        {
+            // ||b|| per column (b has its own leading dimension)
+            for (int c = 0; c < R; ++c) vcopy(B + c * ldb, Z + c * n, n, stream);
+            std::vector<double> bb = dots(Z, Z);
+            for (int c = 0; c < R; ++c) {
+                bnorm[c] = std::sqrt(bb[c]);
+                if (bnorm[c] == 0.0) bnorm[c] = 1.0;
+            }
+        }
+        std::vector<double> rr = dots(Rm, Rm);
+        std::vector<int> it(R, 0);
+        std::vector<char> done(R, 0);
+        int active = 0;
+        for (int c = 0; c < R; ++c) {
+            rnorm[c] = std::sqrt(rr[c]);
+            done[c] = rnorm[c] / bnorm[c] <= tol;
+            active += !done[c];
+        }
+        if (active > 0) {
+            vcopy(Rm, Z, n * R, stream);
+            apply_cols(Z, R, n);
+            vcopy(Z, Pm, n * R, stream);
+            std::vector<double> rz = dots(Rm, Z);
+            std::vector<double> alpha(R), beta(R);
+            for (int k = 1; k <= maxit && active > 0; ++k) {
+                {
+                    Prof pr(this, "spmv", 2.0 * h_ci.size() * R,
+                            (12.0 * h_ci.size() + 16.0 * n) * R);
+                    csr_spmm((int)n, d_rp.p, d_ci.p, d_v.p, Pm, n, Q, n, R, stream);
+                }
+                std::vector<double> pq = dots(Pm, Q);
+                for (int c = 0; c < R; ++c) alpha[c] = done[c] ? 0.0 : rz[c] / pq[c];
+                upload_scalars(alpha, zeros);
+                axpy_cols(n, d_a, Pm, n, X, ldx, R, false, stream);
+                axpy_cols(n, d_ma, Q, n, Rm, n, R, false, stream);
+                rr = dots(Rm, Rm);
+                for (int c = 0; c < R; ++c) {
+                    if (done[c]) continue;
+                    it[c] = k;
+                    rnorm[c] = std::sqrt(rr[c]);
+                    if (rnorm[c] / bnorm[c] <= tol) {
+                        done[c] = 1;
+                        --active;
+                    }
+                }
+                if (active == 0) break;
+                vcopy(Rm, Z, n * R, stream);
+                apply_cols(Z, R, n);
+                std::vector<double> rz_new = dots(Rm, Z);
+                for (int c = 0; c < R; ++c) {
+                    beta[c] = done[c] ? 0.0 : rz_new[c] / rz[c];
+                    if (!done[c]) rz[c] = rz_new[c];
+                }
+                upload_scalars(zeros, beta);
+                axpy_cols(n, d_b, Z, n, Pm, n, R, true, stream);
+            }
+        }
+        HIFDE_CHECK(hipStreamSynchronize(stream));
+        resolve_profile();
+        for (int c = 0; c < R; ++c) {
+            if (iters) iters[c] = it[c];
+            if (relres) relres[c] = rnorm[c] / bnorm[c];
+        }
     }
 
     // ------------------------------------------------------------------
@@ -1767,6 +2052,21 @@ int hifde_pcg_device(hifde_factor* f, const double* d_b, double* d_x, double tol
     });
 }
 
+int hifde_pcg_multi_device(hifde_factor* f, const double* d_b, long long ldb, double* d_x,
+                           long long ldx, int nrhs, double tol, int maxit, int* iters,
+                           double* relres) {
+    return guarded([&]() -> int {
+        if (!f || !d_b || !d_x || maxit < 0 || nrhs < 0)
+            throw Error(HIFDE_ERR_ARG, "pcg_multi: bad arguments");
+        const long long n = (long long)f->NC * f->NC * f->NC;
+        if (nrhs > 1 && (ldb < n || ldx < n))
+            throw Error(HIFDE_ERR_ARG, "pcg_multi: leading dimension too small");
+        if (nrhs == 0) return HIFDE_OK;
+        f->pcg_multi_dev(d_b, ldb, d_x, ldx, nrhs, tol, maxit, iters, relres);
+        return HIFDE_OK;
+    });
+}
+
 int hifde_pcg(hifde_factor* f, const double* b, double* x, double tol, int maxit, int* iters,
               double* relres) {
     return guarded([&]() -> int {
@@ -1861,6 +2161,7 @@ int hifde_set_option(hifde_factor* f, const char* name, double value) {
         else if (n == "symmetric") f->sym_option = (int)value;
         else if (n == "elements") f->elem_option = (int)value;
         else if (n == "stage_sync") f->stage_sync = value != 0;
+        else if (n == "multi_fused_max") f->multi_fused_max = (int)value;
         else throw Error(HIFDE_ERR_ARG, "set_option: unknown option " + n);
         return HIFDE_OK;
     });

@@ -110,6 +110,43 @@ struct LevelTable {
     bool sym = false;  // one entry per unordered pair, keyed (min, max)
 };
 
+struct GemmDesc {
+    long long offA, offB, offC;
+    int m, n, k, lda, ldb, ldc;
+    int lower = 0;  // 1: only output tiles on or below the block diagonal
+    int flag = -1;  // >= 0: skipped when flags[flag] == 0 (device flags)
+};
+struct CopyDesc {
+    long long off;  // source block base
+    int r0, c0, rows, ld;
+};
+// Descriptors of many batched GEMM launches, built on the host, uploaded in
+// one copy each and then launched one by one (gemm_run).
+struct GemmPlan {
+    struct Launch {
+        int mfma, transA, nb, d0, t0, total;
+        double flops, ab_elems, cells;  // algorithmic work of the launch
+    };
+    std::vector<GemmDesc> desc;
+    std::vector<int> tiles;  // per launch: nb + 1 local tile offsets
+    std::vector<Launch> launches;
+    DevBuf<GemmDesc> d_desc;
+    DevBuf<int> d_tiles;
+    void clear() {
+        desc.clear();
+        tiles.clear();
+        launches.clear();
+    }
+    // Appends one batched GEMM (outputs narrower than half an MFMA tile on
+    // the FMA kernel, the rest on the MFMA kernel); returns its launches
+    // [first, second).
+    std::pair<int, int> add(const std::vector<GemmDesc>& d, bool transA);
+    void upload(hipStream_t s) {
+        d_desc.upload(desc, s);
+        d_tiles.upload(tiles, s);
+    }
+};
+
 // One batch of boxes of a half level (cells of level l, or faces of level l).
 // Dense blocks are column-major and packed box after box; offsets are in
 // elements.
@@ -139,8 +176,14 @@ struct BoxBatch {
     std::vector<int> o, other_off, other_gid;
     std::vector<long long> off_ee;
     long long tot_ee = 0;
+    // Multi-RHS apply: per op, the batched GEMM of every box for rplan_R
+    // right-hand sides (0: not built for the current factorization).
+    GemmPlan rplan[5];
+    int rplan_R[5] = {0, 0, 0, 0, 0};
+    int rplan_zero[5] = {0, 0, 0, 0, 0};  // some box has outputs but no inputs
     // Clears the index description; device buffers are kept for reuse.
     void reset_host() {
+        for (int& x : rplan_R) x = 0;
         nbox = 0;
         r.clear(); s.clear(); o.clear();
         off_rr.clear(); off_rs.clear(); off_ss.clear(); off_ee.clear();
@@ -223,42 +266,6 @@ struct Copy2D {
 };
 
 
-struct GemmDesc {
-    long long offA, offB, offC;
-    int m, n, k, lda, ldb, ldc;
-    int lower = 0;  // 1: only output tiles on or below the block diagonal
-    int flag = -1;  // >= 0: skipped when flags[flag] == 0 (device flags)
-};
-struct CopyDesc {
-    long long off;  // source block base
-    int r0, c0, rows, ld;
-};
-// Descriptors of many batched GEMM launches, built on the host, uploaded in
-// one copy each and then launched one by one (gemm_run).
-struct GemmPlan {
-    struct Launch {
-        int mfma, transA, nb, d0, t0, total;
-        double flops, ab_elems, cells;  // algorithmic work of the launch
-    };
-    std::vector<GemmDesc> desc;
-    std::vector<int> tiles;  // per launch: nb + 1 local tile offsets
-    std::vector<Launch> launches;
-    DevBuf<GemmDesc> d_desc;
-    DevBuf<int> d_tiles;
-    void clear() {
-        desc.clear();
-        tiles.clear();
-        launches.clear();
-    }
-    // Appends one batched GEMM (outputs narrower than half an MFMA tile on
-    // the FMA kernel, the rest on the MFMA kernel); returns its launches
-    // [first, second).
-    std::pair<int, int> add(const std::vector<GemmDesc>& d, bool transA);
-    void upload(hipStream_t s) {
-        d_desc.upload(desc, s);
-        d_tiles.upload(tiles, s);
-    }
-};
 template <class T, class TC = T>
 void gemm_run(const GemmPlan& p, std::pair<int, int> h, const T* d_A, const T* d_B, TC* d_C,
               T alpha, T beta, hipStream_t s, const int* d_flags = nullptr);
@@ -387,6 +394,33 @@ void apply_small(int nbox, int nout_total, const int* d_r, const int* d_s, const
 void side_add(int n, const int* d_gid, const unsigned char* d_sp, const double* d_side,
               long long side_n, double* d_u, hipStream_t s);
 void zero_boundary(double* d_u, int NC, hipStream_t s);
+
+// several right-hand sides (multi_rhs.hip); U column-major with leading dim ldu
+template <class T>
+void rhs_gather(int nbox, long long nentries, const int* d_off, const int* d_gid, const double* U,
+                long long ldu, int R, T* pack, hipStream_t s);
+// mode 0: add alpha * pack; 1: write pack; 2: per-pass side buffers
+template <class T>
+void rhs_scatter(int nbox, long long nentries, const int* d_off, const int* d_gid, const T* pack,
+                 int R, double* U, long long ldu, double alpha, int mode, const unsigned char* d_sp,
+                 long long side_n, hipStream_t s);
+// gather -> product -> scatter in one kernel (boxes of moderate size)
+template <class T>
+void apply_multi_fused(int nbox, int ntiles, const int* d_tile_off, const int* d_r, const int* d_s,
+                       const int* d_red_off, const int* d_skel_off, const int* d_red_gid,
+                       const int* d_skel_gid, const unsigned char* d_skel_pass,
+                       const long long* d_off, const T* d_M, int op, double alpha, double* U,
+                       long long ldu, int R, double* side, long long side_n, double flops,
+                       double bytes, hipStream_t s);
+void rhs_side_add(long long nentries, const int* d_gid, const unsigned char* d_sp,
+                  const double* side, long long side_n, int R, double* U, long long ldu,
+                  hipStream_t s);
+void csr_spmm(int n, const int* d_rp, const int* d_ci, const double* d_v, const double* X,
+              long long ldx, double* Y, long long ldy, int R, hipStream_t s);
+void dot_cols(long long n, const double* X, long long ldx, const double* Y, long long ldy, int R,
+              double* d_part, int nblk, double* d_out, hipStream_t s);
+void axpy_cols(long long n, const double* d_a, const double* X, long long ldx, double* Y,
+               long long ldy, int R, bool xpby, hipStream_t s);
 
 // vectors / CSR SpMV for PCG
 void csr_spmv(int n, const int* d_rowptr, const int* d_colind, const double* d_val,

@@ -1,0 +1,338 @@
+// Several right-hand sides at once: the apply sweeps as batched GEMMs on
+// packed per-box blocks, and the vector kernels of the multi-RHS PCG.
+//
+// The single-vector apply (apply_pcg.hip) is a GEMV per box.  With R
+// right-hand sides each box's step is a GEMM: the inputs of every box are
+// gathered into a packed column-major block (nin x R), multiplied by the box's
+// factor block on the matrix cores, and the outputs scattered back (added,
+// written, or parked in the per-pass side buffers of the cell sweep).  The
+// factor blocks are read once for all R vectors.  Vectors are column-major,
+// column c of U at U + c * ldu.
+
+#include "hifde_internal.hpp"
+
+namespace hifde {
+
+namespace {
+
+template <class O>
+__device__ __forceinline__ int find_slot(const O* __restrict__ off, int n, long long e) {
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if ((long long)off[mid] <= e) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+inline unsigned grid1(long long n) {
+    long long g = (n + 255) / 256;
+    if (g < 1) g = 1;
+    if (g > (1LL << 20)) g = 1LL << 20;
+    return (unsigned)g;
+}
+
+// pack[off[b] R + i + c nin_b] = U[gid[off[b] + i] + c ldu]; one thread per
+// entry, all R columns
+template <class T>
+__global__ void k_rhs_gather(int nbox, const int* __restrict__ off, const int* __restrict__ gid,
+                             const double* __restrict__ U, long long ldu, int R, T* pack) {
+    const long long ne = off[nbox];
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < ne;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int b = find_slot(off, nbox, t);
+        const long long nb = off[b + 1] - off[b];
+        T* dst = pack + (long long)off[b] * R + (t - off[b]);
+        const double* src = U + gid[t];
+        for (int c = 0; c < R; ++c) dst[c * nb] = (T)src[c * ldu];
+    }
+}
+
+// mode 0: U[g + c ldu] += alpha * pack;  1: U[...] = pack;
+// 2: side[(sp * R + c) side_n + g] = alpha * pack
+template <class T>
+__global__ void k_rhs_scatter(int nbox, const int* __restrict__ off, const int* __restrict__ gid,
+                              const T* __restrict__ pack, int R, double* U, long long ldu,
+                              double alpha, int mode, const unsigned char* __restrict__ sp,
+                              long long side_n) {
+    const long long ne = off[nbox];
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < ne;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int b = find_slot(off, nbox, t);
+        const long long nb = off[b + 1] - off[b];
+        const T* src = pack + (long long)off[b] * R + (t - off[b]);
+        const int g = gid[t];
+        if (mode == 0) {
+            for (int c = 0; c < R; ++c) U[g + c * ldu] += alpha * (double)src[c * nb];
+        } else if (mode == 1) {
+            for (int c = 0; c < R; ++c) U[g + c * ldu] = (double)src[c * nb];
+        } else {
+            double* dst = U + (long long)sp[t] * R * side_n + g;
+            for (int c = 0; c < R; ++c) dst[c * side_n] = alpha * (double)src[c * nb];
+        }
+    }
+}
+
+// Fused gather -> product -> scatter for boxes of moderate size.  One
+// workgroup forms 64 outputs of one box for CB = 4 CPT right-hand sides
+// (columns c0 = blockIdx.y CB ...): the box's input entries are staged
+// through LDS 64 at a time together with the matching 64 x 64 block of M
+// (read along its contiguous dimension), and every thread accumulates one
+// output for CPT columns in FP64.  op as in k_apply_tiles; op 4 writes U in
+// place, which needs every box to fit one tile (the host checks r <= 64).
+// With side_n > 0 the outputs go to the pass-flag side buffers
+// side[(sp R + c) side_n + dof].
+template <class T, int CPT>
+__global__ void __launch_bounds__(256) k_apply_multi(
+    int nbox, const int* __restrict__ tile_off, const int* __restrict__ dr,
+    const int* __restrict__ ds, const int* __restrict__ red_off,
+    const int* __restrict__ skel_off, const int* __restrict__ red_gid,
+    const int* __restrict__ skel_gid, const unsigned char* __restrict__ skel_pass,
+    const long long* __restrict__ off, const T* __restrict__ Mb, int op, double alpha, double* U,
+    long long ldu, int R, double* side, long long side_n) {
+    constexpr int CB = 4 * CPT;
+    const int t = blockIdx.x;
+    const int b = find_slot(tile_off, nbox, t);
+    const int tile = t - tile_off[b];
+    const int c0 = blockIdx.y * CB;
+    const int r = dr[b], s = ds[b];
+    if (r == 0 || (op != 4 && s == 0)) return;
+    const T* M = Mb + off[b];
+    const bool in_red = (op == 0 || op == 3 || op == 4);
+    const bool out_red = (op == 1 || op == 2 || op == 4);
+    const bool trans = (op == 0 || op == 2);
+    const int nin = in_red ? r : s, nout = out_red ? r : s;
+    const int* ig = in_red ? red_gid + red_off[b] : skel_gid + skel_off[b];
+    const int* og = out_red ? red_gid + red_off[b] : skel_gid + skel_off[b];
+    const int o0 = tile * 64;
+    const int cnt = min(64, nout - o0);
+    __shared__ double xs[CB][64];
+    __shared__ T Ms[64][65];  // Ms[output][input]
+    const int tid = threadIdx.x, o = tid & 63, g = tid >> 6;
+    double acc[CPT];
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) acc[q] = 0.0;
+    for (int j0 = 0; j0 < nin; j0 += 64) {
+        const int kb = min(64, nin - j0);
+#pragma unroll
+        for (int q = 0; q < CPT; ++q) {
+            const int e = tid + 256 * q;
+            const int j = e & 63, c = e >> 6;
+            double v = 0.0;
+            if (j < kb && c0 + c < R) v = U[ig[j0 + j] + (long long)(c0 + c) * ldu];
+            xs[c][j] = v;
+        }
+        if (!trans) {  // M: nout x nin, leading dim nout
+#pragma unroll 4
+            for (int q = 0; q < 16; ++q) {
+                const int e = tid + 256 * q;
+                const int oo = e & 63, jj = e >> 6;
+                Ms[oo][jj] = (oo < cnt && jj < kb) ? M[(size_t)(j0 + jj) * nout + o0 + oo] : (T)0;
+            }
+        } else {  // M: nin x nout, leading dim nin
+#pragma unroll 4
+            for (int q = 0; q < 16; ++q) {
+                const int e = tid + 256 * q;
+                const int jj = e & 63, oo = e >> 6;
+                Ms[oo][jj] = (oo < cnt && jj < kb) ? M[(size_t)(o0 + oo) * nin + j0 + jj] : (T)0;
+            }
+        }
+        __syncthreads();
+        for (int jj = 0; jj < kb; ++jj) {
+            const double m = (double)Ms[o][jj];
+#pragma unroll
+            for (int q = 0; q < CPT; ++q) acc[q] += m * xs[g * CPT + q][jj];
+        }
+        __syncthreads();
+    }
+    if (o >= cnt) return;
+    const int dof = og[o0 + o];
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+        const int c = c0 + g * CPT + q;
+        if (c >= R) break;
+        if (op == 4) {
+            U[dof + (long long)c * ldu] = acc[q];
+        } else if (side_n > 0) {
+            const int p = skel_pass[skel_off[b] + o0 + o];
+            side[((long long)p * R + c) * side_n + dof] = alpha * acc[q];
+        } else {
+            U[dof + (long long)c * ldu] += alpha * acc[q];
+        }
+    }
+}
+
+// U[g + c ldu] += side[(p R + c) side_n + g] for the entries with pass flag p
+__global__ void k_rhs_side_add(long long ne, const int* __restrict__ gid,
+                               const unsigned char* __restrict__ sp,
+                               const double* __restrict__ side, long long side_n, int R, int p,
+                               double* U, long long ldu) {
+    const long long total = ne * R;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int c = (int)(e / ne);
+        const long long t = e - (long long)c * ne;
+        if ((int)sp[t] != p) continue;
+        const int g = gid[t];
+        U[g + c * ldu] += side[((long long)p * R + c) * side_n + g];
+    }
+}
+
+// Y(:, c) = A X(:, c) for R columns (CSR A, n rows)
+__global__ void k_csr_spmm(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                           const double* __restrict__ v, const double* __restrict__ X, long long ldx,
+                           double* Y, long long ldy, int R) {
+    const long long total = (long long)n * R;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int c = (int)(e / n);
+        const int i = (int)(e - (long long)c * n);
+        const double* x = X + c * ldx;
+        double acc = 0.0;
+        for (int q = rp[i]; q < rp[i + 1]; ++q) acc += v[q] * x[ci[q]];
+        Y[i + c * ldy] = acc;
+    }
+}
+
+// part[c * nblk + blk] = partial sum of X(:, c) . Y(:, c)
+__global__ void __launch_bounds__(256) k_dot_cols(long long n, const double* __restrict__ X,
+                                                  long long ldx, const double* __restrict__ Y,
+                                                  long long ldy, double* part) {
+    const int c = blockIdx.y;
+    const double* x = X + c * ldx;
+    const double* y = Y + c * ldy;
+    double acc = 0.0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        acc += x[i] * y[i];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    __shared__ double ws[4];
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) part[(long long)c * gridDim.x + blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
+}
+
+__global__ void k_sum_cols(const double* __restrict__ part, int nblk, int R, double* out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= R) return;
+    double acc = 0.0;
+    for (int b = 0; b < nblk; ++b) acc += part[(long long)c * nblk + b];
+    out[c] = acc;
+}
+
+// Y(:, c) = Y(:, c) + a[c] X(:, c)   (xpby == 0)
+// Y(:, c) = X(:, c) + a[c] Y(:, c)   (xpby == 1)
+__global__ void k_axpy_cols(long long n, const double* __restrict__ a, const double* X,
+                            long long ldx, double* Y, long long ldy, int R, int xpby) {
+    const long long total = n * R;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int c = (int)(e / n);
+        const long long i = e - (long long)c * n;
+        const double x = X[i + c * ldx];
+        double& y = Y[i + c * ldy];
+        y = xpby ? x + a[c] * y : y + a[c] * x;
+    }
+}
+
+}  // namespace
+
+template <class T>
+void rhs_gather(int nbox, long long nentries, const int* d_off, const int* d_gid, const double* U,
+                long long ldu, int R, T* pack, hipStream_t s) {
+    if (nbox <= 0 || nentries <= 0) return;
+    k_rhs_gather<T><<<grid1(nentries), 256, 0, s>>>(nbox, d_off, d_gid, U, ldu, R, pack);
+    HIFDE_LAUNCH_CHECK();
+}
+
+template <class T>
+void rhs_scatter(int nbox, long long nentries, const int* d_off, const int* d_gid, const T* pack,
+                 int R, double* U, long long ldu, double alpha, int mode, const unsigned char* d_sp,
+                 long long side_n, hipStream_t s) {
+    if (nbox <= 0 || nentries <= 0) return;
+    k_rhs_scatter<T><<<grid1(nentries), 256, 0, s>>>(nbox, d_off, d_gid, pack, R, U, ldu, alpha,
+                                                     mode, d_sp, side_n);
+    HIFDE_LAUNCH_CHECK();
+}
+
+template <class T>
+void apply_multi_fused(int nbox, int ntiles, const int* d_tile_off, const int* d_r, const int* d_s,
+                       const int* d_red_off, const int* d_skel_off, const int* d_red_gid,
+                       const int* d_skel_gid, const unsigned char* d_skel_pass,
+                       const long long* d_off, const T* d_M, int op, double alpha, double* U,
+                       long long ldu, int R, double* side, long long side_n, double flops,
+                       double bytes, hipStream_t s) {
+    if (nbox <= 0 || ntiles <= 0 || R <= 0) return;
+    KernelTimer* kt = active_timer();
+    if (kt) kt->begin(s);
+    const int cpt = R <= 4 ? 1 : R <= 8 ? 2 : R <= 16 ? 4 : 8;
+    const dim3 grid(ntiles, (R + 4 * cpt - 1) / (4 * cpt));
+#define HIFDE_AM(CPT)                                                                           \
+    k_apply_multi<T, CPT><<<grid, 256, 0, s>>>(nbox, d_tile_off, d_r, d_s, d_red_off,           \
+                                               d_skel_off, d_red_gid, d_skel_gid, d_skel_pass, \
+                                               d_off, d_M, op, alpha, U, ldu, R, side, side_n)
+    if (cpt == 1) HIFDE_AM(1);
+    else if (cpt == 2) HIFDE_AM(2);
+    else if (cpt == 4) HIFDE_AM(4);
+    else HIFDE_AM(8);
+#undef HIFDE_AM
+    HIFDE_LAUNCH_CHECK();
+    if (kt)
+        kt->end(s, sizeof(T) == sizeof(float) ? "k_apply_multi<float>" : "k_apply_multi<double>",
+                flops, bytes);
+}
+
+void rhs_side_add(long long nentries, const int* d_gid, const unsigned char* d_sp,
+                  const double* side, long long side_n, int R, double* U, long long ldu,
+                  hipStream_t s) {
+    if (nentries <= 0) return;
+    for (int p = 0; p < 2; ++p) {
+        k_rhs_side_add<<<grid1(nentries * R), 256, 0, s>>>(nentries, d_gid, d_sp, side, side_n, R,
+                                                           p, U, ldu);
+        HIFDE_LAUNCH_CHECK();
+    }
+}
+
+void csr_spmm(int n, const int* d_rp, const int* d_ci, const double* d_v, const double* X,
+              long long ldx, double* Y, long long ldy, int R, hipStream_t s) {
+    k_csr_spmm<<<grid1((long long)n * R), 256, 0, s>>>(n, d_rp, d_ci, d_v, X, ldx, Y, ldy, R);
+    HIFDE_LAUNCH_CHECK();
+}
+
+void dot_cols(long long n, const double* X, long long ldx, const double* Y, long long ldy, int R,
+              double* d_part, int nblk, double* d_out, hipStream_t s) {
+    k_dot_cols<<<dim3(nblk, R), 256, 0, s>>>(n, X, ldx, Y, ldy, d_part);
+    HIFDE_LAUNCH_CHECK();
+    k_sum_cols<<<(R + 63) / 64, 64, 0, s>>>(d_part, nblk, R, d_out);
+    HIFDE_LAUNCH_CHECK();
+}
+
+void axpy_cols(long long n, const double* d_a, const double* X, long long ldx, double* Y,
+               long long ldy, int R, bool xpby, hipStream_t s) {
+    k_axpy_cols<<<grid1(n * R), 256, 0, s>>>(n, d_a, X, ldx, Y, ldy, R, xpby ? 1 : 0);
+    HIFDE_LAUNCH_CHECK();
+}
+
+template void rhs_gather<float>(int, long long, const int*, const int*, const double*, long long,
+                                int, float*, hipStream_t);
+template void rhs_gather<double>(int, long long, const int*, const int*, const double*, long long,
+                                 int, double*, hipStream_t);
+template void apply_multi_fused<float>(int, int, const int*, const int*, const int*, const int*,
+                                       const int*, const int*, const int*, const unsigned char*,
+                                       const long long*, const float*, int, double, double*,
+                                       long long, int, double*, long long, double, double,
+                                       hipStream_t);
+template void apply_multi_fused<double>(int, int, const int*, const int*, const int*, const int*,
+                                        const int*, const int*, const int*, const unsigned char*,
+                                        const long long*, const double*, int, double, double*,
+                                        long long, int, double*, long long, double, double,
+                                        hipStream_t);
+template void rhs_scatter<float>(int, long long, const int*, const int*, const float*, int,
+                                 double*, long long, double, int, const unsigned char*, long long,
+                                 hipStream_t);
+template void rhs_scatter<double>(int, long long, const int*, const int*, const double*, int,
+                                  double*, long long, double, int, const unsigned char*, long long,
+                                  hipStream_t);
+
+}  // namespace hifde

@@ -76,7 +76,10 @@ int hifde_factor_run(hifde_factor* f);
  * (N+1)^3 doubles, overwritten with F u or F^{-1} u (boundary entries are
  * zeroed first, src/Factor.cpp:1001). */
 int hifde_apply(hifde_factor* f, double* u, int inverse);
-/* Same on device memory: nrhs vectors at d_u + c*ldu. */
+/* Same on device memory: nrhs vectors at d_u + c*ldu.  For nrhs > 1 every
+ * box step is one batched GEMM over the nrhs columns (the factor blocks are
+ * read once per sweep); mixed-precision levels then round the vectors to
+ * FP32 for the product, as their factor blocks are FP32. */
 int hifde_apply_device(hifde_factor* f, double* d_u, int nrhs, long long ldu, int inverse);
 
 /* SpMV(A, x, y) -- src/Factor.cpp:1099-1113.  transpose == 1 computes the
@@ -90,6 +93,16 @@ int hifde_pcg(hifde_factor* f, const double* b, double* x, double tol, int maxit
               double* relres);
 int hifde_pcg_device(hifde_factor* f, const double* d_b, double* d_x, double tol, int maxit,
                      int* iters, double* relres);
+/* nrhs independent PCG solves advanced together (no reference counterpart):
+ * column c of B at d_b + c*ldb, of X at d_x + c*ldx (initial guesses on
+ * entry).  Every column follows hifde_pcg's recurrence and stopping rule; a
+ * column that has converged is left unchanged while the others continue.
+ * iters[c] and relres[c] receive each column's count and final residual.
+ * The preconditioner applies F^{-1} to all columns at once (batched GEMMs
+ * per box), as hifde_apply_device does for nrhs > 1. */
+int hifde_pcg_multi_device(hifde_factor* f, const double* d_b, long long ldb, double* d_x,
+                           long long ldx, int nrhs, double tol, int maxit, int* iters,
+                           double* relres);
 
 /* Introspection (the reference prints these, src/Factor.cpp:211-212,258-259). */
 int hifde_num_levels(const hifde_factor* f);

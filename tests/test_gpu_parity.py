@@ -433,3 +433,93 @@ def test_nonlocal_operator_takes_table_path(H, oracle_mod):
     b = pb.rhs_normal(NC, 4)
     y, y_o = f.Apply(b, True), o.apply(b, True)
     assert _rel(y, y_o) < 1e-8
+
+
+# --------------------------------------------------------------------------
+# several right-hand sides
+# --------------------------------------------------------------------------
+def _device_cols(U, ld):
+    import torch
+    n, R = U.shape
+    d = torch.zeros(R, ld, dtype=torch.float64, device="cuda")
+    d[:, :n] = torch.from_numpy(np.ascontiguousarray(U.T))
+    return d
+
+
+@pytest.mark.parametrize("precision,R,tol,fused_max", [
+    ("fp64", 3, 1e-12, 512), ("mixed", 3, 1e-6, 512), ("fp64", 70, 1e-12, 512),
+    ("fp64", 3, 1e-12, 0), ("mixed", 3, 1e-4, 0), ("fp64", 70, 1e-12, 0),
+    ("mixed", 20, 1e-6, 100)])
+@pytest.mark.parametrize("inverse", [True, False])
+def test_multi_rhs_apply_matches_column_by_column(H, oracle_mod, precision, R, tol, fused_max,
+                                                  inverse):
+    """nrhs > 1 treats all columns per box step at once (fused kernel for
+    boxes up to fused_max inputs/outputs, packed batched GEMMs beyond); each
+    column must equal the single-vector apply of that column.  The packed
+    GEMMs of FP32 levels round the vectors to FP32 (unit roundoff 6e-8),
+    hence the looser bars of the mixed cases, loosest where every box takes
+    the packed path; the fused kernel accumulates in FP64 like the
+    single-vector one."""
+    import torch
+    f, _, _ = _both(H, oracle_mod, 31, 4, 1e-4, "pow10", True, precision,
+                    options={"multi_fused_max": fused_max})
+    n = 32 ** 3
+    U = np.stack([pb.rhs_normal(32, 40 + c) for c in range(R)], axis=1)
+    ref = np.stack([f.Apply(U[:, c].copy(), inverse) for c in range(R)], axis=1)
+    ld = n + 5
+    d = _device_cols(U, ld)
+    f.apply_device(d.data_ptr(), R, ld, inverse)
+    torch.cuda.synchronize()
+    got = d[:, :n].cpu().numpy().T
+    for c in range(R):
+        assert _rel(got[:, c], ref[:, c]) < tol, c
+    assert bool((d[:, n:] == 0).all())  # padding between columns untouched
+
+
+@pytest.mark.parametrize("sym", [True, False])
+def test_multi_rhs_apply_table_path_and_general_storage(H, oracle_mod, sym):
+    """Same check with the hash-table level matrix and the general
+    (non-symmetric) fast path, FP64."""
+    import torch
+    f, _, _ = _both(H, oracle_mod, 23, 3, 1e-5, "pow10", sym, "fp64",
+                    options={"elements": 0})
+    n = 24 ** 3
+    R = 4
+    U = np.stack([pb.rhs_normal(24, 60 + c) for c in range(R)], axis=1)
+    for inverse in (True, False):
+        ref = np.stack([f.Apply(U[:, c].copy(), inverse) for c in range(R)], axis=1)
+        d = _device_cols(U, n)
+        f.apply_device(d.data_ptr(), R, n, inverse)
+        torch.cuda.synchronize()
+        got = d.cpu().numpy().T
+        for c in range(R):
+            assert _rel(got[:, c], ref[:, c]) < 1e-12
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_pcg_multi_columns_converge_independently(H, oracle_mod, precision):
+    """Block of independent PCG solves: every column reaches 1e-10 and agrees
+    with the single-RHS solve of it; a zero column stays at zero with no
+    iteration; a scaled copy of a column takes the same iterations."""
+    import torch
+    f, _, A = _both(H, oracle_mod, 31, 4, 1e-4, "pow10", True, precision)
+    n = 32 ** 3
+    R = 5
+    B = np.stack([pb.rhs_normal(32, 80 + c) for c in range(R)], axis=1)
+    B[:, 2] = 0.0
+    B[:, 3] = 1e6 * B[:, 0]
+    ldb, ldx = n + 3, n + 11
+    dB = _device_cols(B, ldb)
+    dX = _device_cols(np.zeros((n, R)), ldx)
+    its, rrs = f.pcg_multi_device(dB.data_ptr(), dX.data_ptr(), R, ldb=ldb, ldx=ldx,
+                                  tol=1e-10, maxit=200)
+    X = dX[:, :n].cpu().numpy().T
+    assert its[2] == 0 and rrs[2] == 0.0 and not X[:, 2].any()
+    assert abs(int(its[3]) - int(its[0])) <= 1
+    for c in (0, 1, 3, 4):
+        assert rrs[c] <= 1e-10
+        assert _rel(A @ X[:, c], B[:, c]) <= 2e-10
+        x1, it1, rr1 = f.pcg(B[:, c], tol=1e-10, maxit=200)
+        assert abs(int(its[c]) - it1) <= 1
+        assert _rel(X[:, c], x1) < 1e-8
+    assert bool((dX[:, n:] == 0).all())
