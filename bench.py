@@ -242,7 +242,9 @@ def main():
     kname, avg_launch, roof = roofline_from_stats(stats)
     log(f"[bench] dominant kernel {kname}: avg launch {avg_launch * 1e3:.3f} ms")
     for k_, v_ in sorted(stats.items(), key=lambda kv: -kv[1]["sec"]):
-        log(f"[bench]   {k_:40s} launches {v_['launches']:6d} dev {v_['sec']:.4f}s")
+        sec_ = max(v_["sec"], 1e-12)
+        log(f"[bench]   {k_:40s} launches {v_['launches']:6d} dev {v_['sec']:.4f}s "
+            f"{v_['flops'] / sec_ / 1e12:7.2f} TF/s {v_['bytes'] / sec_ / 1e9:8.1f} GB/s")
     fbytes_mixed = f.factor_bytes()
     f.close()
 

@@ -109,7 +109,7 @@ __device__ void block_argmax(T v, int idx, T* redv, int* redi, T* outv, int* out
 // ---------------------------------------------------------------------------
 constexpr int kSmallGJ = 64;
 constexpr int kGJB = 64;
-constexpr int kGJPanel = 256;
+constexpr int kGJPanel = 512;
 
 // Wave-wide arg-max of non-negative values, ties to the lower lane (values
 // equal in all but their 6 lowest mantissa bits count as tied).  The key is
@@ -268,22 +268,65 @@ __global__ void __launch_bounds__(256) k_gj_small(const int* __restrict__ dims,
     if (tid == 0) info[b] = 0;
 }
 
-// Inverse D of the kb x kb diagonal block K of a window (kb <= 64), one wave
-// per box.  Lane i owns row i of diag(A_KK, I_{64-kb}) in registers.  Rotated
-// in-place Gauss-Jordan: the pivot column is always register 0; every step
-// shifts each row left by one and appends the new in-place column as register
-// 63, so after 64 steps register t holds in-place column t and no register is
-// indexed at run time.  The pivot row is broadcast with readlane, so a step
-// needs no LDS traffic and no barrier.  The pivot of step k is the not yet used row of
-// largest |a_ik| (ties to the lower row), i.e. partial pivoting.  With p_k the
-// pivot row of step k, the in-place array Z satisfies A^{-1}(k, p_t) =
-// Z(p_k, t), which is how each lane writes its row out.
+// Inverse D of the kb x kb diagonal block K of a window (kb <= 64), one
+// workgroup of kDiagWaves waves per box.  Lane i of every wave holds row i of
+// diag(A_KK, I_{64-kb}); wave w holds the columns w + kDiagWaves q in
+// registers, so the pivot row of a step is lane p of each wave and is read
+// with readlane, without LDS traffic.  In-place Gauss-Jordan with partial
+// pivoting: the pivot of step k is the not yet used row of largest |a_ik|
+// (compared on a 32-bit key, ties to the lower row).  The wave holding column
+// k publishes it in LDS
+// (double-buffered, one barrier per step); every wave reads it, finds the same
+// pivot and updates its own columns.  With p_k the pivot row of step k, the
+// in-place array Z satisfies A^{-1}(k, p_c) = Z(p_k, c), which is how each
+// lane writes its row out.
+constexpr int kDiagWaves = 8;
+constexpr int kDiagCols = kGJB / kDiagWaves;
+// 32-bit arg-max key of a non-negative value: the high word of a double (sign,
+// exponent, 20 mantissa bits) or a float's bits, orders like the value; its
+// low 6 bits are replaced by 63 - lane, so values equal in the remaining bits
+// (relative 2^-14 for double, 2^-17 for float) tie to the lower lane.
+__device__ __forceinline__ unsigned argmax_key(double v, int lane) {
+    return ((unsigned)__double2hiint(v) & ~0x3Fu) | (unsigned)(63 - lane);
+}
+__device__ __forceinline__ unsigned argmax_key(float v, int lane) {
+    return ((unsigned)__float_as_int(v) & ~0x3Fu) | (unsigned)(63 - lane);
+}
+template <int N>
+__device__ __forceinline__ unsigned row_shr_u(unsigned x) {
+    return (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x110 + N, 0xF, 0xF, true);
+}
+// Arg-max of the keys over the wave (uniform); nonzero is false when every
+// value is zero.
+__device__ __forceinline__ int wave_argmax_key(unsigned k, bool& nonzero) {
+    k = max(k, row_shr_u<1>(k));
+    k = max(k, row_shr_u<2>(k));
+    k = max(k, row_shr_u<4>(k));
+    k = max(k, row_shr_u<8>(k));
+    const unsigned m0 = (unsigned)__builtin_amdgcn_readlane((int)k, 15);
+    const unsigned m1 = (unsigned)__builtin_amdgcn_readlane((int)k, 31);
+    const unsigned m2 = (unsigned)__builtin_amdgcn_readlane((int)k, 47);
+    const unsigned m3 = (unsigned)__builtin_amdgcn_readlane((int)k, 63);
+    const unsigned m = max(max(m0, m1), max(m2, m3));
+    nonzero = (m & ~0x3Fu) != 0;
+    return 63 - (int)(m & 0x3Fu);
+}
+// 1/x from the hardware reciprocal and Newton steps r <- r + r (1 - x r).
+__device__ __forceinline__ double recip(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(fma(-x, r, 1.0), r, r);
+    return fma(fma(-x, r, 1.0), r, r);
+}
+__device__ __forceinline__ float recip(float x) {
+    const float r = __builtin_amdgcn_rcpf(x);
+    return fmaf(fmaf(-x, r, 1.0f), r, r);
+}
 template <class T>
-__global__ void __launch_bounds__(64) k_bgj_diag(const int* __restrict__ dims,
-                                                 const long long* __restrict__ off,
-                                                 const int* __restrict__ lds,
-                                                 const T* __restrict__ mats, T* Dws, int step,
-                                                 int ibase, int* info) {
+__global__ void __launch_bounds__(kDiagWaves * 64) k_bgj_diag(const int* __restrict__ dims,
+                                                             const long long* __restrict__ off,
+                                                             const int* __restrict__ lds,
+                                                             const T* __restrict__ mats, T* Dws,
+                                                             int step, int ibase, int* info) {
     const int b = blockIdx.x;
     const int n = dims[b];
     const int k0 = step * kGJB;
@@ -291,52 +334,73 @@ __global__ void __launch_bounds__(64) k_bgj_diag(const int* __restrict__ dims,
     const int kb = min(kGJB, n - k0);
     const int ld = lds[b];
     const T* A = mats + off[b] + k0 + (size_t)k0 * ld;
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __shared__ T colbuf[2][kGJB];
     __shared__ int piv[kGJB];
-    T reg[kGJB];
+    T a[kDiagCols];
 #pragma unroll
-    for (int c = 0; c < kGJB; ++c)
-        reg[c] = (lane < kb && c < kb) ? A[lane + (size_t)c * ld] : (T)(c == lane ? 1 : 0);
+    for (int q = 0; q < kDiagCols; ++q) {
+        const int c = w + kDiagWaves * q;
+        a[q] = (lane < kb && c < kb) ? A[lane + (size_t)c * ld] : (T)(c == lane ? 1 : 0);
+    }
+    if (w == 0) colbuf[0][lane] = a[0];
+    __syncthreads();
     bool used = false;
     int my_step = 0;
     int bad = 0;
-    for (int k = 0; k < kGJB; ++k) {
-        bool nz;
-        const int p = __builtin_amdgcn_readfirstlane(
-            wave_argmax_nonneg<T>(used ? (T)0 : fabs(reg[0]), lane, nz));
-        if (__builtin_amdgcn_readfirstlane(nz ? 0 : 1)) { bad = k + 1; break; }
-        const bool me = lane == p;
-        T d = (T)0;
-        if (me) {
-            d = (T)1 / reg[0];
+    // step k = kq kDiagWaves + kw: column k is register kq of wave kw, so with
+    // kq unrolled every register index is a constant
 #pragma unroll
-            for (int j = 1; j < kGJB; ++j) reg[j] *= d;
-            piv[k] = lane;
-            used = true;
-            my_step = k;
+    for (int kq = 0; kq < kDiagCols; ++kq) {
+#pragma unroll 1
+        for (int kw = 0; kw < kDiagWaves; ++kw) {
+            const int k = kq * kDiagWaves + kw;
+            const T f = colbuf[k & 1][lane];
+            bool nz;
+            const int p = __builtin_amdgcn_readfirstlane(
+                wave_argmax_key(argmax_key(used ? (T)0 : fabs(f), lane), nz));
+            if (!nz) {
+                bad = k + 1;
+                goto done;
+            }
+            const bool me = lane == p;
+            const T d = recip(read_lane(f, p));
+#pragma unroll
+            for (int q = 0; q < kDiagCols; ++q) {
+                const T prd = read_lane(a[q], p) * d;  // scaled pivot row
+                a[q] = me ? prd : fma(-f, prd, a[q]);
+            }
+            if (w == kw) a[kq] = me ? d : -f * d;  // the pivot column itself
+            if (me) {
+                used = true;
+                my_step = k;
+                if (w == 0) piv[k] = lane;
+            }
+            // publish column k + 1
+            if (kw + 1 < kDiagWaves) {
+                if (w == kw + 1) colbuf[(k + 1) & 1][lane] = a[kq];
+            } else if (kq + 1 < kDiagCols) {
+                if (w == 0) colbuf[(k + 1) & 1][lane] = a[min(kq + 1, kDiagCols - 1)];
+            }
+            __syncthreads();
         }
-        // the scaled pivot row is read from lane p register by register
-        const T dd = read_lane(d, p);
-        const T f = me ? (T)0 : reg[0];
-#pragma unroll
-        for (int j = 1; j < kGJB; ++j) reg[j - 1] = fma(-f, read_lane(reg[j], p), reg[j]);
-        reg[kGJB - 1] = me ? dd : -f * dd;
     }
-    __syncthreads();
+done:
     if (bad) {
-        if (lane == 0) info[b] = ibase + k0 + bad;
+        if (threadIdx.x == 0) info[b] = ibase + k0 + bad;
         return;
     }
     T* D = Dws + (size_t)b * kGJB * kGJB;
     if (my_step < kb) {
 #pragma unroll
-        for (int t = 0; t < kGJB; ++t) {
-            const int c = piv[t];
-            if (c < kb) D[my_step + (size_t)c * kGJB] = reg[t];
+        for (int q = 0; q < kDiagCols; ++q) {
+            const int c = w + kDiagWaves * q;
+            const int pc = piv[c];
+            if (c < kb && pc < kb) D[my_step + (size_t)pc * kGJB] = a[q];
         }
     }
     // rows and columns >= kb of the 64 x 64 workspace are zero
-    for (int e = lane; e < kGJB * kGJB; e += kWave) {
+    for (int e = threadIdx.x; e < kGJB * kGJB; e += kDiagWaves * 64) {
         const int c = e >> 6, i = e & 63;
         if (c >= kb || i >= kb) D[e] = (T)0;
     }
@@ -1216,7 +1280,7 @@ void batched_gj_inverse(int nbox, const int* h_n, const long long* h_off, const 
             const dim3 gch((max_w + 63) / 64, nbig);
             for (int st = 0; st < isteps; ++st) {
                 if (exec) {
-                    k_bgj_diag<T><<<nbig, kWave, 0, s>>>(dwn, dwoff, dwld, d_A, Dws, st, K0,
+                    k_bgj_diag<T><<<nbig, kDiagWaves * 64, 0, s>>>(dwn, dwoff, dwld, d_A, Dws, st, K0,
                                                           cinfo);
                     k_bgj_row<T><<<gch, 256, 0, s>>>(dwn, dwoff, dwld, d_A, Dws, Rws, ws.b_off.p,
                                                       st, cinfo);
@@ -1338,7 +1402,10 @@ std::pair<int, int> GemmPlan::add(const std::vector<GemmDesc>& d, bool transA) {
         tiles.push_back(0);
         for (const GemmDesc& g : d) {
             if (g.m <= 0 || g.n <= 0) continue;
-            const bool large = g.m >= TILE / 2 && g.n >= TILE / 2;
+            // short products over small outputs (the in-window updates of the
+            // block inverse) finish sooner on the FMA kernel's 64 x 64 tiles
+            const bool small = g.k <= 64 && (long long)g.m * g.n <= 256LL * 256;
+            const bool large = g.m >= TILE / 2 && g.n >= TILE / 2 && !small;
             if (large != (pass == 1)) continue;
             const int tm = (g.m + tile - 1) / tile, tn = (g.n + tile - 1) / tile;
             desc.push_back(g);
