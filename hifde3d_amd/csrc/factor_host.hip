@@ -327,7 +327,7 @@ struct hifde_factor {
         // The boxes are walked in memory order (i fastest); the lists are
         // then put in the reference's scan order (i, j, k lexicographic, k
         // fastest) by sorting the live points.
-#pragma omp parallel for schedule(dynamic, 16)
+#pragma omp parallel for schedule(dynamic, 1)
         for (int b = 0; b < B.nbox; ++b) {
             int cc[3] = {b / (c * c), (b / c) % c, b % c};
             int lo[3], hi[3];
@@ -372,12 +372,17 @@ struct hifde_factor {
                 }
                 ctab.off[b + 1] = ctab.off[b] + vol;
             }
-            ctab.tab.assign(ctab.off.back(), -1);
+            // every position starts as -1 (no live DOF); filled box by box
+            // in parallel, the buffer only ever grows
+            if ((long long)ctab.tab.size() < ctab.off.back()) ctab.tab.resize(ctab.off.back());
+#pragma omp parallel for schedule(dynamic, 1)
+            for (int b = 0; b < B.nbox; ++b)
+                std::fill(ctab.tab.begin() + ctab.off[b], ctab.tab.begin() + ctab.off[b + 1], -1);
         }
 #pragma omp parallel
         {
             std::vector<std::pair<long long, int>> pts;  // (scan key, gid)
-#pragma omp for schedule(dynamic, 16)
+#pragma omp for schedule(dynamic, 1)
             for (int b = 0; b < B.nbox; ++b) {
                 int cc[3] = {b / (c * c), (b / c) % c, b % c};
                 int lo[3], hi[3];
@@ -436,8 +441,16 @@ struct hifde_factor {
     }
 
     // Frontal position of DOF g in cell b of the current level's table, or -1.
+    // g / NC as a multiply and shift: with mul = floor(2^40 / NC) + 1 the
+    // quotient is exact for g < 2^40 / NC, which holds for every grid index
+    // when NC < 1024 (checked where the factorization is created).
+    unsigned div_nc(unsigned g) const {
+        const unsigned long long mul = (1ULL << 40) / (unsigned)NC + 1;
+        return (unsigned)(((unsigned long long)g * mul) >> 40);
+    }
     int cell_pos(int b, int g) const {
-        const int x = g % NC, y = (g / NC) % NC, z = g / (NC * NC);
+        const unsigned q1 = div_nc((unsigned)g), q2 = div_nc(q1);
+        const int x = g - (int)q1 * NC, y = (int)q1 - (int)q2 * NC, z = (int)q2;
         const int lx = x - ctab.lo[3 * b], ly = y - ctab.lo[3 * b + 1], lz = z - ctab.lo[3 * b + 2];
         const int ex = ctab.ext[3 * b], ey = ctab.ext[3 * b + 1], ez = ctab.ext[3 * b + 2];
         if (lx < 0 || ly < 0 || lz < 0 || lx >= ex || ly >= ey || lz >= ez) return -1;
@@ -519,7 +532,7 @@ struct hifde_factor {
         const int ncand = 3 * c * c * c;
         static const int order[3] = {TOP, FRONT, LEFT};  // src/Factor.cpp:249-254
         std::vector<int> cm(ncand, 0), cn(ncand, 0);
-#pragma omp parallel for schedule(dynamic, 16)
+#pragma omp parallel for schedule(dynamic, 1)
         for (int t = 0; t < ncand; ++t) {
             int b = t / 3, f = order[t % 3];
             int ci = b / (c * c), cj = (b / c) % c, ck = b % c;
@@ -543,7 +556,7 @@ struct hifde_factor {
         F.cols_off = prefix_i(F.n);
         F.rows.resize(F.rows_off.back());
         F.cols.resize(F.cols_off.back());
-#pragma omp parallel for schedule(dynamic, 16)
+#pragma omp parallel for schedule(dynamic, 1)
         for (int t = 0; t < ncand; ++t) {
             if (slot[t] < 0) continue;
             int b = t / 3, f = order[t % 3];
@@ -764,7 +777,7 @@ struct hifde_factor {
                       g);
         }
         d_init.upload(desc, stream);
-        d_etab.upload(ctab.tab, stream);
+        d_etab.upload(ctab.tab.data(), ctab.off.back(), stream);
         d_egid.upload(gids, stream);
     }
 
@@ -813,7 +826,7 @@ struct hifde_factor {
             if (moff.back() > (long long)INT32_MAX) throw Error(HIFDE_ERR_SIZE, "extend-add maps");
             std::vector<int> maps(moff.back());
             std::vector<std::vector<ExtendDesc>> pass(8, std::vector<ExtendDesc>(nb));
-#pragma omp parallel for schedule(dynamic, 4)
+#pragma omp parallel for schedule(dynamic, 1)
             for (int b = 0; b < nb; ++b)
                 for (int k = 0; k < 8; ++k) {
                     const int ch = child(b, k);
@@ -882,7 +895,7 @@ struct hifde_factor {
         fd.cpos1.assign(F.cols_off.back(), 0);
         std::vector<FaceGatherDesc> gd(nf);
         int bad = 0;
-#pragma omp parallel for schedule(dynamic, 16) reduction(max : bad)
+#pragma omp parallel for schedule(dynamic, 1) reduction(max : bad)
         for (int f = 0; f < nf; ++f) {
             const int c1 = F.c1[f], c2 = F.c2[f], m = F.m[f], n = F.n[f];
             auto epos = [&](int cell, int g) -> unsigned {
@@ -1949,6 +1962,7 @@ int hifde_create(int N, int P, double epsilon, int precision, int flags, hifde_f
         int q = NC / P;
         if (q < 1 || (q & (q - 1)) != 0 || NC % P != 0)
             throw Error(HIFDE_ERR_ARG, "create: (N+1)/P must be a power of two");
+        if (NC >= 1024) throw Error(HIFDE_ERR_SIZE, "create: N + 1 must be below 1024");
         std::unique_ptr<hifde_factor> f(new hifde_factor());
         f->N = N;
         f->P = P;

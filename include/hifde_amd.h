@@ -48,7 +48,9 @@ typedef struct hifde_factor hifde_factor;
 #define HIFDE_ERR_STATE (-4)
 
 /* HIFFactor<Scalar>(N, P, epsilon) -- include/Factor.hpp:12-13.
- * Requires NC/P to be a power of two (src/Factor.cpp:36,51-53). */
+ * Requires NC/P to be a power of two (src/Factor.cpp:36,51-53) and
+ * NC = N + 1 < 1024 (HIFDE_ERR_SIZE otherwise; 1023^3 unknowns is far past
+ * one GPU's memory for this method). */
 int hifde_create(int N, int P, double epsilon, int precision, int flags, hifde_factor** out);
 /* ~HIFFactor -- include/Factor.hpp:15 */
 void hifde_destroy(hifde_factor* f);
