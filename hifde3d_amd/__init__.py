@@ -30,11 +30,13 @@ _dp = ctypes.POINTER(ctypes.c_double)
 EXPORTS = [
     "hifde_create", "hifde_destroy", "hifde_last_error", "hifde_set_matrix_coo",
     "hifde_set_stencil", "hifde_factor_run", "hifde_apply", "hifde_apply_device", "hifde_spmv",
-    "hifde_pcg", "hifde_pcg_device", "hifde_pcg_multi_device", "hifde_num_levels", "hifde_level_counts",
+    "hifde_pcg", "hifde_pcg_device", "hifde_pcg_multi_device", "hifde_num_levels",
+    "hifde_level_counts",
     "hifde_box_count", "hifde_box_sizes", "hifde_box_data", "hifde_stage_times",
     "hifde_level_stage_times",
     "hifde_stage_name", "hifde_factor_bytes", "hifde_interp_decomp_batch", "hifde_schur_batch",
-    "hifde_set_profiling", "hifde_reset_kernel_stats", "hifde_kernel_stats", "hifde_set_option",
+    "hifde_set_profiling", "hifde_set_profile_kernel", "hifde_reset_kernel_stats",
+    "hifde_kernel_stats", "hifde_set_option",
     "hifde_gemm_batch",
 ]
 
@@ -86,6 +88,7 @@ def lib():
     L.hifde_factor_bytes.argtypes = [_vp]
     L.hifde_factor_bytes.restype = _ll
     L.hifde_set_profiling.argtypes = [_vp, _i]
+    L.hifde_set_profile_kernel.argtypes = [_vp, ctypes.c_char_p]
     L.hifde_set_option.argtypes = [_vp, ctypes.c_char_p, _d]
     L.hifde_reset_kernel_stats.argtypes = [_vp]
     L.hifde_kernel_stats.argtypes = [_vp, _i, ctypes.c_char_p, _i, _dp]
@@ -261,8 +264,11 @@ class HIFFactor:
     def set_option(self, name, value):
         _check(lib().hifde_set_option(self._h, name.encode(), float(value)))
 
-    def set_profiling(self, on=True):
+    def set_profiling(self, on=True, kernel=None):
+        """Per-family device timing; with `kernel`, only that kernel's launches
+        are timed (low overhead)."""
         _check(lib().hifde_set_profiling(self._h, int(on)))
+        _check(lib().hifde_set_profile_kernel(self._h, kernel.encode() if kernel else None))
 
     def reset_kernel_stats(self):
         _check(lib().hifde_reset_kernel_stats(self._h))

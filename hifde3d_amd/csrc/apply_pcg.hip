@@ -315,14 +315,13 @@ void apply_small(int nbox, int nout_total, const int* d_r, const int* d_s, const
                  double bytes, hipStream_t s, long long side_n) {
     if (nbox <= 0 || nout_total <= 0) return;
     KernelTimer* kt = active_timer();
-    if (kt) kt->begin(s);
+    if (kt)
+        kt->begin(s, sizeof(T) == sizeof(float) ? "k_apply_small<float>" : "k_apply_small<double>");
     k_apply_small<T><<<(nbox + 3) / 4, 256, 0, s>>>(nbox, d_r, d_s, d_red_off, d_skel_off,
                                                     d_red_gid, d_skel_gid, d_skel_pass, d_off,
                                                     d_M, op, pass, alpha, d_in, d_out, side_n);
     HIFDE_LAUNCH_CHECK();
-    if (kt)
-        kt->end(s, sizeof(T) == sizeof(float) ? "k_apply_small<float>" : "k_apply_small<double>",
-                flops, bytes);
+    if (kt) kt->end(s, flops, bytes);
 }
 
 template <class T>
@@ -336,14 +335,13 @@ void apply_tiles(int nbox, int ntiles, const int* d_tile_off, const int* d_r, co
     size_t shm = (size_t)std::max(max_in, 1) * sizeof(double);
     if (shm > 150 * 1024) throw Error(-3, "apply: box too large for LDS");
     KernelTimer* kt = active_timer();
-    if (kt) kt->begin(s);
+    if (kt)
+        kt->begin(s, sizeof(T) == sizeof(float) ? "k_apply_tiles<float>" : "k_apply_tiles<double>");
     k_apply_tiles<T><<<ntiles, 256, shm, s>>>(nbox, d_tile_off, d_r, d_s, d_red_off, d_skel_off,
                                               d_red_gid, d_skel_gid, d_skel_pass, d_off, d_M, op,
                                               pass, alpha, d_in, d_out, side_n);
     HIFDE_LAUNCH_CHECK();
-    if (kt)
-        kt->end(s, sizeof(T) == sizeof(float) ? "k_apply_tiles<float>" : "k_apply_tiles<double>",
-                flops, bytes);
+    if (kt) kt->end(s, flops, bytes);
 }
 
 void side_add(int n, const int* d_gid, const unsigned char* d_sp, const double* d_side,

@@ -647,13 +647,49 @@ struct MfmaCfg<double> {
     }
 };
 
-constexpr int kMfmaTile = 128;
+
+// Raw buffer loads: an offset at or past num_records bytes reads zero
+// instead of faulting, which bounds the tile loads without branches.
+constexpr int kBufWord3 = 0x00020000;  // raw (unformatted) buffer
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes,
+                                             kBufWord3);
+}
+__device__ __forceinline__ double buf_load(__amdgpu_buffer_rsrc_t r, unsigned off, double) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+}
+__device__ __forceinline__ float buf_load(__amdgpu_buffer_rsrc_t r, unsigned off, float) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+}
+
+// LDS row pitch (elements) of a k-slab row: rows are read 32 bytes per lane
+// (its 4 doubles or 8 floats of k), and the pitch spreads 16 consecutive rows
+// over all 64 banks.
+template <class T>
+struct SlabPitch;
+template <>
+struct SlabPitch<double> {
+    static constexpr int value = 18;  // 144 B: 36 words, 16 rows hit 16 distinct bank quads
+};
+template <>
+struct SlabPitch<float> {
+    static constexpr int value = 20;  // 80 B: 20 words, idem
+};
 
 // C(m x n) = alpha op(A) B + beta C over a flattened list of 128 x 128 output
 // tiles of every descriptor, 4 waves in a 2 x 2 arrangement.  The k dimension
 // is walked in slabs of 16 held in two LDS buffers: the next slab is loaded
 // into registers while the current one feeds the matrix cores, so global
 // latency overlaps the products and each slab costs one barrier.
+//
+// Both operands sit in LDS as [row][k] (k contiguous).  The matrix-core
+// instruction sums over its KT k-slots, one per lane group g = lane / MT;
+// slot g of step s is mapped to k = g SPL + s (SPL = 16 / KT steps), so each
+// lane reads its whole slab share of a row, SPL consecutive k, with two
+// 128-bit LDS reads.  Tile rows and columns past m and n read whatever lies
+// within the operand's buffer (those outputs are not stored); k past the
+// end reads zero, from the buffer bound or an explicit select where the
+// operand is k-contiguous.  The host keeps each operand under 2 GiB.
 template <class T, bool TRANSA, class TC = T>
 __global__ void __launch_bounds__(256, 2) k_gemm_mfma(int nbox, const int* __restrict__ tile_off,
                                                       const GemmDesc* __restrict__ desc,
@@ -662,9 +698,11 @@ __global__ void __launch_bounds__(256, 2) k_gemm_mfma(int nbox, const int* __res
                                                       T beta, const int* __restrict__ flags) {
     typedef MfmaCfg<T> C;
     constexpr int MT = C::MT, KT = C::KT, BLK = C::BLK, WT = BLK * MT, TILE = 2 * WT, KB = 16;
+    constexpr int SPL = KB / KT;           // matrix-core steps per slab
+    constexpr int LDK = SlabPitch<T>::value;
+    constexpr int NLD = TILE * KB / 256;  // elements per thread and operand per slab
     static_assert(TILE == kMfmaTile, "tile");
-    constexpr int LDT = TILE + 4;
-    constexpr int NLD = TILE * KB / 256;
+    static_assert(SPL * sizeof(T) == 32, "a lane's slab share is 32 bytes");
     const int t = blockIdx.x;
     const int b = find_box(tile_off, nbox, t);
     const int local = t - tile_off[b];
@@ -674,39 +712,56 @@ __global__ void __launch_bounds__(256, 2) k_gemm_mfma(int nbox, const int* __res
     const int i0 = (local % tm) * TILE, j0 = (local / tm) * TILE;
     if (dd.lower && j0 > i0) return;
     if (dd.flag >= 0 && flags && flags[dd.flag] == 0) return;
-    const T* A = Ab + dd.offA;
-    const T* B = Bb + dd.offB;
     TC* Cm = Cb + dd.offC;
-    __shared__ T As[2][KB][LDT];
-    __shared__ T Bs[2][KB][LDT];
+    // operand extents in bytes: A is m x k (lda) or k x m (lda, TRANSA), B is k x n (ldb)
+    const __amdgpu_buffer_rsrc_t ra_rsrc =
+        make_rsrc(Ab + dd.offA, (unsigned)((size_t)dd.lda * (TRANSA ? m : k) * sizeof(T)));
+    const __amdgpu_buffer_rsrc_t rb_rsrc =
+        make_rsrc(Bb + dd.offB, (unsigned)((size_t)dd.ldb * n * sizeof(T)));
+    __shared__ __attribute__((aligned(16))) T As[2][TILE][LDK];
+    __shared__ __attribute__((aligned(16))) T Bs[2][TILE][LDK];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wr = (wave & 1) * WT, wc = (wave >> 1) * WT;
+    // Per-thread load slots.  k-contiguous operands (B, and A when TRANSA):
+    // 16 lanes cover the 16 k of one row, e = tid + 256 q -> (k e % 16,
+    // row e / 16).  A column-major with rows contiguous: (row e % 128,
+    // k e / 128).
+    unsigned offa[NLD], offb[NLD];
+#pragma unroll
+    for (int q = 0; q < NLD; ++q) {
+        const int e = tid + 256 * q;
+        if (!TRANSA) offa[q] = (unsigned)(((i0 + e % TILE) + (size_t)(e / TILE) * dd.lda) * sizeof(T));
+        else offa[q] = (unsigned)(((e % KB) + (size_t)(i0 + e / KB) * dd.lda) * sizeof(T));
+        offb[q] = (unsigned)(((e % KB) + (size_t)(j0 + e / KB) * dd.ldb) * sizeof(T));
+    }
+    const unsigned stepa = (unsigned)((TRANSA ? KB : (size_t)KB * dd.lda) * sizeof(T));
+    const unsigned stepb = (unsigned)(KB * sizeof(T));
     T ra[NLD], rb[NLD];
-    auto gload = [&](int k0) {
+    auto gload = [&](int sl) {
+        const int k0 = sl * KB;
+        const bool tail = k0 + KB > k;  // uniform: only the last slab
 #pragma unroll
         for (int q = 0; q < NLD; ++q) {
             const int e = tid + 256 * q;
-            if (!TRANSA) {
-                const int r = e % TILE, c = e / TILE;
-                const int gi = i0 + r, gk = k0 + c;
-                ra[q] = (gi < m && gk < k) ? A[gi + (size_t)gk * dd.lda] : (T)0;
-            } else {
-                const int c = e % KB, r = e / KB;
-                const int gi = i0 + r, gk = k0 + c;
-                ra[q] = (gi < m && gk < k) ? A[gk + (size_t)gi * dd.lda] : (T)0;
+            ra[q] = buf_load(ra_rsrc, offa[q] + sl * stepa, T());
+            rb[q] = buf_load(rb_rsrc, offb[q] + sl * stepb, T());
+            if (tail) {
+                // k-contiguous operands read the next rows of their column
+                // past k: zero them (A column-major reads zero from the bound)
+                if (k0 + e % KB >= k) {
+                    rb[q] = (T)0;
+                    if (TRANSA) ra[q] = (T)0;
+                }
             }
-            const int cb = e % KB, jb = e / KB;
-            const int gkb = k0 + cb, gj = j0 + jb;
-            rb[q] = (gkb < k && gj < n) ? B[gkb + (size_t)gj * dd.ldb] : (T)0;
         }
     };
     auto sstore = [&](int buf) {
 #pragma unroll
         for (int q = 0; q < NLD; ++q) {
             const int e = tid + 256 * q;
-            if (!TRANSA) As[buf][e / TILE][e % TILE] = ra[q];
-            else As[buf][e % KB][e / KB] = ra[q];
-            Bs[buf][e % KB][e / KB] = rb[q];
+            if (!TRANSA) As[buf][e % TILE][e / TILE] = ra[q];
+            else As[buf][e / KB][e % KB] = ra[q];
+            Bs[buf][e / KB][e % KB] = rb[q];
         }
     };
     typename C::acc_t acc[BLK][BLK];
@@ -716,27 +771,42 @@ __global__ void __launch_bounds__(256, 2) k_gemm_mfma(int nbox, const int* __res
         for (int c = 0; c < BLK; ++c)
 #pragma unroll
             for (int i = 0; i < C::NACC; ++i) acc[a][c][i] = (T)0;
-    const int nsl = (k + KB - 1) / KB;
-    gload(0);
-    sstore(0);
+    const int nsl = (k + KB - 1) / KB;  // k == 0: no loads, C = beta C
+    if (nsl > 0) {
+        gload(0);
+        sstore(0);
+    }
     __syncthreads();
+    const int g = lane / MT, rr = lane % MT;
     for (int sl = 0; sl < nsl; ++sl) {
         const int cur = sl & 1;
-        if (sl + 1 < nsl) gload((sl + 1) * KB);
+        if (sl + 1 < nsl) gload(sl + 1);
+        T bv[BLK][SPL];
 #pragma unroll
-        for (int kk = 0; kk < KB; kk += KT) {
-            const int kr = kk + lane / MT, rr = lane % MT;
-            T av[BLK], bv[BLK];
+        for (int c = 0; c < BLK; ++c) {
+            const T* src = &Bs[cur][wc + c * MT + rr][g * SPL];
 #pragma unroll
-            for (int a = 0; a < BLK; ++a) av[a] = As[cur][kr][wr + a * MT + rr];
-#pragma unroll
-            for (int c = 0; c < BLK; ++c) bv[c] = Bs[cur][kr][wc + c * MT + rr];
-#pragma unroll
-            for (int a = 0; a < BLK; ++a)
-#pragma unroll
-                for (int c = 0; c < BLK; ++c) acc[a][c] = C::mma(bv[c], av[a], acc[a][c]);
+            for (int s2 = 0; s2 < SPL; ++s2) bv[c][s2] = src[s2];
         }
-        if (sl + 1 < nsl) sstore(cur ^ 1);
+        // The next slab goes to the other LDS buffer halfway through the
+        // products (its loads have landed by then), so the slab ends with
+        // the barrier alone instead of a store-and-wait.
+#pragma unroll
+        for (int a = 0; a < BLK; ++a) {
+            if (a == BLK / 2 && sl + 1 < nsl) {
+                __builtin_amdgcn_sched_barrier(0);
+                sstore(cur ^ 1);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            T av[SPL];
+            const T* src = &As[cur][wr + a * MT + rr][g * SPL];
+#pragma unroll
+            for (int s2 = 0; s2 < SPL; ++s2) av[s2] = src[s2];
+#pragma unroll
+            for (int s2 = 0; s2 < SPL; ++s2)
+#pragma unroll
+                for (int c = 0; c < BLK; ++c) acc[a][c] = C::mma(bv[c][s2], av[s2], acc[a][c]);
+        }
         __syncthreads();
     }
 #pragma unroll
@@ -1182,6 +1252,23 @@ __global__ void k_pack_boxes(int nbox, const long long* __restrict__ dst_off,
     }
 }
 
+// G(i, j) = sum_s P_s(i, j) for i >= j; grid (chunks of 256 columns' worth
+// of elements, descriptors)
+__global__ void __launch_bounds__(256) k_sum_slices_lower(const SliceSum* __restrict__ desc,
+                                                          double* G) {
+    const SliceSum d = desc[blockIdx.y];
+    const long long nn = (long long)d.n * d.n;
+    for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < nn;
+         e += (long long)gridDim.x * 256) {
+        const int i = (int)(e % d.n), j = (int)(e / d.n);
+        if (i < j) continue;
+        const double* src = G + d.src + e;
+        double acc = src[0];
+        for (int q = 1; q < d.slices; ++q) acc += src[q * nn];
+        G[d.dst + e] = acc;
+    }
+}
+
 int grid_for(long long n, int block = 256, int cap = 1 << 16) {
     long long g = (n + block - 1) / block;
     if (g < 1) g = 1;
@@ -1405,7 +1492,13 @@ std::pair<int, int> GemmPlan::add(const std::vector<GemmDesc>& d, bool transA) {
             // short products over small outputs (the in-window updates of the
             // block inverse) finish sooner on the FMA kernel's 64 x 64 tiles
             const bool small = g.k <= 64 && (long long)g.m * g.n <= 256LL * 256;
-            const bool large = g.m >= TILE / 2 && g.n >= TILE / 2 && !small;
+            // the MFMA kernel addresses each operand with 32-bit byte offsets:
+            // keep operand extents (8-byte elements, plus a slab and a tile of
+            // overrun) under 2 GiB
+            const double ext_a = (double)g.lda * (transA ? g.m : g.k) + 16.0 * g.lda + 256;
+            const double ext_b = (double)g.ldb * g.n + 16.0 * g.ldb + 256;
+            const bool fits = 8.0 * std::max(ext_a, ext_b) < 2147483648.0;
+            const bool large = g.m >= TILE / 2 && g.n >= TILE / 2 && !small && fits;
             if (large != (pass == 1)) continue;
             const int tm = (g.m + tile - 1) / tile, tn = (g.n + tile - 1) / tile;
             desc.push_back(g);
@@ -1434,13 +1527,16 @@ void gemm_run(const GemmPlan& p, std::pair<int, int> h, const T* d_A, const T* d
         const GemmPlan::Launch& L = p.launches[li];
         const GemmDesc* dd = p.d_desc.p + L.d0;
         const int* tt = p.d_tiles.p + L.t0;
-        if (kt) kt->begin(s);
         char name[96];
         const char* tn = type_name<T>();
         const char* cn = type_name<TC>();
         const char* ta = L.transA ? "true" : "false";
+        if (kt) {
+            std::snprintf(name, sizeof name, "%s<%s, %s, %s>", L.mfma ? "k_gemm_mfma" : "k_gemm",
+                          tn, ta, cn);
+            kt->begin(s, name);
+        }
         if (!L.mfma) {
-            std::snprintf(name, sizeof name, "k_gemm<%s, %s, %s>", tn, ta, cn);
             if (L.transA)
                 k_gemm<T, true, TC><<<L.total, 256, 0, s>>>(L.nb, tt, dd, d_A, d_B, d_C, alpha,
                                                             beta, d_flags);
@@ -1448,7 +1544,6 @@ void gemm_run(const GemmPlan& p, std::pair<int, int> h, const T* d_A, const T* d
                 k_gemm<T, false, TC><<<L.total, 256, 0, s>>>(L.nb, tt, dd, d_A, d_B, d_C, alpha,
                                                              beta, d_flags);
         } else {
-            std::snprintf(name, sizeof name, "k_gemm_mfma<%s, %s, %s>", tn, ta, cn);
             if (L.transA)
                 k_gemm_mfma<T, true, TC><<<L.total, 256, 0, s>>>(L.nb, tt, dd, d_A, d_B, d_C,
                                                                  alpha, beta, d_flags);
@@ -1458,7 +1553,7 @@ void gemm_run(const GemmPlan& p, std::pair<int, int> h, const T* d_A, const T* d
         }
         HIFDE_LAUNCH_CHECK();
         if (kt)
-            kt->end(s, name, L.flops,
+            kt->end(s, L.flops,
                     L.ab_elems * sizeof(T) + L.cells * sizeof(TC) * (beta == (T)0 ? 1 : 2));
     }
 }
@@ -1570,6 +1665,17 @@ void gram_pivchol(int nface, const int* h_n, const unsigned char* h_mode, const 
             if (ws.host_active[0] == 0) break;
         }
     }
+}
+
+void sum_slices_lower(const std::vector<SliceSum>& desc, DevBuf<SliceSum>& d_desc, double* d_G,
+                      hipStream_t s) {
+    if (desc.empty()) return;
+    long long max_nn = 0;
+    for (const SliceSum& d : desc) max_nn = std::max(max_nn, (long long)d.n * d.n);
+    d_desc.upload(desc, s);
+    const dim3 grid((unsigned)std::min<long long>((max_nn + 255) / 256, 1024), (unsigned)desc.size());
+    k_sum_slices_lower<<<grid, 256, 0, s>>>(d_desc.p, d_G);
+    HIFDE_LAUNCH_CHECK();
 }
 
 void gram_w(const std::vector<GramRDesc>& desc, const std::vector<long long>& toff,

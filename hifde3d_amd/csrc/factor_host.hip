@@ -139,6 +139,7 @@ struct hifde_factor {
         double flops, bytes;
     };
     bool profiling = false;
+    std::string profile_only;  // non-empty: time only this kernel's launches
     std::vector<std::string> fam_names;
     std::vector<KStat> fam_stats;
     std::vector<PendingRec> pending;
@@ -156,6 +157,8 @@ struct hifde_factor {
     DevBuf<int> ws_gm, ws_gk;
     DevBuf<long long> ws_goffM, ws_goffC;
     DevBuf<double> ws_G;
+    DevBuf<SliceSum> d_gsum;
+    static constexpr long long kGramSplitTiles = 1536;  // three waves of 512 workgroups
     int gram_min_n = 16;          // faces with more columns use the Gram path
     double gram_min_eps = 1e-7;   // below this tolerance every face uses Householder
     DevBuf<double> pcg_buf, pcg_part, pcg_scalar, apply_tmp, apply_side;
@@ -228,7 +231,7 @@ struct hifde_factor {
         double flops, bytes;
         Prof(hifde_factor* f_, const std::string& name, double fl, double by)
             : f(f_), flops(fl), bytes(by) {
-            if (!f->profiling) return;
+            if (!f->profiling || !f->profile_only.empty()) return;
             fam = f->family(name);
             e0 = f->take_event();
             HIFDE_CHECK(hipEventRecord(e0, f->stream));
@@ -241,18 +244,26 @@ struct hifde_factor {
         }
     };
     // Per-launch kernel timing (families named "kernel:<name>").
+    // With profile_only set, only launches of that kernel are timed and the
+    // family records are off, so the timing itself costs little.
     struct LaunchTimer : KernelTimer {
         hifde_factor* f;
         hipEvent_t e0 = nullptr;
+        int fam = -1;
         explicit LaunchTimer(hifde_factor* f_) : f(f_) {}
-        void begin(hipStream_t s) override {
+        void begin(hipStream_t s, const char* kernel) override {
+            fam = -1;
+            if (!f->profile_only.empty() && f->profile_only != kernel) return;
+            fam = f->family(std::string("kernel:") + kernel);
             e0 = f->take_event();
             HIFDE_CHECK(hipEventRecord(e0, s));
         }
-        void end(hipStream_t s, const char* kernel, double flops, double bytes) override {
+        void end(hipStream_t s, double flops, double bytes) override {
+            if (fam < 0) return;
             hipEvent_t e1 = f->take_event();
             HIFDE_CHECK(hipEventRecord(e1, s));
-            f->pending.push_back({f->family(std::string("kernel:") + kernel), e0, e1, flops, bytes});
+            f->pending.push_back({fam, e0, e1, flops, bytes});
+            fam = -1;
         }
     };
     LaunchTimer launch_timer{this};
@@ -1033,20 +1044,48 @@ struct hifde_factor {
             ws_gk.upload(gk, stream);
             ws_goffM.upload(goffM, stream);
             ws_goffC.upload(goffC, stream);
-            ws_G.reserve(offC.back());
-            ws_T.reserve(offC.back());
-            {
-                // G = M^T M is symmetric: only the tiles on or below the diagonal
-                std::vector<GemmDesc> gd;
-                for (int t = 0; t < ng; ++t) {
-                    GemmDesc g{goffM[t], goffM[t], goffC[t], gm[t], gn[t], gk[t],
-                               std::max(gk[t], 1), std::max(gk[t], 1), std::max(gm[t], 1)};
+            // G = M^T M is symmetric: only the tiles on or below the diagonal.
+            // A few large faces (the top levels) give too few output tiles
+            // to fill the GPU over their long k: their k is then split into
+            // slices of at least 256 rows, each product written to its own
+            // partial block after the level's G, and the partials summed in
+            // slice order (deterministic).
+            long long lower_tiles = 0;
+            for (int t = 0; t < ng; ++t) {
+                const long long tm = (gm[t] + kMfmaTile - 1) / kMfmaTile;
+                lower_tiles += tm * (tm + 1) / 2;
+            }
+            const int split = lower_tiles >= kGramSplitTiles
+                                  ? 1
+                                  : (int)std::min<long long>(
+                                        8, (kGramSplitTiles + lower_tiles - 1) / lower_tiles);
+            std::vector<GemmDesc> gd;
+            std::vector<SliceSum> sums;
+            long long part = offC.back();  // partial blocks follow the level's G
+            for (int t = 0; t < ng; ++t) {
+                const int sf = std::min(split, std::max(1, gk[t] / 256));
+                const long long nn = (long long)gn[t] * gn[t];
+                for (int q = 0; q < sf; ++q) {
+                    const int k0 = (int)((long long)gk[t] * q / sf);
+                    const int k1 = (int)((long long)gk[t] * (q + 1) / sf);
+                    GemmDesc g{goffM[t] + k0, goffM[t] + k0, sf == 1 ? goffC[t] : part + q * nn,
+                               gm[t], gn[t], k1 - k0, std::max(gk[t], 1), std::max(gk[t], 1),
+                               std::max(gm[t], 1)};
                     g.lower = 1;
                     gd.push_back(g);
                 }
+                if (sf > 1) {
+                    sums.push_back({goffC[t], part, gn[t], sf});
+                    part += sf * nn;
+                }
+            }
+            ws_G.reserve(part);
+            ws_T.reserve(offC.back());
+            {
                 Prof pr(this, "gram_gemm", fl_g / 2, by_g);
                 batched_gemm_desc<double>(true, gd, ws_M.p, ws_M.p, ws_G.p, 1.0, 0.0, gemm_ws,
                                           stream);
+                sum_slices_lower(sums, d_gsum, ws_G.p, stream);
             }
             {
                 Prof pr(this, "gram_pivchol", fl_c, 8.0 * offC.back());
@@ -2184,6 +2223,12 @@ int hifde_set_option(hifde_factor* f, const char* name, double value) {
 int hifde_set_profiling(hifde_factor* f, int on) {
     if (!f) return HIFDE_ERR_ARG;
     f->profiling = on != 0;
+    return HIFDE_OK;
+}
+
+int hifde_set_profile_kernel(hifde_factor* f, const char* kernel) {
+    if (!f) return HIFDE_ERR_ARG;
+    f->profile_only = kernel ? kernel : "";
     return HIFDE_OK;
 }
 

@@ -56,8 +56,9 @@ inline void upload_bytes(void* dst, const void* src, size_t bytes, hipStream_t s
 // bytes.  Active while a profiled factorization or solve runs.
 struct KernelTimer {
     virtual ~KernelTimer() = default;
-    virtual void begin(hipStream_t s) = 0;
-    virtual void end(hipStream_t s, const char* kernel, double flops, double bytes) = 0;
+    // brackets one launch of `kernel` on stream s
+    virtual void begin(hipStream_t s, const char* kernel) = 0;
+    virtual void end(hipStream_t s, double flops, double bytes) = 0;
 };
 KernelTimer* active_timer();
 void set_active_timer(KernelTimer* t);
@@ -109,6 +110,9 @@ struct LevelTable {
     size_t cap = 0;    // power of two
     bool sym = false;  // one entry per unordered pair, keyed (min, max)
 };
+
+// Output tile of one MFMA GEMM workgroup (128 x 128).
+constexpr int kMfmaTile = 128;
 
 struct GemmDesc {
     long long offA, offB, offC;
@@ -332,6 +336,14 @@ void gram_pivchol(int nface, const int* h_n, const unsigned char* h_mode, const 
                   const long long* d_offG, double* d_R, double eps, int* d_perm,
                   const int* h_perm_off, const int* d_perm_off, int* d_k, GramWork& ws,
                   GemmWork& gws, hipStream_t s);
+// Lower triangle of an n x n block as the sum of `slices` partial blocks
+// (split-k Gram products), added in slice order.
+struct SliceSum {
+    long long dst, src;  // block offsets; slice s at src + s n^2
+    int n, slices;
+};
+void sum_slices_lower(const std::vector<SliceSum>& desc, DevBuf<SliceSum>& d_desc, double* d_G,
+                      hipStream_t s);
 void gram_w(const std::vector<GramRDesc>& desc, const std::vector<long long>& toff,
             const double* d_R, const int* d_perm, const std::vector<long long>& offW,
             double* d_W, GramWork& ws, GJWork& gj, GemmWork& gws, hipStream_t s);

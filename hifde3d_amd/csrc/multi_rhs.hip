@@ -265,7 +265,8 @@ void apply_multi_fused(int nbox, int ntiles, const int* d_tile_off, const int* d
                        double bytes, hipStream_t s) {
     if (nbox <= 0 || ntiles <= 0 || R <= 0) return;
     KernelTimer* kt = active_timer();
-    if (kt) kt->begin(s);
+    if (kt)
+        kt->begin(s, sizeof(T) == sizeof(float) ? "k_apply_multi<float>" : "k_apply_multi<double>");
     const int cpt = R <= 4 ? 1 : R <= 8 ? 2 : R <= 16 ? 4 : 8;
     const dim3 grid(ntiles, (R + 4 * cpt - 1) / (4 * cpt));
 #define HIFDE_AM(CPT)                                                                           \
@@ -278,9 +279,7 @@ void apply_multi_fused(int nbox, int ntiles, const int* d_tile_off, const int* d
     else HIFDE_AM(8);
 #undef HIFDE_AM
     HIFDE_LAUNCH_CHECK();
-    if (kt)
-        kt->end(s, sizeof(T) == sizeof(float) ? "k_apply_multi<float>" : "k_apply_multi<double>",
-                flops, bytes);
+    if (kt) kt->end(s, flops, bytes);
 }
 
 void rhs_side_add(long long nentries, const int* d_gid, const unsigned char* d_sp,

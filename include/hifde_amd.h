@@ -152,6 +152,11 @@ int hifde_set_option(hifde_factor* f, const char* name, double value);
  * families; for 0 <= idx < that number it fills name and
  * out[0..3] = launches, seconds, algorithmic flops, algorithmic bytes. */
 int hifde_set_profiling(hifde_factor* f, int on);
+/* While profiling, time only the launches of the kernel with this name (as
+ * listed by hifde_kernel_stats without the "kernel:" prefix) and record no
+ * family totals; NULL or "" restores full profiling.  Keeps the event
+ * overhead of a timed run to that one kernel. */
+int hifde_set_profile_kernel(hifde_factor* f, const char* kernel);
 int hifde_reset_kernel_stats(hifde_factor* f);
 int hifde_kernel_stats(const hifde_factor* f, int idx, char* name, int name_len, double* out);
 /* Bytes of factor data held on the device. */
