@@ -190,6 +190,7 @@ struct hifde_factor {
     DevBuf<int> d_emap, d_spos, d_etab, d_egid;
 
     ~hifde_factor() {
+        clear_graphs();
         for (PendingRec& r : pending) {
             (void)hipEventDestroy(r.e0);
             (void)hipEventDestroy(r.e1);
@@ -254,6 +255,9 @@ struct hifde_factor {
         void begin(hipStream_t s, const char* kernel) override {
             fam = -1;
             if (!f->profile_only.empty() && f->profile_only != kernel) return;
+            hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+            HIFDE_CHECK(hipStreamIsCapturing(s, &cs));
+            if (cs != hipStreamCaptureStatusNone) return;  // not timed inside a graph
             fam = f->family(std::string("kernel:") + kernel);
             e0 = f->take_event();
             HIFDE_CHECK(hipEventRecord(e0, s));
@@ -1255,6 +1259,7 @@ struct hifde_factor {
 
     void run_factor() {
         if (!have_matrix) throw Error(HIFDE_ERR_STATE, "factor: no matrix set");
+        clear_graphs();
         for (double& t : stage_t) t = 0.0;
         stage_lt.assign((size_t)L * ST_COUNT, 0.0);
         lu_info = 0;
@@ -1451,13 +1456,94 @@ struct hifde_factor {
     void apply_dev(double* d_u, int nrhs, long long ldu, bool inv) {
         TimerGuard tguard(this);
         if (!factored) throw Error(HIFDE_ERR_STATE, "apply: factor first");
-        if (nrhs > 1) {
-            apply_multi(d_u, nrhs, ldu, inv);
-        } else if (nrhs == 1) {
+        if (nrhs >= 1) apply_sweeps(d_u, nrhs, ldu, inv);
+        resolve_profile();
+    }
+
+    // ------------------------------------------------------------------
+    // The apply sweeps are a fixed sequence of some 40-100 launches for a
+    // given factorization, vector and right-hand-side count.  The first call
+    // with a given (vector, count, leading dimension, direction) runs
+    // eagerly; it also sizes every workspace and builds every descriptor
+    // table.  The second captures the sequence into a HIP graph, and later
+    // calls replay it in one launch.  Graphs are dropped when the factor
+    // changes.  Under full profiling (per-family events) the sweeps always
+    // run eagerly; in single-kernel timing mode launches inside a capture
+    // are not timed.
+    struct ApplyGraph {
+        double* u;
+        int R;
+        long long ld;
+        bool inv;
+        unsigned long long gen;  // apply_gen of the eager run that prepared it
+        hipGraphExec_t exec;
+    };
+    std::vector<ApplyGraph> graphs;
+    // Bumped whenever something a recorded sweep depends on changes: a
+    // workspace moves or a multi-RHS descriptor table is rebuilt.  A graph is
+    // captured and replayed only while it matches its eager run's value.
+    unsigned long long apply_gen = 0;
+    bool graph_option = true;
+    static constexpr size_t kMaxGraphs = 8;
+    void clear_graphs() {
+        for (ApplyGraph& g : graphs)
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        graphs.clear();
+    }
+    void apply_eager(double* d_u, int R, long long ld, bool inv) {
+        if (R > 1) {
+            apply_multi(d_u, R, ld, inv);
+        } else {
             apply_tmp.reserve((size_t)NC * NC * NC);
             apply_one(d_u, inv);
         }
-        resolve_profile();
+    }
+    void apply_sweeps(double* d_u, int R, long long ld, bool inv) {
+        const bool can_graph = graph_option && (!profiling || !profile_only.empty());
+        if (!can_graph) {
+            apply_eager(d_u, R, ld, inv);
+            return;
+        }
+        ApplyGraph* g = nullptr;
+        for (ApplyGraph& e : graphs)
+            if (e.u == d_u && e.R == R && e.ld == ld && e.inv == inv) g = &e;
+        if (!g || g->gen != apply_gen) {
+            // first use, or the state changed since: eager, sizes the
+            // workspaces and builds the descriptor tables
+            const void* before[5] = {apply_tmp.p, apply_side.p, rhs_in.p, rhs_out.p, rhs_side.p};
+            apply_eager(d_u, R, ld, inv);
+            const void* after[5] = {apply_tmp.p, apply_side.p, rhs_in.p, rhs_out.p, rhs_side.p};
+            for (int i = 0; i < 5; ++i)
+                if (before[i] != after[i]) ++apply_gen;
+            if (!g) {
+                if (graphs.size() == kMaxGraphs) {
+                    if (graphs.front().exec) (void)hipGraphExecDestroy(graphs.front().exec);
+                    graphs.erase(graphs.begin());
+                }
+                graphs.push_back({d_u, R, ld, inv, apply_gen, nullptr});
+            } else {
+                if (g->exec) (void)hipGraphExecDestroy(g->exec);
+                g->exec = nullptr;
+                g->gen = apply_gen;
+            }
+            return;
+        }
+        if (!g->exec) {
+            hipGraph_t graph = nullptr;
+            HIFDE_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            try {
+                apply_eager(d_u, R, ld, inv);
+            } catch (...) {
+                (void)hipStreamEndCapture(stream, &graph);
+                if (graph) (void)hipGraphDestroy(graph);
+                throw;
+            }
+            HIFDE_CHECK(hipStreamEndCapture(stream, &graph));
+            const hipError_t e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            HIFDE_CHECK(e);
+        }
+        HIFDE_CHECK(hipGraphLaunch(g->exec, stream));
     }
 
     // ------------------------------------------------------------------
@@ -1518,6 +1604,7 @@ struct hifde_factor {
         }
         GemmPlan& pl = B.rplan[op];
         if (B.rplan_R[op] != R) {
+            ++apply_gen;  // recorded sweeps read this table
             pl.clear();
             std::vector<GemmDesc> d;
             d.reserve(B.nbox);
@@ -1685,14 +1772,7 @@ struct hifde_factor {
     // step lengths are zero) while the others continue.
     DevBuf<double> mpcg_buf, mpcg_part, mpcg_scal;
     // F^{-1} on R columns (the single-vector sweeps for one column)
-    void apply_cols(double* Z, int R, long long ld) {
-        if (R == 1) {
-            apply_tmp.reserve((size_t)NC * NC * NC);
-            apply_one(Z, true);
-        } else {
-            apply_multi(Z, R, ld, true);
-        }
-    }
+    void apply_cols(double* Z, int R, long long ld) { apply_sweeps(Z, R, ld, true); }
     void pcg_multi_dev(const double* B, long long ldb, double* X, long long ldx, int R,
                        double tol, int maxit, int* iters, double* relres) {
         TimerGuard tguard(this);
@@ -2214,7 +2294,13 @@ int hifde_set_option(hifde_factor* f, const char* name, double value) {
         else if (n == "symmetric") f->sym_option = (int)value;
         else if (n == "elements") f->elem_option = (int)value;
         else if (n == "stage_sync") f->stage_sync = value != 0;
-        else if (n == "multi_fused_max") f->multi_fused_max = (int)value;
+        else if (n == "multi_fused_max") {
+            f->multi_fused_max = (int)value;
+            f->clear_graphs();
+        } else if (n == "graphs") {
+            f->graph_option = value != 0;
+            f->clear_graphs();
+        }
         else throw Error(HIFDE_ERR_ARG, "set_option: unknown option " + n);
         return HIFDE_OK;
     });

@@ -523,3 +523,51 @@ def test_pcg_multi_columns_converge_independently(H, oracle_mod, precision):
         assert abs(int(its[c]) - it1) <= 1
         assert _rel(X[:, c], x1) < 1e-8
     assert bool((dX[:, n:] == 0).all())
+
+
+# --------------------------------------------------------------------------
+# HIP-graph replay of the apply sweeps
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_apply_graph_replays_match_eager(H, oracle_mod, precision):
+    """Repeated applies on one buffer run eagerly once, are then captured
+    into a HIP graph and replayed; every replay, with eager work between
+    them and an interleaved apply of another shape that rebuilds the
+    multi-RHS tables, must reproduce the eager result bit for bit."""
+    import torch
+    f, _, _ = _both(H, oracle_mod, 31, 4, 1e-4, "pow10", True, precision)
+    n = 32 ** 3
+    for R in (1, 4):
+        U0 = torch.from_numpy(np.stack([pb.rhs_normal(32, 90 + c) for c in range(R)])).cuda()
+        f.set_option("graphs", 0)
+        ref = U0.clone()
+        f.apply_device(ref.data_ptr(), R, n, True)
+        torch.cuda.synchronize()
+        f.set_option("graphs", 1)
+        d = U0.clone()
+        other = torch.from_numpy(np.stack([pb.rhs_normal(32, 7 + c) for c in range(3)])).cuda()
+        for rep in range(5):
+            d.copy_(U0)
+            f.apply_device(d.data_ptr(), R, n, True)
+            torch.cuda.synchronize()
+            assert torch.equal(d, ref), (R, rep)
+            if rep == 2:  # another shape in between: rebuilds tables, then replays recapture
+                o = other.clone()
+                f.apply_device(o.data_ptr(), 3, n, True)
+                torch.cuda.synchronize()
+
+
+def test_pcg_with_graph_replays_matches_eager(H, oracle_mod):
+    """PCG replays its preconditioner from a graph after the first two
+    applies; the iterates must equal the eager run's bit for bit."""
+    import torch
+    f, _, _ = _both(H, oracle_mod, 31, 4, 1e-4, "pow10", True, "mixed")
+    b = torch.from_numpy(pb.rhs_normal(32, 3)).cuda()
+    out = {}
+    for graphs in (0, 1):
+        f.set_option("graphs", graphs)
+        x = torch.zeros_like(b)
+        it, rr = f.pcg_device(b.data_ptr(), x.data_ptr(), tol=1e-10, maxit=200)
+        out[graphs] = (it, rr, x.clone())
+    assert out[0][0] == out[1][0] and out[0][1] == out[1][1]
+    assert torch.equal(out[0][2], out[1][2])
