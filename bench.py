@@ -164,6 +164,67 @@ def cpu_baseline(n_sample, n_target, seed, corr, eps, P, tol):
             "factor_s_scaled": t_f * scale, "solve_s_scaled": t_s * scale}
 
 
+def batch_workload(args):
+    """48 independent problems, NC drawn from {24,...,96} (seed 3), a = 10^s with
+    s ~ U[-2, 2], 16 right-hand sides each: mixed-precision factor and a block
+    of 16 PCG solves to 1e-10 per problem, one after another on the GPU.  Each
+    problem's operator is assembled and its right-hand sides are resident in
+    HBM before the timed region; a step is one pass over all 48 problems."""
+    import torch
+    import hifde3d_amd as H
+    from hifde3d_amd import problems as pb
+
+    inst = pb.batch_instances()
+    facs, rhs, xs = [], [], []
+    for p in inst:
+        f = H.HIFFactor(p["NC"] - 1, p["P"], 1e-4, precision="mixed")
+        f.set_stencil(pb.pow10_coeff(p["NC"], p["coeff_seed"], -2.0, 2.0), pb.zeros_coeff(p["NC"]),
+                      symmetric=True)
+        facs.append(f)
+        rhs.append(torch.from_numpy(pb.rhs_normal(p["NC"], p["rhs_seed"], 16)).cuda())
+        xs.append(torch.zeros_like(rhs[-1]))
+    torch.cuda.synchronize()
+
+    def one_pass():
+        worst, its = 0.0, []
+        for f, b, x in zip(facs, rhs, xs):
+            f.Factor()
+            x.zero_()
+            it, rr = f.pcg_multi_device(b.data_ptr(), x.data_ptr(), 16, tol=args.tol, maxit=500)
+            worst = max(worst, float(rr.max()))
+            its.append(int(it.max()))
+        torch.cuda.synchronize()
+        return worst, its
+
+    for w in range(args.warmup):
+        t0 = time.perf_counter()
+        worst, its = one_pass()
+        log(f"[bench] batch warmup {w}: {time.perf_counter() - t0:.3f}s worst relres {worst:.2e}")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        worst, its = one_pass()
+        log(f"[bench] batch step {k}: worst relres {worst:.2e} max iters {max(its)}")
+    elapsed = time.perf_counter() - t0
+    dofs = sum((p["NC"] - 1) ** 3 for p in inst)
+    out = {
+        "metric": "batched uneven instances: factor + 16-RHS PCG to 1e-10, problems/s",
+        "value": len(inst) * args.steps / elapsed, "unit": "problems/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": ("48 problems, NC from {24,32,40,48,64,80,96} (seed 3), P so that "
+                                "NC/P is a power of two, a = 10^U[-2,2], eps=1e-4, 16 RHS "
+                                "N(0,1) each, mixed factor + block PCG to 1e-10"),
+                   "problems": len(inst), "nrhs": 16, "tol": args.tol},
+        "unknowns": dofs, "rhs_per_s": 16 * len(inst) * args.steps / elapsed,
+        "worst_relres": worst, "max_pcg_iters": max(its),
+    }
+    for f in facs:
+        f.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -177,6 +238,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=32)
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--workload", choices=("n128", "batch"), default="n128",
+                    help="n128: the headline factor + PCG at n=128 (default); batch: the 48 "
+                         "independent uneven problems with 16 right-hand sides each")
     ap.add_argument("--pmc-json", nargs=4, metavar=("FETCH", "WRITE", "CALFETCH", "CALWRITE"),
                     help="write the per-kernel HBM traffic summary of four rocprofv3 --pmc "
                          "counter_collection.csv files to profiles/pmc_traffic_n128.json")
@@ -193,6 +257,9 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
+    if args.workload == "batch":
+        batch_workload(args)
+        return
 
     import torch
     import hifde3d_amd as H

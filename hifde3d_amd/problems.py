@@ -107,3 +107,22 @@ def rhs_normal(NC, seed, nrhs=1):
 def reference_test_vector(NC):
     """v = 1 on the interior, 0 on the boundary (tests/HIFDE3D/main.cpp:88-99)."""
     return interior_mask(NC).astype(np.float64)
+
+
+# Batched uneven instances: grid sizes drawn uniformly from C4_SIZES, each with
+# the leaf width that makes NC/P a power of two, a = 10^s with s ~ U[-2, 2],
+# 16 right-hand sides N(0, 1).
+C4_SIZES = (24, 32, 40, 48, 64, 80, 96)
+C4_LEAF = {24: 3, 48: 3, 96: 3, 32: 4, 64: 4, 40: 5, 80: 5}
+
+
+def batch_instances(seed=3, count=48):
+    """The independent problems of the batched workload: per problem NC, P and
+    the seeds of its coefficient (s ~ U[-2, 2]) and right-hand sides."""
+    u = uniform(seed, count)
+    out = []
+    for i in range(count):
+        NC = C4_SIZES[min(int(u[i] * len(C4_SIZES)), len(C4_SIZES) - 1)]
+        out.append({"NC": NC, "P": C4_LEAF[NC], "coeff_seed": seed * 1000 + 2 * i,
+                    "rhs_seed": seed * 1000 + 2 * i + 1})
+    return out
