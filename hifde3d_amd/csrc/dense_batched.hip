@@ -724,13 +724,18 @@ __global__ void __launch_bounds__(256, 2) k_gemm_mfma(int nbox, const int* __res
     const int wr = (wave & 1) * WT, wc = (wave >> 1) * WT;
     // Per-thread load slots.  k-contiguous operands (B, and A when TRANSA):
     // 16 lanes cover the 16 k of one row, e = tid + 256 q -> (k e % 16,
-    // row e / 16).  A column-major with rows contiguous: (row e % 128,
-    // k e / 128).
+    // row e / 16).  A column-major with rows contiguous: a wave's load covers
+    // 16 rows x 4 k (four 128-byte column segments), slot q of wave w and
+    // lane l holding (row 16 q' + l % 16, k 4 w' + l / 16) with (q', w') the
+    // slot's (row block, k block); that shape also lands each half-wave's LDS
+    // write on 64 distinct banks.
+    auto a_row = [&](int q) { return 16 * ((q * 4 + wave) % (TILE / 16)) + (lane & 15); };
+    auto a_k = [&](int q) { return 4 * ((q * 4 + wave) / (TILE / 16)) + (lane >> 4); };
     unsigned offa[NLD], offb[NLD];
 #pragma unroll
     for (int q = 0; q < NLD; ++q) {
         const int e = tid + 256 * q;
-        if (!TRANSA) offa[q] = (unsigned)(((i0 + e % TILE) + (size_t)(e / TILE) * dd.lda) * sizeof(T));
+        if (!TRANSA) offa[q] = (unsigned)(((i0 + a_row(q)) + (size_t)a_k(q) * dd.lda) * sizeof(T));
         else offa[q] = (unsigned)(((e % KB) + (size_t)(i0 + e / KB) * dd.lda) * sizeof(T));
         offb[q] = (unsigned)(((e % KB) + (size_t)(j0 + e / KB) * dd.ldb) * sizeof(T));
     }
@@ -759,7 +764,7 @@ __global__ void __launch_bounds__(256, 2) k_gemm_mfma(int nbox, const int* __res
 #pragma unroll
         for (int q = 0; q < NLD; ++q) {
             const int e = tid + 256 * q;
-            if (!TRANSA) As[buf][e % TILE][e / TILE] = ra[q];
+            if (!TRANSA) As[buf][a_row(q)][a_k(q)] = ra[q];
             else As[buf][e / KB][e % KB] = ra[q];
             Bs[buf][e / KB][e % KB] = rb[q];
         }
