@@ -37,7 +37,7 @@ EXPORTS = [
     "hifde_stage_name", "hifde_factor_bytes", "hifde_interp_decomp_batch", "hifde_schur_batch",
     "hifde_set_profiling", "hifde_set_profile_kernel", "hifde_reset_kernel_stats",
     "hifde_kernel_stats", "hifde_set_option",
-    "hifde_gemm_batch",
+    "hifde_gemm_batch", "hifde_factor_save", "hifde_factor_load",
 ]
 
 
@@ -89,6 +89,8 @@ def lib():
     L.hifde_factor_bytes.restype = _ll
     L.hifde_set_profiling.argtypes = [_vp, _i]
     L.hifde_set_profile_kernel.argtypes = [_vp, ctypes.c_char_p]
+    L.hifde_factor_save.argtypes = [_vp, ctypes.c_char_p]
+    L.hifde_factor_load.argtypes = [_vp, ctypes.c_char_p]
     L.hifde_set_option.argtypes = [_vp, ctypes.c_char_p, _d]
     L.hifde_reset_kernel_stats.argtypes = [_vp]
     L.hifde_kernel_stats.argtypes = [_vp, _i, ctypes.c_char_p, _i, _dp]
@@ -215,6 +217,16 @@ class HIFFactor:
             ctypes.c_void_p(x_ptr), ldx if ldx is not None else self.size, nrhs, tol, maxit,
             it.ctypes.data_as(_ip), rr.ctypes.data_as(_dp)))
         return it[:nrhs].copy(), rr[:nrhs].copy()
+
+    def save(self, path):
+        """Write the factorization (operator, box index, factor blocks) to path."""
+        _check(lib().hifde_factor_save(self._h, os.fsencode(path)))
+
+    def load(self, path):
+        """Read a factorization saved by an object of the same N, P, epsilon and
+        precision; apply and solve then work without factoring."""
+        self.lu_info = _check(lib().hifde_factor_load(self._h, os.fsencode(path)))
+        return self.lu_info
 
     # --- introspection ------------------------------------------------------
     def level_counts(self):

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -1885,6 +1886,171 @@ struct hifde_factor {
     }
 
     // ------------------------------------------------------------------
+    // Factor persistence: the operator, every box's index and its factor
+    // blocks, so that a later process can apply and solve without factoring.
+    // Binary, native byte order; the header names the grid, tolerance and
+    // precision, and a load checks them against the object's own.
+    static constexpr char kFileMagic[8] = {'H', 'I', 'F', 'D', 'E', 'F', '0', '1'};
+    static constexpr char kFileEnd[8] = {'H', 'I', 'F', 'D', 'E', 'E', 'N', 'D'};
+    struct File {
+        FILE* fp = nullptr;
+        bool writing = false;
+        File(const char* path, bool w) : writing(w) {
+            fp = std::fopen(path, w ? "wb" : "rb");
+            if (!fp) throw Error(HIFDE_ERR_ARG, std::string("cannot open ") + path);
+        }
+        ~File() {
+            if (fp) std::fclose(fp);
+        }
+        void put(const void* p, size_t n) {
+            if (n && std::fwrite(p, 1, n, fp) != n) throw Error(HIFDE_ERR_STATE, "write failed");
+        }
+        void get(void* p, size_t n) {
+            if (n && std::fread(p, 1, n, fp) != n)
+                throw Error(HIFDE_ERR_STATE, "factor file truncated or unreadable");
+        }
+        template <class V>
+        void put_vec(const std::vector<V>& v) {
+            const long long n = (long long)v.size();
+            put(&n, sizeof n);
+            put(v.data(), v.size() * sizeof(V));
+        }
+        template <class V>
+        void get_vec(std::vector<V>& v, long long max_n) {
+            long long n = 0;
+            get(&n, sizeof n);
+            if (n < 0 || n > max_n) throw Error(HIFDE_ERR_STATE, "factor file corrupt");
+            v.resize((size_t)n);
+            get(v.data(), (size_t)n * sizeof(V));
+        }
+    };
+    // Device bytes to the file through a bounded host buffer.
+    void put_dev(File& fl, const unsigned char* d, long long bytes) {
+        std::vector<unsigned char> h((size_t)std::min<long long>(bytes, 64LL << 20));
+        for (long long o = 0; o < bytes; o += (long long)h.size()) {
+            const size_t n = (size_t)std::min<long long>((long long)h.size(), bytes - o);
+            HIFDE_CHECK(hipMemcpy(h.data(), d + o, n, hipMemcpyDeviceToHost));
+            fl.put(h.data(), n);
+        }
+    }
+    void get_dev(File& fl, DevBuf<unsigned char>& d, long long bytes) {
+        d.reserve((size_t)std::max<long long>(bytes, 1));
+        std::vector<unsigned char> h((size_t)std::min<long long>(std::max<long long>(bytes, 1), 64LL << 20));
+        for (long long o = 0; o < bytes; o += (long long)h.size()) {
+            const size_t n = (size_t)std::min<long long>((long long)h.size(), bytes - o);
+            fl.get(h.data(), n);
+            HIFDE_CHECK(hipMemcpy(d.p + o, h.data(), n, hipMemcpyHostToDevice));
+        }
+    }
+
+    void save(const char* path) {
+        if (!factored) throw Error(HIFDE_ERR_STATE, "save: factor first");
+        HIFDE_CHECK(hipStreamSynchronize(stream));
+        File fl(path, true);
+        fl.put(kFileMagic, 8);
+        const int hdr[6] = {N, P, precision, L, quirk ? 1 : 0, lu_info};
+        fl.put(hdr, sizeof hdr);
+        fl.put(&eps, sizeof eps);
+        // operator, as COO in CSR order
+        std::vector<int> rows(h_ci.size());
+        for (size_t i = 0; i + 1 < h_rp.size(); ++i)
+            for (int q = h_rp[i]; q < h_rp[i + 1]; ++q) rows[q] = (int)i;
+        fl.put_vec(rows);
+        fl.put_vec(h_ci);
+        fl.put_vec(h_v);
+        for (int lv = 0; lv < L; ++lv)
+            for (int kind = 0; kind < 2; ++kind) {
+                const BoxBatch& B = kind == 0 ? cells[lv] : faces[lv];
+                const int bh[2] = {B.nbox, B.esize};
+                fl.put(bh, sizeof bh);
+                fl.put_vec(B.r);
+                fl.put_vec(B.s);
+                fl.put_vec(B.red_gid);
+                fl.put_vec(B.skel_gid);
+                fl.put_vec(B.skel_pass);
+                if (B.nbox == 0) continue;
+                put_dev(fl, B.A22.p, B.tot_rr * B.esize);
+                put_dev(fl, B.A22inv.p, B.tot_rr * B.esize);
+                put_dev(fl, B.X.p, B.tot_rs * B.esize);
+                if (kind == 1) put_dev(fl, B.W.p, B.tot_rs * B.esize);
+            }
+        fl.put(kFileEnd, 8);
+    }
+
+    void load(const char* path) {
+        File fl(path, false);
+        char magic[8];
+        fl.get(magic, 8);
+        if (std::memcmp(magic, kFileMagic, 8) != 0) throw Error(HIFDE_ERR_ARG, "not a factor file");
+        int hdr[6];
+        double feps = 0;
+        fl.get(hdr, sizeof hdr);
+        fl.get(&feps, sizeof feps);
+        if (hdr[0] != N || hdr[1] != P || hdr[2] != precision || hdr[3] != L || feps != eps ||
+            hdr[4] != (quirk ? 1 : 0))
+            throw Error(HIFDE_ERR_ARG, "load: the file holds another grid, tolerance or precision");
+        const long long nd = (long long)NC * NC * NC;
+        std::vector<int> rows, cols;
+        std::vector<double> vals;
+        fl.get_vec(rows, 64 * nd);
+        fl.get_vec(cols, 64 * nd);
+        fl.get_vec(vals, 64 * nd);
+        if (rows.size() != cols.size() || rows.size() != vals.size())
+            throw Error(HIFDE_ERR_STATE, "factor file corrupt");
+        set_coo((long long)rows.size(), rows.data(), cols.data(), vals.data());
+        factored = false;
+        clear_graphs();
+        cells.resize(L);
+        faces.resize(L);
+        for (int lv = 0; lv < L; ++lv)
+            for (int kind = 0; kind < 2; ++kind) {
+                BoxBatch& B = kind == 0 ? cells[lv] : faces[lv];
+                B.reset_host();
+                int bh[2];
+                fl.get(bh, sizeof bh);
+                if (bh[0] < 0 || bh[0] > nd || (bh[1] != 4 && bh[1] != 8))
+                    throw Error(HIFDE_ERR_STATE, "factor file corrupt");
+                B.nbox = bh[0];
+                B.esize = bh[1];
+                fl.get_vec(B.r, nd);
+                fl.get_vec(B.s, nd);
+                fl.get_vec(B.red_gid, nd);
+                fl.get_vec(B.skel_gid, nd);
+                fl.get_vec(B.skel_pass, nd);
+                if ((int)B.r.size() != B.nbox || (int)B.s.size() != B.nbox)
+                    throw Error(HIFDE_ERR_STATE, "factor file corrupt");
+                B.red_off = prefix_i(B.r);
+                B.skel_off = prefix_i(B.s);
+                if ((long long)B.red_gid.size() != B.red_off.back() ||
+                    (long long)B.skel_gid.size() != B.skel_off.back() ||
+                    (!B.skel_pass.empty() && B.skel_pass.size() != B.skel_gid.size()))
+                    throw Error(HIFDE_ERR_STATE, "factor file corrupt");
+                for (int g : B.red_gid)
+                    if (g < 0 || g >= nd) throw Error(HIFDE_ERR_STATE, "factor file corrupt");
+                for (int g : B.skel_gid)
+                    if (g < 0 || g >= nd) throw Error(HIFDE_ERR_STATE, "factor file corrupt");
+                B.off_rr = prefix_ll(B.r, B.r);
+                B.off_rs = prefix_ll(B.r, B.s);
+                B.off_ss = prefix_ll(B.s, B.s);
+                B.tot_rr = B.off_rr.back();
+                B.tot_rs = B.off_rs.back();
+                B.tot_ss = B.off_ss.back();
+                if (B.nbox == 0) continue;
+                upload_batch_index(B);
+                get_dev(fl, B.A22, B.tot_rr * B.esize);
+                get_dev(fl, B.A22inv, B.tot_rr * B.esize);
+                get_dev(fl, B.X, B.tot_rs * B.esize);
+                if (kind == 1) get_dev(fl, B.W, B.tot_rs * B.esize);
+            }
+        fl.get(magic, 8);
+        if (std::memcmp(magic, kFileEnd, 8) != 0) throw Error(HIFDE_ERR_STATE, "factor file corrupt");
+        HIFDE_CHECK(hipStreamSynchronize(stream));
+        lu_info = hdr[5];
+        plan0_valid = false;  // the level-0 index is rebuilt by the next factorization
+        factored = true;
+    }
+
+    // ------------------------------------------------------------------
     // Operator assembly: rows/cols/vals with Sparse::Add semantics.
     void set_coo(long long nnz, const int* rows, const int* cols, const double* vals) {
         const long long nd = (long long)NC * NC * NC;
@@ -2213,6 +2379,22 @@ int hifde_pcg(hifde_factor* f, const double* b, double* x, double tol, int maxit
                                    f->stream));
         HIFDE_CHECK(hipStreamSynchronize(f->stream));
         return HIFDE_OK;
+    });
+}
+
+int hifde_factor_save(hifde_factor* f, const char* path) {
+    return guarded([&]() -> int {
+        if (!f || !path) throw Error(HIFDE_ERR_ARG, "save: bad arguments");
+        f->save(path);
+        return HIFDE_OK;
+    });
+}
+
+int hifde_factor_load(hifde_factor* f, const char* path) {
+    return guarded([&]() -> int {
+        if (!f || !path) throw Error(HIFDE_ERR_ARG, "load: bad arguments");
+        f->load(path);
+        return f->lu_info;
     });
 }
 

@@ -106,6 +106,18 @@ int hifde_pcg_multi_device(hifde_factor* f, const double* d_b, long long ldb, do
                            long long ldx, int nrhs, double tol, int maxit, int* iters,
                            double* relres);
 
+/* Factor persistence (no reference counterpart; the reference keeps its
+ * factor in memory only).  hifde_factor_save writes the operator, every box's
+ * DOF lists and its factor blocks (A22, A22^{-1}, X, W in the level's
+ * precision) to a binary file; hifde_factor_load reads it into an object
+ * created with the same N, P, epsilon, precision and flags (HIFDE_ERR_ARG
+ * otherwise), which can then apply, solve and re-factor as if it had factored
+ * itself.  Returns the saved factorization's status (>= 0) on success.  A
+ * load that fails part-way (truncated or corrupt file) leaves the object
+ * without a factorization; a header mismatch leaves it untouched. */
+int hifde_factor_save(hifde_factor* f, const char* path);
+int hifde_factor_load(hifde_factor* f, const char* path);
+
 /* Introspection (the reference prints these, src/Factor.cpp:211-212,258-259). */
 int hifde_num_levels(const hifde_factor* f);
 /* out[0..4] = cells, DOFs eliminated by cells, faces, DOFs eliminated by

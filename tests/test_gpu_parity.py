@@ -608,3 +608,62 @@ def test_batched_uneven_instances_converge(H, oracle_mod):
             worst = max(worst, res)
             assert res <= 2e-10, (k, NC, j, res)
     assert worst > 0.0
+
+
+# --------------------------------------------------------------------------
+# factor persistence
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_factor_save_load_roundtrip(H, oracle_mod, precision, tmp_path):
+    """A factorization written to a file and read by a fresh object applies
+    and solves bit for bit like the original, and the loaded object can
+    factor again."""
+    import torch
+    f, _, _ = _both(H, oracle_mod, 31, 4, 1e-4, "pow10", True, precision)
+    path = str(tmp_path / "factor.bin")
+    f.save(path)
+    g = H.HIFFactor(31, 4, 1e-4, precision=precision)
+    assert g.load(path) == 0
+    assert g.level_counts() == f.level_counts()
+    b = pb.rhs_normal(32, 11)
+    for inv in (True, False):
+        assert np.array_equal(f.Apply(b, inv), g.Apply(b, inv))
+    U = torch.from_numpy(np.stack([pb.rhs_normal(32, 20 + c) for c in range(5)])).cuda()
+    Uf, Ug = U.clone(), U.clone()
+    f.apply_device(Uf.data_ptr(), 5, 32 ** 3, True)
+    g.apply_device(Ug.data_ptr(), 5, 32 ** 3, True)
+    torch.cuda.synchronize()
+    assert torch.equal(Uf, Ug)
+    xf, itf, rrf = f.pcg(b, tol=1e-10)
+    xg, itg, rrg = g.pcg(b, tol=1e-10)
+    assert itf == itg and rrf == rrg and np.array_equal(xf, xg)
+    assert g.Factor() == 0  # the loaded operator factors again
+    assert g.level_counts() == f.level_counts()
+    assert np.array_equal(f.Apply(b, True), g.Apply(b, True))
+    f.close()
+    g.close()
+
+
+def test_factor_load_rejects_other_problem_and_bad_files(H, oracle_mod, tmp_path):
+    f, _, _ = _both(H, oracle_mod, 31, 4, 1e-4, "pow10", True, "mixed")
+    path = tmp_path / "factor.bin"
+    f.save(str(path))
+    for args in ((31, 4, 1e-6, "mixed"), (31, 4, 1e-4, "fp64"), (15, 4, 1e-4, "mixed")):
+        g = H.HIFFactor(args[0], args[1], args[2], precision=args[3])
+        with pytest.raises(H.HifdeError):
+            g.load(str(path))
+        g.close()
+    data = path.read_bytes()
+    cut = tmp_path / "cut.bin"
+    cut.write_bytes(data[: len(data) // 2])
+    junk = tmp_path / "junk.bin"
+    junk.write_bytes(b"not a factor" * 10)
+    g = H.HIFFactor(31, 4, 1e-4, precision="mixed")
+    for bad in (cut, junk, tmp_path / "missing.bin"):
+        with pytest.raises(H.HifdeError):
+            g.load(str(bad))
+    assert g.load(str(path)) == 0  # a failed load leaves the object usable
+    b = pb.rhs_normal(32, 5)
+    assert np.array_equal(f.Apply(b, True), g.Apply(b, True))
+    f.close()
+    g.close()
