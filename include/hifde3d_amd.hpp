@@ -197,6 +197,36 @@ private:
 };
 typedef NumTns<int> IntNumTns;
 
+// Per-box factor record (include/data.hpp:11-110), the members Schur fills.
+class IndexData {
+public:
+    std::vector<int>& global_inds() { return global_; }
+    std::vector<int>& redundant_inds() { return red_; }
+    std::vector<int>& skeleton_inds() { return skel_; }
+
+private:
+    std::vector<int> global_, red_, skel_;
+};
+
+template <class Scalar>
+class FactorData {
+public:
+    Dense<Scalar>& A_22() { return A22_; }
+    Dense<Scalar>& A_22_inv() { return A22inv_; }
+    Dense<Scalar>& X_mat() { return X_; }
+    Dense<Scalar>& Schur_comp() { return S_; }
+    Dense<Scalar>& W_mat() { return W_; }
+    IndexData& ind_data() { return ind_; }
+    int NumDOFsEliminated() { return (int)ind_.redundant_inds().size(); }
+    void set_face(Face f) { face_ = f; }
+    Face face() { return face_; }
+
+private:
+    IndexData ind_;
+    Dense<Scalar> A22_, A22inv_, X_, S_, W_;
+    Face face_ = TOP;
+};
+
 // ---------------------------------------------------------------------------
 // HIFFactor<Scalar> (include/Factor.hpp:8-203)
 template <class Scalar>
@@ -228,6 +258,7 @@ public:
                                            vals.data()));
         sp_matrix_.Clear();
         int rc = hifde_factor_run(h_);
+        ++factor_stamp_;
         detail::check(rc);
         if (rc > 0) throw std::runtime_error("hifde: singular pivot in a Schur block");
     }
@@ -285,7 +316,65 @@ public:
     }
     hifde_factor* handle() { return h_; }
 
+    // FactorData of every cell box and every face box, per level
+    // (include/Factor.hpp:50-51 declares these and the reference never
+    // defines them).  Materialised from the device on the first call after a
+    // factorization, which copies the whole factor to the host.
+    // global_inds() lists a box's redundant DOFs, then its skeleton DOFs, in
+    // the box's order; redundant_inds() and skeleton_inds() are positions
+    // into it.  Face boxes also carry W (|skel| x |red|).
+    std::vector<std::vector<FactorData<Scalar>>>& schur_level_data() {
+        materialize(0, schur_data_);
+        return schur_data_;
+    }
+    std::vector<std::vector<FactorData<Scalar>>>& skel_level_data() {
+        materialize(1, skel_data_);
+        return skel_data_;
+    }
+
 private:
+    void materialize(int kind, std::vector<std::vector<FactorData<Scalar>>>& out) {
+        if (!h_) throw std::runtime_error("hifde: factor first");
+        if (!out.empty() && data_stamp_[kind] == factor_stamp_) return;
+        const int L = hifde_num_levels(h_);
+        out.assign(L > 0 ? L : 0, {});
+        for (int l = 0; l < L; ++l) {
+            const int nb = hifde_box_count(h_, l, kind);
+            detail::check(nb);
+            out[l].resize(nb);
+            for (int b = 0; b < nb; ++b) {
+                int rs[2];
+                detail::check(hifde_box_sizes(h_, l, kind, b, rs));
+                const int r = rs[0], s = rs[1];
+                std::vector<int> rg(r + 1), sg(s + 1);
+                std::vector<double> a22((size_t)r * r + 1), ainv((size_t)r * r + 1),
+                    x((size_t)r * s + 1), w((size_t)s * r + 1);
+                detail::check(hifde_box_data(h_, l, kind, b, rg.data(), sg.data(), a22.data(),
+                                             ainv.data(), x.data(), kind == 1 ? w.data() : nullptr));
+                FactorData<Scalar>& d = out[l][b];
+                std::vector<int>& g = d.ind_data().global_inds();
+                g.assign(rg.begin(), rg.begin() + r);
+                g.insert(g.end(), sg.begin(), sg.begin() + s);
+                d.ind_data().redundant_inds().resize(r);
+                d.ind_data().skeleton_inds().resize(s);
+                for (int i = 0; i < r; ++i) d.ind_data().redundant_inds()[i] = i;
+                for (int i = 0; i < s; ++i) d.ind_data().skeleton_inds()[i] = r + i;
+                auto fill = [](Dense<Scalar>& D, const std::vector<double>& v, int m, int n) {
+                    D.Resize(m, n);
+                    for (int j = 0; j < n; ++j)
+                        for (int i = 0; i < m; ++i) D.Set(i, j, (Scalar)v[i + (size_t)j * m]);
+                };
+                fill(d.A_22(), a22, r, r);
+                fill(d.A_22_inv(), ainv, r, r);
+                fill(d.X_mat(), x, r, s);
+                if (kind == 1) fill(d.W_mat(), w, s, r);
+            }
+        }
+        data_stamp_[kind] = factor_stamp_;
+    }
+    std::vector<std::vector<FactorData<Scalar>>> schur_data_, skel_data_;
+    long long factor_stamp_ = 0, data_stamp_[2] = {-1, -1};
+
     static int precision() {
         return std::is_same<Scalar, float>::value ? HIFDE_MIXED : HIFDE_FP64;
     }
@@ -374,36 +463,6 @@ void InterpDecomp(Dense<Scalar>& M, Dense<Scalar>& W, std::vector<int>& skeleton
     for (int j = 0; j < n - k; ++j)
         for (int i = 0; i < k; ++i) W.Set(i, j, (Scalar)Wp[i + (size_t)j * k]);
 }
-
-// Per-box factor record (include/data.hpp:11-110), the members Schur fills.
-class IndexData {
-public:
-    std::vector<int>& global_inds() { return global_; }
-    std::vector<int>& redundant_inds() { return red_; }
-    std::vector<int>& skeleton_inds() { return skel_; }
-
-private:
-    std::vector<int> global_, red_, skel_;
-};
-
-template <class Scalar>
-class FactorData {
-public:
-    Dense<Scalar>& A_22() { return A22_; }
-    Dense<Scalar>& A_22_inv() { return A22inv_; }
-    Dense<Scalar>& X_mat() { return X_; }
-    Dense<Scalar>& Schur_comp() { return S_; }
-    Dense<Scalar>& W_mat() { return W_; }
-    IndexData& ind_data() { return ind_; }
-    int NumDOFsEliminated() { return (int)ind_.redundant_inds().size(); }
-    void set_face(Face f) { face_ = f; }
-    Face face() { return face_; }
-
-private:
-    IndexData ind_;
-    Dense<Scalar> A22_, A22inv_, X_, S_, W_;
-    Face face_ = TOP;
-};
 
 // Schur(matrix, data) (include/Schur.hpp:72-100) on the GPU as a batch of one:
 // A22 = M(red, red), X = A22^{-1} M(red, skel), S = -M(skel, red) X.

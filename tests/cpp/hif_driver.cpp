@@ -4,6 +4,7 @@
 // e_s = ||v - F^{-1} w|| / ||v||.  Usage: hif_driver -tol T -N N -P P
 #include <cstdio>
 #include <cstdlib>
+#include <cmath>
 #include <map>
 #include <string>
 
@@ -48,5 +49,27 @@ int main(int argc, char** argv) {
         printf("level %zu: cells %d eliminated %d faces %d eliminated %d\n", l, lc[l][0], lc[l][1],
                lc[l][2], lc[l][3]);
     printf("e_a %.17g\ne_s %.17g\n", e_a, e_s);
+    // FactorData of every box, materialised through the drop-in
+    std::vector<std::vector<FactorData<double>>>& sd = factor.schur_level_data();
+    std::vector<std::vector<FactorData<double>>>& kd = factor.skel_level_data();
+    double inv_err = 0.0;
+    for (size_t l = 0; l < sd.size(); ++l) {
+        int rc = 0, rf = 0;
+        for (FactorData<double>& d : sd[l]) {
+            rc += d.NumDOFsEliminated();
+            // A_22 A_22^{-1} = I on every cell box
+            const int r = d.A_22().Height();
+            for (int i = 0; i < r; ++i)
+                for (int j = 0; j < r; ++j) {
+                    double acc = 0.0;
+                    for (int t = 0; t < r; ++t) acc += d.A_22().Get(i, t) * d.A_22_inv().Get(t, j);
+                    inv_err = std::max(inv_err, std::fabs(acc - (i == j ? 1.0 : 0.0)));
+                }
+        }
+        for (FactorData<double>& d : kd[l]) rf += d.NumDOFsEliminated();
+        printf("data level %zu: cells %zu eliminated %d faces %zu eliminated %d\n", l, sd[l].size(), rc,
+               kd[l].size(), rf);
+    }
+    printf("inv_err %.3g\n", inv_err);
     return 0;
 }

@@ -306,6 +306,12 @@ def test_cpp_dropin_driver_reference_protocol(H, oracle_mod):
     es_o = _rel(o.apply(w, True), v)
     assert float(vals["e_a"]) <= 2 * ea_o + 1e-14
     assert float(vals["e_s"]) <= 2 * es_o + 1e-14
+    # schur_level_data() / skel_level_data() agree with the level counts
+    counts = [ln.split(":")[1] for ln in out.splitlines() if ln.startswith("level ")]
+    data = [ln.split(":")[1] for ln in out.splitlines() if ln.startswith("data level ")]
+    assert counts and counts == data
+    inv_err = float([ln.split()[1] for ln in out.splitlines() if ln.startswith("inv_err")][0])
+    assert inv_err < 1e-9
 
 
 def _golden(name):
