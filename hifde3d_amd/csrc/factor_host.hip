@@ -1511,10 +1511,12 @@ struct hifde_factor {
         if (!g || g->gen != apply_gen) {
             // first use, or the state changed since: eager, sizes the
             // workspaces and builds the descriptor tables
-            const void* before[5] = {apply_tmp.p, apply_side.p, rhs_in.p, rhs_out.p, rhs_side.p};
+            const void* before[6] = {apply_tmp.p, apply_side.p, rhs_in.p,
+                                     rhs_out.p,   rhs_side.p,   rhs_il.p};
             apply_eager(d_u, R, ld, inv);
-            const void* after[5] = {apply_tmp.p, apply_side.p, rhs_in.p, rhs_out.p, rhs_side.p};
-            for (int i = 0; i < 5; ++i)
+            const void* after[6] = {apply_tmp.p, apply_side.p, rhs_in.p,
+                                    rhs_out.p,   rhs_side.p,   rhs_il.p};
+            for (int i = 0; i < 6; ++i)
                 if (before[i] != after[i]) ++apply_gen;
             if (!g) {
                 if (graphs.size() == kMaxGraphs) {
@@ -1554,7 +1556,7 @@ struct hifde_factor {
     // once for all R vectors.  The GEMM descriptors of a batch are built once
     // per factorization and R.
     DevBuf<unsigned char> rhs_in, rhs_out;
-    DevBuf<double> rhs_side;
+    DevBuf<double> rhs_side, rhs_il;
     // batches whose boxes have at most this many inputs and outputs take the
     // fused kernel; larger ones the packed GEMM path
     int multi_fused_max = 512;
@@ -1563,9 +1565,9 @@ struct hifde_factor {
         return red ? B.d_red_gid.p : B.d_skel_gid.p;
     }
     void upd_multi(BoxBatch& B, const DevBuf<unsigned char>& Mbuf, int op, double alpha,
-                   double* U, long long ldu, int R, bool side = false) {
-        if (B.esize == sizeof(float)) upd_multi_t<float>(B, Mbuf, op, alpha, U, ldu, R, side);
-        else upd_multi_t<double>(B, Mbuf, op, alpha, U, ldu, R, side);
+                   double* U, RhsLayout ly, int R, bool side = false) {
+        if (B.esize == sizeof(float)) upd_multi_t<float>(B, Mbuf, op, alpha, U, ly, R, side);
+        else upd_multi_t<double>(B, Mbuf, op, alpha, U, ly, R, side);
     }
     // op as in apply_one: 0 X^T u_red -> skel, 1 X u_skel -> red, 2 W^T u_skel
     // -> red, 3 W u_red -> skel, 4 M u_red -> red (written, not added).  With
@@ -1573,7 +1575,7 @@ struct hifde_factor {
     // added in pass order.
     template <class T>
     void upd_multi_t(BoxBatch& B, const DevBuf<unsigned char>& Mbuf, int op, double alpha,
-                     double* U, long long ldu, int R, bool side) {
+                     double* U, RhsLayout ly, int R, bool side) {
         if (B.nbox == 0) return;
         const bool in_red = (op == 0 || op == 3 || op == 4);
         const bool out_red = (op == 1 || op == 2 || op == 4);
@@ -1583,8 +1585,12 @@ struct hifde_factor {
         if (nout == 0) return;
         const long long n3 = (long long)NC * NC * NC;
         const int max_in = in_red ? B.max_r : B.max_s, max_out = out_red ? B.max_r : B.max_s;
-        const bool fused = op == 4 ? B.max_r <= 64
-                                   : (max_in <= multi_fused_max && max_out <= multi_fused_max);
+        // few columns: the fused kernel for every size (the packed GEMMs'
+        // tiles would be mostly idle); many: packed GEMMs beyond
+        // multi_fused_max (measured at n=128: R=4 10.3 vs 12.5 ms per apply,
+        // R=64 33.7 vs 42.5 ms with the threshold)
+        const int fmax = R <= 16 && multi_fused_max > 0 ? (1 << 30) : multi_fused_max;
+        const bool fused = op == 4 ? B.max_r <= 64 : (max_in <= fmax && max_out <= fmax);
         if (fused) {
             const double fl = 2.0 * (op == 4 ? B.tot_rr : B.tot_rs) * R;
             const double by = (double)(op == 4 ? B.tot_rr : B.tot_rs) * sizeof(T) +
@@ -1596,11 +1602,11 @@ struct hifde_factor {
                                  B.d_r.p, B.d_s.p, B.d_red_off.p, B.d_skel_off.p, B.d_red_gid.p,
                                  B.d_skel_gid.p, B.d_skel_pass.p,
                                  op == 4 ? B.d_off_rr.p : B.d_off_rs.p,
-                                 reinterpret_cast<const T*>(Mbuf.p), op, alpha, U, ldu, R,
-                                 side ? rhs_side.p : nullptr, side ? n3 : 0, fl, by, stream);
+                                 reinterpret_cast<const T*>(Mbuf.p), op, alpha, U, ly, R,
+                                 side ? rhs_side.p : nullptr, side ? n3 * R : 0, fl, by, stream);
             if (side)
-                rhs_side_add(nout, ogid_of(B, out_red), B.d_skel_pass.p, rhs_side.p, n3, R, U,
-                             ldu, stream);
+                rhs_side_add(nout, ogid_of(B, out_red), B.d_skel_pass.p, rhs_side.p, n3 * R, R, U,
+                             ly, stream);
             return;
         }
         GemmPlan& pl = B.rplan[op];
@@ -1646,18 +1652,18 @@ struct hifde_factor {
         const int* d_ooff = out_red ? B.d_red_off.p : B.d_skel_off.p;
         const int* igid = in_red ? B.d_red_gid.p : B.d_skel_gid.p;
         const int* ogid = out_red ? B.d_red_gid.p : B.d_skel_gid.p;
-        rhs_gather<T>(B.nbox, nin, d_ioff, igid, U, ldu, R, pin, stream);
+        rhs_gather<T>(B.nbox, nin, d_ioff, igid, U, ly, R, pin, stream);
         if (B.rplan_zero[op]) fill_zero<T>(pout, nout * R, stream);
         if (!pl.launches.empty())
             gemm_run<T, T>(pl, {0, (int)pl.launches.size()}, reinterpret_cast<const T*>(Mbuf.p),
                            pin, pout, (T)1, (T)0, stream);
         if (side) {
             rhs_side.reserve((size_t)2 * R * n3);
-            rhs_scatter<T>(B.nbox, nout, d_ooff, ogid, pout, R, rhs_side.p, 0, alpha, 2,
-                           B.d_skel_pass.p, n3, stream);
-            rhs_side_add(nout, ogid, B.d_skel_pass.p, rhs_side.p, n3, R, U, ldu, stream);
+            rhs_scatter<T>(B.nbox, nout, d_ooff, ogid, pout, R, rhs_side.p, ly, alpha, 2,
+                           B.d_skel_pass.p, n3 * R, stream);
+            rhs_side_add(nout, ogid, B.d_skel_pass.p, rhs_side.p, n3 * R, R, U, ly, stream);
         } else {
-            rhs_scatter<T>(B.nbox, nout, d_ooff, ogid, pout, R, U, ldu, alpha, op == 4 ? 1 : 0,
+            rhs_scatter<T>(B.nbox, nout, d_ooff, ogid, pout, R, U, ly, alpha, op == 4 ? 1 : 0,
                            nullptr, 0, stream);
         }
     }
@@ -1666,41 +1672,48 @@ struct hifde_factor {
     // reads and writes only its own redundant DOFs, every interior DOF is
     // redundant in exactly one box, and a box's inputs are gathered before
     // its outputs are written.
-    void apply_multi(double* U, int R, long long ldu, bool inv) {
-        for (int c = 0; c < R; ++c) zero_boundary(U + c * ldu, NC, stream);
+    void apply_multi(double* Ucm, int R, long long ldu, bool inv) {
+        // the sweeps run on an interleaved copy (a DOF's R values contiguous)
+        const long long n3 = (long long)NC * NC * NC;
+        rhs_il.reserve((size_t)n3 * R);
+        double* U = rhs_il.p;
+        const RhsLayout ly{R, 1};
+        to_interleaved(n3, R, Ucm, ldu, U, stream);
+        zero_boundary_il(NC, R, U, stream);
         for (int lv = 0; lv < L - 1; ++lv) {
             BoxBatch& C = cells[lv];
-            if (inv) upd_multi(C, C.X, 0, -1.0, U, ldu, R, true);
-            else upd_multi(C, C.X, 1, 1.0, U, ldu, R);
+            if (inv) upd_multi(C, C.X, 0, -1.0, U, ly, R, true);
+            else upd_multi(C, C.X, 1, 1.0, U, ly, R);
             BoxBatch& F = faces[lv];
             if (F.nbox == 0) continue;
             if (inv) {
-                upd_multi(F, F.W, 2, -1.0, U, ldu, R);
-                upd_multi(F, F.X, 0, -1.0, U, ldu, R);
+                upd_multi(F, F.W, 2, -1.0, U, ly, R);
+                upd_multi(F, F.X, 0, -1.0, U, ly, R);
             } else {
-                upd_multi(F, F.W, 3, 1.0, U, ldu, R);
-                upd_multi(F, F.X, 1, 1.0, U, ldu, R);
+                upd_multi(F, F.W, 3, 1.0, U, ly, R);
+                upd_multi(F, F.X, 1, 1.0, U, ly, R);
             }
         }
         for (int lv = 0; lv < L; ++lv) {
-            upd_multi(cells[lv], inv ? cells[lv].A22inv : cells[lv].A22, 4, 1.0, U, ldu, R);
-            upd_multi(faces[lv], inv ? faces[lv].A22inv : faces[lv].A22, 4, 1.0, U, ldu, R);
+            upd_multi(cells[lv], inv ? cells[lv].A22inv : cells[lv].A22, 4, 1.0, U, ly, R);
+            upd_multi(faces[lv], inv ? faces[lv].A22inv : faces[lv].A22, 4, 1.0, U, ly, R);
         }
         for (int lv = L - 2; lv >= 0; --lv) {
             BoxBatch& F = faces[lv];
             if (F.nbox) {
                 if (inv) {
-                    upd_multi(F, F.X, 1, -1.0, U, ldu, R);
-                    upd_multi(F, F.W, 3, -1.0, U, ldu, R);
+                    upd_multi(F, F.X, 1, -1.0, U, ly, R);
+                    upd_multi(F, F.W, 3, -1.0, U, ly, R);
                 } else {
-                    upd_multi(F, F.X, 0, 1.0, U, ldu, R);
-                    upd_multi(F, F.W, 2, 1.0, U, ldu, R);
+                    upd_multi(F, F.X, 0, 1.0, U, ly, R);
+                    upd_multi(F, F.W, 2, 1.0, U, ly, R);
                 }
             }
             BoxBatch& C = cells[lv];
-            if (inv) upd_multi(C, C.X, 1, -1.0, U, ldu, R);
-            else upd_multi(C, C.X, 0, 1.0, U, ldu, R, true);
+            if (inv) upd_multi(C, C.X, 1, -1.0, U, ly, R);
+            else upd_multi(C, C.X, 0, 1.0, U, ly, R, true);
         }
+        from_interleaved(n3, R, U, Ucm, ldu, stream);
     }
 
     // ------------------------------------------------------------------

@@ -407,25 +407,35 @@ void side_add(int n, const int* d_gid, const unsigned char* d_sp, const double* 
               long long side_n, double* d_u, hipStream_t s);
 void zero_boundary(double* d_u, int NC, hipStream_t s);
 
-// several right-hand sides (multi_rhs.hip); U column-major with leading dim ldu
+// several right-hand sides (multi_rhs.hip).  A block of R vectors: entry
+// (dof g, column c) at U[g sg + c sc]; column-major is {1, ld}, interleaved
+// (a DOF's R values contiguous) is {R, 1}.
+struct RhsLayout {
+    long long sg, sc;
+};
 template <class T>
 void rhs_gather(int nbox, long long nentries, const int* d_off, const int* d_gid, const double* U,
-                long long ldu, int R, T* pack, hipStream_t s);
-// mode 0: add alpha * pack; 1: write pack; 2: per-pass side buffers
+                RhsLayout ly, int R, T* pack, hipStream_t s);
+// mode 0: add alpha * pack; 1: write pack; 2: per-pass side buffers (laid
+// out like U, pstride apart)
 template <class T>
 void rhs_scatter(int nbox, long long nentries, const int* d_off, const int* d_gid, const T* pack,
-                 int R, double* U, long long ldu, double alpha, int mode, const unsigned char* d_sp,
-                 long long side_n, hipStream_t s);
+                 int R, double* U, RhsLayout ly, double alpha, int mode, const unsigned char* d_sp,
+                 long long pstride, hipStream_t s);
+void to_interleaved(long long n, int R, const double* U, long long ld, double* W, hipStream_t s);
+void from_interleaved(long long n, int R, const double* W, double* U, long long ld,
+                      hipStream_t s);
+void zero_boundary_il(int NC, int R, double* W, hipStream_t s);
 // gather -> product -> scatter in one kernel (boxes of moderate size)
 template <class T>
 void apply_multi_fused(int nbox, int ntiles, const int* d_tile_off, const int* d_r, const int* d_s,
                        const int* d_red_off, const int* d_skel_off, const int* d_red_gid,
                        const int* d_skel_gid, const unsigned char* d_skel_pass,
                        const long long* d_off, const T* d_M, int op, double alpha, double* U,
-                       long long ldu, int R, double* side, long long side_n, double flops,
+                       RhsLayout ly, int R, double* side, long long pstride, double flops,
                        double bytes, hipStream_t s);
 void rhs_side_add(long long nentries, const int* d_gid, const unsigned char* d_sp,
-                  const double* side, long long side_n, int R, double* U, long long ldu,
+                  const double* side, long long pstride, int R, double* U, RhsLayout ly,
                   hipStream_t s);
 void csr_spmm(int n, const int* d_rp, const int* d_ci, const double* d_v, const double* X,
               long long ldx, double* Y, long long ldy, int R, hipStream_t s);

@@ -6,8 +6,13 @@
 // gathered into a packed column-major block (nin x R), multiplied by the box's
 // factor block on the matrix cores, and the outputs scattered back (added,
 // written, or parked in the per-pass side buffers of the cell sweep).  The
-// factor blocks are read once for all R vectors.  Vectors are column-major,
-// column c of U at U + c * ldu.
+// factor blocks are read once for all R vectors.
+//
+// A block of R vectors is addressed through a layout {sg, sc}: entry (dof g,
+// column c) at U[g * sg + c * sc].  The caller's blocks are column-major
+// ({1, ldu}); the sweeps run on an interleaved working copy ({R, 1}), where
+// the R values of one DOF are contiguous, so every gather and scatter of a
+// box's DOFs reads and writes whole cache lines.
 
 #include "hifde_internal.hpp"
 
@@ -33,43 +38,45 @@ inline unsigned grid1(long long n) {
     return (unsigned)g;
 }
 
-// pack[off[b] R + i + c nin_b] = U[gid[off[b] + i] + c ldu]; one thread per
+// pack[off[b] R + i + c nin_b] = U(gid[off[b] + i], c); one thread per
 // entry, all R columns
 template <class T>
 __global__ void k_rhs_gather(int nbox, const int* __restrict__ off, const int* __restrict__ gid,
-                             const double* __restrict__ U, long long ldu, int R, T* pack) {
+                             const double* __restrict__ U, long long sg, long long sc, int R,
+                             T* pack) {
     const long long ne = off[nbox];
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < ne;
          t += (long long)gridDim.x * blockDim.x) {
         const int b = find_slot(off, nbox, t);
         const long long nb = off[b + 1] - off[b];
         T* dst = pack + (long long)off[b] * R + (t - off[b]);
-        const double* src = U + gid[t];
-        for (int c = 0; c < R; ++c) dst[c * nb] = (T)src[c * ldu];
+        const double* src = U + gid[t] * sg;
+        for (int c = 0; c < R; ++c) dst[c * nb] = (T)src[c * sc];
     }
 }
 
-// mode 0: U[g + c ldu] += alpha * pack;  1: U[...] = pack;
-// 2: side[(sp * R + c) side_n + g] = alpha * pack
+// mode 0: U(g, c) += alpha * pack;  1: U(g, c) = pack;
+// 2: side(pass flag, g, c) = alpha * pack, the side buffer laid out like U,
+// one block per pass flag, `pstride` apart
 template <class T>
 __global__ void k_rhs_scatter(int nbox, const int* __restrict__ off, const int* __restrict__ gid,
-                              const T* __restrict__ pack, int R, double* U, long long ldu,
-                              double alpha, int mode, const unsigned char* __restrict__ sp,
-                              long long side_n) {
+                              const T* __restrict__ pack, int R, double* U, long long sg,
+                              long long sc, double alpha, int mode,
+                              const unsigned char* __restrict__ sp, long long pstride) {
     const long long ne = off[nbox];
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < ne;
          t += (long long)gridDim.x * blockDim.x) {
         const int b = find_slot(off, nbox, t);
         const long long nb = off[b + 1] - off[b];
         const T* src = pack + (long long)off[b] * R + (t - off[b]);
-        const int g = gid[t];
+        double* dst = U + gid[t] * sg;
         if (mode == 0) {
-            for (int c = 0; c < R; ++c) U[g + c * ldu] += alpha * (double)src[c * nb];
+            for (int c = 0; c < R; ++c) dst[c * sc] += alpha * (double)src[c * nb];
         } else if (mode == 1) {
-            for (int c = 0; c < R; ++c) U[g + c * ldu] = (double)src[c * nb];
+            for (int c = 0; c < R; ++c) dst[c * sc] = (double)src[c * nb];
         } else {
-            double* dst = U + (long long)sp[t] * R * side_n + g;
-            for (int c = 0; c < R; ++c) dst[c * side_n] = alpha * (double)src[c * nb];
+            dst += (long long)sp[t] * pstride;
+            for (int c = 0; c < R; ++c) dst[c * sc] = alpha * (double)src[c * nb];
         }
     }
 }
@@ -81,8 +88,9 @@ __global__ void k_rhs_scatter(int nbox, const int* __restrict__ off, const int* 
 // (read along its contiguous dimension), and every thread accumulates one
 // output for CPT columns in FP64.  op as in k_apply_tiles; op 4 writes U in
 // place, which needs every box to fit one tile (the host checks r <= 64).
-// With side_n > 0 the outputs go to the pass-flag side buffers
-// side[(sp R + c) side_n + dof].
+// With side non-null the outputs go to the pass-flag side buffers (laid out
+// like U, one block per flag, pstride apart).  When the columns of a DOF are
+// contiguous (sc == 1) the staging loads walk the columns first.
 template <class T, int CPT>
 __global__ void __launch_bounds__(256) k_apply_multi(
     int nbox, const int* __restrict__ tile_off, const int* __restrict__ dr,
@@ -90,7 +98,7 @@ __global__ void __launch_bounds__(256) k_apply_multi(
     const int* __restrict__ skel_off, const int* __restrict__ red_gid,
     const int* __restrict__ skel_gid, const unsigned char* __restrict__ skel_pass,
     const long long* __restrict__ off, const T* __restrict__ Mb, int op, double alpha, double* U,
-    long long ldu, int R, double* side, long long side_n) {
+    long long sg, long long sc, int R, double* side, long long pstride) {
     constexpr int CB = 4 * CPT;
     const int t = blockIdx.x;
     const int b = find_slot(tile_off, nbox, t);
@@ -107,7 +115,7 @@ __global__ void __launch_bounds__(256) k_apply_multi(
     const int* og = out_red ? red_gid + red_off[b] : skel_gid + skel_off[b];
     const int o0 = tile * 64;
     const int cnt = min(64, nout - o0);
-    __shared__ double xs[CB][64];
+    __shared__ double xs[CB][65];  // padded: column-first staging stores hit distinct banks
     __shared__ T Ms[64][65];  // Ms[output][input]
     const int tid = threadIdx.x, o = tid & 63, g = tid >> 6;
     double acc[CPT];
@@ -118,9 +126,9 @@ __global__ void __launch_bounds__(256) k_apply_multi(
 #pragma unroll
         for (int q = 0; q < CPT; ++q) {
             const int e = tid + 256 * q;
-            const int j = e & 63, c = e >> 6;
+            const int j = sc == 1 ? e / CB : e & 63, c = sc == 1 ? e % CB : e >> 6;
             double v = 0.0;
-            if (j < kb && c0 + c < R) v = U[ig[j0 + j] + (long long)(c0 + c) * ldu];
+            if (j < kb && c0 + c < R) v = U[ig[j0 + j] * sg + (long long)(c0 + c) * sc];
             xs[c][j] = v;
         }
         if (!trans) {  // M: nout x nin, leading dim nout
@@ -147,35 +155,79 @@ __global__ void __launch_bounds__(256) k_apply_multi(
         __syncthreads();
     }
     if (o >= cnt) return;
-    const int dof = og[o0 + o];
+    const long long dof = (long long)og[o0 + o] * sg;
+    double* dst = side ? side + (long long)skel_pass[skel_off[b] + o0 + o] * pstride + dof
+                       : U + dof;
 #pragma unroll
     for (int q = 0; q < CPT; ++q) {
         const int c = c0 + g * CPT + q;
         if (c >= R) break;
-        if (op == 4) {
-            U[dof + (long long)c * ldu] = acc[q];
-        } else if (side_n > 0) {
-            const int p = skel_pass[skel_off[b] + o0 + o];
-            side[((long long)p * R + c) * side_n + dof] = alpha * acc[q];
-        } else {
-            U[dof + (long long)c * ldu] += alpha * acc[q];
-        }
+        if (op == 4) dst[c * sc] = acc[q];
+        else if (side) dst[c * sc] = alpha * acc[q];
+        else dst[c * sc] += alpha * acc[q];
     }
 }
 
-// U[g + c ldu] += side[(p R + c) side_n + g] for the entries with pass flag p
+// U(g, c) += side(p, g, c) for the entries with pass flag p (side laid out
+// like U, pstride per flag); one thread per entry, all columns
 __global__ void k_rhs_side_add(long long ne, const int* __restrict__ gid,
                                const unsigned char* __restrict__ sp,
-                               const double* __restrict__ side, long long side_n, int R, int p,
-                               double* U, long long ldu) {
-    const long long total = ne * R;
-    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
-         e += (long long)gridDim.x * blockDim.x) {
-        const int c = (int)(e / ne);
-        const long long t = e - (long long)c * ne;
+                               const double* __restrict__ side, long long pstride, int R, int p,
+                               double* U, long long sg, long long sc) {
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < ne;
+         t += (long long)gridDim.x * blockDim.x) {
         if ((int)sp[t] != p) continue;
-        const int g = gid[t];
-        U[g + c * ldu] += side[((long long)p * R + c) * side_n + g];
+        const long long g = (long long)gid[t] * sg;
+        const double* src = side + (long long)p * pstride + g;
+        double* dst = U + g;
+        for (int c = 0; c < R; ++c) dst[c * sc] += src[c * sc];
+    }
+}
+
+// Column-major (n x R, leading dim ld) <-> interleaved (W[g R + c]) through
+// a 64-DOF x R LDS tile (R <= 64 per launch column chunk): both sides are
+// read and written along their contiguous direction.
+__global__ void __launch_bounds__(256) k_to_interleaved(long long n, int R, const double* U,
+                                                        long long ld, double* W) {
+    __shared__ double tile[64][65];
+    const long long g0 = (long long)blockIdx.x * 64;
+    const int c0 = blockIdx.y * 64;
+    const int cn = min(64, R - c0);
+    for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+        const int i = e & 63, c = e >> 6;
+        if (c < cn && g0 + i < n) tile[c][i] = U[g0 + i + (long long)(c0 + c) * ld];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < 64 * cn; e += 256) {
+        const int c = e % cn, i = e / cn;
+        if (g0 + i < n) W[(g0 + i) * R + c0 + c] = tile[c][i];
+    }
+}
+__global__ void __launch_bounds__(256) k_from_interleaved(long long n, int R, const double* W,
+                                                          double* U, long long ld) {
+    __shared__ double tile[64][65];
+    const long long g0 = (long long)blockIdx.x * 64;
+    const int c0 = blockIdx.y * 64;
+    const int cn = min(64, R - c0);
+    for (int e = threadIdx.x; e < 64 * cn; e += 256) {
+        const int c = e % cn, i = e / cn;
+        if (g0 + i < n) tile[c][i] = W[(g0 + i) * R + c0 + c];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+        const int i = e & 63, c = e >> 6;
+        if (c < cn && g0 + i < n) U[g0 + i + (long long)(c0 + c) * ld] = tile[c][i];
+    }
+}
+
+// zero the rows of the boundary DOFs (a coordinate 0) of an interleaved block
+__global__ void k_zero_boundary_il(int NC, int R, double* W) {
+    const long long n = (long long)NC * NC * NC;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n * R;
+         e += (long long)gridDim.x * blockDim.x) {
+        const long long g = e / R;
+        const int i = (int)(g % NC), j = (int)((g / NC) % NC), k = (int)(g / ((long long)NC * NC));
+        if (i == 0 || j == 0 || k == 0) W[e] = 0.0;
     }
 }
 
@@ -240,19 +292,38 @@ __global__ void k_axpy_cols(long long n, const double* __restrict__ a, const dou
 
 template <class T>
 void rhs_gather(int nbox, long long nentries, const int* d_off, const int* d_gid, const double* U,
-                long long ldu, int R, T* pack, hipStream_t s) {
+                RhsLayout ly, int R, T* pack, hipStream_t s) {
     if (nbox <= 0 || nentries <= 0) return;
-    k_rhs_gather<T><<<grid1(nentries), 256, 0, s>>>(nbox, d_off, d_gid, U, ldu, R, pack);
+    k_rhs_gather<T><<<grid1(nentries), 256, 0, s>>>(nbox, d_off, d_gid, U, ly.sg, ly.sc, R, pack);
     HIFDE_LAUNCH_CHECK();
 }
 
 template <class T>
 void rhs_scatter(int nbox, long long nentries, const int* d_off, const int* d_gid, const T* pack,
-                 int R, double* U, long long ldu, double alpha, int mode, const unsigned char* d_sp,
-                 long long side_n, hipStream_t s) {
+                 int R, double* U, RhsLayout ly, double alpha, int mode, const unsigned char* d_sp,
+                 long long pstride, hipStream_t s) {
     if (nbox <= 0 || nentries <= 0) return;
-    k_rhs_scatter<T><<<grid1(nentries), 256, 0, s>>>(nbox, d_off, d_gid, pack, R, U, ldu, alpha,
-                                                     mode, d_sp, side_n);
+    k_rhs_scatter<T><<<grid1(nentries), 256, 0, s>>>(nbox, d_off, d_gid, pack, R, U, ly.sg, ly.sc,
+                                                     alpha, mode, d_sp, pstride);
+    HIFDE_LAUNCH_CHECK();
+}
+
+void to_interleaved(long long n, int R, const double* U, long long ld, double* W, hipStream_t s) {
+    if (n <= 0 || R <= 0) return;
+    k_to_interleaved<<<dim3((unsigned)((n + 63) / 64), (R + 63) / 64), 256, 0, s>>>(n, R, U, ld, W);
+    HIFDE_LAUNCH_CHECK();
+}
+
+void from_interleaved(long long n, int R, const double* W, double* U, long long ld,
+                      hipStream_t s) {
+    if (n <= 0 || R <= 0) return;
+    k_from_interleaved<<<dim3((unsigned)((n + 63) / 64), (R + 63) / 64), 256, 0, s>>>(n, R, W, U,
+                                                                                     ld);
+    HIFDE_LAUNCH_CHECK();
+}
+
+void zero_boundary_il(int NC, int R, double* W, hipStream_t s) {
+    k_zero_boundary_il<<<grid1((long long)NC * NC * NC * R), 256, 0, s>>>(NC, R, W);
     HIFDE_LAUNCH_CHECK();
 }
 
@@ -261,7 +332,7 @@ void apply_multi_fused(int nbox, int ntiles, const int* d_tile_off, const int* d
                        const int* d_red_off, const int* d_skel_off, const int* d_red_gid,
                        const int* d_skel_gid, const unsigned char* d_skel_pass,
                        const long long* d_off, const T* d_M, int op, double alpha, double* U,
-                       long long ldu, int R, double* side, long long side_n, double flops,
+                       RhsLayout ly, int R, double* side, long long pstride, double flops,
                        double bytes, hipStream_t s) {
     if (nbox <= 0 || ntiles <= 0 || R <= 0) return;
     KernelTimer* kt = active_timer();
@@ -272,7 +343,8 @@ void apply_multi_fused(int nbox, int ntiles, const int* d_tile_off, const int* d
 #define HIFDE_AM(CPT)                                                                           \
     k_apply_multi<T, CPT><<<grid, 256, 0, s>>>(nbox, d_tile_off, d_r, d_s, d_red_off,           \
                                                d_skel_off, d_red_gid, d_skel_gid, d_skel_pass, \
-                                               d_off, d_M, op, alpha, U, ldu, R, side, side_n)
+                                               d_off, d_M, op, alpha, U, ly.sg, ly.sc, R,  \
+                                               side, pstride)
     if (cpt == 1) HIFDE_AM(1);
     else if (cpt == 2) HIFDE_AM(2);
     else if (cpt == 4) HIFDE_AM(4);
@@ -283,12 +355,12 @@ void apply_multi_fused(int nbox, int ntiles, const int* d_tile_off, const int* d
 }
 
 void rhs_side_add(long long nentries, const int* d_gid, const unsigned char* d_sp,
-                  const double* side, long long side_n, int R, double* U, long long ldu,
+                  const double* side, long long pstride, int R, double* U, RhsLayout ly,
                   hipStream_t s) {
     if (nentries <= 0) return;
     for (int p = 0; p < 2; ++p) {
-        k_rhs_side_add<<<grid1(nentries * R), 256, 0, s>>>(nentries, d_gid, d_sp, side, side_n, R,
-                                                           p, U, ldu);
+        k_rhs_side_add<<<grid1(nentries), 256, 0, s>>>(nentries, d_gid, d_sp, side, pstride, R, p,
+                                                       U, ly.sg, ly.sc);
         HIFDE_LAUNCH_CHECK();
     }
 }
@@ -313,25 +385,25 @@ void axpy_cols(long long n, const double* d_a, const double* X, long long ldx, d
     HIFDE_LAUNCH_CHECK();
 }
 
-template void rhs_gather<float>(int, long long, const int*, const int*, const double*, long long,
+template void rhs_gather<float>(int, long long, const int*, const int*, const double*, RhsLayout,
                                 int, float*, hipStream_t);
-template void rhs_gather<double>(int, long long, const int*, const int*, const double*, long long,
+template void rhs_gather<double>(int, long long, const int*, const int*, const double*, RhsLayout,
                                  int, double*, hipStream_t);
 template void apply_multi_fused<float>(int, int, const int*, const int*, const int*, const int*,
                                        const int*, const int*, const int*, const unsigned char*,
                                        const long long*, const float*, int, double, double*,
-                                       long long, int, double*, long long, double, double,
+                                       RhsLayout, int, double*, long long, double, double,
                                        hipStream_t);
 template void apply_multi_fused<double>(int, int, const int*, const int*, const int*, const int*,
                                         const int*, const int*, const int*, const unsigned char*,
                                         const long long*, const double*, int, double, double*,
-                                        long long, int, double*, long long, double, double,
+                                        RhsLayout, int, double*, long long, double, double,
                                         hipStream_t);
 template void rhs_scatter<float>(int, long long, const int*, const int*, const float*, int,
-                                 double*, long long, double, int, const unsigned char*, long long,
+                                 double*, RhsLayout, double, int, const unsigned char*, long long,
                                  hipStream_t);
 template void rhs_scatter<double>(int, long long, const int*, const int*, const double*, int,
-                                  double*, long long, double, int, const unsigned char*, long long,
+                                  double*, RhsLayout, double, int, const unsigned char*, long long,
                                   hipStream_t);
 
 }  // namespace hifde
