@@ -1051,6 +1051,175 @@ __global__ void __launch_bounds__(256) k_gram_panel(
     }
 }
 
+// The same panel with the panel rows held in registers: thread t owns columns
+// c = t + u BS (u < CPT) and keeps their kGramNB panel entries and remaining
+// diagonal there, so a pivot step reads only G(p, c) from memory and the
+// pivot column's earlier entries from LDS.  The arithmetic per column is the
+// same sequence of operations as in k_gram_panel, so both give the same bits.
+// Pivot choice: the largest remaining diagonal, ties to the lowest position.
+// Faces need n <= BS CPT.  Dynamic shared memory: jp[n], pinv[n] (int).
+template <int BS, int CPT>
+__global__ void __launch_bounds__(BS) k_gram_panel_reg(
+    const int* __restrict__ dn, const unsigned char* __restrict__ mode, const double* Gb,
+    const long long* __restrict__ offG, double* Rb, double* Pb, const long long* __restrict__ offP,
+    double* dgb, int* perm, const int* __restrict__ perm_off, int* kout, int* active, double* tolv,
+    double eps, int j0) {
+    constexpr int NB = kGramNB;
+    constexpr int NW = BS / kWave;
+    const int b = blockIdx.x;
+    if (!mode[b] || !active[b]) return;
+    const int n = dn[b];
+    const double* G = Gb + offG[b];
+    double* R = Rb + offG[b];
+    double* P = Pb + offP[b];
+    double* dg = dgb + perm_off[b];
+    int* jpg = perm + perm_off[b];
+    extern __shared__ unsigned char smem[];
+    int* jp = reinterpret_cast<int*>(smem);
+    int* pinv = jp + n;
+    __shared__ double prow[NB];
+    __shared__ double redv[NW];
+    __shared__ int redk[NW];
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
+    for (int q = tid; q < n; q += BS) {
+        const int c = j0 == 0 ? q : jpg[q];
+        jp[q] = c;
+        pinv[c] = q;
+    }
+    __syncthreads();
+    double pc[CPT][NB];
+    double d[CPT];
+    bool live[CPT];
+#pragma unroll
+    for (int u = 0; u < CPT; ++u) {
+        const int c = tid + u * BS;
+        live[u] = c < n && pinv[c] >= j0;
+        d[u] = c < n ? (j0 == 0 ? G[c + (size_t)c * n] : dg[c]) : 0.0;
+#pragma unroll
+        for (int l = 0; l < NB; ++l) pc[u][l] = 0.0;
+    }
+    double tol = j0 == 0 ? 0.0 : tolv[b];
+    int stop = -1;
+    const int i1 = min(j0 + NB, n);
+#pragma unroll 1
+    for (int li = 0; li < NB; ++li) {
+        const int i = j0 + li;
+        if (i >= i1) break;
+        // candidate: largest d, then lowest position; key = position << 16 | column
+        double best = -1.0;
+        unsigned key = 0xffffffffu;
+#pragma unroll
+        for (int u = 0; u < CPT; ++u) {
+            const int c = tid + u * BS;
+            if (!live[u]) continue;
+            const unsigned k = ((unsigned)pinv[c] << 16) | (unsigned)c;
+            if (d[u] > best || (d[u] == best && k < key)) {
+                best = d[u];
+                key = k;
+            }
+        }
+#pragma unroll
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+            const double ov = __shfl_xor(best, o, kWave);
+            const unsigned ok = (unsigned)__shfl_xor((int)key, o, kWave);
+            if (ov > best || (ov == best && ok < key)) {
+                best = ov;
+                key = ok;
+            }
+        }
+        if (lane == 0) {
+            redv[wid] = best;
+            redk[wid] = (int)key;
+        }
+        __syncthreads();
+        best = redv[0];
+        key = (unsigned)redk[0];
+#pragma unroll
+        for (int w = 1; w < NW; ++w) {
+            const double ov = redv[w];
+            const unsigned ok = (unsigned)redk[w];
+            if (ov > best || (ov == best && ok < key)) {
+                best = ov;
+                key = ok;
+            }
+        }
+        const double rii = sqrt(best > 0.0 ? best : 0.0);
+        if (i == 0) tol = rii * eps;
+        if (!(rii > tol)) {
+            stop = i;
+            break;
+        }
+        const int pos = (int)(key >> 16), p = (int)(key & 0xffffu);
+        // G(p, c) from the lower triangle, issued before the pivot row is shared
+        double g[CPT];
+#pragma unroll
+        for (int u = 0; u < CPT; ++u) {
+            const int c = tid + u * BS;
+            g[u] = 0.0;
+            if (live[u] && c != p) g[u] = p >= c ? G[p + (size_t)c * n] : G[c + (size_t)p * n];
+        }
+        // The pivot column's owner shares its panel entries (zero from
+        // position li on, so the sums below may run over the whole panel).
+        if (tid == (p % BS)) {
+            const int up = p / BS;
+#pragma unroll
+            for (int u = 0; u < CPT; ++u) {
+                if (u != up) continue;
+#pragma unroll
+                for (int l = 0; l < NB; ++l) {
+                    prow[l] = pc[u][l];
+                    if (l == li) pc[u][l] = rii;
+                }
+                live[u] = false;
+            }
+            R[(size_t)i * n + p] = rii;
+        }
+        if (tid == 0) {
+            const int c = jp[i];
+            jp[pos] = c;
+            pinv[c] = pos;
+            jp[i] = p;
+            pinv[p] = i;
+        }
+        __syncthreads();
+        const double inv = 1.0 / rii;
+#pragma unroll
+        for (int u = 0; u < CPT; ++u) {
+            if (!live[u]) continue;
+            const int c = tid + u * BS;
+            double v = g[u];
+            // terms l >= li are 0 * 0 and leave v unchanged
+#pragma unroll
+            for (int l = 0; l < NB; ++l) v -= prow[l] * pc[u][l];
+            v *= inv;
+            R[(size_t)i * n + c] = v;
+#pragma unroll
+            for (int l = 0; l < NB; ++l)
+                if (l == li) pc[u][l] = v;
+            const double nd = d[u] - v * v;
+            d[u] = nd > 0.0 ? nd : 0.0;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < CPT; ++u) {
+        const int c = tid + u * BS;
+        if (c >= n) continue;
+        dg[c] = d[u];
+        double* pcol = P + (size_t)c * NB;
+#pragma unroll
+        for (int l = 0; l < NB; ++l) pcol[l] = pc[u][l];
+    }
+    for (int q = tid; q < n; q += BS) jpg[q] = jp[q];
+    if (tid == 0) {
+        tolv[b] = tol;
+        if (stop >= 0 || i1 == n) {
+            kout[b] = stop >= 0 ? stop : n;
+            active[b] = 0;
+        }
+    }
+}
+
 // [R11 | R12] (k x n, ld k) of each Gram face from the rows of R in pivot
 // order: R11(i, j) = R(i, jp[j]) for j >= i, zero below the diagonal.
 __global__ void k_gram_gather_r(int nface, const long long* __restrict__ toff,
@@ -1654,10 +1823,22 @@ void gram_pivchol(int nface, const int* h_n, const unsigned char* h_mode, const 
     ws.plan.clear();
     const std::pair<int, int> uh = ws.plan.add(upd, true);
     ws.plan.upload(s);
+    const size_t shm_reg = (size_t)max_n * 2 * sizeof(int);
     for (int j0 = 0; j0 < max_n; j0 += kGramNB) {
-        k_gram_panel<<<nface, 256, shm, s>>>(d_n, d_mode, d_G, d_offG, d_R, ws.P.p, ws.offP.p,
-                                             ws.dg.p, d_perm, d_perm_off, d_k, ws.active.p,
-                                             ws.tol.p, eps, j0);
+#define HIFDE_GRAM_REG(BS, CPT)                                                                   \
+    k_gram_panel_reg<BS, CPT><<<nface, BS, shm_reg, s>>>(                                         \
+        d_n, d_mode, d_G, d_offG, d_R, ws.P.p, ws.offP.p, ws.dg.p, d_perm, d_perm_off, d_k,       \
+        ws.active.p, ws.tol.p, eps, j0)
+        if (!ws.reg_panel || max_n > 1536)
+            k_gram_panel<<<nface, 256, shm, s>>>(d_n, d_mode, d_G, d_offG, d_R, ws.P.p, ws.offP.p,
+                                                 ws.dg.p, d_perm, d_perm_off, d_k, ws.active.p,
+                                                 ws.tol.p, eps, j0);
+        else if (max_n <= 64) HIFDE_GRAM_REG(64, 1);
+        else if (max_n <= 256) HIFDE_GRAM_REG(256, 1);
+        else if (max_n <= 512) HIFDE_GRAM_REG(512, 1);
+        else if (max_n <= 1024) HIFDE_GRAM_REG(512, 2);
+        else HIFDE_GRAM_REG(512, 3);
+#undef HIFDE_GRAM_REG
         HIFDE_LAUNCH_CHECK();
         if (j0 + kGramNB >= max_n) break;
         gemm_run<double, double>(ws.plan, uh, ws.P.p, ws.P.p, d_G, -1.0, 1.0, s, ws.active.p);

@@ -2489,6 +2489,7 @@ int hifde_set_option(hifde_factor* f, const char* name, double value) {
         else if (n == "symmetric") f->sym_option = (int)value;
         else if (n == "elements") f->elem_option = (int)value;
         else if (n == "stage_sync") f->stage_sync = value != 0;
+        else if (n == "gram_panel_reg") f->gram_ws.reg_panel = value != 0;
         else if (n == "multi_fused_max") {
             f->multi_fused_max = (int)value;
             f->clear_graphs();

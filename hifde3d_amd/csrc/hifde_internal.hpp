@@ -330,6 +330,9 @@ struct GramWork {
     DevBuf<int> active, count, kd, info;
     DevBuf<GramRDesc> rdesc;
     std::vector<int> host_active;
+    // panel rows in registers (faces up to 1536 columns); false takes the
+    // LDS/global-memory panel for every size
+    bool reg_panel = true;
 };
 void gram_pivchol(int nface, const int* h_n, const unsigned char* h_mode, const int* d_n,
                   const unsigned char* d_mode, double* d_G, const long long* h_offG,

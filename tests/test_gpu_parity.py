@@ -286,6 +286,26 @@ def test_gram_id_path_matches_oracle(H, oracle_mod, eps):
     assert _rel(A @ f.Apply(b, True), b) <= 2 * _rel(A @ o.apply(b, True), b)
 
 
+def test_gram_register_panel_bitwise_equal_to_lds_panel(H):
+    """The register-resident pivoted-Cholesky panel (every size class: faces of
+    49, 225, ~690 and ~1280 columns at n=128) against the LDS/global-memory
+    panel: the same pivots, ranks and factor bits."""
+    NC = 128
+    a = pb.pow10_coeff(NC, 5)
+    V = pb.zeros_coeff(NC)
+    b = pb.rhs_normal(NC, 9)
+    out = []
+    for reg in (1, 0):
+        f = H.HIFFactor(NC - 1, 4, 1e-4)
+        f.set_option("gram_panel_reg", reg)
+        f.set_stencil(a, V, True)
+        assert f.Factor() == 0
+        out.append((f.level_counts(), f.Apply(b, True)))
+        f.close()
+    assert out[0][0] == out[1][0]
+    assert np.array_equal(out[0][1], out[1][1])
+
+
 def test_cpp_dropin_driver_reference_protocol(H, oracle_mod):
     """tests/cpp/hif_driver (the reference driver protocol written against
     include/hifde3d_amd.hpp) against the oracle's e_a / e_s."""
