@@ -37,14 +37,13 @@ void set_active_timer(KernelTimer* t) { t_timer = t; }
 const void* Stager::stage(const void* src, size_t bytes) {
     const size_t need = (bytes + 255) & ~(size_t)255;
     if (used + need > cap) {
-        HIFDE_CHECK(hipStreamSynchronize(stream));
+        // Continue in a larger buffer instead of waiting for the stream: the
+        // full one is kept until the next synchronization.
+        if (buf) retired.push_back(buf);
+        buf = nullptr;
         used = 0;
-        if (need > cap) {
-            if (buf) HIFDE_CHECK(hipHostFree(buf));
-            buf = nullptr;
-            cap = std::max(need, std::max(2 * cap, (size_t)64 << 20));
-            HIFDE_CHECK(hipHostMalloc(reinterpret_cast<void**>(&buf), cap));
-        }
+        cap = std::max(need, std::max(2 * cap, (size_t)64 << 20));
+        HIFDE_CHECK(hipHostMalloc(reinterpret_cast<void**>(&buf), cap));
     }
     if (bytes >= ((size_t)4 << 20)) {
         // large uploads: the copy into pinned memory is split over the host cores

@@ -33,14 +33,23 @@ struct Stager {
     unsigned char* buf = nullptr;
     size_t cap = 0, used = 0;
     hipStream_t stream = nullptr;
+    // buffers outgrown since the last synchronization (copies from them may
+    // still be in flight); freed by synced()
+    std::vector<unsigned char*> retired;
     Stager() = default;
     Stager(const Stager&) = delete;
     Stager& operator=(const Stager&) = delete;
     ~Stager() {
+        for (unsigned char* p : retired) (void)hipHostFree(p);
         if (buf) (void)hipHostFree(buf);
     }
     const void* stage(const void* src, size_t bytes);
-    void synced() { used = 0; }
+    // the stream has been synchronized: every staged copy has completed
+    void synced() {
+        used = 0;
+        for (unsigned char* p : retired) (void)hipHostFree(p);
+        retired.clear();
+    }
 };
 // Stager of the calling thread's current factorization (null: plain copies).
 Stager* active_stager();
