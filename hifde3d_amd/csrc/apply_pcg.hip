@@ -20,6 +20,7 @@ namespace hifde {
 namespace {
 
 constexpr int kWave = 64;
+constexpr int kXcds = 8;
 
 __device__ __forceinline__ double wave_sum(double v) {
     for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
@@ -54,7 +55,12 @@ __global__ void __launch_bounds__(256) k_apply_tiles(
     const int* __restrict__ skel_gid, const unsigned char* __restrict__ skel_pass,
     const long long* __restrict__ off, const T* __restrict__ Mb, int op, int pass, double alpha,
     const double* u_in, double* u_out, long long side_n) {
-    const int t = blockIdx.x;
+    // Workgroups are dealt to the 8 XCDs round-robin; consecutive tiles (the
+    // row tiles of one box, whose boundary cache lines they share) are given
+    // to workgroups of the same XCD, so those lines are fetched into one L2.
+    const int nb = (int)gridDim.x, per = nb / kXcds, rem = nb % kXcds;
+    const int x = (int)blockIdx.x % kXcds;
+    const int t = x * per + min(x, rem) + (int)blockIdx.x / kXcds;
     const int b = find_box(tile_off, nbox, t);
     const int tile = t - tile_off[b];
     const int r = dr[b], s = ds[b];
@@ -83,17 +89,30 @@ __global__ void __launch_bounds__(256) k_apply_tiles(
         const int i = tid & 63, g = tid >> 6;
         double pacc = 0.0;
         if (i < cnt) {
-            // eight independent sums keep eight loads per thread in flight
+            // sixteen independent sums keep sixteen loads per thread in
+            // flight; past the last column the loads repeat it (cached) and
+            // are weighted by zero, so the loop has no branches
             const T* Mi = M + o0 + i;
-            double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-            int j = g;
-            for (; j + 28 < nin; j += 32) {
+            double a[16];
 #pragma unroll
-                for (int u = 0; u < 8; ++u)
-                    a[u] += (double)Mi[(size_t)(j + 4 * u) * nout] * xs[j + 4 * u];
+            for (int u = 0; u < 16; ++u) a[u] = 0.0;
+            for (int j = g; j < nin; j += 64) {
+                T mv[16];
+#pragma unroll
+                for (int u = 0; u < 16; ++u)
+                    mv[u] = Mi[(size_t)min(j + 4 * u, nin - 1) * nout];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) {
+                    const int jj = j + 4 * u;
+                    const double xv = xs[min(jj, nin - 1)];
+                    a[u] += (double)mv[u] * (jj < nin ? xv : 0.0);
+                }
             }
-            for (; j < nin; j += 4) a[0] += (double)Mi[(size_t)j * nout] * xs[j];
-            pacc = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+#pragma unroll
+            for (int w = 8; w > 0; w >>= 1)
+#pragma unroll
+                for (int u = 0; u < w; ++u) a[u] += a[u + w];
+            pacc = a[0];
         }
         part[g][i] = pacc;
         __syncthreads();
@@ -129,15 +148,23 @@ __global__ void __launch_bounds__(256) k_apply_tiles(
             cmask |= (c ? 1u : 0u) << q;
         }
         double pacc = 0.0;
+        // the next 64-row block is loaded while the current one is summed
+        T v[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int jj = g + 4 * q;
+            v[q] = (oo < nin && ((cmask >> q) & 1u)) ? M[(size_t)(o0 + jj) * nin + oo] : (T)0;
+        }
         for (int i0 = 0; i0 < nin; i0 += 64) {
             const int ib = min(64, nin - i0);
-            const bool row_ok = oo < ib;
-            T v[16];
+            const int i1 = i0 + 64;
+            const bool next_ok = oo < nin - i1;
+            T vn[16];
 #pragma unroll
             for (int q = 0; q < 16; ++q) {
                 const int jj = g + 4 * q;
-                v[q] = (row_ok && ((cmask >> q) & 1u)) ? M[(size_t)(o0 + jj) * nin + i0 + oo]
-                                                       : (T)0;
+                vn[q] = (next_ok && ((cmask >> q) & 1u)) ? M[(size_t)(o0 + jj) * nin + i1 + oo]
+                                                         : (T)0;
             }
 #pragma unroll
             for (int q = 0; q < 16; ++q) blk[g + 4 * q][oo] = v[q];
@@ -148,6 +175,8 @@ __global__ void __launch_bounds__(256) k_apply_tiles(
                 if (ii < ib) pacc += (double)blk[oo][ii] * xs[i0 + ii];
             }
             __syncthreads();
+#pragma unroll
+            for (int q = 0; q < 16; ++q) v[q] = vn[q];
         }
         part[g][oo] = pacc;
         __syncthreads();
