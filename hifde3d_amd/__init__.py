@@ -347,9 +347,13 @@ def gemm_batch(pairs, precision="fp64", transA=False):
     return out
 
 
-def schur_batch(blocks, precision="fp64"):
-    """Schur on a list of (A22, A21, A12).  Returns [(A22inv, X, S, info)]."""
-    prec = {"fp64": FP64, "mixed": MIXED}[precision]
+SCHUR_SYMMETRIC = 0x100
+
+
+def schur_batch(blocks, precision="fp64", symmetric=False):
+    """Schur on a list of (A22, A21, A12).  Returns [(A22inv, X, S, info)].
+    symmetric: every A22 is symmetric (only its lower triangle is read)."""
+    prec = {"fp64": FP64, "mixed": MIXED}[precision] | (SCHUR_SYMMETRIC if symmetric else 0)
     r = np.array([b[0].shape[0] for b in blocks], np.int32)
     s = np.array([b[1].shape[1] for b in blocks], np.int32)
 

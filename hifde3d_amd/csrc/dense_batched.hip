@@ -1008,6 +1008,40 @@ __global__ void __launch_bounds__(256) k_copy2d(const Copy2D* __restrict__ desc,
             dst[d.dst + i + (size_t)c * d.ldd] = src[d.src + i + (size_t)c * d.lds];
 }
 
+// SymCopy descriptors over a flattened list of 64 x 64 destination tiles.
+// Transposed tiles pass through LDS so that both the reads and the writes
+// run along columns.
+template <class T>
+__global__ void __launch_bounds__(256) k_sym_copy(int ndesc, const long long* __restrict__ toff,
+                                                  const SymCopy* __restrict__ desc, T* A, T* SW) {
+    const long long blk = blockIdx.x;
+    const int b = find_box(toff, ndesc, blk);
+    const SymCopy d = desc[b];
+    const int tm = (d.rows + 63) / 64;
+    const int local = (int)(blk - toff[b]);
+    const int i0 = (local % tm) * 64, j0 = (local / tm) * 64;
+    if (d.upper && i0 >= j0 + 63) return;  // no element with i < j
+    T* dst = (d.dsel ? SW : A) + d.dst;
+    const T* src = (d.ssel ? SW : A) + d.src;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    __shared__ T t[64][65];
+    if (d.trans) {
+        // t[i - i0][j - j0] = src(j, i), read along src's columns
+        for (int c = ty; c < 64; c += 4) {
+            const int j = j0 + tx, i = i0 + c;
+            t[c][tx] = (j < d.cols && i < d.rows) ? src[j + (size_t)i * d.lds] : (T)0;
+        }
+        __syncthreads();
+    }
+    for (int c = ty; c < 64; c += 4) {
+        const int i = i0 + tx, j = j0 + c;
+        if (i >= d.rows || j >= d.cols || (d.upper && i >= j)) continue;
+        const T v = d.trans ? t[tx][c] : src[i + (size_t)j * d.lds];
+        const int sg = j >= d.neg_from ? -d.s0 : d.s0;
+        dst[i + (size_t)j * d.ldd] = sg > 0 ? v : -v;
+    }
+}
+
 // dst(box) = (D) src(rows r0.., cols c0.. of a ld-strided block); dst packed
 // column-major with ld = rows.
 template <class D>
@@ -1549,7 +1583,7 @@ int grid_for(long long n, int block = 256, int cap = 1 << 16) {
 template <class T>
 void batched_gj_inverse(int nbox, const int* h_n, const long long* h_off, const int* d_n,
                         const long long* d_off, T* d_A, int* d_info, GJWork& ws, GemmWork& gws,
-                        hipStream_t s) {
+                        hipStream_t s, bool sym) {
     if (nbox <= 0) return;
     // Boxes of at most kSmallGJ rows: one workgroup each.  Larger boxes are
     // listed in slots 0..nbig-1 for the blocked path.
@@ -1599,12 +1633,24 @@ void batched_gj_inverse(int nbox, const int* h_n, const long long* h_off, const 
     k_fill_int<<<(nbig + 255) / 256, 256, 0, s>>>(ws.info.p, nbig, 0);
     ws.B.reserve((size_t)nbig * kGJB * kGJPanel * sizeof(T));
     ws.D.reserve((size_t)nbig * kGJB * kGJB * sizeof(T));
-    ws.RO.reserve((size_t)std::max<long long>(ro_off[nbig], 1) * sizeof(T));
-    ws.CW.reserve((size_t)std::max<long long>(cw_off[nbig], 1) * sizeof(T));
+    // symmetric outer level: per box V and W (n x kGJPanel, ld n), then V^T
+    // (kGJPanel x n); the general one: RO and CW
+    std::vector<long long> sw_off(nbig + 1, 0);
+    for (int j = 0; j < nbig; ++j) {
+        const int n = h_n[bix[j]];
+        sw_off[j + 1] = sw_off[j] + (sym && n > kGJPanel ? 3LL * n * kGJPanel : 0);
+    }
+    if (sym) {
+        ws.SW.reserve((size_t)std::max<long long>(sw_off[nbig], 1) * sizeof(T));
+    } else {
+        ws.RO.reserve((size_t)std::max<long long>(ro_off[nbig], 1) * sizeof(T));
+        ws.CW.reserve((size_t)std::max<long long>(cw_off[nbig], 1) * sizeof(T));
+    }
     T* Rws = reinterpret_cast<T*>(ws.B.p);
     T* Dws = reinterpret_cast<T*>(ws.D.p);
     T* RO = reinterpret_cast<T*>(ws.RO.p);
     T* CW = reinterpret_cast<T*>(ws.CW.p);
+    T* SW = reinterpret_cast<T*>(ws.SW.p);
     int* cinfo = ws.info.p;
     // The same walk runs twice: first it records every GEMM and strided copy
     // (uploaded in one copy each), then it launches them in order between
@@ -1615,8 +1661,35 @@ void batched_gj_inverse(int nbox, const int* h_n, const long long* h_off, const 
     std::vector<Copy2D> cps;
     std::vector<int> cfirst, cnum, cmaxc;  // per outer step: copy range and widest copy
     std::vector<GemmDesc> upd, g1, g2, g3;
+    // symmetric-path copies: recorded as launch ranges in the first walk
+    std::vector<SymCopy> scps;
+    std::vector<long long> stoff;
+    struct SRange {
+        int d0, n;
+        long long t0, tiles;
+    };
+    std::vector<SRange> srng;
+    constexpr int kAll = 1 << 30;  // neg_from: no column negated
     for (int exec = 0; exec < 2; ++exec) {
-        size_t gi = 0;
+        size_t gi = 0, si = 0;
+        auto scopy = [&](const std::vector<SymCopy>& d) {
+            if (!exec) {
+                SRange r{(int)scps.size(), (int)d.size(), (long long)stoff.size(), 0};
+                stoff.push_back(0);
+                for (const SymCopy& c : d) {
+                    r.tiles += (long long)((c.rows + 63) / 64) * ((c.cols + 63) / 64);
+                    stoff.push_back(r.tiles);
+                    scps.push_back(c);
+                }
+                srng.push_back(r);
+                return;
+            }
+            const SRange& r = srng[si++];
+            if (r.n == 0 || r.tiles == 0) return;
+            k_sym_copy<T><<<(unsigned)r.tiles, 256, 0, s>>>(r.n, ws.stile.p + r.t0,
+                                                            ws.sdesc.p + r.d0, d_A, SW);
+            HIFDE_LAUNCH_CHECK();
+        };
         auto gemm = [&](const std::vector<GemmDesc>& d, const T* A, const T* B, T* C, T alpha,
                         T beta) {
             if (!exec) gh.push_back(plan.add(d, false));
@@ -1630,6 +1703,19 @@ void batched_gj_inverse(int nbox, const int* h_n, const long long* h_off, const 
             const long long* dwoff = ws.woff.p + w0;
             int max_w = 0;
             for (int j = 0; j < nbig; ++j) max_w = std::max(max_w, wn[w0 + j]);
+            if (sym) {
+                // only lower triangles are current: the window's upper
+                // triangle is rebuilt from its lower one before inversion
+                std::vector<SymCopy> mir;
+                for (int j = 0; j < nbig && !exec; ++j) {
+                    const int kb = wn[w0 + j];
+                    if (kb == 0) continue;
+                    const long long wo = woff[w0 + j];
+                    const int ld = wld[w0 + j];
+                    mir.push_back({wo, wo, kb, kb, ld, ld, 1, 1, kAll, 1, 0, 0});
+                }
+                scopy(mir);
+            }
             // inner level: every window inverted in place
             const int isteps = (max_w + kGJB - 1) / kGJB;
             const dim3 gch((max_w + 63) / 64, nbig);
@@ -1661,6 +1747,57 @@ void batched_gj_inverse(int nbox, const int* h_n, const long long* h_off, const 
                         dwn, dwoff, dwld, d_A, Rws, ws.b_off.p, st, cinfo);
                     k_bgj_col<T><<<gch, 256, 0, s>>>(dwn, dwoff, dwld, d_A, Dws, st, cinfo);
                 }
+            }
+            if (sym) {
+                // Symmetric outer level.  With P the windows already
+                // processed, the in-place iterate Z keeps Z_PP and Z_QQ
+                // symmetric and Z_QP = -Z_PQ^T, so its lower triangle holds
+                // everything.  Step K with V = Z(:, K) (rows != K; rows in P
+                // read as -Z(K, .)^T) and W = V D:
+                //   Z_IJ -= s_J W_I V_J^T    (I >= J; s_J = -1 in P, +1 after K)
+                //   Z(I, K) = -W_I,  Z(K, J) = -W_J^T  (J in P)
+                // which is the rank-kb update of the general level on the lower
+                // triangle only.
+                std::vector<SymCopy> gv, vt, wb;
+                std::vector<GemmDesc> gw, su;
+                for (int j = 0; j < nbig && !exec; ++j) {
+                    const int n = h_n[bix[j]];
+                    const int kb = wn[w0 + j];
+                    if (kb == 0 || n <= kGJPanel) continue;
+                    const long long o = h_off[bix[j]];
+                    const long long sv = sw_off[j], swo = sv + (long long)n * kGJPanel,
+                                    svt = swo + (long long)n * kGJPanel;
+                    const int K1 = K0 + kb, nq = n - K1, ldv = n;
+                    if (nq > 0)
+                        gv.push_back({sv + K0, o + K1 + (long long)K0 * n, nq, kb, ldv, n, 0, 0,
+                                      kAll, 1, 1, 0});
+                    if (K0 > 0) gv.push_back({sv, o + K0, K0, kb, ldv, n, 1, 0, 0, 1, 1, 0});
+                    gw.push_back({sv, o + K0 + (long long)K0 * n, swo, n - kb, kb, kb, ldv, n,
+                                  ldv});
+                    vt.push_back({svt, sv, kb, n - kb, kGJPanel, ldv, 1, 0, K0, 1, 1, 1});
+                    if (K0 > 0) {
+                        GemmDesc tl{swo, svt, o, K0, K0, kb, ldv, kGJPanel, n};
+                        tl.lower = 1;
+                        su.push_back(tl);
+                    }
+                    if (K0 > 0 && nq > 0)
+                        su.push_back({swo + K0, svt, o + K1, nq, K0, kb, ldv, kGJPanel, n});
+                    if (nq > 0) {
+                        GemmDesc br{swo + K0, svt + (long long)K0 * kGJPanel,
+                                    o + K1 + (long long)K1 * n, nq, nq, kb, ldv, kGJPanel, n};
+                        br.lower = 1;
+                        su.push_back(br);
+                        wb.push_back({o + K1 + (long long)K0 * n, swo + K0, nq, kb, n, ldv, 0, 0,
+                                      0, 1, 0, 1});
+                    }
+                    if (K0 > 0) wb.push_back({o + K0, swo, kb, K0, n, ldv, 1, 0, 0, 1, 0, 1});
+                }
+                scopy(gv);
+                gemm(gw, SW, d_A, SW, (T)1, (T)0);
+                scopy(vt);
+                gemm(su, SW, SW, d_A, (T)1, (T)1);
+                scopy(wb);
+                continue;
             }
             // outer level, boxes wider than one window
             g1.clear();
@@ -1714,9 +1851,24 @@ void batched_gj_inverse(int nbox, const int* h_n, const long long* h_off, const 
                 HIFDE_LAUNCH_CHECK();
             }
         }
+        if (sym) {
+            // every window processed: Z = A^{-1}, symmetric
+            std::vector<SymCopy> mir;
+            for (int j = 0; j < nbig && !exec; ++j) {
+                const int n = h_n[bix[j]];
+                if (n <= kGJPanel) continue;
+                const long long o = h_off[bix[j]];
+                mir.push_back({o, o, n, n, n, n, 1, 1, kAll, 1, 0, 0});
+            }
+            scopy(mir);
+        }
         if (!exec) {
             plan.upload(s);
             ws.cdesc.upload(cps, s);
+            if (sym) {
+                ws.sdesc.upload(scps, s);
+                ws.stile.upload(stoff, s);
+            }
         }
     }
     k_scatter_info<<<(nbig + 255) / 256, 256, 0, s>>>(nbig, ws.bix.p, cinfo, d_info);
@@ -2048,10 +2200,10 @@ void convert_d2f(const double* src, float* dst, long long n, hipStream_t s) {
 
 template void batched_gj_inverse<double>(int, const int*, const long long*, const int*,
                                          const long long*, double*, int*, GJWork&, GemmWork&,
-                                         hipStream_t);
+                                         hipStream_t, bool);
 template void batched_gj_inverse<float>(int, const int*, const long long*, const int*,
                                         const long long*, float*, int*, GJWork&, GemmWork&,
-                                        hipStream_t);
+                                        hipStream_t, bool);
 
 template void gemm_run<double, double>(const GemmPlan&, std::pair<int, int>, const double*,
                                        const double*, double*, double, double, hipStream_t,

@@ -159,6 +159,7 @@ struct hifde_factor {
     DevBuf<double> ws_G;
     DevBuf<SliceSum> d_gsum;
     static constexpr long long kGramSplitTiles = 1536;  // three waves of 512 workgroups
+    bool sym_gj = true;           // symmetric level matrix: half-flop block inverses
     int gram_min_n = 16;          // faces with more columns use the Gram path
     double gram_min_eps = 1e-7;   // below this tolerance every face uses Householder
     DevBuf<double> pcg_buf, pcg_part, pcg_scalar, apply_tmp, apply_side;
@@ -629,8 +630,9 @@ struct hifde_factor {
         }
         {
             Prof pr(this, fam<T>("gj_inverse"), fl_inv, by_inv);
+            // with the symmetric level matrix every A22 is symmetric
             batched_gj_inverse<T>(nb, B.r.data(), B.off_rr.data(), B.d_r.p, B.d_off_rr.p, Ainv,
-                                  ws_info.p, gj_ws, gemm_ws, stream);
+                                  ws_info.p, gj_ws, gemm_ws, stream, sym_on && sym_gj);
         }
         fold_info(nb, ws_info.p, d_status.p, stream);
         timed(ST_INVERSE, t0);
@@ -2489,6 +2491,7 @@ int hifde_set_option(hifde_factor* f, const char* name, double value) {
         else if (n == "elements") f->elem_option = (int)value;
         else if (n == "stage_sync") f->stage_sync = value != 0;
         else if (n == "gram_panel_reg") f->gram_ws.reg_panel = value != 0;
+        else if (n == "symmetric_inverse") f->sym_gj = value != 0;
         else if (n == "multi_fused_max") {
             f->multi_fused_max = (int)value;
             f->clear_graphs();
@@ -2598,7 +2601,8 @@ int hifde_interp_decomp_batch(int nbatch, const int* m, const int* n, const doub
 namespace {
 template <class T>
 int schur_batch_impl(int nb, const int* r, const int* s, const double* A22, const double* A21,
-                     const double* A12, double* A22inv, double* X, double* S, int* info) {
+                     const double* A12, double* A22inv, double* X, double* S, int* info,
+                     bool sym) {
     hipStream_t st = nullptr;
     std::vector<int> hr(r, r + nb), hs(s, s + nb);
     std::vector<long long> orr = prefix_ll(hr, hr), ors = prefix_ll(hr, hs), oss = prefix_ll(hs, hs);
@@ -2625,7 +2629,8 @@ int schur_batch_impl(int nb, const int* r, const int* s, const double* A22, cons
     HIFDE_CHECK(hipMemset(dinfo.p, 0, nb * sizeof(int)));
     GJWork gw;
     GemmWork gws;
-    batched_gj_inverse<T>(nb, hr.data(), orr.data(), dr.p, drr.p, dA.p, dinfo.p, gw, gws, st);
+    batched_gj_inverse<T>(nb, hr.data(), orr.data(), dr.p, drr.p, dA.p, dinfo.p, gw, gws, st,
+                          sym);
     batched_gemm<T>(false, nb, hr.data(), hs.data(), hr.data(), orr.data(), ors.data(), ors.data(),
                     dA.p, dA21.p, dX.p, (T)1, gws, st);
     batched_gemm<T>(false, nb, hs.data(), hs.data(), hr.data(), ors.data(), ors.data(), oss.data(),
@@ -2695,9 +2700,10 @@ int hifde_schur_batch(int precision, int nbatch, const int* r, const int* s, con
                                       !info)))
             throw Error(HIFDE_ERR_ARG, "schur_batch: bad arguments");
         if (nbatch == 0) return HIFDE_OK;
-        if (precision == HIFDE_MIXED)
-            return schur_batch_impl<float>(nbatch, r, s, A22, A21, A12, A22inv, X, S, info);
-        return schur_batch_impl<double>(nbatch, r, s, A22, A21, A12, A22inv, X, S, info);
+        const bool sym = (precision & HIFDE_SCHUR_SYMMETRIC) != 0;
+        if ((precision & ~HIFDE_SCHUR_SYMMETRIC) == HIFDE_MIXED)
+            return schur_batch_impl<float>(nbatch, r, s, A22, A21, A12, A22inv, X, S, info, sym);
+        return schur_batch_impl<double>(nbatch, r, s, A22, A21, A12, A22inv, X, S, info, sym);
     });
 }
 

@@ -289,6 +289,14 @@ struct GemmWork {
     DevBuf<CopyDesc> cdesc;
     DevBuf<long long> off;
 };
+// dst(i, j) = sign(j) src(i, j), or src(j, i) when trans, for i < rows and
+// j < cols (only i < j when upper); sign(j) = -s0 for j >= neg_from, s0
+// before.  sel: 0 the matrices, 1 the symmetric-path workspace.
+struct SymCopy {
+    long long dst, src;
+    int rows, cols, ldd, lds;
+    int trans, upper, neg_from, s0, dsel, ssel;
+};
 struct GJWork {
     DevBuf<int> wn, wld;              // per outer step and box: window size, leading dim
     DevBuf<long long> woff, b_off;    // window offsets; inner R offsets
@@ -297,6 +305,9 @@ struct GJWork {
     DevBuf<Copy2D> cdesc;
     DevBuf<int> bix, info;            // blocked-path boxes and their status
     GemmPlan plan;                    // every GEMM of one inversion
+    DevBuf<unsigned char> SW;         // symmetric path: V, W (n x kGJPanel), V^T per box
+    DevBuf<SymCopy> sdesc;
+    DevBuf<long long> stile;          // per copy launch: tile prefix of its descriptors
 };
 
 // C = alpha op(A) B + beta C per descriptor (op(A) = A^T when transA).  C
@@ -313,10 +324,13 @@ void batched_gemm(bool transA, int nbox, const int* h_m, const int* h_n, const i
 template <class D>
 void copy_blocks(const std::vector<CopyDesc>& desc, const std::vector<long long>& dst_off,
                  const double* d_src, D* d_dst, GemmWork& ws, hipStream_t s);
+// In-place inverses of the boxes (n x n, ld n).  sym: every box is symmetric;
+// boxes wider than one window then read only their lower triangle and the
+// outer level updates only lower triangles (half the flops).
 template <class T>
 void batched_gj_inverse(int nbox, const int* h_n, const long long* h_off, const int* d_n,
                         const long long* d_off, T* d_A, int* d_info, GJWork& ws, GemmWork& gws,
-                        hipStream_t s);
+                        hipStream_t s, bool sym = false);
 void permute_sym(int nbox, const long long* d_off, const int* d_n, const int* d_perm,
                  const int* d_perm_off, const double* d_C, double* d_Cp, long long total,
                  hipStream_t s);

@@ -157,7 +157,13 @@ const char* hifde_stage_name(int stage);
  *   "elements"     -1 (default): when every coupling of the operator lies
  *                  inside one level-0 cell closure (all stencils), hold the
  *                  level matrix as per-cell dense elements assembled by
- *                  extend-add; 0: device hash table of (row, col) entries */
+ *                  extend-add; 0: device hash table of (row, col) entries
+ *   "symmetric_inverse"  1 (default): with the symmetric level matrix the
+ *                  block inverses read only lower triangles and update only
+ *                  lower triangles (half the flops); 0: general inverse
+ *   "gram_panel_reg"  1 (default): Gram pivoted-Cholesky panels of faces up
+ *                  to 1536 columns hold the panel rows in registers; 0: the
+ *                  LDS/global-memory panel for every face (same results) */
 int hifde_set_option(hifde_factor* f, const char* name, double value);
 /* Per-kernel-family device time, recorded with HIP events on the factor's
  * stream while profiling is on.  hifde_kernel_stats returns the number of
@@ -184,7 +190,11 @@ int hifde_interp_decomp_batch(int nbatch, const int* m, const int* n, const doub
 /* Schur(matrix, data) -- include/Schur.hpp:72-100, over a batch, given the
  * three blocks A22 = M(red,red) (r x r), A21 = M(red,skel) (r x s),
  * A12 = M(skel,red) (s x r), packed per box.  Outputs packed A22^{-1}, X and
- * S = -A12 X.  precision selects FP64 or FP32 arithmetic.  info[b] as LU. */
+ * S = -A12 X.  precision selects FP64 or FP32 arithmetic; OR-ed with
+ * HIFDE_SCHUR_SYMMETRIC it declares every A22 symmetric, and the inverses of
+ * blocks wider than 512 take half the flops (no counterpart in the
+ * reference, whose Schur inverts by LU).  info[b] as LU. */
+#define HIFDE_SCHUR_SYMMETRIC 0x100
 int hifde_schur_batch(int precision, int nbatch, const int* r, const int* s, const double* A22,
                       const double* A21, const double* A12, double* A22inv, double* X, double* S,
                       int* info);

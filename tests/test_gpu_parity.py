@@ -92,6 +92,32 @@ def test_blocked_inverse_large_nonsymmetric(H, precision, tol):
         assert _rel(Ainv, ref) < tol, (n, _rel(Ainv, ref))
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 1e-5)])
+def test_symmetric_blocked_inverse(H, precision, tol):
+    """The symmetric outer level (lower triangles only, half the flops) on
+    symmetric positive definite blocks of one to four 512-wide windows, with
+    the upper triangle of the multi-window blocks overwritten (it must not be
+    read), against numpy; S = -A12 X as in the general path."""
+    rng = _rng(11)
+    sizes = [40, 300, 512, 513, 700, 1100, 1600, 2000]
+    blocks, refs = [], []
+    for n in sizes:
+        B = rng.standard_normal((n, n)) / np.sqrt(n)
+        A = B @ B.T + 0.5 * np.eye(n)
+        refs.append(np.linalg.inv(A))
+        A21 = rng.standard_normal((n, 3))
+        Au = A.copy()
+        if n > 512:
+            Au[np.triu_indices(n, 1)] = 1e30
+        blocks.append((Au, A21, A21.T.copy()))
+    out = H.schur_batch(blocks, precision, symmetric=True)
+    for n, (Ainv, X, S, info), ref, (_, A21, _) in zip(sizes, out, refs, blocks):
+        assert info == 0, n
+        assert _rel(Ainv, ref) < tol, (n, _rel(Ainv, ref))
+        assert _rel(X, ref @ A21) < 10 * tol, n
+        assert _rel(S, -A21.T @ ref @ A21) < 10 * tol, n
+
+
 def test_blocked_inverse_reports_singular_column(H):
     """A zero column in the second diagonal window is reported 1-based, as
     getrf's info."""
