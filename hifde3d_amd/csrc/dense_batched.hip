@@ -1633,12 +1633,12 @@ void batched_gj_inverse(int nbox, const int* h_n, const long long* h_off, const 
     k_fill_int<<<(nbig + 255) / 256, 256, 0, s>>>(ws.info.p, nbig, 0);
     ws.B.reserve((size_t)nbig * kGJB * kGJPanel * sizeof(T));
     ws.D.reserve((size_t)nbig * kGJB * kGJB * sizeof(T));
-    // symmetric outer level: per box V and W (n x kGJPanel, ld n), then V^T
+    // symmetric outer level: per box W (n x kGJPanel, ld n) and V^T
     // (kGJPanel x n); the general one: RO and CW
     std::vector<long long> sw_off(nbig + 1, 0);
     for (int j = 0; j < nbig; ++j) {
         const int n = h_n[bix[j]];
-        sw_off[j + 1] = sw_off[j] + (sym && n > kGJPanel ? 3LL * n * kGJPanel : 0);
+        sw_off[j + 1] = sw_off[j] + (sym && n > kGJPanel ? 2LL * n * kGJPanel : 0);
     }
     if (sym) {
         ws.SW.reserve((size_t)std::max<long long>(sw_off[nbig], 1) * sizeof(T));
@@ -1691,8 +1691,8 @@ void batched_gj_inverse(int nbox, const int* h_n, const long long* h_off, const 
             HIFDE_LAUNCH_CHECK();
         };
         auto gemm = [&](const std::vector<GemmDesc>& d, const T* A, const T* B, T* C, T alpha,
-                        T beta) {
-            if (!exec) gh.push_back(plan.add(d, false));
+                        T beta, bool transA = false) {
+            if (!exec) gh.push_back(plan.add(d, transA));
             else gemm_run<T, T>(plan, gh[gi++], A, B, C, alpha, beta, s);
         };
         for (int so = 0; so < osteps; ++so) {
@@ -1752,50 +1752,53 @@ void batched_gj_inverse(int nbox, const int* h_n, const long long* h_off, const 
                 // Symmetric outer level.  With P the windows already
                 // processed, the in-place iterate Z keeps Z_PP and Z_QQ
                 // symmetric and Z_QP = -Z_PQ^T, so its lower triangle holds
-                // everything.  Step K with V = Z(:, K) (rows != K; rows in P
-                // read as -Z(K, .)^T) and W = V D:
+                // everything.  Step K, with V = Z(:, K) over the rows != K
+                // and W = V D (D = Z_KK after the inner level), is
                 //   Z_IJ -= s_J W_I V_J^T    (I >= J; s_J = -1 in P, +1 after K)
                 //   Z(I, K) = -W_I,  Z(K, J) = -W_J^T  (J in P)
-                // which is the rank-kb update of the general level on the lower
-                // triangle only.
-                std::vector<SymCopy> gv, vt, wb;
-                std::vector<GemmDesc> gw, su;
+                // -- the general level's rank-kb update on the lower triangle
+                // only.  Both halves of V come from the lower triangle as
+                // Vt = (sigma V)^T with sigma = +1 in P, -1 after K:
+                //   Vt(:, J in P)  = -Z(K, J)       (a row block of Z)
+                //   Vt(:, J > K)   = -Z(J, K)^T     (a column block, transposed)
+                // and Wt = Vt^T D = sigma W, so the update is +Wt_I Vt_J for
+                // I in P and -Wt_I Vt_J below K.
+                std::vector<SymCopy> gv, wb;
+                std::vector<GemmDesc> gw, tl, bq;
                 for (int j = 0; j < nbig && !exec; ++j) {
                     const int n = h_n[bix[j]];
                     const int kb = wn[w0 + j];
                     if (kb == 0 || n <= kGJPanel) continue;
                     const long long o = h_off[bix[j]];
-                    const long long sv = sw_off[j], swo = sv + (long long)n * kGJPanel,
-                                    svt = swo + (long long)n * kGJPanel;
+                    const long long swo = sw_off[j], svt = swo + (long long)n * kGJPanel;
                     const int K1 = K0 + kb, nq = n - K1, ldv = n;
+                    if (K0 > 0) gv.push_back({svt, o + K0, kb, K0, kGJPanel, n, 0, 0, 0, 1, 1, 0});
                     if (nq > 0)
-                        gv.push_back({sv + K0, o + K1 + (long long)K0 * n, nq, kb, ldv, n, 0, 0,
-                                      kAll, 1, 1, 0});
-                    if (K0 > 0) gv.push_back({sv, o + K0, K0, kb, ldv, n, 1, 0, 0, 1, 1, 0});
-                    gw.push_back({sv, o + K0 + (long long)K0 * n, swo, n - kb, kb, kb, ldv, n,
-                                  ldv});
-                    vt.push_back({svt, sv, kb, n - kb, kGJPanel, ldv, 1, 0, K0, 1, 1, 1});
+                        gv.push_back({svt + (long long)K0 * kGJPanel, o + K1 + (long long)K0 * n,
+                                      kb, nq, kGJPanel, n, 1, 0, 0, 1, 1, 0});
+                    gw.push_back({svt, o + K0 + (long long)K0 * n, swo, n - kb, kb, kb, kGJPanel,
+                                  n, ldv});
                     if (K0 > 0) {
-                        GemmDesc tl{swo, svt, o, K0, K0, kb, ldv, kGJPanel, n};
-                        tl.lower = 1;
-                        su.push_back(tl);
+                        GemmDesc d{swo, svt, o, K0, K0, kb, ldv, kGJPanel, n};
+                        d.lower = 1;
+                        tl.push_back(d);
                     }
                     if (K0 > 0 && nq > 0)
-                        su.push_back({swo + K0, svt, o + K1, nq, K0, kb, ldv, kGJPanel, n});
+                        bq.push_back({swo + K0, svt, o + K1, nq, K0, kb, ldv, kGJPanel, n});
                     if (nq > 0) {
-                        GemmDesc br{swo + K0, svt + (long long)K0 * kGJPanel,
-                                    o + K1 + (long long)K1 * n, nq, nq, kb, ldv, kGJPanel, n};
-                        br.lower = 1;
-                        su.push_back(br);
+                        GemmDesc d{swo + K0, svt + (long long)K0 * kGJPanel,
+                                   o + K1 + (long long)K1 * n, nq, nq, kb, ldv, kGJPanel, n};
+                        d.lower = 1;
+                        bq.push_back(d);
                         wb.push_back({o + K1 + (long long)K0 * n, swo + K0, nq, kb, n, ldv, 0, 0,
-                                      0, 1, 0, 1});
+                                      kAll, 1, 0, 1});
                     }
                     if (K0 > 0) wb.push_back({o + K0, swo, kb, K0, n, ldv, 1, 0, 0, 1, 0, 1});
                 }
                 scopy(gv);
-                gemm(gw, SW, d_A, SW, (T)1, (T)0);
-                scopy(vt);
-                gemm(su, SW, SW, d_A, (T)1, (T)1);
+                gemm(gw, SW, d_A, SW, (T)1, (T)0, true);
+                gemm(tl, SW, SW, d_A, (T)1, (T)1);
+                gemm(bq, SW, SW, d_A, (T)-1, (T)1);
                 scopy(wb);
                 continue;
             }
