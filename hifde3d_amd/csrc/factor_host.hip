@@ -30,9 +30,12 @@ Stager* active_stager() { return t_stager; }
 void set_active_stager(Stager* st) { t_stager = st; }
 namespace {
 thread_local KernelTimer* t_timer = nullptr;
+thread_local hipStream_t t_owner = nullptr;
 }
 KernelTimer* active_timer() { return t_timer; }
 void set_active_timer(KernelTimer* t) { t_timer = t; }
+hipStream_t owner_stream() { return t_owner; }
+void set_owner_stream(hipStream_t s) { t_owner = s; }
 
 const void* Stager::stage(const void* src, size_t bytes) {
     const size_t need = (bytes + 255) & ~(size_t)255;
@@ -160,6 +163,7 @@ struct hifde_factor {
     DevBuf<SliceSum> d_gsum;
     static constexpr long long kGramSplitTiles = 1536;  // three waves of 512 workgroups
     bool sym_gj = true;           // symmetric level matrix: half-flop block inverses
+    int host_threads = 0;         // OpenMP threads of the host planning (0: the caller's)
     int gram_min_n = 16;          // faces with more columns use the Gram path
     double gram_min_eps = 1e-7;   // below this tolerance every face uses Householder
     DevBuf<double> pcg_buf, pcg_part, pcg_scalar, apply_tmp, apply_side;
@@ -190,7 +194,18 @@ struct hifde_factor {
     DevBuf<long long> d_eoff;
     DevBuf<int> d_emap, d_spos, d_etab, d_egid;
 
+    // Device buffers handed in by the caller: the factor's stream waits for
+    // the work queued on the default stream before the call (where PyTorch
+    // and most callers produce them), without blocking the host.
+    hipEvent_t entry_ev = nullptr;
+    void after_default_stream() {
+        if (!entry_ev) HIFDE_CHECK(hipEventCreateWithFlags(&entry_ev, hipEventDisableTiming));
+        HIFDE_CHECK(hipEventRecord(entry_ev, nullptr));
+        HIFDE_CHECK(hipStreamWaitEvent(stream, entry_ev, 0));
+    }
+
     ~hifde_factor() {
+        if (entry_ev) (void)hipEventDestroy(entry_ev);
         clear_graphs();
         for (PendingRec& r : pending) {
             (void)hipEventDestroy(r.e0);
@@ -273,12 +288,19 @@ struct hifde_factor {
     };
     LaunchTimer launch_timer{this};
     // Installs the launch timer for the duration of a profiled call.
+    // Per call of a factor method: its kernel timer (when profiling) and its
+    // stream as the owner of the buffers it grows.
     struct TimerGuard {
         KernelTimer* prev;
-        TimerGuard(hifde_factor* f) : prev(active_timer()) {
+        hipStream_t prev_owner;
+        TimerGuard(hifde_factor* f) : prev(active_timer()), prev_owner(owner_stream()) {
             if (f->profiling) set_active_timer(&f->launch_timer);
+            set_owner_stream(f->stream);
         }
-        ~TimerGuard() { set_active_timer(prev); }
+        ~TimerGuard() {
+            set_active_timer(prev);
+            set_owner_stream(prev_owner);
+        }
     };
 
     void resolve_profile() {
@@ -1261,6 +1283,16 @@ struct hifde_factor {
 
     void run_factor() {
         if (!have_matrix) throw Error(HIFDE_ERR_STATE, "factor: no matrix set");
+        // host planning threads of this call (several factor objects driven
+        // from several host threads share the cores); the caller's setting
+        // is restored on return
+        struct OmpThreads {
+            int prev;
+            explicit OmpThreads(int n) : prev(omp_get_max_threads()) {
+                if (n > 0) omp_set_num_threads(n);
+            }
+            ~OmpThreads() { omp_set_num_threads(prev); }
+        } omp_threads(host_threads);
         clear_graphs();
         for (double& t : stage_t) t = 0.0;
         stage_lt.assign((size_t)L * ST_COUNT, 0.0);
@@ -2315,6 +2347,7 @@ int hifde_apply_device(hifde_factor* f, double* d_u, int nrhs, long long ldu, in
         if (!f || !d_u || nrhs < 0) throw Error(HIFDE_ERR_ARG, "apply: bad arguments");
         long long n = (long long)f->NC * f->NC * f->NC;
         if (nrhs > 1 && ldu < n) throw Error(HIFDE_ERR_ARG, "apply: ldu too small");
+        f->after_default_stream();
         f->apply_dev(d_u, nrhs, ldu, inverse != 0);
         HIFDE_CHECK(hipStreamSynchronize(f->stream));
         return HIFDE_OK;
@@ -2360,6 +2393,7 @@ int hifde_pcg_device(hifde_factor* f, const double* d_b, double* d_x, double tol
                      int* iters, double* relres) {
     return guarded([&]() -> int {
         if (!f || !d_b || !d_x || maxit < 0) throw Error(HIFDE_ERR_ARG, "pcg: bad arguments");
+        f->after_default_stream();
         f->pcg_dev(d_b, d_x, tol, maxit, iters, relres);
         return HIFDE_OK;
     });
@@ -2375,6 +2409,7 @@ int hifde_pcg_multi_device(hifde_factor* f, const double* d_b, long long ldb, do
         if (nrhs > 1 && (ldb < n || ldx < n))
             throw Error(HIFDE_ERR_ARG, "pcg_multi: leading dimension too small");
         if (nrhs == 0) return HIFDE_OK;
+        f->after_default_stream();
         f->pcg_multi_dev(d_b, ldb, d_x, ldx, nrhs, tol, maxit, iters, relres);
         return HIFDE_OK;
     });
@@ -2492,6 +2527,7 @@ int hifde_set_option(hifde_factor* f, const char* name, double value) {
         else if (n == "stage_sync") f->stage_sync = value != 0;
         else if (n == "gram_panel_reg") f->gram_ws.reg_panel = value != 0;
         else if (n == "symmetric_inverse") f->sym_gj = value != 0;
+        else if (n == "host_threads") f->host_threads = std::max(0, (int)value);
         else if (n == "multi_fused_max") {
             f->multi_fused_max = (int)value;
             f->clear_graphs();

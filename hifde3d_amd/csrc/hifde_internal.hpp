@@ -72,6 +72,11 @@ struct KernelTimer {
 KernelTimer* active_timer();
 void set_active_timer(KernelTimer* t);
 
+// Stream of the factor object the calling thread is working for (null
+// outside its calls): the only stream that uses that object's buffers.
+hipStream_t owner_stream();
+void set_owner_stream(hipStream_t s);
+
 // Owning device buffer.  Grows, never shrinks, unless release() is called.
 template <class T>
 struct DevBuf {
@@ -88,8 +93,14 @@ struct DevBuf {
     ~DevBuf() { release(); }
     void reserve(size_t cnt) {
         if (cnt <= n && p) return;
-        // work queued earlier may still read the old buffer
-        if (p) HIFDE_CHECK(hipDeviceSynchronize());
+        // work queued earlier may still read the old buffer: wait for the
+        // owning object's stream (not the whole device, which would also wait
+        // for, and break the graph captures of, other objects' streams)
+        if (p) {
+            const hipStream_t os = owner_stream();
+            if (os) HIFDE_CHECK(hipStreamSynchronize(os));
+            else HIFDE_CHECK(hipDeviceSynchronize());
+        }
         release();
         HIFDE_CHECK(hipMalloc(reinterpret_cast<void**>(&p), (cnt ? cnt : 1) * sizeof(T)));
         n = cnt;

@@ -81,7 +81,11 @@ int hifde_apply(hifde_factor* f, double* u, int inverse);
 /* Same on device memory: nrhs vectors at d_u + c*ldu.  For nrhs > 1 every
  * box step is one batched GEMM over the nrhs columns (the factor blocks are
  * read once per sweep); mixed-precision levels then round the vectors to
- * FP32 for the product, as their factor blocks are FP32. */
+ * FP32 for the product, as their factor blocks are FP32.
+ * The device entry points (this one and the PCG ones) run on the factor's
+ * own stream, after the work already queued on the default stream (without
+ * blocking the host), and return when done; buffers written on another
+ * stream must be complete before the call. */
 int hifde_apply_device(hifde_factor* f, double* d_u, int nrhs, long long ldu, int inverse);
 
 /* SpMV(A, x, y) -- src/Factor.cpp:1099-1113.  transpose == 1 computes the
@@ -161,6 +165,10 @@ const char* hifde_stage_name(int stage);
  *   "symmetric_inverse"  1 (default): with the symmetric level matrix the
  *                  block inverses read only lower triangles and update only
  *                  lower triangles (half the flops); 0: general inverse
+ *   "host_threads" OpenMP threads of the host-side planning in
+ *                  hifde_factor_run (0, the default: the calling thread's
+ *                  setting); for several factor objects run from several
+ *                  host threads at once
  *   "gram_panel_reg"  1 (default): Gram pivoted-Cholesky panels of faces up
  *                  to 1536 columns hold the panel rows in registers; 0: the
  *                  LDS/global-memory panel for every face (same results) */
