@@ -164,6 +164,7 @@ struct hifde_factor {
     static constexpr long long kGramSplitTiles = 1536;  // three waves of 512 workgroups
     bool sym_gj = true;           // symmetric level matrix: half-flop block inverses
     int host_threads = 0;         // OpenMP threads of the host planning (0: the caller's)
+    bool keep_workspaces = true;  // keep the factorization workspaces for the next one
     int gram_min_n = 16;          // faces with more columns use the Gram path
     double gram_min_eps = 1e-7;   // below this tolerance every face uses Householder
     DevBuf<double> pcg_buf, pcg_part, pcg_scalar, apply_tmp, apply_side;
@@ -1344,13 +1345,29 @@ struct hifde_factor {
             }
         }
         // Every DOF has been eliminated: the level matrix is consumed.  Its
-        // buffers and the workspaces are kept for the next factorization.
+        // buffers and the workspaces are kept for the next factorization
+        // unless keep_workspaces is off.
         HIFDE_CHECK(hipMemcpyAsync(&lu_info, d_status.p, sizeof(int), hipMemcpyDeviceToHost,
                                    stream));
         HIFDE_CHECK(hipStreamSynchronize(stream));
         stager.synced();
         resolve_profile();
         factored = true;
+        if (!keep_workspaces) release_workspaces();
+    }
+
+    // The factorization's large workspaces (level-matrix elements, face and
+    // Schur blocks, block-inverse and Gram panels); every user reserves them
+    // again before use, so the next factorization reallocates them.
+    void release_workspaces() {
+        HIFDE_CHECK(hipStreamSynchronize(stream));
+        for (DevBuf<double>& e : elem) e.release();
+        for (DevBuf<unsigned char>* b : {&ws_a, &ws_b, &ws_c, &gj_ws.B, &gj_ws.D, &gj_ws.RO,
+                                         &gj_ws.CW, &gj_ws.SW})
+            b->release();
+        for (DevBuf<double>* b : {&ws_M, &ws_C, &ws_W, &ws_T, &ws_G, &gram_ws.P, &gram_ws.RK,
+                                  &gram_ws.dg})
+            b->release();
     }
 
     // ------------------------------------------------------------------
@@ -2528,6 +2545,10 @@ int hifde_set_option(hifde_factor* f, const char* name, double value) {
         else if (n == "gram_panel_reg") f->gram_ws.reg_panel = value != 0;
         else if (n == "symmetric_inverse") f->sym_gj = value != 0;
         else if (n == "host_threads") f->host_threads = std::max(0, (int)value);
+        else if (n == "keep_workspaces") {
+            f->keep_workspaces = value != 0;
+            if (!f->keep_workspaces && f->factored) f->release_workspaces();
+        }
         else if (n == "multi_fused_max") {
             f->multi_fused_max = (int)value;
             f->clear_graphs();

@@ -165,6 +165,12 @@ const char* hifde_stage_name(int stage);
  *   "symmetric_inverse"  1 (default): with the symmetric level matrix the
  *                  block inverses read only lower triangles and update only
  *                  lower triangles (half the flops); 0: general inverse
+ *   "keep_workspaces"  1 (default): the factorization's workspaces (level-
+ *                  matrix elements, face and Schur blocks, inverse panels)
+ *                  stay allocated for the next hifde_factor_run; 0 frees
+ *                  them after each factorization (and at once if factored),
+ *                  leaving the device memory to the factor and the solves
+ *                  (at NC=256, P=8 the workspaces exceed the factor)
  *   "host_threads" OpenMP threads of the host-side planning in
  *                  hifde_factor_run (0, the default: the calling thread's
  *                  setting); for several factor objects run from several

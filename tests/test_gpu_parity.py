@@ -332,6 +332,31 @@ def test_gram_register_panel_bitwise_equal_to_lds_panel(H):
     assert np.array_equal(out[0][1], out[1][1])
 
 
+def test_released_workspaces_give_the_same_factor(H):
+    """keep_workspaces=0 frees the factorization's workspaces after each
+    factorization; factoring again (which reallocates them) and solving give
+    the same bits as with the workspaces kept."""
+    NC = 32
+    a = pb.pow10_coeff(NC, 12)
+    V = pb.zeros_coeff(NC)
+    b = pb.rhs_normal(NC, 13)
+    out = []
+    for keep in (1, 0):
+        f = H.HIFFactor(NC - 1, 4, 1e-4, precision="mixed")
+        f.set_option("keep_workspaces", keep)
+        res = []
+        for rep in range(2):
+            f.set_stencil(a, V, True)
+            assert f.Factor() == 0
+            res.append(f.Apply(b, True))
+        x, it, rr = f.pcg(b, tol=1e-10, maxit=100)
+        out.append((res, x))
+        f.close()
+    for r0, r1 in zip(out[0][0], out[1][0]):
+        assert np.array_equal(r0, r1)
+    assert np.array_equal(out[0][1], out[1][1])
+
+
 def test_factor_objects_from_several_threads_match_sequential(H):
     """Factor objects driven from several host threads at once (each on its
     own stream, graph captures included) give the same bits as one after
