@@ -1917,27 +1917,42 @@ std::pair<int, int> GemmPlan::add(const std::vector<GemmDesc>& d, bool transA) {
         if (g.m > 0 && g.n > 0 && shortk(g))
             sk_tiles += (long long)((g.m + BM - 1) / BM) * ((g.n + BN - 1) / BN);
     const bool use_sk = sk_tiles <= kShortKMaxTiles;
-    // kinds: 0 k_gemm (FMA, 64 x 64 tiles), 1 k_gemm_mfma, 2 k_gemm_sk
+    // kinds: 0 k_gemm (FMA, 64 x 64 tiles), 1 k_gemm_mfma, 2 k_gemm_sk; -1
+    // empty.  Classified once, then one launch per kind in descriptor order.
+    std::vector<signed char> kinds(d.size());
+    int count[3] = {0, 0, 0};
+    for (size_t i = 0; i < d.size(); ++i) {
+        const GemmDesc& g = d[i];
+        if (g.m <= 0 || g.n <= 0) {
+            kinds[i] = -1;
+            continue;
+        }
+        // short products over small outputs (the in-window updates of the
+        // block inverse) finish sooner on 64 x 64 tiles
+        const bool sk = use_sk && shortk(g);
+        const bool small = g.k <= 64 && (long long)g.m * g.n <= 256LL * 256;
+        // the MFMA kernel addresses each operand with 32-bit byte offsets:
+        // keep operand extents (8-byte elements, plus a slab and a tile of
+        // overrun) under 2 GiB
+        const double ext_a = (double)g.lda * (transA ? g.m : g.k) + 16.0 * g.lda + 256;
+        const double ext_b = (double)g.ldb * g.n + 16.0 * g.ldb + 256;
+        const bool fits = 8.0 * std::max(ext_a, ext_b) < 2147483648.0;
+        const bool large = g.m >= TILE / 2 && g.n >= TILE / 2 && !sk && !small && fits;
+        const int kind = sk ? 2 : large ? 1 : 0;
+        kinds[i] = (signed char)kind;
+        ++count[kind];
+    }
+    desc.reserve(desc.size() + d.size());
+    tiles.reserve(tiles.size() + d.size() + 3);
     for (int pass = 0; pass < 3; ++pass) {
+        if (count[pass] == 0) continue;
         const int d0 = (int)desc.size(), t0 = (int)tiles.size();
         const int tile = pass == 1 ? TILE : BM;
         Launch L{pass, transA ? 1 : 0, 0, d0, t0, 0, 0.0, 0.0, 0.0};
         tiles.push_back(0);
-        for (const GemmDesc& g : d) {
-            if (g.m <= 0 || g.n <= 0) continue;
-            // short products over small outputs (the in-window updates of the
-            // block inverse) finish sooner on 64 x 64 tiles
-            const bool sk = use_sk && shortk(g);
-            const bool small = g.k <= 64 && (long long)g.m * g.n <= 256LL * 256;
-            // the MFMA kernel addresses each operand with 32-bit byte offsets:
-            // keep operand extents (8-byte elements, plus a slab and a tile of
-            // overrun) under 2 GiB
-            const double ext_a = (double)g.lda * (transA ? g.m : g.k) + 16.0 * g.lda + 256;
-            const double ext_b = (double)g.ldb * g.n + 16.0 * g.ldb + 256;
-            const bool fits = 8.0 * std::max(ext_a, ext_b) < 2147483648.0;
-            const bool large = g.m >= TILE / 2 && g.n >= TILE / 2 && !sk && !small && fits;
-            const int kind = sk ? 2 : large ? 1 : 0;
-            if (kind != pass) continue;
+        for (size_t i = 0; i < d.size(); ++i) {
+            if (kinds[i] != pass) continue;
+            const GemmDesc& g = d[i];
             const int tm = (g.m + tile - 1) / tile, tn = (g.n + tile - 1) / tile;
             desc.push_back(g);
             tiles.push_back(tiles.back() + tm * tn);
@@ -1946,10 +1961,6 @@ std::pair<int, int> GemmPlan::add(const std::vector<GemmDesc>& d, bool transA) {
             L.flops += 2.0 * cells * g.k;
             L.cells += cells;
             L.ab_elems += (double)g.m * g.k + (double)g.k * g.n;
-        }
-        if (L.nb == 0) {
-            tiles.pop_back();
-            continue;
         }
         L.total = tiles.back();
         launches.push_back(L);
