@@ -36,6 +36,12 @@ KernelTimer* active_timer() { return t_timer; }
 void set_active_timer(KernelTimer* t) { t_timer = t; }
 hipStream_t owner_stream() { return t_owner; }
 void set_owner_stream(hipStream_t s) { t_owner = s; }
+// Marks the calling thread as working for the object that owns stream s.
+struct OwnerGuard {
+    hipStream_t prev;
+    explicit OwnerGuard(hipStream_t s) : prev(owner_stream()) { set_owner_stream(s); }
+    ~OwnerGuard() { set_owner_stream(prev); }
+};
 
 const void* Stager::stage(const void* src, size_t bytes) {
     const size_t need = (bytes + 255) & ~(size_t)255;
@@ -2335,6 +2341,7 @@ void hifde_destroy(hifde_factor* f) {
 int hifde_set_matrix_coo(hifde_factor* f, long long nnz, const int* rows, const int* cols,
                          const double* vals) {
     return guarded([&]() -> int {
+        OwnerGuard og(f ? f->stream : nullptr);
         if (!f || nnz < 0 || (nnz && (!rows || !cols || !vals)))
             throw Error(HIFDE_ERR_ARG, "set_matrix_coo: bad arguments");
         f->set_coo(nnz, rows, cols, vals);
@@ -2344,6 +2351,7 @@ int hifde_set_matrix_coo(hifde_factor* f, long long nnz, const int* rows, const 
 
 int hifde_set_stencil(hifde_factor* f, const double* a, const double* v, int symmetric) {
     return guarded([&]() -> int {
+        OwnerGuard og(f ? f->stream : nullptr);
         if (!f || !a || !v) throw Error(HIFDE_ERR_ARG, "set_stencil: null argument");
         f->set_stencil(a, v, symmetric);
         return HIFDE_OK;
@@ -2373,6 +2381,7 @@ int hifde_apply_device(hifde_factor* f, double* d_u, int nrhs, long long ldu, in
 
 int hifde_apply(hifde_factor* f, double* u, int inverse) {
     return guarded([&]() -> int {
+        OwnerGuard og(f ? f->stream : nullptr);
         if (!f || !u) throw Error(HIFDE_ERR_ARG, "apply: bad arguments");
         long long n = (long long)f->NC * f->NC * f->NC;
         DevBuf<double> d;
@@ -2386,6 +2395,7 @@ int hifde_apply(hifde_factor* f, double* u, int inverse) {
 
 int hifde_spmv(hifde_factor* f, const double* x, double* y, int transpose) {
     return guarded([&]() -> int {
+        OwnerGuard og(f ? f->stream : nullptr);
         if (!f || !x || !y || !f->have_matrix) throw Error(HIFDE_ERR_ARG, "spmv: bad arguments");
         long long n = (long long)f->NC * f->NC * f->NC;
         if (transpose) {
@@ -2435,6 +2445,7 @@ int hifde_pcg_multi_device(hifde_factor* f, const double* d_b, long long ldb, do
 int hifde_pcg(hifde_factor* f, const double* b, double* x, double tol, int maxit, int* iters,
               double* relres) {
     return guarded([&]() -> int {
+        OwnerGuard og(f ? f->stream : nullptr);
         if (!f || !b || !x) throw Error(HIFDE_ERR_ARG, "pcg: bad arguments");
         long long n = (long long)f->NC * f->NC * f->NC;
         DevBuf<double> db, dx;
@@ -2450,6 +2461,7 @@ int hifde_pcg(hifde_factor* f, const double* b, double* x, double tol, int maxit
 
 int hifde_factor_save(hifde_factor* f, const char* path) {
     return guarded([&]() -> int {
+        OwnerGuard og(f ? f->stream : nullptr);
         if (!f || !path) throw Error(HIFDE_ERR_ARG, "save: bad arguments");
         f->save(path);
         return HIFDE_OK;
@@ -2458,6 +2470,7 @@ int hifde_factor_save(hifde_factor* f, const char* path) {
 
 int hifde_factor_load(hifde_factor* f, const char* path) {
     return guarded([&]() -> int {
+        OwnerGuard og(f ? f->stream : nullptr);
         if (!f || !path) throw Error(HIFDE_ERR_ARG, "load: bad arguments");
         f->load(path);
         return f->lu_info;
@@ -2534,6 +2547,7 @@ int hifde_level_stage_times(const hifde_factor* f, int level, double* out, int n
 
 int hifde_set_option(hifde_factor* f, const char* name, double value) {
     return guarded([&]() -> int {
+        OwnerGuard og(f ? f->stream : nullptr);
         if (!f || !name) throw Error(HIFDE_ERR_ARG, "set_option: null");
         std::string n(name);
         if (n == "gram_min_n") f->gram_min_n = (int)value;
