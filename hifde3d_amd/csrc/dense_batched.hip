@@ -242,13 +242,87 @@ __device__ bool lds_gj_inverse64(T (*S)[kSmallGJ + 1], int n, T* colk, T* rowk, 
     return true;
 }
 
+// Boxes of at most kWaveGJ rows: one wave per box, no barriers and no LDS.
+// Lane i holds row i in registers.  Row interchanges exchange positions, not
+// data: the pivot of step k is the largest |a| among the rows at positions
+// >= k, ties to the lowest position, as with physical interchanges, and the
+// arithmetic per entry is that of lds_gj_inverse64, so both give the same
+// result.  The columns are permuted on output, as there.
+constexpr int kWaveGJ = 32;
+template <class T>
+__global__ void __launch_bounds__(256) k_gj_wave(int nbox, const int* __restrict__ dims,
+                                                 const long long* __restrict__ off, T* mats,
+                                                 int* info) {
+    const int lane = threadIdx.x & 63;
+    const int b = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (b >= nbox) return;
+    const int n = dims[b];
+    if (n == 0 || n > kWaveGJ) return;
+    T* A = mats + off[b];
+    const bool row = lane < n;
+    T a[kWaveGJ];
+#pragma unroll
+    for (int c = 0; c < kWaveGJ; ++c) a[c] = (row && c < n) ? A[lane + (size_t)c * n] : (T)0;
+    int pos = lane;
+    int prm[kWaveGJ];
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < kWaveGJ; ++k) {
+        prm[k] = k;
+        if (k >= n || !ok) continue;  // (no break: the loop stays unrolled)
+        bool nz;
+        const int pp =
+            wave_argmax_nonneg<T>((row && pos >= k) ? fabs(a[k]) : (T)0, pos, nz);
+        if (!nz) {
+            ok = false;
+            continue;
+        }
+        prm[k] = pp;
+        const int pl = __ffsll((long long)__ballot(row && pos == pp)) - 1;
+        const int old = pos;
+        if (lane == pl) pos = k;
+        else if (old == k) pos = pp;
+        const T d = (T)1 / read_lane(a[k], pl);
+        const T f = a[k];
+        const bool me = lane == pl;
+#pragma unroll
+        for (int c = 0; c < kWaveGJ; ++c) {
+            const T rc = (c == k) ? d : read_lane(a[c], pl) * d;
+            if (me) {
+                a[c] = rc;
+            } else {
+                const T av = (c == k) ? (T)0 : a[c];
+                a[c] = av - f * rc;
+            }
+        }
+    }
+    if (!ok) {
+        if (lane == 0) info[b] = 1;
+        return;
+    }
+    // column c of the in-place result is column j of the inverse, where j is
+    // c taken through the interchanges in reverse order
+#pragma unroll
+    for (int c = 0; c < kWaveGJ; ++c) {
+        if (c >= n) continue;
+        int j = c;
+#pragma unroll
+        for (int k = kWaveGJ - 1; k >= 0; --k) {
+            if (j == k) j = prm[k];
+            else if (j == prm[k]) j = k;
+        }
+        if (row) A[pos + (size_t)j * n] = a[c];
+    }
+    if (lane == 0) info[b] = 0;
+}
+
 template <class T>
 __global__ void __launch_bounds__(256) k_gj_small(const int* __restrict__ dims,
                                                   const long long* __restrict__ off, T* mats,
                                                   int* info) {
     const int b = blockIdx.x;
     const int n = dims[b];
-    if (n == 0 || n > kSmallGJ) return;
+    if (n <= kWaveGJ || n > kSmallGJ) return;  // smaller boxes: k_gj_wave
     T* A = mats + off[b];
     __shared__ T S[kSmallGJ][kSmallGJ + 1];  // S[col][row]
     __shared__ T colk[kSmallGJ], rowk[kSmallGJ];
@@ -1588,16 +1662,22 @@ void batched_gj_inverse(int nbox, const int* h_n, const long long* h_off, const 
     // Boxes of at most kSmallGJ rows: one workgroup each.  Larger boxes are
     // listed in slots 0..nbig-1 for the blocked path.
     std::vector<int> bix;
-    bool any_small = false;
+    bool any_wave = false, any_small = false;
     int max_n = 0;
     for (int b = 0; b < nbox; ++b) {
         const int n = h_n[b];
         if (n > kSmallGJ) {
             bix.push_back(b);
             max_n = std::max(max_n, n);
-        } else if (n > 0) {
+        } else if (n > kWaveGJ) {
             any_small = true;
+        } else if (n > 0) {
+            any_wave = true;
         }
+    }
+    if (any_wave) {
+        k_gj_wave<T><<<(nbox + 3) / 4, 256, 0, s>>>(nbox, d_n, d_off, d_A, d_info);
+        HIFDE_LAUNCH_CHECK();
     }
     if (any_small) {
         k_gj_small<T><<<nbox, 256, 0, s>>>(d_n, d_off, d_A, d_info);
