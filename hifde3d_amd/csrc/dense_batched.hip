@@ -1440,6 +1440,10 @@ __global__ void k_gram_gather_r(int nface, const long long* __restrict__ toff,
 
 // Column-pivoted QR + interpolative decomposition of one face per workgroup.
 // Dynamic shared memory: vn1[n], vn2[n] (double), jp[n], pos[n] (int).
+// LDSM: the face matrix is staged in LDS (after the norm and index arrays)
+// and the factorization runs there, then R is written back; the operations
+// and their order are the same either way.
+template <bool LDSM>
 __global__ void __launch_bounds__(256) k_interp_decomp(const int* __restrict__ dm,
                                                        const int* __restrict__ dn,
                                                        const long long* __restrict__ offM,
@@ -1456,12 +1460,19 @@ __global__ void __launch_bounds__(256) k_interp_decomp(const int* __restrict__ d
         return;
     }
     const int lda = m > 1 ? m : 1;
-    double* M = Mb + offM[b];
+    double* Mg = Mb + offM[b];
     extern __shared__ unsigned char smem[];
     double* vn1 = reinterpret_cast<double*>(smem);
     double* vn2 = vn1 + n;
     int* jp = reinterpret_cast<int*>(vn2 + n);
     int* pos = jp + n;
+    double* M = Mg;
+    const int mtot = m > 0 ? lda * n : 0;
+    if constexpr (LDSM) {
+        M = reinterpret_cast<double*>(pos + n);
+        for (int e = threadIdx.x; e < mtot; e += blockDim.x) M[e] = Mg[e];
+        __syncthreads();
+    }
     __shared__ double red[16];
     __shared__ double redv[16];
     __shared__ int redi[16];
@@ -1588,6 +1599,8 @@ __global__ void __launch_bounds__(256) k_interp_decomp(const int* __restrict__ d
         kout[b] = k;
     }
     __syncthreads();
+    if constexpr (LDSM)
+        for (int e = tid; e < mtot; e += bs) Mg[e] = M[e];
     const int k = s_k, q = n - k;
     int* pb = perm + perm_off[b];
     for (int t = tid; t < n; t += bs) pb[t] = jp[pos[t]];
@@ -2268,8 +2281,15 @@ void batched_interp_decomp(int nface, const int* d_m, const int* d_n, const long
     size_t shm = (size_t)max_n * (2 * sizeof(double) + 2 * sizeof(int));
     if (shm > 150 * 1024) throw Error(-3, "interp_decomp: face too large for LDS");
     int threads = (max_m <= 128 && max_n <= 16) ? 64 : 256;
-    k_interp_decomp<<<nface, threads, shm, s>>>(d_m, d_n, d_offM, d_M, eps, d_perm, d_perm_off,
-                                                d_k, d_W, d_offW, d_mode);
+    // Small faces (the level-0 faces) run in LDS: their steps are chains of
+    // dependent reductions, which global-memory latency would dominate.
+    const size_t shm_m = (size_t)std::max(max_m, 1) * max_n * sizeof(double);
+    if (shm + shm_m <= 32 * 1024)
+        k_interp_decomp<true><<<nface, threads, shm + shm_m, s>>>(
+            d_m, d_n, d_offM, d_M, eps, d_perm, d_perm_off, d_k, d_W, d_offW, d_mode);
+    else
+        k_interp_decomp<false><<<nface, threads, shm, s>>>(d_m, d_n, d_offM, d_M, eps, d_perm,
+                                                          d_perm_off, d_k, d_W, d_offW, d_mode);
     HIFDE_LAUNCH_CHECK();
 }
 
