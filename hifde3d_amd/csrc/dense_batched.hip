@@ -628,24 +628,38 @@ __global__ void __launch_bounds__(256) k_gemm(int nbox, const int* __restrict__ 
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = (T)0;
-    for (int k0 = 0; k0 < k; k0 += BK) {
+    // The next slab's operands are loaded into registers while the current
+    // one is multiplied out of LDS.
+    T ra[4], rb[4];
+    auto gload = [&](int k0) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int e = threadIdx.x + 256 * q;
             if (!TRANSA) {
                 const int r = e & (BM - 1), c = e >> 6;  // A tile: 64 rows x 16 cols
                 const int gi = i0 + r, gk = k0 + c;
-                As[c][r] = (gi < m && gk < k) ? A[gi + (size_t)gk * dd.lda] : (T)0;
+                ra[q] = (gi < m && gk < k) ? A[gi + (size_t)gk * dd.lda] : (T)0;
             } else {
                 const int c = e & (BK - 1), r = e >> 4;  // (A^T) tile from A(k0+c, i0+r)
                 const int gi = i0 + r, gk = k0 + c;
-                As[c][r] = (gi < m && gk < k) ? A[gk + (size_t)gi * dd.lda] : (T)0;
+                ra[q] = (gi < m && gk < k) ? A[gk + (size_t)gi * dd.lda] : (T)0;
             }
-            const int rb = e & (BK - 1), cb = e >> 4;  // B tile: 16 rows x 64 cols
-            const int gkb = k0 + rb, gj = j0 + cb;
-            Bs[rb][cb] = (gkb < k && gj < n) ? B[gkb + (size_t)gj * dd.ldb] : (T)0;
+            const int r2 = e & (BK - 1), cb = e >> 4;  // B tile: 16 rows x 64 cols
+            const int gkb = k0 + r2, gj = j0 + cb;
+            rb[q] = (gkb < k && gj < n) ? B[gkb + (size_t)gj * dd.ldb] : (T)0;
+        }
+    };
+    if (k > 0) gload(0);
+    for (int k0 = 0; k0 < k; k0 += BK) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = threadIdx.x + 256 * q;
+            if (!TRANSA) As[e >> 6][e & (BM - 1)] = ra[q];
+            else As[e & (BK - 1)][e >> 4] = ra[q];
+            Bs[e & (BK - 1)][e >> 4] = rb[q];
         }
         __syncthreads();
+        if (k0 + BK < k) gload(k0 + BK);
 #pragma unroll
         for (int kk = 0; kk < BK; ++kk) {
             T a[4], bv[4];
