@@ -1813,10 +1813,11 @@ void batched_gj_inverse(int nbox, const int* h_n, const long long* h_off, const 
             if (sym) {
                 // only lower triangles are current: the window's upper
                 // triangle is rebuilt from its lower one before inversion
+                // (a box of one window is read whole, as given)
                 std::vector<SymCopy> mir;
                 for (int j = 0; j < nbig && !exec; ++j) {
                     const int kb = wn[w0 + j];
-                    if (kb == 0) continue;
+                    if (kb == 0 || h_n[bix[j]] <= kGJPanel) continue;
                     const long long wo = woff[w0 + j];
                     const int ld = wld[w0 + j];
                     mir.push_back({wo, wo, kb, kb, ld, ld, 1, 1, kAll, 1, 0, 0});
