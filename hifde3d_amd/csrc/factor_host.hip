@@ -676,6 +676,7 @@ struct hifde_factor {
             // the diagonal.  A box with r == 0 still writes S = 0 (the
             // reference's k == 0 GEMM) unless S accumulates.
             std::vector<GemmDesc> d;
+            d.reserve(nb);
             for (int b = 0; b < nb; ++b) {
                 const int r = B.r[b], sb = B.s[b];
                 if (sb == 0 || (accumulate && r == 0)) continue;
@@ -1216,6 +1217,7 @@ struct hifde_factor {
             const BoxBatch& Cb = cells[level];
             std::vector<int> spos(B.skel_off.back());
             std::vector<FaceAddDesc> ad;
+            ad.reserve(nf);
             for (int f = 0; f < nf; ++f) {
                 const int* p = perm.data() + F.cols_off[f];
                 for (int t = 0; t < B.s[f]; ++t)
@@ -1250,6 +1252,7 @@ struct hifde_factor {
         permute_sym(nf, fd.offCd.p, fd.n.p, ws_perm.p, fd.cols_off.p, ws_C.p, ws_T.p, offC.back(),
                     stream);
         std::vector<GemmDesc> d1, d2, d3, d4;
+        for (std::vector<GemmDesc>* v : {&d1, &d2, &d3, &d4}) v->reserve(nf);
         std::vector<CopyDesc> c22(nf), c21(nf), c12(nf);
         for (int f = 0; f < nf; ++f) {
             const int n = F.n[f], k = B.s[f], q = B.r[f];
