@@ -2038,7 +2038,9 @@ std::pair<int, int> GemmPlan::add(const std::vector<GemmDesc>& d, bool transA) {
         // short products over small outputs (the in-window updates of the
         // block inverse) finish sooner on 64 x 64 tiles
         const bool sk = use_sk && shortk(g);
-        const bool small = g.k <= 64 && (long long)g.m * g.n <= 256LL * 256;
+        // products of depth 32 or less (the Gram panels' trailing updates)
+        // are bound by the output tile's read and write on any kernel
+        const bool small = g.k <= 32 || (g.k <= 64 && (long long)g.m * g.n <= 256LL * 256);
         // the MFMA kernel addresses each operand with 32-bit byte offsets:
         // keep operand extents (8-byte elements, plus a slab and a tile of
         // overrun) under 2 GiB
