@@ -2040,7 +2040,7 @@ std::pair<int, int> GemmPlan::add(const std::vector<GemmDesc>& d, bool transA) {
         const bool sk = use_sk && shortk(g);
         // products of depth 32 or less (the Gram panels' trailing updates)
         // are bound by the output tile's read and write on any kernel
-        const bool small = g.k <= 32 || (g.k <= 64 && (long long)g.m * g.n <= 256LL * 256);
+        const bool small = g.k <= 64;
         // the MFMA kernel addresses each operand with 32-bit byte offsets:
         // keep operand extents (8-byte elements, plus a slab and a tile of
         // overrun) under 2 GiB
