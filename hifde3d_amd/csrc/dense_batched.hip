@@ -689,100 +689,6 @@ __global__ void __launch_bounds__(256) k_gemm(int nbox, const int* __restrict__ 
     }
 }
 
-// Products with k <= kShortK over outputs of at most kShortKCells elements
-// (the in-window updates of the block inverse, short apply products): both
-// operand tiles over the whole k extent are loaded into LDS in one round, and
-// the C tile with them, so a tile pays a single memory latency where k_gemm
-// pays one per k-slab.  64 x 64 output tiles, 4 x 4 outputs per thread.  LDS
-// holds A as [k][row] and B as [k][col]; operands read along k (B, and A^T)
-// are stored with the row/column index XOR-ed by k, which spreads both the
-// k-major stores and the row reads over the banks.
-constexpr int kShortK = 64;
-constexpr long long kShortKCells = 512LL * 512;
-constexpr long long kShortKMaxTiles = 1024;  // per launch
-template <class T, bool TRANSA, class TC = T>
-__global__ void __launch_bounds__(256) k_gemm_sk(int nbox, const int* __restrict__ tile_off,
-                                                 const GemmDesc* __restrict__ desc,
-                                                 const T* __restrict__ Ab,
-                                                 const T* __restrict__ Bb, TC* Cb, T alpha,
-                                                 T beta, const int* __restrict__ flags) {
-    const int t = blockIdx.x;
-    const int b = find_box(tile_off, nbox, t);
-    const int local = t - tile_off[b];
-    const GemmDesc dd = desc[b];
-    const int m = dd.m, n = dd.n, k = dd.k;
-    const int tm = (m + BM - 1) / BM;
-    const int i0 = (local % tm) * BM, j0 = (local / tm) * BN;
-    if (dd.lower && j0 > i0) return;
-    if (dd.flag >= 0 && flags && flags[dd.flag] == 0) return;
-    const T* A = Ab + dd.offA;
-    const T* B = Bb + dd.offB;
-    TC* C = Cb + dd.offC;
-    __shared__ T As[kShortK][BM];
-    __shared__ T Bs[kShortK][BN];
-    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    const bool rdc = beta != (T)0;
-    TC c0[4][4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int gj = j0 + ty + 16 * j;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int gi = i0 + tx + 16 * i;
-            c0[i][j] = (rdc && gi < m && gj < n) ? C[gi + (size_t)gj * dd.ldc] : (TC)0;
-        }
-    }
-    if (!TRANSA) {
-        for (int e = threadIdx.x; e < k * BM; e += 256) {
-            const int r = e & (BM - 1), c = e >> 6;
-            const int gi = i0 + r;
-            As[c][r] = gi < m ? A[gi + (size_t)c * dd.lda] : (T)0;
-        }
-    } else {
-        for (int e = threadIdx.x; e < kShortK * BM; e += 256) {
-            const int c = e & (kShortK - 1), r = e >> 6;
-            if (c >= k) continue;
-            const int gi = i0 + r;
-            As[c][r ^ c] = gi < m ? A[c + (size_t)gi * dd.lda] : (T)0;
-        }
-    }
-    for (int e = threadIdx.x; e < kShortK * BN; e += 256) {
-        const int c = e & (kShortK - 1), col = e >> 6;
-        if (c >= k) continue;
-        const int gj = j0 + col;
-        Bs[c][col ^ c] = gj < n ? B[c + (size_t)gj * dd.ldb] : (T)0;
-    }
-    __syncthreads();
-    T acc[4][4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = (T)0;
-    for (int kk = 0; kk < k; ++kk) {
-        T a[4], bv[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) a[i] = As[kk][TRANSA ? ((tx + 16 * i) ^ kk) : tx + 16 * i];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) bv[j] = Bs[kk][(ty + 16 * j) ^ kk];
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) acc[i][j] = fma(a[i], bv[j], acc[i][j]);
-    }
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int gj = j0 + ty + 16 * j;
-        if (gj >= n) continue;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int gi = i0 + tx + 16 * i;
-            if (gi >= m) continue;
-            const TC v = (TC)(alpha * acc[i][j]);
-            C[gi + (size_t)gj * dd.ldc] = rdc ? (TC)beta * c0[i][j] + v : v;
-        }
-    }
-}
-
 // MFMA variant of k_gemm for the larger boxes.  Four waves in a 2x2 grid;
 // each wave owns a (2 MT) x (2 MT) block of 2x2 MFMA tiles.
 //   float : v_mfma_f32_32x32x2f32, MT = 32, workgroup tile 128 x 128
@@ -2013,33 +1919,21 @@ constexpr const char* type_name() {
 std::pair<int, int> GemmPlan::add(const std::vector<GemmDesc>& d, bool transA) {
     constexpr int TILE = kMfmaTile;
     const int first = (int)launches.size();
-    // The short-k kernel pays off where a launch is latency-bound: few tiles
-    // (one or a few boxes, as in the top levels' block inverses).  With more
-    // tiles its 64 KB of LDS per workgroup limits occupancy, and the
-    // slab-pipelined kernels are faster.
-    auto shortk = [](const GemmDesc& g) {
-        return g.k <= kShortK && (long long)g.m * g.n <= kShortKCells;
-    };
-    long long sk_tiles = 0;
-    for (const GemmDesc& g : d)
-        if (g.m > 0 && g.n > 0 && shortk(g))
-            sk_tiles += (long long)((g.m + BM - 1) / BM) * ((g.n + BN - 1) / BN);
-    const bool use_sk = sk_tiles <= kShortKMaxTiles;
-    // kinds: 0 k_gemm (FMA, 64 x 64 tiles), 1 k_gemm_mfma, 2 k_gemm_sk; -1
-    // empty.  Classified once, then one launch per kind in descriptor order.
+    // kinds: 0 k_gemm (FMA, 64 x 64 tiles), 1 k_gemm_mfma; -1 empty.
+    // Classified once, then one launch per kind in descriptor order.
     std::vector<signed char> kinds(d.size());
-    int count[3] = {0, 0, 0};
+    int count[2] = {0, 0};
     for (size_t i = 0; i < d.size(); ++i) {
         const GemmDesc& g = d[i];
         if (g.m <= 0 || g.n <= 0) {
             kinds[i] = -1;
             continue;
         }
-        // short products over small outputs (the in-window updates of the
-        // block inverse) finish sooner on 64 x 64 tiles
-        const bool sk = use_sk && shortk(g);
-        // products of depth 32 or less (the Gram panels' trailing updates)
-        // are bound by the output tile's read and write on any kernel
+        // Products of depth 64 or less (the block inverse's in-window
+        // updates, the Gram panels' trailing updates) are bound by the output
+        // tile's read and write, or by latency where a level has few boxes:
+        // the FMA kernel's 64 x 64 tiles, with the next slab prefetched,
+        // finish them sooner than the 128 x 128 matrix-core tiles.
         const bool small = g.k <= 64;
         // the MFMA kernel addresses each operand with 32-bit byte offsets:
         // keep operand extents (8-byte elements, plus a slab and a tile of
@@ -2047,14 +1941,14 @@ std::pair<int, int> GemmPlan::add(const std::vector<GemmDesc>& d, bool transA) {
         const double ext_a = (double)g.lda * (transA ? g.m : g.k) + 16.0 * g.lda + 256;
         const double ext_b = (double)g.ldb * g.n + 16.0 * g.ldb + 256;
         const bool fits = 8.0 * std::max(ext_a, ext_b) < 2147483648.0;
-        const bool large = g.m >= TILE / 2 && g.n >= TILE / 2 && !sk && !small && fits;
-        const int kind = sk ? 2 : large ? 1 : 0;
+        const bool large = g.m >= TILE / 2 && g.n >= TILE / 2 && !small && fits;
+        const int kind = large ? 1 : 0;
         kinds[i] = (signed char)kind;
         ++count[kind];
     }
     desc.reserve(desc.size() + d.size());
     tiles.reserve(tiles.size() + d.size() + 3);
-    for (int pass = 0; pass < 3; ++pass) {
+    for (int pass = 0; pass < 2; ++pass) {
         if (count[pass] == 0) continue;
         const int d0 = (int)desc.size(), t0 = (int)tiles.size();
         const int tile = pass == 1 ? TILE : BM;
@@ -2091,18 +1985,11 @@ void gemm_run(const GemmPlan& p, std::pair<int, int> h, const T* d_A, const T* d
         const char* cn = type_name<TC>();
         const char* ta = L.transA ? "true" : "false";
         if (kt) {
-            static const char* const kKernel[3] = {"k_gemm", "k_gemm_mfma", "k_gemm_sk"};
-            std::snprintf(name, sizeof name, "%s<%s, %s, %s>", kKernel[L.mfma], tn, ta, cn);
+            std::snprintf(name, sizeof name, "%s<%s, %s, %s>", L.mfma ? "k_gemm_mfma" : "k_gemm",
+                          tn, ta, cn);
             kt->begin(s, name);
         }
-        if (L.mfma == 2) {
-            if (L.transA)
-                k_gemm_sk<T, true, TC><<<L.total, 256, 0, s>>>(L.nb, tt, dd, d_A, d_B, d_C, alpha,
-                                                               beta, d_flags);
-            else
-                k_gemm_sk<T, false, TC><<<L.total, 256, 0, s>>>(L.nb, tt, dd, d_A, d_B, d_C,
-                                                                alpha, beta, d_flags);
-        } else if (!L.mfma) {
+        if (!L.mfma) {
             if (L.transA)
                 k_gemm<T, true, TC><<<L.total, 256, 0, s>>>(L.nb, tt, dd, d_A, d_B, d_C, alpha,
                                                             beta, d_flags);

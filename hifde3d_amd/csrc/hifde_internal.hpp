@@ -148,7 +148,7 @@ struct CopyDesc {
 // one copy each and then launched one by one (gemm_run).
 struct GemmPlan {
     struct Launch {
-        int mfma;  // kernel: 0 FMA tiles, 1 matrix cores, 2 short-k (one load round)
+        int mfma;  // kernel: 0 FMA tiles, 1 matrix cores
         int transA, nb, d0, t0, total;
         double flops, ab_elems, cells;  // algorithmic work of the launch
     };

@@ -465,9 +465,9 @@ def test_c2_n64_against_oracle_golden(H, precision):
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 1e-5)])
 @pytest.mark.parametrize("transA", [False, True])
 def test_gemm_batch_matches_numpy(H, precision, tol, transA):
-    """The three GEMM kernels against numpy: k <= 64 (one load round per 64x64
-    tile), 64x64 FMA tiles for thin boxes with longer k, MFMA tiles for large
-    ones (f32 32x32x2 and f64 16x16x4 accumulator layouts)."""
+    """Both GEMM kernels against numpy: 64x64 FMA tiles (depth <= 64, and thin
+    boxes), MFMA tiles for the large ones (f32 32x32x2 and f64 16x16x4
+    accumulator layouts)."""
     rng = _rng(4)
     shapes = [(27, 54, 27), (54, 54, 27), (130, 70, 33), (200, 260, 64), (64, 64, 1), (1, 300, 5),
               (129, 131, 17), (33, 40, 200), (150, 140, 96), (260, 200, 130), (70, 300, 65)]
