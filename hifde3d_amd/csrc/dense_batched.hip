@@ -1014,7 +1014,8 @@ __global__ void __launch_bounds__(256) k_sym_copy(int ndesc, const long long* __
     const int tm = (d.rows + 63) / 64;
     const int local = (int)(blk - toff[b]);
     const int i0 = (local % tm) * 64, j0 = (local / tm) * 64;
-    if (d.upper && i0 >= j0 + 63) return;  // no element with i < j
+    if (d.upper == 1 && i0 >= j0 + 63) return;  // no element with i < j
+    if (d.upper == 2 && j0 >= i0 + 63) return;  // no element with i > j
     T* dst = (d.dsel ? SW : A) + d.dst;
     const T* src = (d.ssel ? SW : A) + d.src;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
@@ -1029,10 +1030,13 @@ __global__ void __launch_bounds__(256) k_sym_copy(int ndesc, const long long* __
     }
     for (int c = ty; c < 64; c += 4) {
         const int i = i0 + tx, j = j0 + c;
-        if (i >= d.rows || j >= d.cols || (d.upper && i >= j)) continue;
+        if (i >= d.rows || j >= d.cols || (d.upper == 1 && i >= j) || (d.upper == 2 && i <= j))
+            continue;
         const T v = d.trans ? t[tx][c] : src[i + (size_t)j * d.lds];
         const int sg = j >= d.neg_from ? -d.s0 : d.s0;
-        dst[i + (size_t)j * d.ldd] = sg > 0 ? v : -v;
+        T* e = dst + i + (size_t)j * d.ldd;
+        const T w = sg > 0 ? v : -v;
+        *e = d.avg ? (T)0.5 * (*e + w) : w;
     }
 }
 
@@ -1777,6 +1781,21 @@ void batched_gj_inverse(int nbox, const int* h_n, const long long* h_off, const 
                 //   Vt(:, J > K)   = -Z(J, K)^T     (a column block, transposed)
                 // and Wt = Vt^T D = sigma W, so the update is +Wt_I Vt_J for
                 // I in P and -Wt_I Vt_J below K.
+                // D, the inverted window, is symmetric in exact arithmetic and
+                // the update below relies on D^T = D: it is made exactly
+                // symmetric, (D + D^T) / 2, which is also nearer the true
+                // inverse, before use.
+                std::vector<SymCopy> sy1, sy2;
+                for (int j = 0; j < nbig && !exec; ++j) {
+                    const int kb = wn[w0 + j];
+                    if (kb == 0 || h_n[bix[j]] <= kGJPanel) continue;
+                    const long long wo = woff[w0 + j];
+                    const int ld = wld[w0 + j];
+                    sy1.push_back({wo, wo, kb, kb, ld, ld, 1, 1, kAll, 1, 0, 0, 1});
+                    sy2.push_back({wo, wo, kb, kb, ld, ld, 1, 2, kAll, 1, 0, 0, 0});
+                }
+                scopy(sy1);
+                scopy(sy2);
                 std::vector<SymCopy> gv, wb;
                 std::vector<GemmDesc> gw, tl, bq;
                 for (int j = 0; j < nbig && !exec; ++j) {

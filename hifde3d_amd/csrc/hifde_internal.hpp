@@ -301,12 +301,14 @@ struct GemmWork {
     DevBuf<long long> off;
 };
 // dst(i, j) = sign(j) src(i, j), or src(j, i) when trans, for i < rows and
-// j < cols (only i < j when upper); sign(j) = -s0 for j >= neg_from, s0
-// before.  sel: 0 the matrices, 1 the symmetric-path workspace.
+// j < cols (only i < j when upper == 1, only i > j when upper == 2);
+// sign(j) = -s0 for j >= neg_from, s0 before; avg: dst(i, j) becomes the mean
+// of its value and that.  sel: 0 the matrices, 1 the symmetric-path workspace.
 struct SymCopy {
     long long dst, src;
     int rows, cols, ldd, lds;
     int trans, upper, neg_from, s0, dsel, ssel;
+    int avg;
 };
 struct GJWork {
     DevBuf<int> wn, wld;              // per outer step and box: window size, leading dim
