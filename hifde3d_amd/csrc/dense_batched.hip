@@ -1028,6 +1028,27 @@ __global__ void __launch_bounds__(256) k_sym_copy(int ndesc, const long long* __
         }
         __syncthreads();
     }
+    if (d.avg == 2) {
+        // symmetrize in place (dst == src, trans, upper == 1): both triangles
+        // take (a_ij + a_ji) / 2.  Each thread leaves its mean in the LDS
+        // slot it read, and the lower triangle is written from there so
+        // that its writes run along columns too.
+        for (int c = ty; c < 64; c += 4) {
+            const int i = i0 + tx, j = j0 + c;
+            if (i >= d.rows || j >= d.cols || i >= j) continue;
+            T* e = dst + i + (size_t)j * d.ldd;
+            const T v = (T)0.5 * (*e + t[tx][c]);
+            *e = v;
+            t[tx][c] = v;
+        }
+        __syncthreads();
+        for (int c = ty; c < 64; c += 4) {
+            const int j = j0 + tx, i = i0 + c;  // lower element (j, i), j > i
+            if (i >= d.rows || j >= d.cols || i >= j) continue;
+            dst[j + (size_t)i * d.ldd] = t[c][tx];
+        }
+        return;
+    }
     for (int c = ty; c < 64; c += 4) {
         const int i = i0 + tx, j = j0 + c;
         if (i >= d.rows || j >= d.cols || (d.upper == 1 && i >= j) || (d.upper == 2 && i <= j))
@@ -1785,17 +1806,15 @@ void batched_gj_inverse(int nbox, const int* h_n, const long long* h_off, const 
                 // the update below relies on D^T = D: it is made exactly
                 // symmetric, (D + D^T) / 2, which is also nearer the true
                 // inverse, before use.
-                std::vector<SymCopy> sy1, sy2;
+                std::vector<SymCopy> sy;
                 for (int j = 0; j < nbig && !exec; ++j) {
                     const int kb = wn[w0 + j];
                     if (kb == 0 || h_n[bix[j]] <= kGJPanel) continue;
                     const long long wo = woff[w0 + j];
                     const int ld = wld[w0 + j];
-                    sy1.push_back({wo, wo, kb, kb, ld, ld, 1, 1, kAll, 1, 0, 0, 1});
-                    sy2.push_back({wo, wo, kb, kb, ld, ld, 1, 2, kAll, 1, 0, 0, 0});
+                    sy.push_back({wo, wo, kb, kb, ld, ld, 1, 1, kAll, 1, 0, 0, 2});
                 }
-                scopy(sy1);
-                scopy(sy2);
+                scopy(sy);
                 std::vector<SymCopy> gv, wb;
                 std::vector<GemmDesc> gw, tl, bq;
                 for (int j = 0; j < nbig && !exec; ++j) {

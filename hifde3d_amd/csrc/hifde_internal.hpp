@@ -302,8 +302,10 @@ struct GemmWork {
 };
 // dst(i, j) = sign(j) src(i, j), or src(j, i) when trans, for i < rows and
 // j < cols (only i < j when upper == 1, only i > j when upper == 2);
-// sign(j) = -s0 for j >= neg_from, s0 before; avg: dst(i, j) becomes the mean
-// of its value and that.  sel: 0 the matrices, 1 the symmetric-path workspace.
+// sign(j) = -s0 for j >= neg_from, s0 before; avg 1: dst(i, j) becomes the
+// mean of its value and that; avg 2 (dst == src, trans, upper == 1): the
+// block is made symmetric, both triangles taking (a_ij + a_ji) / 2.
+// sel: 0 the matrices, 1 the symmetric-path workspace.
 struct SymCopy {
     long long dst, src;
     int rows, cols, ldd, lds;
