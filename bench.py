@@ -321,6 +321,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--bf16x3", action="store_true",
+                    help="FP32 levels' Schur products as bf16x3 split products")
     ap.add_argument("--cpu-n", type=int, default=32)
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--batch-threads", type=int, default=8,
@@ -366,6 +368,8 @@ def main():
     d_x = torch.zeros_like(d_b)
 
     f = H.HIFFactor(N, P, eps, precision="mixed")
+    if args.bf16x3:
+        f.set_option("bf16x3", 1)
     f.set_stencil(a, V, symmetric=True)
     f.set_profiling(True)
     torch.cuda.synchronize()
@@ -447,9 +451,12 @@ def main():
     out = {
         "metric": METRIC, "value": value, "unit": "factor+solve/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16x3+f32+f64" if args.bf16x3 else "f32+f64",
         "data": "synthetic",
-        "config": {"workload": (f"HIF-DE factor (mixed: FP32 Schur/LU, FP64 ID pivots) + FP64 "
+        "config": {"workload": (f"HIF-DE factor (mixed: FP32 Schur/LU"
+                                f"{' with bf16x3 Schur products' if args.bf16x3 else ''}, "
+                                f"FP64 ID pivots) + FP64 "
                                 f"PCG to {tol:g}, n={n} ({N}^3 unknowns), lognormal GRF a "
                                 f"(corr {corr}, seed {seed}), symmetric 7-pt stencil, P={P}, "
                                 f"eps={eps:g}, 1 RHS N(0,1)"),

@@ -325,9 +325,14 @@ def interp_decomp_batch(mats, eps):
     return out
 
 
-def gemm_batch(pairs, precision="fp64", transA=False):
-    """[op(A) @ B for (A, B) in pairs] through the library's batched GEMM."""
-    prec = {"fp64": FP64, "mixed": MIXED}[precision]
+BF16X3 = 0x200
+
+
+def gemm_batch(pairs, precision="fp64", transA=False, bf16x3=False):
+    """[op(A) @ B for (A, B) in pairs] through the library's batched GEMM.
+    bf16x3 (precision "mixed" only): matrix-core products as bf16x3 split
+    products."""
+    prec = {"fp64": FP64, "mixed": MIXED}[precision] | (BF16X3 if bf16x3 else 0)
     m = np.array([(a.shape[1] if transA else a.shape[0]) for a, _ in pairs], np.int32)
     k = np.array([(a.shape[0] if transA else a.shape[1]) for a, _ in pairs], np.int32)
     n = np.array([b.shape[1] for _, b in pairs], np.int32)
@@ -350,10 +355,12 @@ def gemm_batch(pairs, precision="fp64", transA=False):
 SCHUR_SYMMETRIC = 0x100
 
 
-def schur_batch(blocks, precision="fp64", symmetric=False):
+def schur_batch(blocks, precision="fp64", symmetric=False, bf16x3=False):
     """Schur on a list of (A22, A21, A12).  Returns [(A22inv, X, S, info)].
-    symmetric: every A22 is symmetric (only its lower triangle is read)."""
-    prec = {"fp64": FP64, "mixed": MIXED}[precision] | (SCHUR_SYMMETRIC if symmetric else 0)
+    symmetric: every A22 is symmetric (only its lower triangle is read).
+    bf16x3 (precision "mixed" only): X and S as bf16x3 split products."""
+    prec = ({"fp64": FP64, "mixed": MIXED}[precision] | (SCHUR_SYMMETRIC if symmetric else 0)
+            | (BF16X3 if bf16x3 else 0))
     r = np.array([b[0].shape[0] for b in blocks], np.int32)
     s = np.array([b[1].shape[1] for b in blocks], np.int32)
 

@@ -16,6 +16,7 @@
 //                   Schur needs.
 
 #include <cstdio>
+#include <type_traits>
 
 #include "hifde_internal.hpp"
 
@@ -750,6 +751,22 @@ __device__ __forceinline__ float buf_load(__amdgpu_buffer_rsrc_t r, unsigned off
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
 }
 
+// Split products (bf16x3): x = hi + lo with hi = bf16(x) and lo = bf16(x - hi),
+// both rounded to nearest even; a b ~ hi_a hi_b + hi_a lo_b + lo_a hi_b, the
+// lo_a lo_b term (2^-16 relative) dropped, summed in FP32 by
+// v_mfma_f32_32x32x16_bf16.  A lane's 8 k of a slab are one operand of that
+// instruction; its k order inside the instruction is the same for both
+// operands, so any consistent assignment sums every k once.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ void split_bf16(const float* x, bf16x8& hi, bf16x8& lo) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const __bf16 h = (__bf16)x[j];
+        hi[j] = h;
+        lo[j] = (__bf16)(x[j] - (float)h);
+    }
+}
+
 // LDS row pitch (elements) of a k-slab row: rows are read 32 bytes per lane
 // (its 4 doubles or 8 floats of k), and the pitch spreads 16 consecutive rows
 // over all 64 banks.
@@ -778,7 +795,9 @@ struct SlabPitch<float> {
 // within the operand's buffer (those outputs are not stored); k past the
 // end reads zero, from the buffer bound or an explicit select where the
 // operand is k-contiguous.  The host keeps each operand under 2 GiB.
-template <class T, bool TRANSA, class TC = T>
+// SPLIT (float only): each slab's products are three bf16 matrix-core
+// products of the split operands (split_bf16) instead of eight FP32 ones.
+template <class T, bool TRANSA, class TC = T, bool SPLIT = false>
 __global__ void __launch_bounds__(256, 2) k_gemm_mfma(int nbox, const int* __restrict__ tile_off,
                                                       const GemmDesc* __restrict__ desc,
                                                       const T* __restrict__ Ab,
@@ -881,6 +900,12 @@ __global__ void __launch_bounds__(256, 2) k_gemm_mfma(int nbox, const int* __res
 #pragma unroll
             for (int s2 = 0; s2 < SPL; ++s2) bv[c][s2] = src[s2];
         }
+        bf16x8 bh[SPLIT ? BLK : 1], bl[SPLIT ? BLK : 1];
+        if constexpr (SPLIT) {
+            static_assert(sizeof(T) == 4 && SPL == 8, "split products: float slabs");
+#pragma unroll
+            for (int c = 0; c < BLK; ++c) split_bf16(bv[c], bh[c], bl[c]);
+        }
         // The next slab goes to the other LDS buffer halfway through the
         // products (its loads have landed by then), so the slab ends with
         // the barrier alone instead of a store-and-wait.
@@ -895,10 +920,21 @@ __global__ void __launch_bounds__(256, 2) k_gemm_mfma(int nbox, const int* __res
             const T* src = &As[cur][wr + a * MT + rr][g * SPL];
 #pragma unroll
             for (int s2 = 0; s2 < SPL; ++s2) av[s2] = src[s2];
+            if constexpr (SPLIT) {
+                bf16x8 ah, al;
+                split_bf16(av, ah, al);
 #pragma unroll
-            for (int s2 = 0; s2 < SPL; ++s2)
+                for (int c = 0; c < BLK; ++c) {
+                    acc[a][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bl[c], ah, acc[a][c], 0, 0, 0);
+                    acc[a][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bh[c], al, acc[a][c], 0, 0, 0);
+                    acc[a][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bh[c], ah, acc[a][c], 0, 0, 0);
+                }
+            } else {
 #pragma unroll
-                for (int c = 0; c < BLK; ++c) acc[a][c] = C::mma(bv[c][s2], av[s2], acc[a][c]);
+                for (int s2 = 0; s2 < SPL; ++s2)
+#pragma unroll
+                    for (int c = 0; c < BLK; ++c) acc[a][c] = C::mma(bv[c][s2], av[s2], acc[a][c]);
+            }
         }
         __syncthreads();
     }
@@ -1990,7 +2026,7 @@ std::pair<int, int> GemmPlan::add(const std::vector<GemmDesc>& d, bool transA) {
         if (count[pass] == 0) continue;
         const int d0 = (int)desc.size(), t0 = (int)tiles.size();
         const int tile = pass == 1 ? TILE : BM;
-        Launch L{pass, transA ? 1 : 0, 0, d0, t0, 0, 0.0, 0.0, 0.0};
+        Launch L{pass, transA ? 1 : 0, 0, d0, t0, 0, 0.0, 0.0, 0.0, pass == 1 && split ? 1 : 0};
         tiles.push_back(0);
         for (size_t i = 0; i < d.size(); ++i) {
             if (kinds[i] != pass) continue;
@@ -2023,7 +2059,8 @@ void gemm_run(const GemmPlan& p, std::pair<int, int> h, const T* d_A, const T* d
         const char* cn = type_name<TC>();
         const char* ta = L.transA ? "true" : "false";
         if (kt) {
-            std::snprintf(name, sizeof name, "%s<%s, %s, %s>", L.mfma ? "k_gemm_mfma" : "k_gemm",
+            std::snprintf(name, sizeof name, "%s<%s, %s, %s>",
+                          L.mfma ? (L.split ? "k_gemm_mfma_bf16x3" : "k_gemm_mfma") : "k_gemm",
                           tn, ta, cn);
             kt->begin(s, name);
         }
@@ -2034,6 +2071,17 @@ void gemm_run(const GemmPlan& p, std::pair<int, int> h, const T* d_A, const T* d
             else
                 k_gemm<T, false, TC><<<L.total, 256, 0, s>>>(L.nb, tt, dd, d_A, d_B, d_C, alpha,
                                                              beta, d_flags);
+        } else if (L.split) {
+            if constexpr (std::is_same<T, float>::value) {
+                if (L.transA)
+                    k_gemm_mfma<T, true, TC, true><<<L.total, 256, 0, s>>>(
+                        L.nb, tt, dd, d_A, d_B, d_C, alpha, beta, d_flags);
+                else
+                    k_gemm_mfma<T, false, TC, true><<<L.total, 256, 0, s>>>(
+                        L.nb, tt, dd, d_A, d_B, d_C, alpha, beta, d_flags);
+            } else {
+                throw Error(-1, "split (bf16x3) products need FP32 operands");
+            }
         } else {
             if (L.transA)
                 k_gemm_mfma<T, true, TC><<<L.total, 256, 0, s>>>(L.nb, tt, dd, d_A, d_B, d_C,

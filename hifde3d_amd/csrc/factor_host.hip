@@ -169,6 +169,7 @@ struct hifde_factor {
     DevBuf<SliceSum> d_gsum;
     static constexpr long long kGramSplitTiles = 1536;  // three waves of 512 workgroups
     bool sym_gj = true;           // symmetric level matrix: half-flop block inverses
+    bool split_schur = false;     // FP32 levels: Schur products as bf16x3 split products
     int host_threads = 0;         // OpenMP threads of the host planning (0: the caller's)
     bool keep_workspaces = true;  // keep the factorization workspaces for the next one
     int gram_min_n = 16;          // faces with more columns use the Gram path
@@ -665,6 +666,12 @@ struct hifde_factor {
         }
         fold_info(nb, ws_info.p, d_status.p, stream);
         timed(ST_INVERSE, t0);
+        // FP32 levels may take the Schur products as bf16x3 split products
+        struct SplitGuard {
+            GemmPlan& p;
+            SplitGuard(GemmPlan& q, bool on) : p(q) { p.split = on; }
+            ~SplitGuard() { p.split = false; }
+        } split_guard(gemm_ws.plan, split_schur && sizeof(T) == sizeof(float));
         {
             Prof pr(this, fam<T>("gemm_x"), fl_x, by_x);
             batched_gemm<T>(false, nb, B.r.data(), B.s.data(), B.r.data(), B.off_rr.data(),
@@ -2561,6 +2568,7 @@ int hifde_set_option(hifde_factor* f, const char* name, double value) {
         else if (n == "stage_sync") f->stage_sync = value != 0;
         else if (n == "gram_panel_reg") f->gram_ws.reg_panel = value != 0;
         else if (n == "symmetric_inverse") f->sym_gj = value != 0;
+        else if (n == "bf16x3") f->split_schur = value != 0;
         else if (n == "host_threads") f->host_threads = std::max(0, (int)value);
         else if (n == "keep_workspaces") {
             f->keep_workspaces = value != 0;
@@ -2676,7 +2684,7 @@ namespace {
 template <class T>
 int schur_batch_impl(int nb, const int* r, const int* s, const double* A22, const double* A21,
                      const double* A12, double* A22inv, double* X, double* S, int* info,
-                     bool sym) {
+                     bool sym, bool split) {
     hipStream_t st = nullptr;
     std::vector<int> hr(r, r + nb), hs(s, s + nb);
     std::vector<long long> orr = prefix_ll(hr, hr), ors = prefix_ll(hr, hs), oss = prefix_ll(hs, hs);
@@ -2705,6 +2713,7 @@ int schur_batch_impl(int nb, const int* r, const int* s, const double* A22, cons
     GemmWork gws;
     batched_gj_inverse<T>(nb, hr.data(), orr.data(), dr.p, drr.p, dA.p, dinfo.p, gw, gws, st,
                           sym);
+    gws.plan.split = split;
     batched_gemm<T>(false, nb, hr.data(), hs.data(), hr.data(), orr.data(), ors.data(), ors.data(),
                     dA.p, dA21.p, dX.p, (T)1, gws, st);
     batched_gemm<T>(false, nb, hs.data(), hs.data(), hr.data(), ors.data(), ors.data(), oss.data(),
@@ -2725,7 +2734,7 @@ int schur_batch_impl(int nb, const int* r, const int* s, const double* A22, cons
 namespace {
 template <class T>
 int gemm_batch_impl(int transA, int nb, const int* m, const int* n, const int* k, const double* A,
-                    const double* B, double* C) {
+                    const double* B, double* C, bool split) {
     hipStream_t st = nullptr;
     std::vector<long long> oA(nb + 1, 0), oB(nb + 1, 0), oC(nb + 1, 0);
     for (int b = 0; b < nb; ++b) {
@@ -2743,6 +2752,7 @@ int gemm_batch_impl(int transA, int nb, const int* m, const int* n, const int* k
     up(B, oB[nb], dB);
     dC.reserve(oC[nb]);
     GemmWork gws;
+    gws.plan.split = split;
     batched_gemm<T>(transA != 0, nb, m, n, k, oA.data(), oB.data(), oC.data(), dA.p, dB.p, dC.p,
                     (T)1, gws, st);
     std::vector<T> tmp(oC[nb]);
@@ -2761,8 +2771,13 @@ int hifde_gemm_batch(int precision, int transA, int nbatch, const int* m, const 
         if (nbatch < 0 || (nbatch && (!m || !n || !k || !A || !B || !C)))
             throw Error(HIFDE_ERR_ARG, "gemm_batch: bad arguments");
         if (nbatch == 0) return HIFDE_OK;
-        if (precision == HIFDE_MIXED) return gemm_batch_impl<float>(transA, nbatch, m, n, k, A, B, C);
-        return gemm_batch_impl<double>(transA, nbatch, m, n, k, A, B, C);
+        const bool split = (precision & HIFDE_BF16X3) != 0;
+        const int prec = precision & ~HIFDE_BF16X3;
+        if (split && prec != HIFDE_MIXED)
+            throw Error(HIFDE_ERR_ARG, "gemm_batch: HIFDE_BF16X3 needs HIFDE_MIXED");
+        if (prec == HIFDE_MIXED)
+            return gemm_batch_impl<float>(transA, nbatch, m, n, k, A, B, C, split);
+        return gemm_batch_impl<double>(transA, nbatch, m, n, k, A, B, C, false);
     });
 }
 
@@ -2775,9 +2790,15 @@ int hifde_schur_batch(int precision, int nbatch, const int* r, const int* s, con
             throw Error(HIFDE_ERR_ARG, "schur_batch: bad arguments");
         if (nbatch == 0) return HIFDE_OK;
         const bool sym = (precision & HIFDE_SCHUR_SYMMETRIC) != 0;
-        if ((precision & ~HIFDE_SCHUR_SYMMETRIC) == HIFDE_MIXED)
-            return schur_batch_impl<float>(nbatch, r, s, A22, A21, A12, A22inv, X, S, info, sym);
-        return schur_batch_impl<double>(nbatch, r, s, A22, A21, A12, A22inv, X, S, info, sym);
+        const bool split = (precision & HIFDE_BF16X3) != 0;
+        const int prec = precision & ~(HIFDE_SCHUR_SYMMETRIC | HIFDE_BF16X3);
+        if (split && prec != HIFDE_MIXED)
+            throw Error(HIFDE_ERR_ARG, "schur_batch: HIFDE_BF16X3 needs HIFDE_MIXED");
+        if (prec == HIFDE_MIXED)
+            return schur_batch_impl<float>(nbatch, r, s, A22, A21, A12, A22inv, X, S, info, sym,
+                                           split);
+        return schur_batch_impl<double>(nbatch, r, s, A22, A21, A12, A22inv, X, S, info, sym,
+                                        false);
     });
 }
 

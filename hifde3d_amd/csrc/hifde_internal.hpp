@@ -151,7 +151,11 @@ struct GemmPlan {
         int mfma;  // kernel: 0 FMA tiles, 1 matrix cores
         int transA, nb, d0, t0, total;
         double flops, ab_elems, cells;  // algorithmic work of the launch
+        int split;  // matrix-core launch with bf16x3 split products (FP32 only)
     };
+    // Launches added while set use the split (bf16x3) matrix-core products;
+    // clear() leaves it as it is.
+    bool split = false;
     std::vector<GemmDesc> desc;
     std::vector<int> tiles;  // per launch: nb + 1 local tile offsets
     std::vector<Launch> launches;

@@ -165,6 +165,10 @@ const char* hifde_stage_name(int stage);
  *   "symmetric_inverse"  1 (default): with the symmetric level matrix the
  *                  block inverses read only lower triangles and update only
  *                  lower triangles (half the flops); 0: general inverse
+ *   "bf16x3"       0 (default); 1: on the FP32 levels of HIFDE_MIXED, the
+ *                  Schur products X = A22^{-1} A21 and S = -A12 X of depth
+ *                  above 64 run as bf16x3 split products (see HIFDE_BF16X3);
+ *                  inverses, IDs and the FP64 levels are unchanged
  *   "keep_workspaces"  1 (default): the factorization's workspaces (level-
  *                  matrix elements, face and Schur blocks, inverse panels)
  *                  stay allocated for the next hifde_factor_run; 0 frees
@@ -209,6 +213,13 @@ int hifde_interp_decomp_batch(int nbatch, const int* m, const int* n, const doub
  * blocks wider than 512 take half the flops (no counterpart in the
  * reference, whose Schur inverts by LU).  info[b] as LU. */
 #define HIFDE_SCHUR_SYMMETRIC 0x100
+/* OR-ed with HIFDE_MIXED into the precision of hifde_schur_batch (its X and S
+ * products) or hifde_gemm_batch: matrix-core products of depth above 64 are
+ * bf16x3 split products (each FP32 operand split into two bf16 parts, three
+ * bf16 products summed in FP32; about 2^-16 relative per product instead of
+ * FP32's 2^-24).  No counterpart in the reference, which multiplies in the
+ * factor's precision. */
+#define HIFDE_BF16X3 0x200
 int hifde_schur_batch(int precision, int nbatch, const int* r, const int* s, const double* A22,
                       const double* A21, const double* A12, double* A22inv, double* X, double* S,
                       int* info);

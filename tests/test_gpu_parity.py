@@ -481,6 +481,70 @@ def test_gemm_batch_matches_numpy(H, precision, tol, transA):
         assert _rel(C, ref) < tol
 
 
+@pytest.mark.parametrize("transA", [False, True])
+def test_bf16x3_split_gemm_error_bound(H, transA):
+    """bf16x3 split products: x = hi + lo, hi = bf16(x), lo = bf16(x - hi)
+    (both round-to-nearest, so |x - hi - lo| <= 2^-18 |x|), summed as
+    hi hi + hi lo + lo hi in FP32.  Elementwise bound: 2^-14 (|A| |B|), which
+    covers the three 2^-18 terms and the FP32 accumulation.  Depth <= 64 and
+    thin products stay on the FP32 FMA kernel and must equal the plain FP32
+    path bit for bit; the others must differ from it (the split kernel ran)
+    and a result with the lo parts dropped (plain bf16, 2^-9) must not pass."""
+    rng = _rng(11)
+    shapes = [(130, 70, 33), (200, 260, 64), (150, 140, 96), (260, 200, 130), (128, 128, 65),
+              (300, 257, 1000), (64, 64, 80), (70, 300, 65)]
+    pairs = []
+    for m, n, k in shapes:
+        A = rng.standard_normal((k, m) if transA else (m, k)) * rng.uniform(0.1, 10, size=1)
+        pairs.append((A, rng.uniform(-1, 1, (k, n))))
+    got = H.gemm_batch(pairs, "mixed", transA, bf16x3=True)
+    plain = H.gemm_batch(pairs, "mixed", transA)
+
+    def bf16(x):
+        u = np.asarray(x, np.float32).view(np.uint32).astype(np.uint64)
+        u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+        return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+    for (m, n, k), (A, B), C, P in zip(shapes, pairs, got, plain):
+        Aop = A.T if transA else A
+        A32 = Aop.astype(np.float32).astype(np.float64)
+        B32 = B.astype(np.float32).astype(np.float64)
+        ref = A32 @ B32
+        bound = 2.0 ** -14 * (np.abs(A32) @ np.abs(B32)) + 1e-30
+        assert np.all(np.abs(C - ref) <= bound), (m, n, k)
+        if k <= 64 or m < 64 or n < 64:
+            assert np.array_equal(C, P), (m, n, k)
+        else:
+            assert not np.array_equal(C, P), (m, n, k)
+            # the bound is tight enough to fail a product of the hi parts only
+            hi_only = bf16(A32) @ bf16(B32)
+            assert not np.all(np.abs(hi_only - ref) <= bound), (m, n, k)
+
+
+def test_bf16x3_schur_factor_converges(H, oracle_mod):
+    """Factor option bf16x3: the FP32 levels' Schur products as split
+    products.  The kernel must run (its launches are recorded), and PCG to
+    1e-10 must take at most two iterations more than the FP32 products."""
+    N, P, eps = 63, 4, 1e-4
+    a, V, coo, A = _coo(oracle_mod, N, "pow10", True)
+    b = pb.rhs_normal(N + 1, 7)
+    its = {}
+    for split in (0, 1):
+        f = H.HIFFactor(N, P, eps, precision="mixed")
+        f.set_option("bf16x3", split)
+        f.set_profiling(True)
+        f.set_stencil(a, V, True)
+        assert f.Factor() == 0
+        names = [k for k in f.kernel_stats() if "bf16x3" in k]
+        assert bool(names) == bool(split), names
+        x, it, rr = f.pcg(b, tol=1e-10, maxit=200)
+        assert rr <= 1e-10
+        assert _rel(A @ x, b) <= 2e-10
+        its[split] = it
+        f.close()
+    assert its[1] <= its[0] + 2, its
+
+
 def test_symmetric_path_matches_general_path(H, oracle_mod):
     """The symmetric fast path (pair-keyed level matrix, lower-triangle S)
     against the general path on the same symmetric operator."""
