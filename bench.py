@@ -321,6 +321,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--host-threads", type=int, default=0,
+                    help="OpenMP threads of the factorization's host planning (0: the default)")
     ap.add_argument("--bf16x3", action="store_true",
                     help="FP32 levels' Schur products as bf16x3 split products")
     ap.add_argument("--cpu-n", type=int, default=32)
@@ -370,6 +372,8 @@ def main():
     f = H.HIFFactor(N, P, eps, precision="mixed")
     if args.bf16x3:
         f.set_option("bf16x3", 1)
+    if args.host_threads:
+        f.set_option("host_threads", args.host_threads)
     f.set_stencil(a, V, symmetric=True)
     f.set_profiling(True)
     torch.cuda.synchronize()
