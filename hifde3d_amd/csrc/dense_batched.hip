@@ -2059,9 +2059,12 @@ void gemm_run(const GemmPlan& p, std::pair<int, int> h, const T* d_A, const T* d
         const char* cn = type_name<TC>();
         const char* ta = L.transA ? "true" : "false";
         if (kt) {
-            std::snprintf(name, sizeof name, "%s<%s, %s, %s>",
-                          L.mfma ? (L.split ? "k_gemm_mfma_bf16x3" : "k_gemm_mfma") : "k_gemm",
-                          tn, ta, cn);
+            // the kernel's name as the profilers print it
+            if (L.mfma)
+                std::snprintf(name, sizeof name, "k_gemm_mfma<%s, %s, %s, %s>", tn, ta, cn,
+                              L.split ? "true" : "false");
+            else
+                std::snprintf(name, sizeof name, "k_gemm<%s, %s, %s>", tn, ta, cn);
             kt->begin(s, name);
         }
         if (!L.mfma) {

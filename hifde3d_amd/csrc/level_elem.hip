@@ -43,21 +43,11 @@ inline unsigned elem_grid(long long n, int per = 256) {
     return (unsigned)g;
 }
 
-// p[0, n) = 0: 16-byte stores over the 16-byte-aligned body, element stores
-// for the unaligned head and the tail.
 template <class T>
-__global__ void __launch_bounds__(256) k_fill(T* p, long long n) {
-    constexpr int V = 16 / sizeof(T);
-    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    const unsigned mis = (unsigned)(reinterpret_cast<uintptr_t>(p) & 15);
-    const long long head = min(n, (long long)(((16 - mis) & 15) / sizeof(T)));
-    if (tid < head) p[tid] = (T)0;
-    const long long nv = (n - head) / V;
-    uint4* q = reinterpret_cast<uint4*>(p + head);
-    for (long long e = tid; e < nv; e += stride) q[e] = make_uint4(0u, 0u, 0u, 0u);
-    const long long t0 = head + nv * V;
-    if (tid < n - t0) p[t0 + tid] = (T)0;
+__global__ void k_fill(T* p, long long n) {
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (long long)gridDim.x * blockDim.x)
+        p[e] = (T)0;
 }
 
 // Level-0 frontal blocks from the assembled operator (CSR).  One workgroup
@@ -275,10 +265,7 @@ std::vector<long long> col_prefix(const std::vector<D>& desc, int (*cols)(const 
 template <class T>
 void fill_zero(T* p, long long n, hipStream_t s) {
     if (n <= 0) return;
-    // 16 bytes per thread and pass, at most 64 workgroups per CU
-    const long long nv = (n * (long long)sizeof(T) + 15) / 16;
-    const long long g = std::min<long long>(std::max<long long>((nv + 255) / 256, 1), 16384);
-    k_fill<T><<<(unsigned)g, 256, 0, s>>>(p, n);
+    k_fill<T><<<elem_grid(n), 256, 0, s>>>(p, n);
     HIFDE_LAUNCH_CHECK();
 }
 

@@ -535,7 +535,8 @@ def test_bf16x3_schur_factor_converges(H, oracle_mod):
         f.set_profiling(True)
         f.set_stencil(a, V, True)
         assert f.Factor() == 0
-        names = [k for k in f.kernel_stats() if "bf16x3" in k]
+        names = [k for k in f.kernel_stats() if k.startswith("kernel:k_gemm_mfma<float")
+                 and k.endswith(", true>")]
         assert bool(names) == bool(split), names
         x, it, rr = f.pcg(b, tol=1e-10, maxit=200)
         assert rr <= 1e-10
