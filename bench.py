@@ -321,6 +321,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--gemm-tile64", type=int, default=0, choices=(0, 1, 2),
+                    help="factorization option gemm_tile64 (64 x 64 matrix-core tiles)")
     ap.add_argument("--host-threads", type=int, default=0,
                     help="OpenMP threads of the factorization's host planning (0: the default)")
     ap.add_argument("--bf16x3", action="store_true",
@@ -374,6 +376,8 @@ def main():
         f.set_option("bf16x3", 1)
     if args.host_threads:
         f.set_option("host_threads", args.host_threads)
+    if args.gemm_tile64:
+        f.set_option("gemm_tile64", args.gemm_tile64)
     f.set_stencil(a, V, symmetric=True)
     f.set_profiling(True)
     torch.cuda.synchronize()

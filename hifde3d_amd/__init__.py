@@ -328,11 +328,16 @@ def interp_decomp_batch(mats, eps):
 BF16X3 = 0x200
 
 
-def gemm_batch(pairs, precision="fp64", transA=False, bf16x3=False):
+GEMM_TILE64 = 0x400
+
+
+def gemm_batch(pairs, precision="fp64", transA=False, bf16x3=False, tile64=False):
     """[op(A) @ B for (A, B) in pairs] through the library's batched GEMM.
     bf16x3 (precision "mixed" only): matrix-core products as bf16x3 split
-    products."""
-    prec = {"fp64": FP64, "mixed": MIXED}[precision] | (BF16X3 if bf16x3 else 0)
+    products.  tile64: products with both output sides of at least 32 on
+    64 x 64 matrix-core tiles."""
+    prec = ({"fp64": FP64, "mixed": MIXED}[precision] | (BF16X3 if bf16x3 else 0)
+            | (GEMM_TILE64 if tile64 else 0))
     m = np.array([(a.shape[1] if transA else a.shape[0]) for a, _ in pairs], np.int32)
     k = np.array([(a.shape[0] if transA else a.shape[1]) for a, _ in pairs], np.int32)
     n = np.array([b.shape[1] for _, b in pairs], np.int32)

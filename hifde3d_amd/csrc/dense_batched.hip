@@ -797,18 +797,20 @@ struct SlabPitch<float> {
 // operand is k-contiguous.  The host keeps each operand under 2 GiB.
 // SPLIT (float only): each slab's products are three bf16 matrix-core
 // products of the split operands (split_bf16) instead of eight FP32 ones.
-template <class T, bool TRANSA, class TC = T, bool SPLIT = false>
+// TL: workgroup tile, 128 (the default) or 64.
+template <class T, bool TRANSA, class TC = T, bool SPLIT = false, int TL = kMfmaTile>
 __global__ void __launch_bounds__(256, 2) k_gemm_mfma(int nbox, const int* __restrict__ tile_off,
                                                       const GemmDesc* __restrict__ desc,
                                                       const T* __restrict__ Ab,
                                                       const T* __restrict__ Bb, TC* Cb, T alpha,
                                                       T beta, const int* __restrict__ flags) {
     typedef MfmaCfg<T> C;
-    constexpr int MT = C::MT, KT = C::KT, BLK = C::BLK, WT = BLK * MT, TILE = 2 * WT, KB = 16;
+    constexpr int MT = C::MT, KT = C::KT, BLK = TL / (2 * MT), WT = BLK * MT, TILE = 2 * WT,
+                  KB = 16;
     constexpr int SPL = KB / KT;           // matrix-core steps per slab
     constexpr int LDK = SlabPitch<T>::value;
     constexpr int NLD = TILE * KB / 256;  // elements per thread and operand per slab
-    static_assert(TILE == kMfmaTile, "tile");
+    static_assert(TILE == TL && BLK >= 1 && TILE % 16 == 0, "tile");
     static_assert(SPL * sizeof(T) == 32, "a lane's slab share is 32 bytes");
     const int t = blockIdx.x;
     const int b = find_box(tile_off, nbox, t);
@@ -1996,7 +1998,15 @@ std::pair<int, int> GemmPlan::add(const std::vector<GemmDesc>& d, bool transA) {
     // kinds: 0 k_gemm (FMA, 64 x 64 tiles), 1 k_gemm_mfma; -1 empty.
     // Classified once, then one launch per kind in descriptor order.
     std::vector<signed char> kinds(d.size());
-    int count[2] = {0, 0};
+    int count[3] = {0, 0, 0};
+    // output tiles a launch computes (lower: tiles with column <= row)
+    auto tiles_of = [](const GemmDesc& g, int tile) {
+        const long long tm = (g.m + tile - 1) / tile, tn = (g.n + tile - 1) / tile;
+        if (!g.lower) return tm * tn;
+        long long c = 0;
+        for (long long j = 0; j < tn; ++j) c += std::max(0LL, tm - j);
+        return c;
+    };
     for (size_t i = 0; i < d.size(); ++i) {
         const GemmDesc& g = d[i];
         if (g.m <= 0 || g.n <= 0) {
@@ -2016,17 +2026,26 @@ std::pair<int, int> GemmPlan::add(const std::vector<GemmDesc>& d, bool transA) {
         const double ext_b = (double)g.ldb * g.n + 16.0 * g.ldb + 256;
         const bool fits = 8.0 * std::max(ext_a, ext_b) < 2147483648.0;
         const bool large = g.m >= TILE / 2 && g.n >= TILE / 2 && !small && fits;
-        const int kind = large ? 1 : 0;
+        int kind = large ? 1 : 0;
+        // 64 x 64 matrix-core tiles: where the 128 x 128 tiles would compute
+        // over a quarter more output (tile1), and also for the products of
+        // depth 64 or less with both sides of at least 32 (tile2)
+        if (tile64 >= 1 && large &&
+            tiles_of(g, TILE) * TILE * TILE > 1.25 * tiles_of(g, TILE / 2) * (TILE / 2) * (TILE / 2))
+            kind = 2;
+        if (tile64 >= 2 && small && g.k > 0 && g.m >= 32 && g.n >= 32 && fits) kind = 2;
+        // 3: every product with both output sides of at least 32
+        if (tile64 >= 3 && g.k > 0 && g.m >= 32 && g.n >= 32 && fits) kind = 2;
         kinds[i] = (signed char)kind;
         ++count[kind];
     }
     desc.reserve(desc.size() + d.size());
     tiles.reserve(tiles.size() + d.size() + 3);
-    for (int pass = 0; pass < 2; ++pass) {
+    for (int pass = 0; pass < 3; ++pass) {
         if (count[pass] == 0) continue;
         const int d0 = (int)desc.size(), t0 = (int)tiles.size();
-        const int tile = pass == 1 ? TILE : BM;
-        Launch L{pass, transA ? 1 : 0, 0, d0, t0, 0, 0.0, 0.0, 0.0, pass == 1 && split ? 1 : 0};
+        const int tile = pass == 1 ? TILE : pass == 2 ? TILE / 2 : BM;
+        Launch L{pass, transA ? 1 : 0, 0, d0, t0, 0, 0.0, 0.0, 0.0, pass >= 1 && split ? 1 : 0};
         tiles.push_back(0);
         for (size_t i = 0; i < d.size(); ++i) {
             if (kinds[i] != pass) continue;
@@ -2060,7 +2079,10 @@ void gemm_run(const GemmPlan& p, std::pair<int, int> h, const T* d_A, const T* d
         const char* ta = L.transA ? "true" : "false";
         if (kt) {
             // the kernel's name as the profilers print it
-            if (L.mfma)
+            if (L.mfma == 2)
+                std::snprintf(name, sizeof name, "k_gemm_mfma<%s, %s, %s, %s, 64>", tn, ta, cn,
+                              L.split ? "true" : "false");
+            else if (L.mfma)
                 std::snprintf(name, sizeof name, "k_gemm_mfma<%s, %s, %s, %s>", tn, ta, cn,
                               L.split ? "true" : "false");
             else
@@ -2074,6 +2096,25 @@ void gemm_run(const GemmPlan& p, std::pair<int, int> h, const T* d_A, const T* d
             else
                 k_gemm<T, false, TC><<<L.total, 256, 0, s>>>(L.nb, tt, dd, d_A, d_B, d_C, alpha,
                                                              beta, d_flags);
+        } else if (L.mfma == 2) {
+            if (L.split) {
+                if constexpr (std::is_same<T, float>::value) {
+                    if (L.transA)
+                        k_gemm_mfma<T, true, TC, true, 64><<<L.total, 256, 0, s>>>(
+                            L.nb, tt, dd, d_A, d_B, d_C, alpha, beta, d_flags);
+                    else
+                        k_gemm_mfma<T, false, TC, true, 64><<<L.total, 256, 0, s>>>(
+                            L.nb, tt, dd, d_A, d_B, d_C, alpha, beta, d_flags);
+                } else {
+                    throw Error(-1, "split (bf16x3) products need FP32 operands");
+                }
+            } else if (L.transA) {
+                k_gemm_mfma<T, true, TC, false, 64><<<L.total, 256, 0, s>>>(
+                    L.nb, tt, dd, d_A, d_B, d_C, alpha, beta, d_flags);
+            } else {
+                k_gemm_mfma<T, false, TC, false, 64><<<L.total, 256, 0, s>>>(
+                    L.nb, tt, dd, d_A, d_B, d_C, alpha, beta, d_flags);
+            }
         } else if (L.split) {
             if constexpr (std::is_same<T, float>::value) {
                 if (L.transA)

@@ -2335,6 +2335,9 @@ int hifde_create(int N, int P, double epsilon, int precision, int flags, hifde_f
         f->eps = epsilon;
         f->precision = precision;
         f->quirk = (flags & HIFDE_FLAG_FRONT_QUIRK) != 0;
+        // 64 x 64 matrix-core tiles where the 128 x 128 ones would waste
+        // over a quarter of their output (option "gemm_tile64")
+        f->gemm_ws.plan.tile64 = f->gj_ws.plan.tile64 = 1;
         HIFDE_CHECK(hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking));
         *out = f.release();
         return HIFDE_OK;
@@ -2569,6 +2572,10 @@ int hifde_set_option(hifde_factor* f, const char* name, double value) {
         else if (n == "gram_panel_reg") f->gram_ws.reg_panel = value != 0;
         else if (n == "symmetric_inverse") f->sym_gj = value != 0;
         else if (n == "bf16x3") f->split_schur = value != 0;
+        else if (n == "gemm_tile64") {
+            if (value < 0 || value > 2) throw Error(HIFDE_ERR_ARG, "gemm_tile64: 0, 1 or 2");
+            f->gemm_ws.plan.tile64 = f->gj_ws.plan.tile64 = (int)value;
+        }
         else if (n == "host_threads") f->host_threads = std::max(0, (int)value);
         else if (n == "keep_workspaces") {
             f->keep_workspaces = value != 0;
@@ -2734,7 +2741,7 @@ int schur_batch_impl(int nb, const int* r, const int* s, const double* A22, cons
 namespace {
 template <class T>
 int gemm_batch_impl(int transA, int nb, const int* m, const int* n, const int* k, const double* A,
-                    const double* B, double* C, bool split) {
+                    const double* B, double* C, bool split, int tile64) {
     hipStream_t st = nullptr;
     std::vector<long long> oA(nb + 1, 0), oB(nb + 1, 0), oC(nb + 1, 0);
     for (int b = 0; b < nb; ++b) {
@@ -2753,6 +2760,7 @@ int gemm_batch_impl(int transA, int nb, const int* m, const int* n, const int* k
     dC.reserve(oC[nb]);
     GemmWork gws;
     gws.plan.split = split;
+    gws.plan.tile64 = tile64;
     batched_gemm<T>(transA != 0, nb, m, n, k, oA.data(), oB.data(), oC.data(), dA.p, dB.p, dC.p,
                     (T)1, gws, st);
     std::vector<T> tmp(oC[nb]);
@@ -2772,12 +2780,13 @@ int hifde_gemm_batch(int precision, int transA, int nbatch, const int* m, const 
             throw Error(HIFDE_ERR_ARG, "gemm_batch: bad arguments");
         if (nbatch == 0) return HIFDE_OK;
         const bool split = (precision & HIFDE_BF16X3) != 0;
-        const int prec = precision & ~HIFDE_BF16X3;
+        const int tile64 = (precision & HIFDE_GEMM_TILE64) ? 3 : 0;
+        const int prec = precision & ~(HIFDE_BF16X3 | HIFDE_GEMM_TILE64);
         if (split && prec != HIFDE_MIXED)
             throw Error(HIFDE_ERR_ARG, "gemm_batch: HIFDE_BF16X3 needs HIFDE_MIXED");
         if (prec == HIFDE_MIXED)
-            return gemm_batch_impl<float>(transA, nbatch, m, n, k, A, B, C, split);
-        return gemm_batch_impl<double>(transA, nbatch, m, n, k, A, B, C, false);
+            return gemm_batch_impl<float>(transA, nbatch, m, n, k, A, B, C, split, tile64);
+        return gemm_batch_impl<double>(transA, nbatch, m, n, k, A, B, C, false, tile64);
     });
 }
 

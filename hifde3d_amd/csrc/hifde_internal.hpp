@@ -148,7 +148,7 @@ struct CopyDesc {
 // one copy each and then launched one by one (gemm_run).
 struct GemmPlan {
     struct Launch {
-        int mfma;  // kernel: 0 FMA tiles, 1 matrix cores
+        int mfma;  // kernel: 0 FMA tiles, 1 matrix cores (128 x 128), 2 (64 x 64)
         int transA, nb, d0, t0, total;
         double flops, ab_elems, cells;  // algorithmic work of the launch
         int split;  // matrix-core launch with bf16x3 split products (FP32 only)
@@ -156,6 +156,12 @@ struct GemmPlan {
     // Launches added while set use the split (bf16x3) matrix-core products;
     // clear() leaves it as it is.
     bool split = false;
+    // 64 x 64 matrix-core tiles (kind 2): 0 never; 1 for products of depth
+    // above 64 whose 128 x 128 tiles would compute over a quarter more
+    // output; 2 also for products of depth 64 or less with both output sides
+    // of at least 32; 3 every product with both output sides of at least 32.
+    // clear() leaves it as it is.
+    int tile64 = 0;
     std::vector<GemmDesc> desc;
     std::vector<int> tiles;  // per launch: nb + 1 local tile offsets
     std::vector<Launch> launches;

@@ -169,6 +169,11 @@ const char* hifde_stage_name(int stage);
  *                  Schur products X = A22^{-1} A21 and S = -A12 X of depth
  *                  above 64 run as bf16x3 split products (see HIFDE_BF16X3);
  *                  inverses, IDs and the FP64 levels are unchanged
+ *   "gemm_tile64"  1 (default): products of depth above 64 whose 128 x 128
+ *                  matrix-core tiles would compute over a quarter more output
+ *                  than 64 x 64 ones run on 64 x 64 tiles; 0: always
+ *                  128 x 128; 2: also products of depth 64 or less with both
+ *                  output sides of at least 32 (otherwise on the FMA kernel)
  *   "keep_workspaces"  1 (default): the factorization's workspaces (level-
  *                  matrix elements, face and Schur blocks, inverse panels)
  *                  stay allocated for the next hifde_factor_run; 0 frees
@@ -220,6 +225,11 @@ int hifde_interp_decomp_batch(int nbatch, const int* m, const int* n, const doub
  * FP32's 2^-24).  No counterpart in the reference, which multiplies in the
  * factor's precision. */
 #define HIFDE_BF16X3 0x200
+/* OR-ed into the precision of hifde_gemm_batch: every product with both
+ * output sides of at least 32 runs on 64 x 64 matrix-core tiles (the kernel
+ * behind the factorization's option "gemm_tile64").  No counterpart in the
+ * reference. */
+#define HIFDE_GEMM_TILE64 0x400
 int hifde_schur_batch(int precision, int nbatch, const int* r, const int* s, const double* A22,
                       const double* A21, const double* A12, double* A22inv, double* X, double* S,
                       int* info);

@@ -481,6 +481,48 @@ def test_gemm_batch_matches_numpy(H, precision, tol, transA):
         assert _rel(C, ref) < tol
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 1e-5)])
+@pytest.mark.parametrize("transA", [False, True])
+def test_gemm_batch_64_tiles_match_numpy(H, precision, tol, transA):
+    """The 64 x 64 matrix-core tiles (f32 one 32x32x2 block per wave, f64
+    2x2 16x16x4 blocks) on ragged shapes, depth from 1 to 300, against numpy;
+    shapes with a side under 32 stay on the FMA kernel."""
+    rng = _rng(5)
+    shapes = [(64, 64, 1), (65, 33, 17), (32, 32, 64), (100, 70, 65), (130, 200, 300),
+              (31, 64, 40), (257, 129, 16), (40, 300, 5)]
+    pairs = []
+    for m, n, k in shapes:
+        A = rng.standard_normal((k, m) if transA else (m, k))
+        pairs.append((A, rng.standard_normal((k, n))))
+    got = H.gemm_batch(pairs, precision, transA, tile64=True)
+    for (A, B), C in zip(pairs, got):
+        ref = (A.T if transA else A) @ B
+        assert _rel(C, ref) < tol
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_gemm_tile64_factor_matches_default(H, oracle_mod, mode):
+    """Factor option gemm_tile64 (1 is the default) against 0, all 128 x 128
+    tiles: the same factorization to rounding (FP64), with the 64 x 64 kernel
+    recorded among its launches."""
+    N, P, eps = 63, 4, 1e-6
+    a, V, coo, A = _coo(oracle_mod, N, "pow10", True)
+    b = pb.rhs_normal(N + 1, 3)
+    out = {}
+    for t64 in (0, mode):
+        f = H.HIFFactor(N, P, eps)
+        f.set_option("gemm_tile64", t64)
+        f.set_profiling(True)
+        f.set_stencil(a, V, True)
+        assert f.Factor() == 0
+        names = [k for k in f.kernel_stats() if k.endswith(", 64>")]
+        assert bool(names) == bool(t64), names
+        out[t64] = (f.Apply(b, True), f.level_counts())
+        f.close()
+    assert out[0][1] == out[mode][1]
+    assert _rel(out[mode][0], out[0][0]) < 1e-9
+
+
 @pytest.mark.parametrize("transA", [False, True])
 def test_bf16x3_split_gemm_error_bound(H, transA):
     """bf16x3 split products: x = hi + lo, hi = bf16(x), lo = bf16(x - hi)
