@@ -2079,12 +2079,9 @@ void gemm_run(const GemmPlan& p, std::pair<int, int> h, const T* d_A, const T* d
         const char* ta = L.transA ? "true" : "false";
         if (kt) {
             // the kernel's name as the profilers print it
-            if (L.mfma == 2)
-                std::snprintf(name, sizeof name, "k_gemm_mfma<%s, %s, %s, %s, 64>", tn, ta, cn,
-                              L.split ? "true" : "false");
-            else if (L.mfma)
-                std::snprintf(name, sizeof name, "k_gemm_mfma<%s, %s, %s, %s>", tn, ta, cn,
-                              L.split ? "true" : "false");
+            if (L.mfma)
+                std::snprintf(name, sizeof name, "k_gemm_mfma<%s, %s, %s, %s, %d>", tn, ta, cn,
+                              L.split ? "true" : "false", L.mfma == 2 ? 64 : kMfmaTile);
             else
                 std::snprintf(name, sizeof name, "k_gemm<%s, %s, %s>", tn, ta, cn);
             kt->begin(s, name);

@@ -577,8 +577,9 @@ def test_bf16x3_schur_factor_converges(H, oracle_mod):
         f.set_profiling(True)
         f.set_stencil(a, V, True)
         assert f.Factor() == 0
+        # kernel:k_gemm_mfma<T, transA, TC, split, tile>
         names = [k for k in f.kernel_stats() if k.startswith("kernel:k_gemm_mfma<float")
-                 and k.endswith(", true>")]
+                 and k[k.index("<") + 1:-1].split(", ")[3] == "true"]
         assert bool(names) == bool(split), names
         x, it, rr = f.pcg(b, tol=1e-10, maxit=200)
         assert rr <= 1e-10
