@@ -1992,6 +1992,10 @@ constexpr const char* type_name() {
     return sizeof(T) == sizeof(float) ? "float" : "double";
 }
 
+// Output computed on 128 x 128 tiles over that on 64 x 64 tiles above which a
+// product of depth above 64 takes the 64 x 64 tiles (GemmPlan::tile64 >= 1).
+constexpr double kTile64Waste = 1.25;
+
 std::pair<int, int> GemmPlan::add(const std::vector<GemmDesc>& d, bool transA) {
     constexpr int TILE = kMfmaTile;
     const int first = (int)launches.size();
@@ -2031,7 +2035,8 @@ std::pair<int, int> GemmPlan::add(const std::vector<GemmDesc>& d, bool transA) {
         // over a quarter more output (tile1), and also for the products of
         // depth 64 or less with both sides of at least 32 (tile2)
         if (tile64 >= 1 && large &&
-            tiles_of(g, TILE) * TILE * TILE > 1.25 * tiles_of(g, TILE / 2) * (TILE / 2) * (TILE / 2))
+            tiles_of(g, TILE) * TILE * TILE >
+                kTile64Waste * tiles_of(g, TILE / 2) * (TILE / 2) * (TILE / 2))
             kind = 2;
         if (tile64 >= 2 && small && g.k > 0 && g.m >= 32 && g.n >= 32 && fits) kind = 2;
         // 3: every product with both output sides of at least 32
