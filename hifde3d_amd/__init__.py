@@ -8,6 +8,7 @@ fallback -- importing on a machine without the built library raises.
 """
 import ctypes
 import os
+import sys
 
 import numpy as np
 
@@ -37,7 +38,7 @@ EXPORTS = [
     "hifde_stage_name", "hifde_factor_bytes", "hifde_interp_decomp_batch", "hifde_schur_batch",
     "hifde_set_profiling", "hifde_set_profile_kernel", "hifde_reset_kernel_stats",
     "hifde_kernel_stats", "hifde_set_option",
-    "hifde_gemm_batch", "hifde_factor_save", "hifde_factor_load",
+    "hifde_gemm_batch", "hifde_factor_save", "hifde_factor_load", "hifde_set_stream",
 ]
 
 
@@ -72,6 +73,7 @@ def lib():
     L.hifde_factor_run.argtypes = [_vp]
     L.hifde_apply.argtypes = [_vp, _dp, _i]
     L.hifde_apply_device.argtypes = [_vp, _vp, _i, _ll, _i]
+    L.hifde_set_stream.argtypes = [_vp, _vp]
     L.hifde_spmv.argtypes = [_vp, _dp, _dp, _i]
     L.hifde_pcg.argtypes = [_vp, _dp, _dp, _d, _i, _ip, _dp]
     L.hifde_pcg_device.argtypes = [_vp, _vp, _vp, _d, _i, _ip, _dp]
@@ -109,6 +111,15 @@ def _check(rc):
 
 def _ptr(a, t=ctypes.c_double):
     return a.ctypes.data_as(ctypes.POINTER(t))
+
+
+def _current_stream():
+    """PyTorch's current stream on this thread (a hipStream_t handle), or 0
+    (the legacy default stream) when PyTorch is not in use."""
+    torch = sys.modules.get("torch")
+    if torch is None or not torch.cuda.is_initialized():
+        return 0
+    return torch.cuda.current_stream().cuda_stream
 
 
 class HIFFactor:
@@ -181,7 +192,12 @@ class HIFFactor:
 
     apply = Apply
 
+    def _follow_caller_stream(self):
+        # device entry points order their work after the caller's stream
+        _check(lib().hifde_set_stream(self._h, ctypes.c_void_p(_current_stream())))
+
     def apply_device(self, ptr, nrhs=1, ldu=None, inverse=True):
+        self._follow_caller_stream()
         _check(lib().hifde_apply_device(self._h, ctypes.c_void_p(ptr), nrhs,
                                         ldu if ldu is not None else self.size, int(inverse)))
 
@@ -203,6 +219,7 @@ class HIFFactor:
     def pcg_device(self, b_ptr, x_ptr, tol=1e-10, maxit=200):
         it = ctypes.c_int(0)
         rr = ctypes.c_double(0)
+        self._follow_caller_stream()
         _check(lib().hifde_pcg_device(self._h, ctypes.c_void_p(b_ptr), ctypes.c_void_p(x_ptr),
                                       tol, maxit, ctypes.byref(it), ctypes.byref(rr)))
         return it.value, rr.value
@@ -212,6 +229,7 @@ class HIFFactor:
         column-major).  Returns per-column (iterations, relative residuals)."""
         it = np.zeros(max(nrhs, 1), np.int32)
         rr = np.zeros(max(nrhs, 1))
+        self._follow_caller_stream()
         _check(lib().hifde_pcg_multi_device(
             self._h, ctypes.c_void_p(b_ptr), ldb if ldb is not None else self.size,
             ctypes.c_void_p(x_ptr), ldx if ldx is not None else self.size, nrhs, tol, maxit,

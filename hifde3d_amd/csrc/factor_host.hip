@@ -42,6 +42,26 @@ struct OwnerGuard {
     explicit OwnerGuard(hipStream_t s) : prev(owner_stream()) { set_owner_stream(s); }
     ~OwnerGuard() { set_owner_stream(prev); }
 };
+// Stream of the calling thread for the batch free functions (no factor
+// object): one non-blocking stream per thread and device, so that these
+// calls never touch the legacy null stream or wait for other threads' work.
+namespace {
+struct AuxStreams {
+    std::vector<hipStream_t> s;
+    ~AuxStreams() {
+        for (hipStream_t x : s)
+            if (x) (void)hipStreamDestroy(x);
+    }
+};
+}  // namespace
+hipStream_t aux_stream() {
+    thread_local AuxStreams t;
+    int dev = 0;
+    HIFDE_CHECK(hipGetDevice(&dev));
+    if ((int)t.s.size() <= dev) t.s.resize(dev + 1, nullptr);
+    if (!t.s[dev]) HIFDE_CHECK(hipStreamCreateWithFlags(&t.s[dev], hipStreamNonBlocking));
+    return t.s[dev];
+}
 
 const void* Stager::stage(const void* src, size_t bytes) {
     const size_t need = (bytes + 255) & ~(size_t)255;
@@ -203,12 +223,17 @@ struct hifde_factor {
     DevBuf<int> d_emap, d_spos, d_etab, d_egid;
 
     // Device buffers handed in by the caller: the factor's stream waits for
-    // the work queued on the default stream before the call (where PyTorch
-    // and most callers produce them), without blocking the host.
+    // the work queued on the caller's stream before the call, without
+    // blocking the host.  The caller's stream is the one named by
+    // hifde_set_stream (PyTorch's current stream through the Python binding);
+    // until one is named it is the legacy default stream.  Naming a stream of
+    // the caller's own keeps the entry points off the legacy stream, which a
+    // multithreaded caller needs while other threads capture graphs.
     hipEvent_t entry_ev = nullptr;
-    void after_default_stream() {
+    hipStream_t caller_stream = nullptr;
+    void after_caller_stream() {
         if (!entry_ev) HIFDE_CHECK(hipEventCreateWithFlags(&entry_ev, hipEventDisableTiming));
-        HIFDE_CHECK(hipEventRecord(entry_ev, nullptr));
+        HIFDE_CHECK(hipEventRecord(entry_ev, caller_stream));
         HIFDE_CHECK(hipStreamWaitEvent(stream, entry_ev, 0));
     }
 
@@ -1597,22 +1622,53 @@ struct hifde_factor {
             }
             return;
         }
-        if (!g->exec) {
-            hipGraph_t graph = nullptr;
-            HIFDE_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-            try {
-                apply_eager(d_u, R, ld, inv);
-            } catch (...) {
-                (void)hipStreamEndCapture(stream, &graph);
-                if (graph) (void)hipGraphDestroy(graph);
-                throw;
-            }
-            HIFDE_CHECK(hipStreamEndCapture(stream, &graph));
-            const hipError_t e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(graph);
-            HIFDE_CHECK(e);
+        if (!g->exec && !capture_graph(g, d_u, R, ld, inv)) {
+            // the capture was refused or invalidated (for instance by a
+            // device-wide synchronize on another host thread): none of the
+            // captured launches ran, so the sweeps run eagerly instead, with
+            // the same launches and the same bits
+            apply_eager(d_u, R, ld, inv);
+            return;
         }
         HIFDE_CHECK(hipGraphLaunch(g->exec, stream));
+    }
+    // Captures the sweeps into g->exec.  Returns false, with the stream out of
+    // capture mode, the partial graph destroyed and the thread's HIP error
+    // state cleared, when the capture fails for any reason (HIP refuses or
+    // invalidates it); the object then stops capturing (graph_failures
+    // counts these).  A failure that is not about the capture recurs in the
+    // eager run that follows and is reported from there.
+    int graph_failures = 0;
+    bool capture_graph(ApplyGraph* g, double* d_u, int R, long long ld, bool inv) {
+        if (graph_failures > 0) return false;
+        hipGraph_t graph = nullptr;
+        if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            (void)hipGetLastError();
+            ++graph_failures;
+            return false;
+        }
+        bool launched = true;
+        try {
+            apply_eager(d_u, R, ld, inv);
+        } catch (const Error&) {
+            launched = false;
+        }
+        const hipError_t ee = hipStreamEndCapture(stream, &graph);
+        if (!launched || ee != hipSuccess || !graph) {
+            if (graph) (void)hipGraphDestroy(graph);
+            (void)hipGetLastError();  // clears the thread's error state
+            ++graph_failures;
+            return false;
+        }
+        const hipError_t e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            g->exec = nullptr;
+            ++graph_failures;
+            return false;
+        }
+        return true;
     }
 
     // ------------------------------------------------------------------
@@ -2008,7 +2064,8 @@ struct hifde_factor {
         std::vector<unsigned char> h((size_t)std::min<long long>(bytes, 64LL << 20));
         for (long long o = 0; o < bytes; o += (long long)h.size()) {
             const size_t n = (size_t)std::min<long long>((long long)h.size(), bytes - o);
-            HIFDE_CHECK(hipMemcpy(h.data(), d + o, n, hipMemcpyDeviceToHost));
+            HIFDE_CHECK(hipMemcpyAsync(h.data(), d + o, n, hipMemcpyDeviceToHost, stream));
+            HIFDE_CHECK(hipStreamSynchronize(stream));
             fl.put(h.data(), n);
         }
     }
@@ -2018,7 +2075,8 @@ struct hifde_factor {
         for (long long o = 0; o < bytes; o += (long long)h.size()) {
             const size_t n = (size_t)std::min<long long>((long long)h.size(), bytes - o);
             fl.get(h.data(), n);
-            HIFDE_CHECK(hipMemcpy(d.p + o, h.data(), n, hipMemcpyHostToDevice));
+            HIFDE_CHECK(hipMemcpyAsync(d.p + o, h.data(), n, hipMemcpyHostToDevice, stream));
+            HIFDE_CHECK(hipStreamSynchronize(stream));
         }
     }
 
@@ -2344,6 +2402,12 @@ int hifde_create(int N, int P, double epsilon, int precision, int flags, hifde_f
     });
 }
 
+int hifde_set_stream(hifde_factor* f, void* stream) {
+    if (!f) return HIFDE_ERR_ARG;
+    f->caller_stream = static_cast<hipStream_t>(stream);
+    return HIFDE_OK;
+}
+
 void hifde_destroy(hifde_factor* f) {
     if (!f) return;
     hipStream_t s = f->stream;
@@ -2385,7 +2449,7 @@ int hifde_apply_device(hifde_factor* f, double* d_u, int nrhs, long long ldu, in
         if (!f || !d_u || nrhs < 0) throw Error(HIFDE_ERR_ARG, "apply: bad arguments");
         long long n = (long long)f->NC * f->NC * f->NC;
         if (nrhs > 1 && ldu < n) throw Error(HIFDE_ERR_ARG, "apply: ldu too small");
-        f->after_default_stream();
+        f->after_caller_stream();
         f->apply_dev(d_u, nrhs, ldu, inverse != 0);
         HIFDE_CHECK(hipStreamSynchronize(f->stream));
         return HIFDE_OK;
@@ -2433,7 +2497,7 @@ int hifde_pcg_device(hifde_factor* f, const double* d_b, double* d_x, double tol
                      int* iters, double* relres) {
     return guarded([&]() -> int {
         if (!f || !d_b || !d_x || maxit < 0) throw Error(HIFDE_ERR_ARG, "pcg: bad arguments");
-        f->after_default_stream();
+        f->after_caller_stream();
         f->pcg_dev(d_b, d_x, tol, maxit, iters, relres);
         return HIFDE_OK;
     });
@@ -2449,7 +2513,7 @@ int hifde_pcg_multi_device(hifde_factor* f, const double* d_b, long long ldb, do
         if (nrhs > 1 && (ldb < n || ldx < n))
             throw Error(HIFDE_ERR_ARG, "pcg_multi: leading dimension too small");
         if (nrhs == 0) return HIFDE_OK;
-        f->after_default_stream();
+        f->after_caller_stream();
         f->pcg_multi_dev(d_b, ldb, d_x, ldx, nrhs, tol, maxit, iters, relres);
         return HIFDE_OK;
     });
@@ -2586,6 +2650,7 @@ int hifde_set_option(hifde_factor* f, const char* name, double value) {
             f->clear_graphs();
         } else if (n == "graphs") {
             f->graph_option = value != 0;
+            f->graph_failures = 0;
             f->clear_graphs();
         }
         else throw Error(HIFDE_ERR_ARG, "set_option: unknown option " + n);
@@ -2654,7 +2719,8 @@ int hifde_interp_decomp_batch(int nbatch, const int* m, const int* n, const doub
         if (nbatch < 0 || (nbatch && (!m || !n || !M_packed || !k_out || !perm_out || !W_out)))
             throw Error(HIFDE_ERR_ARG, "interp_decomp_batch: bad arguments");
         if (nbatch == 0) return HIFDE_OK;
-        hipStream_t s = nullptr;
+        hipStream_t s = aux_stream();
+        OwnerGuard og(s);
         std::vector<int> hm(m, m + nbatch), hn(n, n + nbatch);
         std::vector<long long> offM = prefix_ll(hm, hn), offW = prefix_ll(hn, hn);
         std::vector<int> poff = prefix_i(hn);
@@ -2678,9 +2744,12 @@ int hifde_interp_decomp_batch(int nbatch, const int* m, const int* n, const doub
         dk.reserve(nbatch);
         batched_interp_decomp(nbatch, dm.p, dn.p, doffM.p, dM.p, eps, dperm.p, dpoff.p, dk.p,
                               dW.p, doffW.p, max_n, max_m, nullptr, s);
-        HIFDE_CHECK(hipMemcpy(k_out, dk.p, nbatch * sizeof(int), hipMemcpyDeviceToHost));
-        HIFDE_CHECK(hipMemcpy(perm_out, dperm.p, poff.back() * sizeof(int), hipMemcpyDeviceToHost));
-        HIFDE_CHECK(hipMemcpy(W_out, dW.p, offW.back() * sizeof(double), hipMemcpyDeviceToHost));
+        HIFDE_CHECK(hipMemcpyAsync(k_out, dk.p, nbatch * sizeof(int), hipMemcpyDeviceToHost, s));
+        HIFDE_CHECK(hipMemcpyAsync(perm_out, dperm.p, poff.back() * sizeof(int),
+                                   hipMemcpyDeviceToHost, s));
+        HIFDE_CHECK(hipMemcpyAsync(W_out, dW.p, offW.back() * sizeof(double),
+                                   hipMemcpyDeviceToHost, s));
+        HIFDE_CHECK(hipStreamSynchronize(s));
         return HIFDE_OK;
     });
 }
@@ -2692,7 +2761,8 @@ template <class T>
 int schur_batch_impl(int nb, const int* r, const int* s, const double* A22, const double* A21,
                      const double* A12, double* A22inv, double* X, double* S, int* info,
                      bool sym, bool split) {
-    hipStream_t st = nullptr;
+    hipStream_t st = aux_stream();
+    OwnerGuard og(st);
     std::vector<int> hr(r, r + nb), hs(s, s + nb);
     std::vector<long long> orr = prefix_ll(hr, hr), ors = prefix_ll(hr, hs), oss = prefix_ll(hs, hs);
     int max_r = 0;
@@ -2715,7 +2785,7 @@ int schur_batch_impl(int nb, const int* r, const int* s, const double* A22, cons
     dX.reserve(ors.back());
     dS.reserve(oss.back());
     dinfo.reserve(nb);
-    HIFDE_CHECK(hipMemset(dinfo.p, 0, nb * sizeof(int)));
+    HIFDE_CHECK(hipMemsetAsync(dinfo.p, 0, nb * sizeof(int), st));
     GJWork gw;
     GemmWork gws;
     batched_gj_inverse<T>(nb, hr.data(), orr.data(), dr.p, drr.p, dA.p, dinfo.p, gw, gws, st,
@@ -2727,13 +2797,16 @@ int schur_batch_impl(int nb, const int* r, const int* s, const double* A22, cons
                     dA12.p, dX.p, dS.p, (T)-1, gws, st);
     auto down = [&](DevBuf<T>& d, long long cnt, double* dst) {
         std::vector<T> tmp(cnt);
-        if (cnt) HIFDE_CHECK(hipMemcpy(tmp.data(), d.p, cnt * sizeof(T), hipMemcpyDeviceToHost));
+        if (cnt)
+            HIFDE_CHECK(hipMemcpyAsync(tmp.data(), d.p, cnt * sizeof(T), hipMemcpyDeviceToHost, st));
+        HIFDE_CHECK(hipStreamSynchronize(st));
         for (long long i = 0; i < cnt; ++i) dst[i] = (double)tmp[i];
     };
     down(dA, orr.back(), A22inv);
     down(dX, ors.back(), X);
     down(dS, oss.back(), S);
-    HIFDE_CHECK(hipMemcpy(info, dinfo.p, nb * sizeof(int), hipMemcpyDeviceToHost));
+    HIFDE_CHECK(hipMemcpyAsync(info, dinfo.p, nb * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIFDE_CHECK(hipStreamSynchronize(st));
     return HIFDE_OK;
 }
 }  // namespace
@@ -2742,7 +2815,8 @@ namespace {
 template <class T>
 int gemm_batch_impl(int transA, int nb, const int* m, const int* n, const int* k, const double* A,
                     const double* B, double* C, bool split, int tile64) {
-    hipStream_t st = nullptr;
+    hipStream_t st = aux_stream();
+    OwnerGuard og(st);
     std::vector<long long> oA(nb + 1, 0), oB(nb + 1, 0), oC(nb + 1, 0);
     for (int b = 0; b < nb; ++b) {
         if (m[b] < 0 || n[b] < 0 || k[b] < 0) throw Error(HIFDE_ERR_ARG, "negative size");
@@ -2765,7 +2839,9 @@ int gemm_batch_impl(int transA, int nb, const int* m, const int* n, const int* k
                     (T)1, gws, st);
     std::vector<T> tmp(oC[nb]);
     if (oC[nb])
-        HIFDE_CHECK(hipMemcpy(tmp.data(), dC.p, oC[nb] * sizeof(T), hipMemcpyDeviceToHost));
+        HIFDE_CHECK(
+            hipMemcpyAsync(tmp.data(), dC.p, oC[nb] * sizeof(T), hipMemcpyDeviceToHost, st));
+    HIFDE_CHECK(hipStreamSynchronize(st));
     for (long long i = 0; i < oC[nb]; ++i) C[i] = (double)tmp[i];
     return HIFDE_OK;
 }

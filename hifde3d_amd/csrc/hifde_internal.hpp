@@ -94,12 +94,14 @@ struct DevBuf {
     void reserve(size_t cnt) {
         if (cnt <= n && p) return;
         // work queued earlier may still read the old buffer: wait for the
-        // owning object's stream (not the whole device, which would also wait
-        // for, and break the graph captures of, other objects' streams)
+        // owning call's stream.  Never the whole device or the legacy null
+        // stream: a device-wide synchronize waits for other objects' streams
+        // and invalidates their graph captures.  Every entry point that grows
+        // a buffer names its stream (OwnerGuard / TimerGuard).
         if (p) {
             const hipStream_t os = owner_stream();
-            if (os) HIFDE_CHECK(hipStreamSynchronize(os));
-            else HIFDE_CHECK(hipDeviceSynchronize());
+            if (!os) throw Error(-4, "internal: device buffer grown outside an owning call");
+            HIFDE_CHECK(hipStreamSynchronize(os));
         }
         release();
         HIFDE_CHECK(hipMalloc(reinterpret_cast<void**>(&p), (cnt ? cnt : 1) * sizeof(T)));

@@ -19,7 +19,14 @@
  *     pointers (the *_device entry points) must be device memory of the
  *     current HIP device; work is ordered on the factor's own stream and the
  *     call returns after it completes.
- *   - One host thread per factor object.
+ *   - One host thread per factor object at a time.  Distinct objects may be
+ *     driven from distinct threads at once: each has its own stream and
+ *     buffers, and no entry point synchronizes the whole device or the
+ *     legacy default stream (unless that is the stream named by
+ *     hifde_set_stream).  A device-wide synchronize issued by the caller
+ *     while another thread's call is capturing a graph is refused by HIP
+ *     (hipErrorStreamCaptureUnsupported) in the caller's thread; the library
+ *     call falls back to eager sweeps.
  */
 #ifndef HIFDE_AMD_H_
 #define HIFDE_AMD_H_
@@ -83,10 +90,21 @@ int hifde_apply(hifde_factor* f, double* u, int inverse);
  * read once per sweep); mixed-precision levels then round the vectors to
  * FP32 for the product, as their factor blocks are FP32.
  * The device entry points (this one and the PCG ones) run on the factor's
- * own stream, after the work already queued on the default stream (without
- * blocking the host), and return when done; buffers written on another
- * stream must be complete before the call. */
+ * own stream, after the work already queued on the caller's stream (see
+ * hifde_set_stream; without blocking the host), and return when done;
+ * buffers written on any other stream must be complete before the call.
+ * Repeated calls with the same (d_u, nrhs, ldu, inverse) replay the sweeps
+ * from a HIP graph (option "graphs"); if HIP refuses or invalidates the
+ * capture (for instance because another host thread synchronizes the whole
+ * device meanwhile), the call runs the same sweeps eagerly, with the same
+ * results, and the object stops capturing. */
 int hifde_apply_device(hifde_factor* f, double* d_u, int nrhs, long long ldu, int inverse);
+/* The stream (a hipStream_t, passed as void*) on which the caller produces
+ * the device inputs of the *_device entry points and consumes their outputs;
+ * NULL (the default) is the legacy default stream.  No reference
+ * counterpart (the reference is host-only).  A multithreaded caller names a
+ * stream of its own per thread, so that no call touches the legacy stream. */
+int hifde_set_stream(hifde_factor* f, void* stream);
 
 /* SpMV(A, x, y) -- src/Factor.cpp:1099-1113.  transpose == 1 computes the
  * reference's y = A^T x; transpose == 0 computes y = A x.  Host vectors. */

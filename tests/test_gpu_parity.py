@@ -357,48 +357,6 @@ def test_released_workspaces_give_the_same_factor(H):
     assert np.array_equal(out[0][1], out[1][1])
 
 
-def test_factor_objects_from_several_threads_match_sequential(H):
-    """Factor objects driven from several host threads at once (each on its
-    own stream, graph captures included) give the same bits as one after
-    another."""
-    import threading
-    import torch
-    probs = [(32, 4, s) for s in range(4)] + [(24, 3, 7)]
-    facs, B = [], []
-    for NC, P, seed in probs:
-        f = H.HIFFactor(NC - 1, P, 1e-4, precision="mixed")
-        f.set_option("host_threads", 2)
-        f.set_stencil(pb.pow10_coeff(NC, seed, -2.0, 2.0), pb.zeros_coeff(NC), True)
-        facs.append(f)
-        B.append(torch.from_numpy(pb.rhs_normal(NC, 50 + seed, 4)).cuda())
-    torch.cuda.synchronize()
-
-    def run(i, out):
-        f = facs[i]
-        assert f.Factor() == 0
-        x = torch.zeros_like(B[i])
-        it, rr = f.pcg_multi_device(B[i].data_ptr(), x.data_ptr(), 4, tol=1e-10, maxit=200)
-        torch.cuda.synchronize()
-        out[i] = (list(it), x.cpu().numpy())
-
-    seq = {}
-    for i in range(len(facs)):
-        run(i, seq)
-    for rep in range(2):
-        par = {}
-        th = [threading.Thread(target=run, args=(i, par)) for i in range(len(facs))]
-        for t in th:
-            t.start()
-        for t in th:
-            t.join()
-        assert sorted(par) == list(range(len(facs))), rep
-        for i in range(len(facs)):
-            assert par[i][0] == seq[i][0], (rep, i)
-            assert np.array_equal(par[i][1], seq[i][1]), (rep, i)
-    for f in facs:
-        f.close()
-
-
 def test_cpp_dropin_driver_reference_protocol(H, oracle_mod):
     """tests/cpp/hif_driver (the reference driver protocol written against
     include/hifde3d_amd.hpp) against the oracle's e_a / e_s."""
