@@ -290,6 +290,7 @@ void interp_decomp(int m, int n, double* M, double eps, std::vector<int>& skel_c
 
 struct Box {
     std::vector<int> g, red, skel;  // red/skel: positions into g
+    std::vector<int> rows;          // faces: rows of the ID matrix (global DOF ids)
     std::vector<double> A22, A22inv, X, W, S;
     int face = -1;
 };
@@ -628,6 +629,7 @@ struct Oracle {
             interp_decomp((int)rows_list[t].size(), (int)b.g.size(), M.data(), eps, b.skel, b.red,
                           b.W);
             schur_after_id(b);
+            b.rows = std::move(rows_list[t]);
         }
     }
 
@@ -871,6 +873,17 @@ int hifo_box_data(void* h, int level, int kind, int box, int* g, int* red, int* 
     if (X) std::copy(b.X.begin(), b.X.end(), X);
     if (W) std::copy(b.W.begin(), b.W.end(), W);
     return 0;
+}
+
+// Rows of a face's ID matrix (SkelInteractionIndexData's global_rows, in its
+// order, duplicates kept).  Returns the row count; out may be null.
+int hifo_face_rows(void* h, int level, int box, int* out) {
+    Oracle* o = static_cast<Oracle*>(h);
+    if (level < 0 || level >= o->L) return -1;
+    const std::vector<Box>& d = o->skel_data[level];
+    if (box < 0 || box >= (int)d.size()) return -1;
+    if (out) std::copy(d[box].rows.begin(), d[box].rows.end(), out);
+    return (int)d[box].rows.size();
 }
 
 // Per-level wall time of the last factor (seconds): out[0..2] = schur, skel, update.

@@ -43,6 +43,7 @@ def lib():
         L.hifo_box_sizes.argtypes = [_vp, _i, _i, _i, _ip]
         L.hifo_box_data.argtypes = [_vp, _i, _i, _i, _ip, _ip, _ip, _dp, _dp, _dp, _dp]
         L.hifo_level_times.argtypes = [_vp, _i, _dp]
+        L.hifo_face_rows.argtypes = [_vp, _i, _i, _ip]
         L.hifo_stencil_coo.restype = _l
         L.hifo_stencil_coo.argtypes = [_i, _d, _dp, _dp, _i, _ip, _ip, _dp]
         L.hifo_qrcp.argtypes = [_i, _i, _dp, _i, _ip, _dp]
@@ -131,6 +132,15 @@ class Oracle:
         if kind == 1:
             out["W"] = W.reshape(s, r, order="F")
         return out
+
+    def face_rows(self, level, b):
+        """Global ids of the rows of face b's ID matrix at `level`, in the
+        order SkelInteractionIndexData lists them (duplicates kept)."""
+        n = lib().hifo_face_rows(self._h, level, b, None)
+        assert n >= 0
+        out = np.zeros(max(n, 1), np.int32)
+        lib().hifo_face_rows(self._h, level, b, _p(out, ctypes.c_int))
+        return out[:n]
 
     def level_counts(self):
         """Per level: (cells, cell DOFs eliminated, faces, face DOFs eliminated)."""

@@ -287,6 +287,132 @@ def test_front_quirk_changes_rows_not_validity(oracle_mod):
 
 
 # --------------------------------------------------------------------------
+# face ID-matrix rows (SkelInteractionIndexData, src/Factor.cpp:645-755),
+# pinned by the geometry of the level-0 partition
+# --------------------------------------------------------------------------
+def _level0_face_rows_from_geometry(N, P, quirk):
+    """Expected (cell, face) order and row multisets of the level-0 faces,
+    built from the geometry of the partition, not from a scan of faces.
+
+    After the level-0 cells are eliminated, the live DOFs are the points with
+    at least one coordinate a positive multiple of P (cell boundaries); a
+    point's kind is its number of interior coordinates (2: face interior,
+    1: edge interior).  The rows of the face f shared by cell c and its
+    lower neighbour c' (TOP: c' = c - e_i, LEFT: c - e_j, FRONT: c - e_k) are:
+      * every face-interior point of the closures of c and c' that is not on
+        f's plane inside c (the ten other faces of the two cells), and
+      * the edge-interior points of f;
+    plus, with the FRONT quirk (src/Factor.cpp:486 tests k == 0, so a cell
+    with ck = 0 reports the interior plane k = 1 as its FRONT face), for each
+    of c and c' with ck = 0 whose FRONT face is listed, the face-interior
+    points of that cell's plane k = 1, which are live points of its other
+    faces and are listed again (or are f's own columns)."""
+    from collections import Counter
+    NC = N + 1
+    c = NC // P
+
+    def rng(ca):
+        return max(P * ca, 1), min(P * (ca + 1), N)
+
+    def kind(p):
+        return sum(1 for x in p if x % P != 0)
+
+    def closure(cell):
+        (i0, i1), (j0, j1), (k0, k1) = (rng(x) for x in cell)
+        return [(i, j, k) for i in range(i0, i1 + 1) for j in range(j0, j1 + 1)
+                for k in range(k0, k1 + 1)]
+
+    def lin(p):
+        return p[0] + NC * p[1] + NC * NC * p[2]
+
+    faces = []
+    for ci in range(c):
+        for cj in range(c):
+            for ck in range(c):
+                cell = (ci, cj, ck)
+                for face, ax in (("TOP", 0), ("FRONT", 2), ("LEFT", 1)):
+                    if cell[ax] == 0:
+                        continue  # the face lies on the domain's min boundary
+                    nbr = tuple(x - (1 if a == ax else 0) for a, x in enumerate(cell))
+                    plane = P * cell[ax]
+                    on_f = [p for p in closure(cell) if p[ax] == plane]
+                    rows = Counter()
+                    for cc in (cell, nbr):
+                        for p in closure(cc):
+                            if kind(p) == 2 and not (cc == cell and p[ax] == plane) \
+                                    and not (cc == nbr and p[ax] == plane):
+                                rows[lin(p)] += 1
+                    for p in on_f:
+                        if kind(p) == 1:
+                            rows[lin(p)] += 1
+                    if quirk:
+                        # FRONT is listed for c unless f is c's FRONT face, and
+                        # for c' unless f is c''s BACK face (FRONT skeletonized)
+                        for cc, listed in ((cell, face != "FRONT"), (nbr, True)):
+                            if listed and cc[2] == 0:
+                                for p in closure(cc):
+                                    if p[2] == 1 and kind(p) == 2:
+                                        rows[lin(p)] += 1
+                    cols = sorted(lin(p) for p in on_f if kind(p) == 2)
+                    faces.append((cell, face, cols, rows))
+    return faces
+
+
+@pytest.mark.parametrize("N,P", [(15, 4), (31, 4), (23, 3), (39, 5)])
+def test_level0_face_rows_match_geometry(oracle_mod, N, P):
+    """C1 (n=16) and n=32 at P=4, and the P=3 / P=5 leaves: every level-0
+    face's ID-matrix rows, as a multiset, and its columns equal the geometric
+    description; the face order is cells (i, j, k) with k fastest, faces
+    TOP, FRONT, LEFT (src/Factor.cpp:232-260)."""
+    from collections import Counter
+    coo, _ = _problem(oracle_mod, N)
+    o = _factor(oracle_mod, N, P, 1e-3, coo)
+    expect = _level0_face_rows_from_geometry(N, P, True)
+    assert o.num_boxes(0, 1) == len(expect)
+    face_id = {"TOP": 0, "LEFT": 3, "FRONT": 4}
+    for b, (cell, face, cols, rows) in enumerate(expect):
+        box = o.box(0, 1, b)
+        assert box["face"] == face_id[face], (b, cell, face)
+        assert sorted(box["g"]) == cols, (b, cell, face)
+        assert Counter(o.face_rows(0, b).tolist()) == rows, (b, cell, face)
+    if P == 4 and N == 15:
+        # closed-form sizes: an interior face of C1 has 12 edge rows and 45
+        # face rows (five 3 x 3 faces) from each of its two cells.  At ck = 0
+        # the face loses its k = 0 edge (9 edge rows), each cell has four real
+        # other faces (36 rows) and the quirk's plane k = 1 adds 12 rows per
+        # cell (2 x 3 points on each of the two i planes and two j planes)
+        sizes = {(cell, face): sum(rows.values()) for cell, face, _, rows in expect}
+        assert sizes[((2, 2, 2), "TOP")] == 12 + 45 + 45
+        assert sizes[((2, 2, 0), "TOP")] == 9 + 36 + 36 + 12 + 12
+
+
+def test_level0_face_rows_pin_detects_dropped_quirk_and_wrong_face(oracle_mod):
+    """Planted mistakes fail the geometric pin: the oracle without the FRONT
+    quirk, and a row list with the neighbour's BOTTOM face in place of its
+    TOP face."""
+    from collections import Counter
+    coo, _ = _problem(oracle_mod, 15)
+    expect = _level0_face_rows_from_geometry(15, 4, True)
+    o = _factor(oracle_mod, 15, 4, 1e-3, coo, quirk=False)
+    diff = sum(Counter(o.face_rows(0, b).tolist()) != rows
+               for b, (_, _, _, rows) in enumerate(expect))
+    assert diff > 0
+    # the quirk-free oracle matches the quirk-free geometry
+    plain = _level0_face_rows_from_geometry(15, 4, False)
+    assert all(Counter(o.face_rows(0, b).tolist()) == rows
+               for b, (_, _, _, rows) in enumerate(plain))
+    # wrong face in the neighbour list: TOP face of cell (1, 1, 1); its
+    # neighbour (0, 1, 1) has no TOP face (domain boundary) but a BOTTOM one
+    oq = _factor(oracle_mod, 15, 4, 1e-3, coo)
+    b = next(b for b, (cell, face, _, _) in enumerate(expect)
+             if cell == (1, 1, 1) and face == "TOP")
+    rows = oq.face_rows(0, b).tolist()
+    cols = set(oq.box(0, 1, b)["g"].tolist())
+    planted = rows + sorted(cols)  # neighbour BOTTOM = the face itself
+    assert Counter(planted) != expect[b][3]
+
+
+# --------------------------------------------------------------------------
 # generators
 # --------------------------------------------------------------------------
 def test_splitmix64_published_first_output():
