@@ -39,6 +39,7 @@ EXPORTS = [
     "hifde_set_profiling", "hifde_set_profile_kernel", "hifde_reset_kernel_stats",
     "hifde_kernel_stats", "hifde_set_option",
     "hifde_gemm_batch", "hifde_factor_save", "hifde_factor_load", "hifde_set_stream",
+    "hifde_apply_device_noalloc",
 ]
 
 
@@ -74,6 +75,7 @@ def lib():
     L.hifde_apply.argtypes = [_vp, _dp, _i]
     L.hifde_apply_device.argtypes = [_vp, _vp, _i, _ll, _i]
     L.hifde_set_stream.argtypes = [_vp, _vp]
+    L.hifde_apply_device_noalloc.argtypes = [_vp, _vp, _i]
     L.hifde_spmv.argtypes = [_vp, _dp, _dp, _i]
     L.hifde_pcg.argtypes = [_vp, _dp, _dp, _d, _i, _ip, _dp]
     L.hifde_pcg_device.argtypes = [_vp, _vp, _vp, _d, _i, _ip, _dp]
@@ -200,6 +202,12 @@ class HIFFactor:
         self._follow_caller_stream()
         _check(lib().hifde_apply_device(self._h, ctypes.c_void_p(ptr), nrhs,
                                         ldu if ldu is not None else self.size, int(inverse)))
+
+    def apply_device_noalloc(self, ptr, inverse=True):
+        """One device vector through the allocation-free apply (grids up to
+        128^3; calls overlapping on one GPU race on its shared scratch)."""
+        self._follow_caller_stream()
+        _check(lib().hifde_apply_device_noalloc(self._h, ctypes.c_void_p(ptr), int(inverse)))
 
     def spmv(self, x, transpose=False):
         x = np.ascontiguousarray(x, np.float64)

@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -61,6 +62,32 @@ hipStream_t aux_stream() {
     if ((int)t.s.size() <= dev) t.s.resize(dev + 1, nullptr);
     if (!t.s[dev]) HIFDE_CHECK(hipStreamCreateWithFlags(&t.s[dev], hipStreamNonBlocking));
     return t.s[dev];
+}
+
+// Scratch of the allocation-free apply (hifde_apply_device_noalloc): the
+// single-vector sweeps' diagonal-stage vector and two pass-flag side
+// vectors, 3 NC^3 doubles, for grids up to NC^3 = 128^3 (48 MiB).  Every
+// GPU the library runs on holds its own copy of this array until the
+// process ends.  In host code the array's name is not a device address:
+// host code uses only the per-device pointer from hipGetSymbolAddress.
+constexpr long long kNoAllocMaxCells = 1LL << 21;  // NC^3 <= 128^3
+static __device__ __align__(256) double scratch[3 * kNoAllocMaxCells];
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+// Resolves the calling thread's current device's copy of the scratch array
+// (once per device, with that device current); returns its address.
+double* resolve_scratch(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) throw Error(HIFDE_ERR_ARG, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        HIFDE_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    return g_scratch_ptr[dev];
 }
 
 const void* Stager::stage(const void* src, size_t bytes) {
@@ -231,6 +258,9 @@ struct hifde_factor {
     // multithreaded caller needs while other threads capture graphs.
     hipEvent_t entry_ev = nullptr;
     hipStream_t caller_stream = nullptr;
+    int device = 0;                  // the HIP device current at creation
+    double* scratch_dev = nullptr;   // that device's copy of the no-allocation scratch
+    bool no_alloc_call = false;      // inside hifde_apply_device_noalloc: no event timing
     void after_caller_stream() {
         if (!entry_ev) HIFDE_CHECK(hipEventCreateWithFlags(&entry_ev, hipEventDisableTiming));
         HIFDE_CHECK(hipEventRecord(entry_ev, caller_stream));
@@ -281,7 +311,7 @@ struct hifde_factor {
         double flops, bytes;
         Prof(hifde_factor* f_, const std::string& name, double fl, double by)
             : f(f_), flops(fl), bytes(by) {
-            if (!f->profiling || !f->profile_only.empty()) return;
+            if (!f->profiling || !f->profile_only.empty() || f->no_alloc_call) return;
             fam = f->family(name);
             e0 = f->take_event();
             HIFDE_CHECK(hipEventRecord(e0, f->stream));
@@ -1420,12 +1450,13 @@ struct hifde_factor {
     // op 0 of a cell batch, whose skeleton DOFs on shared faces get two
     // contributions: every output is formed once into the side buffers, then
     // the contributions are added in pass order (max faces, then min faces).
-    void upd_sides(BoxBatch& B, const DevBuf<unsigned char>& Mbuf, double alpha, double* u) {
+    // side: two NC^3 side buffers (one per pass flag)
+    void upd_sides(BoxBatch& B, const DevBuf<unsigned char>& Mbuf, double alpha, double* u,
+                   double* side) {
         if (B.nbox == 0) return;
         const long long n3 = (long long)NC * NC * NC;
-        apply_side.reserve(2 * n3);
-        upd(B, Mbuf, 0, -1, alpha, u, apply_side.p);
-        side_add(B.skel_off.back(), B.d_skel_gid.p, B.d_skel_pass.p, apply_side.p, n3, u,
+        upd(B, Mbuf, 0, -1, alpha, u, side);
+        side_add(B.skel_off.back(), B.d_skel_gid.p, B.d_skel_pass.p, side, n3, u,
                  stream);
     }
     void diag(BoxBatch& B, const DevBuf<unsigned char>& Mbuf, const double* u, double* out) {
@@ -1494,14 +1525,17 @@ struct hifde_factor {
     }
 
     // Apply (src/Factor.cpp:992-1081), one vector.  Each batch's blocks are
-    // read in the precision they were factored in.
-    void apply_one(double* u, bool inv) {
+    // read in the precision they were factored in.  Scratch: tmp holds NC^3
+    // doubles (the diagonal stage's output), side 2 NC^3 (the cell sweeps'
+    // contributions by pass flag); both are written before they are read, so
+    // they need no clearing.
+    void apply_one(double* u, bool inv, double* tmp, double* side) {
         zero_boundary(u, NC, stream);
         const double sg = inv ? -1.0 : 1.0;
         for (int lv = 0; lv < L - 1; ++lv) {
             BoxBatch& C = cells[lv];
             if (inv) {
-                upd_sides(C, C.X, -1.0, u);
+                upd_sides(C, C.X, -1.0, u, side);
             } else {
                 upd(C, C.X, 1, -1, 1.0, u);
             }
@@ -1517,7 +1551,6 @@ struct hifde_factor {
         }
         // Every interior DOF is redundant in exactly one box, so the diagonal
         // stage maps u to a fresh vector box by box and the result is copied back.
-        double* tmp = apply_tmp.p;
         vcopy(u, tmp, (long long)NC * NC * NC, stream);
         for (int lv = 0; lv < L; ++lv) {
             diag(cells[lv], inv ? cells[lv].A22inv : cells[lv].A22, u, tmp);
@@ -1539,7 +1572,7 @@ struct hifde_factor {
             if (inv) {
                 upd(C, C.X, 1, -1, -1.0, u);
             } else {
-                upd_sides(C, C.X, 1.0, u);
+                upd_sides(C, C.X, 1.0, u, side);
             }
         }
         (void)sg;
@@ -1586,8 +1619,10 @@ struct hifde_factor {
         if (R > 1) {
             apply_multi(d_u, R, ld, inv);
         } else {
-            apply_tmp.reserve((size_t)NC * NC * NC);
-            apply_one(d_u, inv);
+            const size_t n3 = (size_t)NC * NC * NC;
+            apply_tmp.reserve(n3);
+            apply_side.reserve(2 * n3);
+            apply_one(d_u, inv, apply_tmp.p, apply_side.p);
         }
     }
     void apply_sweeps(double* d_u, int R, long long ld, bool inv) {
@@ -2396,7 +2431,10 @@ int hifde_create(int N, int P, double epsilon, int precision, int flags, hifde_f
         // 64 x 64 matrix-core tiles where the 128 x 128 ones would waste
         // over a quarter of their output (option "gemm_tile64")
         f->gemm_ws.plan.tile64 = f->gj_ws.plan.tile64 = 1;
+        HIFDE_CHECK(hipGetDevice(&f->device));
         HIFDE_CHECK(hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking));
+        HIFDE_CHECK(hipEventCreateWithFlags(&f->entry_ev, hipEventDisableTiming));
+        f->scratch_dev = resolve_scratch(f->device);
         *out = f.release();
         return HIFDE_OK;
     });
@@ -2452,6 +2490,36 @@ int hifde_apply_device(hifde_factor* f, double* d_u, int nrhs, long long ldu, in
         f->after_caller_stream();
         f->apply_dev(d_u, nrhs, ldu, inverse != 0);
         HIFDE_CHECK(hipStreamSynchronize(f->stream));
+        return HIFDE_OK;
+    });
+}
+
+int hifde_apply_device_noalloc(hifde_factor* f, double* d_u, int inverse) {
+    return guarded([&]() -> int {
+        if (!f || !d_u) throw Error(HIFDE_ERR_ARG, "apply_noalloc: bad arguments");
+        const long long n = (long long)f->NC * f->NC * f->NC;
+        if (n > kNoAllocMaxCells)
+            throw Error(HIFDE_ERR_SIZE, "apply_noalloc: (N+1)^3 = " + std::to_string(n) +
+                                            " exceeds the scratch limit 128^3");
+        if (!f->factored) throw Error(HIFDE_ERR_STATE, "apply_noalloc: factor first");
+        if (!f->scratch_dev) throw Error(HIFDE_ERR_STATE, "apply_noalloc: no scratch");
+        OwnerGuard og(f->stream);
+        struct NoAlloc {
+            hifde_factor* f;
+            explicit NoAlloc(hifde_factor* x) : f(x) { f->no_alloc_call = true; }
+            ~NoAlloc() { f->no_alloc_call = false; }
+        } na(f);
+        KernelTimer* prev = active_timer();
+        set_active_timer(nullptr);
+        try {
+            f->after_caller_stream();
+            f->apply_one(d_u, inverse != 0, f->scratch_dev, f->scratch_dev + n);
+            HIFDE_CHECK(hipStreamSynchronize(f->stream));
+        } catch (...) {
+            set_active_timer(prev);
+            throw;
+        }
+        set_active_timer(prev);
         return HIFDE_OK;
     });
 }

@@ -99,6 +99,17 @@ int hifde_apply(hifde_factor* f, double* u, int inverse);
  * device meanwhile), the call runs the same sweeps eagerly, with the same
  * results, and the object stops capturing. */
 int hifde_apply_device(hifde_factor* f, double* d_u, int nrhs, long long ldu, int inverse);
+/* hifde_apply_device for one vector (nrhs = 1) that allocates nothing: no
+ * hipMalloc, hipFree, event, graph or other allocating call.  No counterpart
+ * in the reference; it is for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid allocation).
+ * Its scratch (3 (N+1)^3 doubles) is a fixed array in the library, one copy
+ * per GPU, resolved when a factor object is created; (N+1)^3 above 128^3
+ * returns HIFDE_ERR_SIZE.  Same kernels, same order and same results as
+ * hifde_apply_device with nrhs = 1, run eagerly (never from a graph) and
+ * without kernel timing.  Every call on one GPU shares that GPU's scratch:
+ * calls that overlap on one GPU (from several threads or objects) race. */
+int hifde_apply_device_noalloc(hifde_factor* f, double* d_u, int inverse);
 /* The stream (a hipStream_t, passed as void*) on which the caller produces
  * the device inputs of the *_device entry points and consumes their outputs;
  * NULL (the default) is the legacy default stream.  No reference
