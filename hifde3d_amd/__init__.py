@@ -39,7 +39,7 @@ EXPORTS = [
     "hifde_set_profiling", "hifde_set_profile_kernel", "hifde_reset_kernel_stats",
     "hifde_kernel_stats", "hifde_set_option",
     "hifde_gemm_batch", "hifde_factor_save", "hifde_factor_load", "hifde_set_stream",
-    "hifde_apply_device_noalloc",
+    "hifde_apply_device_noalloc", "hifde_pivoted_qr_batch", "hifde_triangular_solve_batch",
 ]
 
 
@@ -99,6 +99,8 @@ def lib():
     L.hifde_reset_kernel_stats.argtypes = [_vp]
     L.hifde_kernel_stats.argtypes = [_vp, _i, ctypes.c_char_p, _i, _dp]
     L.hifde_interp_decomp_batch.argtypes = [_i, _ip, _ip, _dp, _d, _ip, _ip, _dp]
+    L.hifde_pivoted_qr_batch.argtypes = [_i, _ip, _ip, _dp, _ip, _dp]
+    L.hifde_triangular_solve_batch.argtypes = [_i, _ip, _ip, _dp, _dp]
     L.hifde_gemm_batch.argtypes = [_i, _i, _i, _ip, _ip, _ip, _dp, _dp, _dp]
     L.hifde_schur_batch.argtypes = [_i, _i, _ip, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _ip]
     _lib = L
@@ -348,6 +350,49 @@ def interp_decomp_batch(mats, eps):
         out.append((p[:kb].copy(), p[kb:].copy(), Wb.copy()))
         po += nb
         wo += nb * nb
+    return out
+
+
+def pivoted_qr_batch(mats):
+    """PivotedQRWrapper on a list of 2-D arrays.  Returns [(QR, jpvt, tau)]:
+    QR holds R on and above the diagonal and the Householder vectors below
+    it (dgeqp3's layout), jpvt the 0-based column pivots."""
+    m = np.array([a.shape[0] for a in mats], np.int32)
+    n = np.array([a.shape[1] for a in mats], np.int32)
+    packed = np.concatenate([np.asfortranarray(a, np.float64).ravel(order="F") for a in mats]
+                            + [np.zeros(1)])
+    jpvt = np.zeros(max(int(n.sum()), 1), np.int32)
+    mn = np.minimum(m, n)
+    tau = np.zeros(max(int(mn.sum()), 1))
+    _check(lib().hifde_pivoted_qr_batch(len(mats), _ptr(m, ctypes.c_int), _ptr(n, ctypes.c_int),
+                                        _ptr(packed), _ptr(jpvt, ctypes.c_int), _ptr(tau)))
+    out, o, po, to = [], 0, 0, 0
+    for b in range(len(mats)):
+        mb, nb = int(m[b]), int(n[b])
+        out.append((packed[o:o + mb * nb].reshape(mb, nb, order="F").copy(),
+                    jpvt[po:po + nb].copy(), tau[to:to + int(mn[b])].copy()))
+        o += mb * nb
+        po += nb
+        to += int(mn[b])
+    return out
+
+
+def triangular_solve_batch(pairs):
+    """TriangularSolveWrapper on a list of (R, B): returns [R^{-1} B] with R
+    upper triangular (only its upper triangle is read)."""
+    m = np.array([r.shape[0] for r, _ in pairs], np.int32)
+    n = np.array([b.shape[1] for _, b in pairs], np.int32)
+    R = np.concatenate([np.asfortranarray(r, np.float64).ravel(order="F") for r, _ in pairs]
+                       + [np.zeros(1)])
+    B = np.concatenate([np.asfortranarray(b, np.float64).ravel(order="F") for _, b in pairs]
+                       + [np.zeros(1)])
+    _check(lib().hifde_triangular_solve_batch(len(pairs), _ptr(m, ctypes.c_int),
+                                              _ptr(n, ctypes.c_int), _ptr(R), _ptr(B)))
+    out, o = [], 0
+    for b in range(len(pairs)):
+        cnt = int(m[b]) * int(n[b])
+        out.append(B[o:o + cnt].reshape(int(m[b]), int(n[b]), order="F").copy())
+        o += cnt
     return out
 
 

@@ -1434,7 +1434,8 @@ __global__ void __launch_bounds__(256) k_interp_decomp(const int* __restrict__ d
                                                        const int* __restrict__ perm_off,
                                                        int* kout, double* Wb,
                                                        const long long* __restrict__ offW,
-                                                       const unsigned char* __restrict__ mode) {
+                                                       const unsigned char* __restrict__ mode,
+                                                       int* jpvt_out, double* tau_out) {
     const int b = blockIdx.x;
     if (mode && mode[b]) return;
     const int m = dm[b], n = dn[b];
@@ -1519,6 +1520,7 @@ __global__ void __launch_bounds__(256) k_interp_decomp(const int* __restrict__ d
         }
         __syncthreads();
         const double tau = s_tau;
+        if (tau_out && tid == 0) tau_out[perm_off[b] + i] = tau;
         if (tau != 0.0) {
             const double scal = s_pv;
             for (int r = i + 1 + tid; r < m; r += bs) col[r] *= scal;
@@ -1587,6 +1589,8 @@ __global__ void __launch_bounds__(256) k_interp_decomp(const int* __restrict__ d
     const int k = s_k, q = n - k;
     int* pb = perm + perm_off[b];
     for (int t = tid; t < n; t += bs) pb[t] = jp[pos[t]];
+    if (jpvt_out)
+        for (int t = tid; t < n; t += bs) jpvt_out[perm_off[b] + t] = jp[t];
     // W = R11^{-1} Wfull(:, red), Wfull(i, j) = R(skel[i], j) for j >= i else 0
     double* W = Wb + offW[b];
     for (int jj = tid; jj < q; jj += bs) {
@@ -2312,7 +2316,8 @@ void gram_w(const std::vector<GramRDesc>& desc, const std::vector<long long>& to
 void batched_interp_decomp(int nface, const int* d_m, const int* d_n, const long long* d_offM,
                            double* d_M, double eps, int* d_perm, const int* d_perm_off, int* d_k,
                            double* d_W, const long long* d_offW, int max_n, int max_m,
-                           const unsigned char* d_mode, hipStream_t s) {
+                           const unsigned char* d_mode, hipStream_t s, int* d_jpvt,
+                           double* d_tau) {
     if (nface <= 0) return;
     size_t shm = (size_t)max_n * (2 * sizeof(double) + 2 * sizeof(int));
     if (shm > 150 * 1024) throw Error(-3, "interp_decomp: face too large for LDS");
@@ -2322,10 +2327,47 @@ void batched_interp_decomp(int nface, const int* d_m, const int* d_n, const long
     const size_t shm_m = (size_t)std::max(max_m, 1) * max_n * sizeof(double);
     if (shm + shm_m <= 32 * 1024)
         k_interp_decomp<true><<<nface, threads, shm + shm_m, s>>>(
-            d_m, d_n, d_offM, d_M, eps, d_perm, d_perm_off, d_k, d_W, d_offW, d_mode);
+            d_m, d_n, d_offM, d_M, eps, d_perm, d_perm_off, d_k, d_W, d_offW, d_mode, d_jpvt,
+            d_tau);
     else
         k_interp_decomp<false><<<nface, threads, shm, s>>>(d_m, d_n, d_offM, d_M, eps, d_perm,
-                                                          d_perm_off, d_k, d_W, d_offW, d_mode);
+                                                          d_perm_off, d_k, d_W, d_offW, d_mode,
+                                                          d_jpvt, d_tau);
+    HIFDE_LAUNCH_CHECK();
+}
+
+// B := R^{-1} B for upper-triangular R, one box per workgroup, one column of
+// B per thread (TriangularSolveWrapper, src/InterpDecomp.cpp:213-231: dtrsm
+// side L, upper, no transpose, non-unit diagonal).  Back substitution in the
+// reference BLAS order: an entry that is exactly zero is skipped, a zero
+// diagonal divides by zero as dtrsm does.
+__global__ void __launch_bounds__(256) k_trsm_upper(const int* __restrict__ dm,
+                                                    const int* __restrict__ dn,
+                                                    const long long* __restrict__ offR,
+                                                    const double* __restrict__ Rb,
+                                                    const long long* __restrict__ offB,
+                                                    double* Bb) {
+    const int b = blockIdx.y;
+    const int m = dm[b], n = dn[b];
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n || m <= 0) return;
+    const double* R = Rb + offR[b];
+    double* y = Bb + offB[b] + (size_t)j * m;
+    for (int i = m - 1; i >= 0; --i) {
+        if (y[i] != 0.0) {
+            y[i] /= R[i + (size_t)i * m];
+            const double t = y[i];
+            for (int r = 0; r < i; ++r) y[r] -= t * R[r + (size_t)i * m];
+        }
+    }
+}
+
+void batched_trsm_upper(int nbox, const int* d_m, const int* d_n, const long long* d_offR,
+                        const double* d_R, const long long* d_offB, double* d_B, int max_n,
+                        hipStream_t s) {
+    if (nbox <= 0 || max_n <= 0) return;
+    dim3 grid((max_n + 255) / 256, nbox);
+    k_trsm_upper<<<grid, 256, 0, s>>>(d_m, d_n, d_offR, d_R, d_offB, d_B);
     HIFDE_LAUNCH_CHECK();
 }
 

@@ -2392,6 +2392,63 @@ int guarded(F&& fn) {
     }
 }
 
+// Householder column-pivoted QR of a batch of host matrices on the GPU (the
+// kernel of the factorization's ID).  M_packed is overwritten with the QR
+// factors.  With k_out/perm_out/W_out the ID (rank rule with eps, skeleton
+// then redundant columns, W) is returned; with jpvt_out/tau_out the pivots
+// and reflector scalars.
+void qr_batch(int nbatch, const int* m, const int* n, double* M_packed, double eps, int* k_out,
+              int* perm_out, double* W_out, int* jpvt_out, double* tau_out) {
+    hipStream_t s = aux_stream();
+    OwnerGuard og(s);
+    std::vector<int> hm(m, m + nbatch), hn(n, n + nbatch);
+    std::vector<long long> offM = prefix_ll(hm, hn), offW = prefix_ll(hn, hn);
+    std::vector<int> poff = prefix_i(hn);
+    int max_n = 0, max_m = 0;
+    for (int b = 0; b < nbatch; ++b) {
+        if (hm[b] < 0 || hn[b] < 0) throw Error(HIFDE_ERR_ARG, "negative size");
+        max_n = std::max(max_n, hn[b]);
+        max_m = std::max(max_m, hm[b]);
+    }
+    DevBuf<int> dm, dn, dperm, dpoff, dk, djp;
+    DevBuf<long long> doffM, doffW;
+    DevBuf<double> dM, dW, dtau;
+    dm.upload(hm, s);
+    dn.upload(hn, s);
+    doffM.upload(offM, s);
+    doffW.upload(offW, s);
+    dpoff.upload(poff, s);
+    dM.upload(M_packed, offM.back(), s);
+    dW.reserve(offW.back());
+    dperm.reserve(poff.back());
+    dk.reserve(nbatch);
+    if (jpvt_out) djp.reserve(poff.back());
+    if (tau_out) dtau.reserve(poff.back());
+    batched_interp_decomp(nbatch, dm.p, dn.p, doffM.p, dM.p, eps, dperm.p, dpoff.p, dk.p, dW.p,
+                          doffW.p, max_n, max_m, nullptr, s, jpvt_out ? djp.p : nullptr,
+                          tau_out ? dtau.p : nullptr);
+    auto down = [&](void* dst, const void* src, size_t bytes) {
+        if (bytes) HIFDE_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+    };
+    down(M_packed, dM.p, offM.back() * sizeof(double));
+    if (k_out) down(k_out, dk.p, nbatch * sizeof(int));
+    if (perm_out) down(perm_out, dperm.p, poff.back() * sizeof(int));
+    if (W_out) down(W_out, dW.p, offW.back() * sizeof(double));
+    if (jpvt_out) down(jpvt_out, djp.p, poff.back() * sizeof(int));
+    std::vector<double> tau_all(tau_out ? poff.back() : 0);
+    if (tau_out) down(tau_all.data(), dtau.p, poff.back() * sizeof(double));
+    HIFDE_CHECK(hipStreamSynchronize(s));
+    if (tau_out) {
+        // min(m, n) reflector scalars per matrix, packed one after another
+        long long o = 0;
+        for (int b = 0; b < nbatch; ++b) {
+            const int mn = std::min(hm[b], hn[b]);
+            std::copy(tau_all.begin() + poff[b], tau_all.begin() + poff[b] + mn, tau_out + o);
+            o += mn;
+        }
+    }
+}
+
 template <class T>
 void read_dev(const DevBuf<unsigned char>& buf, long long off, long long cnt, double* out,
               hipStream_t s) {
@@ -2781,42 +2838,56 @@ long long hifde_factor_bytes(const hifde_factor* f) {
     return tot;
 }
 
-int hifde_interp_decomp_batch(int nbatch, const int* m, const int* n, const double* M_packed,
+int hifde_interp_decomp_batch(int nbatch, const int* m, const int* n, double* M_packed,
                               double eps, int* k_out, int* perm_out, double* W_out) {
     return guarded([&]() -> int {
         if (nbatch < 0 || (nbatch && (!m || !n || !M_packed || !k_out || !perm_out || !W_out)))
             throw Error(HIFDE_ERR_ARG, "interp_decomp_batch: bad arguments");
         if (nbatch == 0) return HIFDE_OK;
+        qr_batch(nbatch, m, n, M_packed, eps, k_out, perm_out, W_out, nullptr, nullptr);
+        return HIFDE_OK;
+    });
+}
+
+int hifde_pivoted_qr_batch(int nbatch, const int* m, const int* n, double* A_packed,
+                           int* jpvt_out, double* tau_out) {
+    return guarded([&]() -> int {
+        if (nbatch < 0 || (nbatch && (!m || !n || !A_packed || !jpvt_out)))
+            throw Error(HIFDE_ERR_ARG, "pivoted_qr_batch: bad arguments");
+        if (nbatch == 0) return HIFDE_OK;
+        qr_batch(nbatch, m, n, A_packed, 0.0, nullptr, nullptr, nullptr, jpvt_out, tau_out);
+        return HIFDE_OK;
+    });
+}
+
+int hifde_triangular_solve_batch(int nbatch, const int* m, const int* n, const double* R_packed,
+                                 double* B_packed) {
+    return guarded([&]() -> int {
+        if (nbatch < 0 || (nbatch && (!m || !n || !R_packed || !B_packed)))
+            throw Error(HIFDE_ERR_ARG, "triangular_solve_batch: bad arguments");
+        if (nbatch == 0) return HIFDE_OK;
         hipStream_t s = aux_stream();
         OwnerGuard og(s);
         std::vector<int> hm(m, m + nbatch), hn(n, n + nbatch);
-        std::vector<long long> offM = prefix_ll(hm, hn), offW = prefix_ll(hn, hn);
-        std::vector<int> poff = prefix_i(hn);
-        int max_n = 0, max_m = 0;
+        int max_n = 0;
         for (int b = 0; b < nbatch; ++b) {
             if (hm[b] < 0 || hn[b] < 0) throw Error(HIFDE_ERR_ARG, "negative size");
             max_n = std::max(max_n, hn[b]);
-            max_m = std::max(max_m, hm[b]);
         }
-        DevBuf<int> dm, dn, dperm, dpoff, dk;
-        DevBuf<long long> doffM, doffW;
-        DevBuf<double> dM, dW;
+        std::vector<long long> offR = prefix_ll(hm, hm), offB = prefix_ll(hm, hn);
+        DevBuf<int> dm, dn;
+        DevBuf<long long> doffR, doffB;
+        DevBuf<double> dR, dB;
         dm.upload(hm, s);
         dn.upload(hn, s);
-        doffM.upload(offM, s);
-        doffW.upload(offW, s);
-        dpoff.upload(poff, s);
-        dM.upload(M_packed, offM.back(), s);
-        dW.reserve(offW.back());
-        dperm.reserve(poff.back());
-        dk.reserve(nbatch);
-        batched_interp_decomp(nbatch, dm.p, dn.p, doffM.p, dM.p, eps, dperm.p, dpoff.p, dk.p,
-                              dW.p, doffW.p, max_n, max_m, nullptr, s);
-        HIFDE_CHECK(hipMemcpyAsync(k_out, dk.p, nbatch * sizeof(int), hipMemcpyDeviceToHost, s));
-        HIFDE_CHECK(hipMemcpyAsync(perm_out, dperm.p, poff.back() * sizeof(int),
-                                   hipMemcpyDeviceToHost, s));
-        HIFDE_CHECK(hipMemcpyAsync(W_out, dW.p, offW.back() * sizeof(double),
-                                   hipMemcpyDeviceToHost, s));
+        doffR.upload(offR, s);
+        doffB.upload(offB, s);
+        dR.upload(R_packed, offR.back(), s);
+        dB.upload(B_packed, offB.back(), s);
+        batched_trsm_upper(nbatch, dm.p, dn.p, doffR.p, dR.p, doffB.p, dB.p, max_n, s);
+        if (offB.back())
+            HIFDE_CHECK(hipMemcpyAsync(B_packed, dB.p, offB.back() * sizeof(double),
+                                       hipMemcpyDeviceToHost, s));
         HIFDE_CHECK(hipStreamSynchronize(s));
         return HIFDE_OK;
     });

@@ -362,10 +362,20 @@ void permute_sym(int nbox, const long long* d_off, const int* d_n, const int* d_
                  const int* d_perm_off, const double* d_C, double* d_Cp, long long total,
                  hipStream_t s);
 // mode[f] != 0 selects the Gram path for face f; the Householder kernel skips it.
+// d_M is overwritten with the QR factors (R on and above the diagonal, the
+// Householder vectors below it, dgeqp3's layout).  d_jpvt / d_tau, when not
+// null, receive each face's column pivots (0-based, at d_perm_off[f]) and
+// reflector scalars (min(m, n) of them, at d_perm_off[f]).
 void batched_interp_decomp(int nface, const int* d_m, const int* d_n, const long long* d_offM,
                            double* d_M, double eps, int* d_perm, const int* d_perm_off,
                            int* d_k, double* d_W, const long long* d_offW, int max_n,
-                           int max_m, const unsigned char* d_mode, hipStream_t s);
+                           int max_m, const unsigned char* d_mode, hipStream_t s,
+                           int* d_jpvt = nullptr, double* d_tau = nullptr);
+// B_b := R_b^{-1} B_b, R_b upper triangular m_b x m_b, B_b m_b x n_b, both
+// packed column-major at their offsets.
+void batched_trsm_upper(int nbox, const int* d_m, const int* d_n, const long long* d_offR,
+                        const double* d_R, const long long* d_offB, double* d_B, int max_n,
+                        hipStream_t s);
 // Gram-path ID (faces with mode != 0): blocked pivoted Cholesky of the
 // lower triangle of G (overwritten), rows of R, pivot order and rank; then,
 // once the ranks are known on the host, W = R11^{-1} R12.

@@ -12,8 +12,9 @@
 // (include/hifde3d/core/{dense,vector,sparse}.hpp, the members the path
 // uses), NumTns (include/numtns.hpp), Index3 (include/vec3t.hpp),
 // SetupStencil (include/setup_stencil.hpp), SpMV / RelativeErrorNorm2
-// (include/Factor.hpp:205-209), InterpDecomp (include/InterpDecomp.hpp:18-21)
-// and Schur (include/Schur.hpp:72-100).
+// (include/Factor.hpp:205-209), InterpDecomp, PivotedQRWrapper and
+// TriangularSolveWrapper (include/InterpDecomp.hpp:18-30), DenseSubmatrix
+// (include/Schur.hpp:13-61) and Schur (include/Schur.hpp:72-100).
 // Scalar = double runs the FP64 path; Scalar = float runs the mixed path
 // (FP32 blocks on the lower levels, FP64 pivots and top levels).  Complex
 // scalars are not supported.
@@ -22,6 +23,7 @@
 #ifndef HIFDE3D_AMD_HPP_
 #define HIFDE3D_AMD_HPP_
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstdint>
@@ -442,21 +444,94 @@ void SpMV(Sparse<Scalar>& A, Vector<Scalar>& x, Vector<Scalar>& y) {
     }
 }
 
+// DenseSubmatrix (include/Schur.hpp:13-61): submatrix = matrix(rows, cols),
+// matrix(:, cols), and the sparse gather sp_matrix(rows, cols) with absent
+// entries read as 0 (Sparse::Find).  Host containers, so host gathers.
+template <class Scalar>
+void DenseSubmatrix(const Dense<Scalar>& matrix, const std::vector<int>& rows,
+                    const std::vector<int>& cols, Dense<Scalar>& submatrix) {
+    submatrix.Resize((int)rows.size(), (int)cols.size());
+    for (size_t j = 0; j < cols.size(); ++j)
+        for (size_t i = 0; i < rows.size(); ++i)
+            submatrix.Set((int)i, (int)j, matrix.Get(rows[i], cols[j]));
+}
+template <class Scalar>
+void DenseSubmatrix(const Dense<Scalar>& matrix, const std::vector<int>& cols,
+                    Dense<Scalar>& submatrix) {
+    submatrix.Resize(matrix.Height(), (int)cols.size());
+    for (size_t j = 0; j < cols.size(); ++j)
+        for (int i = 0; i < matrix.Height(); ++i)
+            submatrix.Set(i, (int)j, matrix.Get(i, cols[j]));
+}
+template <class Scalar>
+void DenseSubmatrix(Sparse<Scalar>& matrix, std::vector<int>& rows, std::vector<int>& cols,
+                    Dense<Scalar>& submatrix) {
+    Vector<int> iidx((int)rows.size()), jidx((int)cols.size());
+    for (size_t i = 0; i < rows.size(); ++i) iidx.Set((int)i, rows[i]);
+    for (size_t j = 0; j < cols.size(); ++j) jidx.Set((int)j, cols[j]);
+    submatrix.Resize(iidx.Size(), jidx.Size());
+    matrix.Find(iidx, jidx, submatrix);
+}
+
+namespace detail {
+template <class Scalar>
+std::vector<double> pack(const Dense<Scalar>& D) {
+    const int m = D.Height(), n = D.Width();
+    std::vector<double> p((size_t)m * n + 1);
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < m; ++i) p[i + (size_t)j * m] = (double)D.Get(i, j);
+    return p;
+}
+template <class Scalar>
+void unpack(const std::vector<double>& p, Dense<Scalar>& D) {
+    const int m = D.Height(), n = D.Width();
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < m; ++i) D.Set(i, j, (Scalar)p[i + (size_t)j * m]);
+}
+}  // namespace detail
+
+// PivotedQRWrapper(A, jpvt) (src/InterpDecomp.cpp:156-166,198-211): column-
+// pivoted Householder QR on the GPU (FP64).  A is overwritten with R on and
+// above the diagonal and the Householder vectors below it (dgeqp3's layout);
+// jpvt is cleared and filled with the 0-based pivots.
+template <class Scalar>
+void PivotedQRWrapper(Dense<Scalar>& A, std::vector<int>& jpvt) {
+    int m = A.Height(), n = A.Width();
+    std::vector<double> Ap = detail::pack(A);
+    jpvt.assign(n, 0);
+    std::vector<int> jp(n + 1);
+    detail::check(hifde_pivoted_qr_batch(1, &m, &n, Ap.data(), jp.data(), nullptr));
+    std::copy(jp.begin(), jp.begin() + n, jpvt.begin());
+    detail::unpack(Ap, A);
+}
+
+// TriangularSolveWrapper(R, B) (src/InterpDecomp.cpp:213-231): B := R^{-1} B
+// with R upper triangular (B.Height() x B.Height(), its upper triangle read),
+// on the GPU (FP64).
+template <class Scalar>
+void TriangularSolveWrapper(Dense<Scalar>& R, Dense<Scalar>& B) {
+    int m = B.Height(), n = B.Width();
+    std::vector<double> Rp((size_t)m * m + 1), Bp = detail::pack(B);
+    for (int j = 0; j < m; ++j)
+        for (int i = 0; i <= j; ++i) Rp[i + (size_t)j * m] = (double)R.Get(i, j);
+    detail::check(hifde_triangular_solve_batch(1, &m, &n, Rp.data(), Bp.data()));
+    detail::unpack(Bp, B);
+}
+
 // InterpDecomp(M, W, skel, red, eps) (src/InterpDecomp.cpp:86-138), on the
-// GPU as a batch of one.  skel/red are appended to, W is resized; M is left
-// unchanged (the reference overwrites it with the QR factors).
+// GPU as a batch of one.  skel/red are appended to, W is resized, and M is
+// overwritten with its QR factors, as in the reference.
 template <class Scalar>
 void InterpDecomp(Dense<Scalar>& M, Dense<Scalar>& W, std::vector<int>& skeleton_cols,
                   std::vector<int>& redundant_cols, double epsilon) {
     int m = M.Height(), n = M.Width();
-    std::vector<double> Mp((size_t)m * n + 1);
-    for (int j = 0; j < n; ++j)
-        for (int i = 0; i < m; ++i) Mp[i + (size_t)j * m] = (double)M.Get(i, j);
+    std::vector<double> Mp = detail::pack(M);
     int k = 0;
     std::vector<int> perm(n + 1);
     std::vector<double> Wp((size_t)n * n + 1);
     detail::check(hifde_interp_decomp_batch(1, &m, &n, Mp.data(), epsilon, &k, perm.data(),
                                             Wp.data()));
+    detail::unpack(Mp, M);
     for (int t = 0; t < k; ++t) skeleton_cols.push_back(perm[t]);
     for (int t = k; t < n; ++t) redundant_cols.push_back(perm[t]);
     W.Resize(k, n - k);

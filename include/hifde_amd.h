@@ -233,12 +233,33 @@ int hifde_kernel_stats(const hifde_factor* f, int idx, char* name, int name_len,
 long long hifde_factor_bytes(const hifde_factor* f);
 
 /* InterpDecomp(M, W, skel, red, eps) -- src/InterpDecomp.cpp:86-138, over a
- * batch.  M_packed: the m[b] x n[b] matrices packed one after another.
- * Outputs per face: k_out[b]; perm_out (sum n[b] ints): the k skeleton
- * columns then the n-k redundant columns, in the reference's order;
- * W_out (sum n[b]^2 doubles, face b at offset sum_{c<b} n[c]^2): W, k x (n-k). */
-int hifde_interp_decomp_batch(int nbatch, const int* m, const int* n, const double* M_packed,
+ * batch.  M_packed: the m[b] x n[b] matrices packed one after another; as in
+ * the reference (include/InterpDecomp.hpp:12, "M gets overwritten") each is
+ * overwritten with its column-pivoted QR factors (see
+ * hifde_pivoted_qr_batch).  Outputs per face: k_out[b]; perm_out (sum n[b]
+ * ints): the k skeleton columns then the n-k redundant columns, in the
+ * reference's order; W_out (sum n[b]^2 doubles, face b at offset
+ * sum_{c<b} n[c]^2): W, k x (n-k). */
+int hifde_interp_decomp_batch(int nbatch, const int* m, const int* n, double* M_packed,
                               double eps, int* k_out, int* perm_out, double* W_out);
+/* PivotedQRWrapper(A, jpvt) -- src/InterpDecomp.cpp:156-166,198-211: LAPACK
+ * dgeqp3 with every column free, pivots made 0-based; over a batch.  A_packed
+ * (in/out): the m[b] x n[b] matrices packed one after another, overwritten
+ * with R on and above the diagonal and the Householder vectors below it
+ * (dgeqp3's layout, unit leading entries implicit).  jpvt_out (sum n[b]
+ * ints): column i of A P is column jpvt[i] of A.  tau_out (may be NULL;
+ * sum min(m[b], n[b]) doubles): the reflector scalars.  FP64 arithmetic on
+ * the GPU (the factorization's ID kernel). */
+int hifde_pivoted_qr_batch(int nbatch, const int* m, const int* n, double* A_packed,
+                           int* jpvt_out, double* tau_out);
+/* TriangularSolveWrapper(R, B) -- src/InterpDecomp.cpp:213-231 (dtrsm: left,
+ * upper, no transpose, non-unit diagonal), over a batch: B_b := R_b^{-1} B_b
+ * with R_b (m[b] x m[b], upper triangle read) packed one after another in
+ * R_packed and B_b (m[b] x n[b]) likewise in B_packed (overwritten).  FP64
+ * on the GPU; entries that are exactly zero are skipped as reference BLAS
+ * dtrsm skips them, and a zero diagonal divides by zero as dtrsm does. */
+int hifde_triangular_solve_batch(int nbatch, const int* m, const int* n, const double* R_packed,
+                                 double* B_packed);
 /* Schur(matrix, data) -- include/Schur.hpp:72-100, over a batch, given the
  * three blocks A22 = M(red,red) (r x r), A21 = M(red,skel) (r x s),
  * A12 = M(skel,red) (s x r), packed per box.  Outputs packed A22^{-1}, X and

@@ -57,10 +57,12 @@ def test_library_has_no_unresolved_internal_symbols():
 def test_cpp_dropin_header_compiles_and_links():
     import subprocess
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-    src = os.path.join(root, "tests", "cpp", "hif_driver.cpp")
-    subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(root, "include"),
-                    src], check=True)
-    drv = os.path.join(root, "tests", "cpp", "hif_driver")
-    assert os.path.exists(drv), "build() compiles tests/cpp/hif_driver"
-    out = subprocess.run(["ldd", drv], capture_output=True, text=True).stdout
-    assert "libhifde_amd.so" in out and "not found" not in out.split("libhifde_amd.so")[1].split("\n")[0]
+    for prog in ("hif_driver", "free_funcs"):
+        src = os.path.join(root, "tests", "cpp", prog + ".cpp")
+        subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(root, "include"),
+                        src], check=True)
+        drv = os.path.join(root, "tests", "cpp", prog)
+        assert os.path.exists(drv), f"build() compiles tests/cpp/{prog}"
+        out = subprocess.run(["ldd", drv], capture_output=True, text=True).stdout
+        assert "libhifde_amd.so" in out
+        assert "not found" not in out.split("libhifde_amd.so")[1].split("\n")[0]

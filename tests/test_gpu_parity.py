@@ -882,3 +882,137 @@ def test_leaf8_reaction_term_block_pcg_n128(H, oracle_mod):
     X = dX.cpu().numpy()
     for j in range(16):
         assert _rel(A @ X[j], B[j]) <= 2e-10, j
+
+
+def _assert_qr_factors_close(QR, R_o, M):
+    """R on and above the diagonal to rounding of ||M||; the Householder
+    vector of column i to rounding amplified by ||M|| / |R_ii| (it is the
+    column's remainder divided by a quantity of size |R_ii|)."""
+    m, n = M.shape
+    scale = np.abs(M).max() * max(m, 1)
+    assert np.allclose(np.triu(QR), np.triu(R_o), rtol=0, atol=1e-12 * scale)
+    for i in range(min(m, n)):
+        tol = 1e-12 * scale / max(abs(R_o[i, i]), 1e-300)
+        assert np.allclose(QR[i + 1:, i], R_o[i + 1:, i], rtol=0, atol=tol), i
+
+
+def test_cpp_free_functions_match_oracle(H, oracle_mod, tmp_path):
+    """tests/cpp/free_funcs calls DenseSubmatrix (x3), PivotedQRWrapper,
+    TriangularSolveWrapper, InterpDecomp and Schur through the drop-in header
+    (include/Schur.hpp:13-100, include/InterpDecomp.hpp:18-30); each result
+    against the oracle's restatement, SciPy or NumPy."""
+    import os
+    import subprocess
+    import scipy.linalg as sla
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    rng = _rng(21)
+    # generic matrices (no pivot ties), tall, square and wide, and one whose
+    # singular values fall from 1 to 1e-7.5 (numerical rank below its width
+    # at eps = 1e-6, every pivot still well above rounding)
+    U, _ = np.linalg.qr(rng.standard_normal((40, 16)))
+    Vt, _ = np.linalg.qr(rng.standard_normal((16, 16)))
+    mats = [rng.standard_normal((30, 9)), rng.standard_normal((12, 12)),
+            rng.standard_normal((7, 11)), (U * 10.0 ** (-0.5 * np.arange(16))) @ Vt]
+    eps = 1e-6
+    solves = []
+    for m, n in [(9, 5), (1, 3), (40, 7)]:
+        R = np.triu(rng.standard_normal((m, m))) + 4 * np.eye(m)
+        R[np.tril_indices(m, -1)] = 1e30  # below the diagonal: never read
+        B = rng.standard_normal((m, n))
+        B[0, 0] = 0.0
+        solves.append((R, B))
+    D = rng.standard_normal((8, 6))
+    rows, cols = [5, 0, 7, 7], [2, 5, 0]
+    sp_ent = [(1, 2, 0.5), (5, 2, -1.25), (7, 0, 3.0), (7, 0, 1.0), (0, 5, 2.0), (4, 4, 9.0)]
+    Sch = _spd(rng, 11)
+    r_s = (4, 7)
+
+    def i32(*v):
+        return np.array(v, np.int32).tobytes()
+    buf = [i32(len(mats))]
+    for M in mats:
+        buf += [i32(*M.shape), np.asfortranarray(M).ravel(order="F").tobytes()]
+    buf.append(np.array([eps]).tobytes())
+    buf.append(i32(len(solves)))
+    for R, B in solves:
+        buf += [i32(*B.shape), np.asfortranarray(R).ravel(order="F").tobytes(),
+                np.asfortranarray(B).ravel(order="F").tobytes()]
+    buf += [i32(*D.shape), np.asfortranarray(D).ravel(order="F").tobytes(),
+            i32(len(rows), *rows), i32(len(cols), *cols),
+            np.array([len(sp_ent)], np.int64).tobytes()]
+    for i, j, v in sp_ent:
+        buf += [i32(i, j), np.array([v]).tobytes()]
+    buf += [i32(*r_s), np.asfortranarray(Sch).ravel(order="F").tobytes()]
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    fin.write_bytes(b"".join(buf))
+    subprocess.run([os.path.join(root, "tests", "cpp", "free_funcs"), str(fin), str(fout)],
+                   check=True, timeout=120)
+    out = fout.read_bytes()
+    pos = [0]
+
+    def take(cnt, dt=np.float64):
+        n = cnt * np.dtype(dt).itemsize
+        a = np.frombuffer(out[pos[0]:pos[0] + n], dt)
+        pos[0] += n
+        return a
+
+    for M in mats:
+        m, n = M.shape
+        QR = take(m * n).reshape(m, n, order="F")
+        jp = take(n, np.int32)
+        R_o, jp_o, _ = oracle_mod.qrcp(M)
+        _, P_s = sla.qr(M, pivoting=True, mode="r")
+        assert np.array_equal(jp, jp_o) and np.array_equal(jp, P_s)
+        # R and the Householder vectors in dgeqp3's layout
+        _assert_qr_factors_close(QR, R_o, M)
+    for M in mats:
+        m, n = M.shape
+        Mq = take(m * n).reshape(m, n, order="F")
+        k = int(take(1, np.int32)[0])
+        skel, red = take(k, np.int32), take(n - k, np.int32)
+        W = take(k * (n - k)).reshape(k, n - k, order="F")
+        s_o, r_o, W_o, M_o = oracle_mod.interp_decomp(M, eps)
+        assert np.array_equal(skel, s_o) and np.array_equal(red, r_o)
+        assert np.allclose(W, W_o, rtol=1e-10, atol=1e-10)
+        # InterpDecomp overwrites M with its QR factors, as the reference does
+        _assert_qr_factors_close(Mq, M_o, M)
+    for R, B in solves:
+        X = take(B.size).reshape(B.shape, order="F")
+        X_s = sla.solve_triangular(np.triu(R), B, lower=False)
+        assert np.allclose(X, X_s, rtol=1e-12, atol=1e-12)
+    S1 = take(len(rows) * len(cols)).reshape(len(rows), len(cols), order="F")
+    S2 = take(D.shape[0] * len(cols)).reshape(D.shape[0], len(cols), order="F")
+    S3 = take(len(rows) * len(cols)).reshape(len(rows), len(cols), order="F")
+    assert np.array_equal(S1, D[np.ix_(rows, cols)])
+    assert np.array_equal(S2, D[:, cols])
+    Sd = np.zeros(D.shape)
+    for i, j, v in sp_ent:
+        Sd[i, j] += v  # Sparse::Add sums duplicates
+    assert np.array_equal(S3, Sd[np.ix_(rows, cols)])
+    r, s = r_s
+    Ai, X, S = (take(r * r).reshape(r, r, order="F"), take(r * s).reshape(r, s, order="F"),
+                take(s * s).reshape(s, s, order="F"))
+    Ai_o, X_o, S_o, info = oracle_mod.schur(Sch, np.arange(r), np.arange(r, r + s))
+    assert info == 0
+    assert _rel(Ai, Ai_o) < 1e-12 and _rel(X, X_o) < 1e-12 and _rel(S, S_o) < 1e-12
+    assert pos[0] == len(out)
+
+
+def test_pivoted_qr_and_triangular_solve_batches_match_lapack(H, oracle_mod):
+    """hifde_pivoted_qr_batch / hifde_triangular_solve_batch over ragged
+    batches (the C ABI behind PivotedQRWrapper / TriangularSolveWrapper)."""
+    import scipy.linalg as sla
+    rng = _rng(22)
+    mats = [rng.standard_normal(sh) for sh in [(30, 9), (9, 9), (5, 12), (1, 4), (60, 41)]]
+    for M, (QR, jp, tau) in zip(mats, H.pivoted_qr_batch(mats)):
+        R_o, jp_o, tau_o = oracle_mod.qrcp(M)
+        assert np.array_equal(jp, jp_o)
+        _assert_qr_factors_close(QR, R_o, M)
+        assert np.allclose(tau, tau_o[:min(M.shape)], rtol=0, atol=1e-12)
+    pairs = []
+    for m, n in [(16, 3), (3, 16), (200, 5), (0, 2), (2, 0)]:
+        R = np.triu(rng.standard_normal((m, m))) + 3 * np.eye(m)
+        pairs.append((R, rng.standard_normal((m, n))))
+    for (R, B), X in zip(pairs, H.triangular_solve_batch(pairs)):
+        if B.size:
+            assert np.allclose(X, sla.solve_triangular(R, B), rtol=1e-11, atol=1e-11)
