@@ -40,6 +40,8 @@ EXPORTS = [
     "hifde_kernel_stats", "hifde_set_option",
     "hifde_gemm_batch", "hifde_factor_save", "hifde_factor_load", "hifde_set_stream",
     "hifde_apply_device_noalloc", "hifde_pivoted_qr_batch", "hifde_triangular_solve_batch",
+    "hifde_set_stencil_device", "hifde_gen_uniform", "hifde_gen_normal", "hifde_gen_rhs_normal",
+    "hifde_gen_gaussian_field", "hifde_operator_csr",
 ]
 
 
@@ -75,6 +77,14 @@ def lib():
     L.hifde_apply.argtypes = [_vp, _dp, _i]
     L.hifde_apply_device.argtypes = [_vp, _vp, _i, _ll, _i]
     L.hifde_set_stream.argtypes = [_vp, _vp]
+    L.hifde_set_stencil_device.argtypes = [_vp, _vp, _vp, _i]
+    L.hifde_operator_csr.argtypes = [_vp, _ip, _ip, _dp]
+    L.hifde_operator_csr.restype = _ll
+    _u64 = ctypes.c_ulonglong
+    L.hifde_gen_uniform.argtypes = [_vp, _ll, _u64, _d, _d, _i, _vp]
+    L.hifde_gen_normal.argtypes = [_vp, _ll, _u64, _vp]
+    L.hifde_gen_rhs_normal.argtypes = [_vp, _i, _i, _ll, _u64, _vp]
+    L.hifde_gen_gaussian_field.argtypes = [_vp, _i, _u64, _d, _i, _vp]
     L.hifde_apply_device_noalloc.argtypes = [_vp, _vp, _i]
     L.hifde_spmv.argtypes = [_vp, _dp, _dp, _i]
     L.hifde_pcg.argtypes = [_vp, _dp, _dp, _d, _i, _ip, _dp]
@@ -178,6 +188,24 @@ class HIFFactor:
         if a.size != m ** 3 or V.size != m ** 3:
             raise ValueError("nodal tensors must have (N+2)^3 entries")
         _check(lib().hifde_set_stencil(self._h, _ptr(a), _ptr(V), int(symmetric)))
+
+    def set_stencil_device(self, a_ptr, V_ptr, symmetric=False):
+        """SetupStencil from device nodal tensors ((N+2)^3 doubles each, i
+        fastest): the operator is assembled in HBM."""
+        self._follow_caller_stream()
+        _check(lib().hifde_set_stencil_device(self._h, ctypes.c_void_p(a_ptr),
+                                              ctypes.c_void_p(V_ptr), int(symmetric)))
+
+    def operator_csr(self):
+        """The assembled operator as (rowptr, colind, vals)."""
+        nnz = lib().hifde_operator_csr(self._h, None, None, None)
+        _check(int(nnz))
+        rp = np.zeros(self.size + 1, np.int32)
+        ci = np.zeros(max(nnz, 1), np.int32)
+        v = np.zeros(max(nnz, 1))
+        _check(int(lib().hifde_operator_csr(self._h, _ptr(rp, ctypes.c_int), _ptr(ci, ctypes.c_int),
+                                            _ptr(v))))
+        return rp, ci[:nnz], v[:nnz]
 
     # --- factor / apply -----------------------------------------------------
     def Factor(self):
@@ -328,6 +356,58 @@ class HIFFactor:
 
     def factor_bytes(self):
         return int(lib().hifde_factor_bytes(self._h))
+
+
+# --- device generators (the definitions of problems.py, computed on the GPU)
+def _stream_arg():
+    return ctypes.c_void_p(_current_stream())
+
+
+def gen_uniform(ptr, n, seed, lo=0.0, hi=1.0, pow10=False):
+    """n doubles at device address ptr: lo + (hi - lo) U (or 10^ of it)."""
+    _check(lib().hifde_gen_uniform(ctypes.c_void_p(ptr), int(n), int(seed) & ((1 << 64) - 1),
+                                   float(lo), float(hi), int(pow10), _stream_arg()))
+
+
+def gen_normal(ptr, n, seed):
+    _check(lib().hifde_gen_normal(ctypes.c_void_p(ptr), int(n), int(seed) & ((1 << 64) - 1),
+                                  _stream_arg()))
+
+
+def gen_rhs_normal(ptr, NC, seed, nrhs=1, ld=None):
+    _check(lib().hifde_gen_rhs_normal(ctypes.c_void_p(ptr), int(NC), int(nrhs),
+                                      int(ld if ld is not None else NC ** 3),
+                                      int(seed) & ((1 << 64) - 1), _stream_arg()))
+
+
+def gen_gaussian_field(ptr, NC, seed, corr=0.1, lognormal=False):
+    _check(lib().hifde_gen_gaussian_field(ctypes.c_void_p(ptr), int(NC),
+                                          int(seed) & ((1 << 64) - 1), float(corr),
+                                          int(lognormal), _stream_arg()))
+
+
+def device_problem(NC, coeff, seed, V="zeros", v_seed=None, corr=0.1, lo=-1.0, hi=1.0):
+    """Nodal tensors a and V ((NC+1)^3 float64 torch tensors on the current
+    device, i fastest) generated on the GPU: coeff "pow10" (a = 10^U[lo, hi]),
+    "lognormal" (exp of the Gaussian field) or "ones"; V "zeros" or "uniform"
+    (U[0, 1], seed v_seed)."""
+    import torch
+    m3 = (NC + 1) ** 3
+    a = torch.empty(m3, dtype=torch.float64, device="cuda")
+    if coeff == "pow10":
+        gen_uniform(a.data_ptr(), m3, seed, lo, hi, pow10=True)
+    elif coeff == "lognormal":
+        gen_gaussian_field(a.data_ptr(), NC, seed, corr, lognormal=True)
+    elif coeff == "ones":
+        a.fill_(1.0)
+    else:
+        raise ValueError(coeff)
+    Vt = torch.zeros(m3, dtype=torch.float64, device="cuda")
+    if V == "uniform":
+        gen_uniform(Vt.data_ptr(), m3, v_seed, 0.0, 1.0)
+    elif V != "zeros":
+        raise ValueError(V)
+    return a, Vt
 
 
 def interp_decomp_batch(mats, eps):

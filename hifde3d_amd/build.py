@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libhifde_amd.so")
 SOURCES = ["level_table.hip", "level_elem.hip", "dense_batched.hip", "apply_pcg.hip",
-           "multi_rhs.hip", "factor_host.hip"]
+           "multi_rhs.hip", "assemble.hip", "factor_host.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

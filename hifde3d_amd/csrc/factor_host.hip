@@ -180,8 +180,30 @@ struct hifde_factor {
     LevelTable& table_ref() { return tabs[cur]; }
     bool have_matrix = false, factored = false;
     // operator kept for SpMV / PCG (CSR)
+    // Assembled operator: CSR on the device (d_rp, d_ci, d_v, d_rr) and a host
+    // copy (h_*), which device assembly leaves stale until a host-side use
+    // (the reference-form SpMV, save) downloads it.
     std::vector<int> h_rp, h_ci;
     std::vector<double> h_v;
+    long long nnz_op = 0;
+    bool host_csr_valid = false;
+    void ensure_host_csr() {
+        if (host_csr_valid) return;
+        const long long nd = (long long)NC * NC * NC;
+        h_rp.resize(nd + 1);
+        h_ci.resize(nnz_op);
+        h_v.resize(nnz_op);
+        HIFDE_CHECK(hipMemcpyAsync(h_rp.data(), d_rp.p, (nd + 1) * sizeof(int),
+                                   hipMemcpyDeviceToHost, stream));
+        if (nnz_op) {
+            HIFDE_CHECK(hipMemcpyAsync(h_ci.data(), d_ci.p, nnz_op * sizeof(int),
+                                       hipMemcpyDeviceToHost, stream));
+            HIFDE_CHECK(hipMemcpyAsync(h_v.data(), d_v.p, nnz_op * sizeof(double),
+                                       hipMemcpyDeviceToHost, stream));
+        }
+        HIFDE_CHECK(hipStreamSynchronize(stream));
+        host_csr_valid = true;
+    }
     DevBuf<int> d_rp, d_ci, d_rr;
     DevBuf<double> d_v;
     // per-kernel-family device time (HIP events), enabled by hifde_set_profiling
@@ -913,7 +935,7 @@ struct hifde_factor {
         if (level == 0) {
             const int c = NC / P;
             if (!plan0_valid) init_level0_index(B);
-            Prof pr(this, fam<T>("assemble"), 0.0, fr_bytes + (double)h_ci.size() * 12.0);
+            Prof pr(this, fam<T>("assemble"), 0.0, fr_bytes + (double)nnz_op * 12.0);
             frontal_init<T>(nb, d_init.p, d_etab.p, d_egid.p, d_rp.p, d_ci.p, d_v.p, NC, P, c,
                             A22, A21, A12, E, stream);
         } else {
@@ -1396,7 +1418,7 @@ struct hifde_factor {
         if (!elem_on) {
             // Initialize(): the level matrix starts as the assembled operator.
             auto t0 = clk::now();
-            long long nnz = (long long)h_ci.size();
+            long long nnz = nnz_op;
             cur = 0;
             tabs[0].sym = tabs[1].sym = sym_on;
             table_init(tabs[cur], (size_t)(sym_on ? (nnz + NC * NC * NC) / 2 : nnz), stream);
@@ -1888,7 +1910,7 @@ struct hifde_factor {
     }
 
     void spmv_dev(const double* x, double* y) {
-        Prof pr(this, "spmv", 2.0 * h_ci.size(), 12.0 * h_ci.size() + 20.0 * NC * NC * NC);
+        Prof pr(this, "spmv", 2.0 * nnz_op, 12.0 * nnz_op + 20.0 * NC * NC * NC);
         csr_spmv(NC * NC * NC, d_rp.p, d_ci.p, d_v.p, x, y, stream);
     }
 
@@ -1981,7 +2003,7 @@ struct hifde_factor {
         };
         // r = b - A x, boundary rows cleared
         {
-            Prof pr(this, "spmv", 2.0 * h_ci.size() * R, (12.0 * h_ci.size() + 16.0 * n) * R);
+            Prof pr(this, "spmv", 2.0 * nnz_op * R, (12.0 * nnz_op + 16.0 * n) * R);
             csr_spmm((int)n, d_rp.p, d_ci.p, d_v.p, X, ldx, Q, n, R, stream);
         }
         for (int c = 0; c < R; ++c) vcopy(B + c * ldb, Rm + c * n, n, stream);
@@ -2016,8 +2038,7 @@ struct hifde_factor {
             std::vector<double> alpha(R), beta(R);
             for (int k = 1; k <= maxit && active > 0; ++k) {
                 {
-                    Prof pr(this, "spmv", 2.0 * h_ci.size() * R,
-                            (12.0 * h_ci.size() + 16.0 * n) * R);
+                    Prof pr(this, "spmv", 2.0 * nnz_op * R, (12.0 * nnz_op + 16.0 * n) * R);
                     csr_spmm((int)n, d_rp.p, d_ci.p, d_v.p, Pm, n, Q, n, R, stream);
                 }
                 std::vector<double> pq = dots(Pm, Q);
@@ -2124,6 +2145,7 @@ struct hifde_factor {
         fl.put(hdr, sizeof hdr);
         fl.put(&eps, sizeof eps);
         // operator, as COO in CSR order
+        ensure_host_csr();
         std::vector<int> rows(h_ci.size());
         for (size_t i = 0; i + 1 < h_rp.size(); ++i)
             for (int q = h_rp[i]; q < h_rp[i + 1]; ++q) rows[q] = (int)i;
@@ -2270,6 +2292,8 @@ struct hifde_factor {
                     }
                 op_symmetric = found;
             }
+        nnz_op = (long long)h_ci.size();
+        host_csr_valid = true;
         d_rp.upload(h_rp, stream);
         d_ci.upload(h_ci, stream);
         d_v.upload(h_v, stream);
@@ -2371,6 +2395,21 @@ struct hifde_factor {
                 ++w;
             }
         set_coo(w, rows.data(), cols.data(), vals.data());
+    }
+
+    // SetupStencil from device nodal tensors (include/setup_stencil.hpp:6-76):
+    // the CSR is built in HBM, with the same rows, entry order and values as
+    // set_stencil.  Nearest-neighbour couplings always share a level-0 cell
+    // closure, so the element storage applies.
+    void set_stencil_device(const double* d_a, const double* d_V, int symmetric) {
+        const long long nd = (long long)NC * NC * NC;
+        nnz_op = stencil_csr(NC, d_a, d_V, symmetric, d_rp, d_ci, d_v, d_rr, stream);
+        host_csr_valid = false;
+        op_symmetric = csr_symmetric(nd, d_rp.p, d_ci.p, d_v.p, stream);
+        local_ok = true;
+        HIFDE_CHECK(hipStreamSynchronize(stream));
+        have_matrix = true;
+        factored = false;
     }
 };
 
@@ -2530,6 +2569,33 @@ int hifde_set_stencil(hifde_factor* f, const double* a, const double* v, int sym
     });
 }
 
+int hifde_set_stencil_device(hifde_factor* f, const double* d_a, const double* d_v,
+                             int symmetric) {
+    return guarded([&]() -> int {
+        OwnerGuard og(f ? f->stream : nullptr);
+        if (!f || !d_a || !d_v) throw Error(HIFDE_ERR_ARG, "set_stencil_device: null argument");
+        f->after_caller_stream();
+        f->set_stencil_device(d_a, d_v, symmetric);
+        return HIFDE_OK;
+    });
+}
+
+long long hifde_operator_csr(hifde_factor* f, int* rowptr, int* colind, double* vals) {
+    long long nnz = 0;
+    const int rc = guarded([&]() -> int {
+        if (!f) throw Error(HIFDE_ERR_ARG, "operator_csr: null");
+        if (!f->have_matrix) throw Error(HIFDE_ERR_STATE, "operator_csr: no matrix");
+        OwnerGuard og(f->stream);
+        f->ensure_host_csr();
+        if (rowptr) std::copy(f->h_rp.begin(), f->h_rp.end(), rowptr);
+        if (colind) std::copy(f->h_ci.begin(), f->h_ci.end(), colind);
+        if (vals) std::copy(f->h_v.begin(), f->h_v.end(), vals);
+        nnz = f->nnz_op;
+        return HIFDE_OK;
+    });
+    return rc < 0 ? (long long)rc : nnz;
+}
+
 int hifde_factor_run(hifde_factor* f) {
     return guarded([&]() -> int {
         if (!f) throw Error(HIFDE_ERR_ARG, "factor: null");
@@ -2602,6 +2668,7 @@ int hifde_spmv(hifde_factor* f, const double* x, double* y, int transpose) {
         long long n = (long long)f->NC * f->NC * f->NC;
         if (transpose) {
             // y = A^T x (the reference's column iteration, src/Factor.cpp:1099-1113)
+            f->ensure_host_csr();
             std::fill(y, y + n, 0.0);
             for (long long i = 0; i < n; ++i)
                 for (int p = f->h_rp[i]; p < f->h_rp[i + 1]; ++p) y[f->h_ci[p]] += f->h_v[p] * x[i];
@@ -2845,6 +2912,54 @@ int hifde_interp_decomp_batch(int nbatch, const int* m, const int* n, double* M_
             throw Error(HIFDE_ERR_ARG, "interp_decomp_batch: bad arguments");
         if (nbatch == 0) return HIFDE_OK;
         qr_batch(nbatch, m, n, M_packed, eps, k_out, perm_out, W_out, nullptr, nullptr);
+        return HIFDE_OK;
+    });
+}
+
+int hifde_gen_uniform(double* d_out, long long n, unsigned long long seed, double lo, double hi,
+                      int pow10, void* stream) {
+    return guarded([&]() -> int {
+        if (n < 0 || (n && !d_out)) throw Error(HIFDE_ERR_ARG, "gen_uniform: bad arguments");
+        gen_uniform(d_out, n, seed, lo, hi, pow10 != 0, static_cast<hipStream_t>(stream));
+        return HIFDE_OK;
+    });
+}
+
+int hifde_gen_normal(double* d_out, long long n, unsigned long long seed, void* stream) {
+    return guarded([&]() -> int {
+        if (n < 0 || (n && !d_out)) throw Error(HIFDE_ERR_ARG, "gen_normal: bad arguments");
+        gen_normal(d_out, n, seed, static_cast<hipStream_t>(stream));
+        return HIFDE_OK;
+    });
+}
+
+int hifde_gen_rhs_normal(double* d_out, int NC, int nrhs, long long ld, unsigned long long seed,
+                         void* stream) {
+    return guarded([&]() -> int {
+        if (NC < 2 || nrhs < 0 || (nrhs && !d_out) ||
+            (nrhs > 1 && ld < (long long)NC * NC * NC))
+            throw Error(HIFDE_ERR_ARG, "gen_rhs_normal: bad arguments");
+        gen_rhs(d_out, NC, nrhs, nrhs > 1 ? ld : (long long)NC * NC * NC, seed,
+                static_cast<hipStream_t>(stream));
+        return HIFDE_OK;
+    });
+}
+
+int hifde_gen_gaussian_field(double* d_out, int NC, unsigned long long seed, double corr,
+                             int lognormal, void* stream) {
+    return guarded([&]() -> int {
+        if (NC < 2 || !d_out || !(corr > 0.0))
+            throw Error(HIFDE_ERR_ARG, "gen_gaussian_field: bad arguments");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        if (!s) {
+            // the caller's stream is the legacy default stream: start after
+            // its work, then run on this thread's own stream (the
+            // temporaries are grown and freed in stream order)
+            HIFDE_CHECK(hipStreamSynchronize(nullptr));
+            s = aux_stream();
+        }
+        OwnerGuard og(s);
+        gen_gaussian_field(d_out, NC, seed, corr, lognormal != 0, s);
         return HIFDE_OK;
     });
 }

@@ -371,6 +371,19 @@ void batched_interp_decomp(int nface, const int* d_m, const int* d_n, const long
                            int* d_k, double* d_W, const long long* d_offW, int max_n,
                            int max_m, const unsigned char* d_mode, hipStream_t s,
                            int* d_jpvt = nullptr, double* d_tau = nullptr);
+// Device problem assembly and input generators (assemble.hip).
+void gen_uniform(double* d_out, long long n, unsigned long long seed, double lo, double hi,
+                 bool pow10, hipStream_t s);
+void gen_normal(double* d_out, long long n, unsigned long long seed, hipStream_t s);
+void gen_rhs(double* d_out, int NC, int nrhs, long long ld, unsigned long long seed,
+             hipStream_t s);
+void gen_gaussian_field(double* d_out, int NC, unsigned long long seed, double corr,
+                        bool lognormal, hipStream_t s);
+long long stencil_csr(int NC, const double* d_a, const double* d_V, int symmetric,
+                      DevBuf<int>& rp, DevBuf<int>& ci, DevBuf<double>& val, DevBuf<int>& rr,
+                      hipStream_t s);
+bool csr_symmetric(long long nd, const int* d_rp, const int* d_ci, const double* d_val,
+                   hipStream_t s);
 // B_b := R_b^{-1} B_b, R_b upper triangular m_b x m_b, B_b m_b x n_b, both
 // packed column-major at their offsets.
 void batched_trsm_upper(int nbox, const int* d_m, const int* d_n, const long long* d_offR,

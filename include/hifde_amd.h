@@ -76,6 +76,47 @@ int hifde_set_matrix_coo(hifde_factor* f, long long nnz, const int* rows, const 
 int hifde_set_stencil(hifde_factor* f, const double* a_nodal, const double* v_nodal,
                       int symmetric);
 
+/* SetupStencil as hifde_set_stencil, from nodal tensors in device memory
+ * (d_a, d_v: (N+2)^3 doubles each, same layout): the CSR operator is built
+ * in HBM with the same rows, entry order and values (bitwise) as the host
+ * assembly.  Ordered after the caller's stream (hifde_set_stream); returns
+ * when done. */
+int hifde_set_stencil_device(hifde_factor* f, const double* d_a, const double* d_v,
+                             int symmetric);
+
+/* The assembled operator as CSR (the reference's sp_matrix() before
+ * Factor() consumes it; it is kept here): rowptr ((N+1)^3 + 1 ints), colind
+ * and vals (nnz each), rows in linear order, entries in assembly order with
+ * duplicates summed.  Any pointer may be NULL.  Returns nnz, or < 0. */
+long long hifde_operator_csr(hifde_factor* f, int* rowptr, int* colind, double* vals);
+
+/* Synthetic-input generators on device memory (no reference counterpart:
+ * the reference fills its tensors by hand with a = 1, V = 0,
+ * tests/HIFDE3D/main.cpp:59-74).  Each runs on the given stream (a
+ * hipStream_t as void*, NULL for the legacy default stream).  Number t of a
+ * seed's sequence is the SplitMix64 output t + 1 (seed + (t+1) * 0x9E3779B97F4A7C15
+ * through the published finaliser); a uniform is its top 53 bits / 2^53.
+ *   hifde_gen_uniform: d_out[t] = lo + (hi - lo) u_t, or 10^(that) with pow10
+ *   hifde_gen_normal: Box-Muller, d_out[2p] = r cos(2 pi u_{2p+1}),
+ *     d_out[2p+1] = r sin(2 pi u_{2p+1}), r = sqrt(-2 log(1 - u_{2p}))
+ *   hifde_gen_rhs_normal: nrhs vectors of NC^3 (column c at d_out + c*ld):
+ *     normal numbers c*N^3 + rank on the interior DOFs (rank = position in
+ *     linear-index order), 0 on the boundary planes
+ *   hifde_gen_gaussian_field: unit-variance Gaussian random field on the
+ *     (NC+1)^3 nodes of the unit cube with covariance exp(-|x-y|^2/(2 corr^2)):
+ *     normal white noise (numbers in order) on the doubled grid
+ *     P = 2(NC+1), filtered by the square root of the spectral density
+ *     (a periodic separable filter on spacing 1/NC), the first NC+1 nodes per
+ *     axis kept, shifted to mean 0 and scaled to unit population variance;
+ *     with lognormal != 0, exp of it.  Returns after the stream completes. */
+int hifde_gen_uniform(double* d_out, long long n, unsigned long long seed, double lo, double hi,
+                      int pow10, void* stream);
+int hifde_gen_normal(double* d_out, long long n, unsigned long long seed, void* stream);
+int hifde_gen_rhs_normal(double* d_out, int NC, int nrhs, long long ld, unsigned long long seed,
+                         void* stream);
+int hifde_gen_gaussian_field(double* d_out, int NC, unsigned long long seed, double corr,
+                             int lognormal, void* stream);
+
 /* Initialize() + Factor() -- src/Factor.cpp:6-39, 41-63.  The level matrix is
  * consumed, as in the reference; the assembled operator is kept for SpMV
  * and PCG. */

@@ -419,6 +419,36 @@ def test_splitmix64_published_first_output():
     assert int(pb.splitmix64(0, 1)[0]) == 0xE220A8397B1DCDAF
 
 
+def test_splitmix64_matches_scalar_restatement():
+    """The vectorised generator against a plain-integer SplitMix64 (output t
+    of a seed is the finaliser applied to seed + t * 0x9E3779B97F4A7C15)."""
+    M = (1 << 64) - 1
+
+    def scalar(seed, t):
+        z = (seed + t * 0x9E3779B97F4A7C15) & M
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+        return z ^ (z >> 31)
+    for seed in (0, 1, 2, 3000, (1 << 64) - 5):
+        v = pb.splitmix64(seed, 40)
+        assert [int(x) for x in v] == [scalar(seed, t) for t in range(1, 41)]
+    u = pb.uniform(7, 5)
+    assert list(u) == [(scalar(7, t) >> 11) / 2.0 ** 53 for t in range(1, 6)]
+
+
+def test_generators_reproduce_pinned_first_values():
+    import json
+    import os
+    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "generators.json")))
+    assert pb.uniform(1, 16).tolist() == g["uniform_seed1"]
+    assert pb.normal(2, 16).tolist() == g["normal_seed2"]
+    assert pb.pow10_coeff(8, 1).ravel(order="F")[:16].tolist() == g["pow10_NC8_seed1_first16"]
+    assert np.allclose(pb.gaussian_field(16, 2, 0.1).ravel(order="F")[:16],
+                       g["gaussian_field_NC16_seed2_corr0.1_first16"], rtol=0, atol=1e-12)
+    assert pb.rhs_normal(8, 0)[pb.interior_mask(8)][:16].tolist() == \
+        g["rhs_normal_NC8_seed0_interior_first16"]
+
+
 def test_normal_and_field_statistics():
     z = pb.normal(1, 200000)
     assert abs(z.mean()) < 0.01 and abs(z.std() - 1) < 0.01
