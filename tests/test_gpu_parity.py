@@ -241,6 +241,44 @@ def test_c1_face_boxes_match_oracle(H, oracle_mod):
             assert set(g["red_gid"]) | set(g["skel_gid"]) == set(r["g"])
 
 
+def test_c1_grid_face_blocks_match_oracle_per_face(H, oracle_mod):
+    """C1's grid (n=16, P=4, eps 1e-6) with a generic coefficient (10^U,
+    symmetric stencil: no pivot ties) and the Householder ID on every face.
+    Per face: the skeleton and redundant DOF sequences equal the oracle's,
+    and W, X and A22^{-1} of SchurAfterID agree -- to rounding at level 0
+    (the same operations in the same order), to 1e-9 above.  The same
+    factorization without the FRONT quirk (src/Factor.cpp:486) changes the
+    skeleton sequences of the level-0 faces whose ID rows the quirk adds to,
+    so the comparison pins the GPU's face row lists too."""
+    a, V, coo, _ = _coo(oracle_mod, 15, "pow10", True)
+    o = oracle_mod.Oracle(15, 4, 1e-6)
+    o.set_matrix_coo(*coo)
+    assert o.factor(4) == 0
+    for quirk in (True, False):
+        f = H.HIFFactor(15, 4, 1e-6, precision="fp64", front_quirk=quirk)
+        f.set_option("gram_min_n", 1 << 30)
+        f.set_stencil(a, V, True)
+        assert f.Factor() == 0
+        mismatched = 0
+        for lv in range(f.num_levels - 1):
+            assert f.num_boxes(lv, 1) == o.num_boxes(lv, 1)
+            tol = 1e-13 if lv == 0 else 1e-9
+            for b in range(f.num_boxes(lv, 1)):
+                g, r = f.box(lv, 1, b), o.box(lv, 1, b)
+                same = (np.array_equal(g["skel_gid"], r["g"][r["skel"]])
+                        and np.array_equal(g["red_gid"], r["g"][r["red"]]))
+                if not quirk:
+                    mismatched += lv == 0 and not same
+                    continue
+                assert same, (lv, b)
+                for key in ("W", "X", "A22inv"):
+                    if r[key].size:
+                        assert _rel(g[key], r[key]) < tol, (lv, b, key)
+        if not quirk:
+            assert mismatched > 0
+        f.close()
+
+
 def test_eps_tiny_gives_exact_factor(H, oracle_mod):
     f, _, A = _both(H, oracle_mod, 15, 4, 1e-14, "pow10", True)
     v = pb.rhs_normal(16, 0)
