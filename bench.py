@@ -257,17 +257,20 @@ def c5_workload(args):
     the factor at this size) are then released so that the block PCG fits.
     A step is one factorization plus one block solve."""
     import torch
-    from hifde3d_amd import HIFFactor, problems as pb
+    import hifde3d_amd as H
     NC, P, eps, R = 256, 8, 1e-3, args.c5_rhs
     t0 = time.perf_counter()
-    a = pb.lognormal_coeff(NC, 4)
-    V = pb.uniform_coeff(NC, 40)
-    f = HIFFactor(NC - 1, P, eps, precision="mixed")
-    f.set_stencil(a, V, True)
-    log(f"[bench] c5 problem set up in {time.perf_counter() - t0:.1f}s")
+    # coefficients generated and the operator assembled on the GPU
+    a, V = H.device_problem(NC, "lognormal", 4, V="uniform", v_seed=40)
+    f = H.HIFFactor(NC - 1, P, eps, precision="mixed")
+    if args.bf16x3:
+        f.set_option("bf16x3", 1)
+    f.set_stencil_device(a.data_ptr(), V.data_ptr(), True)
+    torch.cuda.synchronize()
+    log(f"[bench] c5 problem set up on the GPU in {time.perf_counter() - t0:.1f}s")
     assert f.Factor() == 0
-    f.set_stencil(a, V, True)
-    del a
+    f.set_stencil_device(a.data_ptr(), V.data_ptr(), True)
+    del a, V
     torch.cuda.synchronize()
     t = time.perf_counter()
     assert f.Factor() == 0
@@ -278,7 +281,8 @@ def c5_workload(args):
     f.set_option("keep_workspaces", 0)
     log(f"[bench] c5 factor {factor_s:.3f}s, factor {fbytes / 2**30:.1f} GiB, "
         f"free {torch.cuda.mem_get_info()[0] / 2**30:.1f} GiB")
-    B = torch.from_numpy(pb.rhs_normal(NC, 41, R)).cuda()
+    B = torch.empty((R, NC ** 3), dtype=torch.float64, device="cuda")
+    H.gen_rhs_normal(B.data_ptr(), NC, 41, nrhs=R)
     X = torch.zeros_like(B)
     solve = []
     for rep in range(1 + args.steps):
@@ -296,11 +300,13 @@ def c5_workload(args):
         "metric": f"NC=256 P=8 factor + {R}-RHS block PCG to 1e-10, steps/s",
         "value": 1.0 / (factor_s + solve_s), "unit": "steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": 1, "ms_per_step": 1e3 * (factor_s + solve_s),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
-        "data": "synthetic",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16x3+f32+f64" if args.bf16x3 else "f32+f64",
+        "data": "synthetic (generated on the GPU)",
         "config": {"workload": ("NC=256 (255^3 unknowns), P=8, eps=1e-3, lognormal a (corr 0.1, "
                                 "seed 4), V ~ U[0,1] (seed 40), symmetric 7-pt stencil, mixed "
-                                f"precision, {R} RHS N(0,1) (seed 41), block PCG to 1e-10"),
+                                f"precision{' with bf16x3 Schur products' if args.bf16x3 else ''}, "
+                                f"{R} RHS N(0,1) (seed 41), block PCG to 1e-10"),
                    "NC": NC, "P": P, "eps": eps, "nrhs": R, "tol": args.tol},
         "factor_s": factor_s, "solve_s": solve_s, "rhs_per_s": R / solve_s,
         "pcg_iters_max": int(max(its)), "relres_max": float(max(rr)),
