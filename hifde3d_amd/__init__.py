@@ -41,7 +41,8 @@ EXPORTS = [
     "hifde_gemm_batch", "hifde_factor_save", "hifde_factor_load", "hifde_set_stream",
     "hifde_apply_device_noalloc", "hifde_pivoted_qr_batch", "hifde_triangular_solve_batch",
     "hifde_set_stencil_device", "hifde_gen_uniform", "hifde_gen_normal", "hifde_gen_rhs_normal",
-    "hifde_gen_gaussian_field", "hifde_operator_csr",
+    "hifde_gen_gaussian_field", "hifde_operator_csr", "hifde_gemm_table_load",
+    "hifde_gemm_table_clear", "hifde_gemm_log", "hifde_gemm_time",
 ]
 
 
@@ -80,6 +81,11 @@ def lib():
     L.hifde_set_stencil_device.argtypes = [_vp, _vp, _vp, _i]
     L.hifde_operator_csr.argtypes = [_vp, _ip, _ip, _dp]
     L.hifde_operator_csr.restype = _ll
+    L.hifde_gemm_table_load.argtypes = [ctypes.c_char_p]
+    L.hifde_gemm_table_clear.argtypes = []
+    L.hifde_gemm_log.argtypes = [_vp, _ip, _ll]
+    L.hifde_gemm_log.restype = _ll
+    L.hifde_gemm_time.argtypes = [_i, _i, _i, _i, _ip, _ip, _ip, _i, _i, _dp]
     _u64 = ctypes.c_ulonglong
     L.hifde_gen_uniform.argtypes = [_vp, _ll, _u64, _d, _d, _i, _vp]
     L.hifde_gen_normal.argtypes = [_vp, _ll, _u64, _vp]
@@ -357,6 +363,14 @@ class HIFFactor:
     def factor_bytes(self):
         return int(lib().hifde_factor_bytes(self._h))
 
+    def gemm_log(self):
+        """Descriptors of the last factorization (option "gemm_record" = 1):
+        int array of rows (call, esize, transA, lower, m, n, k, ncall, kind)."""
+        n = int(lib().hifde_gemm_log(self._h, None, 0))
+        out = np.zeros(max(n, 9), np.int32)
+        lib().hifde_gemm_log(self._h, _ptr(out, ctypes.c_int), n)
+        return out[:n].reshape(-1, 9)
+
 
 # --- device generators (the definitions of problems.py, computed on the GPU)
 def _stream_arg():
@@ -476,19 +490,46 @@ def triangular_solve_batch(pairs):
     return out
 
 
+def gemm_time(precision, transA, lower, shapes, kind, reps=5):
+    """Average device seconds of one batched launch of products (m, n, k)
+    forced onto kernel kind (0 FMA, 1 matrix-core 128, 2 matrix-core 64)."""
+    m = np.array([x[0] for x in shapes], np.int32)
+    n = np.array([x[1] for x in shapes], np.int32)
+    k = np.array([x[2] for x in shapes], np.int32)
+    t = ctypes.c_double(0)
+    _check(lib().hifde_gemm_time({"fp64": FP64, "mixed": MIXED}[precision], int(transA),
+                                 int(lower), len(shapes), _ptr(m, ctypes.c_int),
+                                 _ptr(n, ctypes.c_int), _ptr(k, ctypes.c_int), int(kind),
+                                 int(reps), ctypes.byref(t)))
+    return t.value
+
+
+def gemm_table_load(path=None):
+    """Load a GEMM routing table (None: the one shipped beside the library)."""
+    return _check(lib().hifde_gemm_table_load(os.fsencode(path) if path else None))
+
+
+def gemm_table_clear():
+    _check(lib().hifde_gemm_table_clear())
+
+
 BF16X3 = 0x200
 
 
 GEMM_TILE64 = 0x400
 
 
-def gemm_batch(pairs, precision="fp64", transA=False, bf16x3=False, tile64=False):
+GEMM_FORCE = {0: 0x800, 1: 0x1000, 2: 0x2000}
+
+
+def gemm_batch(pairs, precision="fp64", transA=False, bf16x3=False, tile64=False, kind=None):
     """[op(A) @ B for (A, B) in pairs] through the library's batched GEMM.
     bf16x3 (precision "mixed" only): matrix-core products as bf16x3 split
     products.  tile64: products with both output sides of at least 32 on
-    64 x 64 matrix-core tiles."""
+    64 x 64 matrix-core tiles.  kind: force one kernel (0 FMA, 1 matrix-core
+    128, 2 matrix-core 64)."""
     prec = ({"fp64": FP64, "mixed": MIXED}[precision] | (BF16X3 if bf16x3 else 0)
-            | (GEMM_TILE64 if tile64 else 0))
+            | (GEMM_TILE64 if tile64 else 0) | (GEMM_FORCE[kind] if kind is not None else 0))
     m = np.array([(a.shape[1] if transA else a.shape[0]) for a, _ in pairs], np.int32)
     k = np.array([(a.shape[0] if transA else a.shape[1]) for a, _ in pairs], np.int32)
     n = np.array([b.shape[1] for _, b in pairs], np.int32)

@@ -15,6 +15,8 @@
 //                   (src/Factor.cpp:113-153) restricted to the three blocks
 //                   Schur needs.
 
+#include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <type_traits>
 
@@ -1737,6 +1739,7 @@ void batched_gj_inverse(int nbox, const int* h_n, const long long* h_off, const 
     // the window kernels.
     GemmPlan& plan = ws.plan;
     plan.clear();
+    plan.esize = (int)sizeof(T);
     std::vector<std::pair<int, int>> gh;  // GEMM launch ranges in walk order
     std::vector<Copy2D> cps;
     std::vector<int> cfirst, cnum, cmaxc;  // per outer step: copy range and widest copy
@@ -2000,6 +2003,42 @@ constexpr const char* type_name() {
 // product of depth above 64 takes the 64 x 64 tiles (GemmPlan::tile64 >= 1).
 constexpr double kTile64Waste = 1.25;
 
+namespace {
+std::atomic<const signed char*> g_tune_table{nullptr};
+thread_local GemmRecorder* t_recorder = nullptr;
+int bin_of(long long v, const long long* edges, int nedge) {
+    int b = 0;
+    while (b < nedge && v > edges[b]) ++b;
+    return b;
+}
+}  // namespace
+GemmRecorder* active_gemm_recorder() { return t_recorder; }
+void set_active_gemm_recorder(GemmRecorder* r) { t_recorder = r; }
+
+int gemm_tune_key(int esize, int transA, int lower, int m, int n, int k, int nb) {
+    static const long long kE[kTuneKBins - 1] = {16, 32, 64, 128, 256, 512, 1024};
+    static const long long mnE[kTuneMNBins - 1] = {31, 63, 127, 255, 511};
+    static const long long nbE[kTuneNBBins - 1] = {1, 7, 63, 511};
+    const int kb = bin_of(k, kE, kTuneKBins - 1);
+    const int mb = bin_of(std::min(m, n), mnE, kTuneMNBins - 1);
+    const int bb = bin_of(nb, nbE, kTuneNBBins - 1);
+    return ((((((esize == 4 ? 0 : 1) * 2 + (transA ? 1 : 0)) * 2 + (lower ? 1 : 0)) * kTuneKBins +
+              kb) * kTuneMNBins + mb) * kTuneNBBins) + bb;
+}
+int gemm_tune_lookup(int key) {
+    const signed char* t = g_tune_table.load(std::memory_order_acquire);
+    if (!t || key < 0 || key >= kTuneEntries) return -1;
+    return t[key];
+}
+// Installs a routing table (kTuneEntries kinds); the table lives until the
+// process ends (a replaced one is kept too: plans on other threads may be
+// reading it).
+void gemm_tune_install(const signed char* kinds) {
+    signed char* t = new signed char[kTuneEntries];
+    std::copy(kinds, kinds + kTuneEntries, t);
+    g_tune_table.store(t, std::memory_order_release);
+}
+
 std::pair<int, int> GemmPlan::add(const std::vector<GemmDesc>& d, bool transA) {
     constexpr int TILE = kMfmaTile;
     const int first = (int)launches.size();
@@ -2045,9 +2084,22 @@ std::pair<int, int> GemmPlan::add(const std::vector<GemmDesc>& d, bool transA) {
         if (tile64 >= 2 && small && g.k > 0 && g.m >= 32 && g.n >= 32 && fits) kind = 2;
         // 3: every product with both output sides of at least 32
         if (tile64 >= 3 && g.k > 0 && g.m >= 32 && g.n >= 32 && fits) kind = 2;
+        // kinds the matrix-core kernel has been validated for: both output
+        // sides of at least 32, a positive depth, operands within its
+        // 32-bit offsets
+        const bool core_ok = fits && g.k > 0 && g.m >= 32 && g.n >= 32;
+        int want = force_kind;
+        if (want < 0 && use_table)
+            want = gemm_tune_lookup(
+                gemm_tune_key(esize, transA, g.lower, g.m, g.n, g.k, (int)d.size()));
+        if (want == 0 || (want > 0 && core_ok)) kind = want;
         kinds[i] = (signed char)kind;
         ++count[kind];
+        if (GemmRecorder* rec = t_recorder)
+            rec->log.insert(rec->log.end(), {rec->calls, esize, transA ? 1 : 0, g.lower, g.m,
+                                             g.n, g.k, (int)d.size(), kind});
     }
+    if (t_recorder) ++t_recorder->calls;
     desc.reserve(desc.size() + d.size());
     tiles.reserve(tiles.size() + d.size() + 3);
     for (int pass = 0; pass < 3; ++pass) {
@@ -2153,6 +2205,7 @@ void batched_gemm_desc(bool transA, const std::vector<GemmDesc>& desc, const T* 
                        const int* d_flags) {
     if (desc.empty()) return;
     ws.plan.clear();
+    ws.plan.esize = (int)sizeof(T);
     const std::pair<int, int> h = ws.plan.add(desc, transA);
     if (h.first == h.second) return;
     ws.plan.upload(s);
@@ -2236,6 +2289,7 @@ void gram_pivchol(int nface, const int* h_n, const unsigned char* h_mode, const 
     ws.host_active.resize(1);
     // the trailing update has the same descriptors after every panel
     ws.plan.clear();
+    ws.plan.esize = 8;
     const std::pair<int, int> uh = ws.plan.add(upd, true);
     ws.plan.upload(s);
     const size_t shm_reg = (size_t)max_n * 2 * sizeof(int);

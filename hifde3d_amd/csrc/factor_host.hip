@@ -8,6 +8,7 @@
 // boxes of a half level read the same pre-level matrix, so each half level is
 // one batch of boxes on the GPU.
 
+#include <dlfcn.h>
 #include <omp.h>
 
 #include <algorithm>
@@ -1398,6 +1399,16 @@ struct hifde_factor {
             ~StagerGuard() { set_active_stager(prev); }
         } guard(&stager);
         TimerGuard tguard(this);
+        // GEMM routing log of this factorization (the tuner's input)
+        struct RecGuard {
+            GemmRecorder* prev;
+            RecGuard(GemmRecorder* r) : prev(active_gemm_recorder()) {
+                if (r) set_active_gemm_recorder(r);
+            }
+            ~RecGuard() { set_active_gemm_recorder(prev); }
+        } rguard(gemm_record ? &gemm_log : nullptr);
+        gemm_log.log.clear();
+        gemm_log.calls = 0;
         d_status.reserve(1);
         fill_int(d_status.p, 1, 0, stream);
         // Initialize (src/Factor.cpp:22-34)
@@ -1696,6 +1707,9 @@ struct hifde_factor {
     // counts these).  A failure that is not about the capture recurs in the
     // eager run that follows and is reported from there.
     int graph_failures = 0;
+    bool gemm_record = false;  // log every GEMM descriptor of a factorization
+    GemmRecorder gemm_log;
+    bool gemm_table_on = true;  // route GEMMs by the tuning table where it has an entry
     bool capture_graph(ApplyGraph* g, double* d_u, int R, long long ld, bool inv) {
         if (graph_failures > 0) return false;
         hipGraph_t graph = nullptr;
@@ -1815,6 +1829,8 @@ struct hifde_factor {
                 g.ldc = m;
                 d.push_back(g);
             }
+            pl.esize = (int)sizeof(T);
+            pl.use_table = gemm_table_on;
             pl.add(d, op == 0 || op == 2);
             pl.upload(stream);
             B.rplan_R[op] = R;
@@ -2505,6 +2521,154 @@ extern "C" {
 
 const char* hifde_last_error(void) { return g_last_error.c_str(); }
 
+namespace {
+// Parses a routing table written by the tuner: one entry per line,
+// "esize transA lower kbin mnbin nbbin kind", '#' starts a comment.
+int load_tune_table(const char* path) {
+    FILE* fp = fopen(path, "r");
+    if (!fp) return -1;
+    std::vector<signed char> t(kTuneEntries, -1);
+    char line[512];
+    int n = 0;
+    while (fgets(line, sizeof line, fp)) {
+        if (char* c = strchr(line, '#')) *c = 0;
+        int v[7];
+        if (sscanf(line, "%d %d %d %d %d %d %d", v, v + 1, v + 2, v + 3, v + 4, v + 5, v + 6) !=
+            7)
+            continue;
+        if ((v[0] != 4 && v[0] != 8) || v[1] < 0 || v[1] > 1 || v[2] < 0 || v[2] > 1 ||
+            v[3] < 0 || v[3] >= kTuneKBins || v[4] < 0 || v[4] >= kTuneMNBins || v[5] < 0 ||
+            v[5] >= kTuneNBBins || v[6] < 0 || v[6] > 2) {
+            fclose(fp);
+            throw Error(HIFDE_ERR_ARG, std::string("gemm table: bad entry in ") + path);
+        }
+        const int key =
+            (((((v[0] == 4 ? 0 : 1) * 2 + v[1]) * 2 + v[2]) * kTuneKBins + v[3]) * kTuneMNBins +
+             v[4]) * kTuneNBBins + v[5];
+        t[key] = (signed char)v[6];
+        ++n;
+    }
+    fclose(fp);
+    gemm_tune_install(t.data());
+    return n;
+}
+// The table shipped next to the library (gemm_tune.txt), loaded once when
+// the first factor object is created.
+std::once_flag g_table_once;
+std::string default_table_path() {
+    Dl_info info;
+    if (!dladdr(reinterpret_cast<void*>(&load_tune_table), &info) || !info.dli_fname) return "";
+    std::string p = info.dli_fname;
+    const size_t slash = p.rfind('/');
+    return (slash == std::string::npos ? std::string(".") : p.substr(0, slash)) +
+           "/gemm_tune.txt";
+}
+}  // namespace
+
+int hifde_gemm_table_load(const char* path) {
+    return guarded([&]() -> int {
+        const std::string p = path ? std::string(path) : default_table_path();
+        const int n = load_tune_table(p.c_str());
+        if (n < 0) throw Error(HIFDE_ERR_ARG, "gemm_table_load: cannot read " + p);
+        return n;
+    });
+}
+
+int hifde_gemm_table_clear(void) {
+    std::vector<signed char> none(kTuneEntries, -1);
+    gemm_tune_install(none.data());
+    return HIFDE_OK;
+}
+
+long long hifde_gemm_log(const hifde_factor* f, int* out, long long cap) {
+    if (!f) return HIFDE_ERR_ARG;
+    const std::vector<int>& v = f->gemm_log.log;
+    if (out) std::copy(v.begin(), v.begin() + std::min<long long>(cap, (long long)v.size()), out);
+    return (long long)v.size();
+}
+
+int hifde_gemm_time(int precision, int transA, int lower, int nbatch, const int* m, const int* n,
+                    const int* k, int kind, int reps, double* avg_seconds) {
+    return guarded([&]() -> int {
+        if (nbatch <= 0 || !m || !n || !k || kind < 0 || kind > 2 || reps < 1 || !avg_seconds)
+            throw Error(HIFDE_ERR_ARG, "gemm_time: bad arguments");
+        if (precision != HIFDE_FP64 && precision != HIFDE_MIXED)
+            throw Error(HIFDE_ERR_ARG, "gemm_time: precision");
+        hipStream_t s = aux_stream();
+        OwnerGuard og(s);
+        std::vector<GemmDesc> d(nbatch);
+        long long oa = 0, ob = 0, oc = 0;
+        for (int b = 0; b < nbatch; ++b) {
+            if (m[b] < 1 || n[b] < 1 || k[b] < 1) throw Error(HIFDE_ERR_ARG, "gemm_time: bad sizes");
+            GemmDesc& g = d[b];
+            g.offA = oa;
+            g.offB = ob;
+            g.offC = oc;
+            g.m = m[b];
+            g.n = n[b];
+            g.k = k[b];
+            g.lda = transA ? k[b] : m[b];
+            g.ldb = k[b];
+            g.ldc = m[b];
+            g.lower = lower ? 1 : 0;
+            oa += (long long)m[b] * k[b];
+            ob += (long long)k[b] * n[b];
+            oc += (long long)m[b] * n[b];
+        }
+        const size_t es = precision == HIFDE_MIXED ? 4 : 8;
+        DevBuf<unsigned char> A, B, C;
+        A.reserve(oa * es);
+        B.reserve(ob * es);
+        C.reserve(oc * es);
+        // operands of values in [-1, 1] (timing does not depend on them)
+        if (es == 8) {
+            gen_uniform(reinterpret_cast<double*>(A.p), oa, 11, -1.0, 1.0, false, s);
+            gen_uniform(reinterpret_cast<double*>(B.p), ob, 12, -1.0, 1.0, false, s);
+        } else {
+            DevBuf<double> tmp;
+            tmp.reserve(std::max(oa, ob));
+            gen_uniform(tmp.p, oa, 11, -1.0, 1.0, false, s);
+            convert_d2f(tmp.p, reinterpret_cast<float*>(A.p), oa, s);
+            gen_uniform(tmp.p, ob, 12, -1.0, 1.0, false, s);
+            convert_d2f(tmp.p, reinterpret_cast<float*>(B.p), ob, s);
+            HIFDE_CHECK(hipStreamSynchronize(s));
+        }
+        GemmWork ws;
+        ws.plan.force_kind = kind;
+        ws.plan.use_table = false;
+        hipEvent_t e0, e1;
+        HIFDE_CHECK(hipEventCreate(&e0));
+        HIFDE_CHECK(hipEventCreate(&e1));
+        auto run = [&]() {
+            if (es == 4)
+                batched_gemm_desc<float>(transA != 0, d, reinterpret_cast<const float*>(A.p),
+                                         reinterpret_cast<const float*>(B.p),
+                                         reinterpret_cast<float*>(C.p), 1.0f, 0.0f, ws, s);
+            else
+                batched_gemm_desc<double>(transA != 0, d, reinterpret_cast<const double*>(A.p),
+                                          reinterpret_cast<const double*>(B.p),
+                                          reinterpret_cast<double*>(C.p), 1.0, 0.0, ws, s);
+        };
+        float ms = 0.f;
+        try {
+            run();  // warm-up (and plan upload)
+            HIFDE_CHECK(hipEventRecord(e0, s));
+            for (int r = 0; r < reps; ++r) run();
+            HIFDE_CHECK(hipEventRecord(e1, s));
+            HIFDE_CHECK(hipEventSynchronize(e1));
+            HIFDE_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        } catch (...) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *avg_seconds = ms * 1e-3 / reps;
+        return HIFDE_OK;
+    });
+}
+
 int hifde_create(int N, int P, double epsilon, int precision, int flags, hifde_factor** out) {
     return guarded([&]() -> int {
         if (!out || N < 1 || P < 2 || !(epsilon > 0.0))
@@ -2516,6 +2680,10 @@ int hifde_create(int N, int P, double epsilon, int precision, int flags, hifde_f
         if (q < 1 || (q & (q - 1)) != 0 || NC % P != 0)
             throw Error(HIFDE_ERR_ARG, "create: (N+1)/P must be a power of two");
         if (NC >= 1024) throw Error(HIFDE_ERR_SIZE, "create: N + 1 must be below 1024");
+        std::call_once(g_table_once, [] {
+            const std::string p = default_table_path();
+            if (!p.empty()) (void)load_tune_table(p.c_str());
+        });
         std::unique_ptr<hifde_factor> f(new hifde_factor());
         f->N = N;
         f->P = P;
@@ -2833,6 +3001,16 @@ int hifde_set_option(hifde_factor* f, const char* name, double value) {
             f->gemm_ws.plan.tile64 = f->gj_ws.plan.tile64 = (int)value;
         }
         else if (n == "host_threads") f->host_threads = std::max(0, (int)value);
+        else if (n == "gemm_record") f->gemm_record = value != 0;
+        else if (n == "gemm_table") {
+            f->gemm_table_on = value != 0;
+            f->gemm_ws.plan.use_table = f->gj_ws.plan.use_table = f->gram_ws.plan.use_table =
+                f->gemm_table_on;
+            for (auto* v : {&f->cells, &f->faces})
+                for (BoxBatch& B : *v)
+                    for (int op = 0; op < 5; ++op) B.rplan_R[op] = 0;  // re-planned
+            f->clear_graphs();
+        }
         else if (n == "keep_workspaces") {
             f->keep_workspaces = value != 0;
             if (!f->keep_workspaces && f->factored) f->release_workspaces();
@@ -3068,7 +3246,7 @@ int schur_batch_impl(int nb, const int* r, const int* s, const double* A22, cons
 namespace {
 template <class T>
 int gemm_batch_impl(int transA, int nb, const int* m, const int* n, const int* k, const double* A,
-                    const double* B, double* C, bool split, int tile64) {
+                    const double* B, double* C, bool split, int tile64, int force) {
     hipStream_t st = aux_stream();
     OwnerGuard og(st);
     std::vector<long long> oA(nb + 1, 0), oB(nb + 1, 0), oC(nb + 1, 0);
@@ -3089,6 +3267,7 @@ int gemm_batch_impl(int transA, int nb, const int* m, const int* n, const int* k
     GemmWork gws;
     gws.plan.split = split;
     gws.plan.tile64 = tile64;
+    gws.plan.force_kind = force;
     batched_gemm<T>(transA != 0, nb, m, n, k, oA.data(), oB.data(), oC.data(), dA.p, dB.p, dC.p,
                     (T)1, gws, st);
     std::vector<T> tmp(oC[nb]);
@@ -3111,12 +3290,17 @@ int hifde_gemm_batch(int precision, int transA, int nbatch, const int* m, const 
         if (nbatch == 0) return HIFDE_OK;
         const bool split = (precision & HIFDE_BF16X3) != 0;
         const int tile64 = (precision & HIFDE_GEMM_TILE64) ? 3 : 0;
-        const int prec = precision & ~(HIFDE_BF16X3 | HIFDE_GEMM_TILE64);
+        const int force = (precision & HIFDE_GEMM_FORCE_FMA)      ? 0
+                          : (precision & HIFDE_GEMM_FORCE_CORE128) ? 1
+                          : (precision & HIFDE_GEMM_FORCE_CORE64)  ? 2
+                                                                   : -1;
+        const int prec = precision & ~(HIFDE_BF16X3 | HIFDE_GEMM_TILE64 | HIFDE_GEMM_FORCE_FMA |
+                                       HIFDE_GEMM_FORCE_CORE128 | HIFDE_GEMM_FORCE_CORE64);
         if (split && prec != HIFDE_MIXED)
             throw Error(HIFDE_ERR_ARG, "gemm_batch: HIFDE_BF16X3 needs HIFDE_MIXED");
         if (prec == HIFDE_MIXED)
-            return gemm_batch_impl<float>(transA, nbatch, m, n, k, A, B, C, split, tile64);
-        return gemm_batch_impl<double>(transA, nbatch, m, n, k, A, B, C, false, tile64);
+            return gemm_batch_impl<float>(transA, nbatch, m, n, k, A, B, C, split, tile64, force);
+        return gemm_batch_impl<double>(transA, nbatch, m, n, k, A, B, C, false, tile64, force);
     });
 }
 

@@ -164,6 +164,15 @@ struct GemmPlan {
     // of at least 32; 3 every product with both output sides of at least 32.
     // clear() leaves it as it is.
     int tile64 = 0;
+    // Bytes per operand element of the products added (4 or 8): part of the
+    // tuning-table key; set by the caller before add().
+    int esize = 8;
+    // Route by the process-wide tuning table (gemm_table_*) where it has an
+    // entry for a descriptor's class; otherwise by the fixed rule.
+    bool use_table = true;
+    // >= 0: every descriptor that kernel kind can run goes to it (timing
+    // replays of the tuner); -1: normal routing.
+    int force_kind = -1;
     std::vector<GemmDesc> desc;
     std::vector<int> tiles;  // per launch: nb + 1 local tile offsets
     std::vector<Launch> launches;
@@ -306,6 +315,26 @@ struct Copy2D {
 template <class T, class TC = T>
 void gemm_run(const GemmPlan& p, std::pair<int, int> h, const T* d_A, const T* d_B, TC* d_C,
               T alpha, T beta, hipStream_t s, const int* d_flags = nullptr);
+
+// GEMM routing table written by the project's tuner (hifde3d_amd/tune.py):
+// per descriptor class -- element size, transA, lower, depth bin, bin of the
+// smaller output side, bin of the call's descriptor count -- the kernel kind
+// that ran its recorded products fastest (0 FMA 64 x 64, 1 matrix-core
+// 128 x 128, 2 matrix-core 64 x 64), or -1 (no entry: fixed rule).
+constexpr int kTuneKBins = 8, kTuneMNBins = 6, kTuneNBBins = 5;
+constexpr int kTuneEntries = 2 * 2 * 2 * kTuneKBins * kTuneMNBins * kTuneNBBins;
+int gemm_tune_key(int esize, int transA, int lower, int m, int n, int k, int nb);
+int gemm_tune_lookup(int key);  // -1 when absent
+void gemm_tune_install(const signed char* kinds);  // kTuneEntries kinds
+// Every descriptor added while a recorder is active on this thread is
+// logged: call id, esize, transA, lower, m, n, k, descriptors in the call,
+// routed kind.
+struct GemmRecorder {
+    std::vector<int> log;  // 9 ints per descriptor
+    int calls = 0;
+};
+GemmRecorder* active_gemm_recorder();
+void set_active_gemm_recorder(GemmRecorder* r);
 
 struct GemmWork {
     GemmPlan plan;

@@ -321,6 +321,15 @@ int hifde_triangular_solve_batch(int nbatch, const int* m, const int* n, const d
  * behind the factorization's option "gemm_tile64").  No counterpart in the
  * reference. */
 #define HIFDE_GEMM_TILE64 0x400
+/* OR-ed into the precision of hifde_gemm_batch: run every product on one
+ * kernel, the 64 x 64 FMA tiles (FMA), the 128 x 128 matrix-core tiles
+ * (CORE128) or the 64 x 64 matrix-core tiles (CORE64); a product the
+ * matrix-core kernel is not validated for (an output side below 32, depth 0)
+ * stays on the FMA kernel.  The candidates of the GEMM tuner; no counterpart
+ * in the reference. */
+#define HIFDE_GEMM_FORCE_FMA 0x800
+#define HIFDE_GEMM_FORCE_CORE128 0x1000
+#define HIFDE_GEMM_FORCE_CORE64 0x2000
 int hifde_schur_batch(int precision, int nbatch, const int* r, const int* s, const double* A22,
                       const double* A21, const double* A12, double* A22inv, double* X, double* S,
                       int* info);
@@ -333,6 +342,32 @@ int hifde_schur_batch(int precision, int nbatch, const int* r, const int* s, con
  * Schur and Gram products use). */
 int hifde_gemm_batch(int precision, int transA, int nbatch, const int* m, const int* n,
                      const int* k, const double* A, const double* B, double* C);
+
+/* GEMM routing tuner (no reference counterpart; the reference calls BLAS).
+ * Every batched product of the factorization is routed per descriptor to
+ * one of three kernels -- 0: FMA 64 x 64 tiles, 1: matrix-core 128 x 128,
+ * 2: matrix-core 64 x 64 -- by a process-wide table indexed by the
+ * descriptor's class (element size, transA, lower, depth bin, bin of the
+ * smaller output side, bin of the number of descriptors in the call), where
+ * the table has an entry, and by the fixed rule otherwise.  The table is
+ * written by hifde3d_amd/tune.py, which times the candidates on the
+ * products the workloads record; the one shipped next to the library
+ * (gemm_tune.txt) is loaded when the first factor object is created.
+ *   hifde_gemm_table_load: load a table file (NULL: the shipped one);
+ *     returns the number of entries.  hifde_gemm_table_clear: fixed rule
+ *     everywhere.  Option "gemm_table" = 0 makes one object ignore it.
+ *   hifde_gemm_log: with option "gemm_record" = 1, the descriptors of the
+ *     last factorization, 9 ints each (call, esize, transA, lower, m, n, k,
+ *     descriptors in the call, kind routed); returns the int count (copies
+ *     at most cap).
+ *   hifde_gemm_time: average device seconds over reps launches of a batch
+ *     of packed products (m[b] x n[b], depth k[b]; lower: lower-triangle
+ *     tiles only) forced onto one kind, operands generated on the GPU. */
+int hifde_gemm_table_load(const char* path);
+int hifde_gemm_table_clear(void);
+long long hifde_gemm_log(const hifde_factor* f, int* out, long long cap);
+int hifde_gemm_time(int precision, int transA, int lower, int nbatch, const int* m, const int* n,
+                    const int* k, int kind, int reps, double* avg_seconds);
 
 #ifdef __cplusplus
 }

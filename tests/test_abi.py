@@ -66,3 +66,25 @@ def test_cpp_dropin_header_compiles_and_links():
         out = subprocess.run(["ldd", drv], capture_output=True, text=True).stdout
         assert "libhifde_amd.so" in out
         assert "not found" not in out.split("libhifde_amd.so")[1].split("\n")[0]
+
+
+def test_shipped_gemm_table_is_well_formed():
+    """hifde3d_amd/gemm_tune.txt (written by hifde3d_amd/tune.py): every entry
+    is a valid class and kind, each class appears once, and the library
+    accepts the file (parsing needs no GPU)."""
+    path = os.path.join(os.path.dirname(hifde3d_amd.LIB_PATH), "gemm_tune.txt")
+    assert os.path.exists(path)
+    seen = set()
+    for line in open(path):
+        line = line.split("#")[0].split()
+        if not line:
+            continue
+        es, ta, lo, kb, mb, nb, kind = map(int, line)
+        assert es in (4, 8) and ta in (0, 1) and lo in (0, 1)
+        assert 0 <= kb < 8 and 0 <= mb < 6 and 0 <= nb < 5 and kind in (0, 1, 2)
+        assert (es, ta, lo, kb, mb, nb) not in seen
+        seen.add((es, ta, lo, kb, mb, nb))
+    assert len(seen) > 50
+    L = hifde3d_amd.lib()
+    assert L.hifde_gemm_table_load(path.encode()) == len(seen)
+    assert L.hifde_gemm_table_load(b"/nonexistent/table.txt") == -1

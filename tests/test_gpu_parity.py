@@ -1054,3 +1054,57 @@ def test_pivoted_qr_and_triangular_solve_batches_match_lapack(H, oracle_mod):
     for (R, B), X in zip(pairs, H.triangular_solve_batch(pairs)):
         if B.size:
             assert np.allclose(X, sla.solve_triangular(R, B), rtol=1e-11, atol=1e-11)
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 1e-5)])
+@pytest.mark.parametrize("transA", [False, True])
+@pytest.mark.parametrize("kind", [0, 1, 2])
+def test_gemm_forced_kernels_match_numpy(H, precision, tol, transA, kind):
+    """Each candidate kernel of the GEMM tuner (forced for every product it
+    can run) on ragged shapes, against NumPy."""
+    rng = _rng(31 + kind)
+    shapes = [(32, 32, 1), (33, 40, 17), (64, 64, 64), (130, 70, 65), (257, 129, 300),
+              (40, 300, 8), (31, 50, 9)]
+    pairs = []
+    for m, n, k in shapes:
+        A = rng.standard_normal((k, m) if transA else (m, k))
+        pairs.append((A, rng.standard_normal((k, n))))
+    out = H.gemm_batch(pairs, precision, transA=transA, kind=kind)
+    for (A, B), C in zip(pairs, out):
+        ref = (A.T if transA else A) @ B
+        assert _rel(C, ref) < tol
+
+
+def test_gemm_tuner_table_loads_and_routing_keeps_the_factor(H, oracle_mod, tmp_path):
+    """The shipped routing table parses; a factorization routed by it and one
+    routed by the fixed rule give the same level counts and preconditioners
+    of the same quality (the kernels differ only in summation order)."""
+    import os
+    import torch
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    table = os.path.join(root, "hifde3d_amd", "gemm_tune.txt")
+    if os.path.exists(table):
+        assert H.gemm_table_load(table) > 0
+    n = 64
+    a, V, b = pb.lognormal_coeff(n, 2), pb.zeros_coeff(n), pb.rhs_normal(n, 2)
+    res = {}
+    for on in (1, 0):
+        f = H.HIFFactor(n - 1, 4, 1e-4, precision="mixed")
+        f.set_option("gemm_table", on)
+        f.set_option("gemm_record", 1)
+        f.set_stencil(a, V, True)
+        assert f.Factor() == 0
+        log = f.gemm_log()
+        db = torch.from_numpy(b).cuda()
+        dx = torch.zeros_like(db)
+        it, rr = f.pcg_device(db.data_ptr(), dx.data_ptr(), tol=1e-10, maxit=200)
+        res[on] = (f.level_counts(), it, rr, dx.cpu().numpy(), log)
+        f.close()
+    c1, c0 = res[1][0], res[0][0]
+    for lv in range(len(c0)):
+        assert c1[lv][0] == c0[lv][0] and c1[lv][2] == c0[lv][2]
+        for idx in (1, 3, 4):
+            assert abs(c1[lv][idx] - c0[lv][idx]) <= 0.01 * max(c0[lv][idx], 100)
+    assert abs(res[1][1] - res[0][1]) <= 1 and res[1][2] <= 1e-10
+    assert _rel(res[1][3], res[0][3]) < 1e-8
+    assert len(res[0][4]) > 0 and res[0][4].shape[1] == 9
