@@ -119,29 +119,67 @@ def copy_calibration():
     print(json.dumps({"copy_calibration_bytes": x.numel() * 8, "copies": 3}))
 
 
-def cpu_baseline(n_sample, n_target, seed, corr, eps, P, tol):
-    """Oracle (single-threaded CPU restatement) factor + PCG on a smaller grid
-    of the same workload family, scaled to the target grid by DOF count."""
-    import scipy.sparse as sp
+def _oracle_boxes(o):
+    """Per level and kind (0 cells, 1 faces): list of (r, s) of the oracle's boxes."""
+    import ctypes
     from oracle import hifo
-    a, V, b = make_problem(n_sample, seed, corr)
-    N = n_sample - 1
-    r, c, v = hifo.stencil_coo(N, a, V, True)
-    A = sp.csr_matrix((v, (r, c)), shape=(n_sample ** 3,) * 2)
-    o = hifo.Oracle(N, P, eps)
-    o.set_matrix_coo(r, c, v)
-    t0 = time.perf_counter()
-    o.factor(1)
-    t_f = time.perf_counter() - t0
-    t0 = time.perf_counter()
+    L = hifo.lib()
+    sz = np.zeros(4, np.int32)
+    out = []
+    for lv in range(o.num_levels):
+        per = []
+        for kind in (0, 1):
+            nb = L.hifo_num_boxes(o._h, lv, kind)
+            boxes = []
+            for b in range(max(nb, 0)):
+                L.hifo_box_sizes(o._h, lv, kind, b, sz.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+                boxes.append((int(sz[1]), int(sz[2])))
+            per.append(boxes)
+        out.append(per)
+    return out
+
+
+def _gpu_boxes(f):
+    """Per level and kind: list of (r, s) of a factor object's boxes."""
+    import ctypes
+    import hifde3d_amd as H
+    L = H.lib()
+    sz = np.zeros(2, np.int32)
+    out = []
+    for lv in range(f.num_levels):
+        per = []
+        for kind in (0, 1):
+            boxes = []
+            for b in range(f.num_boxes(lv, kind)):
+                L.hifde_box_sizes(f._h, lv, kind, b, sz.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+                boxes.append((int(sz[0]), int(sz[1])))
+            per.append(boxes)
+        out.append(per)
+    return out
+
+
+def _work(boxes):
+    """Dense-work proxies of one level: cells sum(r^3 + r^2 s + r s^2) (LU,
+    explicit inverse, X and S), faces sum(n^3) with n = r + s (pivoted QR and
+    the transform), the level-matrix update sum(s^2) (entries added); and the
+    apply's sum(r^2 + 2 r s) (A22^{-1} once, X or W twice)."""
+    cells, faces = boxes
+    wc = sum(r ** 3 + r * r * s + r * s * s for r, s in cells)
+    wf = sum((r + s) ** 3 for r, s in faces)
+    wu = sum(s * s for r, s in cells) + sum((r + s) ** 2 for r, s in faces)
+    wa = sum(r * r + 2 * r * s for r, s in cells + faces)
+    return wc, wf, wu, wa, len(cells), len(faces)
+
+
+def _oracle_pcg(o, A, b, tol, maxit=300):
     x = np.zeros_like(b)
-    rr = b - A @ x
+    rr = b.copy()
     z = o.apply(rr, True)
     p = z.copy()
     rz = rr @ z
     bn = np.linalg.norm(b)
     it = 0
-    while np.linalg.norm(rr) / bn > tol and it < 200:
+    while np.linalg.norm(rr) / bn > tol and it < maxit:
         it += 1
         q = A @ p
         al = rz / (p @ q)
@@ -153,68 +191,220 @@ def cpu_baseline(n_sample, n_target, seed, corr, eps, P, tol):
         rz2 = rr @ z
         p = z + (rz2 / rz) * p
         rz = rz2
+    return it
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_run(n, seed, corr, eps, P, tol, threads):
+    """The oracle (FP64 CPU restatement, its own dense kernels, OpenMP over
+    the boxes of a level) on the same workload family at grid n: factor and
+    oracle-preconditioned CG to tol, with per-level times and box sizes."""
+    import scipy.sparse as sp
+    from oracle import hifo
+    a, V, b = make_problem(n, seed, corr)
+    N = n - 1
+    r, c, v = hifo.stencil_coo(N, a, V, True)
+    A = sp.csr_matrix((v, (r, c)), shape=(n ** 3,) * 2)
+    o = hifo.Oracle(N, P, eps)
+    o.set_matrix_coo(r, c, v)
+    t0 = time.perf_counter()
+    o.factor(threads)
+    t_f = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    it = _oracle_pcg(o, A, b, tol)
     t_s = time.perf_counter() - t0
-    scale = (n_target / n_sample) ** 3
-    step = (t_f + t_s) * scale
-    return {"value": 1.0 / step, "unit": "factor+solve/s", "cores": 1, "kind": "port",
-            "sample": (f"oracle (FP64 CPU restatement, 1 thread) factor + PCG to {tol:g} at "
-                       f"n={n_sample} ({(n_sample - 1) ** 3} unknowns, same coefficient family, "
-                       f"P={P}, eps={eps:g}): factor {t_f:.2f}s, solve {t_s:.2f}s ({it} it); "
-                       f"scaled by the DOF ratio {scale:.0f} to n={n_target}"),
-            "factor_s_scaled": t_f * scale, "solve_s_scaled": t_s * scale}
+    return {"n": n, "threads": threads, "factor_s": t_f, "solve_s": t_s, "pcg_iters": it,
+            "level_s": [list(t) for t in o.level_times()], "boxes": _oracle_boxes(o)}
+
+
+def estimate_from(run, target_boxes, target_iters, threads):
+    """Level-by-level estimate of the oracle's factor and solve time on the
+    target grid from a measured run: each target level takes the times of
+    the sample level with the same box width (the sample's top level for the
+    levels above it), scaled by the ratio of dense-work proxies and of the
+    boxes that run in parallel (OpenMP over boxes: min(boxes, threads)); the
+    level-matrix update is serial.  The solve is the target's PCG iteration
+    count times the sample's time per iteration scaled by the apply's work."""
+    sb = run["boxes"]
+    Ls = len(sb)
+    sw = [_work(b_) for b_ in sb]
+
+    def src(lt, idx):
+        # the sample level of the same box width, or the highest below it
+        # that has work of this kind (the root has no faces and no update)
+        for ls in range(min(lt, Ls - 1), -1, -1):
+            if sw[ls][idx] > 0:
+                return ls
+        return None
+
+    est_levels = []
+    for lt, tb in enumerate(target_boxes):
+        wc_t, wf_t, wu_t, _, nc_t, nf_t = _work(tb)
+        e = 0.0
+        ls = src(lt, 0)
+        if wc_t and ls is not None:
+            e += run["level_s"][ls][0] * (wc_t / sw[ls][0]) * \
+                (min(sw[ls][4], threads) / max(min(nc_t, threads), 1))
+        ls = src(lt, 1)
+        if wf_t and ls is not None:
+            e += run["level_s"][ls][1] * (wf_t / sw[ls][1]) * \
+                (min(sw[ls][5], threads) / max(min(nf_t, threads), 1))
+        ls = src(lt, 2)
+        if wu_t and ls is not None:
+            e += run["level_s"][ls][2] * (wu_t / sw[ls][2])
+        est_levels.append(e)
+    wa_t = sum(_work(tb)[3] for tb in target_boxes)
+    wa_s = sum(_work(b_)[3] for b_ in sb)
+    per_it = run["solve_s"] / max(run["pcg_iters"], 1)
+    est_solve = target_iters * per_it * wa_t / max(wa_s, 1)
+    return sum(est_levels), est_solve, est_levels
+
+
+def cpu_baseline(args, target_boxes, target_iters, n_target, seed, corr, eps, P, tol):
+    """The CPU baseline: the oracle timed on this host on a bounded sample of
+    the workload (grid --cpu-n, one thread and all cores), and the n_target
+    time estimated level by level from each run (estimate_from)."""
+    threads_all = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+    runs = [cpu_run(args.cpu_n, seed, corr, eps, P, tol, 1)]
+    if threads_all > 1:
+        runs.append(cpu_run(args.cpu_n, seed, corr, eps, P, tol, threads_all))
+    out_runs = []
+    for r_ in runs:
+        ef, es, el = estimate_from(r_, target_boxes, target_iters, r_["threads"])
+        out_runs.append({"n": r_["n"], "threads": r_["threads"], "factor_s": r_["factor_s"],
+                         "solve_s": r_["solve_s"], "pcg_iters": r_["pcg_iters"],
+                         "level_s": r_["level_s"],
+                         "estimate_n_target": {"factor_s": ef, "solve_s": es,
+                                               "level_factor_s": el}})
+    best = min(out_runs, key=lambda x: x["estimate_n_target"]["factor_s"] +
+               x["estimate_n_target"]["solve_s"])
+    e = best["estimate_n_target"]
+    step = e["factor_s"] + e["solve_s"]
+    return {"value": 1.0 / step, "unit": "factor+solve/s", "cores": best["threads"],
+            "kind": "port",
+            "sample": (f"oracle (FP64 CPU restatement with its own dense kernels, OpenMP over "
+                       f"the boxes of a level) on this host ({_cpu_model()}), measured at "
+                       f"n={args.cpu_n} with 1 and {threads_all} threads: factor + "
+                       f"oracle-preconditioned CG to {tol:g}; value is the n={n_target} "
+                       f"ESTIMATE built level by level from the {best['threads']}-thread run "
+                       f"(per-level times scaled by dense-work proxies of the measured box "
+                       f"sizes and by box parallelism; solve = {target_iters} iterations x the "
+                       f"sample's time per iteration scaled by the apply's work)"),
+            "cpu_model": _cpu_model(), "threads_all": threads_all,
+            "estimate": True, "runs": out_runs}
+
+
+def cpu_validate(args):
+    """Checks the level-by-level estimate against a measurement: the oracle
+    at grid --cpu-n and at grid --cpu-validate (all cores), the larger time
+    estimated from the smaller run with the larger run's own box sizes."""
+    seed, corr = 2, 0.1
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+    small = cpu_run(args.cpu_n, seed, corr, args.eps, args.P, args.tol, threads)
+    big = cpu_run(args.cpu_validate, seed, corr, args.eps, args.P, args.tol, threads)
+    ef, es, el = estimate_from(small, big["boxes"], big["pcg_iters"], threads)
+    out = {"cpu_model": _cpu_model(), "threads": threads,
+           "sample": {k: small[k] for k in ("n", "factor_s", "solve_s", "pcg_iters", "level_s")},
+           "measured": {k: big[k] for k in ("n", "factor_s", "solve_s", "pcg_iters", "level_s")},
+           "estimate": {"factor_s": ef, "solve_s": es, "level_factor_s": el},
+           "ratio_estimate_over_measured": {"factor": ef / big["factor_s"],
+                                            "solve": es / big["solve_s"]}}
+    print(json.dumps(out), flush=True)
 
 
 def batch_workload(args):
     """48 independent problems, NC drawn from {24,...,96} (seed 3), a = 10^s with
     s ~ U[-2, 2], 16 right-hand sides each: mixed-precision factor and a block
-    of 16 PCG solves to 1e-10 per problem.  --batch-threads host threads each
-    take a share of the problems (largest first, to the least loaded thread)
-    and run them one after another; every factor object has its own stream,
-    so the shares overlap on the GPU.  Each problem's operator is assembled
-    and its right-hand sides are resident in HBM before the timed region; a
-    step is one pass over all 48 problems."""
+    of 16 PCG solves to 1e-10 per problem.
+
+    --batch-mode packed (default): the problems of each grid size are packed,
+    up to eight at a time, as the octants of one factor object
+    (hifde_create_batch), so each level of their factorizations runs in shared
+    launches, and their 16 right-hand sides each go through one block of
+    PCG solves with per-(problem, column) recurrences.  --batch-mode separate:
+    one factor object per problem.  Either way --batch-threads host threads
+    take the objects (largest first, to the least loaded thread), each object
+    on its own stream.  Coefficients, operators and right-hand sides are
+    generated and assembled on the GPU before the timed region; a step is one
+    pass over all 48 problems."""
     import threading
     import torch
     import hifde3d_amd as H
     from hifde3d_amd import problems as pb
 
     inst = pb.batch_instances()
-    nthr = max(1, args.batch_threads)
+    R = 16
+    units = []  # (object, B, X, nprob, NC)
+    if args.batch_mode == "packed":
+        by_nc = {}
+        for p in inst:
+            by_nc.setdefault(p["NC"], []).append(p)
+        for NC, ps in sorted(by_nc.items()):
+            for c0 in range(0, len(ps), 8):
+                chunk = ps[c0:c0 + 8]
+                coeffs = [H.device_problem(NC, "pow10", p["coeff_seed"], lo=-2.0, hi=2.0)
+                          for p in chunk]
+                f = H.HIFBatch(NC - 1, chunk[0]["P"], 1e-4, len(chunk), precision="mixed")
+                f.set_stencil_batch_device([a.data_ptr() for a, _ in coeffs],
+                                           [V.data_ptr() for _, V in coeffs], True)
+                del coeffs
+                Bs = torch.empty((len(chunk) * R, NC ** 3), dtype=torch.float64, device="cuda")
+                for o, p in enumerate(chunk):
+                    H.gen_rhs_normal(Bs[o * R].data_ptr(), NC, p["rhs_seed"], nrhs=R)
+                Bp = torch.empty((R, f.size), dtype=torch.float64, device="cuda")
+                f.pack(Bs.data_ptr(), Bp.data_ptr(), R)
+                units.append((f, Bp, torch.zeros_like(Bp), len(chunk), NC))
+    else:
+        for p in inst:
+            NC = p["NC"]
+            a, V = H.device_problem(NC, "pow10", p["coeff_seed"], lo=-2.0, hi=2.0)
+            f = H.HIFFactor(NC - 1, p["P"], 1e-4, precision="mixed")
+            f.set_stencil_device(a.data_ptr(), V.data_ptr(), True)
+            B = torch.empty((R, NC ** 3), dtype=torch.float64, device="cuda")
+            H.gen_rhs_normal(B.data_ptr(), NC, p["rhs_seed"], nrhs=R)
+            units.append((f, B, torch.zeros_like(B), 1, NC))
+    nthr = max(1, min(args.batch_threads, len(units)))
+    for f, *_ in units:
+        f.set_option("host_threads", max(1, 16 // nthr))
     groups = [[] for _ in range(nthr)]
     load = [0] * nthr
-    for i in sorted(range(len(inst)), key=lambda i: -inst[i]["NC"] ** 3):
+    for i in sorted(range(len(units)), key=lambda i: -units[i][3] * units[i][4] ** 3):
         g = load.index(min(load))
         groups[g].append(i)
-        load[g] += inst[i]["NC"] ** 3
-    facs, rhs, xs = [], [], []
-    for p in inst:
-        f = H.HIFFactor(p["NC"] - 1, p["P"], 1e-4, precision="mixed")
-        f.set_option("host_threads", max(1, 16 // nthr))
-        f.set_stencil(pb.pow10_coeff(p["NC"], p["coeff_seed"], -2.0, 2.0), pb.zeros_coeff(p["NC"]),
-                      symmetric=True)
-        facs.append(f)
-        rhs.append(torch.from_numpy(pb.rhs_normal(p["NC"], p["rhs_seed"], 16)).cuda())
-        xs.append(torch.zeros_like(rhs[-1]))
+        load[g] += units[i][3] * units[i][4] ** 3
     torch.cuda.synchronize()
+    streams = [torch.cuda.Stream() for _ in range(nthr)]
 
-    def run_group(idx, out):
-        for i in idx:
-            f, b, x = facs[i], rhs[i], xs[i]
-            f.Factor()
-            x.zero_()
-            it, rr = f.pcg_multi_device(b.data_ptr(), x.data_ptr(), 16, tol=args.tol, maxit=500)
-            out.append((float(rr.max()), int(it.max())))
+    def run_group(gi, out):
+        with torch.cuda.stream(streams[gi]):
+            for i in groups[gi]:
+                f, b, x, _, _ = units[i]
+                f.Factor()
+                x.zero_()
+                it, rr = f.pcg_multi_device(b.data_ptr(), x.data_ptr(), R, tol=args.tol,
+                                            maxit=500)
+                out.append((float(np.max(rr)), int(np.max(it))))
+            streams[gi].synchronize()
 
     def one_pass():
         outs = [[] for _ in groups]
-        th = [threading.Thread(target=run_group, args=(g, o)) for g, o in zip(groups, outs)]
+        th = [threading.Thread(target=run_group, args=(g, outs[g])) for g in range(nthr)]
         for t in th:
             t.start()
         for t in th:
             t.join()
         torch.cuda.synchronize()
         res = [r for o in outs for r in o]
-        if len(res) != len(inst):
+        if len(res) != len(units):
             raise RuntimeError("batch: a worker thread failed")
         return max(r[0] for r in res), [r[1] for r in res]
 
@@ -234,16 +424,17 @@ def batch_workload(args):
         "value": len(inst) * args.steps / elapsed, "unit": "problems/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
-        "data": "synthetic",
+        "data": "synthetic (generated on the GPU)",
         "config": {"workload": ("48 problems, NC from {24,32,40,48,64,80,96} (seed 3), P so that "
                                 "NC/P is a power of two, a = 10^U[-2,2], eps=1e-4, 16 RHS "
                                 "N(0,1) each, mixed factor + block PCG to 1e-10"),
-                   "problems": len(inst), "nrhs": 16, "tol": args.tol,
+                   "problems": len(inst), "nrhs": R, "tol": args.tol,
+                   "mode": args.batch_mode, "factor_objects": len(units),
                    "host_threads": nthr},
-        "unknowns": dofs, "rhs_per_s": 16 * len(inst) * args.steps / elapsed,
+        "unknowns": dofs, "rhs_per_s": R * len(inst) * args.steps / elapsed,
         "worst_relres": worst, "max_pcg_iters": max(its),
     }
-    for f in facs:
+    for f, *_ in units:
         f.close()
     print(json.dumps(out), flush=True)
 
@@ -334,9 +525,15 @@ def main():
     ap.add_argument("--bf16x3", action="store_true",
                     help="FP32 levels' Schur products as bf16x3 split products")
     ap.add_argument("--cpu-n", type=int, default=32)
+    ap.add_argument("--cpu-validate", type=int, default=0, metavar="N",
+                    help="time the oracle at --cpu-n and at N (all cores) and compare the "
+                         "level-by-level estimate of N's time with the measurement (CPU only)")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--batch-threads", type=int, default=8,
-                    help="batch workload: host threads driving the problems concurrently")
+                    help="batch workload: host threads driving the factor objects concurrently")
+    ap.add_argument("--batch-mode", choices=("packed", "separate"), default="packed",
+                    help="batch workload: up to 8 problems of a grid size per factor object "
+                         "(packed, shared launches) or one object per problem")
     ap.add_argument("--workload", choices=("n128", "batch", "c5"), default="n128",
                     help="n128: the headline factor + PCG at n=128 (default); batch: the 48 "
                          "independent uneven problems with 16 right-hand sides each; c5: "
@@ -354,6 +551,9 @@ def main():
                        "(one factor + one PCG solve, n=128 mixed)")
         json.dump(out, open(PMC_FILE, "w"), indent=1, sort_keys=True)
         print(f"wrote {PMC_FILE}: {len(out['kernels'])} kernels")
+        return
+    if args.cpu_validate:
+        cpu_validate(args)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -434,6 +634,7 @@ def main():
         log(f"[bench]   {k_:40s} launches {v_['launches']:6d} dev {v_['sec']:.4f}s "
             f"{v_['flops'] / sec_ / 1e12:7.2f} TF/s {v_['bytes'] / sec_ / 1e9:8.1f} GB/s")
     fbytes_mixed = f.factor_bytes()
+    target_boxes = _gpu_boxes(f) if not args.no_cpu else None
     f.close()
 
     fp64 = None
@@ -457,8 +658,13 @@ def main():
     cpu = None
     if not args.no_cpu:
         log(f"[bench] cpu baseline sample n={args.cpu_n}")
-        cpu = cpu_baseline(args.cpu_n, n, seed, corr, eps, P, tol)
-        log(f"[bench] cpu: {cpu['sample']}")
+        it_target = fp64["pcg_iters"] if fp64 else it_l[-1]
+        cpu = cpu_baseline(args, target_boxes, it_target, n, seed, corr, eps, P, tol)
+        for r_ in cpu["runs"]:
+            e = r_["estimate_n_target"]
+            log(f"[bench] cpu n={r_['n']} {r_['threads']} thread(s): factor {r_['factor_s']:.2f}s "
+                f"solve {r_['solve_s']:.2f}s ({r_['pcg_iters']} it) -> n={n} estimate factor "
+                f"{e['factor_s']:.1f}s solve {e['solve_s']:.1f}s")
 
     ms = 1e3 * elapsed / args.steps
     value = args.steps / elapsed

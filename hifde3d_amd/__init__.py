@@ -42,7 +42,8 @@ EXPORTS = [
     "hifde_apply_device_noalloc", "hifde_pivoted_qr_batch", "hifde_triangular_solve_batch",
     "hifde_set_stencil_device", "hifde_gen_uniform", "hifde_gen_normal", "hifde_gen_rhs_normal",
     "hifde_gen_gaussian_field", "hifde_operator_csr", "hifde_gemm_table_load",
-    "hifde_gemm_table_clear", "hifde_gemm_log", "hifde_gemm_time",
+    "hifde_gemm_table_clear", "hifde_gemm_log", "hifde_gemm_time", "hifde_create_batch",
+    "hifde_set_stencil_batch_device", "hifde_batch_pack",
 ]
 
 
@@ -82,6 +83,9 @@ def lib():
     L.hifde_operator_csr.argtypes = [_vp, _ip, _ip, _dp]
     L.hifde_operator_csr.restype = _ll
     L.hifde_gemm_table_load.argtypes = [ctypes.c_char_p]
+    L.hifde_create_batch.argtypes = [_i, _i, _d, _i, _i, _i, ctypes.POINTER(_vp)]
+    L.hifde_set_stencil_batch_device.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _i]
+    L.hifde_batch_pack.argtypes = [_vp, _vp, _ll, _vp, _ll, _i, _i]
     L.hifde_gemm_table_clear.argtypes = []
     L.hifde_gemm_log.argtypes = [_vp, _ip, _ll]
     L.hifde_gemm_log.restype = _ll
@@ -121,6 +125,11 @@ def lib():
     L.hifde_schur_batch.argtypes = [_i, _i, _ip, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _ip]
     _lib = L
     return L
+
+
+def _ptr_array(ptrs):
+    arr = (_vp * len(ptrs))(*[ctypes.c_void_p(int(p)) for p in ptrs])
+    return ctypes.cast(arr, ctypes.POINTER(_vp)), arr
 
 
 def _check(rc):
@@ -163,9 +172,9 @@ class HIFFactor:
         self.lu_info = 0
 
     def close(self):
-        if getattr(self, "_h", None):
-            lib().hifde_destroy(self._h)
-            self._h = None
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.hifde_destroy(self._h)
+        self._h = None
 
     def __del__(self):
         self.close()
@@ -422,6 +431,65 @@ def device_problem(NC, coeff, seed, V="zeros", v_seed=None, corr=0.1, lo=-1.0, h
     elif V != "zeros":
         raise ValueError(V)
     return a, Vt
+
+
+class HIFBatch(HIFFactor):
+    """nprob (1 to 8) independent problems of one grid -- N interior points
+    per side, leaf P, ID tolerance epsilon -- factored by one object: packed
+    as the octants of a grid twice as wide (identity rows on the separating
+    planes and unused octants), so that every level's boxes of all of them go
+    through the same launches.  Vectors move between the problems' own layout
+    ((N+1)^3 each) and the packed one with pack / unpack; pcg_multi_device
+    runs one PCG per (problem, column) and returns per-pair iteration counts
+    and residuals shaped (nprob, nrhs)."""
+
+    def __init__(self, N, P, epsilon, nprob, precision="mixed", front_quirk=True):
+        self.N, self.P, self.epsilon = 2 * (int(N) + 1) - 1, int(P), float(epsilon)
+        self.NC = self.N + 1
+        self.N_sub, self.NC_sub, self.nprob = int(N), int(N) + 1, int(nprob)
+        self.precision = precision
+        prec = {"fp64": FP64, "mixed": MIXED}[precision]
+        h = _vp()
+        _check(lib().hifde_create_batch(int(N), self.P, self.epsilon, prec,
+                                        FLAG_FRONT_QUIRK if front_quirk else 0, self.nprob,
+                                        ctypes.byref(h)))
+        self._h = h
+        self.lu_info = 0
+
+    def set_stencil_batch_device(self, a_ptrs, V_ptrs, symmetric=True):
+        """Problem o's stencil from device nodal tensors a_ptrs[o], V_ptrs[o]
+        ((N+2)^3 doubles each, i fastest)."""
+        if len(a_ptrs) != self.nprob or len(V_ptrs) != self.nprob:
+            raise ValueError("one coefficient pair per problem")
+        self._follow_caller_stream()
+        pa, keep_a = _ptr_array(a_ptrs)
+        pv, keep_v = _ptr_array(V_ptrs)
+        _check(lib().hifde_set_stencil_batch_device(self._h, pa, pv, int(symmetric)))
+
+    def pack(self, sub_ptr, packed_ptr, nrhs=1, ld_sub=None, ld_packed=None):
+        """Problems' vectors (vector c of problem o at sub + (o*nrhs + c)*ld_sub)
+        into packed columns."""
+        self._follow_caller_stream()
+        _check(lib().hifde_batch_pack(self._h, ctypes.c_void_p(sub_ptr),
+                                      ld_sub or self.NC_sub ** 3, ctypes.c_void_p(packed_ptr),
+                                      ld_packed or self.size, int(nrhs), 0))
+
+    def unpack(self, packed_ptr, sub_ptr, nrhs=1, ld_sub=None, ld_packed=None):
+        self._follow_caller_stream()
+        _check(lib().hifde_batch_pack(self._h, ctypes.c_void_p(sub_ptr),
+                                      ld_sub or self.NC_sub ** 3, ctypes.c_void_p(packed_ptr),
+                                      ld_packed or self.size, int(nrhs), 1))
+
+    def pcg_multi_device(self, b_ptr, x_ptr, nrhs, ldb=None, ldx=None, tol=1e-10, maxit=200):
+        it = np.zeros(max(nrhs * self.nprob, 1), np.int32)
+        rr = np.zeros(max(nrhs * self.nprob, 1))
+        self._follow_caller_stream()
+        _check(lib().hifde_pcg_multi_device(
+            self._h, ctypes.c_void_p(b_ptr), ldb if ldb is not None else self.size,
+            ctypes.c_void_p(x_ptr), ldx if ldx is not None else self.size, nrhs, tol, maxit,
+            it.ctypes.data_as(_ip), rr.ctypes.data_as(_dp)))
+        n = nrhs * self.nprob
+        return it[:n].reshape(self.nprob, nrhs).copy(), rr[:n].reshape(self.nprob, nrhs).copy()
 
 
 def interp_decomp_batch(mats, eps):

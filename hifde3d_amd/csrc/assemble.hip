@@ -241,6 +241,141 @@ __global__ void k_normalize(const double* __restrict__ g, long long n,
     }
 }
 
+// --- packed batches ----------------------------------------------------
+// nprob independent problems of grid NC (unknowns 1..NC-1 per axis) stored
+// as the octants of one grid of M = 2 NC points per side: octant o holds
+// problem o at offset NC * (o & 1, o >> 1 & 1, o >> 2 & 1).  The planes at
+// coordinate NC separate the octants; their points, and every point of an
+// octant past nprob, carry identity rows with no couplings.
+struct PackPoint {
+    int kind;  // 0 boundary (no row), 1 identity row, 2 unknown of problem o
+    int o, i, j, k;
+};
+__device__ __forceinline__ PackPoint pack_point(long long p, int NC, int nprob) {
+    const int M = 2 * NC;
+    const int x = (int)(p % M), y = (int)((p / M) % M), z = (int)(p / ((long long)M * M));
+    PackPoint q{0, -1, 0, 0, 0};
+    if (x == 0 || y == 0 || z == 0) return q;
+    if (x == NC || y == NC || z == NC) {
+        q.kind = 1;
+        return q;
+    }
+    q.o = (x > NC) + 2 * (y > NC) + 4 * (z > NC);
+    if (q.o >= nprob) {
+        q.kind = 1;
+        q.o = -1;
+        return q;
+    }
+    q.kind = 2;
+    q.i = x > NC ? x - NC : x;
+    q.j = y > NC ? y - NC : y;
+    q.k = z > NC ? z - NC : z;
+    return q;
+}
+
+__global__ void k_pack_count(int NC, int nprob, int* len, unsigned char* seg) {
+    const int M = 2 * NC;
+    const long long n3 = (long long)M * M * M;
+    for (long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x; p < n3;
+         p += (long long)gridDim.x * blockDim.x) {
+        const PackPoint q = pack_point(p, NC, nprob);
+        len[p] = q.kind == 0 ? 0 : q.kind == 1 ? 1 : stencil_row_len(q.i, q.j, q.k, NC - 1);
+        seg[p] = q.kind == 2 ? (unsigned char)q.o : (unsigned char)255;
+    }
+}
+
+// Rows in the standalone assembly's order and arithmetic (k_stencil_fill),
+// columns relabelled into the packed grid; problem o's nodal tensors at
+// a[o], V[o] ((NC+1)^3 each).
+__global__ void k_pack_fill(int NC, int nprob, const double* const* __restrict__ a,
+                            const double* const* __restrict__ V, int symmetric, double hh,
+                            const int* __restrict__ rp, int* __restrict__ ci,
+                            double* __restrict__ val, int* __restrict__ rr) {
+#pragma clang fp contract(off)
+    const int M = 2 * NC, N = NC - 1;
+    const long long n3 = (long long)M * M * M;
+    const long long m = NC + 1;
+    for (long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x; p < n3;
+         p += (long long)gridDim.x * blockDim.x) {
+        const PackPoint q = pack_point(p, NC, nprob);
+        if (q.kind == 0) continue;
+        int w = rp[p];
+        if (q.kind == 1) {
+            ci[w] = (int)p;
+            val[w] = 1.0;
+            rr[w] = (int)p;
+            continue;
+        }
+        const double* A = a[q.o];
+        const double* Vo = V[q.o];
+        const int i = q.i, j = q.j, k = q.k;
+        const int ox = (int)(p % M) - i, oy = (int)((p / M) % M) - j,
+                  oz = (int)(p / ((long long)M * M)) - k;
+        auto T = [&](const double* t, int x, int y, int z) { return t[x + m * (y + m * z)]; };
+        const int nb[6][3] = {{i - 1, j, k}, {i + 1, j, k}, {i, j - 1, k},
+                              {i, j + 1, k}, {i, j, k - 1}, {i, j, k + 1}};
+        const double ap = T(A, i, j, k);
+        double d = 0.0;
+        if (!symmetric) {
+            d += (T(A, i - 1, j, k) + T(A, i + 1, j, k)) / hh;
+            d += (T(A, i, j - 1, k) + T(A, i, j + 1, k)) / hh;
+            d += (T(A, i, j, k - 1) + T(A, i, j, k + 1)) / hh;
+        } else {
+            for (int t = 0; t < 6; ++t) d += 0.5 * (ap + T(A, nb[t][0], nb[t][1], nb[t][2])) / hh;
+        }
+        d += T(Vo, i, j, k);
+        ci[w] = (int)p;
+        val[w] = d;
+        rr[w] = (int)p;
+        ++w;
+        for (int t = 0; t < 6; ++t) {
+            const int x = nb[t][0], y = nb[t][1], z = nb[t][2];
+            if (x < 1 || x > N || y < 1 || y > N || z < 1 || z > N) continue;
+            const double aq = T(A, x, y, z);
+            ci[w] = (x + ox) + M * ((y + oy) + M * (z + oz));
+            val[w] = symmetric ? -0.5 * (ap + aq) / hh : -aq / hh;
+            rr[w] = (int)p;
+            ++w;
+        }
+    }
+}
+
+// dir 0: packed column c <- problem o's vector c (sub + (o R + c) ld_sub,
+// standalone layout) on each problem's unknowns, 0 elsewhere.  dir 1: the
+// reverse, 0 on each problem's boundary planes.
+__global__ void k_pack_vectors(int NC, int nprob, int R, double* sub, long long ld_sub,
+                               double* packed, long long ld_packed, int dir) {
+    const int M = 2 * NC;
+    const long long n3 = (long long)M * M * M, s3 = (long long)NC * NC * NC;
+    if (dir == 0) {
+        for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n3 * R;
+             e += (long long)gridDim.x * blockDim.x) {
+            const int c = (int)(e / n3);
+            const long long p = e - c * n3;
+            const PackPoint q = pack_point(p, NC, nprob);
+            packed[c * ld_packed + p] =
+                q.kind == 2 ? sub[((long long)q.o * R + c) * ld_sub + q.i +
+                                  (long long)NC * (q.j + (long long)NC * q.k)]
+                            : 0.0;
+        }
+    } else {
+        for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+             e < s3 * R * nprob; e += (long long)gridDim.x * blockDim.x) {
+            const long long oc = e / s3;
+            const long long l = e - oc * s3;
+            const int o = (int)(oc / R), c = (int)(oc - (long long)o * R);
+            const int i = (int)(l % NC), j = (int)((l / NC) % NC), k = (int)(l / ((long long)NC * NC));
+            double v = 0.0;
+            if (i > 0 && j > 0 && k > 0) {
+                const int x = i + NC * (o & 1), y = j + NC * ((o >> 1) & 1),
+                          z = k + NC * ((o >> 2) & 1);
+                v = packed[c * ld_packed + x + (long long)M * (y + (long long)M * z)];
+            }
+            sub[oc * ld_sub + l] = v;
+        }
+    }
+}
+
 int grid_of(long long n) {
     const long long b = (n + 255) / 256;
     return (int)std::max<long long>(1, std::min<long long>(b, 65536));
@@ -346,6 +481,47 @@ long long stencil_csr(int NC, const double* d_a, const double* d_V, int symmetri
                                                rr.p);
     HIFDE_LAUNCH_CHECK();
     return nnz;
+}
+
+long long pack_stencil_csr(int NC, int nprob, const double* const* d_a_ptrs,
+                           const double* const* d_V_ptrs, int symmetric, DevBuf<int>& rp,
+                           DevBuf<int>& ci, DevBuf<double>& val, DevBuf<int>& rr,
+                           DevBuf<unsigned char>& seg, hipStream_t s) {
+    const long long M = 2LL * NC, n3 = M * M * M;
+    DevBuf<int> len;
+    len.reserve(n3 + 1);
+    rp.reserve(n3 + 1);
+    seg.reserve(n3);
+    HIFDE_CHECK(hipMemsetAsync(len.p + n3, 0, sizeof(int), s));
+    k_pack_count<<<grid_of(n3), 256, 0, s>>>(NC, nprob, len.p, seg.p);
+    HIFDE_LAUNCH_CHECK();
+    size_t tmp_bytes = 0;
+    HIFDE_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, len.p, rp.p, (int)(n3 + 1),
+                                                 s));
+    DevBuf<unsigned char> tmp;
+    tmp.reserve(tmp_bytes);
+    HIFDE_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp.p, tmp_bytes, len.p, rp.p, (int)(n3 + 1), s));
+    int nnz = 0;
+    HIFDE_CHECK(hipMemcpyAsync(&nnz, rp.p + n3, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIFDE_CHECK(hipStreamSynchronize(s));
+    ci.reserve(std::max(nnz, 1));
+    val.reserve(std::max(nnz, 1));
+    rr.reserve(std::max(nnz, 1));
+    const double h = 1.0 / NC;
+    k_pack_fill<<<grid_of(n3), 256, 0, s>>>(NC, nprob, d_a_ptrs, d_V_ptrs, symmetric, h * h, rp.p,
+                                            ci.p, val.p, rr.p);
+    HIFDE_LAUNCH_CHECK();
+    return nnz;
+}
+
+void pack_vectors(int NC, int nprob, int R, double* sub, long long ld_sub, double* packed,
+                  long long ld_packed, bool unpack, hipStream_t s) {
+    const long long M = 2LL * NC;
+    const long long n = unpack ? (long long)NC * NC * NC * R * nprob : M * M * M * R;
+    if (n <= 0) return;
+    k_pack_vectors<<<grid_of(n), 256, 0, s>>>(NC, nprob, R, sub, ld_sub, packed, ld_packed,
+                                              unpack ? 1 : 0);
+    HIFDE_LAUNCH_CHECK();
 }
 
 bool csr_symmetric(long long nd, const int* d_rp, const int* d_ci, const double* d_val,

@@ -1707,6 +1707,11 @@ struct hifde_factor {
     // counts these).  A failure that is not about the capture recurs in the
     // eager run that follows and is reported from there.
     int graph_failures = 0;
+    // Packed batch (hifde_create_batch): nprob > 0 independent problems of
+    // grid NC_sub packed as the octants of this object's grid; seg maps
+    // every packed entry to its problem (255: none).
+    int nprob = 0, NC_sub = 0;
+    DevBuf<unsigned char> d_seg;
     bool gemm_record = false;  // log every GEMM descriptor of a factorization
     GemmRecorder gemm_log;
     bool gemm_table_on = true;  // route GEMMs by the tuning table where it has an entry
@@ -1982,40 +1987,55 @@ struct hifde_factor {
     DevBuf<double> mpcg_buf, mpcg_part, mpcg_scal;
     // F^{-1} on R columns (the single-vector sweeps for one column)
     void apply_cols(double* Z, int R, long long ld) { apply_sweeps(Z, R, ld, true); }
+    // R independent PCG solves advanced together.  On a packed batch every
+    // (column, problem) pair is its own solve: the operator and the factor
+    // are block diagonal over the problems, so the recurrences separate once
+    // the dots and step lengths are taken per problem (segment).  Results are
+    // indexed [problem * R + column] there.
     void pcg_multi_dev(const double* B, long long ldb, double* X, long long ldx, int R,
                        double tol, int maxit, int* iters, double* relres) {
         TimerGuard tguard(this);
         if (!factored) throw Error(HIFDE_ERR_STATE, "pcg: factor first");
         const long long n = (long long)NC * NC * NC;
+        const int S = nprob > 0 ? nprob : 1;
+        const int RS = R * S;  // solves, [column * S + segment] internally
         mpcg_buf.reserve((size_t)4 * n * R);
         double* Rm = mpcg_buf.p;
         double* Z = Rm + n * R;
         double* Pm = Z + n * R;
         double* Q = Pm + n * R;
         const int nblk = std::max(1, std::min(256, 4096 / R));
-        mpcg_part.reserve((size_t)nblk * R);
-        mpcg_scal.reserve((size_t)4 * R);
-        double* d_dot = mpcg_scal.p;       // R dots
-        double* d_a = d_dot + R;           // R step lengths
-        double* d_ma = d_a + R;            // their negatives
-        double* d_b = d_ma + R;            // R betas
-        std::vector<double> h(4 * R);
+        mpcg_part.reserve((size_t)nblk * RS);
+        mpcg_scal.reserve((size_t)4 * RS);
+        double* d_dot = mpcg_scal.p;       // RS dots
+        double* d_a = d_dot + RS;          // RS step lengths
+        double* d_ma = d_a + RS;           // their negatives
+        double* d_b = d_ma + RS;           // RS betas
+        std::vector<double> h(4 * RS);
         auto dots = [&](const double* x, const double* y) {
-            dot_cols(n, x, n, y, n, R, mpcg_part.p, nblk, d_dot, stream);
-            HIFDE_CHECK(hipMemcpyAsync(h.data(), d_dot, R * sizeof(double),
+            if (nprob > 0)
+                dot_cols_seg(n, x, n, y, n, R, d_seg.p, S, mpcg_part.p, nblk, d_dot, stream);
+            else
+                dot_cols(n, x, n, y, n, R, mpcg_part.p, nblk, d_dot, stream);
+            HIFDE_CHECK(hipMemcpyAsync(h.data(), d_dot, RS * sizeof(double),
                                        hipMemcpyDeviceToHost, stream));
             HIFDE_CHECK(hipStreamSynchronize(stream));
-            return std::vector<double>(h.begin(), h.begin() + R);
+            return std::vector<double>(h.begin(), h.begin() + RS);
         };
         auto upload_scalars = [&](const std::vector<double>& a, const std::vector<double>& b) {
-            for (int c = 0; c < R; ++c) {
-                h[R + c] = a[c];
-                h[2 * R + c] = -a[c];
-                h[3 * R + c] = b[c];
+            for (int c = 0; c < RS; ++c) {
+                h[RS + c] = a[c];
+                h[2 * RS + c] = -a[c];
+                h[3 * RS + c] = b[c];
             }
-            HIFDE_CHECK(hipMemcpyAsync(d_a, h.data() + R, 3 * R * sizeof(double),
+            HIFDE_CHECK(hipMemcpyAsync(d_a, h.data() + RS, 3 * RS * sizeof(double),
                                        hipMemcpyHostToDevice, stream));
             HIFDE_CHECK(hipStreamSynchronize(stream));
+        };
+        auto axpy = [&](const double* a, const double* Xs, long long lx, double* Ys, long long ly,
+                        bool xpby) {
+            if (nprob > 0) axpy_cols_seg(n, a, Xs, lx, Ys, ly, R, xpby, d_seg.p, S, stream);
+            else axpy_cols(n, a, Xs, lx, Ys, ly, R, xpby, stream);
         };
         // r = b - A x, boundary rows cleared
         {
@@ -2023,25 +2043,25 @@ struct hifde_factor {
             csr_spmm((int)n, d_rp.p, d_ci.p, d_v.p, X, ldx, Q, n, R, stream);
         }
         for (int c = 0; c < R; ++c) vcopy(B + c * ldb, Rm + c * n, n, stream);
-        std::vector<double> ones(R, -1.0), zeros(R, 0.0);
+        std::vector<double> ones(RS, -1.0), zeros(RS, 0.0);
         upload_scalars(ones, zeros);
-        axpy_cols(n, d_a, Q, n, Rm, n, R, false, stream);
+        axpy(d_a, Q, n, Rm, n, false);
         for (int c = 0; c < R; ++c) zero_boundary(Rm + c * n, NC, stream);
-        std::vector<double> bnorm(R), rnorm(R);
+        std::vector<double> bnorm(RS), rnorm(RS);
         {
-            // ||b|| per column (b has its own leading dimension)
+            // ||b|| per solve (b has its own leading dimension)
             for (int c = 0; c < R; ++c) vcopy(B + c * ldb, Z + c * n, n, stream);
             std::vector<double> bb = dots(Z, Z);
-            for (int c = 0; c < R; ++c) {
+            for (int c = 0; c < RS; ++c) {
                 bnorm[c] = std::sqrt(bb[c]);
                 if (bnorm[c] == 0.0) bnorm[c] = 1.0;
             }
         }
         std::vector<double> rr = dots(Rm, Rm);
-        std::vector<int> it(R, 0);
-        std::vector<char> done(R, 0);
+        std::vector<int> it(RS, 0);
+        std::vector<char> done(RS, 0);
         int active = 0;
-        for (int c = 0; c < R; ++c) {
+        for (int c = 0; c < RS; ++c) {
             rnorm[c] = std::sqrt(rr[c]);
             done[c] = rnorm[c] / bnorm[c] <= tol;
             active += !done[c];
@@ -2051,19 +2071,19 @@ struct hifde_factor {
             apply_cols(Z, R, n);
             vcopy(Z, Pm, n * R, stream);
             std::vector<double> rz = dots(Rm, Z);
-            std::vector<double> alpha(R), beta(R);
+            std::vector<double> alpha(RS), beta(RS);
             for (int k = 1; k <= maxit && active > 0; ++k) {
                 {
                     Prof pr(this, "spmv", 2.0 * nnz_op * R, (12.0 * nnz_op + 16.0 * n) * R);
                     csr_spmm((int)n, d_rp.p, d_ci.p, d_v.p, Pm, n, Q, n, R, stream);
                 }
                 std::vector<double> pq = dots(Pm, Q);
-                for (int c = 0; c < R; ++c) alpha[c] = done[c] ? 0.0 : rz[c] / pq[c];
+                for (int c = 0; c < RS; ++c) alpha[c] = done[c] ? 0.0 : rz[c] / pq[c];
                 upload_scalars(alpha, zeros);
-                axpy_cols(n, d_a, Pm, n, X, ldx, R, false, stream);
-                axpy_cols(n, d_ma, Q, n, Rm, n, R, false, stream);
+                axpy(d_a, Pm, n, X, ldx, false);
+                axpy(d_ma, Q, n, Rm, n, false);
                 rr = dots(Rm, Rm);
-                for (int c = 0; c < R; ++c) {
+                for (int c = 0; c < RS; ++c) {
                     if (done[c]) continue;
                     it[c] = k;
                     rnorm[c] = std::sqrt(rr[c]);
@@ -2076,20 +2096,22 @@ struct hifde_factor {
                 vcopy(Rm, Z, n * R, stream);
                 apply_cols(Z, R, n);
                 std::vector<double> rz_new = dots(Rm, Z);
-                for (int c = 0; c < R; ++c) {
+                for (int c = 0; c < RS; ++c) {
                     beta[c] = done[c] ? 0.0 : rz_new[c] / rz[c];
                     if (!done[c]) rz[c] = rz_new[c];
                 }
                 upload_scalars(zeros, beta);
-                axpy_cols(n, d_b, Z, n, Pm, n, R, true, stream);
+                axpy(d_b, Z, n, Pm, n, true);
             }
         }
         HIFDE_CHECK(hipStreamSynchronize(stream));
         resolve_profile();
-        for (int c = 0; c < R; ++c) {
-            if (iters) iters[c] = it[c];
-            if (relres) relres[c] = rnorm[c] / bnorm[c];
-        }
+        for (int c = 0; c < R; ++c)
+            for (int sg = 0; sg < S; ++sg) {
+                const int q = c * S + sg, o = sg * R + c;
+                if (iters) iters[o] = it[q];
+                if (relres) relres[o] = rnorm[q] / bnorm[q];
+            }
     }
 
     // ------------------------------------------------------------------
@@ -2427,6 +2449,29 @@ struct hifde_factor {
         have_matrix = true;
         factored = false;
     }
+
+    // The nprob problems' stencils assembled into the packed grid: each
+    // problem's rows are its own operator's (as set_stencil_device would
+    // build them) relabelled into its octant; the separating planes and any
+    // unused octant get identity rows.
+    void set_stencil_batch_device(const double* const* a, const double* const* V, int symmetric) {
+        const long long nd = (long long)NC * NC * NC;
+        DevBuf<const double*> ptrs;
+        std::vector<const double*> h(2 * nprob);
+        for (int o = 0; o < nprob; ++o) {
+            h[o] = a[o];
+            h[nprob + o] = V[o];
+        }
+        ptrs.upload(h, stream);
+        nnz_op = pack_stencil_csr(NC_sub, nprob, ptrs.p, ptrs.p + nprob, symmetric, d_rp, d_ci,
+                                  d_v, d_rr, d_seg, stream);
+        host_csr_valid = false;
+        op_symmetric = csr_symmetric(nd, d_rp.p, d_ci.p, d_v.p, stream);
+        local_ok = true;
+        HIFDE_CHECK(hipStreamSynchronize(stream));
+        have_matrix = true;
+        factored = false;
+    }
 };
 
 // ===========================================================================
@@ -2710,6 +2755,54 @@ int hifde_set_stream(hifde_factor* f, void* stream) {
     return HIFDE_OK;
 }
 
+int hifde_create_batch(int N, int P, double epsilon, int precision, int flags, int nprob,
+                       hifde_factor** out) {
+    return guarded([&]() -> int {
+        if (!out || nprob < 1 || nprob > 8 || N < 1)
+            throw Error(HIFDE_ERR_ARG, "create_batch: 1 to 8 problems");
+        hifde_factor* f = nullptr;
+        const int rc = hifde_create(2 * (N + 1) - 1, P, epsilon, precision, flags, &f);
+        if (rc < 0) throw Error(rc, g_last_error);
+        f->nprob = nprob;
+        f->NC_sub = N + 1;
+        *out = f;
+        return HIFDE_OK;
+    });
+}
+
+int hifde_set_stencil_batch_device(hifde_factor* f, const double* const* d_a,
+                                   const double* const* d_v, int symmetric) {
+    return guarded([&]() -> int {
+        OwnerGuard og(f ? f->stream : nullptr);
+        if (!f || !d_a || !d_v) throw Error(HIFDE_ERR_ARG, "set_stencil_batch: null argument");
+        if (f->nprob < 1) throw Error(HIFDE_ERR_STATE, "set_stencil_batch: not a packed batch");
+        for (int o = 0; o < f->nprob; ++o)
+            if (!d_a[o] || !d_v[o]) throw Error(HIFDE_ERR_ARG, "set_stencil_batch: null tensor");
+        f->after_caller_stream();
+        f->set_stencil_batch_device(d_a, d_v, symmetric);
+        return HIFDE_OK;
+    });
+}
+
+int hifde_batch_pack(hifde_factor* f, double* d_sub, long long ld_sub, double* d_packed,
+                     long long ld_packed, int nrhs, int unpack) {
+    return guarded([&]() -> int {
+        if (!f || !d_sub || !d_packed || nrhs < 0) throw Error(HIFDE_ERR_ARG, "batch_pack: bad args");
+        if (f->nprob < 1) throw Error(HIFDE_ERR_STATE, "batch_pack: not a packed batch");
+        const long long ns = (long long)f->NC_sub * f->NC_sub * f->NC_sub;
+        const long long np = (long long)f->NC * f->NC * f->NC;
+        if ((nrhs * f->nprob > 1 && ld_sub < ns) || (nrhs > 1 && ld_packed < np))
+            throw Error(HIFDE_ERR_ARG, "batch_pack: leading dimension too small");
+        if (nrhs == 0) return HIFDE_OK;
+        OwnerGuard og(f->stream);
+        f->after_caller_stream();
+        pack_vectors(f->NC_sub, f->nprob, nrhs, d_sub, std::max(ld_sub, ns), d_packed,
+                     std::max(ld_packed, np), unpack != 0, f->stream);
+        HIFDE_CHECK(hipStreamSynchronize(f->stream));
+        return HIFDE_OK;
+    });
+}
+
 void hifde_destroy(hifde_factor* f) {
     if (!f) return;
     hipStream_t s = f->stream;
@@ -2857,6 +2950,8 @@ int hifde_pcg_device(hifde_factor* f, const double* d_b, double* d_x, double tol
                      int* iters, double* relres) {
     return guarded([&]() -> int {
         if (!f || !d_b || !d_x || maxit < 0) throw Error(HIFDE_ERR_ARG, "pcg: bad arguments");
+        if (f->nprob > 0)
+            throw Error(HIFDE_ERR_STATE, "pcg: a packed batch solves with hifde_pcg_multi_device");
         f->after_caller_stream();
         f->pcg_dev(d_b, d_x, tol, maxit, iters, relres);
         return HIFDE_OK;
@@ -2884,6 +2979,8 @@ int hifde_pcg(hifde_factor* f, const double* b, double* x, double tol, int maxit
     return guarded([&]() -> int {
         OwnerGuard og(f ? f->stream : nullptr);
         if (!f || !b || !x) throw Error(HIFDE_ERR_ARG, "pcg: bad arguments");
+        if (f->nprob > 0)
+            throw Error(HIFDE_ERR_STATE, "pcg: a packed batch solves with hifde_pcg_multi_device");
         long long n = (long long)f->NC * f->NC * f->NC;
         DevBuf<double> db, dx;
         db.upload(b, n, f->stream);

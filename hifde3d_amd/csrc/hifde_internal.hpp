@@ -549,6 +549,23 @@ void dot_cols(long long n, const double* X, long long ldx, const double* Y, long
               double* d_part, int nblk, double* d_out, hipStream_t s);
 void axpy_cols(long long n, const double* d_a, const double* X, long long ldx, double* Y,
                long long ldy, int R, bool xpby, hipStream_t s);
+// The same per column and segment (d_seg: segment of each entry, 255 for
+// none; 1 <= nseg <= 8): d_out[c nseg + s]; scalars d_a[c nseg + s].
+void dot_cols_seg(long long n, const double* X, long long ldx, const double* Y, long long ldy,
+                  int R, const unsigned char* d_seg, int nseg, double* d_part, int nblk,
+                  double* d_out, hipStream_t s);
+void axpy_cols_seg(long long n, const double* d_a, const double* X, long long ldx, double* Y,
+                   long long ldy, int R, bool xpby, const unsigned char* d_seg, int nseg,
+                   hipStream_t s);
+// Packed batches (assemble.hip): nprob problems of grid NC as the octants of
+// a grid 2 NC wide; CSR with identity rows on the separating planes and on
+// unused octants, and the segment (problem) of every packed entry.
+long long pack_stencil_csr(int NC, int nprob, const double* const* d_a_ptrs,
+                           const double* const* d_V_ptrs, int symmetric, DevBuf<int>& rp,
+                           DevBuf<int>& ci, DevBuf<double>& val, DevBuf<int>& rr,
+                           DevBuf<unsigned char>& seg, hipStream_t s);
+void pack_vectors(int NC, int nprob, int R, double* sub, long long ld_sub, double* packed,
+                  long long ld_packed, bool unpack, hipStream_t s);
 
 // vectors / CSR SpMV for PCG
 void csr_spmv(int n, const int* d_rowptr, const int* d_colind, const double* d_val,

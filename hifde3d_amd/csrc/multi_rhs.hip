@@ -273,6 +273,60 @@ __global__ void k_sum_cols(const double* __restrict__ part, int nblk, int R, dou
     out[c] = acc;
 }
 
+// Dots per column and segment: part[(c S + s) nblk + block] sums x_i y_i
+// over the entries with seg[i] == s (entries of segment 255 belong to none).
+template <int S>
+__global__ void __launch_bounds__(256) k_dot_cols_seg(long long n, const double* __restrict__ X,
+                                                      long long ldx,
+                                                      const double* __restrict__ Y,
+                                                      long long ldy,
+                                                      const unsigned char* __restrict__ seg,
+                                                      int nseg, double* part) {
+    const int c = blockIdx.y;
+    const double* x = X + c * ldx;
+    const double* y = Y + c * ldy;
+    double acc[S];
+#pragma unroll
+    for (int t = 0; t < S; ++t) acc[t] = 0.0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const int sg = seg[i];
+        const double v = x[i] * y[i];
+#pragma unroll
+        for (int t = 0; t < S; ++t) acc[t] += sg == t ? v : 0.0;
+    }
+    __shared__ double ws[4][S];
+#pragma unroll
+    for (int t = 0; t < S; ++t) {
+        double a = acc[t];
+        for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+        if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6][t] = a;
+    }
+    __syncthreads();
+    if (threadIdx.x < nseg)
+        part[((long long)c * nseg + threadIdx.x) * gridDim.x + blockIdx.x] =
+            ws[0][threadIdx.x] + ws[1][threadIdx.x] + ws[2][threadIdx.x] + ws[3][threadIdx.x];
+}
+
+// As k_axpy_cols with one scalar per column and segment, a[c S + seg[i]];
+// entries of no segment are left unchanged.
+__global__ void k_axpy_cols_seg(long long n, const double* __restrict__ a, const double* X,
+                                long long ldx, double* Y, long long ldy, int R, int xpby,
+                                const unsigned char* __restrict__ seg, int nseg) {
+    const long long total = n * R;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int c = (int)(e / n);
+        const long long i = e - (long long)c * n;
+        const int sg = seg[i];
+        if (sg >= nseg) continue;
+        const double al = a[c * nseg + sg];
+        const double x = X[i + c * ldx];
+        double& y = Y[i + c * ldy];
+        y = xpby ? x + al * y : y + al * x;
+    }
+}
+
 // Y(:, c) = Y(:, c) + a[c] X(:, c)   (xpby == 0)
 // Y(:, c) = X(:, c) + a[c] Y(:, c)   (xpby == 1)
 __global__ void k_axpy_cols(long long n, const double* __restrict__ a, const double* X,
@@ -376,6 +430,24 @@ void dot_cols(long long n, const double* X, long long ldx, const double* Y, long
     k_dot_cols<<<dim3(nblk, R), 256, 0, s>>>(n, X, ldx, Y, ldy, d_part);
     HIFDE_LAUNCH_CHECK();
     k_sum_cols<<<(R + 63) / 64, 64, 0, s>>>(d_part, nblk, R, d_out);
+    HIFDE_LAUNCH_CHECK();
+}
+
+void dot_cols_seg(long long n, const double* X, long long ldx, const double* Y, long long ldy,
+                  int R, const unsigned char* d_seg, int nseg, double* d_part, int nblk,
+                  double* d_out, hipStream_t s) {
+    if (nseg < 1 || nseg > 8) throw Error(-1, "dot_cols_seg: 1 to 8 segments");
+    k_dot_cols_seg<8><<<dim3(nblk, R), 256, 0, s>>>(n, X, ldx, Y, ldy, d_seg, nseg, d_part);
+    HIFDE_LAUNCH_CHECK();
+    k_sum_cols<<<(R * nseg + 63) / 64, 64, 0, s>>>(d_part, nblk, R * nseg, d_out);
+    HIFDE_LAUNCH_CHECK();
+}
+
+void axpy_cols_seg(long long n, const double* d_a, const double* X, long long ldx, double* Y,
+                   long long ldy, int R, bool xpby, const unsigned char* d_seg, int nseg,
+                   hipStream_t s) {
+    k_axpy_cols_seg<<<grid1(n * R), 256, 0, s>>>(n, d_a, X, ldx, Y, ldy, R, xpby ? 1 : 0, d_seg,
+                                                 nseg);
     HIFDE_LAUNCH_CHECK();
 }
 

@@ -90,6 +90,33 @@ int hifde_set_stencil_device(hifde_factor* f, const double* d_a, const double* d
  * duplicates summed.  Any pointer may be NULL.  Returns nnz, or < 0. */
 long long hifde_operator_csr(hifde_factor* f, int* rowptr, int* colind, double* vals);
 
+/* Independent problems batched through shared launches (no reference
+ * counterpart: the reference factors one problem at a time).
+ * hifde_create_batch makes one factor object for nprob (1 to 8) problems of
+ * the same grid (N interior points per side, leaf P, epsilon): they are
+ * packed as the octants of a grid 2(N+1) points wide (problem o at offset
+ * (N+1) * (o & 1, o >> 1 & 1, o >> 2 & 1)), the planes between the octants
+ * and any unused octant carrying identity rows with no couplings.  Each
+ * level of the factorization then eliminates the boxes of every problem in
+ * the same launches; every box of a problem is its standalone box (the
+ * separating points are extra, decoupled DOFs), except that the FRONT-face
+ * guard of src/Factor.cpp:486 applies at the packed grid's k = 0 only.
+ * hifde_set_stencil_batch_device assembles problem o's stencil from its
+ * device nodal tensors d_a[o], d_v[o] (host arrays of nprob device
+ * pointers; tensors as hifde_set_stencil_device).  hifde_batch_pack moves
+ * vectors between the problems' own layout -- vector c of problem o at
+ * d_sub + (o*nrhs + c)*ld_sub, (N+1)^3 entries -- and the packed layout --
+ * column c at d_packed + c*ld_packed, (2(N+1))^3 entries -- (unpack != 0:
+ * packed to problems).  hifde_pcg_multi_device on a batch runs one PCG per
+ * (problem, column) with its own dots, step lengths and stopping test;
+ * iters and relres then hold nprob*nrhs entries, [o*nrhs + c]. */
+int hifde_create_batch(int N, int P, double epsilon, int precision, int flags, int nprob,
+                       hifde_factor** out);
+int hifde_set_stencil_batch_device(hifde_factor* f, const double* const* d_a,
+                                   const double* const* d_v, int symmetric);
+int hifde_batch_pack(hifde_factor* f, double* d_sub, long long ld_sub, double* d_packed,
+                     long long ld_packed, int nrhs, int unpack);
+
 /* Synthetic-input generators on device memory (no reference counterpart:
  * the reference fills its tensors by hand with a = 1, V = 0,
  * tests/HIFDE3D/main.cpp:59-74).  Each runs on the given stream (a

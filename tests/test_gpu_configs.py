@@ -145,3 +145,50 @@ def test_large_leaf_config_full_size_block_pcg(H):
         r = B[c] - A @ X[c]
         assert float(torch.linalg.norm(r) / torch.linalg.norm(B[c])) < 2e-10, c
     f.close()
+
+
+def test_batched_config_packed_shared_launches(H):
+    """The batched config through packed factor objects (up to eight problems
+    of one grid size per object, every level's boxes in shared launches): all
+    48 problems, 16 right-hand sides each, every (problem, column) PCG to
+    1e-10 and its true residual -- against the problem's own operator,
+    assembled by a standalone object -- below 2e-10."""
+    import torch
+    inst = pb.batch_instances()
+    R = 16
+    by_nc = {}
+    for p in inst:
+        by_nc.setdefault(p["NC"], []).append(p)
+    done = 0
+    for NC, ps in sorted(by_nc.items()):
+        for c0 in range(0, len(ps), 8):
+            chunk = ps[c0:c0 + 8]
+            coeffs = [H.device_problem(NC, "pow10", p["coeff_seed"], lo=-2.0, hi=2.0)
+                      for p in chunk]
+            f = H.HIFBatch(NC - 1, chunk[0]["P"], 1e-4, len(chunk), precision="mixed")
+            f.set_stencil_batch_device([a.data_ptr() for a, _ in coeffs],
+                                       [V.data_ptr() for _, V in coeffs], True)
+            assert f.Factor() == 0
+            Bs = torch.empty((len(chunk) * R, NC ** 3), dtype=torch.float64, device="cuda")
+            for o, p in enumerate(chunk):
+                H.gen_rhs_normal(Bs[o * R].data_ptr(), NC, p["rhs_seed"], nrhs=R)
+            Bp = torch.empty((R, f.size), dtype=torch.float64, device="cuda")
+            f.pack(Bs.data_ptr(), Bp.data_ptr(), R)
+            Xp = torch.zeros_like(Bp)
+            its, rr = f.pcg_multi_device(Bp.data_ptr(), Xp.data_ptr(), R, tol=1e-10, maxit=300)
+            assert (rr <= 1e-10).all() and its.max() < 100, (NC, rr.max(), its.max())
+            Xs = torch.empty_like(Bs)
+            f.unpack(Xp.data_ptr(), Xs.data_ptr(), R)
+            f.close()
+            for o, (a, V) in enumerate(coeffs):
+                g = H.HIFFactor(NC - 1, chunk[o]["P"], 1e-4)
+                g.set_stencil_device(a.data_ptr(), V.data_ptr(), True)
+                A = _csr(g)
+                g.close()
+                Bo = Bs[o * R:(o + 1) * R].cpu().numpy()
+                Xo = Xs[o * R:(o + 1) * R].cpu().numpy()
+                for c in range(R):
+                    res = np.linalg.norm(A @ Xo[c] - Bo[c]) / np.linalg.norm(Bo[c])
+                    assert res < 2e-10, (NC, o, c, res)
+                done += 1
+    assert done == 48
