@@ -3343,7 +3343,8 @@ int schur_batch_impl(int nb, const int* r, const int* s, const double* A22, cons
 namespace {
 template <class T>
 int gemm_batch_impl(int transA, int nb, const int* m, const int* n, const int* k, const double* A,
-                    const double* B, double* C, bool split, int tile64, int force) {
+                    const double* B, double* C, bool split, int tile64, int force,
+                    bool table) {
     hipStream_t st = aux_stream();
     OwnerGuard og(st);
     std::vector<long long> oA(nb + 1, 0), oB(nb + 1, 0), oC(nb + 1, 0);
@@ -3365,6 +3366,7 @@ int gemm_batch_impl(int transA, int nb, const int* m, const int* n, const int* k
     gws.plan.split = split;
     gws.plan.tile64 = tile64;
     gws.plan.force_kind = force;
+    gws.plan.use_table = table;
     batched_gemm<T>(transA != 0, nb, m, n, k, oA.data(), oB.data(), oC.data(), dA.p, dB.p, dC.p,
                     (T)1, gws, st);
     std::vector<T> tmp(oC[nb]);
@@ -3391,13 +3393,17 @@ int hifde_gemm_batch(int precision, int transA, int nbatch, const int* m, const 
                           : (precision & HIFDE_GEMM_FORCE_CORE128) ? 1
                           : (precision & HIFDE_GEMM_FORCE_CORE64)  ? 2
                                                                    : -1;
+        const bool table = (precision & HIFDE_GEMM_RULE) == 0;
         const int prec = precision & ~(HIFDE_BF16X3 | HIFDE_GEMM_TILE64 | HIFDE_GEMM_FORCE_FMA |
-                                       HIFDE_GEMM_FORCE_CORE128 | HIFDE_GEMM_FORCE_CORE64);
+                                       HIFDE_GEMM_FORCE_CORE128 | HIFDE_GEMM_FORCE_CORE64 |
+                                       HIFDE_GEMM_RULE);
         if (split && prec != HIFDE_MIXED)
             throw Error(HIFDE_ERR_ARG, "gemm_batch: HIFDE_BF16X3 needs HIFDE_MIXED");
         if (prec == HIFDE_MIXED)
-            return gemm_batch_impl<float>(transA, nbatch, m, n, k, A, B, C, split, tile64, force);
-        return gemm_batch_impl<double>(transA, nbatch, m, n, k, A, B, C, false, tile64, force);
+            return gemm_batch_impl<float>(transA, nbatch, m, n, k, A, B, C, split, tile64, force,
+                                          table);
+        return gemm_batch_impl<double>(transA, nbatch, m, n, k, A, B, C, false, tile64, force,
+                                       table);
     });
 }
 

@@ -357,6 +357,11 @@ int hifde_triangular_solve_batch(int nbatch, const int* m, const int* n, const d
 #define HIFDE_GEMM_FORCE_FMA 0x800
 #define HIFDE_GEMM_FORCE_CORE128 0x1000
 #define HIFDE_GEMM_FORCE_CORE64 0x2000
+/* OR-ed into the precision of hifde_gemm_batch: route by the fixed rule (the
+ * flags above), ignoring a loaded routing table (hifde_gemm_table_load), whose
+ * entries otherwise take precedence over the rule for the classes they cover.
+ * No counterpart in the reference. */
+#define HIFDE_GEMM_RULE 0x4000
 int hifde_schur_batch(int precision, int nbatch, const int* r, const int* s, const double* A22,
                       const double* A21, const double* A12, double* A22inv, double* X, double* S,
                       int* info);

@@ -490,7 +490,7 @@ def test_gemm_batch_64_tiles_match_numpy(H, precision, tol, transA):
     for m, n, k in shapes:
         A = rng.standard_normal((k, m) if transA else (m, k))
         pairs.append((A, rng.standard_normal((k, n))))
-    got = H.gemm_batch(pairs, precision, transA, tile64=True)
+    got = H.gemm_batch(pairs, precision, transA, tile64=True, rule=True)
     for (A, B), C in zip(pairs, got):
         ref = (A.T if transA else A) @ B
         assert _rel(C, ref) < tol
@@ -500,13 +500,15 @@ def test_gemm_batch_64_tiles_match_numpy(H, precision, tol, transA):
 def test_gemm_tile64_factor_matches_default(H, oracle_mod, mode):
     """Factor option gemm_tile64 (1 is the default) against 0, all 128 x 128
     tiles: the same factorization to rounding (FP64), with the 64 x 64 kernel
-    recorded among its launches."""
+    recorded among its launches.  The routing table is switched off: its
+    entries take precedence over the tile rule."""
     N, P, eps = 63, 4, 1e-6
     a, V, coo, A = _coo(oracle_mod, N, "pow10", True)
     b = pb.rhs_normal(N + 1, 3)
     out = {}
     for t64 in (0, mode):
         f = H.HIFFactor(N, P, eps)
+        f.set_option("gemm_table", 0)
         f.set_option("gemm_tile64", t64)
         f.set_profiling(True)
         f.set_stencil(a, V, True)
@@ -535,8 +537,10 @@ def test_bf16x3_split_gemm_error_bound(H, transA):
     for m, n, k in shapes:
         A = rng.standard_normal((k, m) if transA else (m, k)) * rng.uniform(0.1, 10, size=1)
         pairs.append((A, rng.uniform(-1, 1, (k, n))))
-    got = H.gemm_batch(pairs, "mixed", transA, bf16x3=True)
-    plain = H.gemm_batch(pairs, "mixed", transA)
+    # both by the fixed rule: a loaded routing table may send depth <= 64
+    # products to the matrix-core kernel, where the split applies
+    got = H.gemm_batch(pairs, "mixed", transA, bf16x3=True, rule=True)
+    plain = H.gemm_batch(pairs, "mixed", transA, rule=True)
 
     def bf16(x):
         u = np.asarray(x, np.float32).view(np.uint32).astype(np.uint64)
