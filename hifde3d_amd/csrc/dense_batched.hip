@@ -1661,6 +1661,7 @@ void batched_gj_inverse(int nbox, const int* h_n, const long long* h_off, const 
                         const long long* d_off, T* d_A, int* d_info, GJWork& ws, GemmWork& gws,
                         hipStream_t s, bool sym) {
     if (nbox <= 0) return;
+    const int panel = ws.panel > 0 ? ws.panel : kGJPanel;
     // Boxes of at most kSmallGJ rows: one workgroup each.  Larger boxes are
     // listed in slots 0..nbig-1 for the blocked path.
     std::vector<int> bix;
@@ -1687,21 +1688,21 @@ void batched_gj_inverse(int nbox, const int* h_n, const long long* h_off, const 
     }
     const int nbig = (int)bix.size();
     if (nbig == 0) return;
-    const int osteps = (max_n + kGJPanel - 1) / kGJPanel;
+    const int osteps = (max_n + panel - 1) / panel;
     // Per outer step and slot: window size, offset of A_KK, leading dimension.
     std::vector<int> wn((size_t)osteps * nbig, 0), wld((size_t)osteps * nbig, 1);
     std::vector<long long> woff((size_t)osteps * nbig, 0), r_off(nbig);
     std::vector<long long> ro_off(nbig + 1, 0), cw_off(nbig + 1, 0);
     for (int j = 0; j < nbig; ++j) {
         const int b = bix[j], n = h_n[b];
-        r_off[j] = (long long)j * kGJB * kGJPanel;
-        const bool outer = n > kGJPanel;
-        ro_off[j + 1] = ro_off[j] + (outer ? (long long)kGJPanel * n : 0);
-        cw_off[j + 1] = cw_off[j] + (outer ? (long long)n * kGJPanel : 0);
-        for (int so = 0; so * kGJPanel < n; ++so) {
-            const int K0 = so * kGJPanel;
+        r_off[j] = (long long)j * kGJB * panel;
+        const bool outer = n > panel;
+        ro_off[j + 1] = ro_off[j] + (outer ? (long long)panel * n : 0);
+        cw_off[j + 1] = cw_off[j] + (outer ? (long long)n * panel : 0);
+        for (int so = 0; so * panel < n; ++so) {
+            const int K0 = so * panel;
             const size_t w = (size_t)so * nbig + j;
-            wn[w] = std::min(kGJPanel, n - K0);
+            wn[w] = std::min(panel, n - K0);
             woff[w] = h_off[b] + K0 + (long long)K0 * n;
             wld[w] = n;
         }
@@ -1713,14 +1714,14 @@ void batched_gj_inverse(int nbox, const int* h_n, const long long* h_off, const 
     ws.bix.upload(bix, s);
     ws.info.reserve(nbig);
     k_fill_int<<<(nbig + 255) / 256, 256, 0, s>>>(ws.info.p, nbig, 0);
-    ws.B.reserve((size_t)nbig * kGJB * kGJPanel * sizeof(T));
+    ws.B.reserve((size_t)nbig * kGJB * panel * sizeof(T));
     ws.D.reserve((size_t)nbig * kGJB * kGJB * sizeof(T));
-    // symmetric outer level: per box W (n x kGJPanel, ld n) and V^T
-    // (kGJPanel x n); the general one: RO and CW
+    // symmetric outer level: per box W (n x panel, ld n) and V^T
+    // (panel x n); the general one: RO and CW
     std::vector<long long> sw_off(nbig + 1, 0);
     for (int j = 0; j < nbig; ++j) {
         const int n = h_n[bix[j]];
-        sw_off[j + 1] = sw_off[j] + (sym && n > kGJPanel ? 2LL * n * kGJPanel : 0);
+        sw_off[j + 1] = sw_off[j] + (sym && n > panel ? 2LL * n * panel : 0);
     }
     if (sym) {
         ws.SW.reserve((size_t)std::max<long long>(sw_off[nbig], 1) * sizeof(T));
@@ -1779,7 +1780,7 @@ void batched_gj_inverse(int nbox, const int* h_n, const long long* h_off, const 
             else gemm_run<T, T>(plan, gh[gi++], A, B, C, alpha, beta, s);
         };
         for (int so = 0; so < osteps; ++so) {
-            const int K0 = so * kGJPanel;
+            const int K0 = so * panel;
             const size_t w0 = (size_t)so * nbig;
             const int* dwn = ws.wn.p + w0;
             const int* dwld = ws.wld.p + w0;
@@ -1793,7 +1794,7 @@ void batched_gj_inverse(int nbox, const int* h_n, const long long* h_off, const 
                 std::vector<SymCopy> mir;
                 for (int j = 0; j < nbig && !exec; ++j) {
                     const int kb = wn[w0 + j];
-                    if (kb == 0 || h_n[bix[j]] <= kGJPanel) continue;
+                    if (kb == 0 || h_n[bix[j]] <= panel) continue;
                     const long long wo = woff[w0 + j];
                     const int ld = wld[w0 + j];
                     mir.push_back({wo, wo, kb, kb, ld, ld, 1, 1, kAll, 1, 0, 0});
@@ -1854,7 +1855,7 @@ void batched_gj_inverse(int nbox, const int* h_n, const long long* h_off, const 
                 std::vector<SymCopy> sy;
                 for (int j = 0; j < nbig && !exec; ++j) {
                     const int kb = wn[w0 + j];
-                    if (kb == 0 || h_n[bix[j]] <= kGJPanel) continue;
+                    if (kb == 0 || h_n[bix[j]] <= panel) continue;
                     const long long wo = woff[w0 + j];
                     const int ld = wld[w0 + j];
                     sy.push_back({wo, wo, kb, kb, ld, ld, 1, 1, kAll, 1, 0, 0, 2});
@@ -1865,26 +1866,26 @@ void batched_gj_inverse(int nbox, const int* h_n, const long long* h_off, const 
                 for (int j = 0; j < nbig && !exec; ++j) {
                     const int n = h_n[bix[j]];
                     const int kb = wn[w0 + j];
-                    if (kb == 0 || n <= kGJPanel) continue;
+                    if (kb == 0 || n <= panel) continue;
                     const long long o = h_off[bix[j]];
-                    const long long swo = sw_off[j], svt = swo + (long long)n * kGJPanel;
+                    const long long swo = sw_off[j], svt = swo + (long long)n * panel;
                     const int K1 = K0 + kb, nq = n - K1, ldv = n;
-                    if (K0 > 0) gv.push_back({svt, o + K0, kb, K0, kGJPanel, n, 0, 0, 0, 1, 1, 0});
+                    if (K0 > 0) gv.push_back({svt, o + K0, kb, K0, panel, n, 0, 0, 0, 1, 1, 0});
                     if (nq > 0)
-                        gv.push_back({svt + (long long)K0 * kGJPanel, o + K1 + (long long)K0 * n,
-                                      kb, nq, kGJPanel, n, 1, 0, 0, 1, 1, 0});
-                    gw.push_back({svt, o + K0 + (long long)K0 * n, swo, n - kb, kb, kb, kGJPanel,
+                        gv.push_back({svt + (long long)K0 * panel, o + K1 + (long long)K0 * n,
+                                      kb, nq, panel, n, 1, 0, 0, 1, 1, 0});
+                    gw.push_back({svt, o + K0 + (long long)K0 * n, swo, n - kb, kb, kb, panel,
                                   n, ldv});
                     if (K0 > 0) {
-                        GemmDesc d{swo, svt, o, K0, K0, kb, ldv, kGJPanel, n};
+                        GemmDesc d{swo, svt, o, K0, K0, kb, ldv, panel, n};
                         d.lower = 1;
                         tl.push_back(d);
                     }
                     if (K0 > 0 && nq > 0)
-                        bq.push_back({swo + K0, svt, o + K1, nq, K0, kb, ldv, kGJPanel, n});
+                        bq.push_back({swo + K0, svt, o + K1, nq, K0, kb, ldv, panel, n});
                     if (nq > 0) {
-                        GemmDesc d{swo + K0, svt + (long long)K0 * kGJPanel,
-                                   o + K1 + (long long)K1 * n, nq, nq, kb, ldv, kGJPanel, n};
+                        GemmDesc d{swo + K0, svt + (long long)K0 * panel,
+                                   o + K1 + (long long)K1 * n, nq, nq, kb, ldv, panel, n};
                         d.lower = 1;
                         bq.push_back(d);
                         wb.push_back({o + K1 + (long long)K0 * n, swo + K0, nq, kb, n, ldv, 0, 0,
@@ -1909,7 +1910,7 @@ void batched_gj_inverse(int nbox, const int* h_n, const long long* h_off, const 
                 for (int j = 0; j < nbig; ++j) {
                     const int n = h_n[bix[j]];
                     const int kb = wn[w0 + j];
-                    if (kb == 0 || n <= kGJPanel) continue;
+                    if (kb == 0 || n <= panel) continue;
                     const long long o = h_off[bix[j]], dk = woff[w0 + j];
                     const long long ro = ro_off[j], co = cw_off[j];
                     const int j1 = K0 + kb;
@@ -1918,24 +1919,24 @@ void batched_gj_inverse(int nbox, const int* h_n, const long long* h_off, const 
                         if (len[q] == 0) continue;
                         // R(:, J) = D A(K, J)
                         g1.push_back({dk, o + K0 + (long long)lo[q] * n,
-                                      ro + (long long)lo[q] * kGJPanel, kb, len[q], kb, n, n,
-                                      kGJPanel});
+                                      ro + (long long)lo[q] * panel, kb, len[q], kb, n, n,
+                                      panel});
                         // C(I, :) = -A(I, K) D
                         g3.push_back({o + lo[q] + (long long)K0 * n, dk, co + lo[q], len[q], kb,
                                       kb, n, n, n});
                         // A(I, K) = C(I, :);  A(K, J) = R(:, J)
                         cps.push_back({co + lo[q], o + lo[q] + (long long)K0 * n, len[q], kb, n,
                                        n, 0});
-                        cps.push_back({ro + (long long)lo[q] * kGJPanel,
-                                       o + K0 + (long long)lo[q] * n, kb, len[q], kGJPanel, n, 1});
+                        cps.push_back({ro + (long long)lo[q] * panel,
+                                       o + K0 + (long long)lo[q] * n, kb, len[q], panel, n, 1});
                         maxc = std::max(maxc, std::max(kb, len[q]));
                         // A(I, J) -= A(I, K) R(:, J)
                         for (int r = 0; r < 2; ++r) {
                             if (len[r] == 0) continue;
                             g2.push_back({o + lo[r] + (long long)K0 * n,
-                                          ro + (long long)lo[q] * kGJPanel,
+                                          ro + (long long)lo[q] * panel,
                                           o + lo[r] + (long long)lo[q] * n, len[r], len[q], kb, n,
-                                          kGJPanel, n});
+                                          panel, n});
                         }
                     }
                 }
@@ -1956,7 +1957,7 @@ void batched_gj_inverse(int nbox, const int* h_n, const long long* h_off, const 
             std::vector<SymCopy> mir;
             for (int j = 0; j < nbig && !exec; ++j) {
                 const int n = h_n[bix[j]];
-                if (n <= kGJPanel) continue;
+                if (n <= panel) continue;
                 const long long o = h_off[bix[j]];
                 mir.push_back({o, o, n, n, n, n, 1, 1, kAll, 1, 0, 0});
             }

@@ -242,6 +242,12 @@ struct hifde_factor {
     bool split_schur = false;     // FP32 levels: Schur products as bf16x3 split products
     int host_threads = 0;         // OpenMP threads of the host planning (0: the caller's)
     bool keep_workspaces = true;  // keep the factorization workspaces for the next one
+    // Keep every box's A22 beside its inverse (the forward Apply F reads it).
+    // Off: A22 is gathered straight into the inverse's buffer and inverted
+    // there (no copy), and A22 buffers of earlier factorizations are freed;
+    // Apply F^{-1}, PCG and every solve are unchanged, Apply F is refused.
+    bool keep_a22 = true;
+    bool a22_kept = false;  // the current factor holds A22 (set by factor / load)
     int gram_min_n = 16;          // faces with more columns use the Gram path
     double gram_min_eps = 1e-7;   // below this tolerance every face uses Householder
     DevBuf<double> pcg_buf, pcg_part, pcg_scalar, apply_tmp, apply_side;
@@ -715,14 +721,15 @@ struct hifde_factor {
         const int nb = B.nbox;
         int max_r = 0;
         for (int x : B.r) max_r = std::max(max_r, x);
-        B.A22inv.reserve(B.tot_rr * sizeof(T));
+        if (keep_a22) {
+            B.A22inv.reserve(B.tot_rr * sizeof(T));
+            if (B.tot_rr)
+                HIFDE_CHECK(hipMemcpyAsync(B.A22inv.p, B.A22.p, B.tot_rr * sizeof(T),
+                                           hipMemcpyDeviceToDevice, stream));
+        }  // else A22 was gathered into B.A22inv (a22_dest) and is inverted there
         B.X.reserve(B.tot_rs * sizeof(T));
-        T* A22 = reinterpret_cast<T*>(B.A22.p);
         T* Ainv = reinterpret_cast<T*>(B.A22inv.p);
         T* X = reinterpret_cast<T*>(B.X.p);
-        if (B.tot_rr)
-            HIFDE_CHECK(hipMemcpyAsync(Ainv, A22, B.tot_rr * sizeof(T), hipMemcpyDeviceToDevice,
-                                       stream));
         ws_info.reserve(nb);
         HIFDE_CHECK(hipMemsetAsync(ws_info.p, 0, nb * sizeof(int), stream));
         const double w = sizeof(T);
@@ -775,6 +782,15 @@ struct hifde_factor {
                                      accumulate ? (T)1 : (T)0, gemm_ws, stream);
         }
         timed(ST_GEMM, t0);
+    }
+
+    // Where a batch's A22 blocks are gathered: their own buffer, or with
+    // keep_a22 off the inverse's buffer, which is then inverted in place.
+    template <class T>
+    T* a22_dest(BoxBatch& B) {
+        DevBuf<unsigned char>& d = keep_a22 ? B.A22 : B.A22inv;
+        d.reserve(B.tot_rr * sizeof(T));
+        return reinterpret_cast<T*>(d.p);
     }
 
     void mark_eliminated(const BoxBatch& B) {
@@ -843,7 +859,7 @@ struct hifde_factor {
             level_cells_elem<T>(level, t0);
             return;
         }
-        B.A22.reserve(B.tot_rr * sizeof(T));
+        T* A22 = a22_dest<T>(B);
         ws_a.reserve(B.tot_rs * sizeof(T));
         ws_b.reserve(B.tot_rs * sizeof(T));
         ws_c.reserve(B.tot_ss * sizeof(T));
@@ -857,8 +873,7 @@ struct hifde_factor {
         {
             Prof pr(this, fam<T>("gather"), 0.0, (double)B.tot_rr * (sizeof(T) + 16));
             table_gather<T>(tabs[cur], B.nbox, B.d_red_gid.p, B.d_red_off.p, B.d_red_gid.p,
-                            B.d_red_off.p, B.d_off_rr.p, B.tot_rr, reinterpret_cast<T*>(B.A22.p),
-                            stream);
+                            B.d_red_off.p, B.d_off_rr.p, B.tot_rr, A22, stream);
         }
         {
             Prof pr(this, fam<T>("gather"), 0.0, (double)B.tot_rs * (sizeof(T) + 16));
@@ -921,12 +936,11 @@ struct hifde_factor {
     void level_cells_elem(int level, clk::time_point& t0) {
         BoxBatch& B = cells[level];
         const int nb = B.nbox;
-        B.A22.reserve(B.tot_rr * sizeof(T));
+        T* A22 = a22_dest<T>(B);
         ws_a.reserve(B.tot_rs * sizeof(T));
         if (!sym_on) ws_b.reserve(B.tot_rs * sizeof(T));
         const int ne = 1 - elem_cur;
         elem[ne].reserve(std::max<long long>(B.tot_ee, 1));
-        T* A22 = reinterpret_cast<T*>(B.A22.p);
         T* A21 = reinterpret_cast<T*>(ws_a.p);
         T* A12 = sym_on ? nullptr : reinterpret_cast<T*>(ws_b.p);
         double* E = elem[ne].p;
@@ -1274,7 +1288,7 @@ struct hifde_factor {
         timed(ST_PLAN, t0);
         upload_batch_index(B);
         timed(ST_UPLOAD, t0);
-        B.A22.reserve(B.tot_rr * sizeof(T));
+        T* A22 = a22_dest<T>(B);
         ws_a.reserve(B.tot_rs * sizeof(T));
         ws_b.reserve(B.tot_rs * sizeof(T));
         ws_c.reserve(B.tot_ss * sizeof(T));
@@ -1289,7 +1303,7 @@ struct hifde_factor {
                 by += 8 * (n * n + k * q) + sizeof(T) * (q * q + 2 * k * q);
             }
             Prof pr(this, "face_transform", fl, by);
-            face_transform<T>(fd, B, A21, A12);
+            face_transform<T>(fd, B, A22, A21, A12);
         }
         // W (k x q) kept for Apply, packed at the r*s offsets.
         B.W.reserve(B.tot_rs * sizeof(T));
@@ -1329,7 +1343,7 @@ struct hifde_factor {
     // C is first permuted to skeleton-then-redundant order; all products are
     // FP64 batched GEMMs on sub-blocks of it; the blocks are then packed in T.
     template <class T>
-    void face_transform(const FaceDev& fd, BoxBatch& B, T* A21, T* A12) {
+    void face_transform(const FaceDev& fd, BoxBatch& B, T* A22, T* A21, T* A12) {
         const FacePlan& F = fd.F;
         const std::vector<long long>& offC = fd.offC;
         const int nf = B.nbox;
@@ -1355,7 +1369,7 @@ struct hifde_factor {
         batched_gemm_desc<double>(false, d2, ws_T.p, ws_W.p, ws_T.p, -1.0, 1.0, gemm_ws, stream);
         batched_gemm_desc<double>(true, d3, ws_W.p, ws_T.p, ws_T.p, -1.0, 1.0, gemm_ws, stream);
         batched_gemm_desc<double>(true, d4, ws_W.p, ws_T.p, ws_T.p, -1.0, 1.0, gemm_ws, stream);
-        copy_blocks<T>(c22, B.off_rr, ws_T.p, reinterpret_cast<T*>(B.A22.p), gemm_ws, stream);
+        copy_blocks<T>(c22, B.off_rr, ws_T.p, A22, gemm_ws, stream);
         copy_blocks<T>(c21, B.off_rs, ws_T.p, A21, gemm_ws, stream);
         if (!sym_on) copy_blocks<T>(c12, B.off_rs, ws_T.p, A12, gemm_ws, stream);
     }
@@ -1457,6 +1471,10 @@ struct hifde_factor {
         stager.synced();
         resolve_profile();
         factored = true;
+        a22_kept = keep_a22;
+        if (!keep_a22)
+            for (std::vector<BoxBatch>* v : {&cells, &faces})
+                for (BoxBatch& B : *v) B.A22.release();
         if (!keep_workspaces) release_workspaces();
     }
 
@@ -1611,9 +1629,15 @@ struct hifde_factor {
         (void)sg;
     }
 
+    void check_apply(bool inv) const {
+        if (!factored) throw Error(HIFDE_ERR_STATE, "apply: factor first");
+        if (!inv && !a22_kept)
+            throw Error(HIFDE_ERR_STATE,
+                        "apply: the forward Apply F needs the A22 blocks (option keep_a22 was 0)");
+    }
     void apply_dev(double* d_u, int nrhs, long long ldu, bool inv) {
         TimerGuard tguard(this);
-        if (!factored) throw Error(HIFDE_ERR_STATE, "apply: factor first");
+        check_apply(inv);
         if (nrhs >= 1) apply_sweeps(d_u, nrhs, ldu, inv);
         resolve_profile();
     }
@@ -2119,7 +2143,7 @@ struct hifde_factor {
     // blocks, so that a later process can apply and solve without factoring.
     // Binary, native byte order; the header names the grid, tolerance and
     // precision, and a load checks them against the object's own.
-    static constexpr char kFileMagic[8] = {'H', 'I', 'F', 'D', 'E', 'F', '0', '1'};
+    static constexpr char kFileMagic[8] = {'H', 'I', 'F', 'D', 'E', 'F', '0', '2'};
     static constexpr char kFileEnd[8] = {'H', 'I', 'F', 'D', 'E', 'E', 'N', 'D'};
     struct File {
         FILE* fp = nullptr;
@@ -2179,7 +2203,7 @@ struct hifde_factor {
         HIFDE_CHECK(hipStreamSynchronize(stream));
         File fl(path, true);
         fl.put(kFileMagic, 8);
-        const int hdr[6] = {N, P, precision, L, quirk ? 1 : 0, lu_info};
+        const int hdr[7] = {N, P, precision, L, quirk ? 1 : 0, lu_info, a22_kept ? 1 : 0};
         fl.put(hdr, sizeof hdr);
         fl.put(&eps, sizeof eps);
         // operator, as COO in CSR order
@@ -2201,7 +2225,7 @@ struct hifde_factor {
                 fl.put_vec(B.skel_gid);
                 fl.put_vec(B.skel_pass);
                 if (B.nbox == 0) continue;
-                put_dev(fl, B.A22.p, B.tot_rr * B.esize);
+                if (a22_kept) put_dev(fl, B.A22.p, B.tot_rr * B.esize);
                 put_dev(fl, B.A22inv.p, B.tot_rr * B.esize);
                 put_dev(fl, B.X.p, B.tot_rs * B.esize);
                 if (kind == 1) put_dev(fl, B.W.p, B.tot_rs * B.esize);
@@ -2214,13 +2238,15 @@ struct hifde_factor {
         char magic[8];
         fl.get(magic, 8);
         if (std::memcmp(magic, kFileMagic, 8) != 0) throw Error(HIFDE_ERR_ARG, "not a factor file");
-        int hdr[6];
+        int hdr[7];
         double feps = 0;
         fl.get(hdr, sizeof hdr);
         fl.get(&feps, sizeof feps);
         if (hdr[0] != N || hdr[1] != P || hdr[2] != precision || hdr[3] != L || feps != eps ||
             hdr[4] != (quirk ? 1 : 0))
             throw Error(HIFDE_ERR_ARG, "load: the file holds another grid, tolerance or precision");
+        if (hdr[6] != 0 && hdr[6] != 1) throw Error(HIFDE_ERR_STATE, "factor file corrupt");
+        const bool file_a22 = hdr[6] == 1;
         const long long nd = (long long)NC * NC * NC;
         std::vector<int> rows, cols;
         std::vector<double> vals;
@@ -2251,6 +2277,16 @@ struct hifde_factor {
                 fl.get_vec(B.skel_pass, nd);
                 if ((int)B.r.size() != B.nbox || (int)B.s.size() != B.nbox)
                     throw Error(HIFDE_ERR_STATE, "factor file corrupt");
+                // every size within the grid, every prefix sum within int
+                // range, before any offset is formed
+                long long sr = 0, ss = 0;
+                for (int b = 0; b < B.nbox; ++b) {
+                    if (B.r[b] < 0 || B.r[b] > nd || B.s[b] < 0 || B.s[b] > nd)
+                        throw Error(HIFDE_ERR_STATE, "factor file corrupt");
+                    sr += B.r[b];
+                    ss += B.s[b];
+                }
+                if (sr > nd || ss > nd) throw Error(HIFDE_ERR_STATE, "factor file corrupt");
                 B.red_off = prefix_i(B.r);
                 B.skel_off = prefix_i(B.s);
                 if ((long long)B.red_gid.size() != B.red_off.back() ||
@@ -2269,7 +2305,8 @@ struct hifde_factor {
                 B.tot_ss = B.off_ss.back();
                 if (B.nbox == 0) continue;
                 upload_batch_index(B);
-                get_dev(fl, B.A22, B.tot_rr * B.esize);
+                if (file_a22) get_dev(fl, B.A22, B.tot_rr * B.esize);
+                else B.A22.release();
                 get_dev(fl, B.A22inv, B.tot_rr * B.esize);
                 get_dev(fl, B.X, B.tot_rs * B.esize);
                 if (kind == 1) get_dev(fl, B.W, B.tot_rs * B.esize);
@@ -2278,6 +2315,7 @@ struct hifde_factor {
         if (std::memcmp(magic, kFileEnd, 8) != 0) throw Error(HIFDE_ERR_STATE, "factor file corrupt");
         HIFDE_CHECK(hipStreamSynchronize(stream));
         lu_info = hdr[5];
+        a22_kept = file_a22;
         plan0_valid = false;  // the level-0 index is rebuilt by the next factorization
         factored = true;
     }
@@ -2886,6 +2924,7 @@ int hifde_apply_device_noalloc(hifde_factor* f, double* d_u, int inverse) {
             throw Error(HIFDE_ERR_SIZE, "apply_noalloc: (N+1)^3 = " + std::to_string(n) +
                                             " exceeds the scratch limit 128^3");
         if (!f->factored) throw Error(HIFDE_ERR_STATE, "apply_noalloc: factor first");
+        f->check_apply(inverse != 0);
         if (!f->scratch_dev) throw Error(HIFDE_ERR_STATE, "apply_noalloc: no scratch");
         OwnerGuard og(f->stream);
         struct NoAlloc {
@@ -3053,6 +3092,8 @@ int hifde_box_data(const hifde_factor* f, int level, int kind, int box, int* red
             if (B.esize == sizeof(float)) read_dev<float>(buf, off, cnt, o, f->stream);
             else read_dev<double>(buf, off, cnt, o, f->stream);
         };
+        if (A22 && !f->a22_kept)
+            throw Error(HIFDE_ERR_STATE, "box_data: A22 was not kept (option keep_a22)");
         rd(B.A22, B.off_rr[box], (long long)r * r, A22);
         rd(B.A22inv, B.off_rr[box], (long long)r * r, A22inv);
         rd(B.X, B.off_rs[box], (long long)r * s, X);
@@ -3097,6 +3138,12 @@ int hifde_set_option(hifde_factor* f, const char* name, double value) {
             if (value < 0 || value > 2) throw Error(HIFDE_ERR_ARG, "gemm_tile64: 0, 1 or 2");
             f->gemm_ws.plan.tile64 = f->gj_ws.plan.tile64 = (int)value;
         }
+        else if (n == "gj_window") {
+            const int w = (int)value;
+            if (w != 0 && (w < 64 || w > 4096 || w % 64 != 0))
+                throw Error(HIFDE_ERR_ARG, "gj_window: 0 or a multiple of 64 in [64, 4096]");
+            f->gj_ws.panel = w;
+        }
         else if (n == "host_threads") f->host_threads = std::max(0, (int)value);
         else if (n == "gemm_record") f->gemm_record = value != 0;
         else if (n == "gemm_table") {
@@ -3108,6 +3155,7 @@ int hifde_set_option(hifde_factor* f, const char* name, double value) {
                     for (int op = 0; op < 5; ++op) B.rplan_R[op] = 0;  // re-planned
             f->clear_graphs();
         }
+        else if (n == "keep_a22") f->keep_a22 = value != 0;
         else if (n == "keep_workspaces") {
             f->keep_workspaces = value != 0;
             if (!f->keep_workspaces && f->factored) f->release_workspaces();
@@ -3173,7 +3221,8 @@ long long hifde_factor_bytes(const hifde_factor* f) {
     long long tot = 0;
     for (int lv = 0; lv < (int)f->cells.size(); ++lv) {
         for (const BoxBatch* B : {&f->cells[lv], &f->faces[lv]}) {
-            tot += (long long)(2 * B->tot_rr + B->tot_rs) * (long long)B->esize;
+            tot += (long long)((f->a22_kept ? 2 : 1) * B->tot_rr + B->tot_rs) *
+                   (long long)B->esize;
             if (B == &f->faces[lv]) tot += (long long)B->tot_rs * (long long)B->esize;
         }
     }

@@ -364,6 +364,8 @@ struct GJWork {
     DevBuf<unsigned char> SW;         // symmetric path: V, W (n x kGJPanel), V^T per box
     DevBuf<SymCopy> sdesc;
     DevBuf<long long> stile;          // per copy launch: tile prefix of its descriptors
+    // outer window width (a multiple of 64); 0: the default (512)
+    int panel = 0;
 };
 
 // C = alpha op(A) B + beta C per descriptor (op(A) = A^T when transA).  C

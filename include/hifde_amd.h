@@ -230,7 +230,8 @@ int hifde_box_count(const hifde_factor* f, int level, int kind);
 int hifde_box_sizes(const hifde_factor* f, int level, int kind, int box, int* out);
 /* FactorData of one box (include/data.hpp:83-110): global ids of red and
  * skel DOFs in the box's order, A_22, A_22^{-1}, X (r x s) and, for faces,
- * W (s x r).  Any pointer may be null. */
+ * W (s x r).  Any pointer may be null; A22 must be null when the factor
+ * was made with option "keep_a22" = 0 (HIFDE_ERR_STATE otherwise). */
 int hifde_box_data(const hifde_factor* f, int level, int kind, int box, int* red_gid,
                    int* skel_gid, double* A22, double* A22inv, double* X, double* W);
 /* Wall-clock seconds of the last factor by stage; returns the number of
@@ -277,6 +278,15 @@ const char* hifde_stage_name(int stage);
  *                  them after each factorization (and at once if factored),
  *                  leaving the device memory to the factor and the solves
  *                  (at NC=256, P=8 the workspaces exceed the factor)
+ *   "keep_a22"     1 (default): every box keeps A22 beside A22^{-1} (as the
+ *                  reference's FactorData); 0: each box is inverted where it
+ *                  was gathered and the A22 blocks are freed when the
+ *                  factorization ends (the factor shrinks by a third or
+ *                  more); Apply F^{-1}, PCG and every solve give the same
+ *                  bits, the forward Apply F returns HIFDE_ERR_STATE
+ *   "gj_window"    0 (default: 512) or a multiple of 64 in [64, 4096]: the
+ *                  outer window width of the blocked inverses (same results
+ *                  to rounding)
  *   "host_threads" OpenMP threads of the host-side planning in
  *                  hifde_factor_run (0, the default: the calling thread's
  *                  setting); for several factor objects run from several

@@ -897,6 +897,75 @@ def test_factor_load_rejects_other_problem_and_bad_files(H, oracle_mod, tmp_path
     g.close()
 
 
+def test_factor_load_rejects_negative_box_sizes(H, oracle_mod, tmp_path):
+    """A file whose first batch lists a negative box size, with the other
+    sizes moved so that every total still matches, is refused before any
+    block is read."""
+    import struct
+    f, _, _ = _both(H, oracle_mod, 31, 4, 1e-4, "pow10", True, "mixed")
+    path = tmp_path / "factor.bin"
+    f.save(str(path))
+    data = bytearray(path.read_bytes())
+    o = 8 + 7 * 4 + 8  # magic, header, eps
+    for _ in range(3):  # operator rows, cols (int32), vals (float64)
+        n, = struct.unpack_from("<q", data, o)
+        o += 8 + n * (8 if _ == 2 else 4)
+    o += 8  # first batch: nbox, esize
+    n, = struct.unpack_from("<q", data, o)
+    assert n >= 2
+    r0, r1 = struct.unpack_from("<ii", data, o + 8)
+    struct.pack_into("<ii", data, o + 8, -r0 - 1, r1 + 2 * r0 + 1)
+    bad = tmp_path / "neg.bin"
+    bad.write_bytes(bytes(data))
+    g = H.HIFFactor(31, 4, 1e-4, precision="mixed")
+    with pytest.raises(H.HifdeError):
+        g.load(str(bad))
+    assert g.load(str(path)) == 0
+    f.close()
+    g.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_keep_a22_off_same_solves_smaller_factor(H, oracle_mod, precision, tmp_path):
+    """Option keep_a22 = 0: the factor drops every A22 block (factor_bytes
+    shrinks by exactly their size), and Apply F^{-1}, the multi-vector apply
+    and PCG give the same bits as the default factor, after a second
+    factorization of the same object too; the forward Apply F and reading A22
+    are refused; save/load carries the choice."""
+    import torch
+    a, V, coo, A = _coo(oracle_mod, 31, "pow10", True)
+    b = pb.rhs_normal(32, 7)
+    out = {}
+    for keep in (1, 0):
+        f = H.HIFFactor(31, 4, 1e-4, precision=precision)
+        f.set_option("keep_a22", keep)
+        f.set_stencil(a, V, True)
+        for _ in range(2):
+            assert f.Factor() == 0
+        U = torch.from_numpy(np.stack([pb.rhs_normal(32, 30 + c) for c in range(3)])).cuda()
+        f.apply_device(U.data_ptr(), 3, 32 ** 3, True)
+        torch.cuda.synchronize()
+        out[keep] = (f.Apply(b, True), U.cpu().numpy(), f.pcg(b, tol=1e-10), f.factor_bytes(),
+                     f.level_counts())
+        if keep == 0:
+            with pytest.raises(H.HifdeError):
+                f.Apply(b, False)
+            path = str(tmp_path / "noa22.bin")
+            f.save(path)
+            g = H.HIFFactor(31, 4, 1e-4, precision=precision)
+            assert g.load(path) == 0
+            assert np.array_equal(g.Apply(b, True), out[0][0])
+            with pytest.raises(H.HifdeError):
+                g.Apply(b, False)
+            g.close()
+        f.close()
+    assert out[0][4] == out[1][4]
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    (x0, i0, r0), (x1, i1, r1) = out[0][2], out[1][2]
+    assert i0 == i1 and r0 == r1 and np.array_equal(x0, x1)
+    assert out[0][3] < out[1][3]
+
+
 def test_leaf8_reaction_term_block_pcg_n128(H, oracle_mod):
     """The large-leaf config at one eighth of its size: NC=128 with leaf P=8
     (level-0 boxes of 343 interior DOFs take the blocked inverse), ID
