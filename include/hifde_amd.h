@@ -39,10 +39,10 @@ typedef struct hifde_factor hifde_factor;
 
 /* precision of the factorization */
 #define HIFDE_FP64 0  /* every block in FP64 (the reference's HIFFactor<double>) */
-#define HIFDE_MIXED 1 /* Schur/LU blocks and stored factors of the lower levels
-                         in FP32 (all but the top two by default); ID pivots,
-                         rank decisions, the level matrix and the top levels
-                         in FP64 */
+#define HIFDE_MIXED 1 /* Schur/LU blocks and stored factors in FP32 (every
+                         level when epsilon >= 1e-5, else all but the top
+                         two; option "fp32_levels"); ID pivots, rank
+                         decisions and the level matrix in FP64 */
 
 /* flags */
 #define HIFDE_FLAG_FRONT_QUIRK 1 /* reproduce the FRONT-face guard of
@@ -247,7 +247,8 @@ const char* hifde_stage_name(int stage);
  *                  column-pivoted QR; default 1e-7
  *   "fp32_levels"  HIFDE_MIXED only: levels 0 .. fp32_levels-1 are factored
  *                  and stored in FP32, the levels above in FP64; default
- *                  num_levels - 2 (the two top levels in FP64)
+ *                  num_levels (every level in FP32) when epsilon >= 1e-5,
+ *                  else num_levels - 2 (the two top levels in FP64)
  *   "symmetric"    -1 (default): when the assembled operator equals its
  *                  transpose bitwise, form only the lower triangle of each
  *                  Schur complement S = -A21^T A22^{-1} A21 (and store the
