@@ -24,6 +24,7 @@ sys.path.insert(0, ROOT)
 METRIC = "factor s + PCG solves/s at 128^3 to 1e-10 resid: mixed vs FP64 GPU vs CPU"
 PEAK = {"hbm_GBs": 8000.0, "fp64_TF": 78.6, "fp32_TF": 157.3}
 PMC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_n128.json")
+PMC_UTIL_FILE = os.path.join(ROOT, "profiles", "pmc_mfma_n128.json")
 
 
 def log(*a):
@@ -104,6 +105,49 @@ def pmc_json(fetch_csv, write_csv, calfetch_csv, calwrite_csv, workload):
             "calibration": (f"1 GiB device copy (__amd_rocclr_copyBuffer): FETCH_SIZE "
                             f"{cf[1] / cf[0]:.0f} KiB, WRITE_SIZE {cw[1] / cw[0]:.0f} KiB per dispatch"),
             "kernels": kernels}
+
+
+def pmc_util_json(mfma_csv, tcc_csv, workload):
+    """Per-kernel matrix-core utilisation and L2 hit rate of two rocprofv3
+    --pmc runs: (1) MfmaUtil (rocprofv3's derived counter: MFMA busy cycles
+    over the active cycles of all SIMDs) with the SQ_INSTS_VALU_MFMA_MOPS_F32
+    and _F64 op counts, (2) TCC_HIT_sum and TCC_MISS_sum.  A kernel's
+    utilisation is the duration-weighted mean over its dispatches."""
+    import csv
+
+    def load(path):
+        out = {}
+        for r in csv.DictReader(open(path)):
+            k = short_kernel_name(r["Kernel_Name"])
+            did = r.get("Dispatch_Id") or r.get("Correlation_Id")
+            d = out.setdefault(k, {}).setdefault(did, {})
+            if r.get("Start_Timestamp") and r.get("End_Timestamp"):
+                d["_ns"] = float(r["End_Timestamp"]) - float(r["Start_Timestamp"])
+            d[r["Counter_Name"]] = d.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+        return out
+
+    m, t = load(mfma_csv), load(tcc_csv)
+    kernels = {}
+    for k, disp in m.items():
+        f32 = 512.0 * sum(d.get("SQ_INSTS_VALU_MFMA_MOPS_F32", 0.0) for d in disp.values())
+        f64 = 512.0 * sum(d.get("SQ_INSTS_VALU_MFMA_MOPS_F64", 0.0) for d in disp.values())
+        if f32 + f64 == 0:
+            continue
+        ns = sum(d.get("_ns", 0.0) for d in disp.values())
+        e = {"dispatches": len(disp), "mfma_flops_f32": f32, "mfma_flops_f64": f64}
+        if ns > 0:
+            e["avg_us"] = ns / 1e3 / len(disp)
+            e["mfma_tflops"] = (f32 + f64) / ns / 1e3
+            e["mfma_util_pct"] = sum(d.get("MfmaUtil", 0.0) * d.get("_ns", 0.0)
+                                     for d in disp.values()) / ns
+        tc = t.get(k)
+        if tc:
+            hit = sum(d.get("TCC_HIT_sum", 0.0) for d in tc.values())
+            miss = sum(d.get("TCC_MISS_sum", 0.0) for d in tc.values())
+            if hit + miss > 0:
+                e["l2_hit_pct"] = 100.0 * hit / (hit + miss)
+        kernels[k] = e
+    return {"workload": workload, "kernels": kernels}
 
 
 def copy_calibration():
@@ -543,6 +587,10 @@ def main():
     ap.add_argument("--keep-a22", type=int, default=0, choices=(0, 1),
                     help="factor option keep_a22: 0 (default) stores only what F^-1 needs "
                          "(the solves never read A22); 1 keeps A22 beside its inverse")
+    ap.add_argument("--pmc-util", nargs=2, metavar=("MFMA", "TCC"),
+                    help="write the per-kernel matrix-core utilisation and L2 hit rate of two "
+                         "rocprofv3 --pmc counter_collection.csv files to "
+                         "profiles/pmc_mfma_n128.json")
     ap.add_argument("--pmc-json", nargs=4, metavar=("FETCH", "WRITE", "CALFETCH", "CALWRITE"),
                     help="write the per-kernel HBM traffic summary of four rocprofv3 --pmc "
                          "counter_collection.csv files to profiles/pmc_traffic_n128.json")
@@ -555,6 +603,12 @@ def main():
                        "(one factor + one PCG solve, n=128 mixed)")
         json.dump(out, open(PMC_FILE, "w"), indent=1, sort_keys=True)
         print(f"wrote {PMC_FILE}: {len(out['kernels'])} kernels")
+        return
+    if args.pmc_util:
+        out = pmc_util_json(*args.pmc_util, workload="bench.py --steps 1 --warmup 0 --no-cpu "
+                            "--no-fp64 (one factor + one PCG solve, n=128 mixed)")
+        json.dump(out, open(PMC_UTIL_FILE, "w"), indent=1, sort_keys=True)
+        print(f"wrote {PMC_UTIL_FILE}: {len(out['kernels'])} kernels")
         return
     if args.cpu_validate:
         cpu_validate(args)

@@ -267,6 +267,9 @@ struct hifde_factor {
     bool local_ok = false;   // operator couplings stay inside level-0 cell closures
     int elem_option = -1;    // -1 auto, 0 off (hash-table level matrix)
     bool elem_on = false;
+    // FP32 elements when every level is FP32 (-1 auto, 0 off: FP64 elements)
+    int elem_f32_option = -1;
+    bool elem32 = false;
     bool plan0_valid = false;  // level-0 cell and face index built and on the device
     DevBuf<double> elem[2];  // FP64 element blocks of the previous / current level
     int elem_cur = 0;
@@ -856,7 +859,13 @@ struct hifde_factor {
         if (!cached) upload_batch_index(B);
         timed(ST_UPLOAD, t0);
         if (elem_on) {
-            level_cells_elem<T>(level, t0);
+            if constexpr (std::is_same<T, float>::value) {
+                if (elem32) {
+                    level_cells_elem<float, float>(level, t0);
+                    return;
+                }
+            }
+            level_cells_elem<T, double>(level, t0);
             return;
         }
         T* A22 = a22_dest<T>(B);
@@ -932,7 +941,16 @@ struct hifde_factor {
     // A21 (A12) and the element over [skel | other] are assembled from the
     // operator (level 0) or by extend-add of the eight children's elements;
     // S = -A12 A22^{-1} A21 is then added to the element by the Schur GEMM.
-    template <class T>
+    // The element buffers hold TE values (elements are FP32 only when every
+    // level is FP32): reserved in doubles, rounded up.
+    template <class TE>
+    TE* elem_ptr(int k, long long count = -1) {
+        if (count >= 0)
+            elem[k].reserve(std::max<long long>((count * (long long)sizeof(TE) + 7) / 8, 1));
+        return reinterpret_cast<TE*>(elem[k].p);
+    }
+
+    template <class T, class TE>
     void level_cells_elem(int level, clk::time_point& t0) {
         BoxBatch& B = cells[level];
         const int nb = B.nbox;
@@ -940,18 +958,17 @@ struct hifde_factor {
         ws_a.reserve(B.tot_rs * sizeof(T));
         if (!sym_on) ws_b.reserve(B.tot_rs * sizeof(T));
         const int ne = 1 - elem_cur;
-        elem[ne].reserve(std::max<long long>(B.tot_ee, 1));
         T* A21 = reinterpret_cast<T*>(ws_a.p);
         T* A12 = sym_on ? nullptr : reinterpret_cast<T*>(ws_b.p);
-        double* E = elem[ne].p;
-        const double w = sizeof(T);
+        TE* E = elem_ptr<TE>(ne, B.tot_ee);
+        const double w = sizeof(T), we = sizeof(TE);
         const double fr_bytes =
-            ((double)B.tot_rr + (sym_on ? 1.0 : 2.0) * B.tot_rs) * w + (double)B.tot_ee * 8;
+            ((double)B.tot_rr + (sym_on ? 1.0 : 2.0) * B.tot_rs) * w + (double)B.tot_ee * we;
         if (level == 0) {
             const int c = NC / P;
             if (!plan0_valid) init_level0_index(B);
             Prof pr(this, fam<T>("assemble"), 0.0, fr_bytes + (double)nnz_op * 12.0);
-            frontal_init<T>(nb, d_init.p, d_etab.p, d_egid.p, d_rp.p, d_ci.p, d_v.p, NC, P, c,
+            frontal_init<T, TE>(nb, d_init.p, d_etab.p, d_egid.p, d_rp.p, d_ci.p, d_v.p, NC, P, c,
                             A22, A21, A12, E, stream);
         } else {
             const BoxBatch& Cb = cells[level - 1];
@@ -994,19 +1011,19 @@ struct hifde_factor {
                 }
             d_emap.upload(maps, stream);
             timed(ST_PLAN, t0);
-            Prof pr(this, fam<T>("assemble"), 0.0, fr_bytes + (double)Cb.tot_ee * (8 + 4.0));
+            Prof pr(this, fam<T>("assemble"), 0.0, fr_bytes + (double)Cb.tot_ee * (we + 4.0));
             fill_zero<T>(A22, B.tot_rr, stream);
             fill_zero<T>(A21, B.tot_rs, stream);
             if (A12) fill_zero<T>(A12, B.tot_rs, stream);
-            fill_zero<double>(E, B.tot_ee, stream);
-            extend_add<T>(pass, d_ext, d_eoff, d_emap.p, elem[elem_cur].p, A22, A21, A12, E,
-                          stream);
+            fill_zero<TE>(E, B.tot_ee, stream);
+            extend_add<T, TE>(pass, d_ext, d_eoff, d_emap.p, elem_ptr<TE>(elem_cur), A22, A21,
+                              A12, E, stream);
         }
         elem_cur = ne;
         timed(ST_GATHER, t0);
         std::vector<int> eld(nb);
         for (int b = 0; b < nb; ++b) eld[b] = std::max(B.s[b] + B.o[b], 1);
-        schur_batch<T, double>(B, A21, A12, E, B.off_ee.data(), eld.data(), true, t0);
+        schur_batch<T, TE>(B, A21, A12, E, B.off_ee.data(), eld.data(), true, t0);
         if (sym_on) {
             std::vector<MirrorDesc> md;
             for (int b = 0; b < nb; ++b)
@@ -1125,9 +1142,14 @@ struct hifde_factor {
         ws_T.reserve(offC.back());
         if (elem_on) {
             // A(rows, F) and A(F, F) summed from the two adjacent cells' elements
-            Prof pr(this, "gather_f64", 0.0, (double)(offM.back() + offC.back()) * (8 + 2.0 * 8));
-            face_gather(nf, fd.gd_cols, fd.gd.p, fd.gd_off.p, fd.rpos.p, fd.cpos.p,
-                        elem[elem_cur].p, ws_M.p, ws_C.p, stream);
+            Prof pr(this, "gather_f64", 0.0,
+                    (double)(offM.back() + offC.back()) * (8 + 2.0 * (elem32 ? 4 : 8)));
+            if (elem32)
+                face_gather<float>(nf, fd.gd_cols, fd.gd.p, fd.gd_off.p, fd.rpos.p, fd.cpos.p,
+                                   elem_ptr<float>(elem_cur), ws_M.p, ws_C.p, stream);
+            else
+                face_gather<double>(nf, fd.gd_cols, fd.gd.p, fd.gd_off.p, fd.rpos.p, fd.cpos.p,
+                                    elem_ptr<double>(elem_cur), ws_M.p, ws_C.p, stream);
         } else {
             {
                 Prof pr(this, "gather_f64", 0.0, (double)offM.back() * (8 + 16));
@@ -1288,6 +1310,15 @@ struct hifde_factor {
         timed(ST_PLAN, t0);
         upload_batch_index(B);
         timed(ST_UPLOAD, t0);
+        if (B.tot_rr == 0) {
+            // Every face keeps all its columns (full numerical rank, as every
+            // level-0 face at P = 4 and eps 1e-4): nothing is eliminated,
+            // SchurAfterID's Schur complement is zero and the level matrix
+            // is unchanged (src/Factor.cpp:113-153, 262-311 with an empty
+            // redundant set), so the transform, the Schur step and the
+            // update are skipped.
+            return;
+        }
         T* A22 = a22_dest<T>(B);
         ws_a.reserve(B.tot_rs * sizeof(T));
         ws_b.reserve(B.tot_rs * sizeof(T));
@@ -1328,7 +1359,17 @@ struct hifde_factor {
             }
             d_spos.upload(spos, stream);
             Prof pr(this, fam<T>("face_add"), 0.0, (double)B.tot_ss * (sizeof(T) + 16));
-            face_add<T>(ad, d_fa, d_eoff, d_spos.p, S, elem[elem_cur].p, sym_on ? 1 : 0, stream);
+            if constexpr (std::is_same<T, float>::value) {
+                if (elem32)
+                    face_add<float, float>(ad, d_fa, d_eoff, d_spos.p, S, elem_ptr<float>(elem_cur),
+                                           sym_on ? 1 : 0, stream);
+                else
+                    face_add<float, double>(ad, d_fa, d_eoff, d_spos.p, S,
+                                            elem_ptr<double>(elem_cur), sym_on ? 1 : 0, stream);
+            } else {
+                face_add<T, double>(ad, d_fa, d_eoff, d_spos.p, S, elem_ptr<double>(elem_cur),
+                                    sym_on ? 1 : 0, stream);
+            }
             timed(ST_SCATTER, t0);
         } else {
             scatter<T>(B, S, false, t0);
@@ -1434,9 +1475,10 @@ struct hifde_factor {
         fill_int(d_status.p, 1, 0, stream);
         // Initialize (src/Factor.cpp:22-34)
         remaining.assign((size_t)NC * NC * NC, 0);
+#pragma omp parallel for schedule(static)
         for (int k = 1; k < NC; ++k)
             for (int j = 1; j < NC; ++j)
-                for (int i = 1; i < NC; ++i) remaining[lin(i, j, k)] = 1;
+                std::fill_n(remaining.begin() + lin(1, j, k), NC - 1, (unsigned char)1);
         cells.resize(L);
         faces.resize(L);
         for (std::vector<BoxBatch>* v : {&cells, &faces})
@@ -1445,6 +1487,7 @@ struct hifde_factor {
         factored = false;
         sym_on = op_symmetric && sym_option != 0;
         elem_on = local_ok && elem_option != 0;
+        elem32 = elem_on && elem_f32_option != 0 && fp32_levels_eff() >= L;
         if (!elem_on) plan0_valid = false;
         elem_cur = 0;
         if (!elem_on) {
@@ -1511,7 +1554,7 @@ struct hifde_factor {
     // side: two NC^3 side buffers (one per pass flag)
     void upd_sides(BoxBatch& B, const DevBuf<unsigned char>& Mbuf, double alpha, double* u,
                    double* side) {
-        if (B.nbox == 0) return;
+        if (B.nbox == 0 || B.tot_rs == 0) return;
         const long long n3 = (long long)NC * NC * NC;
         upd(B, Mbuf, 0, -1, alpha, u, side);
         side_add(B.skel_off.back(), B.d_skel_gid.p, B.d_skel_pass.p, side, n3, u,
@@ -1528,7 +1571,8 @@ struct hifde_factor {
         double* uo = side ? side : u;
         const long long side_n = side ? (long long)NC * NC * NC : 0;
         const unsigned char* spp = (pass >= 0 || side) ? B.d_skel_pass.p : nullptr;
-        if (B.nbox == 0) return;
+        // no red or no skeleton DOFs (faces of full rank): no contribution
+        if (B.nbox == 0 || B.tot_rs == 0) return;
         const bool out_red = (op == 1 || op == 2);
         const bool in_red = (op == 0 || op == 3);
         const std::vector<int>& tl = out_red ? B.tiles_r : B.tiles_s;
@@ -1551,7 +1595,7 @@ struct hifde_factor {
     }
     template <class T>
     void diag_t(BoxBatch& B, const DevBuf<unsigned char>& Mbuf, const double* u, double* out) {
-        if (B.nbox == 0) return;
+        if (B.nbox == 0 || B.tot_rr == 0) return;
         const double by =
             (double)B.tot_rr * sizeof(T) + 16.0 * (B.red_off.empty() ? 0 : B.red_off.back());
         Prof pr(this, fam<T>("apply_diag"), 2.0 * B.tot_rr, by);
@@ -3137,6 +3181,7 @@ int hifde_set_option(hifde_factor* f, const char* name, double value) {
         else if (n == "fp32_levels") f->fp32_levels = (int)value;
         else if (n == "symmetric") f->sym_option = (int)value;
         else if (n == "elements") f->elem_option = (int)value;
+        else if (n == "elements_fp32") f->elem_f32_option = (int)value;
         else if (n == "stage_sync") f->stage_sync = value != 0;
         else if (n == "gram_panel_reg") f->gram_ws.reg_panel = value != 0;
         else if (n == "symmetric_inverse") f->sym_gj = value != 0;

@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 namespace hifde {
@@ -466,32 +467,35 @@ void pack_boxes(int nbox, const long long* d_dst_off, const long long* d_src_off
                 const double* d_src, D* d_dst, long long total, hipStream_t s);
 
 // element storage of the level matrix (level_elem.hip); elements are FP64,
-// the frontal blocks A22 / A21 / A12 are in the level's factor precision T
+// or FP32 (TE) when every level is factored in FP32; the frontal blocks A22 /
+// A21 / A12 are in the level's factor precision T
 template <class T>
 void fill_zero(T* p, long long n, hipStream_t s);
-template <class T>
+template <class T, class TE>
 void frontal_init(int ncell, const InitDesc* d_desc, const int* d_tab, const int* d_gids,
                   const int* d_rp, const int* d_ci, const double* d_v, int NC, int w, int c,
-                  T* A22, T* A21, T* A12, double* E, hipStream_t s);
+                  T* A22, T* A21, T* A12, TE* E, hipStream_t s);
 // One launch per pass (child position); each pass adds one child per parent.
-template <class Td>
+template <class Td, class TE>
 void extend_add(const std::vector<std::vector<ExtendDesc>>& passes, DevBuf<ExtendDesc>& dbuf,
-                DevBuf<long long>& obuf, const int* d_maps, const double* src, Td* A22, Td* A21,
-                Td* A12, double* E, hipStream_t s);
+                DevBuf<long long>& obuf, const int* d_maps, const TE* src, Td* A22, Td* A21,
+                Td* A12, TE* E, hipStream_t s);
 // Uploads the face descriptors and their column prefix; returns the number of
 // face columns (the launch size of face_gather).
 long long face_gather_prepare(const std::vector<FaceGatherDesc>& desc,
                               DevBuf<FaceGatherDesc>& dbuf, DevBuf<long long>& obuf,
                               hipStream_t s);
+template <class TE>
 void face_gather(int nface, long long total_cols, const FaceGatherDesc* d_desc,
                  const long long* d_off, const unsigned* d_rpos, const unsigned* d_cpos,
-                 const double* E, double* M, double* C, hipStream_t s);
-template <class T>
+                 const TE* E, double* M, double* C, hipStream_t s);
+template <class T, class TE>
 void face_add(const std::vector<FaceAddDesc>& desc, DevBuf<FaceAddDesc>& dbuf,
-              DevBuf<long long>& obuf, const int* d_spos, const T* S, double* E, int sym,
+              DevBuf<long long>& obuf, const int* d_spos, const T* S, TE* E, int sym,
               hipStream_t s);
+template <class TE>
 void mirror_lower(const std::vector<MirrorDesc>& desc, DevBuf<MirrorDesc>& dbuf,
-                  DevBuf<long long>& obuf, double* E, hipStream_t s);
+                  DevBuf<long long>& obuf, TE* E, hipStream_t s);
 
 // apply sweeps: one 64-output tile of one box per workgroup (see apply_pcg.hip)
 template <class T>

@@ -63,7 +63,7 @@ __device__ __forceinline__ void axis_cells(int x, int w, int c, int& lo, int& hi
     }
 }
 
-template <class T>
+template <class T, class TE>
 __global__ void __launch_bounds__(256) k_frontal_init(const InitDesc* __restrict__ desc,
                                                       const int* __restrict__ tab,
                                                       const int* __restrict__ gids,
@@ -71,7 +71,7 @@ __global__ void __launch_bounds__(256) k_frontal_init(const InitDesc* __restrict
                                                       const int* __restrict__ ci,
                                                       const double* __restrict__ v, int NC,
                                                       int w, int c, T* A22, T* A21, T* A12,
-                                                      double* E) {
+                                                      TE* E) {
     const InitDesc d = desc[blockIdx.x];
     const int r = d.r, s = d.s, e = d.s + d.o, t = d.r + d.s + d.o;
     for (long long q = threadIdx.x; q < (long long)r * r; q += blockDim.x) A22[d.rr + q] = (T)0;
@@ -79,7 +79,7 @@ __global__ void __launch_bounds__(256) k_frontal_init(const InitDesc* __restrict
         A21[d.rs + q] = (T)0;
         if (A12) A12[d.rs + q] = (T)0;
     }
-    for (long long q = threadIdx.x; q < (long long)e * e; q += blockDim.x) E[d.ee + q] = 0.0;
+    for (long long q = threadIdx.x; q < (long long)e * e; q += blockDim.x) E[d.ee + q] = (TE)0;
     __syncthreads();
     const int NC2 = NC * NC;
     for (int pa = threadIdx.x; pa < t; pa += blockDim.x) {
@@ -112,7 +112,7 @@ __global__ void __launch_bounds__(256) k_frontal_init(const InitDesc* __restrict
                 if (pb < r) A22[d.rr + pa + (long long)pb * r] = val;
                 else if (pb < r + s) A21[d.rs + pa + (long long)(pb - r) * r] = val;
             } else if (pb >= r) {
-                E[d.ee + (pa - r) + (long long)(pb - r) * e] = v[q];
+                E[d.ee + (pa - r) + (long long)(pb - r) * e] = (TE)v[q];
             } else if (pa < r + s && A12) {
                 A12[d.rs + (pa - r) + (long long)pb * s] = val;
             }
@@ -122,12 +122,12 @@ __global__ void __launch_bounds__(256) k_frontal_init(const InitDesc* __restrict
 
 // One pass of the extend-add: for every parent, the element of one child is
 // added into the parent's frontal blocks.  Workgroup = one child column.
-template <class Td>
+template <class Td, class TE>
 __global__ void __launch_bounds__(128) k_extend_add(int npar, const long long* __restrict__ coff,
                                                     const ExtendDesc* __restrict__ desc,
                                                     const int* __restrict__ maps,
-                                                    const double* __restrict__ src, Td* A22,
-                                                    Td* A21, Td* A12, double* E) {
+                                                    const TE* __restrict__ src, Td* A22,
+                                                    Td* A21, Td* A12, TE* E) {
     const long long blk = blockIdx.x;
     const int p = find_slot(coff, npar, blk);
     const ExtendDesc d = desc[p];
@@ -136,7 +136,7 @@ __global__ void __launch_bounds__(128) k_extend_add(int npar, const long long* _
     const int pj = mp[j];
     if (pj < 0) return;
     const int r = d.r, s = d.s;
-    const double* col = src + d.src + (long long)j * d.e;
+    const TE* col = src + d.src + (long long)j * d.e;
     if (pj < r) {
         for (int i = threadIdx.x; i < d.e; i += blockDim.x) {
             const int pi = mp[i];
@@ -164,15 +164,16 @@ __global__ void __launch_bounds__(128) k_extend_add(int npar, const long long* _
 }
 
 // M(i, j) = A(rows_i, cols_j) and C(i, j) = A(cols_i, cols_j) of every face,
-// in FP64, summed from the two adjacent cells' elements.  Workgroup = one
-// face column.
+// in FP64, summed from the two adjacent cells' elements (FP64 or FP32).
+// Workgroup = one face column.
 // Positions are packed two to a word: the low half is the position in the
 // first cell's element, the high half in the second's, kNoPos when absent.
+template <class TE>
 __global__ void __launch_bounds__(128) k_face_gather(int nface, const long long* __restrict__ coff,
                                                      const FaceGatherDesc* __restrict__ desc,
                                                      const unsigned* __restrict__ rpos,
                                                      const unsigned* __restrict__ cpos,
-                                                     const double* __restrict__ E, double* M,
+                                                     const TE* __restrict__ E, double* M,
                                                      double* C) {
     const long long blk = blockIdx.x;
     const int f = find_slot(coff, nface, blk);
@@ -181,39 +182,39 @@ __global__ void __launch_bounds__(128) k_face_gather(int nface, const long long*
     const unsigned* rp = rpos + d.rows;
     const unsigned* cp = cpos + d.cols;
     const unsigned cj = cp[j];
-    const double* e1 = E + d.e1 + (long long)(cj & 0xFFFFu) * d.ld1;
-    const double* e2 = E + d.e2 + (long long)(cj >> 16) * d.ld2;
+    const TE* e1 = E + d.e1 + (long long)(cj & 0xFFFFu) * d.ld1;
+    const TE* e2 = E + d.e2 + (long long)(cj >> 16) * d.ld2;
     for (int i = threadIdx.x; i < d.m; i += blockDim.x) {
         const unsigned x = rp[i];
         const unsigned p1 = x & 0xFFFFu, p2 = x >> 16;
         double v = 0.0;
-        if (p1 != kNoPos) v += e1[p1];
-        if (p2 != kNoPos) v += e2[p2];
+        if (p1 != kNoPos) v += (double)e1[p1];
+        if (p2 != kNoPos) v += (double)e2[p2];
         M[d.offM + i + (long long)j * d.m] = v;
     }
     for (int i = threadIdx.x; i < d.n; i += blockDim.x) {
         const unsigned x = cp[i];
-        C[d.offC + i + (long long)j * d.n] = e1[x & 0xFFFFu] + e2[x >> 16];
+        C[d.offC + i + (long long)j * d.n] = (double)e1[x & 0xFFFFu] + (double)e2[x >> 16];
     }
 }
 
 // E_owner(skel_f, skel_f) += S_f.  Workgroup = one face column.  With sym,
 // only the lower triangle of S_f is valid and is read for both halves.
-template <class T>
+template <class T, class TE>
 __global__ void __launch_bounds__(128) k_face_add(int nface, const long long* __restrict__ coff,
                                                   const FaceAddDesc* __restrict__ desc,
                                                   const int* __restrict__ spos,
-                                                  const T* __restrict__ S, double* E, int sym) {
+                                                  const T* __restrict__ S, TE* E, int sym) {
     const long long blk = blockIdx.x;
     const int f = find_slot(coff, nface, blk);
     const FaceAddDesc d = desc[f];
     const int j = (int)(blk - coff[f]);
     const int* sp = spos + d.sp;
-    double* col = E + d.e1 + (long long)sp[j] * d.ld1;
+    TE* col = E + d.e1 + (long long)sp[j] * d.ld1;
     const T* Sf = S + d.offS;
     for (int i = threadIdx.x; i < d.k; i += blockDim.x) {
         const int a = (sym && i < j) ? j : i, b = (sym && i < j) ? i : j;
-        col[sp[i]] += (double)Sf[a + (long long)b * d.k];
+        col[sp[i]] += (TE)Sf[a + (long long)b * d.k];
     }
 }
 
@@ -222,9 +223,10 @@ __global__ void __launch_bounds__(128) k_face_add(int nface, const long long* __
 // 64 x 64 tile (I, J), I >= J, of the lower triangle: it is read along
 // columns, transposed through LDS and written along the columns of tile
 // (J, I), so both sides are coalesced.
+template <class TE>
 __global__ void __launch_bounds__(256) k_mirror_lower(int nbox, const long long* __restrict__ toff,
                                                       const MirrorDesc* __restrict__ desc,
-                                                      double* E) {
+                                                      TE* E) {
     const long long blk = blockIdx.x;
     const int b = find_slot(toff, nbox, blk);
     const MirrorDesc d = desc[b];
@@ -237,13 +239,13 @@ __global__ void __launch_bounds__(256) k_mirror_lower(int nbox, const long long*
         ++J;
     }
     const int I = J + t;
-    __shared__ double tile[64][65];
-    double* base = E + d.ee;
+    __shared__ TE tile[64][65];
+    TE* base = E + d.ee;
     const int i0 = I * 64, j0 = J * 64;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     for (int c = ty; c < 64; c += 4) {
         const int gi = i0 + tx, gj = j0 + c;
-        tile[c][tx] = (gi < d.s && gj < d.s) ? base[gi + (long long)gj * d.ld] : 0.0;
+        tile[c][tx] = (gi < d.s && gj < d.s) ? base[gi + (long long)gj * d.ld] : (TE)0;
     }
     __syncthreads();
     // write E(j0 + tx, i0 + c) = E(i0 + c, j0 + tx) where i0 + c > j0 + tx
@@ -269,20 +271,20 @@ void fill_zero(T* p, long long n, hipStream_t s) {
     HIFDE_LAUNCH_CHECK();
 }
 
-template <class T>
+template <class T, class TE>
 void frontal_init(int ncell, const InitDesc* d_desc, const int* d_tab, const int* d_gids,
                   const int* d_rp, const int* d_ci, const double* d_v, int NC, int w, int c,
-                  T* A22, T* A21, T* A12, double* E, hipStream_t s) {
+                  T* A22, T* A21, T* A12, TE* E, hipStream_t s) {
     if (ncell <= 0) return;
-    k_frontal_init<T><<<(unsigned)ncell, 256, 0, s>>>(d_desc, d_tab, d_gids, d_rp, d_ci, d_v, NC,
+    k_frontal_init<T, TE><<<(unsigned)ncell, 256, 0, s>>>(d_desc, d_tab, d_gids, d_rp, d_ci, d_v, NC,
                                                       w, c, A22, A21, A12, E);
     HIFDE_LAUNCH_CHECK();
 }
 
-template <class Td>
+template <class Td, class TE>
 void extend_add(const std::vector<std::vector<ExtendDesc>>& passes, DevBuf<ExtendDesc>& dbuf,
-                DevBuf<long long>& obuf, const int* d_maps, const double* src, Td* A22, Td* A21,
-                Td* A12, double* E, hipStream_t s) {
+                DevBuf<long long>& obuf, const int* d_maps, const TE* src, Td* A22, Td* A21,
+                Td* A12, TE* E, hipStream_t s) {
     // all passes' descriptors and column prefixes in one upload each
     std::vector<ExtendDesc> all;
     std::vector<long long> off;
@@ -302,7 +304,7 @@ void extend_add(const std::vector<std::vector<ExtendDesc>>& passes, DevBuf<Exten
     obuf.upload(off, s);
     for (size_t k = 0; k < passes.size(); ++k) {
         if (passes[k].empty() || tot[k] == 0) continue;
-        k_extend_add<Td><<<(unsigned)tot[k], 128, 0, s>>>((int)passes[k].size(), obuf.p + o0[k],
+        k_extend_add<Td, TE><<<(unsigned)tot[k], 128, 0, s>>>((int)passes[k].size(), obuf.p + o0[k],
                                                           dbuf.p + d0[k], d_maps, src, A22, A21,
                                                           A12, E);
         HIFDE_LAUNCH_CHECK();
@@ -320,18 +322,19 @@ long long face_gather_prepare(const std::vector<FaceGatherDesc>& desc,
     return off.back();
 }
 
+template <class TE>
 void face_gather(int nface, long long total_cols, const FaceGatherDesc* d_desc,
                  const long long* d_off, const unsigned* d_rpos, const unsigned* d_cpos,
-                 const double* E, double* M, double* C, hipStream_t s) {
+                 const TE* E, double* M, double* C, hipStream_t s) {
     if (nface <= 0 || total_cols <= 0) return;
-    k_face_gather<<<(unsigned)total_cols, 128, 0, s>>>(nface, d_off, d_desc, d_rpos, d_cpos, E,
+    k_face_gather<TE><<<(unsigned)total_cols, 128, 0, s>>>(nface, d_off, d_desc, d_rpos, d_cpos, E,
                                                        M, C);
     HIFDE_LAUNCH_CHECK();
 }
 
-template <class T>
+template <class T, class TE>
 void face_add(const std::vector<FaceAddDesc>& desc, DevBuf<FaceAddDesc>& dbuf,
-              DevBuf<long long>& obuf, const int* d_spos, const T* S, double* E, int sym,
+              DevBuf<long long>& obuf, const int* d_spos, const T* S, TE* E, int sym,
               hipStream_t s) {
     if (desc.empty()) return;
     const std::vector<long long> off =
@@ -339,13 +342,14 @@ void face_add(const std::vector<FaceAddDesc>& desc, DevBuf<FaceAddDesc>& dbuf,
     if (off.back() == 0) return;
     dbuf.upload(desc, s);
     obuf.upload(off, s);
-    k_face_add<T><<<(unsigned)off.back(), 128, 0, s>>>((int)desc.size(), obuf.p, dbuf.p, d_spos,
+    k_face_add<T, TE><<<(unsigned)off.back(), 128, 0, s>>>((int)desc.size(), obuf.p, dbuf.p, d_spos,
                                                        S, E, sym);
     HIFDE_LAUNCH_CHECK();
 }
 
+template <class TE>
 void mirror_lower(const std::vector<MirrorDesc>& desc, DevBuf<MirrorDesc>& dbuf,
-                  DevBuf<long long>& obuf, double* E, hipStream_t s) {
+                  DevBuf<long long>& obuf, TE* E, hipStream_t s) {
     if (desc.empty()) return;
     const std::vector<long long> off = col_prefix<MirrorDesc>(desc, [](const MirrorDesc& d) {
         const int nt = (d.s + 63) / 64;
@@ -354,23 +358,34 @@ void mirror_lower(const std::vector<MirrorDesc>& desc, DevBuf<MirrorDesc>& dbuf,
     if (off.back() == 0) return;
     dbuf.upload(desc, s);
     obuf.upload(off, s);
-    k_mirror_lower<<<(unsigned)off.back(), 256, 0, s>>>((int)desc.size(), obuf.p, dbuf.p, E);
+    k_mirror_lower<TE><<<(unsigned)off.back(), 256, 0, s>>>((int)desc.size(), obuf.p, dbuf.p, E);
     HIFDE_LAUNCH_CHECK();
 }
 
-#define HIFDE_INST1(T)                                                                          \
-    template void fill_zero<T>(T*, long long, hipStream_t);                                     \
-    template void frontal_init<T>(int, const InitDesc*, const int*, const int*, const int*,      \
-                                  const int*, const double*, int, int, int, T*, T*, T*,         \
-                                  double*, hipStream_t);                                        \
-    template void extend_add<T>(const std::vector<std::vector<ExtendDesc>>&,                    \
-                                DevBuf<ExtendDesc>&, DevBuf<long long>&, const int*,            \
-                                const double*, T*, T*, T*, double*, hipStream_t);               \
-    template void face_add<T>(const std::vector<FaceAddDesc>&, DevBuf<FaceAddDesc>&,            \
-                              DevBuf<long long>&, const int*, const T*, double*, int,           \
-                              hipStream_t);
-HIFDE_INST1(float)
-HIFDE_INST1(double)
+#define HIFDE_INST1(T, TE)                                                                      \
+    template void frontal_init<T, TE>(int, const InitDesc*, const int*, const int*, const int*, \
+                                      const int*, const double*, int, int, int, T*, T*, T*,     \
+                                      TE*, hipStream_t);                                        \
+    template void extend_add<T, TE>(const std::vector<std::vector<ExtendDesc>>&,                \
+                                    DevBuf<ExtendDesc>&, DevBuf<long long>&, const int*,        \
+                                    const TE*, T*, T*, T*, TE*, hipStream_t);                   \
+    template void face_add<T, TE>(const std::vector<FaceAddDesc>&, DevBuf<FaceAddDesc>&,        \
+                                  DevBuf<long long>&, const int*, const T*, TE*, int,           \
+                                  hipStream_t);
+// elements are FP64, or FP32 when every level of the factorization is FP32
+HIFDE_INST1(float, double)
+HIFDE_INST1(double, double)
+HIFDE_INST1(float, float)
 #undef HIFDE_INST1
+#define HIFDE_INST2(TE)                                                                         \
+    template void fill_zero<TE>(TE*, long long, hipStream_t);                                   \
+    template void face_gather<TE>(int, long long, const FaceGatherDesc*, const long long*,      \
+                                  const unsigned*, const unsigned*, const TE*, double*,         \
+                                  double*, hipStream_t);                                        \
+    template void mirror_lower<TE>(const std::vector<MirrorDesc>&, DevBuf<MirrorDesc>&,         \
+                                   DevBuf<long long>&, TE*, hipStream_t);
+HIFDE_INST2(float)
+HIFDE_INST2(double)
+#undef HIFDE_INST2
 
 }  // namespace hifde

@@ -261,6 +261,12 @@ const char* hifde_stage_name(int stage);
  *                  inside one level-0 cell closure (all stencils), hold the
  *                  level matrix as per-cell dense elements assembled by
  *                  extend-add; 0: device hash table of (row, col) entries
+ *   "elements_fp32"  -1 (default): when every level is factored in FP32
+ *                  (HIFDE_MIXED with fp32_levels = num_levels) the elements
+ *                  are FP32 as well, so their assembly, face gathers and
+ *                  Schur-complement accumulation move half the bytes (the
+ *                  face ID matrices are still gathered into FP64); 0: FP64
+ *                  elements always
  *   "symmetric_inverse"  1 (default): with the symmetric level matrix the
  *                  block inverses read only lower triangles and update only
  *                  lower triangles (half the flops); 0: general inverse

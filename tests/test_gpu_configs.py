@@ -79,6 +79,39 @@ def test_c3_family_n64_against_oracle_golden(H, precision):
     f.close()
 
 
+def test_fp32_elements_match_fp64_elements(H):
+    """With every level in FP32 (eps >= 1e-5) the level-matrix elements are
+    FP32 too; option elements_fp32 = 0 keeps them FP64.  Lognormal GRF,
+    n=64, eps 1e-4: the same level counts within 1%, preconditioner
+    residuals within 2x of each other (they differ through skeleton choices
+    at the tolerance: 3.6e-3 and 5.5e-3 here), PCG iterations within one,
+    and solutions within 1e-8."""
+    import torch
+    n = 64
+    a, V, b = pb.lognormal_coeff(n, 2), pb.zeros_coeff(n), pb.rhs_normal(n, 2)
+    db = torch.from_numpy(b).cuda()
+    out = {}
+    for opt in (1, 0):
+        f = H.HIFFactor(n - 1, 4, 1e-4, precision="mixed")
+        f.set_option("elements_fp32", opt)
+        f.set_stencil(a, V, True)
+        assert f.Factor() == 0
+        res = np.linalg.norm(f.spmv(f.Apply(b, True)) - b) / np.linalg.norm(b)
+        dx = torch.zeros_like(db)
+        it, rr = f.pcg_device(db.data_ptr(), dx.data_ptr(), tol=1e-10, maxit=200)
+        assert rr <= 1e-10
+        out[opt] = (f.level_counts(), res, it, dx.cpu().numpy())
+        f.close()
+    (c1, r1, i1, x1), (c0, r0, i0, x0) = out[1], out[0]
+    for lv in range(len(c0)):
+        assert c1[lv][:3:2] == c0[lv][:3:2]
+        for idx in (1, 3, 4):
+            assert abs(c1[lv][idx] - c0[lv][idx]) <= 0.01 * max(c0[lv][idx], 100)
+    assert r1 <= 2 * r0 and r0 <= 2 * r1
+    assert abs(i1 - i0) <= 1
+    assert np.linalg.norm(x1 - x0) / np.linalg.norm(x0) < 1e-8
+
+
 def test_c3_full_size_workload_properties(H):
     """The measured workload itself: n=128 (127^3 unknowns), lognormal GRF
     seed 2, P=4, eps 1e-4; mixed factor + FP64 PCG to 1e-10, and the FP64
