@@ -630,52 +630,85 @@ struct hifde_factor {
         }
     }
 
-    // Rows of the ID matrix: this face's edges, the cell's other five faces,
-    // then the neighbour cell's five faces (src/Factor.cpp:665-685, 694-714,
-    // 723-743).
-    int face_rows(int ci, int cj, int ck, int face, const unsigned char* I, int w,
-                  int* out) const {
-        static const int own_top[5] = {BOTTOM, LEFT, RIGHT, FRONT, BACK};
-        static const int nbr_top[5] = {TOP, LEFT, RIGHT, FRONT, BACK};
-        static const int own_left[5] = {TOP, BOTTOM, RIGHT, FRONT, BACK};
-        static const int nbr_left[5] = {TOP, BOTTOM, LEFT, FRONT, BACK};
-        static const int own_front[5] = {TOP, BOTTOM, LEFT, RIGHT, BACK};
-        static const int nbr_front[5] = {TOP, BOTTOM, LEFT, RIGHT, FRONT};
-        const int* own = face == TOP ? own_top : (face == LEFT ? own_left : own_front);
-        const int* nbr = face == TOP ? nbr_top : (face == LEFT ? nbr_left : nbr_front);
-        int ni = ci - (face == TOP), nj = cj - (face == LEFT), nk = ck - (face == FRONT);
-        int cnt = face_dofs(ci, cj, ck, face, 1, I, w, out);
-        for (int t = 0; t < 5; ++t)
-            cnt += face_dofs(ci, cj, ck, own[t], 2, I, w, out ? out + cnt : nullptr);
-        for (int t = 0; t < 5; ++t)
-            cnt += face_dofs(ni, nj, nk, nbr[t], 2, I, w, out ? out + cnt : nullptr);
-        return cnt;
-    }
-
     struct FacePlan {
         int nface = 0;
         std::vector<int> m, n, rows, cols, rows_off, cols_off;
         std::vector<int> c1, c2;  // the face's cell (max side) and its neighbour
     };
 
+    // Per cell of a level: the face-interior DOFs of its six faces and the
+    // edge DOFs of its three min faces (face_dofs), each list scanned once.
+    // The rows of a face's ID matrix are its edges, the cell's other five
+    // faces, then the neighbour cell's five faces, in the order of
+    // src/Factor.cpp:665-685, 694-714, 723-743; they are copied from these
+    // lists.
+    // Slots: 0-5 face-interior lists by FaceId, 6-8 edge lists of TOP, LEFT,
+    // FRONT.
+    std::vector<long long> flist_off;
+    std::vector<int> flist;
+    std::vector<int> ext_maps;  // extend-add maps of the current level (capacity kept)
+    static int edge_slot(int face) { return face == TOP ? 6 : (face == LEFT ? 7 : 8); }
+
     void plan_faces(int level, FacePlan& F) {
         const int w = width(level), c = NC / w;
         const AxisClass ac = axis_class(level);
         const unsigned char* I = ac.interior.data();
-        const int ncand = 3 * c * c * c;
+        const int nb = c * c * c;
+        const int ncand = 3 * nb;
         static const int order[3] = {TOP, FRONT, LEFT};  // src/Factor.cpp:249-254
-        std::vector<int> cm(ncand, 0), cn(ncand, 0);
-#pragma omp parallel for schedule(dynamic, 1)
-        for (int t = 0; t < ncand; ++t) {
-            int b = t / 3, f = order[t % 3];
-            int ci = b / (c * c), cj = (b / c) % c, ck = b % c;
-            int cc = f == TOP ? ci : (f == LEFT ? cj : ck);
-            if (cc == 0) continue;  // lo == 1: no face (src/Factor.cpp:661,690,719)
-            cn[t] = face_dofs(ci, cj, ck, f, 2, I, w, nullptr);
-            if (cn[t] > 0) cm[t] = face_rows(ci, cj, ck, f, I, w, nullptr);
+        static const int own_top[5] = {BOTTOM, LEFT, RIGHT, FRONT, BACK};
+        static const int nbr_top[5] = {TOP, LEFT, RIGHT, FRONT, BACK};
+        static const int own_left[5] = {TOP, BOTTOM, RIGHT, FRONT, BACK};
+        static const int nbr_left[5] = {TOP, BOTTOM, LEFT, FRONT, BACK};
+        static const int own_front[5] = {TOP, BOTTOM, LEFT, RIGHT, BACK};
+        static const int nbr_front[5] = {TOP, BOTTOM, LEFT, RIGHT, FRONT};
+        auto slot_face = [](int q) { return q < 6 ? q : (q == 6 ? TOP : (q == 7 ? LEFT : FRONT)); };
+        flist_off.assign((size_t)nb * 9 + 1, 0);
+#pragma omp parallel for schedule(dynamic, 16)
+        for (int b = 0; b < nb; ++b) {
+            const int ci = b / (c * c), cj = (b / c) % c, ck = b % c;
+            for (int q = 0; q < 9; ++q)
+                flist_off[(size_t)b * 9 + q + 1] =
+                    face_dofs(ci, cj, ck, slot_face(q), q < 6 ? 2 : 1, I, w, nullptr);
         }
+        for (size_t t = 0; t < (size_t)nb * 9; ++t) flist_off[t + 1] += flist_off[t];
+        flist.resize((size_t)flist_off.back());
+#pragma omp parallel for schedule(dynamic, 16)
+        for (int b = 0; b < nb; ++b) {
+            const int ci = b / (c * c), cj = (b / c) % c, ck = b % c;
+            for (int q = 0; q < 9; ++q)
+                face_dofs(ci, cj, ck, slot_face(q), q < 6 ? 2 : 1, I, w,
+                          flist.data() + flist_off[(size_t)b * 9 + q]);
+        }
+        auto len = [&](int b, int q) {
+            return (int)(flist_off[(size_t)b * 9 + q + 1] - flist_off[(size_t)b * 9 + q]);
+        };
+        // face t = 3 b + (TOP, FRONT, LEFT): its cell b, neighbour across the face
+        auto nbr_cell = [&](int b, int f) { return b - (f == TOP ? c * c : (f == LEFT ? c : 1)); };
+        auto lists = [&](int f, const int*& own, const int*& nbr) {
+            own = f == TOP ? own_top : (f == LEFT ? own_left : own_front);
+            nbr = f == TOP ? nbr_top : (f == LEFT ? nbr_left : nbr_front);
+        };
+        std::vector<int> cm(ncand, 0), cn(ncand, 0);
+#pragma omp parallel for schedule(static)
+        for (int t = 0; t < ncand; ++t) {
+            const int b = t / 3, f = order[t % 3];
+            const int ci = b / (c * c), cj = (b / c) % c, ck = b % c;
+            const int cc = f == TOP ? ci : (f == LEFT ? cj : ck);
+            if (cc == 0) continue;  // lo == 1: no face (src/Factor.cpp:661,690,719)
+            cn[t] = len(b, f);
+            if (cn[t] == 0) continue;
+            const int* own;
+            const int* nbr;
+            lists(f, own, nbr);
+            const int bn = nbr_cell(b, f);
+            int m = len(b, edge_slot(f));
+            for (int u = 0; u < 5; ++u) m += len(b, own[u]) + len(bn, nbr[u]);
+            cm[t] = m;
+        }
+        F.nface = 0;
+        for (std::vector<int>* v : {&F.m, &F.n, &F.c1, &F.c2}) v->clear();
         std::vector<int> slot(ncand, -1);
-        F = FacePlan();
         for (int t = 0; t < ncand; ++t)
             if (cn[t] > 0) {
                 slot[t] = F.nface++;
@@ -683,20 +716,32 @@ struct hifde_factor {
                 F.n.push_back(cn[t]);
                 const int b = t / 3, f = order[t % 3];
                 F.c1.push_back(b);
-                F.c2.push_back(b - (f == TOP ? c * c : (f == LEFT ? c : 1)));
+                F.c2.push_back(nbr_cell(b, f));
             }
         F.rows_off = prefix_i(F.m);
         F.cols_off = prefix_i(F.n);
         F.rows.resize(F.rows_off.back());
         F.cols.resize(F.cols_off.back());
-#pragma omp parallel for schedule(dynamic, 1)
+#pragma omp parallel for schedule(dynamic, 16)
         for (int t = 0; t < ncand; ++t) {
             if (slot[t] < 0) continue;
-            int b = t / 3, f = order[t % 3];
-            int ci = b / (c * c), cj = (b / c) % c, ck = b % c;
-            int sidx = slot[t];
-            face_dofs(ci, cj, ck, f, 2, I, w, F.cols.data() + F.cols_off[sidx]);
-            face_rows(ci, cj, ck, f, I, w, F.rows.data() + F.rows_off[sidx]);
+            const int b = t / 3, f = order[t % 3];
+            const int sidx = slot[t];
+            auto put = [&](int bb, int q, int* dst) {
+                const int* src = flist.data() + flist_off[(size_t)bb * 9 + q];
+                const int n = len(bb, q);
+                std::copy(src, src + n, dst);
+                return n;
+            };
+            put(b, f, F.cols.data() + F.cols_off[sidx]);
+            const int* own;
+            const int* nbr;
+            lists(f, own, nbr);
+            const int bn = nbr_cell(b, f);
+            int* r = F.rows.data() + F.rows_off[sidx];
+            r += put(b, edge_slot(f), r);
+            for (int u = 0; u < 5; ++u) r += put(b, own[u], r);
+            for (int u = 0; u < 5; ++u) r += put(bn, nbr[u], r);
         }
     }
 
@@ -800,6 +845,15 @@ struct hifde_factor {
         for (int g : B.red_gid) remaining[g] = 0;  // UpdateRemainingDOFs (:91-111)
         if (!elem_on) d_alive.upload(remaining, stream);  // read by the table compaction
     }
+    // Cells: the host mask is updated as soon as the level's cells are
+    // planned; the table path's device mask follows after the cells' Schur
+    // step, before the scatter's compaction reads it.
+    void mark_cells_host(const BoxBatch& B) {
+        for (int g : B.red_gid) remaining[g] = 0;  // UpdateRemainingDOFs (:91-111)
+    }
+    void mark_cells_device() {
+        if (!elem_on) d_alive.upload(remaining, stream);
+    }
 
     // Number of S entries written in the second scatter pass (both DOFs on
     // min faces of their cell).
@@ -858,6 +912,7 @@ struct hifde_factor {
         timed(ST_PLAN, t0);
         if (!cached) upload_batch_index(B);
         timed(ST_UPLOAD, t0);
+        mark_cells_host(B);
         if (elem_on) {
             if constexpr (std::is_same<T, float>::value) {
                 if (elem32) {
@@ -896,7 +951,7 @@ struct hifde_factor {
         }
         timed(ST_GATHER, t0);
         schur_batch<T>(B, A21, A12, S, B.off_ss.data(), B.s.data(), false, t0);
-        mark_eliminated(B);
+        mark_cells_device();
         scatter<T>(B, S, true, t0);
     }
 
@@ -987,7 +1042,8 @@ struct hifde_factor {
                 }
             for (size_t t = 0; t < (size_t)nb * 8; ++t) moff[t + 1] += moff[t];
             if (moff.back() > (long long)INT32_MAX) throw Error(HIFDE_ERR_SIZE, "extend-add maps");
-            std::vector<int> maps(moff.back());
+            std::vector<int>& maps = ext_maps;  // every entry is written below
+            if ((long long)maps.size() < moff.back()) maps.resize(moff.back());
             std::vector<std::vector<ExtendDesc>> pass(8, std::vector<ExtendDesc>(nb));
 #pragma omp parallel for schedule(dynamic, 1)
             for (int b = 0; b < nb; ++b)
@@ -1009,7 +1065,7 @@ struct hifde_factor {
                     for (int t = 0; t < Cb.o[ch]; ++t)
                         *m++ = cell_pos(b, Cb.other_gid[Cb.other_off[ch] + t]);
                 }
-            d_emap.upload(maps, stream);
+            d_emap.upload(maps.data(), moff.back(), stream);
             timed(ST_PLAN, t0);
             Prof pr(this, fam<T>("assemble"), 0.0, fr_bytes + (double)Cb.tot_ee * (we + 4.0));
             fill_zero<T>(A22, B.tot_rr, stream);
@@ -1031,7 +1087,7 @@ struct hifde_factor {
             mirror_lower(md, d_mir, d_eoff, E, stream);
         }
         timed(ST_SCATTER, t0);
-        mark_eliminated(B);
+        mark_cells_device();
     }
 
     // Index of one level's faces: the plan and its device copies.
@@ -1045,18 +1101,33 @@ struct hifde_factor {
         DevBuf<FaceGatherDesc> gd;
         DevBuf<long long> gd_off;
         long long gd_cols = 0;
+        // host side of the positions, computed with the plan
+        std::vector<unsigned> h_rpos, h_cpos;
+        std::vector<FaceGatherDesc> h_gd;
     };
     FaceDev fdev[2];  // [0]: level 0, kept while plan0_valid; [1]: the other levels
 
+    // The face plan of a level: plan_faces and the element positions.
+    void plan_faces_host(int level, FaceDev& fd) {
+        plan_faces(level, fd.F);
+        fd.offM = prefix_ll(fd.F.m, fd.F.n);
+        fd.offC = prefix_ll(fd.F.n, fd.F.n);
+        if (elem_on) face_positions_host(level, fd);
+    }
+
     // Element positions of the face rows and columns in the two adjacent
     // cells, packed as low / high halves (kNoPos when absent).
-    void face_positions(int level, FaceDev& fd) {
+    void face_positions_host(int level, FaceDev& fd) {
         const FacePlan& F = fd.F;
         const BoxBatch& Cb = cells[level];
         const int nf = F.nface;
-        std::vector<unsigned> rpos(F.rows_off.back()), cpos(F.cols_off.back());
+        std::vector<unsigned>& rpos = fd.h_rpos;
+        std::vector<unsigned>& cpos = fd.h_cpos;
+        rpos.resize(F.rows_off.back());
+        cpos.resize(F.cols_off.back());
         fd.cpos1.assign(F.cols_off.back(), 0);
-        std::vector<FaceGatherDesc> gd(nf);
+        std::vector<FaceGatherDesc>& gd = fd.h_gd;
+        gd.resize(nf);
         int bad = 0;
 #pragma omp parallel for schedule(dynamic, 1) reduction(max : bad)
         for (int f = 0; f < nf; ++f) {
@@ -1094,9 +1165,11 @@ struct hifde_factor {
         }
         if (bad == 1) throw Error(HIFDE_ERR_SIZE, "element larger than 65535 DOFs");
         if (bad == 2) throw Error(HIFDE_ERR_STATE, "face DOF outside its cells");
-        fd.rpos.upload(rpos, stream);
-        fd.cpos.upload(cpos, stream);
-        fd.gd_cols = face_gather_prepare(gd, fd.gd, fd.gd_off, stream);
+    }
+    void face_positions_upload(FaceDev& fd) {
+        fd.rpos.upload(fd.h_rpos, stream);
+        fd.cpos.upload(fd.h_cpos, stream);
+        fd.gd_cols = face_gather_prepare(fd.h_gd, fd.gd, fd.gd_off, stream);
     }
 
     template <class T>
@@ -1107,11 +1180,7 @@ struct hifde_factor {
         const bool cached = level == 0 && elem_on && plan0_valid;
         FaceDev& fd = fdev[level == 0 ? 0 : 1];
         const FacePlan& F = fd.F;
-        if (!cached) {
-            plan_faces(level, fd.F);
-            fd.offM = prefix_ll(F.m, F.n);
-            fd.offC = prefix_ll(F.n, F.n);
-        }
+        if (!cached) plan_faces_host(level, fd);
         timed(ST_PLAN, t0);
         const int nf = F.nface;
         BoxBatch& B = faces[level];
@@ -1132,7 +1201,7 @@ struct hifde_factor {
             fd.n.upload(F.n, stream);
             fd.offMd.upload(offM, stream);
             fd.offCd.upload(offC, stream);
-            if (elem_on) face_positions(level, fd);
+            if (elem_on) face_positions_upload(fd);
             if (level == 0 && elem_on) plan0_valid = true;
         }
         timed(ST_UPLOAD, t0);
