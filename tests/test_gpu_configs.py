@@ -53,7 +53,9 @@ def _counts_close(cg, co, rel):
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_c3_family_n64_against_oracle_golden(H, precision):
     """Lognormal GRF (seed 2), n=64, P=4, eps 1e-4, symmetric stencil: level
-    counts within 5% of the oracle; FP64 preconditioner residual
+    counts within 1% of the oracle (measured equal, Gram and Householder
+    paths alike; the bound leaves room for a pivot decided by rounding at the
+    tolerance); FP64 preconditioner residual
     ||A F^{-1} b - b|| / ||b|| within 2x (mixed 4x); PCG to 1e-10 in at most
     the oracle-preconditioned CG's iterations + 3, solution within 1e-5 of
     the oracle's largest sampled value at 64 fixed DOFs."""
@@ -63,7 +65,7 @@ def test_c3_family_n64_against_oracle_golden(H, precision):
     f = H.HIFFactor(n - 1, P, eps, precision=precision)
     f.set_stencil(pb.lognormal_coeff(n, g["coeff_seed"]), pb.zeros_coeff(n), True)
     assert f.Factor() == 0
-    _counts_close(f.level_counts(), g["level_counts"], 0.05)
+    _counts_close(f.level_counts(), g["level_counts"], 0.01)
     b = pb.rhs_normal(n, g["rhs_seed"])
     res = np.linalg.norm(f.spmv(f.Apply(b, True)) - b) / np.linalg.norm(b)
     assert res <= (2.0 if precision == "fp64" else 4.0) * g["precond_residual"]

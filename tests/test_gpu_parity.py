@@ -432,8 +432,11 @@ def _golden(name):
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_c2_n64_against_oracle_golden(H, precision):
-    """n=64, P=4, eps=1e-6, a = 10^U[-1,1] (seed 1): counts within 5% and
-    e_a, e_s, ||A F^{-1} b - b|| / ||b|| within 2x of the oracle's fixture."""
+    """n=64, P=4, eps=1e-6, a = 10^U[-1,1] (seed 1): counts within 1% (they
+    are measured equal to the oracle's, Gram and Householder paths alike;
+    the bound leaves room for a pivot decided by rounding at the tolerance)
+    and e_a, e_s, ||A F^{-1} b - b|| / ||b|| within 2x of the oracle's
+    fixture."""
     g = _golden("c2_n64.json")
     n, P, eps = g["n"], g["P"], g["eps"]
     N = n - 1
@@ -446,7 +449,7 @@ def test_c2_n64_against_oracle_golden(H, precision):
     for lv, co in enumerate(g["level_counts"]):
         assert cg[lv][0] == co[0] and cg[lv][2] == co[2]
         for idx in (1, 3):
-            assert abs(cg[lv][idx] - co[idx]) <= 0.05 * max(co[idx], 20)
+            assert abs(cg[lv][idx] - co[idx]) <= 0.01 * max(co[idx], 20)
     v = pb.reference_test_vector(n)
     w = f.spmv(v, transpose=True)
     ea = _rel(f.Apply(v, False), w)
