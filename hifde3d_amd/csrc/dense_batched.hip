@@ -99,12 +99,11 @@ __device__ void block_argmax(T v, int idx, T* redv, int* redi, T* outv, int* out
 //   A_IK <- C,  A_KJ <- R                          strided copies
 // so the bulk of the 2n^3 flops is a rank-256 update, which is compute-bound
 // on the matrix cores.  The inner level inverts a window with kGJB-wide
-// diagonal blocks and the same five phases:
+// diagonal blocks and the same phases, in four launches:
 //   A_kk <- D = A_kk^{-1}                        k_bgj_diag   (1 wave per box)
 //   R    <- D A_kj              (j != k)         k_bgj_row    (column chunks)
 //   A_ij <- A_ij - A_ik R       (i, j != k)      GEMM
-//   A_kj <- R                                    k_bgj_rowcopy
-//   A_ik <- -A_ik D, A_kk <- D  (i != k)         k_bgj_col    (row chunks)
+//   A_kj <- R;  A_ik <- -A_ik D, A_kk <- D       k_bgj_col    (row chunks)
 // Pivoting is partial pivoting inside each kGJB diagonal block.  The blocks
 // of this path are the interior blocks of symmetric positive definite or
 // M-matrix operators, whose diagonal blocks are nonsingular; a singular
@@ -113,6 +112,9 @@ __device__ void block_argmax(T v, int idx, T* redv, int* redi, T* outv, int* out
 constexpr int kSmallGJ = 64;
 constexpr int kGJB = 64;
 constexpr int kGJPanel = 512;
+// GemmPlan::add with one_launch: calls of at most this many descriptors
+// are issued as one launch
+constexpr int kUnifyMaxDesc = 16;
 
 // Wave-wide arg-max of non-negative values, ties to the lower lane (values
 // equal in all but their 6 lowest mantissa bits count as tied).  The key is
@@ -551,11 +553,16 @@ __global__ void __launch_bounds__(256) k_bgj_row(const int* __restrict__ dims,
 }
 
 // A(i, K) = -A(i, K) D for i outside K; A(K, K) = D.  Grid (row chunks, boxes).
+// The same workgroup also sets A(K, j) = R(:, j) for the columns j of its
+// chunk outside K (the two blocks are disjoint), which saves a launch per
+// step.
 template <class T>
 __global__ void __launch_bounds__(256) k_bgj_col(const int* __restrict__ dims,
                                                  const long long* __restrict__ off,
                                                  const int* __restrict__ lds, T* mats,
-                                                 const T* __restrict__ Dws, int step,
+                                                 const T* __restrict__ Dws,
+                                                 const T* __restrict__ Rws,
+                                                 const long long* __restrict__ r_off, int step,
                                                  const int* __restrict__ info) {
     const int b = blockIdx.y;
     const int n = dims[b];
@@ -566,6 +573,15 @@ __global__ void __launch_bounds__(256) k_bgj_col(const int* __restrict__ dims,
     if (i0 >= n) return;
     const int ld = lds[b];
     T* A = mats + off[b];
+    if (i0 != k0) {
+        // row copy: columns i0 .. i0 + 63 of the chunk
+        const T* R = Rws + r_off[b];
+        const int jn = min(64, n - i0);
+        for (int e = threadIdx.x; e < kb * jn; e += 256) {
+            const int c = e % kb, j = i0 + e / kb;
+            A[k0 + c + (size_t)j * ld] = R[c + (size_t)j * kGJB];
+        }
+    }
     const T* D = Dws + (size_t)b * kGJB * kGJB;
     __shared__ T Ls[kGJB][65];  // Ls[c][i] = A(i0 + i, k0 + c)
     __shared__ T Dt[kGJB][65];  // Dt[c][j] = D(c, j)
@@ -957,28 +973,6 @@ __global__ void __launch_bounds__(256, 2) k_gemm_mfma(int nbox, const int* __res
                     *e = (beta == (T)0) ? v : (TC)beta * *e + v;
                 }
             }
-}
-
-// A(K, j) = R(:, j) for the columns j outside the panel K (block GJ).
-template <class T>
-__global__ void __launch_bounds__(256) k_bgj_rowcopy(const int* __restrict__ dims,
-                                                     const long long* __restrict__ off,
-                                                     const int* __restrict__ lds, T* mats,
-                                                     const T* __restrict__ Rws,
-                                                     const long long* __restrict__ r_off,
-                                                     int step, const int* __restrict__ info) {
-    const int b = blockIdx.y;
-    const int n = dims[b];
-    const int k0 = step * kGJB;
-    if (k0 >= n || info[b]) return;
-    const int kb = min(kGJB, n - k0);
-    const int ld = lds[b];
-    for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < (long long)kb * n;
-         e += (long long)gridDim.x * 256) {
-        const int j = (int)(e / kb), c = (int)(e - (long long)j * kb);
-        if (j >= k0 && j < k0 + kb) continue;
-        mats[off[b] + k0 + c + (size_t)j * ld] = Rws[r_off[b] + c + (size_t)j * kGJB];
-    }
 }
 
 __global__ void k_fill_int(int* p, int n, int v) {
@@ -1775,8 +1769,8 @@ void batched_gj_inverse(int nbox, const int* h_n, const long long* h_off, const 
             HIFDE_LAUNCH_CHECK();
         };
         auto gemm = [&](const std::vector<GemmDesc>& d, const T* A, const T* B, T* C, T alpha,
-                        T beta, bool transA = false) {
-            if (!exec) gh.push_back(plan.add(d, transA));
+                        T beta, bool transA = false, bool one_launch = false) {
+            if (!exec) gh.push_back(plan.add(d, transA, one_launch));
             else gemm_run<T, T>(plan, gh[gi++], A, B, C, alpha, beta, s);
         };
         for (int so = 0; so < osteps; ++so) {
@@ -1826,11 +1820,10 @@ void batched_gj_inverse(int nbox, const int* h_n, const long long* h_off, const 
                         upd.push_back({o + (long long)k0 * ld, ro + (long long)j1 * kGJB,
                                        o + (long long)j1 * ld, nw, nw - j1, kb, ld, kGJB, ld});
                 }
-                gemm(upd, d_A, Rws, d_A, (T)-1, (T)1);
+                gemm(upd, d_A, Rws, d_A, (T)-1, (T)1, false, true);
                 if (exec) {
-                    k_bgj_rowcopy<T><<<dim3((kGJB * max_w + 255) / 256, nbig), 256, 0, s>>>(
-                        dwn, dwoff, dwld, d_A, Rws, ws.b_off.p, st, cinfo);
-                    k_bgj_col<T><<<gch, 256, 0, s>>>(dwn, dwoff, dwld, d_A, Dws, st, cinfo);
+                    k_bgj_col<T><<<gch, 256, 0, s>>>(dwn, dwoff, dwld, d_A, Dws, Rws, ws.b_off.p,
+                                                      st, cinfo);
                 }
             }
             if (sym) {
@@ -2040,7 +2033,7 @@ void gemm_tune_install(const signed char* kinds) {
     g_tune_table.store(t, std::memory_order_release);
 }
 
-std::pair<int, int> GemmPlan::add(const std::vector<GemmDesc>& d, bool transA) {
+std::pair<int, int> GemmPlan::add(const std::vector<GemmDesc>& d, bool transA, bool one_launch) {
     constexpr int TILE = kMfmaTile;
     const int first = (int)launches.size();
     // kinds: 0 k_gemm (FMA, 64 x 64 tiles), 1 k_gemm_mfma; -1 empty.
@@ -2101,6 +2094,34 @@ std::pair<int, int> GemmPlan::add(const std::vector<GemmDesc>& d, bool transA) {
                                              g.n, g.k, (int)d.size(), kind});
     }
     if (t_recorder) ++t_recorder->calls;
+    // A latency-bound call of few descriptors (the block inverse's in-window
+    // update of a few large boxes has a left and a right part of different
+    // shapes): when its descriptors were routed to several kernels, they all
+    // go to the one that carries most of its flops (where that kernel takes
+    // every one of them), so the call is one launch.
+    if (one_launch && (int)d.size() <= kUnifyMaxDesc && force_kind < 0 &&
+        (count[0] > 0) + (count[1] > 0) + (count[2] > 0) > 1) {
+        double fl[3] = {0, 0, 0};
+        for (size_t i = 0; i < d.size(); ++i)
+            if (kinds[i] >= 0) fl[kinds[i]] += (double)d[i].m * d[i].n * d[i].k;
+        const int best = fl[0] >= fl[1] ? (fl[0] >= fl[2] ? 0 : 2) : (fl[1] >= fl[2] ? 1 : 2);
+        bool ok = true;
+        for (size_t i = 0; i < d.size() && best > 0; ++i) {
+            const GemmDesc& g = d[i];
+            if (kinds[i] < 0) continue;
+            const double ext_a = (double)g.lda * (transA ? g.m : g.k) + 16.0 * g.lda + 256;
+            const double ext_b = (double)g.ldb * g.n + 16.0 * g.ldb + 256;
+            const bool fits = 8.0 * std::max(ext_a, ext_b) < 2147483648.0;
+            ok = ok && fits && g.k > 0 && g.m >= 32 && g.n >= 32;
+        }
+        if (ok)
+            for (size_t i = 0; i < d.size(); ++i)
+                if (kinds[i] >= 0 && kinds[i] != best) {
+                    --count[kinds[i]];
+                    kinds[i] = (signed char)best;
+                    ++count[best];
+                }
+    }
     desc.reserve(desc.size() + d.size());
     tiles.reserve(tiles.size() + d.size() + 3);
     for (int pass = 0; pass < 3; ++pass) {

@@ -186,8 +186,10 @@ struct GemmPlan {
     }
     // Appends one batched GEMM (outputs narrower than half an MFMA tile on
     // the FMA kernel, the rest on the MFMA kernel); returns its launches
-    // [first, second).
-    std::pair<int, int> add(const std::vector<GemmDesc>& d, bool transA);
+    // [first, second).  one_launch: a latency-bound call of few descriptors
+    // whose descriptors would go to several kernels goes to one of them.
+    std::pair<int, int> add(const std::vector<GemmDesc>& d, bool transA,
+                            bool one_launch = false);
     void upload(hipStream_t s) {
         d_desc.upload(desc, s);
         d_tiles.upload(tiles, s);
