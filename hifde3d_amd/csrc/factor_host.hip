@@ -845,15 +845,6 @@ struct hifde_factor {
         for (int g : B.red_gid) remaining[g] = 0;  // UpdateRemainingDOFs (:91-111)
         if (!elem_on) d_alive.upload(remaining, stream);  // read by the table compaction
     }
-    // Cells: the host mask is updated as soon as the level's cells are
-    // planned; the table path's device mask follows after the cells' Schur
-    // step, before the scatter's compaction reads it.
-    void mark_cells_host(const BoxBatch& B) {
-        for (int g : B.red_gid) remaining[g] = 0;  // UpdateRemainingDOFs (:91-111)
-    }
-    void mark_cells_device() {
-        if (!elem_on) d_alive.upload(remaining, stream);
-    }
 
     // Number of S entries written in the second scatter pass (both DOFs on
     // min faces of their cell).
@@ -912,7 +903,6 @@ struct hifde_factor {
         timed(ST_PLAN, t0);
         if (!cached) upload_batch_index(B);
         timed(ST_UPLOAD, t0);
-        mark_cells_host(B);
         if (elem_on) {
             if constexpr (std::is_same<T, float>::value) {
                 if (elem32) {
@@ -951,7 +941,7 @@ struct hifde_factor {
         }
         timed(ST_GATHER, t0);
         schur_batch<T>(B, A21, A12, S, B.off_ss.data(), B.s.data(), false, t0);
-        mark_cells_device();
+        mark_eliminated(B);
         scatter<T>(B, S, true, t0);
     }
 
@@ -1087,7 +1077,7 @@ struct hifde_factor {
             mirror_lower(md, d_mir, d_eoff, E, stream);
         }
         timed(ST_SCATTER, t0);
-        mark_cells_device();
+        mark_eliminated(B);
     }
 
     // Index of one level's faces: the plan and its device copies.
