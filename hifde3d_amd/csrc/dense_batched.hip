@@ -821,7 +821,8 @@ __global__ void __launch_bounds__(256, 2) k_gemm_mfma(int nbox, const int* __res
                                                       const GemmDesc* __restrict__ desc,
                                                       const T* __restrict__ Ab,
                                                       const T* __restrict__ Bb, TC* Cb, T alpha,
-                                                      T beta, const int* __restrict__ flags) {
+                                                      T beta, const int* __restrict__ flags,
+                                                      int contig) {
     typedef MfmaCfg<T> C;
     constexpr int MT = C::MT, KT = C::KT, BLK = TL / (2 * MT), WT = BLK * MT, TILE = 2 * WT,
                   KB = 16;
@@ -830,14 +831,35 @@ __global__ void __launch_bounds__(256, 2) k_gemm_mfma(int nbox, const int* __res
     constexpr int NLD = TILE * KB / 256;  // elements per thread and operand per slab
     static_assert(TILE == TL && BLK >= 1 && TILE % 16 == 0, "tile");
     static_assert(SPL * sizeof(T) == 32, "a lane's slab share is 32 bytes");
-    const int t = blockIdx.x;
+    // Workgroups are dealt to the eight XCDs round-robin.  With contig each
+    // XCD is given a contiguous range of the tile list (column-major tiles of
+    // a box), so the tiles sharing an operand panel run in one XCD's L2.
+    int t = (int)blockIdx.x;
+    if (contig) {
+        const int nwg = (int)gridDim.x, per = nwg >> 3, rem = nwg & 7;
+        const int xcd = (int)blockIdx.x & 7;
+        t = xcd * per + min(xcd, rem) + ((int)blockIdx.x >> 3);
+    }
     const int b = find_box(tile_off, nbox, t);
     const int local = t - tile_off[b];
     const GemmDesc dd = desc[b];
     const int m = dd.m, n = dd.n, k = dd.k;
     const int tm = (m + TILE - 1) / TILE;
-    const int i0 = (local % tm) * TILE, j0 = (local / tm) * TILE;
-    if (dd.lower && j0 > i0) return;
+    int i0, j0;
+    if (dd.lower) {
+        // lower-triangle descriptors list only the tiles on or below the
+        // diagonal, column after column (column J holds rows J .. tm - 1)
+        int J = 0, r = local;
+        while (r >= tm - J) {
+            r -= tm - J;
+            ++J;
+        }
+        i0 = (J + r) * TILE;
+        j0 = J * TILE;
+    } else {
+        i0 = (local % tm) * TILE;
+        j0 = (local / tm) * TILE;
+    }
     if (dd.flag >= 0 && flags && flags[dd.flag] == 0) return;
     TC* Cm = Cb + dd.offC;
     // operand extents in bytes: A is m x k (lda) or k x m (lda, TRANSA), B is k x n (ldb)
@@ -2135,7 +2157,9 @@ std::pair<int, int> GemmPlan::add(const std::vector<GemmDesc>& d, bool transA, b
             const GemmDesc& g = d[i];
             const int tm = (g.m + tile - 1) / tile, tn = (g.n + tile - 1) / tile;
             desc.push_back(g);
-            tiles.push_back(tiles.back() + tm * tn);
+            // the matrix-core kernel lists only a lower descriptor's tiles on
+            // or below the diagonal
+            tiles.push_back(tiles.back() + (pass >= 1 ? tiles_of(g, tile) : (long long)tm * tn));
             ++L.nb;
             const double cells = g.lower ? 0.5 * g.m * (g.n + 1.0) : (double)g.m * g.n;
             L.flops += 2.0 * cells * g.k;
@@ -2143,6 +2167,7 @@ std::pair<int, int> GemmPlan::add(const std::vector<GemmDesc>& d, bool transA, b
             L.ab_elems += (double)g.m * g.k + (double)g.k * g.n;
         }
         L.total = tiles.back();
+        L.contig = pass >= 1 ? 1 : 0;
         launches.push_back(L);
     }
     return {first, (int)launches.size()};
@@ -2181,38 +2206,38 @@ void gemm_run(const GemmPlan& p, std::pair<int, int> h, const T* d_A, const T* d
                 if constexpr (std::is_same<T, float>::value) {
                     if (L.transA)
                         k_gemm_mfma<T, true, TC, true, 64><<<L.total, 256, 0, s>>>(
-                            L.nb, tt, dd, d_A, d_B, d_C, alpha, beta, d_flags);
+                            L.nb, tt, dd, d_A, d_B, d_C, alpha, beta, d_flags, L.contig);
                     else
                         k_gemm_mfma<T, false, TC, true, 64><<<L.total, 256, 0, s>>>(
-                            L.nb, tt, dd, d_A, d_B, d_C, alpha, beta, d_flags);
+                            L.nb, tt, dd, d_A, d_B, d_C, alpha, beta, d_flags, L.contig);
                 } else {
                     throw Error(-1, "split (bf16x3) products need FP32 operands");
                 }
             } else if (L.transA) {
                 k_gemm_mfma<T, true, TC, false, 64><<<L.total, 256, 0, s>>>(
-                    L.nb, tt, dd, d_A, d_B, d_C, alpha, beta, d_flags);
+                    L.nb, tt, dd, d_A, d_B, d_C, alpha, beta, d_flags, L.contig);
             } else {
                 k_gemm_mfma<T, false, TC, false, 64><<<L.total, 256, 0, s>>>(
-                    L.nb, tt, dd, d_A, d_B, d_C, alpha, beta, d_flags);
+                    L.nb, tt, dd, d_A, d_B, d_C, alpha, beta, d_flags, L.contig);
             }
         } else if (L.split) {
             if constexpr (std::is_same<T, float>::value) {
                 if (L.transA)
                     k_gemm_mfma<T, true, TC, true><<<L.total, 256, 0, s>>>(
-                        L.nb, tt, dd, d_A, d_B, d_C, alpha, beta, d_flags);
+                        L.nb, tt, dd, d_A, d_B, d_C, alpha, beta, d_flags, L.contig);
                 else
                     k_gemm_mfma<T, false, TC, true><<<L.total, 256, 0, s>>>(
-                        L.nb, tt, dd, d_A, d_B, d_C, alpha, beta, d_flags);
+                        L.nb, tt, dd, d_A, d_B, d_C, alpha, beta, d_flags, L.contig);
             } else {
                 throw Error(-1, "split (bf16x3) products need FP32 operands");
             }
         } else {
             if (L.transA)
                 k_gemm_mfma<T, true, TC><<<L.total, 256, 0, s>>>(L.nb, tt, dd, d_A, d_B, d_C,
-                                                                 alpha, beta, d_flags);
+                                                                 alpha, beta, d_flags, L.contig);
             else
                 k_gemm_mfma<T, false, TC><<<L.total, 256, 0, s>>>(L.nb, tt, dd, d_A, d_B, d_C,
-                                                                  alpha, beta, d_flags);
+                                                                  alpha, beta, d_flags, L.contig);
         }
         HIFDE_LAUNCH_CHECK();
         if (kt)

@@ -155,6 +155,7 @@ struct GemmPlan {
         int transA, nb, d0, t0, total;
         double flops, ab_elems, cells;  // algorithmic work of the launch
         int split;  // matrix-core launch with bf16x3 split products (FP32 only)
+        int contig = 0;  // matrix-core launch: contiguous tile range per XCD
     };
     // Launches added while set use the split (bf16x3) matrix-core products;
     // clear() leaves it as it is.
