@@ -627,14 +627,28 @@ __global__ void __launch_bounds__(256) k_gemm(int nbox, const int* __restrict__ 
                                               const T* __restrict__ Ab, const T* __restrict__ Bb,
                                               TC* Cb, T alpha, T beta,
                                               const int* __restrict__ flags) {
-    const int t = blockIdx.x;
+    // a contiguous range of the tile list per XCD, as in k_gemm_mfma
+    const int nwg = (int)gridDim.x, per = nwg >> 3, rem = nwg & 7;
+    const int xcd = (int)blockIdx.x & 7;
+    const int t = xcd * per + min(xcd, rem) + ((int)blockIdx.x >> 3);
     const int b = find_box(tile_off, nbox, t);
     const int local = t - tile_off[b];
     const GemmDesc dd = desc[b];
     const int m = dd.m, n = dd.n, k = dd.k;
     const int tm = (m + BM - 1) / BM;
-    const int i0 = (local % tm) * BM, j0 = (local / tm) * BN;
-    if (dd.lower && j0 > i0) return;
+    int i0, j0;
+    if (dd.lower) {  // only the tiles on or below the diagonal are listed
+        int J = 0, r = local;
+        while (r >= tm - J) {
+            r -= tm - J;
+            ++J;
+        }
+        i0 = (J + r) * BM;
+        j0 = J * BN;
+    } else {
+        i0 = (local % tm) * BM;
+        j0 = (local / tm) * BN;
+    }
     if (dd.flag >= 0 && flags && flags[dd.flag] == 0) return;
     const T* A = Ab + dd.offA;
     const T* B = Bb + dd.offB;
@@ -2157,9 +2171,9 @@ std::pair<int, int> GemmPlan::add(const std::vector<GemmDesc>& d, bool transA, b
             const GemmDesc& g = d[i];
             const int tm = (g.m + tile - 1) / tile, tn = (g.n + tile - 1) / tile;
             desc.push_back(g);
-            // the matrix-core kernel lists only a lower descriptor's tiles on
-            // or below the diagonal
-            tiles.push_back(tiles.back() + (pass >= 1 ? tiles_of(g, tile) : (long long)tm * tn));
+            // both kernels list only a lower descriptor's tiles on or below
+            // the diagonal
+            tiles.push_back(tiles.back() + tiles_of(g, tile));
             ++L.nb;
             const double cells = g.lower ? 0.5 * g.m * (g.n + 1.0) : (double)g.m * g.n;
             L.flops += 2.0 * cells * g.k;
