@@ -593,17 +593,22 @@ GEMM_FORCE = {0: 0x800, 1: 0x1000, 2: 0x2000}
 GEMM_RULE = 0x4000
 
 
+GEMM_LOWER = 0x8000
+
+
 def gemm_batch(pairs, precision="fp64", transA=False, bf16x3=False, tile64=False, kind=None,
-               rule=False):
+               rule=False, lower=False):
     """[op(A) @ B for (A, B) in pairs] through the library's batched GEMM.
     bf16x3 (precision "mixed" only): matrix-core products as bf16x3 split
     products.  tile64: products with both output sides of at least 32 on
     64 x 64 matrix-core tiles.  kind: force one kernel (0 FMA, 1 matrix-core
     128, 2 matrix-core 64).  rule: route by the fixed rule even where a loaded
-    routing table has an entry."""
+    routing table has an entry.  lower: square products compute only their
+    output tiles on or below the diagonal (only the lower triangle of such a
+    result is defined)."""
     prec = ({"fp64": FP64, "mixed": MIXED}[precision] | (BF16X3 if bf16x3 else 0)
             | (GEMM_TILE64 if tile64 else 0) | (GEMM_FORCE[kind] if kind is not None else 0)
-            | (GEMM_RULE if rule else 0))
+            | (GEMM_RULE if rule else 0) | (GEMM_LOWER if lower else 0))
     m = np.array([(a.shape[1] if transA else a.shape[0]) for a, _ in pairs], np.int32)
     k = np.array([(a.shape[0] if transA else a.shape[1]) for a, _ in pairs], np.int32)
     n = np.array([b.shape[1] for _, b in pairs], np.int32)

@@ -813,6 +813,9 @@ struct SlabPitch<float> {
     static constexpr int value = 20;  // 80 B: 20 words, idem
 };
 
+// Tile rows per group in the order of a general descriptor's output tiles.
+constexpr int kGemmGroupRows = 8;
+
 // C(m x n) = alpha op(A) B + beta C over a flattened list of 128 x 128 output
 // tiles of every descriptor, 4 waves in a 2 x 2 arrangement.  The k dimension
 // is walked in slabs of 16 held in two LDS buffers: the next slab is loaded
@@ -871,8 +874,15 @@ __global__ void __launch_bounds__(256, 2) k_gemm_mfma(int nbox, const int* __res
         i0 = (J + r) * TILE;
         j0 = J * TILE;
     } else {
-        i0 = (local % tm) * TILE;
-        j0 = (local / tm) * TILE;
+        // groups of kGemmGroupRows tile rows, column-major within a group:
+        // the tiles an XCD runs at once cover a few rows and a few columns
+        // of the box, so they share their A row panels as well as their B
+        // column panels in that XCD's L2
+        const int tn = (n + TILE - 1) / TILE;
+        const int gsz = kGemmGroupRows * tn, g = local / gsz, first = g * kGemmGroupRows;
+        const int rows = min(kGemmGroupRows, tm - first), w = local - g * gsz;
+        i0 = (first + w % rows) * TILE;
+        j0 = (w / rows) * TILE;
     }
     if (dd.flag >= 0 && flags && flags[dd.flag] == 0) return;
     TC* Cm = Cb + dd.offC;
@@ -2278,7 +2288,8 @@ void batched_gemm_desc(bool transA, const std::vector<GemmDesc>& desc, const T* 
 template <class T>
 void batched_gemm(bool transA, int nbox, const int* h_m, const int* h_n, const int* h_k,
                   const long long* h_offA, const long long* h_offB, const long long* h_offC,
-                  const T* d_A, const T* d_B, T* d_C, T alpha, GemmWork& ws, hipStream_t s) {
+                  const T* d_A, const T* d_B, T* d_C, T alpha, GemmWork& ws, hipStream_t s,
+                  bool lower) {
     std::vector<GemmDesc> desc(nbox);
     for (int b = 0; b < nbox; ++b) {
         GemmDesc& d = desc[b];
@@ -2291,6 +2302,7 @@ void batched_gemm(bool transA, int nbox, const int* h_m, const int* h_n, const i
         d.lda = std::max(transA ? d.k : d.m, 1);
         d.ldb = std::max(d.k, 1);
         d.ldc = std::max(d.m, 1);
+        d.lower = lower && d.m == d.n ? 1 : 0;
     }
     batched_gemm_desc<T>(transA, desc, d_A, d_B, d_C, alpha, (T)0, ws, s);
 }
@@ -2532,11 +2544,11 @@ template void batched_gemm_desc<float, double>(bool, const std::vector<GemmDesc>
 template void batched_gemm<double>(bool, int, const int*, const int*, const int*,
                                    const long long*, const long long*, const long long*,
                                    const double*, const double*, double*, double, GemmWork&,
-                                   hipStream_t);
+                                   hipStream_t, bool);
 template void batched_gemm<float>(bool, int, const int*, const int*, const int*,
                                   const long long*, const long long*, const long long*,
                                   const float*, const float*, float*, float, GemmWork&,
-                                  hipStream_t);
+                                  hipStream_t, bool);
 template void copy_blocks<double>(const std::vector<CopyDesc>&, const std::vector<long long>&,
                                   const double*, double*, GemmWork&, hipStream_t);
 template void copy_blocks<float>(const std::vector<CopyDesc>&, const std::vector<long long>&,

@@ -3504,7 +3504,7 @@ namespace {
 template <class T>
 int gemm_batch_impl(int transA, int nb, const int* m, const int* n, const int* k, const double* A,
                     const double* B, double* C, bool split, int tile64, int force,
-                    bool table) {
+                    bool table, bool lower) {
     hipStream_t st = aux_stream();
     OwnerGuard og(st);
     std::vector<long long> oA(nb + 1, 0), oB(nb + 1, 0), oC(nb + 1, 0);
@@ -3528,7 +3528,7 @@ int gemm_batch_impl(int transA, int nb, const int* m, const int* n, const int* k
     gws.plan.force_kind = force;
     gws.plan.use_table = table;
     batched_gemm<T>(transA != 0, nb, m, n, k, oA.data(), oB.data(), oC.data(), dA.p, dB.p, dC.p,
-                    (T)1, gws, st);
+                    (T)1, gws, st, lower);
     std::vector<T> tmp(oC[nb]);
     if (oC[nb])
         HIFDE_CHECK(
@@ -3554,16 +3554,17 @@ int hifde_gemm_batch(int precision, int transA, int nbatch, const int* m, const 
                           : (precision & HIFDE_GEMM_FORCE_CORE64)  ? 2
                                                                    : -1;
         const bool table = (precision & HIFDE_GEMM_RULE) == 0;
+        const bool lower = (precision & HIFDE_GEMM_LOWER) != 0;
         const int prec = precision & ~(HIFDE_BF16X3 | HIFDE_GEMM_TILE64 | HIFDE_GEMM_FORCE_FMA |
                                        HIFDE_GEMM_FORCE_CORE128 | HIFDE_GEMM_FORCE_CORE64 |
-                                       HIFDE_GEMM_RULE);
+                                       HIFDE_GEMM_RULE | HIFDE_GEMM_LOWER);
         if (split && prec != HIFDE_MIXED)
             throw Error(HIFDE_ERR_ARG, "gemm_batch: HIFDE_BF16X3 needs HIFDE_MIXED");
         if (prec == HIFDE_MIXED)
             return gemm_batch_impl<float>(transA, nbatch, m, n, k, A, B, C, split, tile64, force,
-                                          table);
+                                          table, lower);
         return gemm_batch_impl<double>(transA, nbatch, m, n, k, A, B, C, false, tile64, force,
-                                       table);
+                                       table, lower);
     });
 }
 

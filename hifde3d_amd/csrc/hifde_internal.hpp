@@ -378,11 +378,13 @@ template <class T, class TC = T>
 void batched_gemm_desc(bool transA, const std::vector<GemmDesc>& desc, const T* d_A, const T* d_B,
                        TC* d_C, T alpha, T beta, GemmWork& ws, hipStream_t s,
                        const int* d_flags = nullptr);
-// Packed boxes (ld = rows): C = alpha op(A) B with host offsets.
+// Packed boxes (ld = rows): C = alpha op(A) B with host offsets.  lower:
+// square boxes compute only their output tiles on or below the diagonal.
 template <class T>
 void batched_gemm(bool transA, int nbox, const int* h_m, const int* h_n, const int* h_k,
                   const long long* h_offA, const long long* h_offB, const long long* h_offC,
-                  const T* d_A, const T* d_B, T* d_C, T alpha, GemmWork& ws, hipStream_t s);
+                  const T* d_A, const T* d_B, T* d_C, T alpha, GemmWork& ws, hipStream_t s,
+                  bool lower = false);
 template <class D>
 void copy_blocks(const std::vector<CopyDesc>& desc, const std::vector<long long>& dst_off,
                  const double* d_src, D* d_dst, GemmWork& ws, hipStream_t s);

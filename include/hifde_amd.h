@@ -379,6 +379,13 @@ int hifde_triangular_solve_batch(int nbatch, const int* m, const int* n, const d
  * entries otherwise take precedence over the rule for the classes they cover.
  * No counterpart in the reference. */
 #define HIFDE_GEMM_RULE 0x4000
+/* OR-ed into the precision of hifde_gemm_batch: square products (m == n)
+ * compute only the output tiles on or below the block diagonal, as the
+ * factorization's symmetric products (S = -A21^T X, the symmetric inverse's
+ * trailing updates) do; only the lower triangle of such a C is defined.
+ * Products with m != n are computed whole.  No counterpart in the
+ * reference. */
+#define HIFDE_GEMM_LOWER 0x8000
 int hifde_schur_batch(int precision, int nbatch, const int* r, const int* s, const double* A22,
                       const double* A21, const double* A12, double* A22inv, double* X, double* S,
                       int* info);

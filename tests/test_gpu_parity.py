@@ -1151,6 +1151,31 @@ def test_gemm_forced_kernels_match_numpy(H, precision, tol, transA, kind):
         assert _rel(C, ref) < tol
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 1e-5)])
+@pytest.mark.parametrize("transA", [False, True])
+@pytest.mark.parametrize("kind", [0, 1, 2])
+def test_gemm_lower_tiles_match_numpy(H, precision, tol, transA, kind):
+    """Lower-triangle products (only the output tiles on or below the block
+    diagonal, as the symmetric Schur complements and the symmetric inverse's
+    trailing updates use them) on each candidate kernel, one batch of square
+    products whose sides fall on and off the 64- and 128-wide tile edges,
+    with a non-square product in the same batch (computed whole): the lower
+    triangle against NumPy."""
+    rng = _rng(41 + kind)
+    shapes = [(32, 32, 5), (64, 64, 64), (65, 65, 17), (129, 129, 40), (100, 60, 20),
+              (300, 300, 33), (515, 515, 70), (1030, 1030, 9)]
+    pairs = []
+    for m, n, k in shapes:
+        A = rng.standard_normal((k, m) if transA else (m, k))
+        pairs.append((A, rng.standard_normal((k, n))))
+    out = H.gemm_batch(pairs, precision, transA=transA, kind=kind, lower=True)
+    for (m, n, _), (A, B), C in zip(shapes, pairs, out):
+        ref = (A.T if transA else A) @ B
+        if m == n:
+            ref, C = np.tril(ref), np.tril(C)
+        assert _rel(C, ref) < tol, (m, n)
+
+
 def test_gemm_tuner_table_loads_and_routing_keeps_the_factor(H, oracle_mod, tmp_path):
     """The shipped routing table parses; a factorization routed by it and one
     routed by the fixed rule give the same level counts and preconditioners
