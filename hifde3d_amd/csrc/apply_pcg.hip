@@ -38,7 +38,8 @@ __device__ __forceinline__ int find_box(const O* __restrict__ off, int nbox, lon
     return lo;
 }
 
-// One 64-output tile of one box per workgroup.
+// One tile of 64 RW outputs of one box per workgroup (RW = 2: ops 1, 3 and
+// 4 only, each lane forming two outputs 64 rows apart).
 // op 0: u_s[j] += alpha * sum_i M(i,j) u_r[i]   M: r x s   (X^H u_red)
 // op 1: u_r[i] += alpha * sum_j M(i,j) u_s[j]   M: r x s   (X u_skel)
 // op 2: u_r[j] += alpha * sum_i M(i,j) u_s[i]   M: s x r   (W^H u_skel)
@@ -47,7 +48,7 @@ __device__ __forceinline__ int find_box(const O* __restrict__ off, int nbox, lon
 // Ops 0-3 read and write disjoint DOF classes, so they update u in place;
 // op 4 reads u_in and writes u_out (a different vector).  pass >= 0 restricts
 // op 0/3 writes to skeleton DOFs whose pass flag equals pass.
-template <class T>
+template <class T, int RW>
 __global__ void __launch_bounds__(256) k_apply_tiles(
     int nbox, const int* __restrict__ tile_off, const int* __restrict__ dr,
     const int* __restrict__ ds, const int* __restrict__ red_off,
@@ -76,50 +77,70 @@ __global__ void __launch_bounds__(256) k_apply_tiles(
     const int* ig = in_red ? rg : sg;
     const int* og = out_red ? rg : sg;
     extern __shared__ double xs[];
-    __shared__ double part[4][64];
+    __shared__ double part[4][64 * RW];
     const int tid = threadIdx.x;
     for (int q = tid; q < nin; q += 256) xs[q] = u_in[ig[q]];
     __syncthreads();
-    const int o0 = tile * 64;
-    const int cnt = min(64, nout - o0);
+    const int o0 = tile * 64 * RW;
+    const int cnt = min(64 * RW, nout - o0);
     double acc = 0.0;
     int o = -1;
     if (!trans) {
         // y_i = sum_j M(i,j) x_j ; M has nout rows.  Four column groups.
+        // Each lane forms rows i + 64 q (q < RW); a wave's loads of one
+        // column cover 64 RW consecutive rows (256 RW bytes in FP32).
         const int i = tid & 63, g = tid >> 6;
-        double pacc = 0.0;
+        constexpr int U = 16 / RW;  // columns in flight per thread
+        double pacc[RW];
+#pragma unroll
+        for (int q = 0; q < RW; ++q) pacc[q] = 0.0;
         if (i < cnt) {
-            // sixteen independent sums keep sixteen loads per thread in
+            // U independent sums per row keep 16 loads per thread in
             // flight; past the last column the loads repeat it (cached) and
-            // are weighted by zero, so the loop has no branches
+            // are weighted by zero, and rows past the tile's last repeat
+            // that row (their sums are not used), so the loop has no branches
             const T* Mi = M + o0 + i;
-            double a[16];
+            int rof[RW];
 #pragma unroll
-            for (int u = 0; u < 16; ++u) a[u] = 0.0;
-            for (int j = g; j < nin; j += 64) {
-                T mv[16];
+            for (int q = 0; q < RW; ++q) rof[q] = min(64 * q, cnt - 1 - i);
+            double a[U][RW];
 #pragma unroll
-                for (int u = 0; u < 16; ++u)
-                    mv[u] = Mi[(size_t)min(j + 4 * u, nin - 1) * nout];
+            for (int u = 0; u < U; ++u)
 #pragma unroll
-                for (int u = 0; u < 16; ++u) {
+                for (int q = 0; q < RW; ++q) a[u][q] = 0.0;
+            for (int j = g; j < nin; j += 4 * U) {
+                T mv[U][RW];
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+#pragma unroll
+                    for (int q = 0; q < RW; ++q)
+                        mv[u][q] = Mi[(size_t)min(j + 4 * u, nin - 1) * nout + rof[q]];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
                     const int jj = j + 4 * u;
-                    const double xv = xs[min(jj, nin - 1)];
-                    a[u] += (double)mv[u] * (jj < nin ? xv : 0.0);
+                    const double xv = jj < nin ? xs[min(jj, nin - 1)] : 0.0;
+#pragma unroll
+                    for (int q = 0; q < RW; ++q) a[u][q] += (double)mv[u][q] * xv;
                 }
             }
 #pragma unroll
-            for (int w = 8; w > 0; w >>= 1)
+            for (int w = U / 2; w > 0; w >>= 1)
 #pragma unroll
-                for (int u = 0; u < w; ++u) a[u] += a[u + w];
-            pacc = a[0];
+                for (int u = 0; u < w; ++u)
+#pragma unroll
+                    for (int q = 0; q < RW; ++q) a[u][q] += a[u + w][q];
+#pragma unroll
+            for (int q = 0; q < RW; ++q) pacc[q] = a[0][q];
         }
-        part[g][i] = pacc;
+#pragma unroll
+        for (int q = 0; q < RW; ++q) part[g][i + 64 * q] = pacc[q];
         __syncthreads();
         if (tid < cnt) {
             o = o0 + tid;
             acc = part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
         }
+    } else if constexpr (RW != 1) {
+        return;  // the host launches RW = 2 for ops 1, 3 and 4 only
     } else if (nin <= 64) {
         // y_j = sum_i M(i,j) x_i with short columns: one thread per output
         // (outputs of the other pass are not formed)
@@ -359,16 +380,27 @@ void apply_tiles(int nbox, int ntiles, const int* d_tile_off, const int* d_r, co
                  const int* d_skel_gid, const unsigned char* d_skel_pass, const long long* d_off,
                  const T* d_M, int op, int pass, double alpha, const double* d_in,
                  double* d_out, int max_in, double flops, double bytes, hipStream_t s,
-                 long long side_n) {
+                 long long side_n, int rows_per_lane) {
     if (nbox <= 0 || ntiles <= 0) return;
+    if (rows_per_lane != 1 && (rows_per_lane != 2 || op == 0 || op == 2))
+        throw Error(-3, "apply: two rows per lane are for ops 1, 3 and 4 only");
     size_t shm = (size_t)std::max(max_in, 1) * sizeof(double);
     if (shm > 150 * 1024) throw Error(-3, "apply: box too large for LDS");
     KernelTimer* kt = active_timer();
+    const bool f32 = sizeof(T) == sizeof(float);
     if (kt)
-        kt->begin(s, sizeof(T) == sizeof(float) ? "k_apply_tiles<float>" : "k_apply_tiles<double>");
-    k_apply_tiles<T><<<ntiles, 256, shm, s>>>(nbox, d_tile_off, d_r, d_s, d_red_off, d_skel_off,
-                                              d_red_gid, d_skel_gid, d_skel_pass, d_off, d_M, op,
-                                              pass, alpha, d_in, d_out, side_n);
+        kt->begin(s, rows_per_lane == 2 ? (f32 ? "k_apply_tiles<float, 2>" : "k_apply_tiles<double, 2>")
+                                        : (f32 ? "k_apply_tiles<float, 1>" : "k_apply_tiles<double, 1>"));
+    if (rows_per_lane == 2)
+        k_apply_tiles<T, 2><<<ntiles, 256, shm, s>>>(nbox, d_tile_off, d_r, d_s, d_red_off,
+                                                     d_skel_off, d_red_gid, d_skel_gid,
+                                                     d_skel_pass, d_off, d_M, op, pass, alpha,
+                                                     d_in, d_out, side_n);
+    else
+        k_apply_tiles<T, 1><<<ntiles, 256, shm, s>>>(nbox, d_tile_off, d_r, d_s, d_red_off,
+                                                     d_skel_off, d_red_gid, d_skel_gid,
+                                                     d_skel_pass, d_off, d_M, op, pass, alpha,
+                                                     d_in, d_out, side_n);
     HIFDE_LAUNCH_CHECK();
     if (kt) kt->end(s, flops, bytes);
 }
@@ -433,10 +465,10 @@ template void apply_tiles<double>(int, int, const int*, const int*, const int*, 
                                   const int*, const int*, const int*, const unsigned char*,
                                   const long long*, const double*, int, int, double,
                                   const double*, double*, int, double, double, hipStream_t,
-                                  long long);
+                                  long long, int);
 template void apply_tiles<float>(int, int, const int*, const int*, const int*, const int*,
                                  const int*, const int*, const int*, const unsigned char*,
                                  const long long*, const float*, int, int, double, const double*,
-                                 double*, int, double, double, hipStream_t, long long);
+                                 double*, int, double, double, hipStream_t, long long, int);
 
 }  // namespace hifde

@@ -1647,10 +1647,29 @@ struct hifde_factor {
                            2.0 * B.tot_rs, by, stream, side_n);
             return;
         }
+        if (wide_tiles(op, out_red ? B.max_r : B.max_s, out_red ? B.tiles_r2 : B.tiles_s2)) {
+            const std::vector<int>& t2 = out_red ? B.tiles_r2 : B.tiles_s2;
+            apply_tiles<T>(B.nbox, t2.back(), out_red ? B.d_tiles_r2.p : B.d_tiles_s2.p, B.d_r.p,
+                           B.d_s.p, B.d_red_off.p, B.d_skel_off.p, B.d_red_gid.p, B.d_skel_gid.p,
+                           spp, B.d_off_rs.p, reinterpret_cast<const T*>(Mbuf.p), op, pass,
+                           alpha, u, uo, in_red ? B.max_r : B.max_s, 2.0 * B.tot_rs, by, stream,
+                           side_n, 2);
+            return;
+        }
         apply_tiles<T>(B.nbox, tl.back(), out_red ? B.d_tiles_r.p : B.d_tiles_s.p, B.d_r.p,
                        B.d_s.p, B.d_red_off.p, B.d_skel_off.p, B.d_red_gid.p, B.d_skel_gid.p, spp,
                        B.d_off_rs.p, reinterpret_cast<const T*>(Mbuf.p), op, pass, alpha, u, uo,
                        in_red ? B.max_r : B.max_s, 2.0 * B.tot_rs, by, stream, side_n);
+    }
+    // Tiles of two rows per lane (128 outputs) for the non-transposed ops
+    // (1, 3, 4): a wave's loads of one column then cover 512 contiguous
+    // bytes instead of 256, which streams at 5.7-6.3 TB/s against 4.3 on
+    // large column-major FP32 blocks (about 5.0 against 4.7 TB/s on the
+    // level blocks).  Taken where outputs run past 64 and the launch still
+    // has enough workgroups to fill the chip.
+    static bool wide_tiles(int op, int max_out, const std::vector<int>& tiles2) {
+        return (op == 1 || op == 3 || op == 4) && max_out > 64 && !tiles2.empty() &&
+               tiles2.back() >= kApplyWideMinTiles;
     }
     template <class T>
     void diag_t(BoxBatch& B, const DevBuf<unsigned char>& Mbuf, const double* u, double* out) {
@@ -1665,24 +1684,32 @@ struct hifde_factor {
                            by, stream);
             return;
         }
-        apply_tiles<T>(B.nbox, B.tiles_r.back(), B.d_tiles_r.p, B.d_r.p, B.d_s.p, B.d_red_off.p,
+        const bool wide = wide_tiles(4, B.max_r, B.tiles_r2);
+        apply_tiles<T>(B.nbox, wide ? B.tiles_r2.back() : B.tiles_r.back(),
+                       wide ? B.d_tiles_r2.p : B.d_tiles_r.p, B.d_r.p, B.d_s.p, B.d_red_off.p,
                        B.d_skel_off.p, B.d_red_gid.p, B.d_skel_gid.p, nullptr, B.d_off_rr.p,
                        reinterpret_cast<const T*>(Mbuf.p), 4, -1, 1.0, u, out, B.max_r,
-                       2.0 * B.tot_rr, by, stream);
+                       2.0 * B.tot_rr, by, stream, 0, wide ? 2 : 1);
     }
     // Apply tiling of a planned batch.
     void finalize_tiles(BoxBatch& B) {
         B.tiles_r.assign(B.nbox + 1, 0);
         B.tiles_s.assign(B.nbox + 1, 0);
+        B.tiles_r2.assign(B.nbox + 1, 0);
+        B.tiles_s2.assign(B.nbox + 1, 0);
         B.max_r = B.max_s = 0;
         for (int b = 0; b < B.nbox; ++b) {
             B.tiles_r[b + 1] = B.tiles_r[b] + (B.r[b] + 63) / 64;
             B.tiles_s[b + 1] = B.tiles_s[b] + (B.s[b] + 63) / 64;
+            B.tiles_r2[b + 1] = B.tiles_r2[b] + (B.r[b] + 127) / 128;
+            B.tiles_s2[b + 1] = B.tiles_s2[b] + (B.s[b] + 127) / 128;
             B.max_r = std::max(B.max_r, B.r[b]);
             B.max_s = std::max(B.max_s, B.s[b]);
         }
         B.d_tiles_r.upload(B.tiles_r, stream);
         B.d_tiles_s.upload(B.tiles_s, stream);
+        B.d_tiles_r2.upload(B.tiles_r2, stream);
+        B.d_tiles_s2.upload(B.tiles_s2, stream);
     }
 
     // Apply (src/Factor.cpp:992-1081), one vector.  Each batch's blocks are

@@ -214,9 +214,10 @@ struct BoxBatch {
     // Factor data in the factor precision (float or double), as raw bytes.
     DevBuf<unsigned char> A22, A22inv, X, W;
     long long tot_rr = 0, tot_rs = 0, tot_ss = 0;
-    // apply tiling: prefix sums of ceil(r/64) and ceil(s/64)
-    std::vector<int> tiles_r, tiles_s;
-    DevBuf<int> d_tiles_r, d_tiles_s;
+    // apply tiling: prefix sums of ceil(r/64) and ceil(s/64), and of
+    // ceil(r/128) and ceil(s/128) (tiles of two rows per lane)
+    std::vector<int> tiles_r, tiles_s, tiles_r2, tiles_s2;
+    DevBuf<int> d_tiles_r, d_tiles_s, d_tiles_r2, d_tiles_s2;
     int max_r = 0, max_s = 0;
     // Faces only: per-face box geometry.
     std::vector<int> face_kind;
@@ -240,7 +241,7 @@ struct BoxBatch {
         red_off.clear(); skel_off.clear(); other_off.clear();
         red_gid.clear(); skel_gid.clear(); other_gid.clear(); skel_pass.clear();
         tot_rr = tot_rs = tot_ss = tot_ee = 0;
-        tiles_r.clear(); tiles_s.clear();
+        tiles_r.clear(); tiles_s.clear(); tiles_r2.clear(); tiles_s2.clear();
         max_r = max_s = 0;
         face_kind.clear();
     }
@@ -502,14 +503,17 @@ template <class TE>
 void mirror_lower(const std::vector<MirrorDesc>& desc, DevBuf<MirrorDesc>& dbuf,
                   DevBuf<long long>& obuf, TE* E, hipStream_t s);
 
-// apply sweeps: one 64-output tile of one box per workgroup (see apply_pcg.hip)
+// apply sweeps: one tile of 64 (or 128) outputs of one box per workgroup
+// (see apply_pcg.hip).  Fewest 128-output tiles for which a launch of the non-transposed apply ops
+// uses two rows per lane.
+constexpr int kApplyWideMinTiles = 512;
 template <class T>
 void apply_tiles(int nbox, int ntiles, const int* d_tile_off, const int* d_r, const int* d_s,
                  const int* d_red_off, const int* d_skel_off, const int* d_red_gid,
                  const int* d_skel_gid, const unsigned char* d_skel_pass, const long long* d_off,
                  const T* d_M, int op, int pass, double alpha, const double* d_in,
                  double* d_out, int max_in, double flops, double bytes, hipStream_t s,
-                 long long side_n = 0);
+                 long long side_n = 0, int rows_per_lane = 1);
 // Batches whose boxes have at most 64 inputs and outputs: one thread per output.
 template <class T>
 void apply_small(int nbox, int nout_total, const int* d_r, const int* d_s, const int* d_red_off,
