@@ -238,6 +238,53 @@ __global__ void k_csr_spmv(int n, const int* __restrict__ rp, const int* __restr
     y[i] = acc;
 }
 
+// Row i's entries q < len read from the slot-major copy; the same products,
+// summed in the same order, as k_csr_spmv.
+__global__ void k_ell_spmv(int n, int width, const int* __restrict__ rp,
+                           const double* __restrict__ ev, const int* __restrict__ ec,
+                           const double* __restrict__ x, double* __restrict__ y) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int len = rp[i + 1] - rp[i];
+    double acc = 0.0;
+    for (int q = 0; q < len; ++q) {
+        const long long e = (long long)q * n + i;
+        acc += ev[e] * x[ec[e]];
+    }
+    y[i] = acc;
+}
+
+__global__ void k_csr_to_ell(int n, int width, const int* __restrict__ rp,
+                             const int* __restrict__ ci, const double* __restrict__ v,
+                             double* __restrict__ ev, int* __restrict__ ec) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int b = rp[i], len = rp[i + 1] - b;
+    for (int q = 0; q < width; ++q) {
+        const long long e = (long long)q * n + i;
+        ev[e] = q < len ? v[b + q] : 0.0;
+        ec[e] = q < len ? ci[b + q] : i;
+    }
+}
+
+__global__ void k_fill_zero_int(int* p) {
+    if (threadIdx.x == 0) p[threadIdx.x] = 0;
+}
+
+__global__ void k_max_row(int n, const int* __restrict__ rp, int* out) {
+    __shared__ int m[256];
+    int best = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        best = max(best, rp[i + 1] - rp[i]);
+    m[threadIdx.x] = best;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) m[threadIdx.x] = max(m[threadIdx.x], m[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMax(out, m[0]);
+}
+
 constexpr int kDotThreads = 256;
 
 __global__ void __launch_bounds__(256) k_dot_partial(const double* __restrict__ x,
@@ -313,8 +360,9 @@ __global__ void __launch_bounds__(256) k_apply_small(
     const int* __restrict__ red_gid, const int* __restrict__ skel_gid,
     const unsigned char* __restrict__ skel_pass, const long long* __restrict__ off,
     const T* __restrict__ Mb, int op, int pass, double alpha, const double* u_in,
-    double* u_out, long long side_n) {
+    double* u_out, long long side_n, int msz) {
     __shared__ double xs[4][64];
+    extern __shared__ __attribute__((aligned(16))) unsigned char mstage[];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int b = blockIdx.x * 4 + wv;
     if (b >= nbox) return;
@@ -326,13 +374,30 @@ __global__ void __launch_bounds__(256) k_apply_small(
     const int* rg = red_gid + red_off[b];
     const int* sg = skel_gid + skel_off[b];
     if (lane < nin) xs[wv][lane] = u_in[(in_red ? rg : sg)[lane]];
+    const T* M = Mb + off[b];
+    if (msz > 0) {
+        // The box's block is contiguous: the wave copies it into its LDS
+        // region with coalesced loads (eight in flight per lane), and the
+        // lanes then read their columns or rows from LDS.
+        T* Mw = reinterpret_cast<T*>(mstage) + (size_t)wv * msz;
+        const int cnt = nin * nout;
+        int e = lane;
+        for (; e + 7 * 64 < cnt; e += 8 * 64) {
+            T v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = M[e + 64 * u];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) Mw[e + 64 * u] = v[u];
+        }
+        for (; e < cnt; e += 64) Mw[e] = M[e];
+        M = Mw;
+    }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     if (lane >= nout) return;
     if (op == 0 && skel_pass && pass >= 0 && (int)skel_pass[skel_off[b] + lane] != pass) return;
-    const T* M = Mb + off[b];
     const double* x = xs[wv];
     double acc = 0.0;
     if (op == 0 || op == 2) {  // column lane of M (nin x nout)
@@ -357,19 +422,34 @@ __global__ void k_side_add(int n, const int* __restrict__ gid, const unsigned ch
     u[g] += side[(long long)p * side_n + g];
 }
 
+// The same when every DOF has one pass-0 and one pass-1 entry: the pass-1
+// entry adds both, u[g] = (u[g] + side[0][g]) + side[1][g].
+__global__ void k_side_add_pair(int n, const int* __restrict__ gid,
+                                const unsigned char* __restrict__ sp,
+                                const double* __restrict__ side, long long side_n, double* u) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n || sp[t] != 1) return;
+    const int g = gid[t];
+    u[g] = (u[g] + side[g]) + side[side_n + g];
+}
+
 template <class T>
 void apply_small(int nbox, int nout_total, const int* d_r, const int* d_s, const int* d_red_off,
                  const int* d_skel_off, const int* d_red_gid, const int* d_skel_gid,
                  const unsigned char* d_skel_pass, const long long* d_off, const T* d_M, int op,
                  int pass, double alpha, const double* d_in, double* d_out, double flops,
-                 double bytes, hipStream_t s, long long side_n) {
+                 double bytes, hipStream_t s, long long side_n, int max_block) {
     if (nbox <= 0 || nout_total <= 0) return;
+    // stage each box's block in LDS where four of the largest fit in 64 KB
+    const int msz = (max_block > 0 && 4LL * max_block * sizeof(T) <= 65536) ? max_block : 0;
+    const size_t shm = (size_t)4 * msz * sizeof(T);
     KernelTimer* kt = active_timer();
     if (kt)
         kt->begin(s, sizeof(T) == sizeof(float) ? "k_apply_small<float>" : "k_apply_small<double>");
-    k_apply_small<T><<<(nbox + 3) / 4, 256, 0, s>>>(nbox, d_r, d_s, d_red_off, d_skel_off,
-                                                    d_red_gid, d_skel_gid, d_skel_pass, d_off,
-                                                    d_M, op, pass, alpha, d_in, d_out, side_n);
+    k_apply_small<T><<<(nbox + 3) / 4, 256, shm, s>>>(nbox, d_r, d_s, d_red_off, d_skel_off,
+                                                      d_red_gid, d_skel_gid, d_skel_pass, d_off,
+                                                      d_M, op, pass, alpha, d_in, d_out, side_n,
+                                                      msz);
     HIFDE_LAUNCH_CHECK();
     if (kt) kt->end(s, flops, bytes);
 }
@@ -406,8 +486,13 @@ void apply_tiles(int nbox, int ntiles, const int* d_tile_off, const int* d_r, co
 }
 
 void side_add(int n, const int* d_gid, const unsigned char* d_sp, const double* d_side,
-              long long side_n, double* d_u, hipStream_t s) {
+              long long side_n, double* d_u, hipStream_t s, bool paired) {
     if (n <= 0) return;
+    if (paired) {
+        k_side_add_pair<<<(n + 255) / 256, 256, 0, s>>>(n, d_gid, d_sp, d_side, side_n, d_u);
+        HIFDE_LAUNCH_CHECK();
+        return;
+    }
     for (int p = 0; p < 2; ++p) {
         k_side_add<<<(n + 255) / 256, 256, 0, s>>>(n, d_gid, d_sp, d_side, side_n, p, d_u);
         HIFDE_LAUNCH_CHECK();
@@ -422,6 +507,34 @@ void zero_boundary(double* d_u, int NC, hipStream_t s) {
 void csr_spmv(int n, const int* d_rowptr, const int* d_colind, const double* d_val,
               const double* d_x, double* d_y, hipStream_t s) {
     k_csr_spmv<<<(n + 255) / 256, 256, 0, s>>>(n, d_rowptr, d_colind, d_val, d_x, d_y);
+    HIFDE_LAUNCH_CHECK();
+}
+
+int csr_max_row(int n, const int* d_rowptr, hipStream_t s) {
+    if (n <= 0) return 0;
+    DevBuf<int> d;
+    d.reserve(1);
+    k_fill_zero_int<<<1, 1, 0, s>>>(d.p);
+    HIFDE_LAUNCH_CHECK();
+    k_max_row<<<std::min((n + 255) / 256, 1024), 256, 0, s>>>(n, d_rowptr, d.p);
+    HIFDE_LAUNCH_CHECK();
+    int h = 0;
+    HIFDE_CHECK(hipMemcpyAsync(&h, d.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIFDE_CHECK(hipStreamSynchronize(s));
+    return h;
+}
+
+void csr_to_ell(int n, int width, const int* d_rowptr, const int* d_colind, const double* d_val,
+                double* d_ell_v, int* d_ell_c, hipStream_t s) {
+    if (n <= 0 || width <= 0) return;
+    k_csr_to_ell<<<(n + 255) / 256, 256, 0, s>>>(n, width, d_rowptr, d_colind, d_val, d_ell_v,
+                                                 d_ell_c);
+    HIFDE_LAUNCH_CHECK();
+}
+
+void ell_spmv(int n, int width, const int* d_rowptr, const double* d_ell_v, const int* d_ell_c,
+              const double* d_x, double* d_y, hipStream_t s) {
+    k_ell_spmv<<<(n + 255) / 256, 256, 0, s>>>(n, width, d_rowptr, d_ell_v, d_ell_c, d_x, d_y);
     HIFDE_LAUNCH_CHECK();
 }
 
@@ -456,11 +569,11 @@ void vcopy(const double* x, double* y, long long n, hipStream_t s) {
 template void apply_small<double>(int, int, const int*, const int*, const int*, const int*,
                                   const int*, const int*, const unsigned char*, const long long*,
                                   const double*, int, int, double, const double*, double*, double,
-                                  double, hipStream_t, long long);
+                                  double, hipStream_t, long long, int);
 template void apply_small<float>(int, int, const int*, const int*, const int*, const int*,
                                  const int*, const int*, const unsigned char*, const long long*,
                                  const float*, int, int, double, const double*, double*, double,
-                                 double, hipStream_t, long long);
+                                 double, hipStream_t, long long, int);
 template void apply_tiles<double>(int, int, const int*, const int*, const int*, const int*,
                                   const int*, const int*, const int*, const unsigned char*,
                                   const long long*, const double*, int, int, double,

@@ -206,6 +206,10 @@ struct hifde_factor {
         host_csr_valid = true;
     }
     DevBuf<int> d_rp, d_ci, d_rr;
+    // slot-major copy of the CSR (ell_width entries per row; 0: none)
+    int ell_width = 0;
+    DevBuf<double> d_ell_v;
+    DevBuf<int> d_ell_c;
     DevBuf<double> d_v;
     // per-kernel-family device time (HIP events), enabled by hifde_set_profiling
     struct KStat {
@@ -512,7 +516,8 @@ struct hifde_factor {
             for (int b = 0; b < B.nbox; ++b)
                 std::fill(ctab.tab.begin() + ctab.off[b], ctab.tab.begin() + ctab.off[b + 1], -1);
         }
-#pragma omp parallel
+        long long min_face = 0;  // skeleton entries on min faces
+#pragma omp parallel reduction(+ : min_face)
         {
             std::vector<std::pair<long long, int>> pts;  // (scan key, gid)
 #pragma omp for schedule(dynamic, 1)
@@ -551,7 +556,8 @@ struct hifde_factor {
                         *sg++ = g;
                         int x = !I[i] ? i : (!I[j] ? j : k);
                         int a = !I[i] ? 0 : (!I[j] ? 1 : 2);
-                        *sp++ = (x == w * cc[a]) ? 1 : 0;  // on a min face
+                        *sp = (x == w * cc[a]) ? 1 : 0;  // on a min face
+                        min_face += *sp++;
                         pos = ps++;
                     } else {
                         *og++ = g;
@@ -561,6 +567,10 @@ struct hifde_factor {
                 }
             }
         }
+        // A cell's skeleton DOFs lie on its faces, and a face DOF is on the
+        // min face of one cell and the max face of the other (the domain's
+        // own faces hold no unknowns): the pass flags pair up.
+        B.sides_paired = B.skel_off[B.nbox] > 0 && 2 * min_face == B.skel_off[B.nbox];
         B.off_rr = prefix_ll(B.r, B.r);
         B.off_rs = prefix_ll(B.r, B.s);
         B.off_ss = prefix_ll(B.s, B.s);
@@ -1616,8 +1626,8 @@ struct hifde_factor {
         if (B.nbox == 0 || B.tot_rs == 0) return;
         const long long n3 = (long long)NC * NC * NC;
         upd(B, Mbuf, 0, -1, alpha, u, side);
-        side_add(B.skel_off.back(), B.d_skel_gid.p, B.d_skel_pass.p, side, n3, u,
-                 stream);
+        side_add(B.skel_off.back(), B.d_skel_gid.p, B.d_skel_pass.p, side, n3, u, stream,
+                 B.sides_paired);
     }
     void diag(BoxBatch& B, const DevBuf<unsigned char>& Mbuf, const double* u, double* out) {
         if (B.esize == sizeof(float)) diag_t<float>(B, Mbuf, u, out);
@@ -1644,7 +1654,8 @@ struct hifde_factor {
             apply_small<T>(B.nbox, nout, B.d_r.p, B.d_s.p, B.d_red_off.p, B.d_skel_off.p,
                            B.d_red_gid.p, B.d_skel_gid.p, spp, B.d_off_rs.p,
                            reinterpret_cast<const T*>(Mbuf.p), op, pass, alpha, u, uo,
-                           2.0 * B.tot_rs, by, stream, side_n);
+                           2.0 * B.tot_rs, by, stream, side_n,
+                           (op == 0 || op == 2) ? B.max_rs : 0);
             return;
         }
         if (wide_tiles(op, out_red ? B.max_r : B.max_s, out_red ? B.tiles_r2 : B.tiles_s2)) {
@@ -1681,7 +1692,7 @@ struct hifde_factor {
             apply_small<T>(B.nbox, B.red_off.back(), B.d_r.p, B.d_s.p, B.d_red_off.p,
                            B.d_skel_off.p, B.d_red_gid.p, B.d_skel_gid.p, nullptr, B.d_off_rr.p,
                            reinterpret_cast<const T*>(Mbuf.p), 4, -1, 1.0, u, out, 2.0 * B.tot_rr,
-                           by, stream);
+                           by, stream, 0, B.max_r * B.max_r);
             return;
         }
         const bool wide = wide_tiles(4, B.max_r, B.tiles_r2);
@@ -1697,7 +1708,7 @@ struct hifde_factor {
         B.tiles_s.assign(B.nbox + 1, 0);
         B.tiles_r2.assign(B.nbox + 1, 0);
         B.tiles_s2.assign(B.nbox + 1, 0);
-        B.max_r = B.max_s = 0;
+        B.max_r = B.max_s = B.max_rs = 0;
         for (int b = 0; b < B.nbox; ++b) {
             B.tiles_r[b + 1] = B.tiles_r[b] + (B.r[b] + 63) / 64;
             B.tiles_s[b + 1] = B.tiles_s[b] + (B.s[b] + 63) / 64;
@@ -1705,6 +1716,7 @@ struct hifde_factor {
             B.tiles_s2[b + 1] = B.tiles_s2[b] + (B.s[b] + 127) / 128;
             B.max_r = std::max(B.max_r, B.r[b]);
             B.max_s = std::max(B.max_s, B.s[b]);
+            B.max_rs = std::max(B.max_rs, B.r[b] * B.s[b]);
         }
         B.d_tiles_r.upload(B.tiles_r, stream);
         B.d_tiles_s.upload(B.tiles_s, stream);
@@ -2092,8 +2104,26 @@ struct hifde_factor {
     }
 
     void spmv_dev(const double* x, double* y) {
-        Prof pr(this, "spmv", 2.0 * nnz_op, 12.0 * nnz_op + 20.0 * NC * NC * NC);
-        csr_spmv(NC * NC * NC, d_rp.p, d_ci.p, d_v.p, x, y, stream);
+        const long long n = (long long)NC * NC * NC;
+        if (ell_width > 0) {
+            Prof pr(this, "spmv", 2.0 * nnz_op, 12.0 * ell_width * n + 20.0 * n);
+            ell_spmv((int)n, ell_width, d_rp.p, d_ell_v.p, d_ell_c.p, x, y, stream);
+            return;
+        }
+        Prof pr(this, "spmv", 2.0 * nnz_op, 12.0 * nnz_op + 20.0 * n);
+        csr_spmv((int)n, d_rp.p, d_ci.p, d_v.p, x, y, stream);
+    }
+    // The slot-major copy of the operator for the PCG's products, when its
+    // rows hold at most kEllMaxWidth entries (every stencil operator: 7).
+    void build_ell() {
+        const long long n = (long long)NC * NC * NC;
+        ell_width = 0;
+        const int w = csr_max_row((int)n, d_rp.p, stream);
+        if (w < 1 || w > kEllMaxWidth) return;
+        d_ell_v.reserve((long long)w * n);
+        d_ell_c.reserve((long long)w * n);
+        csr_to_ell((int)n, w, d_rp.p, d_ci.p, d_v.p, d_ell_v.p, d_ell_c.p, stream);
+        ell_width = w;
     }
 
     void pcg_dev(const double* b, double* x, double tol, int maxit, int* iters, double* relres) {
@@ -2412,6 +2442,10 @@ struct hifde_factor {
                 fl.get_vec(B.red_gid, nd);
                 fl.get_vec(B.skel_gid, nd);
                 fl.get_vec(B.skel_pass, nd);
+                B.sides_paired =
+                    !B.skel_pass.empty() &&
+                    2 * (long long)std::count(B.skel_pass.begin(), B.skel_pass.end(), 1) ==
+                        (long long)B.skel_pass.size();
                 if ((int)B.r.size() != B.nbox || (int)B.s.size() != B.nbox)
                     throw Error(HIFDE_ERR_STATE, "factor file corrupt");
                 // every size within the grid, every prefix sum within int
@@ -2514,6 +2548,7 @@ struct hifde_factor {
         for (long long i = 0; i < nd; ++i)
             for (int p = h_rp[i]; p < h_rp[i + 1]; ++p) rr[p] = (int)i;
         d_rr.upload(rr, stream);
+        build_ell();
         // Element storage needs every coupling between two unknowns to lie
         // inside one level-0 cell closure: per axis the coordinates must
         // share a cell (a coordinate on a cell plane belongs to both sides).
@@ -2619,6 +2654,7 @@ struct hifde_factor {
         nnz_op = stencil_csr(NC, d_a, d_V, symmetric, d_rp, d_ci, d_v, d_rr, stream);
         host_csr_valid = false;
         op_symmetric = csr_symmetric(nd, d_rp.p, d_ci.p, d_v.p, stream);
+        build_ell();
         local_ok = true;
         HIFDE_CHECK(hipStreamSynchronize(stream));
         have_matrix = true;
@@ -2642,6 +2678,7 @@ struct hifde_factor {
                                   d_v, d_rr, d_seg, stream);
         host_csr_valid = false;
         op_symmetric = csr_symmetric(nd, d_rp.p, d_ci.p, d_v.p, stream);
+        build_ell();
         local_ok = true;
         HIFDE_CHECK(hipStreamSynchronize(stream));
         have_matrix = true;

@@ -218,7 +218,11 @@ struct BoxBatch {
     // ceil(r/128) and ceil(s/128) (tiles of two rows per lane)
     std::vector<int> tiles_r, tiles_s, tiles_r2, tiles_s2;
     DevBuf<int> d_tiles_r, d_tiles_s, d_tiles_r2, d_tiles_s2;
-    int max_r = 0, max_s = 0;
+    int max_r = 0, max_s = 0, max_rs = 0;  // largest r, s and r s of a box
+    // Cells: as many skeleton entries on min faces as on max faces (each
+    // shared face DOF has one of each), so the two contributions a DOF gets
+    // from the side buffers can be added by one thread.
+    bool sides_paired = false;
     // Faces only: per-face box geometry.
     std::vector<int> face_kind;
     // Cells, element storage of the level matrix: the live edge and corner
@@ -242,7 +246,8 @@ struct BoxBatch {
         red_gid.clear(); skel_gid.clear(); other_gid.clear(); skel_pass.clear();
         tot_rr = tot_rs = tot_ss = tot_ee = 0;
         tiles_r.clear(); tiles_s.clear(); tiles_r2.clear(); tiles_s2.clear();
-        max_r = max_s = 0;
+        max_r = max_s = max_rs = 0;
+        sides_paired = false;
         face_kind.clear();
     }
 };
@@ -515,17 +520,22 @@ void apply_tiles(int nbox, int ntiles, const int* d_tile_off, const int* d_r, co
                  double* d_out, int max_in, double flops, double bytes, hipStream_t s,
                  long long side_n = 0, int rows_per_lane = 1);
 // Batches whose boxes have at most 64 inputs and outputs: one thread per output.
+// max_block: the largest box block (elements); boxes are staged in LDS when
+// four such blocks fit in 64 KB (the callers pass it for ops 0 and 2, whose
+// lanes read columns, and for op 4; 0: read in place).
 template <class T>
 void apply_small(int nbox, int nout_total, const int* d_r, const int* d_s, const int* d_red_off,
                  const int* d_skel_off, const int* d_red_gid, const int* d_skel_gid,
                  const unsigned char* d_skel_pass, const long long* d_off, const T* d_M, int op,
                  int pass, double alpha, const double* d_in, double* d_out, double flops,
-                 double bytes, hipStream_t s, long long side_n = 0);
+                 double bytes, hipStream_t s, long long side_n = 0, int max_block = 0);
 // side_n > 0 (op 0): each output goes to d_out[pass flag * side_n + DOF]
 // instead of being added to the DOF; side_add then adds the pass-0 and then
-// the pass-1 contributions of a batch's skeleton DOFs to u.
+// the pass-1 contributions of a batch's skeleton DOFs to u.  paired: every
+// DOF has exactly one entry of each pass flag; one launch then adds both
+// contributions, in pass order, from the pass-1 entry.
 void side_add(int n, const int* d_gid, const unsigned char* d_sp, const double* d_side,
-              long long side_n, double* d_u, hipStream_t s);
+              long long side_n, double* d_u, hipStream_t s, bool paired = false);
 void zero_boundary(double* d_u, int NC, hipStream_t s);
 
 // several right-hand sides (multi_rhs.hip).  A block of R vectors: entry
@@ -584,6 +594,16 @@ void pack_vectors(int NC, int nprob, int R, double* sub, long long ld_sub, doubl
 
 // vectors / CSR SpMV for PCG
 void csr_spmv(int n, const int* d_rowptr, const int* d_colind, const double* d_val,
+              const double* d_x, double* d_y, hipStream_t s);
+// Slot-major copy of a CSR whose rows hold at most kEllMaxWidth entries:
+// ell_v[q n + i] and ell_c[q n + i] are row i's q-th entry (in CSR order).
+// The SpMV then reads entries coalesced and sums each row in CSR order, so
+// its results equal csr_spmv's bit for bit.
+constexpr int kEllMaxWidth = 8;
+int csr_max_row(int n, const int* d_rowptr, hipStream_t s);
+void csr_to_ell(int n, int width, const int* d_rowptr, const int* d_colind, const double* d_val,
+                double* d_ell_v, int* d_ell_c, hipStream_t s);
+void ell_spmv(int n, int width, const int* d_rowptr, const double* d_ell_v, const int* d_ell_c,
               const double* d_x, double* d_y, hipStream_t s);
 void dot_partial(const double* x, const double* y, long long n, double* d_part, int nblk,
                  hipStream_t s);

@@ -192,6 +192,30 @@ def test_stencil_assembly_matches_reference_form(H, oracle_mod):
         assert np.allclose(f.spmv(x, transpose=True), A.T @ x, rtol=1e-14, atol=1e-9)
 
 
+def test_spmv_wide_rows_fall_back_to_csr(H, oracle_mod):
+    """The PCG's products read a slot-major copy of the operator when its
+    rows hold at most 8 entries (the stencils above); a matrix with a longer
+    row keeps the CSR kernel.  Both against SciPy, before and after the
+    operator is replaced."""
+    N = 5
+    NC = N + 1
+    _, _, (r, c, v), A = _coo(oracle_mod, N, "pow10", True)
+    row = 3 + NC * (3 + NC * 3)
+    extra = [1 + NC * (1 + NC * k) for k in range(1, 6)] + [2 + NC * (2 + NC * k)
+                                                           for k in range(1, 6)]
+    r2 = np.concatenate([r, np.full(len(extra), row)])
+    c2 = np.concatenate([c, np.array(extra)])
+    v2 = np.concatenate([v, np.linspace(0.5, 1.5, len(extra))])
+    A2 = sp.csr_matrix((v2, (r2, c2)), shape=(NC ** 3, NC ** 3))
+    assert np.diff(A2.indptr).max() > 8
+    x = _rng(5).standard_normal(NC ** 3)
+    f = H.HIFFactor(N, 3, 1e-6)
+    for rr, cc, vv, M in ((r, c, v, A), (r2, c2, v2, A2), (r, c, v, A)):
+        f.set_matrix_coo(rr, cc, vv)
+        assert np.allclose(f.spmv(x), M @ x, rtol=1e-14, atol=1e-12)
+    f.close()
+
+
 def test_c1_reference_protocol_matches_oracle(H, oracle_mod):
     """n=16 (N=15), P=4, eps=1e-6, a = 1: the reference driver's e_a / e_s."""
     f, o, A = _both(H, oracle_mod, 15, 4, 1e-6)
