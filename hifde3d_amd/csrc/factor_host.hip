@@ -246,6 +246,7 @@ struct hifde_factor {
     bool split_schur = false;     // FP32 levels: Schur products as bf16x3 split products
     int host_threads = 0;         // OpenMP threads of the host planning (0: the caller's)
     bool keep_workspaces = true;  // keep the factorization workspaces for the next one
+    bool ahead_option = true;     // plan the level-1 cells during the level-0 face IDs
     // Keep every box's A22 beside its inverse (the forward Apply F reads it).
     // Off: A22 is gathered straight into the inverse's buffer and inverted
     // there (no copy), and A22 buffers of earlier factorizations are freed;
@@ -657,6 +658,8 @@ struct hifde_factor {
     std::vector<long long> flist_off;
     std::vector<int> flist;
     std::vector<int> ext_maps;  // extend-add maps of the current level (capacity kept)
+    std::vector<std::vector<ExtendDesc>> ext_pass;  // its descriptors, eight passes
+    int planned_ahead = -1;  // level whose cells plan_cells_ahead built (-1: none)
     static int edge_slot(int face) { return face == TOP ? 6 : (face == LEFT ? 7 : 8); }
 
     void plan_faces(int level, FacePlan& F) {
@@ -908,10 +911,12 @@ struct hifde_factor {
         BoxBatch& B = cells[level];
         // the level-0 cells depend on the grid only (kept with the element storage)
         const bool cached = level == 0 && elem_on && plan0_valid;
-        if (!cached) plan_cells(level, B);
+        // or planned during the previous level's face IDs (plan_cells_ahead)
+        const bool ahead = planned_ahead == level;
+        if (!cached && !ahead) plan_cells(level, B);
         B.esize = sizeof(T);
         timed(ST_PLAN, t0);
-        if (!cached) upload_batch_index(B);
+        if (!cached && !ahead) upload_batch_index(B);
         timed(ST_UPLOAD, t0);
         if (elem_on) {
             if constexpr (std::is_same<T, float>::value) {
@@ -992,6 +997,72 @@ struct hifde_factor {
         d_egid.upload(gids, stream);
     }
 
+    // Extend-add plan of a level (> 0) with the element storage: for every
+    // (parent, child) pair the parent position of each entry of the child's
+    // element (ext_maps, uploaded to d_emap) and one descriptor per pair in
+    // eight passes by child position (ext_pass).  Reads the level's cell plan
+    // and position table (plan_cells) and the previous level's cell lists.
+    void plan_extend(int level) {
+        const BoxBatch& B = cells[level];
+        const BoxBatch& Cb = cells[level - 1];
+        const int nb = B.nbox;
+        const int c = NC / width(level), cc = 2 * c;
+        // child k of a parent: (2 pi + k/4, 2 pj + k/2 % 2, 2 pk + k % 2)
+        auto child = [&](int b, int k) {
+            const int pi = b / (c * c), pj = (b / c) % c, pk = b % c;
+            return (2 * pi + (k >> 2)) * cc * cc + (2 * pj + ((k >> 1) & 1)) * cc +
+                   (2 * pk + (k & 1));
+        };
+        std::vector<long long> moff((size_t)nb * 8 + 1, 0);
+        for (int b = 0; b < nb; ++b)
+            for (int k = 0; k < 8; ++k) {
+                const int ch = child(b, k);
+                moff[(size_t)b * 8 + k + 1] = Cb.s[ch] + Cb.o[ch];
+            }
+        for (size_t t = 0; t < (size_t)nb * 8; ++t) moff[t + 1] += moff[t];
+        if (moff.back() > (long long)INT32_MAX) throw Error(HIFDE_ERR_SIZE, "extend-add maps");
+        std::vector<int>& maps = ext_maps;  // every entry is written below
+        if ((long long)maps.size() < moff.back()) maps.resize(moff.back());
+        ext_pass.assign(8, std::vector<ExtendDesc>(nb));
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int b = 0; b < nb; ++b)
+            for (int k = 0; k < 8; ++k) {
+                const int ch = child(b, k);
+                ExtendDesc& d = ext_pass[k][b];
+                d.src = Cb.off_ee[ch];
+                d.rr = B.off_rr[b];
+                d.rs = B.off_rs[b];
+                d.ee = B.off_ee[b];
+                d.e = Cb.s[ch] + Cb.o[ch];
+                d.map = (int)moff[(size_t)b * 8 + k];
+                d.r = B.r[b];
+                d.s = B.s[b];
+                d.ep = B.s[b] + B.o[b];
+                int* m = maps.data() + d.map;
+                for (int t = 0; t < Cb.s[ch]; ++t)
+                    *m++ = cell_pos(b, Cb.skel_gid[Cb.skel_off[ch] + t]);
+                for (int t = 0; t < Cb.o[ch]; ++t)
+                    *m++ = cell_pos(b, Cb.other_gid[Cb.other_off[ch] + t]);
+            }
+        d_emap.upload(maps.data(), moff.back(), stream);
+    }
+
+    // The next level's cells planned while the device runs this level's
+    // face IDs: the cell plan, its device index and (element storage) the
+    // extend-add plan, built from the remaining-DOF mask as it stands before
+    // the faces' redundant DOFs are removed.  level_faces keeps it only when
+    // its faces eliminate nothing (every face of full numerical rank, as the
+    // level-0 faces at P = 4 and eps 1e-4), in which case the mask the next
+    // level sees is that same mask.  The previous level's position table is
+    // not read after its faces are planned, so it is overwritten here.
+    void plan_cells_ahead(int level) {
+        BoxBatch& B = cells[level];
+        plan_cells(level, B);
+        upload_batch_index(B);
+        plan_extend(level);
+        planned_ahead = level;
+    }
+
     // Cells of a level with the element storage: the frontal blocks A22,
     // A21 (A12) and the element over [skel | other] are assembled from the
     // operator (level 0) or by extend-add of the eight children's elements;
@@ -1027,52 +1098,14 @@ struct hifde_factor {
                             A22, A21, A12, E, stream);
         } else {
             const BoxBatch& Cb = cells[level - 1];
-            const int c = NC / width(level), cc = 2 * c;
-            // child k of a parent: (2 pi + k/4, 2 pj + k/2 % 2, 2 pk + k % 2)
-            auto child = [&](int b, int k) {
-                const int pi = b / (c * c), pj = (b / c) % c, pk = b % c;
-                return (2 * pi + (k >> 2)) * cc * cc + (2 * pj + ((k >> 1) & 1)) * cc +
-                       (2 * pk + (k & 1));
-            };
-            std::vector<long long> moff((size_t)nb * 8 + 1, 0);
-            for (int b = 0; b < nb; ++b)
-                for (int k = 0; k < 8; ++k) {
-                    const int ch = child(b, k);
-                    moff[(size_t)b * 8 + k + 1] = Cb.s[ch] + Cb.o[ch];
-                }
-            for (size_t t = 0; t < (size_t)nb * 8; ++t) moff[t + 1] += moff[t];
-            if (moff.back() > (long long)INT32_MAX) throw Error(HIFDE_ERR_SIZE, "extend-add maps");
-            std::vector<int>& maps = ext_maps;  // every entry is written below
-            if ((long long)maps.size() < moff.back()) maps.resize(moff.back());
-            std::vector<std::vector<ExtendDesc>> pass(8, std::vector<ExtendDesc>(nb));
-#pragma omp parallel for schedule(dynamic, 1)
-            for (int b = 0; b < nb; ++b)
-                for (int k = 0; k < 8; ++k) {
-                    const int ch = child(b, k);
-                    ExtendDesc& d = pass[k][b];
-                    d.src = Cb.off_ee[ch];
-                    d.rr = B.off_rr[b];
-                    d.rs = B.off_rs[b];
-                    d.ee = B.off_ee[b];
-                    d.e = Cb.s[ch] + Cb.o[ch];
-                    d.map = (int)moff[(size_t)b * 8 + k];
-                    d.r = B.r[b];
-                    d.s = B.s[b];
-                    d.ep = B.s[b] + B.o[b];
-                    int* m = maps.data() + d.map;
-                    for (int t = 0; t < Cb.s[ch]; ++t)
-                        *m++ = cell_pos(b, Cb.skel_gid[Cb.skel_off[ch] + t]);
-                    for (int t = 0; t < Cb.o[ch]; ++t)
-                        *m++ = cell_pos(b, Cb.other_gid[Cb.other_off[ch] + t]);
-                }
-            d_emap.upload(maps.data(), moff.back(), stream);
+            if (planned_ahead != level) plan_extend(level);
             timed(ST_PLAN, t0);
             Prof pr(this, fam<T>("assemble"), 0.0, fr_bytes + (double)Cb.tot_ee * (we + 4.0));
             fill_zero<T>(A22, B.tot_rr, stream);
             fill_zero<T>(A21, B.tot_rs, stream);
             if (A12) fill_zero<T>(A12, B.tot_rs, stream);
             fill_zero<TE>(E, B.tot_ee, stream);
-            extend_add<T, TE>(pass, d_ext, d_eoff, d_emap.p, elem_ptr<TE>(elem_cur), A22, A21,
+            extend_add<T, TE>(ext_pass, d_ext, d_eoff, d_emap.p, elem_ptr<TE>(elem_cur), A22, A21,
                               A12, E, stream);
         }
         elem_cur = ne;
@@ -1327,6 +1360,15 @@ struct hifde_factor {
                                   max_m_house, ws_mode.p, stream);
         }
         timed(ST_ID, t0);
+        // The level-0 faces are usually of full numerical rank: the host
+        // plans the level-1 cells while the device runs their IDs instead of
+        // after the synchronization below (kept only if the faces eliminate
+        // nothing; element storage, whose level-0 faces read no position
+        // table after their own plan).
+        if (level == 0 && elem_on && L > 1 && ahead_option) {
+            plan_cells_ahead(1);
+            timed(ST_PLAN, t0);
+        }
         std::vector<int> kk(nf), perm(F.cols_off.back());
         HIFDE_CHECK(hipMemcpyAsync(kk.data(), ws_k.p, nf * sizeof(int), hipMemcpyDeviceToHost,
                                    stream));
@@ -1376,6 +1418,8 @@ struct hifde_factor {
         B.tot_rr = B.off_rr.back();
         B.tot_rs = B.off_rs.back();
         B.tot_ss = B.off_ss.back();
+        // a face eliminates DOFs: the next level's cells see another mask
+        if (B.tot_rr != 0 && planned_ahead == level + 1) planned_ahead = -1;
         timed(ST_PLAN, t0);
         upload_batch_index(B);
         timed(ST_UPLOAD, t0);
@@ -1554,6 +1598,7 @@ struct hifde_factor {
             for (BoxBatch& B : *v)
                 for (int& x : B.rplan_R) x = 0;
         factored = false;
+        planned_ahead = -1;
         sym_on = op_symmetric && sym_option != 0;
         elem_on = local_ok && elem_option != 0;
         elem32 = elem_on && elem_f32_option != 0 && fp32_levels_eff() >= L;
@@ -3331,6 +3376,7 @@ int hifde_set_option(hifde_factor* f, const char* name, double value) {
             f->clear_graphs();
         }
         else if (n == "keep_a22") f->keep_a22 = value != 0;
+        else if (n == "plan_ahead") f->ahead_option = value != 0;
         else if (n == "keep_workspaces") {
             f->keep_workspaces = value != 0;
             if (!f->keep_workspaces && f->factored) f->release_workspaces();

@@ -291,6 +291,10 @@ const char* hifde_stage_name(int stage);
  *                  factorization ends (the factor shrinks by a third or
  *                  more); Apply F^{-1}, PCG and every solve give the same
  *                  bits, the forward Apply F returns HIFDE_ERR_STATE
+ *   "plan_ahead"   1 (default): with the element storage the host plans the
+ *                  level-1 cells while the device runs the level-0 face
+ *                  IDs, and keeps that plan when those faces eliminate no
+ *                  DOF; 0: planned after them.  Same results either way
  *   "gj_window"    0 (default: 512) or a multiple of 64 in [64, 4096]: the
  *                  outer window width of the blocked inverses (same results
  *                  to rounding)

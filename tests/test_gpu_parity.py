@@ -394,6 +394,35 @@ def test_gram_register_panel_bitwise_equal_to_lds_panel(H):
     assert np.array_equal(out[0][1], out[1][1])
 
 
+@pytest.mark.parametrize("eps,level0_full_rank", [(1e-4, True), (1e-2, False)])
+def test_level1_cells_planned_ahead_give_the_same_factor(H, eps, level0_full_rank):
+    """plan_ahead (default) plans the level-1 cells while the level-0 face
+    IDs run and keeps that plan only when those faces eliminate nothing.
+    Both outcomes (kept at eps 1e-4, where every level-0 face has full rank,
+    and dropped at 1e-2) give the same bits as planning after the IDs, on the
+    first factorization (level-0 index built) and on a second one (kept)."""
+    NC = 32
+    a = pb.lognormal_coeff(NC, 3, 0.1)
+    V = pb.zeros_coeff(NC)
+    b = pb.rhs_normal(NC, 4)
+    out = []
+    for ahead in (1, 0):
+        f = H.HIFFactor(NC - 1, 4, eps, precision="mixed")
+        f.set_option("plan_ahead", ahead)
+        f.set_stencil(a, V, True)
+        res = []
+        for rep in range(2):
+            assert f.Factor() == 0
+            res.append((f.level_counts(), f.Apply(b, True)))
+        out.append(res)
+        f.close()
+    face0_red = out[0][0][0][0][3]
+    assert (face0_red == 0) == level0_full_rank
+    for (c0, y0), (c1, y1) in zip(out[0], out[1]):
+        assert c0 == c1
+        assert np.array_equal(y0, y1)
+
+
 def test_released_workspaces_give_the_same_factor(H):
     """keep_workspaces=0 frees the factorization's workspaces after each
     factorization; factoring again (which reallocates them) and solving give
