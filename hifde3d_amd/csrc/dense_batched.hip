@@ -816,6 +816,13 @@ struct SlabPitch<float> {
 // Tile rows per group in the order of a general descriptor's output tiles.
 constexpr int kGemmGroupRows = 8;
 
+// Workgroups per CU the matrix-core GEMM is compiled for.  The FP32
+// 128 x 128 tile uses 40 KB of LDS, so four fit the CU's 160 KB; asking for
+// four holds it to 128 VGPRs (four waves per SIMD instead of three).  The
+// split-product variant needs more registers and keeps two.
+template <class T, int TL, bool SPLIT>
+constexpr int kGemmMinBlocks = (sizeof(T) == 4 && TL == 128 && !SPLIT) ? 4 : 2;
+
 // C(m x n) = alpha op(A) B + beta C over a flattened list of 128 x 128 output
 // tiles of every descriptor, 4 waves in a 2 x 2 arrangement.  The k dimension
 // is walked in slabs of 16 held in two LDS buffers: the next slab is loaded
@@ -834,7 +841,7 @@ constexpr int kGemmGroupRows = 8;
 // products of the split operands (split_bf16) instead of eight FP32 ones.
 // TL: workgroup tile, 128 (the default) or 64.
 template <class T, bool TRANSA, class TC = T, bool SPLIT = false, int TL = kMfmaTile>
-__global__ void __launch_bounds__(256, 2) k_gemm_mfma(int nbox, const int* __restrict__ tile_off,
+__global__ void __launch_bounds__(256, (kGemmMinBlocks<T, TL, SPLIT>)) k_gemm_mfma(int nbox, const int* __restrict__ tile_off,
                                                       const GemmDesc* __restrict__ desc,
                                                       const T* __restrict__ Ab,
                                                       const T* __restrict__ Bb, TC* Cb, T alpha,
@@ -904,32 +911,46 @@ __global__ void __launch_bounds__(256, 2) k_gemm_mfma(int nbox, const int* __res
     // write on 64 distinct banks.
     auto a_row = [&](int q) { return 16 * ((q * 4 + wave) % (TILE / 16)) + (lane & 15); };
     auto a_k = [&](int q) { return 4 * ((q * 4 + wave) / (TILE / 16)) + (lane >> 4); };
-    unsigned offa[NLD], offb[NLD];
-#pragma unroll
-    for (int q = 0; q < NLD; ++q) {
-        const int e = tid + 256 * q;
-        if (!TRANSA) offa[q] = (unsigned)(((i0 + a_row(q)) + (size_t)a_k(q) * dd.lda) * sizeof(T));
-        else offa[q] = (unsigned)(((e % KB) + (size_t)(i0 + e / KB) * dd.lda) * sizeof(T));
-        offb[q] = (unsigned)(((e % KB) + (size_t)(j0 + e / KB) * dd.ldb) * sizeof(T));
-    }
-    const unsigned stepa = (unsigned)((TRANSA ? KB : (size_t)KB * dd.lda) * sizeof(T));
-    const unsigned stepb = (unsigned)(KB * sizeof(T));
+    // Each thread keeps one byte offset per operand (its slot q = 0) and
+    // the slab's running offset; slot q adds a workgroup-uniform distance
+    // (16 rows of a k-contiguous operand per slot; for A column-major, 64
+    // rows for odd q and 4 k per pair of slots in the 128 tile, 4 k per slot
+    // in the 64 tile).  All of it stays in the buffer offset, which the
+    // bound check covers.
+    static_assert(TILE == 128 || TILE == 64, "slot distances assume 64 or 128 tiles");
+    const unsigned sz = sizeof(T);
+    const unsigned offa0 =
+        !TRANSA ? (unsigned)(((i0 + a_row(0)) + (size_t)a_k(0) * dd.lda) * sz)
+                : (unsigned)(((tid % KB) + (size_t)(i0 + tid / KB) * dd.lda) * sz);
+    const unsigned offb0 = (unsigned)(((tid % KB) + (size_t)(j0 + tid / KB) * dd.ldb) * sz);
+    const unsigned lda_b = (unsigned)dd.lda * sz, ldb_b = (unsigned)dd.ldb * sz;
+    auto a_delta = [&](int q) -> unsigned {
+        if (TRANSA) return (unsigned)(q * (256 / KB)) * lda_b;
+        if (TILE == 128) return (unsigned)(64 * (q & 1)) * sz + (unsigned)(4 * (q >> 1)) * lda_b;
+        return (unsigned)(4 * q) * lda_b;
+    };
+    const unsigned stepa = TRANSA ? KB * sz : KB * lda_b;
+    const unsigned stepb = KB * sz;
+    unsigned va = offa0, vb = offb0;  // slot-0 offsets of the slab being loaded
     T ra[NLD], rb[NLD];
     auto gload = [&](int sl) {
         const int k0 = sl * KB;
         const bool tail = k0 + KB > k;  // uniform: only the last slab
 #pragma unroll
         for (int q = 0; q < NLD; ++q) {
-            const int e = tid + 256 * q;
-            ra[q] = buf_load(ra_rsrc, offa[q] + sl * stepa, T());
-            rb[q] = buf_load(rb_rsrc, offb[q] + sl * stepb, T());
-            if (tail) {
-                // k-contiguous operands read the next rows of their column
-                // past k: zero them (A column-major reads zero from the bound)
-                if (k0 + e % KB >= k) {
-                    rb[q] = (T)0;
-                    if (TRANSA) ra[q] = (T)0;
-                }
+            ra[q] = buf_load(ra_rsrc, va + a_delta(q), T());
+            rb[q] = buf_load(rb_rsrc, vb + (unsigned)(q * (256 / KB)) * ldb_b, T());
+        }
+        va += stepa;
+        vb += stepb;
+        if (tail && k0 + tid % KB >= k) {
+            // k-contiguous operands read the next rows of their column past
+            // k (a thread's k is the same in every slot): zero them (A
+            // column-major reads zero from the bound)
+#pragma unroll
+            for (int q = 0; q < NLD; ++q) {
+                rb[q] = (T)0;
+                if (TRANSA) ra[q] = (T)0;
             }
         }
     };
