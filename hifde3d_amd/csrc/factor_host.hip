@@ -660,6 +660,8 @@ struct hifde_factor {
     std::vector<int> ext_maps;  // extend-add maps of the current level (capacity kept)
     std::vector<std::vector<ExtendDesc>> ext_pass;  // its descriptors, eight passes
     int planned_ahead = -1;  // level whose cells plan_cells_ahead built (-1: none)
+    int deferred_faces = -1;  // level whose face boxes wait for finish_deferred_faces
+    std::vector<int> def_kk, def_perm;  // that level's ranks and pivots
     static int edge_slot(int face) { return face == TOP ? 6 : (face == LEFT ? 7 : 8); }
 
     void plan_faces(int level, FacePlan& F) {
@@ -1205,6 +1207,45 @@ struct hifde_factor {
         fd.gd_cols = face_gather_prepare(fd.h_gd, fd.gd, fd.gd_off, stream);
     }
 
+    // Face boxes of a level from the ID's ranks and pivots: red = the ID's
+    // redundant columns, skel = its skeleton columns (Skeletonize,
+    // src/Factor.cpp:171-178), and their device index.
+    void face_boxes(int level, const std::vector<int>& kk, const std::vector<int>& perm) {
+        const FacePlan& F = fdev[level == 0 ? 0 : 1].F;
+        BoxBatch& B = faces[level];
+        const int nf = F.nface;
+        B.r.resize(nf);
+        B.s.resize(nf);
+        for (int f = 0; f < nf; ++f) {
+            B.s[f] = kk[f];
+            B.r[f] = F.n[f] - kk[f];
+        }
+        B.red_off = prefix_i(B.r);
+        B.skel_off = prefix_i(B.s);
+        B.red_gid.resize(B.red_off.back());
+        B.skel_gid.resize(B.skel_off.back());
+        for (int f = 0; f < nf; ++f) {
+            const int* p = perm.data() + F.cols_off[f];
+            const int* cg = F.cols.data() + F.cols_off[f];
+            for (int t = 0; t < B.s[f]; ++t) B.skel_gid[B.skel_off[f] + t] = cg[p[t]];
+            for (int t = 0; t < B.r[f]; ++t) B.red_gid[B.red_off[f] + t] = cg[p[B.s[f] + t]];
+        }
+        B.off_rr = prefix_ll(B.r, B.r);
+        B.off_rs = prefix_ll(B.r, B.s);
+        B.off_ss = prefix_ll(B.s, B.s);
+        B.tot_rr = B.off_rr.back();
+        B.tot_rs = B.off_rs.back();
+        B.tot_ss = B.off_ss.back();
+        upload_batch_index(B);
+    }
+    // The face boxes level_faces left for later (deferred_faces >= 0).
+    void finish_deferred_faces() {
+        if (deferred_faces < 0) return;
+        const int level = deferred_faces;
+        deferred_faces = -1;
+        face_boxes(level, def_kk, def_perm);
+    }
+
     template <class T>
     void level_faces(int level) {
         auto t0 = clk::now();
@@ -1394,35 +1435,22 @@ struct hifde_factor {
             Prof pr(this, "gram_w", fl_w, 8.0 * toff.back() * 2);
             gram_w(rd, toff, ws_T.p, ws_perm.p, offW, ws_W.p, gram_ws, gj_ws, gemm_ws, stream);
         }
-        // Face box: red = ID redundant columns, skel = ID skeleton columns
-        // (Skeletonize, src/Factor.cpp:171-178).
-        B.r.resize(nf);
-        B.s.resize(nf);
-        for (int f = 0; f < nf; ++f) {
-            B.s[f] = kk[f];
-            B.r[f] = F.n[f] - kk[f];
+        // Every face kept all its columns and the next level's cells were
+        // planned during the IDs: the face boxes (read by Apply and save
+        // only) are built after the next level's cell launches are issued.
+        bool none_red = true;
+        for (int f = 0; f < nf && none_red; ++f) none_red = kk[f] == F.n[f];
+        if (none_red && planned_ahead == level + 1) {
+            deferred_faces = level;
+            def_kk = std::move(kk);
+            def_perm = std::move(perm);
+            timed(ST_PLAN, t0);
+            return;
         }
-        B.red_off = prefix_i(B.r);
-        B.skel_off = prefix_i(B.s);
-        B.red_gid.resize(B.red_off.back());
-        B.skel_gid.resize(B.skel_off.back());
-        for (int f = 0; f < nf; ++f) {
-            const int* p = perm.data() + F.cols_off[f];
-            const int* cg = F.cols.data() + F.cols_off[f];
-            for (int t = 0; t < B.s[f]; ++t) B.skel_gid[B.skel_off[f] + t] = cg[p[t]];
-            for (int t = 0; t < B.r[f]; ++t) B.red_gid[B.red_off[f] + t] = cg[p[B.s[f] + t]];
-        }
-        B.off_rr = prefix_ll(B.r, B.r);
-        B.off_rs = prefix_ll(B.r, B.s);
-        B.off_ss = prefix_ll(B.s, B.s);
-        B.tot_rr = B.off_rr.back();
-        B.tot_rs = B.off_rs.back();
-        B.tot_ss = B.off_ss.back();
+        face_boxes(level, kk, perm);
         // a face eliminates DOFs: the next level's cells see another mask
         if (B.tot_rr != 0 && planned_ahead == level + 1) planned_ahead = -1;
         timed(ST_PLAN, t0);
-        upload_batch_index(B);
-        timed(ST_UPLOAD, t0);
         if (B.tot_rr == 0) {
             // Every face keeps all its columns (full numerical rank, as every
             // level-0 face at P = 4 and eps 1e-4): nothing is eliminated,
@@ -1599,6 +1627,7 @@ struct hifde_factor {
                 for (int& x : B.rplan_R) x = 0;
         factored = false;
         planned_ahead = -1;
+        deferred_faces = -1;
         sym_on = op_symmetric && sym_option != 0;
         elem_on = local_ok && elem_option != 0;
         elem32 = elem_on && elem_f32_option != 0 && fp32_levels_eff() >= L;
@@ -1618,14 +1647,15 @@ struct hifde_factor {
         }
         for (int level = 0; level < L; ++level) {
             cur_level = level;
-            if (level < fp32_levels_eff()) {
-                level_cells<float>(level);
-                if (level < L - 1) level_faces<float>(level);
-            } else {
-                level_cells<double>(level);
-                if (level < L - 1) level_faces<double>(level);
+            if (level < fp32_levels_eff()) level_cells<float>(level);
+            else level_cells<double>(level);
+            finish_deferred_faces();
+            if (level < L - 1) {
+                if (level < fp32_levels_eff()) level_faces<float>(level);
+                else level_faces<double>(level);
             }
         }
+        finish_deferred_faces();
         // Every DOF has been eliminated: the level matrix is consumed.  Its
         // buffers and the workspaces are kept for the next factorization
         // unless keep_workspaces is off.
