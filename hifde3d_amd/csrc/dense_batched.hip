@@ -400,13 +400,13 @@ __device__ __forceinline__ float recip(float x) {
     const float r = __builtin_amdgcn_rcpf(x);
     return fmaf(fmaf(-x, r, 1.0f), r, r);
 }
-template <class T>
-__global__ void __launch_bounds__(kDiagWaves * 64) k_bgj_diag(const int* __restrict__ dims,
-                                                             const long long* __restrict__ off,
-                                                             const int* __restrict__ lds,
-                                                             const T* __restrict__ mats, T* Dws,
-                                                             int step, int ibase, int* info) {
-    const int b = blockIdx.x;
+template <class T, int W>
+__device__ __forceinline__ void bgj_diag_body(int b, const int* __restrict__ dims,
+                                              const long long* __restrict__ off,
+                                              const int* __restrict__ lds,
+                                              const T* __restrict__ mats, T* Dws, int step,
+                                              int ibase, int* info) {
+    constexpr int kDiagCols = kGJB / W;
     const int n = dims[b];
     const int k0 = step * kGJB;
     if (k0 >= n || info[b]) return;
@@ -419,7 +419,7 @@ __global__ void __launch_bounds__(kDiagWaves * 64) k_bgj_diag(const int* __restr
     T a[kDiagCols];
 #pragma unroll
     for (int q = 0; q < kDiagCols; ++q) {
-        const int c = w + kDiagWaves * q;
+        const int c = w + W * q;
         a[q] = (lane < kb && c < kb) ? A[lane + (size_t)c * ld] : (T)(c == lane ? 1 : 0);
     }
     if (w == 0) colbuf[0][lane] = a[0];
@@ -427,13 +427,13 @@ __global__ void __launch_bounds__(kDiagWaves * 64) k_bgj_diag(const int* __restr
     bool used = false;
     int my_step = 0;
     int bad = 0;
-    // step k = kq kDiagWaves + kw: column k is register kq of wave kw, so with
+    // step k = kq W + kw: column k is register kq of wave kw, so with
     // kq unrolled every register index is a constant
 #pragma unroll
     for (int kq = 0; kq < kDiagCols; ++kq) {
 #pragma unroll 1
-        for (int kw = 0; kw < kDiagWaves; ++kw) {
-            const int k = kq * kDiagWaves + kw;
+        for (int kw = 0; kw < W; ++kw) {
+            const int k = kq * W + kw;
             const T f = colbuf[k & 1][lane];
             bool nz;
             const int p = __builtin_amdgcn_readfirstlane(
@@ -456,7 +456,7 @@ __global__ void __launch_bounds__(kDiagWaves * 64) k_bgj_diag(const int* __restr
                 if (w == 0) piv[k] = lane;
             }
             // publish column k + 1
-            if (kw + 1 < kDiagWaves) {
+            if (kw + 1 < W) {
                 if (w == kw + 1) colbuf[(k + 1) & 1][lane] = a[kq];
             } else if (kq + 1 < kDiagCols) {
                 if (w == 0) colbuf[(k + 1) & 1][lane] = a[min(kq + 1, kDiagCols - 1)];
@@ -473,16 +473,24 @@ done:
     if (my_step < kb) {
 #pragma unroll
         for (int q = 0; q < kDiagCols; ++q) {
-            const int c = w + kDiagWaves * q;
+            const int c = w + W * q;
             const int pc = piv[c];
             if (c < kb && pc < kb) D[my_step + (size_t)pc * kGJB] = a[q];
         }
     }
     // rows and columns >= kb of the 64 x 64 workspace are zero
-    for (int e = threadIdx.x; e < kGJB * kGJB; e += kDiagWaves * 64) {
+    for (int e = threadIdx.x; e < kGJB * kGJB; e += W * 64) {
         const int c = e >> 6, i = e & 63;
         if (c >= kb || i >= kb) D[e] = (T)0;
     }
+}
+template <class T>
+__global__ void __launch_bounds__(kDiagWaves * 64) k_bgj_diag(const int* __restrict__ dims,
+                                                             const long long* __restrict__ off,
+                                                             const int* __restrict__ lds,
+                                                             const T* __restrict__ mats, T* Dws,
+                                                             int step, int ibase, int* info) {
+    bgj_diag_body<T, kDiagWaves>(blockIdx.x, dims, off, lds, mats, Dws, step, ibase, info);
 }
 
 // acc(4x4 per thread) = sum_c L[c][row] * R[c][col] over a 64x64 tile.
@@ -557,19 +565,18 @@ __global__ void __launch_bounds__(256) k_bgj_row(const int* __restrict__ dims,
 // chunk outside K (the two blocks are disjoint), which saves a launch per
 // step.
 template <class T>
-__global__ void __launch_bounds__(256) k_bgj_col(const int* __restrict__ dims,
-                                                 const long long* __restrict__ off,
-                                                 const int* __restrict__ lds, T* mats,
-                                                 const T* __restrict__ Dws,
-                                                 const T* __restrict__ Rws,
-                                                 const long long* __restrict__ r_off, int step,
-                                                 const int* __restrict__ info) {
-    const int b = blockIdx.y;
+__device__ __forceinline__ void bgj_col_body(int b, int chunk, const int* __restrict__ dims,
+                                             const long long* __restrict__ off,
+                                             const int* __restrict__ lds, T* mats,
+                                             const T* __restrict__ Dws,
+                                             const T* __restrict__ Rws,
+                                             const long long* __restrict__ r_off, int step,
+                                             const int* __restrict__ info) {
     const int n = dims[b];
     const int k0 = step * kGJB;
     if (k0 >= n || info[b]) return;
     const int kb = min(kGJB, n - k0);
-    const int i0 = blockIdx.x * 64;
+    const int i0 = chunk * 64;
     if (i0 >= n) return;
     const int ld = lds[b];
     T* A = mats + off[b];
@@ -615,6 +622,45 @@ __global__ void __launch_bounds__(256) k_bgj_col(const int* __restrict__ dims,
         }
     }
 }
+template <class T>
+__global__ void __launch_bounds__(256) k_bgj_col(const int* __restrict__ dims,
+                                                 const long long* __restrict__ off,
+                                                 const int* __restrict__ lds, T* mats,
+                                                 const T* __restrict__ Dws,
+                                                 const T* __restrict__ Rws,
+                                                 const long long* __restrict__ r_off, int step,
+                                                 const int* __restrict__ info) {
+    bgj_col_body<T>(blockIdx.y, blockIdx.x, dims, off, lds, mats, Dws, Rws, r_off, step, info);
+}
+
+// One launch for the column step of inner step col_step (< 0: none) and the
+// diagonal block of step diag_step (< 0: none), 256 threads each: the column
+// step writes block K's inverse and the column blocks A(I, K), I != K, and
+// the row blocks A(K, J); the next diagonal block A(K', K') is none of them
+// and was updated by the in-window GEMM launched before, so the two are
+// independent.  Their 64 x 64 D workspaces are separate slots (Dcol, Ddiag).
+// Workgroups [0, nchunk * nbig) are column chunks (chunk = id % nchunk, box
+// = id / nchunk), the rest one diagonal block per box (four waves).
+template <class T>
+__global__ void __launch_bounds__(256) k_bgj_col_diag(int nchunk, int nbig,
+                                                      const int* __restrict__ dims,
+                                                      const long long* __restrict__ off,
+                                                      const int* __restrict__ lds, T* mats,
+                                                      const T* Dcol, T* Ddiag,
+                                                      const T* __restrict__ Rws,
+                                                      const long long* __restrict__ r_off,
+                                                      int col_step, int diag_step, int ibase,
+                                                      int* info) {
+    const int id = (int)blockIdx.x;
+    const int ncol = col_step >= 0 ? nchunk * nbig : 0;
+    if (id < ncol) {
+        bgj_col_body<T>(id / nchunk, id % nchunk, dims, off, lds, mats, Dcol, Rws, r_off,
+                        col_step, info);
+    } else {
+        bgj_diag_body<T, 4>(id - ncol, dims, off, lds, mats, Ddiag, diag_step, ibase, info);
+    }
+}
+
 
 constexpr int BM = 64, BN = 64, BK = 16;
 
@@ -1776,7 +1822,9 @@ void batched_gj_inverse(int nbox, const int* h_n, const long long* h_off, const 
     ws.info.reserve(nbig);
     k_fill_int<<<(nbig + 255) / 256, 256, 0, s>>>(ws.info.p, nbig, 0);
     ws.B.reserve((size_t)nbig * kGJB * panel * sizeof(T));
-    ws.D.reserve((size_t)nbig * kGJB * kGJB * sizeof(T));
+    // two D slots: step st uses slot st & 1 (a column step and the next
+    // diagonal block run in one launch)
+    ws.D.reserve(2 * (size_t)nbig * kGJB * kGJB * sizeof(T));
     // symmetric outer level: per box W (n x panel, ld n) and V^T
     // (panel x n); the general one: RO and CW
     std::vector<long long> sw_off(nbig + 1, 0);
@@ -1865,12 +1913,20 @@ void batched_gj_inverse(int nbox, const int* h_n, const long long* h_off, const 
             // inner level: every window inverted in place
             const int isteps = (max_w + kGJB - 1) / kGJB;
             const dim3 gch((max_w + 63) / 64, nbig);
+            const int nchunk = (int)gch.x;
+            const size_t dslot = (size_t)nbig * kGJB * kGJB;
+            auto Dst = [&](int st) { return Dws + (st & 1) * dslot; };
             for (int st = 0; st < isteps; ++st) {
                 if (exec) {
-                    k_bgj_diag<T><<<nbig, kDiagWaves * 64, 0, s>>>(dwn, dwoff, dwld, d_A, Dws, st, K0,
-                                                          cinfo);
-                    k_bgj_row<T><<<gch, 256, 0, s>>>(dwn, dwoff, dwld, d_A, Dws, Rws, ws.b_off.p,
-                                                      st, cinfo);
+                    if (!ws.fused_steps)
+                        k_bgj_diag<T><<<nbig, kDiagWaves * 64, 0, s>>>(dwn, dwoff, dwld, d_A,
+                                                                      Dst(st), st, K0, cinfo);
+                    else if (st == 0)
+                        k_bgj_col_diag<T><<<nbig, 256, 0, s>>>(nchunk, nbig, dwn, dwoff, dwld,
+                                                               d_A, nullptr, Dst(0), Rws,
+                                                               ws.b_off.p, -1, 0, K0, cinfo);
+                    k_bgj_row<T><<<gch, 256, 0, s>>>(dwn, dwoff, dwld, d_A, Dst(st), Rws,
+                                                      ws.b_off.p, st, cinfo);
                 }
                 // A(:, J) -= A(:, K) R(:, J) over the window's columns left and
                 // right of the block; the block rows are then set to R.
@@ -1889,8 +1945,15 @@ void batched_gj_inverse(int nbox, const int* h_n, const long long* h_off, const 
                 }
                 gemm(upd, d_A, Rws, d_A, (T)-1, (T)1, false, true);
                 if (exec) {
-                    k_bgj_col<T><<<gch, 256, 0, s>>>(dwn, dwoff, dwld, d_A, Dws, Rws, ws.b_off.p,
-                                                      st, cinfo);
+                    if (!ws.fused_steps) {
+                        k_bgj_col<T><<<gch, 256, 0, s>>>(dwn, dwoff, dwld, d_A, Dst(st), Rws,
+                                                          ws.b_off.p, st, cinfo);
+                    } else {
+                        const bool next = st + 1 < isteps;
+                        k_bgj_col_diag<T><<<nchunk * nbig + (next ? nbig : 0), 256, 0, s>>>(
+                            nchunk, nbig, dwn, dwoff, dwld, d_A, Dst(st), Dst(st + 1), Rws,
+                            ws.b_off.p, st, next ? st + 1 : -1, K0, cinfo);
+                    }
                 }
             }
             if (sym) {

@@ -3407,6 +3407,7 @@ int hifde_set_option(hifde_factor* f, const char* name, double value) {
         }
         else if (n == "keep_a22") f->keep_a22 = value != 0;
         else if (n == "plan_ahead") f->ahead_option = value != 0;
+        else if (n == "gj_fused_steps") f->gj_ws.fused_steps = value != 0;
         else if (n == "keep_workspaces") {
             f->keep_workspaces = value != 0;
             if (!f->keep_workspaces && f->factored) f->release_workspaces();

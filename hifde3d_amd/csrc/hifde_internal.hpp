@@ -376,6 +376,9 @@ struct GJWork {
     DevBuf<long long> stile;          // per copy launch: tile prefix of its descriptors
     // outer window width (a multiple of 64); 0: the default (512)
     int panel = 0;
+    // each inner step's column kernel shares a launch with the next step's
+    // diagonal block (k_bgj_col_diag); off: separate launches
+    bool fused_steps = true;
 };
 
 // C = alpha op(A) B + beta C per descriptor (op(A) = A^T when transA).  C

@@ -295,6 +295,9 @@ const char* hifde_stage_name(int stage);
  *                  level-1 cells while the device runs the level-0 face
  *                  IDs, and keeps that plan when those faces eliminate no
  *                  DOF; 0: planned after them.  Same results either way
+ *   "gj_fused_steps"  1 (default): each inner step of the blocked inverses
+ *                  launches its column kernel together with the next step's
+ *                  diagonal block; 0: separate launches.  Same bits
  *   "gj_window"    0 (default: 512) or a multiple of 64 in [64, 4096]: the
  *                  outer window width of the blocked inverses (same results
  *                  to rounding)

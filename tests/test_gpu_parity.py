@@ -423,6 +423,30 @@ def test_level1_cells_planned_ahead_give_the_same_factor(H, eps, level0_full_ran
         assert np.array_equal(y0, y1)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_fused_inverse_steps_give_the_same_bits(H, precision):
+    """gj_fused_steps (default) launches each inner block-inverse step's
+    column kernel together with the next step's diagonal block; 0 launches
+    them separately.  The same bits at n=64 (top boxes of several 512-wide
+    windows: every path of the blocked inverse), factor and apply."""
+    NC = 64
+    a = pb.lognormal_coeff(NC, 7, 0.1)
+    V = pb.zeros_coeff(NC)
+    b = pb.rhs_normal(NC, 8)
+    out = []
+    for fused in (1, 0):
+        f = H.HIFFactor(NC - 1, 4, 1e-4, precision=precision)
+        f.set_option("gj_fused_steps", fused)
+        f.set_stencil(a, V, True)
+        assert f.Factor() == 0
+        out.append((f.level_counts(), f.Apply(b, True)))
+        if fused:
+            assert f.level_counts()[-1][1] > 512  # the root box spans windows
+        f.close()
+    assert out[0][0] == out[1][0]
+    assert np.array_equal(out[0][1], out[1][1])
+
+
 def test_released_workspaces_give_the_same_factor(H):
     """keep_workspaces=0 frees the factorization's workspaces after each
     factorization; factoring again (which reallocates them) and solving give
