@@ -1983,7 +1983,6 @@ void batched_gj_inverse(int nbox, const int* h_n, const long long* h_off, const 
                     const int ld = wld[w0 + j];
                     sy.push_back({wo, wo, kb, kb, ld, ld, 1, 1, kAll, 1, 0, 0, 2});
                 }
-                scopy(sy);
                 std::vector<SymCopy> gv, wb;
                 std::vector<GemmDesc> gw, tl, bq;
                 for (int j = 0; j < nbig && !exec; ++j) {
@@ -2016,7 +2015,11 @@ void batched_gj_inverse(int nbox, const int* h_n, const long long* h_off, const 
                     }
                     if (K0 > 0) wb.push_back({o + K0, swo, kb, K0, n, ldv, 1, 0, 0, 1, 0, 1});
                 }
-                scopy(gv);
+                // one launch: the symmetrization touches only the window's
+                // diagonal block, the gathers read the blocks beside it and
+                // write the V^T workspace
+                sy.insert(sy.end(), gv.begin(), gv.end());
+                scopy(sy);
                 gemm(gw, SW, d_A, SW, (T)1, (T)0, true);
                 gemm(tl, SW, SW, d_A, (T)1, (T)1);
                 gemm(bq, SW, SW, d_A, (T)-1, (T)1);
